@@ -1,0 +1,161 @@
+/*
+ * mpsxx_amd.h -- C ABI of the MI355X (gfx950) DMRG effective-Hamiltonian hot path.
+ *
+ * Every entry point takes plain pointers and sizes, returns an int status (0 = ok) and, on failure, leaves a message
+ * readable through mpsxx_last_error() (thread-local).  Tensors are opaque device-resident block-sparse arrays with
+ * U(1)xU(1) quantum numbers (particle number, 2Sz).  The host data layout used by upload/download is canonical:
+ * blocks in ascending row-major block-index order ("tag"), each block dense and row-major.
+ *
+ * Semantics follow the reference (naokin/mpsxx, paths relative to its root):
+ *   - outputs ACCUMULATE (beta = 1): an empty output (mpsxx_tensor_clear'ed, or freshly created empty) is shaped from
+ *     the inputs; a non-empty one must carry the same total QN and quantum shape or MPSXX_ERR_QN is returned
+ *     (include/btas/QSPARSE/QSTBLAS.h:201-211);
+ *   - the output block set is every QN-allowed block that receives at least one (A-block, B-block) pair
+ *     (include/btas/SPARSE/STBLAS.h:316-356), applied step by step along the reference contraction order;
+ *   - calls are asynchronous on the given stream (NULL = default stream) except those that return host values.
+ */
+#ifndef MPSXX_AMD_H
+#define MPSXX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------------------------------- */
+enum {
+  MPSXX_OK = 0,
+  MPSXX_ERR_SHAPE = 1,       /* dense-shape mismatch                                                                */
+  MPSXX_ERR_QN = 2,          /* quantum-number / quantum-shape mismatch (reference: BTAS_THROW -> runtime_error)   */
+  MPSXX_ERR_HIP = 3,         /* HIP runtime error                                                                   */
+  MPSXX_ERR_OOM = 4,         /* device allocation failed                                                            */
+  MPSXX_ERR_ARG = 5,         /* invalid argument                                                                    */
+  MPSXX_ERR_UNSUPPORTED = 6  /* contraction form outside the supported set                                          */
+};
+
+/* ---- arithmetic of the block contractions ------------------------------------------------------------------------- */
+enum {
+  MPSXX_FP64 = 0,   /* FP64 storage and MFMA (v_mfma_f64_16x16x4_f64)                                                */
+  MPSXX_FP32 = 1    /* FP32 storage and MFMA with FP32 accumulation; outputs accumulated into the FP64 tensor       */
+};
+
+/* ---- structure descriptor ----------------------------------------------------------------------------------------- */
+typedef struct mpsxx_qst_desc {
+  int32_t rank;             /* number of indices (1..8)                                                           */
+  int32_t qsym;             /* 1: QSTArray (blocks must satisfy sum_i qshape[i][idx_i] == q_total); 0: STArray      */
+  int32_t q_total[2];       /* total quantum number (particle number, 2Sz); ignored when qsym == 0                  */
+  const int32_t* nsec;      /* [rank] number of quantum-number sectors of each index                              */
+  const int32_t* sec_q;     /* [sum(nsec)][2] sector quantum numbers, index 0 first (NULL allowed if qsym == 0)   */
+  const int32_t* sec_dim;   /* [sum(nsec)] dense dimension of each sector                                          */
+  int64_t nblocks;          /* number of stored blocks                                                             */
+  const int32_t* block_idx; /* [nblocks][rank] sector index of each stored block (any order)                        */
+} mpsxx_qst_desc;
+
+typedef struct mpsxx_tensor_s* mpsxx_tensor;
+
+/* ---- library / error ---------------------------------------------------------------------------------------------- */
+int mpsxx_init(int device);                      /* select the HIP device (called implicitly with 0)              */
+const char* mpsxx_last_error(void);
+int mpsxx_synchronize(void* stream);
+
+/* ---- tensors (reference: btas::QSTArray, include/btas/QSPARSE/QSTArray.h:16-328) ---------------------------------- */
+int mpsxx_tensor_create(const mpsxx_qst_desc* desc, mpsxx_tensor* out);  /* device storage, zero-filled           */
+int mpsxx_tensor_create_empty(mpsxx_tensor* out);                        /* size()==0 array, shaped by first use   */
+int mpsxx_tensor_free(mpsxx_tensor t);
+int mpsxx_tensor_clear(mpsxx_tensor t);                                  /* QSTArray::clear                        */
+int mpsxx_tensor_upload(mpsxx_tensor t, const double* host, int64_t nelem, void* stream);
+int mpsxx_tensor_download(mpsxx_tensor t, double* host, int64_t nelem, void* stream);
+/* structure queries: sizes first, then the arrays (caller-allocated with the sizes returned by _info) */
+int mpsxx_tensor_info(mpsxx_tensor t, int32_t* rank, int32_t* qsym, int32_t q_total[2], int64_t* nsec_total,
+                      int64_t* nblocks, int64_t* nelem);
+int mpsxx_tensor_layout(mpsxx_tensor t, int32_t* nsec, int32_t* sec_q, int32_t* sec_dim, int32_t* block_idx,
+                        int64_t* block_off);
+/* device pointer of the FP64 storage (for callers that own streams/graphs) */
+int mpsxx_tensor_device_ptr(mpsxx_tensor t, double** dptr);
+/* copy of x with identical structure and data (btas::Copy) */
+int mpsxx_tensor_copy(mpsxx_tensor x, mpsxx_tensor y, void* stream);
+
+/* ---- BTAS machinery (include/btas/QSPARSE/QSTCONTRACT.h:16-57, QSTBLAS.h:183-226) --------------------------------- */
+/* c = alpha * contract(a[contract_a], b[contract_b]) + beta * c */
+int mpsxx_contract(double alpha, mpsxx_tensor a, int conj_a, const int32_t* contract_a, mpsxx_tensor b, int conj_b,
+                   const int32_t* contract_b, int32_t K, double beta, mpsxx_tensor c, void* stream);
+/* btas::Gemm with CblasNoTrans (0) / CblasTrans (1) for each operand */
+int mpsxx_gemm(int trans_a, int trans_b, double alpha, mpsxx_tensor a, int conj_a, mpsxx_tensor b, int conj_b,
+               double beta, mpsxx_tensor c, void* stream);
+/* btas::Tie (include/btas/SPARSE/STREINDEX.h:95-147): y = x with indices tie[0..K) merged onto tie[0]; y is STArray */
+int mpsxx_tie(mpsxx_tensor x, const int32_t* tie, int32_t K, mpsxx_tensor y, void* stream);
+/* btas::Copy(x, y, UpCast=1) onto y's existing q/qshape (include/btas/SPARSE/STBLAS.h:48-60) */
+int mpsxx_copy_upcast(mpsxx_tensor x, mpsxx_tensor y, void* stream);
+
+/* ---- BLAS-1 on block-sparse arrays (QSTBLAS.h:40-93, 242-252); FP64 accumulation ---------------------------------- */
+int mpsxx_dotc(mpsxx_tensor x, mpsxx_tensor y, double* result, void* stream);     /* synchronous               */
+int mpsxx_nrm2(mpsxx_tensor x, double* result, void* stream);                     /* synchronous               */
+int mpsxx_axpy(double alpha, mpsxx_tensor x, mpsxx_tensor y, void* stream);       /* inserts missing y blocks  */
+int mpsxx_scal(double alpha, mpsxx_tensor x, void* stream);
+
+/* ---- hot path: sigma vector (src/compute_sigma_vector.hpp) -------------------------------------------------------- */
+/* two-site 4-index form: replaces compute_sigma_vector(lmpo,rmpo,lopr,ropr,wfn0,sgv0), compute_sigma_vector.hpp:54-70 */
+int mpsxx_compute_sigma_vector_twosite(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
+                                       mpsxx_tensor wfn0, mpsxx_tensor sgv0, int precision, void* stream);
+/* one-site form: replaces compute_sigma_vector(mpo0,lopr,ropr,wfn0,sgv0), compute_sigma_vector.hpp:10-22 */
+int mpsxx_compute_sigma_vector_onesite(mpsxx_tensor mpo0, mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor wfn0,
+                                       mpsxx_tensor sgv0, int precision, void* stream);
+/* merged 2-index form: replaces compute_sigma_vector(lopr,ropr,wfn0,sgv0) with 3-index operators,
+ * compute_sigma_vector.hpp:86-96 */
+int mpsxx_compute_sigma_vector_merged(mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor wfn0, mpsxx_tensor sgv0,
+                                      int precision, void* stream);
+/* overlap forms S|psi> for 3-, 4- and 2-index wavefunctions: compute_sigma_vector.hpp:25-35, 73-83, 99-109 */
+int mpsxx_compute_sigma_vector_overlap(mpsxx_tensor lsov, mpsxx_tensor rsov, mpsxx_tensor wfn0, mpsxx_tensor sgv0,
+                                       void* stream);
+/* <psi|H^* form: replaces compute_sigma_vector_conj, compute_sigma_vector.hpp:38-51 */
+int mpsxx_compute_sigma_vector_conj(mpsxx_tensor mpo0, mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor wfn0,
+                                    mpsxx_tensor sgv0, void* stream);
+
+/* ---- hot path: diagonal of H_eff (src/compute_diagonal_elements.hpp) --------------------------------------------- */
+/* diag must already carry the wavefunction's q/qshape (reference constructs it that way, optimize.hpp:183) */
+int mpsxx_compute_diagonal_elements_twosite(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr,
+                                            mpsxx_tensor ropr, mpsxx_tensor diag, void* stream);   /* :35-61  */
+int mpsxx_compute_diagonal_elements_onesite(mpsxx_tensor mpo0, mpsxx_tensor lopr, mpsxx_tensor ropr,
+                                            mpsxx_tensor diag, void* stream);                      /* :11-32  */
+int mpsxx_compute_diagonal_elements_merged(mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor diag,
+                                           void* stream);                                          /* :64-80  */
+
+/* ---- hot path: environment renormalization (src/renormalize.hpp) -------------------------------------------------- */
+int mpsxx_renormalize(int forward, mpsxx_tensor mpo0, mpsxx_tensor opr0, mpsxx_tensor bra0, mpsxx_tensor ket0,
+                      mpsxx_tensor opr1, int precision, void* stream);                              /* :10-32  */
+int mpsxx_renormalize_overlap(int forward, mpsxx_tensor sov0, mpsxx_tensor bra0, mpsxx_tensor ket0,
+                              mpsxx_tensor sov1, void* stream);                    /* :36-53 and merged :78-95 */
+int mpsxx_renormalize_merged(int forward, mpsxx_tensor opr0, mpsxx_tensor bra0, mpsxx_tensor ket0,
+                             mpsxx_tensor opr1, int precision, void* stream);                       /* :57-74  */
+
+/* ---- merged operator (src/optimize.hpp:273-368) ------------------------------------------------------------------- */
+int mpsxx_compute_merged_operator(int forward, mpsxx_tensor xmpo, mpsxx_tensor xopr, mpsxx_tensor mopr,
+                                  void* stream);
+
+/* ---- Davidson (src/diagonalize.hpp:310-449) ----------------------------------------------------------------------- */
+/* Ground state of sum_g H_g with H_g given by the two-site operator groups; wfnc is the initial guess and receives the
+ * converged vector; returns the eigenvalue in *energy.  max_ritz = 10, <= 4 macro iterations, convergence on
+ * ||r||^2 < tolerance as in the reference. precision selects the block-GEMM arithmetic of the sigma vectors. */
+int mpsxx_davidson_twosite(int ngroups, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
+                           const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, mpsxx_tensor diag, mpsxx_tensor wfnc,
+                           double tolerance, int precision, double* energy, int32_t* n_sigma, void* stream);
+
+/* ---- plan introspection (measurement) ---------------------------------------------------------------------------- */
+/* Algorithmic cost of the two-site sigma vector for these inputs in the reference contraction order: flops = 2 m n k
+ * summed over every accepted (A-block, B-block) pair of the four Contract steps, bytes = 8 * (|L|+|R|+|W|+|psi|+|sigma|)
+ * counted over allocated blocks; nsteps = number of device steps of the plan. */
+int mpsxx_sigma_twosite_stats(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
+                              mpsxx_tensor wfn0, int precision, int64_t* flops, int64_t* bytes, int32_t* nsteps);
+/* Runs the two-site sigma plan `reps` times into a scratch output and returns, per device step, the mean device time
+ * (HIP events on the launch stream), the algorithmic flops and the kind (0 grouped GEMM, 1 coefficient AXPY, 2 copy). */
+int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
+                                mpsxx_tensor wfn0, int precision, int reps, int32_t max_steps, double* step_ms,
+                                int64_t* step_flops, int32_t* step_kind, int32_t* nsteps, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPSXX_AMD_H */

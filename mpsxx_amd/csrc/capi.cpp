@@ -1,0 +1,366 @@
+// extern "C" boundary: opaque tensor handles, status codes, thread-local error text.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "mpsxx_amd.h"
+#include "ops.hpp"
+
+struct mpsxx_tensor_s {
+  mpx::Tensor t;
+};
+
+namespace {
+
+thread_local std::string g_err;
+bool g_init = false;
+
+void ensure_init() {
+  if (!g_init) {
+    mpx::hip_check(hipSetDevice(0), "hipSetDevice");
+    g_init = true;
+  }
+}
+
+template <typename F>
+int guard(F f) {
+  try {
+    ensure_init();
+    f();
+    return MPSXX_OK;
+  } catch (const mpx::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return MPSXX_ERR_OOM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return MPSXX_ERR_ARG;
+  }
+}
+
+mpx::Tensor& T(mpsxx_tensor h) {
+  if (!h) throw mpx::Error(MPSXX_ERR_ARG, "null tensor handle");
+  return h->t;
+}
+
+hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+mpx::Layout layout_from_desc(const mpsxx_qst_desc* d) {
+  if (!d || d->rank < 1 || d->rank > mpx::kMaxRank || !d->nsec || !d->sec_dim)
+    throw mpx::Error(MPSXX_ERR_ARG, "invalid descriptor");
+  std::vector<std::vector<mpx::QN>> qs(d->rank);
+  std::vector<std::vector<int>> ds(d->rank);
+  int64_t k = 0;
+  for (int i = 0; i < d->rank; ++i) {
+    if (d->nsec[i] < 0) throw mpx::Error(MPSXX_ERR_ARG, "negative sector count");
+    for (int j = 0; j < d->nsec[i]; ++j, ++k) {
+      ds[i].push_back(d->sec_dim[k]);
+      if (d->qsym) {
+        if (!d->sec_q) throw mpx::Error(MPSXX_ERR_ARG, "sec_q required for qsym arrays");
+        qs[i].push_back(mpx::QN(d->sec_q[2 * k], d->sec_q[2 * k + 1]));
+      }
+    }
+  }
+  mpx::Layout L;
+  L.set_shape(d->rank, d->qsym != 0, mpx::QN(d->q_total[0], d->q_total[1]), qs, ds);
+  for (int64_t b = 0; b < d->nblocks; ++b) {
+    const int32_t* idx = d->block_idx + b * d->rank;
+    for (int i = 0; i < d->rank; ++i)
+      if (idx[i] < 0 || idx[i] >= d->nsec[i]) throw mpx::Error(MPSXX_ERR_ARG, "block index out of range");
+    if (!L.allowed(idx)) throw mpx::Error(MPSXX_ERR_QN, "block is not allowed by the quantum numbers");
+    L.add_block(idx);
+  }
+  L.finalize();
+  return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpsxx_init(int device) {
+  try {
+    mpx::hip_check(hipSetDevice(device), "hipSetDevice");
+    g_init = true;
+    return MPSXX_OK;
+  } catch (const mpx::Error& e) {
+    g_err = e.what();
+    return e.code;
+  }
+}
+
+const char* mpsxx_last_error(void) { return g_err.c_str(); }
+
+int mpsxx_synchronize(void* stream) {
+  return guard([&] { mpx::hip_check(hipStreamSynchronize(S(stream)), "hipStreamSynchronize"); });
+}
+
+int mpsxx_tensor_create(const mpsxx_qst_desc* desc, mpsxx_tensor* out) {
+  return guard([&] {
+    if (!out) throw mpx::Error(MPSXX_ERR_ARG, "null output handle");
+    mpx::Layout L = layout_from_desc(desc);
+    auto* h = new mpsxx_tensor_s;
+    try {
+      h->t.set_layout(L, nullptr);
+      mpx::hip_check(hipStreamSynchronize(nullptr), "create");
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mpsxx_tensor_create_empty(mpsxx_tensor* out) {
+  return guard([&] {
+    if (!out) throw mpx::Error(MPSXX_ERR_ARG, "null output handle");
+    *out = new mpsxx_tensor_s;
+  });
+}
+
+int mpsxx_tensor_free(mpsxx_tensor t) {
+  return guard([&] { delete t; });
+}
+
+int mpsxx_tensor_clear(mpsxx_tensor t) {
+  return guard([&] {
+    mpx::Tensor& x = T(t);
+    x.data.release();
+    x.L = mpx::Layout();
+    x.empty = true;
+  });
+}
+
+int mpsxx_tensor_upload(mpsxx_tensor t, const double* host, int64_t nelem, void* stream) {
+  return guard([&] {
+    mpx::Tensor& x = T(t);
+    if (x.empty || nelem != x.L.nelem) throw mpx::Error(MPSXX_ERR_SHAPE, "upload: element count mismatch");
+    if (nelem > 0)
+      mpx::hip_check(hipMemcpyAsync(x.ptr(), host, nelem * sizeof(double), hipMemcpyHostToDevice, S(stream)), "upload");
+    mpx::hip_check(hipStreamSynchronize(S(stream)), "upload sync");
+  });
+}
+
+int mpsxx_tensor_download(mpsxx_tensor t, double* host, int64_t nelem, void* stream) {
+  return guard([&] {
+    mpx::Tensor& x = T(t);
+    const int64_t n = x.empty ? 0 : x.L.nelem;
+    if (nelem != n) throw mpx::Error(MPSXX_ERR_SHAPE, "download: element count mismatch");
+    if (n > 0)
+      mpx::hip_check(hipMemcpyAsync(host, x.ptr(), n * sizeof(double), hipMemcpyDeviceToHost, S(stream)), "download");
+    mpx::hip_check(hipStreamSynchronize(S(stream)), "download sync");
+  });
+}
+
+int mpsxx_tensor_info(mpsxx_tensor t, int32_t* rank, int32_t* qsym, int32_t q_total[2], int64_t* nsec_total,
+                      int64_t* nblocks, int64_t* nelem) {
+  return guard([&] {
+    const mpx::Tensor& x = T(t);
+    if (rank) *rank = x.empty ? 0 : x.L.rank;
+    if (qsym) *qsym = x.L.qsym ? 1 : 0;
+    if (q_total) { q_total[0] = x.L.q.p; q_total[1] = x.L.q.s; }
+    int64_t ns = 0;
+    for (int i = 0; i < x.L.rank; ++i) ns += x.L.nsec(i);
+    if (nsec_total) *nsec_total = x.empty ? 0 : ns;
+    if (nblocks) *nblocks = x.empty ? 0 : x.L.nblocks();
+    if (nelem) *nelem = x.empty ? 0 : x.L.nelem;
+  });
+}
+
+int mpsxx_tensor_layout(mpsxx_tensor t, int32_t* nsec, int32_t* sec_q, int32_t* sec_dim, int32_t* block_idx,
+                        int64_t* block_off) {
+  return guard([&] {
+    const mpx::Tensor& x = T(t);
+    if (x.empty) return;
+    int64_t k = 0;
+    for (int i = 0; i < x.L.rank; ++i) {
+      if (nsec) nsec[i] = x.L.nsec(i);
+      for (int j = 0; j < x.L.nsec(i); ++j, ++k) {
+        if (sec_dim) sec_dim[k] = x.L.ds[i][j];
+        if (sec_q) { sec_q[2 * k] = x.L.qs[i][j].p; sec_q[2 * k + 1] = x.L.qs[i][j].s; }
+      }
+    }
+    if (block_idx && !x.L.bidx.empty()) std::memcpy(block_idx, x.L.bidx.data(), x.L.bidx.size() * sizeof(int32_t));
+    if (block_off && !x.L.off.empty()) std::memcpy(block_off, x.L.off.data(), x.L.off.size() * sizeof(int64_t));
+  });
+}
+
+int mpsxx_tensor_device_ptr(mpsxx_tensor t, double** dptr) {
+  return guard([&] { *dptr = T(t).empty ? nullptr : T(t).ptr(); });
+}
+
+int mpsxx_tensor_copy(mpsxx_tensor x, mpsxx_tensor y, void* stream) {
+  return guard([&] { mpx::op_copy(T(x), T(y), S(stream)); });
+}
+
+int mpsxx_contract(double alpha, mpsxx_tensor a, int conj_a, const int32_t* contract_a, mpsxx_tensor b, int conj_b,
+                   const int32_t* contract_b, int32_t K, double beta, mpsxx_tensor c, void* stream) {
+  return guard([&] {
+    if (K <= 0 || !contract_a || !contract_b) throw mpx::Error(MPSXX_ERR_ARG, "contract: empty index lists");
+    std::vector<int> ca(contract_a, contract_a + K), cb(contract_b, contract_b + K);
+    mpx::op_contract(alpha, T(a), conj_a != 0, ca, T(b), conj_b != 0, cb, beta, T(c), S(stream));
+  });
+}
+
+int mpsxx_gemm(int trans_a, int trans_b, double alpha, mpsxx_tensor a, int conj_a, mpsxx_tensor b, int conj_b,
+               double beta, mpsxx_tensor c, void* stream) {
+  return guard([&] {
+    mpx::op_gemm(trans_a, trans_b, alpha, T(a), conj_a != 0, T(b), conj_b != 0, beta, T(c), S(stream));
+  });
+}
+
+int mpsxx_tie(mpsxx_tensor x, const int32_t* tie, int32_t K, mpsxx_tensor y, void* stream) {
+  return guard([&] {
+    if (K < 2 || !tie) throw mpx::Error(MPSXX_ERR_ARG, "tie: need at least two indices");
+    mpx::op_tie(T(x), std::vector<int>(tie, tie + K), T(y), S(stream));
+  });
+}
+
+int mpsxx_copy_upcast(mpsxx_tensor x, mpsxx_tensor y, void* stream) {
+  return guard([&] { mpx::op_copy_upcast(T(x), T(y), S(stream)); });
+}
+
+int mpsxx_dotc(mpsxx_tensor x, mpsxx_tensor y, double* result, void* stream) {
+  return guard([&] { *result = mpx::op_dotc(T(x), T(y), S(stream)); });
+}
+
+int mpsxx_nrm2(mpsxx_tensor x, double* result, void* stream) {
+  return guard([&] { *result = mpx::op_nrm2(T(x), S(stream)); });
+}
+
+int mpsxx_axpy(double alpha, mpsxx_tensor x, mpsxx_tensor y, void* stream) {
+  return guard([&] { mpx::op_axpy(alpha, T(x), T(y), S(stream)); });
+}
+
+int mpsxx_scal(double alpha, mpsxx_tensor x, void* stream) {
+  return guard([&] { mpx::op_scal(alpha, T(x), S(stream)); });
+}
+
+int mpsxx_compute_sigma_vector_twosite(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
+                                       mpsxx_tensor wfn0, mpsxx_tensor sgv0, int precision, void* stream) {
+  return guard([&] {
+    mpx::op_sigma_twosite(T(lmpo), T(rmpo), T(lopr), T(ropr), T(wfn0), T(sgv0), precision, S(stream));
+  });
+}
+
+int mpsxx_compute_sigma_vector_onesite(mpsxx_tensor mpo0, mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor wfn0,
+                                       mpsxx_tensor sgv0, int precision, void* stream) {
+  return guard([&] { mpx::op_sigma_onesite(T(mpo0), T(lopr), T(ropr), T(wfn0), T(sgv0), precision, S(stream)); });
+}
+
+int mpsxx_compute_sigma_vector_merged(mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor wfn0, mpsxx_tensor sgv0,
+                                      int precision, void* stream) {
+  return guard([&] { mpx::op_sigma_merged(T(lopr), T(ropr), T(wfn0), T(sgv0), precision, S(stream)); });
+}
+
+int mpsxx_compute_sigma_vector_overlap(mpsxx_tensor lsov, mpsxx_tensor rsov, mpsxx_tensor wfn0, mpsxx_tensor sgv0,
+                                       void* stream) {
+  return guard([&] { mpx::op_sigma_overlap(T(lsov), T(rsov), T(wfn0), T(sgv0), S(stream)); });
+}
+
+int mpsxx_compute_sigma_vector_conj(mpsxx_tensor mpo0, mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor wfn0,
+                                    mpsxx_tensor sgv0, void* stream) {
+  return guard([&] { mpx::op_sigma_conj(T(mpo0), T(lopr), T(ropr), T(wfn0), T(sgv0), S(stream)); });
+}
+
+int mpsxx_compute_diagonal_elements_twosite(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr,
+                                            mpsxx_tensor ropr, mpsxx_tensor diag, void* stream) {
+  return guard([&] { mpx::op_diag_twosite(T(lmpo), T(rmpo), T(lopr), T(ropr), T(diag), S(stream)); });
+}
+
+int mpsxx_compute_diagonal_elements_onesite(mpsxx_tensor mpo0, mpsxx_tensor lopr, mpsxx_tensor ropr,
+                                            mpsxx_tensor diag, void* stream) {
+  return guard([&] { mpx::op_diag_onesite(T(mpo0), T(lopr), T(ropr), T(diag), S(stream)); });
+}
+
+int mpsxx_compute_diagonal_elements_merged(mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor diag, void* stream) {
+  return guard([&] { mpx::op_diag_merged(T(lopr), T(ropr), T(diag), S(stream)); });
+}
+
+int mpsxx_renormalize(int forward, mpsxx_tensor mpo0, mpsxx_tensor opr0, mpsxx_tensor bra0, mpsxx_tensor ket0,
+                      mpsxx_tensor opr1, int precision, void* stream) {
+  return guard([&] {
+    mpx::op_renormalize(forward != 0, T(mpo0), T(opr0), T(bra0), T(ket0), T(opr1), precision, S(stream));
+  });
+}
+
+int mpsxx_renormalize_overlap(int forward, mpsxx_tensor sov0, mpsxx_tensor bra0, mpsxx_tensor ket0,
+                              mpsxx_tensor sov1, void* stream) {
+  return guard([&] { mpx::op_renormalize_overlap(forward != 0, T(sov0), T(bra0), T(ket0), T(sov1), S(stream)); });
+}
+
+int mpsxx_renormalize_merged(int forward, mpsxx_tensor opr0, mpsxx_tensor bra0, mpsxx_tensor ket0,
+                             mpsxx_tensor opr1, int precision, void* stream) {
+  return guard([&] {
+    mpx::op_renormalize_merged(forward != 0, T(opr0), T(bra0), T(ket0), T(opr1), precision, S(stream));
+  });
+}
+
+int mpsxx_compute_merged_operator(int forward, mpsxx_tensor xmpo, mpsxx_tensor xopr, mpsxx_tensor mopr,
+                                  void* stream) {
+  return guard([&] {
+    (void)forward; (void)xmpo; (void)xopr; (void)mopr; (void)stream;
+    throw mpx::Error(MPSXX_ERR_UNSUPPORTED, "compute_merged_operator: quantum-number merge not available yet");
+  });
+}
+
+int mpsxx_davidson_twosite(int ngroups, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
+                           const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, mpsxx_tensor diag, mpsxx_tensor wfnc,
+                           double tolerance, int precision, double* energy, int32_t* n_sigma, void* stream) {
+  return guard([&] {
+    if (ngroups <= 0) throw mpx::Error(MPSXX_ERR_ARG, "Davidson: need at least one operator group");
+    std::vector<mpx::DavGroup> g;
+    for (int i = 0; i < ngroups; ++i) g.push_back({&T(lmpo[i]), &T(rmpo[i]), &T(lopr[i]), &T(ropr[i])});
+    mpx::DavidsonStats st;
+    const double e = mpx::davidson_twosite(g, T(diag), T(wfnc), tolerance, precision, st, S(stream));
+    if (energy) *energy = e;
+    if (n_sigma) *n_sigma = st.nsigma;
+  });
+}
+
+int mpsxx_sigma_twosite_stats(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
+                              mpsxx_tensor wfn0, int precision, int64_t* flops, int64_t* bytes, int32_t* nsteps) {
+  return guard([&] {
+    mpx::Plan& P = mpx::plan_sigma_twosite(T(lmpo).L, T(rmpo).L, T(lopr).L, T(ropr).L, T(wfn0).L, nullptr, precision);
+    if (flops) *flops = P.flops;
+    if (bytes) *bytes = P.bytes;
+    if (nsteps) *nsteps = (int32_t)P.steps.size();
+  });
+}
+
+int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
+                                mpsxx_tensor wfn0, int precision, int reps, int32_t max_steps, double* step_ms,
+                                int64_t* step_flops, int32_t* step_kind, int32_t* nsteps, void* stream) {
+  return guard([&] {
+    mpx::Plan& P = mpx::plan_sigma_twosite(T(lmpo).L, T(rmpo).L, T(lopr).L, T(ropr).L, T(wfn0).L, nullptr, precision);
+    const int ns = (int)P.steps.size();
+    if (nsteps) *nsteps = ns;
+    if (ns > max_steps) throw mpx::Error(MPSXX_ERR_ARG, "profile: max_steps too small");
+    mpx::DevBuf out(std::max<int64_t>(P.out.nelem, 1) * sizeof(double));
+    std::vector<hipEvent_t> ev(ns + 1);
+    for (auto& e : ev) mpx::hip_check(hipEventCreate(&e), "event");
+    std::vector<double> acc(ns, 0.0);
+    std::vector<const double*> in{T(lmpo).ptr(), T(rmpo).ptr(), T(lopr).ptr(), T(ropr).ptr(), T(wfn0).ptr()};
+    for (int r = 0; r < reps; ++r) {
+      mpx::run_plan_timed(P, in, out.as<double>(), S(stream), ev);
+      mpx::hip_check(hipEventSynchronize(ev[ns]), "event sync");
+      for (int i = 0; i < ns; ++i) {
+        float ms = 0.f;
+        mpx::hip_check(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "elapsed");
+        acc[i] += ms;
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int i = 0; i < ns; ++i) {
+      if (step_ms) step_ms[i] = acc[i] / std::max(reps, 1);
+      if (step_flops) step_flops[i] = P.steps[i].flops;
+      if (step_kind) step_kind[i] = (int32_t)P.steps[i].kind;
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+// Device Davidson eigensolver for the two-site effective Hamiltonian (reference: src/diagonalize.hpp:283-449).
+//
+// All Davidson vectors share one block layout U: the closure of psi's block set under the sigma-vector pattern map
+// (sigma of a vector on U has its blocks in U).  Blocks the reference would not have allocated hold exact zeros, so
+// every dot, axpy, scaling and the preconditioner give the reference's values; the vectors are then flat device
+// arrays and all BLAS-1 work is contiguous.  Subspace algebra (<= 10 x 10) is FP64 on the host.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "mpsxx_amd.h"
+#include "ops.hpp"
+
+namespace mpx {
+
+namespace {
+
+// Cyclic Jacobi for a symmetric n x n matrix (row-major).  Eigenvalues ascending; eigenvectors in the columns of v
+// (the layout LAPACKE_dsyev returns for row-major storage, TLAPACK.h:23-47).
+void sym_eig(int n, std::vector<double> a, std::vector<double>& w, std::vector<double>& v) {
+  v.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) v[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        tot += a[i * n + j] * a[i * n + j];
+        if (i != j) off += a[i * n + j] * a[i * n + j];
+      }
+    if (off <= 1e-32 * tot || off < 1e-300) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[p * n + q];
+        if (apq == 0.0) continue;
+        const double theta = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = c * akp - s * akq;
+          a[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = c * apk - s * aqk;
+          a[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = v[k * n + p], vkq = v[k * n + q];
+          v[k * n + p] = c * vkp - s * vkq;
+          v[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) { return a[x * n + x] < a[y * n + y]; });
+  w.resize(n);
+  std::vector<double> v2((size_t)n * n);
+  for (int j = 0; j < n; ++j) {
+    w[j] = a[ord[j] * n + ord[j]];
+    for (int i = 0; i < n; ++i) v2[i * n + j] = v[i * n + ord[j]];
+  }
+  v.swap(v2);
+}
+
+}  // namespace
+
+// Closure of a block layout under the sigma pattern maps of all groups.
+Layout davidson_layout(const std::vector<DavGroup>& groups, const Layout& psi) {
+  Layout U = psi;
+  for (int iter = 0; iter < 16; ++iter) {
+    Layout next = U;
+    for (const DavGroup& g : groups) {
+      Plan& P = plan_sigma_twosite(g.lmpo->L, g.rmpo->L, g.lopr->L, g.ropr->L, U, &U, MPSXX_FP64);
+      for (int64_t b = 0; b < P.out.nblocks(); ++b)
+        if (next.find(P.out.tag[b]) < 0) next.add_block(P.out.index(b));
+    }
+    next.finalize();
+    if (next.hash == U.hash) return U;
+    U = next;
+  }
+  return U;
+}
+
+double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tensor& wfnc, double tol, int prec,
+                        DavidsonStats& st, hipStream_t s) {
+  if (wfnc.empty) throw Error(MPSXX_ERR_ARG, "Davidson: empty initial wavefunction");
+  const int max_ritz = 10;
+  Layout U = davidson_layout(groups, wfnc.L);
+  const int64_t n = U.nelem;
+  if (n == 0) throw Error(MPSXX_ERR_ARG, "Davidson: empty vector space");
+  // vector slots: T[0..9], S[0..9], T2[0..9], S2[0..9], errv
+  const int nslot = 4 * max_ritz + 1;
+  DevBuf V((size_t)nslot * n * sizeof(double));
+  double* base = V.as<double>();
+  auto T = [&](int i) { return base + (int64_t)i * n; };
+  auto Sg = [&](int i) { return base + (int64_t)(max_ritz + i) * n; };
+  double* T2 = base + (int64_t)(2 * max_ritz) * n;
+  double* S2 = base + (int64_t)(3 * max_ritz) * n;
+  double* errv = base + (int64_t)(4 * max_ritz) * n;
+  hip_check(hipMemsetAsync(base, 0, (size_t)nslot * n * sizeof(double), s), "memset");
+
+  // trial[0] <- wfnc placed into U
+  for (int64_t b = 0; b < wfnc.L.nblocks(); ++b) {
+    const int64_t j = U.find(wfnc.L.tag[b]);
+    hip_check(hipMemcpyAsync(T(0) + U.off[j], wfnc.ptr() + wfnc.L.off[b], wfnc.L.bsize[b] * sizeof(double),
+                             hipMemcpyDeviceToDevice, s),
+              "davidson init");
+  }
+  // diagonal aligned to U, with the presence mask of diag's blocks (precondition, diagonalize.hpp:289-303)
+  DevBuf dvals(n * sizeof(double)), dmask(n);
+  {
+    std::vector<unsigned char> mask(n, 0);
+    hip_check(hipMemsetAsync(dvals.p, 0, n * sizeof(double), s), "memset");
+    if (!diag.empty) {
+      for (int64_t b = 0; b < U.nblocks(); ++b) {
+        const int64_t j = diag.L.find(U.tag[b]);
+        if (j < 0) continue;
+        if (diag.L.bsize[j] != U.bsize[b]) throw Error(MPSXX_ERR_SHAPE, "Davidson: diag block shape mismatch");
+        hip_check(hipMemcpyAsync(dvals.as<double>() + U.off[b], diag.ptr() + diag.L.off[j], U.bsize[b] * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s),
+                  "diag align");
+        std::memset(mask.data() + U.off[b], 1, U.bsize[b]);
+      }
+    }
+    hip_check(hipMemcpyAsync(dmask.p, mask.data(), n, hipMemcpyHostToDevice, s), "mask");
+    hip_check(hipStreamSynchronize(s), "mask sync");
+  }
+
+  // device scalars: Gram results and combine coefficients
+  const int npart = (int)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 512);
+  DevBuf gram(2 * max_ritz * max_ritz * sizeof(double)), part((size_t)2 * max_ritz * max_ritz * npart * sizeof(double));
+  DevBuf coef(max_ritz * max_ritz * sizeof(double));
+  auto dot = [&](const double* x, const double* y) {
+    launch_gram(x, n, 1, y, n, 1, n, gram.as<double>(), part.as<double>(), npart, s);
+    double h;
+    hip_check(hipMemcpyAsync(&h, gram.p, sizeof(double), hipMemcpyDeviceToHost, s), "dot");
+    hip_check(hipStreamSynchronize(s), "dot sync");
+    return h;
+  };
+  auto normalize = [&](double* x) { launch_scale(x, n, 1.0 / std::sqrt(dot(x, x)), s); };
+  auto axpy = [&](double a, const double* x, double* y) { launch_axpby(a, x, 1.0, y, n, s); };
+
+  std::vector<Plan*> plans;
+  for (const DavGroup& g : groups)
+    plans.push_back(&plan_sigma_twosite(g.lmpo->L, g.rmpo->L, g.lopr->L, g.ropr->L, U, &U, prec));
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  auto multH = [&](const double* c, double* v) {
+    hip_check(hipEventRecord(e0, s), "event");
+    hip_check(hipMemsetAsync(v, 0, n * sizeof(double), s), "memset");
+    for (size_t g = 0; g < groups.size(); ++g)
+      run_plan(*plans[g], {groups[g].lmpo->ptr(), groups[g].rmpo->ptr(), groups[g].lopr->ptr(), groups[g].ropr->ptr(), c},
+               v, s);
+    hip_check(hipEventRecord(e1, s), "event");
+    hip_check(hipEventSynchronize(e1), "event sync");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    st.sigma_ms += ms;
+    ++st.nsigma;
+  };
+
+  double eval = 0.0;
+  int niter = 0, iconv = 0;
+  std::vector<double> G(max_ritz * max_ritz), R;
+  while (iconv < 1 && niter < 4) {
+    normalize(T(0));
+    multH(T(0), Sg(0));
+    for (int m = 1; m <= max_ritz; ++m) {
+      // heff(i,j) = <trial_i|sigma_j>  (diagonalize.hpp:339-365; the overlap matrix is computed but unused there)
+      launch_gram(T(0), n, m, Sg(0), n, m, n, gram.as<double>(), part.as<double>(), npart, s);
+      hip_check(hipMemcpyAsync(G.data(), gram.p, m * m * sizeof(double), hipMemcpyDeviceToHost, s), "gram");
+      hip_check(hipStreamSynchronize(s), "gram sync");
+      std::vector<double> heff(m * m);
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j <= i; ++j) heff[i * m + j] = heff[j * m + i] = G[i * m + j];  // H(i,j) = <t_i|s_j>, j<=i
+      std::vector<double> rval;
+      sym_eig(m, heff, rval, R);
+      eval = rval[0];
+      // rotate: trial_j <- sum_i rvec(i,j) trial_i (and sigma), diagonalize.hpp:378-394
+      for (int j = 0; j < m; ++j) {
+        std::vector<double> c(m);
+        for (int i = 0; i < m; ++i) c[i] = R[i * m + j];
+        hip_check(hipMemcpyAsync(coef.as<double>() + j * max_ritz, c.data(), m * sizeof(double),
+                                 hipMemcpyHostToDevice, s),
+                  "coef");
+      }
+      // stage the rotated vectors in T2/S2, then copy back
+      for (int j = 0; j < m; ++j) {
+        launch_combine(T(0), n, m, coef.as<double>() + j * max_ritz, T2 + (int64_t)j * n, n, s);
+        launch_combine(Sg(0), n, m, coef.as<double>() + j * max_ritz, S2 + (int64_t)j * n, n, s);
+      }
+      hip_check(hipMemcpyAsync(T(0), T2, (size_t)m * n * sizeof(double), hipMemcpyDeviceToDevice, s), "rotate");
+      hip_check(hipMemcpyAsync(Sg(0), S2, (size_t)m * n * sizeof(double), hipMemcpyDeviceToDevice, s), "rotate");
+      // error vector r = sigma_0 - E trial_0 and ||r||^2 < tol (diagonalize.hpp:416-422)
+      hip_check(hipMemcpyAsync(errv, Sg(0), n * sizeof(double), hipMemcpyDeviceToDevice, s), "errv");
+      axpy(-eval, T(0), errv);
+      const double rnorm = dot(errv, errv);
+      if (rnorm < tol) {
+        ++iconv;
+        break;
+      }
+      if (m < max_ritz) {
+        launch_precondition(errv, dvals.as<double>(), dmask.as<unsigned char>(), eval, n, s);
+        for (int i = 0; i < m; ++i) {
+          normalize(errv);
+          axpy(-dot(T(i), errv), T(i), errv);
+        }
+        normalize(errv);
+        hip_check(hipMemcpyAsync(T(m), errv, n * sizeof(double), hipMemcpyDeviceToDevice, s), "trial");
+        multH(T(m), Sg(m));
+      }
+    }
+    ++niter;
+  }
+  // wfnc <- normalized trial_0 on layout U (Functor::orthonormalize with no lower roots)
+  normalize(T(0));
+  wfnc.set_layout(U, s);
+  hip_check(hipMemcpyAsync(wfnc.ptr(), T(0), n * sizeof(double), hipMemcpyDeviceToDevice, s), "result");
+  hip_check(hipStreamSynchronize(s), "davidson end");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return eval;
+}
+
+}  // namespace mpx
+

@@ -1,0 +1,609 @@
+// Block-sparse contraction planner (host, integer-only) and plan execution.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#include "mpsxx_amd.h"
+
+namespace mpx {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    if (e == hipErrorOutOfMemory) throw Error(MPSXX_ERR_OOM, std::string(what) + ": out of device memory");
+    throw Error(MPSXX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Layout
+
+void Layout::set_shape(int rank_, bool qsym_, QN q_, std::vector<std::vector<QN>> qs_,
+                       std::vector<std::vector<int>> ds_) {
+  if (rank_ < 1 || rank_ > kMaxRank) throw Error(MPSXX_ERR_ARG, "rank out of range");
+  rank = rank_;
+  qsym = qsym_;
+  q = q_;
+  qs = std::move(qs_);
+  ds = std::move(ds_);
+  if ((int)ds.size() != rank) throw Error(MPSXX_ERR_ARG, "dshape rank mismatch");
+  if (qsym) {
+    if ((int)qs.size() != rank) throw Error(MPSXX_ERR_ARG, "qshape rank mismatch");
+    for (int i = 0; i < rank; ++i)
+      if (qs[i].size() != ds[i].size()) throw Error(MPSXX_ERR_ARG, "qshape/dshape sector count mismatch");
+  } else {
+    qs.assign(rank, {});
+    for (int i = 0; i < rank; ++i) qs[i].assign(ds[i].size(), QN());
+  }
+  for (int i = 0; i < rank; ++i)
+    for (int d : ds[i])
+      if (d < 0) throw Error(MPSXX_ERR_ARG, "negative sector dimension");
+  tag.clear();
+  bidx.clear();
+  off.clear();
+  bsize.clear();
+  nelem = 0;
+}
+
+void Layout::finalize() {
+  const int64_t nb = (int64_t)tag.size();
+  std::vector<int64_t> order(nb);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return tag[x] < tag[y]; });
+  std::vector<int64_t> t2;
+  std::vector<int32_t> b2;
+  t2.reserve(nb);
+  b2.reserve(nb * rank);
+  for (int64_t k = 0; k < nb; ++k) {
+    const int64_t o = order[k];
+    if (!t2.empty() && t2.back() == tag[o]) continue;
+    t2.push_back(tag[o]);
+    b2.insert(b2.end(), bidx.begin() + o * rank, bidx.begin() + (o + 1) * rank);
+  }
+  tag.swap(t2);
+  bidx.swap(b2);
+  off.resize(tag.size());
+  bsize.resize(tag.size());
+  nelem = 0;
+  for (size_t b = 0; b < tag.size(); ++b) {
+    off[b] = nelem;
+    bsize[b] = block_elems(index(b));
+    nelem += bsize[b];
+  }
+  uint64_t h = 1469598103934665603ull;
+  h = fnv(h, (uint64_t)rank);
+  h = fnv(h, (uint64_t)qsym);
+  h = fnv(h, (uint64_t)(uint32_t)q.p);
+  h = fnv(h, (uint64_t)(uint32_t)q.s);
+  for (int i = 0; i < rank; ++i) {
+    h = fnv(h, ds[i].size());
+    for (size_t k = 0; k < ds[i].size(); ++k) {
+      h = fnv(h, (uint64_t)(uint32_t)ds[i][k]);
+      h = fnv(h, (uint64_t)(uint32_t)qs[i][k].p);
+      h = fnv(h, (uint64_t)(uint32_t)qs[i][k].s);
+    }
+  }
+  for (int64_t t : tag) h = fnv(h, (uint64_t)t);
+  hash = h;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// DevBuf / Tensor
+
+DevBuf::~DevBuf() { release(); }
+
+DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
+  if (this != &o) {
+    release();
+    p = o.p;
+    bytes = o.bytes;
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  return *this;
+}
+
+void DevBuf::alloc(size_t n) {
+  release();
+  if (n == 0) return;
+  hip_check(hipMalloc(&p, n), "hipMalloc");
+  bytes = n;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+void Tensor::set_layout(const Layout& l, hipStream_t s) {
+  L = l;
+  data.alloc(std::max<int64_t>(L.nelem, 1) * sizeof(double));
+  hip_check(hipMemsetAsync(data.p, 0, std::max<int64_t>(L.nelem, 1) * sizeof(double), s), "memset");
+  empty = false;
+}
+
+void Tensor::relayout(const Layout& l, hipStream_t s) {
+  if (empty) {
+    set_layout(l, s);
+    return;
+  }
+  if (l.hash == L.hash) return;
+  DevBuf nb(std::max<int64_t>(l.nelem, 1) * sizeof(double));
+  hip_check(hipMemsetAsync(nb.p, 0, std::max<int64_t>(l.nelem, 1) * sizeof(double), s), "memset");
+  for (int64_t b = 0; b < L.nblocks(); ++b) {
+    const int64_t j = l.find(L.tag[b]);
+    if (j < 0) continue;
+    hip_check(hipMemcpyAsync(nb.as<double>() + l.off[j], ptr() + L.off[b], L.bsize[b] * sizeof(double),
+                             hipMemcpyDeviceToDevice, s),
+              "relayout copy");
+  }
+  hip_check(hipStreamSynchronize(s), "relayout sync");
+  data = std::move(nb);
+  L = l;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Symbolic contraction
+
+namespace {
+
+// Row-major strides of one block.
+void block_strides(const Layout& L, const int32_t* idx, int64_t* dims, int64_t* st) {
+  int64_t s = 1;
+  for (int i = L.rank - 1; i >= 0; --i) {
+    dims[i] = L.ds[i][idx[i]];
+    st[i] = s;
+    s *= dims[i];
+  }
+}
+
+// A group of indices viewed as one flattened dimension of a permuted block: the non-unit dimensions must appear in
+// the same relative order as stored and be packed (stride[g_i] == stride[g_{i+1}] * dim[g_{i+1}]); then the
+// flattened index advances with the stride of the last non-unit member.  This is exactly the memory walk of the
+// reference's Permute copy (QSTREINDEX.h:22-50) for the index orders that occur on the path.
+bool group_view(const std::vector<int>& g, const int64_t* dims, const int64_t* st, int64_t& size, int64_t& stride) {
+  size = 1;
+  stride = 0;
+  int last = -1;
+  for (int i : g) {
+    if (dims[i] == 1) continue;
+    if (dims[i] == 0) { size = 0; return true; }
+    if (last >= 0 && st[last] != st[i] * dims[i]) return false;
+    size *= dims[i];
+    last = i;
+  }
+  if (last >= 0) stride = st[last];
+  return true;
+}
+
+}  // namespace
+
+void gemm_indices(int transa, int transb, int NA, int NB, int NC, std::vector<int>& ca, std::vector<int>& cb) {
+  const int K = (NA + NB - NC) / 2;
+  if (K <= 0 || 2 * K != NA + NB - NC) throw Error(MPSXX_ERR_ARG, "Gemm: inconsistent ranks");
+  ca.clear();
+  cb.clear();
+  // btas::Gemm contracts the last K indices of A (first K if transposed) with the first K of B (last K if
+  // transposed); QSTBLAS.h:213-225 with transposed_view (STArray.h:558-577)
+  for (int k = 0; k < K; ++k) ca.push_back(transa ? k : NA - K + k);
+  for (int k = 0; k < K; ++k) cb.push_back(transb ? NB - K + k : k);
+}
+
+ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const View& Bv, const std::vector<int>& cb,
+                              const Layout* existing) {
+  const Layout& a = *Av.L;
+  const Layout& b = *Bv.L;
+  const int K = (int)ca.size();
+  const int NA = a.rank, NB = b.rank;
+  if ((int)cb.size() != K || K <= 0 || K >= NA || K >= NB)
+    throw Error(MPSXX_ERR_UNSUPPORTED, "Contract: only 0 < K < rank(A), rank(B) is supported");
+  std::vector<char> inA(NA, 0), inB(NB, 0);
+  for (int k = 0; k < K; ++k) {
+    if (ca[k] < 0 || ca[k] >= NA || inA[ca[k]]) throw Error(MPSXX_ERR_ARG, "Contract: bad A index list");
+    if (cb[k] < 0 || cb[k] >= NB || inB[cb[k]]) throw Error(MPSXX_ERR_ARG, "Contract: bad B index list");
+    inA[ca[k]] = 1;
+    inB[cb[k]] = 1;
+  }
+  std::vector<int> ua, ub;
+  for (int i = 0; i < NA; ++i)
+    if (!inA[i]) ua.push_back(i);
+  for (int i = 0; i < NB; ++i)
+    if (!inB[i]) ub.push_back(i);
+
+  const bool qsym = a.qsym && b.qsym;
+  for (int k = 0; k < K; ++k) {
+    const int ia = ca[k], ib = cb[k];
+    if (a.ds[ia].size() != b.ds[ib].size()) throw Error(MPSXX_ERR_QN, "Contract: contracted sector counts differ");
+    for (size_t s = 0; s < a.ds[ia].size(); ++s) {
+      if (qsym && Av.qs(ia, (int)s) != -Bv.qs(ib, (int)s))
+        throw Error(MPSXX_ERR_QN, "contraction of quantum numbers failed");  // btas_contract_qshape.h:87-107
+      if (a.ds[ia][s] != b.ds[ib][s]) throw Error(MPSXX_ERR_SHAPE, "Contract: contracted dense shapes differ");
+    }
+  }
+
+  ContractSym R;
+  Layout& C = R.C;
+  {
+    std::vector<std::vector<QN>> qs;
+    std::vector<std::vector<int>> ds;
+    for (int i : ua) {
+      std::vector<QN> v;
+      for (size_t s = 0; s < a.ds[i].size(); ++s) v.push_back(Av.qs(i, (int)s));
+      qs.push_back(v);
+      ds.push_back(a.ds[i]);
+    }
+    for (int i : ub) {
+      std::vector<QN> v;
+      for (size_t s = 0; s < b.ds[i].size(); ++s) v.push_back(Bv.qs(i, (int)s));
+      qs.push_back(v);
+      ds.push_back(b.ds[i]);
+    }
+    C.set_shape((int)(ua.size() + ub.size()), qsym, Av.q() + Bv.q(), qs, ds);
+  }
+  if (existing && existing->rank > 0) {
+    if (existing->rank != C.rank) throw Error(MPSXX_ERR_SHAPE, "output rank mismatch");
+    if (qsym && existing->qsym) {
+      if (existing->q != C.q) throw Error(MPSXX_ERR_QN, "Gemm(QSPARSE): quantum number of c must equal to a.q() + b.q().");
+      if (!existing->same_qshape(C)) throw Error(MPSXX_ERR_QN, "Gemm(QSPARSE): c must have the same quantum shape of [ a * b ].");
+    }
+    if (!existing->same_dshape(C)) throw Error(MPSXX_ERR_SHAPE, "output dense shape mismatch");
+  }
+
+  // group blocks by their contracted sector tuple
+  auto keyA = [&](const int32_t* idx) {
+    int64_t t = 0;
+    for (int k = 0; k < K; ++k) t = t * a.nsec(ca[k]) + idx[ca[k]];
+    return t;
+  };
+  auto keyB = [&](const int32_t* idx) {
+    int64_t t = 0;
+    for (int k = 0; k < K; ++k) t = t * b.nsec(cb[k]) + idx[cb[k]];
+    return t;
+  };
+  std::unordered_map<int64_t, std::vector<int64_t>> bmap;
+  bmap.reserve(b.nblocks() * 2 + 1);
+  for (int64_t j = 0; j < b.nblocks(); ++j) bmap[keyB(b.index(j))].push_back(j);
+
+  struct Contrib {
+    int64_t ctag, key, ia, ib;
+  };
+  std::vector<Contrib> con;
+  std::vector<int32_t> cidx(C.rank);
+  // sum of the C QNs for the allowed check
+  for (int64_t i = 0; i < a.nblocks(); ++i) {
+    const int32_t* ai = a.index(i);
+    const int64_t kk = keyA(ai);
+    auto it = bmap.find(kk);
+    if (it == bmap.end()) continue;
+    for (int64_t j : it->second) {
+      const int32_t* bj = b.index(j);
+      int n = 0;
+      for (int x : ua) cidx[n++] = ai[x];
+      for (int x : ub) cidx[n++] = bj[x];
+      if (!C.allowed(cidx.data())) continue;  // STBLAS.h:331 (c.allowed)
+      con.push_back({C.compute_tag(cidx.data()), kk, i, j});
+    }
+  }
+  std::sort(con.begin(), con.end(), [](const Contrib& x, const Contrib& y) {
+    return x.ctag != y.ctag ? x.ctag < y.ctag : x.key < y.key;
+  });
+
+  // output structure: existing blocks U contributed blocks
+  if (existing && existing->rank > 0) {
+    for (int64_t k = 0; k < existing->nblocks(); ++k) C.add_block(existing->index(k));
+  }
+  {
+    int64_t last = -1;
+    for (const Contrib& x : con) {
+      if (x.ctag == last) continue;
+      last = x.ctag;
+      const int32_t* ai = a.index(x.ia);
+      const int32_t* bj = b.index(x.ib);
+      int n = 0;
+      for (int y : ua) cidx[n++] = ai[y];
+      for (int y : ub) cidx[n++] = bj[y];
+      C.add_block(cidx.data());
+    }
+  }
+  C.finalize();
+
+  // tasks and strided pair views
+  int64_t dimsA[kMaxRank], stA[kMaxRank], dimsB[kMaxRank], stB[kMaxRank];
+  size_t p = 0;
+  while (p < con.size()) {
+    size_t q = p;
+    while (q < con.size() && con[q].ctag == con[p].ctag) ++q;
+    const int64_t cb_ = C.find(con[p].ctag);
+    GemmTask t;
+    t.c_off = C.off[cb_];
+    t.pair_begin = (int32_t)R.pairs.size();
+    t.pair_count = (int32_t)(q - p);
+    int64_t m = -1, n = -1;
+    for (size_t r = p; r < q; ++r) {
+      const int32_t* ai = a.index(con[r].ia);
+      const int32_t* bj = b.index(con[r].ib);
+      block_strides(a, ai, dimsA, stA);
+      block_strides(b, bj, dimsB, stB);
+      int64_t mm, sm, kk, sk, kb, skb, nn, sn;
+      if (!group_view(ua, dimsA, stA, mm, sm) || !group_view(ca, dimsA, stA, kk, sk) ||
+          !group_view(cb, dimsB, stB, kb, skb) || !group_view(ub, dimsB, stB, nn, sn))
+        throw Error(MPSXX_ERR_UNSUPPORTED, "Contract: block permutation is not a strided 2-D view");
+      if (kk != kb) throw Error(MPSXX_ERR_SHAPE, "Contract: contracted block dims differ");
+      if (m < 0) { m = mm; n = nn; }
+      GemmPair gp;
+      gp.a_off = a.off[con[r].ia];
+      gp.b_off = b.off[con[r].ib];
+      gp.sa_m = (int32_t)sm;
+      gp.sa_k = (int32_t)sk;
+      gp.sb_k = (int32_t)skb;
+      gp.sb_n = (int32_t)sn;
+      gp.k = (int32_t)kk;
+      gp.pad = 0;
+      // a unit group carries stride 0; the kernel only ever touches index 0 of it, and the stride choice merely
+      // selects which global->LDS thread mapping is used
+      R.pairs.push_back(gp);
+      R.flops += 2 * mm * nn * kk;
+    }
+    t.m = (int32_t)m;
+    t.n = (int32_t)n;
+    R.tasks.push_back(t);
+    p = q;
+  }
+  return R;
+}
+
+Layout tie_layout(const Layout& X, const std::vector<int>& tie, std::vector<CopySeg>& segs) {
+  const int N = X.rank, K = (int)tie.size();
+  if (K < 2) throw Error(MPSXX_ERR_ARG, "Tie: need at least two indices");
+  std::vector<char> tied(N, 0);
+  for (int i : tie) {
+    if (i < 0 || i >= N) throw Error(MPSXX_ERR_ARG, "Tie: index out of range");
+    tied[i] = 1;
+  }
+  std::vector<int> uniq;
+  for (int i = 0; i < N; ++i) {
+    if (!tied[i] || i == tie[0]) uniq.push_back(i);
+    else if (X.ds[i].size() != X.ds[tie[0]].size()) throw Error(MPSXX_ERR_SHAPE, "Tie(DENSE): index to be tied must be the same.");
+  }
+  Layout Y;
+  {
+    std::vector<std::vector<int>> ds;
+    for (int i : uniq) ds.push_back(X.ds[i]);
+    Y.set_shape((int)uniq.size(), false, QN(), {}, ds);
+  }
+  std::vector<int64_t> src;
+  std::vector<int32_t> yi(Y.rank);
+  for (int64_t b = 0; b < X.nblocks(); ++b) {
+    const int32_t* xi = X.index(b);
+    bool ok = true;
+    for (int k = 1; k < K; ++k) ok &= (xi[tie[k]] == xi[tie[0]]);
+    if (!ok) continue;
+    for (int n = 0; n < Y.rank; ++n) yi[n] = xi[uniq[n]];
+    Y.add_block(yi.data());
+  }
+  Y.finalize();
+  segs.clear();
+  int64_t dims[kMaxRank], st[kMaxRank];
+  for (int64_t b = 0; b < X.nblocks(); ++b) {
+    const int32_t* xi = X.index(b);
+    bool ok = true;
+    for (int k = 1; k < K; ++k) ok &= (xi[tie[k]] == xi[tie[0]]);
+    if (!ok) continue;
+    for (int n = 0; n < Y.rank; ++n) yi[n] = xi[uniq[n]];
+    const int64_t yb = Y.find(Y.compute_tag(yi.data()));
+    block_strides(X, xi, dims, st);
+    for (int k = 1; k < K; ++k)
+      if (dims[tie[k]] != dims[tie[0]]) throw Error(MPSXX_ERR_SHAPE, "Tie: tied dense dims differ");
+    // output dims / source strides
+    std::vector<int64_t> od, os;
+    for (int n = 0; n < Y.rank; ++n) {
+      const int i = uniq[n];
+      od.push_back(dims[i]);
+      int64_t s = st[i];
+      if (i == tie[0]) {
+        s = 0;
+        for (int k = 0; k < K; ++k) s += st[tie[k]];
+      }
+      os.push_back(s);
+    }
+    // innermost dimension forms the segment, outer dims enumerate segments
+    const int inner = Y.rank - 1;
+    int64_t outer = 1;
+    for (int n = 0; n < inner; ++n) outer *= od[n];
+    if (od[inner] == 0) continue;
+    for (int64_t o = 0; o < outer; ++o) {
+      // o enumerates the outer output dims row-major; the destination of segment o starts at o * od[inner]
+      int64_t rem = o, soff = 0;
+      for (int n = inner - 1; n >= 0; --n) {
+        const int64_t c = rem % od[n];
+        rem /= od[n];
+        soff += c * os[n];
+      }
+      segs.push_back({Y.off[yb] + o * od[inner], X.off[b] + soff, (int32_t)od[inner], (int32_t)os[inner]});
+    }
+  }
+  return Y;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Steps
+
+Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumulate, double alpha) {
+  Step st;
+  st.a = a;
+  st.b = b;
+  st.c = c;
+  st.accumulate = accumulate;
+  st.alpha = alpha;
+  st.flops = sym.flops;
+  bool scalarB = true, scalarA = true;
+  for (const GemmTask& t : sym.tasks) {
+    for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) {
+      const GemmPair& g = sym.pairs[p];
+      if (!(g.k == 1 && t.n == 1)) scalarB = false;
+      if (!(g.k == 1 && t.m == 1)) scalarA = false;
+    }
+  }
+  if (scalarB || scalarA) {
+    // C[r] += S * V[r*stride]: the vector side is the one whose free extent is the block
+    st.kind = Step::AXPY;
+    if (!scalarB) std::swap(st.a, st.b);  // V must be slot a
+    std::vector<AxpyTask> tasks;
+    std::vector<AxpyPair> pairs;
+    std::vector<Chunk> chunks;
+    for (size_t ti = 0; ti < sym.tasks.size(); ++ti) {
+      const GemmTask& t = sym.tasks[ti];
+      AxpyTask at;
+      at.c_off = t.c_off;
+      at.len = scalarB ? t.m : t.n;
+      at.pair_begin = (int32_t)pairs.size();
+      at.pair_count = t.pair_count;
+      at.pad = 0;
+      for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) {
+        const GemmPair& g = sym.pairs[p];
+        AxpyPair ap;
+        if (scalarB) {
+          ap.v_off = g.a_off;
+          ap.v_stride = g.sa_m;
+          ap.s_off = g.b_off;
+        } else {
+          ap.v_off = g.b_off;
+          ap.v_stride = g.sb_n;
+          ap.s_off = g.a_off;
+        }
+        ap.pad = 0;
+        pairs.push_back(ap);
+      }
+      for (int s = 0; s < at.len; s += kAxpyChunk) chunks.push_back({(int32_t)ti, s});
+      tasks.push_back(at);
+    }
+    st.nwork = (int)chunks.size();
+    st.work = upload_vec(chunks);
+    st.tasks = upload_vec(tasks);
+    st.pairs = upload_vec(pairs);
+    return st;
+  }
+  st.kind = Step::GEMM;
+  std::vector<GemmTile> tiles;
+  // heavy tasks first so the tail of the launch is made of small tiles
+  std::vector<size_t> order(sym.tasks.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::vector<int64_t> cost(sym.tasks.size(), 0);
+  for (size_t ti = 0; ti < sym.tasks.size(); ++ti) {
+    const GemmTask& t = sym.tasks[ti];
+    int64_t ksum = 0;
+    for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) ksum += sym.pairs[p].k;
+    cost[ti] = ksum;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  for (size_t ti : order) {
+    const GemmTask& t = sym.tasks[ti];
+    for (int r = 0; r < t.m; r += kGemmTile)
+      for (int c = 0; c < t.n; c += kGemmTile) tiles.push_back({(int32_t)ti, r, c, 0});
+  }
+  st.nwork = (int)tiles.size();
+  st.work = upload_vec(tiles);
+  st.tasks = upload_vec(sym.tasks);
+  st.pairs = upload_vec(sym.pairs);
+  return st;
+}
+
+Step make_copy_step(const std::vector<CopySeg>& segs, int x, int y, double alpha, double beta) {
+  Step st;
+  st.kind = Step::COPY;
+  st.a = x;
+  st.c = y;
+  st.alpha = alpha;
+  st.beta = beta;
+  std::vector<Chunk> chunks;
+  for (size_t i = 0; i < segs.size(); ++i)
+    for (int s = 0; s < segs[i].len; s += kAxpyChunk) chunks.push_back({(int32_t)i, s});
+  st.nwork = (int)chunks.size();
+  st.work = upload_vec(chunks);
+  st.tasks = upload_vec(segs);
+  return st;
+}
+
+void run_plan(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s) {
+  run_plan_timed(P, inputs, out, s, {});
+}
+
+void run_plan_timed(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
+                    const std::vector<hipEvent_t>& events) {
+  const int ns = (int)P.slot_layout.size();
+  std::vector<void*> ptr(ns, nullptr);
+  for (int i = 0; i < ns; ++i) {
+    if (P.slot_input[i] >= 0) {
+      const double* src = inputs[P.slot_input[i]];
+      if (P.slot_prec[i] == 1) {
+        launch_convert<double, float>(src, P.slot_buf[i].as<float>(), P.slot_layout[i].nelem, s);
+        ptr[i] = P.slot_buf[i].p;
+      } else {
+        ptr[i] = const_cast<double*>(src);
+      }
+    } else if (P.slot_input[i] == -2) {
+      ptr[i] = out;
+    } else {
+      ptr[i] = P.slot_buf[i].p;
+    }
+  }
+  for (size_t si = 0; si < P.steps.size(); ++si) {
+    Step& st = P.steps[si];
+    if (!events.empty()) hip_check(hipEventRecord(events[si], s), "event");
+    const bool f32 = P.slot_prec[st.a] == 1;
+    const bool cf32 = P.slot_prec[st.c] == 1;
+    switch (st.kind) {
+      case Step::GEMM:
+        if (!f32)
+          launch_gemm<double, double>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
+                                      st.pairs.as<GemmPair>(), (const double*)ptr[st.a], (const double*)ptr[st.b],
+                                      (double*)ptr[st.c], st.alpha, st.accumulate, s);
+        else if (cf32)
+          launch_gemm<float, float>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
+                                    st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
+                                    (float*)ptr[st.c], st.alpha, st.accumulate, s);
+        else
+          launch_gemm<float, double>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
+                                     st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
+                                     (double*)ptr[st.c], st.alpha, st.accumulate, s);
+        break;
+      case Step::AXPY:
+        if (!f32)
+          launch_axpy<double, double>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(),
+                                      st.pairs.as<AxpyPair>(), (const double*)ptr[st.a], (const double*)ptr[st.b],
+                                      (double*)ptr[st.c], st.alpha, st.accumulate, s);
+        else if (cf32)
+          launch_axpy<float, float>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(), st.pairs.as<AxpyPair>(),
+                                    (const float*)ptr[st.a], (const float*)ptr[st.b], (float*)ptr[st.c], st.alpha,
+                                    st.accumulate, s);
+        else
+          launch_axpy<float, double>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(),
+                                     st.pairs.as<AxpyPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
+                                     (double*)ptr[st.c], st.alpha, st.accumulate, s);
+        break;
+      case Step::COPY:
+        if (!f32 && !cf32)
+          launch_copy<double, double>(st.work.as<Chunk>(), st.nwork, st.tasks.as<CopySeg>(),
+                                      (const double*)ptr[st.a], (double*)ptr[st.c], st.alpha, st.beta, s);
+        else if (f32 && cf32)
+          launch_copy<float, float>(st.work.as<Chunk>(), st.nwork, st.tasks.as<CopySeg>(), (const float*)ptr[st.a],
+                                    (float*)ptr[st.c], st.alpha, st.beta, s);
+        else if (!f32 && cf32)
+          launch_copy<double, float>(st.work.as<Chunk>(), st.nwork, st.tasks.as<CopySeg>(),
+                                     (const double*)ptr[st.a], (float*)ptr[st.c], st.alpha, st.beta, s);
+        else
+          launch_copy<float, double>(st.work.as<Chunk>(), st.nwork, st.tasks.as<CopySeg>(), (const float*)ptr[st.a],
+                                     (double*)ptr[st.c], st.alpha, st.beta, s);
+        break;
+    }
+  }
+  if (!events.empty()) hip_check(hipEventRecord(events[P.steps.size()], s), "event");
+  hip_check(hipGetLastError(), "kernel launch");
+}
+
+const char* kind_name(int k) { return k == Step::GEMM ? "gemm" : k == Step::AXPY ? "axpy" : "copy"; }
+
+}  // namespace mpx

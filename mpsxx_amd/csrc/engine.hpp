@@ -1,0 +1,117 @@
+// Device tensors, the block-sparse contraction planner and composite-op plans.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace mpx {
+
+void hip_check(hipError_t e, const char* what);
+
+// Owning device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n) { alloc(n); }
+  ~DevBuf();
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept;
+  void alloc(size_t n);
+  void release();
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Host vector -> new device buffer (synchronous copy).
+template <typename T>
+DevBuf upload_vec(const std::vector<T>& v) {
+  DevBuf b(v.size() * sizeof(T));
+  if (!v.empty()) hip_check(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+  return b;
+}
+
+// Device-resident block-sparse FP64 array.
+struct Tensor {
+  Layout L;
+  DevBuf data;      // nelem doubles
+  bool empty = true;  // size() == 0 in the reference sense (no structure yet)
+
+  double* ptr() const { return data.as<double>(); }
+  void set_layout(const Layout& l, hipStream_t s);       // allocate + zero
+  void relayout(const Layout& l, hipStream_t s);         // keep data of blocks present in both
+};
+
+// View of a tensor operand, optionally conjugated (QSTArray::conjugate: QNs negated, data shared).
+struct View {
+  const Layout* L = nullptr;
+  bool conj = false;
+  QN q() const { return conj ? -L->q : L->q; }
+  QN qs(int i, int k) const { return conj ? -L->qs[i][k] : L->qs[i][k]; }
+};
+
+// Result of the symbolic Contract: output structure + one pair list per output block.
+struct ContractSym {
+  Layout C;                               // finalized output structure
+  std::vector<GemmTask> tasks;            // one per output block receiving contributions; c_off in C's layout
+  std::vector<GemmPair> pairs;
+  int64_t flops = 0;                      // 2 * sum m*n*k over pairs
+};
+
+// Symbolic BTAS Contract(a[ca], b[cb]) (include/btas/QSPARSE/QSTCONTRACT.h:16-57 + STBLAS.h:298-359).  If `existing`
+// is non-null and non-empty, its q/qshape are checked and its blocks kept (output = existing U new).
+ContractSym contract_symbolic(const View& A, const std::vector<int>& ca, const View& B, const std::vector<int>& cb,
+                              const Layout* existing);
+
+// Index lists of btas::Gemm(transa, transb) for ranks (NA, NB, NC) (QSTBLAS.h:183-226).
+void gemm_indices(int transa, int transb, int NA, int NB, int NC, std::vector<int>& ca, std::vector<int>& cb);
+
+// Structure of btas::Tie (STREINDEX.h:95-147) and its gather segments.
+Layout tie_layout(const Layout& X, const std::vector<int>& tie, std::vector<CopySeg>& segs);
+
+// One device step of a plan.
+struct Step {
+  enum Kind { GEMM, AXPY, COPY } kind = GEMM;
+  int a = -1, b = -1, c = -1;  // slots: GEMM (A,B,C); AXPY (V,S,C); COPY (x,-,y)
+  int accumulate = 0;
+  double alpha = 1.0, beta = 0.0;
+  int nwork = 0;               // tiles or chunks
+  DevBuf work, tasks, pairs;   // device copies of the task lists
+  int64_t flops = 0;
+};
+
+// Slot table: inputs are bound per call; intermediates are owned by the plan.
+struct Plan {
+  std::vector<Step> steps;
+  std::vector<Layout> slot_layout;     // per slot
+  std::vector<int> slot_input;         // >= 0: index of the bound input tensor; -1 intermediate; -2 output
+  std::vector<DevBuf> slot_buf;        // owned storage of intermediates (FP64 or FP32)
+  std::vector<int> slot_prec;          // 0 FP64, 1 FP32
+  Layout out;                          // output structure
+  int64_t flops = 0;                   // algorithmic flops (reference order)
+  int64_t bytes = 0;                   // algorithmic bytes (inputs + output, FP64)
+  int precision = 0;
+};
+
+// Builders of device steps from symbolic results.
+Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumulate, double alpha);
+Step make_copy_step(const std::vector<CopySeg>& segs, int x, int y, double alpha, double beta);
+
+// Execute a plan: inputs[i] are the FP64 device pointers of the bound inputs, out the FP64 output pointer.
+void run_plan(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s);
+// Same, recording an event before each step and after the last one (events.size() >= steps + 1).
+void run_plan_timed(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
+                    const std::vector<hipEvent_t>& events);
+
+// Workspace pool for FP32 input copies (per plan slot, grown on demand).
+const char* kind_name(int k);
+
+}  // namespace mpx

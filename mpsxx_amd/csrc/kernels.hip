@@ -1,0 +1,358 @@
+// HIP kernels for gfx950 (CDNA4, wave64).
+//
+// grouped GEMM: one 256-thread work-group per 64x64 C tile; 4 waves, each owning a 32x32 quarter computed with
+// 2x2 v_mfma_{f64,f32}_16x16x4 tiles.  A and B tiles (64x16, 16x64) are staged in LDS k-major so that the MFMA
+// operand of lane l is the LDS word [k + l/16][i + l%16].  The global->LDS mapping picks the unit-stride direction of
+// each operand so loads are coalesced whether the block is used as stored or transposed.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace mpx {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef dbl4 acc_t;
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  typedef flt4 acc_t;
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int BM = 64, BN = 64, BK = 16, LDP = 64 + 1;
+
+template <typename T, typename TC>
+__global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __restrict__ tiles,
+                                                           const GemmTask* __restrict__ tasks,
+                                                           const GemmPair* __restrict__ pairs,
+                                                           const T* __restrict__ A, const T* __restrict__ B,
+                                                           TC* __restrict__ C, double alpha, int accumulate) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T As[BK][LDP];
+  __shared__ T Bs[BK][LDP];
+
+  const GemmTile tl = tiles[blockIdx.x];
+  const GemmTask tk = tasks[tl.task];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  const int pend = tk.pair_begin + tk.pair_count;
+  for (int p = tk.pair_begin; p < pend; ++p) {
+    const GemmPair pr = pairs[p];
+    const T* Ap = A + pr.a_off;
+    const T* Bp = B + pr.b_off;
+    const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
+    for (int k0 = 0; k0 < pr.k; k0 += BK) {
+      if (pr.sa_k == 1) {
+        // contiguous along k: 16 consecutive threads read one row segment
+#pragma unroll
+        for (int e = tid; e < BM * BK; e += 256) {
+          const int kk = e & 15, i = e >> 4;
+          const int gi = tl.row0 + i, gk = k0 + kk;
+          As[kk][i] = (gi < tk.m && gk < pr.k) ? Ap[gi * sam + gk] : T(0);
+        }
+      } else {
+#pragma unroll
+        for (int e = tid; e < BM * BK; e += 256) {
+          const int i = e & 63, kk = e >> 6;
+          const int gi = tl.row0 + i, gk = k0 + kk;
+          As[kk][i] = (gi < tk.m && gk < pr.k) ? Ap[gi * sam + gk * sak] : T(0);
+        }
+      }
+      if (pr.sb_n == 1) {
+#pragma unroll
+        for (int e = tid; e < BK * BN; e += 256) {
+          const int j = e & 63, kk = e >> 6;
+          const int gj = tl.col0 + j, gk = k0 + kk;
+          Bs[kk][j] = (gj < tk.n && gk < pr.k) ? Bp[gk * sbk + gj] : T(0);
+        }
+      } else {
+#pragma unroll
+        for (int e = tid; e < BK * BN; e += 256) {
+          const int kk = e & 15, j = e >> 4;
+          const int gj = tl.col0 + j, gk = k0 + kk;
+          Bs[kk][j] = (gj < tk.n && gk < pr.k) ? Bp[gk * sbk + gj * sbn] : T(0);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const T a0 = As[kk + lk][wr * 32 + lr];
+        const T a1 = As[kk + lk][wr * 32 + 16 + lr];
+        const T b0 = Bs[kk + lk][wc * 32 + lr];
+        const T b1 = Bs[kk + lk][wc * 32 + 16 + lr];
+        acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+        acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+        acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+        acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+      }
+      __syncthreads();
+    }
+  }
+
+  // D layout of 16x16x4: lane l holds rows 4*(l/16)+r, column l%16.
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = tl.row0 + wr * 32 + mi * 16 + 4 * lk + r;
+        const int col = tl.col0 + wc * 32 + ni * 16 + lr;
+        if (row < tk.m && col < tk.n) {
+          const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+          const double v = alpha * (double)acc[mi][ni][r];
+          C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+        }
+      }
+}
+
+template <typename T, typename TC>
+__global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restrict__ chunks,
+                                                           const AxpyTask* __restrict__ tasks,
+                                                           const AxpyPair* __restrict__ pairs,
+                                                           const T* __restrict__ V, const T* __restrict__ S,
+                                                           TC* __restrict__ C, double alpha, int accumulate) {
+  const Chunk ch = chunks[blockIdx.x];
+  const AxpyTask tk = tasks[ch.task];
+  const int pend = tk.pair_begin + tk.pair_count;
+  for (int r = ch.start + threadIdx.x; r < tk.len && r < ch.start + kAxpyChunk; r += 256) {
+    T acc = T(0);
+    for (int p = tk.pair_begin; p < pend; ++p) {
+      const AxpyPair pr = pairs[p];
+      acc += S[pr.s_off] * V[pr.v_off + (int64_t)r * pr.v_stride];
+    }
+    const int64_t idx = tk.c_off + r;
+    const double v = alpha * (double)acc;
+    C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+  }
+}
+
+template <typename TX, typename TY>
+__global__ __launch_bounds__(256) void copy_kernel(const Chunk* __restrict__ chunks,
+                                                   const CopySeg* __restrict__ segs, const TX* __restrict__ x,
+                                                   TY* __restrict__ y, double alpha, double beta) {
+  const Chunk ch = chunks[blockIdx.x];
+  const CopySeg sg = segs[ch.task];
+  for (int r = ch.start + threadIdx.x; r < sg.len && r < ch.start + kAxpyChunk; r += 256) {
+    const int64_t d = sg.dst_off + r;
+    const double v = alpha * (double)x[sg.src_off + (int64_t)r * sg.stride];
+    y[d] = (beta == 0.0) ? (TY)v : (TY)(beta * (double)y[d] + v);
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ double block_sum(double v) {
+  __shared__ double red[4];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) t = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(256) void dot_partial_kernel(const Chunk* __restrict__ chunks,
+                                                          const CopySeg* __restrict__ segs,
+                                                          const double* __restrict__ x,
+                                                          const double* __restrict__ y, double* partial) {
+  const Chunk ch = chunks[blockIdx.x];
+  const CopySeg sg = segs[ch.task];
+  double acc = 0.0;
+  for (int r = ch.start + threadIdx.x; r < sg.len && r < ch.start + kAxpyChunk; r += 256)
+    acc += x[sg.dst_off + r] * y[sg.src_off + r];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void reduce_kernel(const double* __restrict__ partial, int n, double* out) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += partial[i];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) out[0] = acc;
+}
+
+__global__ void scale_kernel(double* x, int64_t n, double alpha) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= alpha;
+}
+
+__global__ void fill_kernel(double* x, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+__global__ void axpby_kernel(double a, const double* __restrict__ x, double b, double* __restrict__ y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+
+template <typename TX, typename TY>
+__global__ void convert_kernel(const TX* x, TY* y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (TY)x[i];
+}
+
+__global__ void combine_kernel(const double* __restrict__ base, int64_t ld, int m, const double* __restrict__ c,
+                               double* __restrict__ y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < m; ++j) acc += c[j] * base[j * ld + i];
+    y[i] = acc;
+  }
+}
+
+__global__ void precondition_kernel(double* r, const double* __restrict__ d, const unsigned char* __restrict__ pres,
+                                    double eval, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (pres[i]) {
+      double denm = eval - d[i];
+      if (fabs(denm) < 1.0e-8) denm = 1.0e-8;
+      r[i] /= denm;
+    } else {
+      r[i] *= 1.0 / eval;
+    }
+  }
+}
+
+// Gram block: grid (npart, mx*my); each work-group reduces a strided slice into partial[(pair)*npart + part]
+__global__ __launch_bounds__(256) void gram_partial_kernel(const double* __restrict__ X, int64_t ldx,
+                                                           const double* __restrict__ Y, int64_t ldy, int my,
+                                                           int64_t n, double* partial, int npart) {
+  const int pair = blockIdx.y;
+  const int i = pair / my, j = pair % my;
+  const double* x = X + i * ldx;
+  const double* y = Y + j * ldy;
+  double acc = 0.0;
+  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)npart * 256) acc += x[e] * y[e];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[(int64_t)pair * npart + blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restrict__ partial, int npart, double* G) {
+  const int pair = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < npart; i += 256) acc += partial[(int64_t)pair * npart + i];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) G[pair] = acc;
+}
+
+static inline int grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+template <typename T, typename TC>
+void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
+                 const T* B, TC* C, double alpha, int accumulate, hipStream_t s) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL((gemm_grouped_kernel<T, TC>), dim3(ntiles), dim3(256), 0, s, tiles, tasks, pairs, A, B, C, alpha,
+                     accumulate);
+}
+
+template <typename T, typename TC>
+void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, const T* V,
+                 const T* S, TC* C, double alpha, int accumulate, hipStream_t s) {
+  if (nchunks <= 0) return;
+  hipLaunchKernelGGL((axpy_grouped_kernel<T, TC>), dim3(nchunks), dim3(256), 0, s, chunks, tasks, pairs, V, S, C,
+                     alpha, accumulate);
+}
+
+template <typename TX, typename TY>
+void launch_copy(const Chunk* chunks, int nchunks, const CopySeg* segs, const TX* x, TY* y, double alpha,
+                 double beta, hipStream_t s) {
+  if (nchunks <= 0) return;
+  hipLaunchKernelGGL((copy_kernel<TX, TY>), dim3(nchunks), dim3(256), 0, s, chunks, segs, x, y, alpha, beta);
+}
+
+void launch_dot(const Chunk* chunks, int nchunks, const CopySeg* segs, const double* x, const double* y,
+                double* partial, double* out, hipStream_t s) {
+  if (nchunks > 0)
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(nchunks), dim3(256), 0, s, chunks, segs, x, y, partial);
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(256), 0, s, partial, nchunks, out);
+}
+
+void launch_scale(double* x, int64_t n, double alpha, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, n, alpha);
+}
+
+void launch_fill(double* x, int64_t n, double v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, n, v);
+}
+
+void launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, s, a, x, b, y, n);
+}
+
+template <typename TX, typename TY>
+void launch_convert(const TX* x, TY* y, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL((convert_kernel<TX, TY>), dim3(grid_for(n)), dim3(256), 0, s, x, y, n);
+}
+
+void launch_combine(const double* base, int64_t ld, int m, const double* c, double* y, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(combine_kernel, dim3(grid_for(n)), dim3(256), 0, s, base, ld, m, c, y, n);
+}
+
+void launch_precondition(double* r, const double* diag, const unsigned char* present, double eval, int64_t n,
+                         hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(precondition_kernel, dim3(grid_for(n)), dim3(256), 0, s, r, diag, present, eval, n);
+}
+
+void launch_gram(const double* X, int64_t ldx, int mx, const double* Y, int64_t ldy, int my, int64_t n, double* G,
+                 double* partial, int npart, hipStream_t s) {
+  if (mx <= 0 || my <= 0) return;
+  hipLaunchKernelGGL(gram_partial_kernel, dim3(npart, mx * my), dim3(256), 0, s, X, ldx, Y, ldy, my, n, partial,
+                     npart);
+  hipLaunchKernelGGL(gram_reduce_kernel, dim3(mx * my), dim3(256), 0, s, partial, npart, G);
+}
+
+template void launch_gemm<double, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const double*,
+                                          const double*, double*, double, int, hipStream_t);
+template void launch_gemm<float, float>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
+                                        const float*, float*, double, int, hipStream_t);
+template void launch_gemm<float, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
+                                         const float*, double*, double, int, hipStream_t);
+template void launch_axpy<double, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const double*,
+                                          const double*, double*, double, int, hipStream_t);
+template void launch_axpy<float, float>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const float*,
+                                        const float*, float*, double, int, hipStream_t);
+template void launch_axpy<float, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const float*,
+                                         const float*, double*, double, int, hipStream_t);
+template void launch_copy<double, double>(const Chunk*, int, const CopySeg*, const double*, double*, double, double,
+                                          hipStream_t);
+template void launch_copy<float, float>(const Chunk*, int, const CopySeg*, const float*, float*, double, double,
+                                        hipStream_t);
+template void launch_copy<double, float>(const Chunk*, int, const CopySeg*, const double*, float*, double, double,
+                                         hipStream_t);
+template void launch_copy<float, double>(const Chunk*, int, const CopySeg*, const float*, double*, double, double,
+                                         hipStream_t);
+template void launch_convert<double, float>(const double*, float*, int64_t, hipStream_t);
+template void launch_convert<float, double>(const float*, double*, int64_t, hipStream_t);
+
+}  // namespace mpx

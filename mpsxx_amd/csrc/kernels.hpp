@@ -1,0 +1,100 @@
+// Device task formats and kernel launchers (implemented in kernels.hip).
+//
+// All block-sparse work is expressed as flat task lists built on the host by the planner:
+//   * GemmStep : C_blk(m x n, row-major) (+)= alpha * sum_p A'_p (m x k_p) * B'_p (k_p x n)
+//                where A'_p(i,k) = A[a_off + i*sa_m + k*sa_k] and B'_p(k,j) = B[b_off + k*sb_k + j*sb_n]
+//                (strided views replace the reference's explicit Permute copies, QSTCONTRACT.h:30-47);
+//   * AxpyStep : C_blk[r] (+)= alpha * sum_p S[s_off_p] * V[v_off_p + r*v_stride_p]
+//                (the contractions with a 1x1x1x1 MPO block, whose GEMM has k = 1 and one unit side);
+//   * CopySegs : y[dst + r] = beta*y[dst + r] + alpha * x[src + r*stride]   (Tie gathers, up-casting copies, Axpy).
+#pragma once
+
+#include <cstdint>
+
+#include <hip/hip_runtime_api.h>
+
+namespace mpx {
+
+struct GemmPair {
+  int64_t a_off, b_off;
+  int32_t sa_m, sa_k;
+  int32_t sb_k, sb_n;
+  int32_t k;
+  int32_t pad;
+};
+
+struct GemmTask {
+  int64_t c_off;
+  int32_t m, n;
+  int32_t pair_begin, pair_count;
+};
+
+struct GemmTile {
+  int32_t task;
+  int32_t row0, col0;
+  int32_t pad;
+};
+
+struct AxpyPair {
+  int64_t v_off, s_off;
+  int32_t v_stride;
+  int32_t pad;
+};
+
+struct AxpyTask {
+  int64_t c_off;
+  int32_t len;
+  int32_t pair_begin, pair_count;
+  int32_t pad;
+};
+
+struct Chunk {
+  int32_t task;
+  int32_t start;
+};
+
+struct CopySeg {
+  int64_t dst_off, src_off;
+  int32_t len, stride;
+};
+
+constexpr int kGemmTile = 64;     // C tile edge of the grouped GEMM
+constexpr int kAxpyChunk = 1024;  // elements per work-group in the AXPY / copy kernels
+
+// Launchers.  T = storage/arithmetic type of A and B (double or float), TC = type of C.
+template <typename T, typename TC>
+void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
+                 const T* B, TC* C, double alpha, int accumulate, hipStream_t s);
+
+template <typename T, typename TC>
+void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, const T* V,
+                 const T* S, TC* C, double alpha, int accumulate, hipStream_t s);
+
+template <typename TX, typename TY>
+void launch_copy(const Chunk* chunks, int nchunks, const CopySeg* segs, const TX* x, TY* y, double alpha,
+                 double beta, hipStream_t s);
+
+// Dot products over matched segments (x_off = dst_off, y_off = src_off, stride ignored).  Writes per-work-group
+// partial sums into `partial` (size >= nchunks) and the total into out[0] with a deterministic second pass.
+void launch_dot(const Chunk* chunks, int nchunks, const CopySeg* segs, const double* x, const double* y,
+                double* partial, double* out, hipStream_t s);
+
+// Dense helpers on contiguous buffers.
+void launch_scale(double* x, int64_t n, double alpha, hipStream_t s);
+template <typename TX, typename TY>
+void launch_convert(const TX* x, TY* y, int64_t n, hipStream_t s);
+void launch_fill(double* x, int64_t n, double v, hipStream_t s);
+// y = a*x + b*y
+void launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s);
+
+// Davidson helpers on flat vectors of one layout (length n).
+// y = sum_i c[i] * X_i  with X_i = base + i*ld  (FP64; c lives in device memory)
+void launch_combine(const double* base, int64_t ld, int m, const double* c, double* y, int64_t n, hipStream_t s);
+// r[i] = r[i] / d(i) with the reference's clamp; d_present[i] == 0 marks elements of blocks absent from diag
+void launch_precondition(double* r, const double* diag, const unsigned char* present, double eval, int64_t n,
+                         hipStream_t s);
+// G[i*m + j] = <X_i, Y_j> for i < mx, j < my (FP64, deterministic tree reduction)
+void launch_gram(const double* X, int64_t ldx, int mx, const double* Y, int64_t ldy, int my, int64_t n, double* G,
+                 double* partial, int npart, hipStream_t s);
+
+}  // namespace mpx

@@ -1,0 +1,613 @@
+// Composite operations: each one is the reference's sequence of btas::Contract / btas::Gemm / btas::Tie / btas::Copy
+// calls (cited per function), planned once per input structure and replayed as grouped device launches.
+#include "ops.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <list>
+#include <map>
+#include <memory>
+
+#include "mpsxx_amd.h"
+
+namespace mpx {
+
+OpStats& last_stats() {
+  static thread_local OpStats st;
+  return st;
+}
+
+namespace {
+
+enum OpId {
+  OP_CONTRACT = 1, OP_SIGMA2, OP_SIGMA1, OP_SIGMAM, OP_SIGMA_OVL, OP_SIGMA_CONJ, OP_DIAG2, OP_DIAG1, OP_DIAGM,
+  OP_RENORM_F, OP_RENORM_B, OP_RENORM_OVL_F, OP_RENORM_OVL_B, OP_RENORM_M_F, OP_RENORM_M_B, OP_TIE, OP_UPCAST
+};
+
+enum OutMode { OUT_ACCUMULATE = 0, OUT_OVERWRITE = 1 };
+
+struct CachedPlan {
+  Plan plan;
+  int out_mode = OUT_ACCUMULATE;
+};
+
+// Plan cache keyed by (op, precision, structure hashes, extra ints); least-recently-used eviction.
+class PlanCache {
+ public:
+  CachedPlan* find(const std::vector<uint64_t>& key) {
+    auto it = map_.find(key);
+    if (it == map_.end()) return nullptr;
+    lru_.splice(lru_.begin(), lru_, it->second.second);
+    return it->second.first.get();
+  }
+  CachedPlan* insert(const std::vector<uint64_t>& key, std::unique_ptr<CachedPlan> p) {
+    if (map_.size() >= kMax) {
+      auto victim = lru_.back();
+      lru_.pop_back();
+      map_.erase(victim);
+    }
+    lru_.push_front(key);
+    CachedPlan* raw = p.get();
+    map_[key] = {std::move(p), lru_.begin()};
+    return raw;
+  }
+  void clear() { map_.clear(); lru_.clear(); }
+
+ private:
+  static constexpr size_t kMax = 256;
+  std::map<std::vector<uint64_t>, std::pair<std::unique_ptr<CachedPlan>, std::list<std::vector<uint64_t>>::iterator>>
+      map_;
+  std::list<std::vector<uint64_t>> lru_;
+};
+
+PlanCache& cache() {
+  static PlanCache c;
+  return c;
+}
+
+class PlanBuilder {
+ public:
+  explicit PlanBuilder(int prec) { P_.precision = prec; }
+
+  int input(const Layout& L, int which) {
+    const int id = (int)P_.slot_layout.size();
+    P_.slot_layout.push_back(L);
+    P_.slot_input.push_back(which);
+    P_.slot_prec.push_back(P_.precision == MPSXX_FP32 ? 1 : 0);
+    P_.slot_buf.emplace_back();
+    if (P_.precision == MPSXX_FP32) P_.slot_buf.back().alloc(std::max<int64_t>(L.nelem, 1) * sizeof(float));
+    P_.bytes += L.nelem * 8;
+    return id;
+  }
+
+  // intermediate = Contract(A[ca], B[cb])
+  int contract(int a, bool ca_conj, const std::vector<int>& ca, int b, bool cb_conj, const std::vector<int>& cb) {
+    View va{&P_.slot_layout[a], ca_conj}, vb{&P_.slot_layout[b], cb_conj};
+    ContractSym sym = contract_symbolic(va, ca, vb, cb, nullptr);
+    const int id = add_inter(sym.C, P_.precision == MPSXX_FP32 ? 1 : 0);
+    P_.steps.push_back(make_contract_step(sym, a, b, id, 0, 1.0));
+    P_.flops += sym.flops;
+    return id;
+  }
+
+  // output (+)= Contract(A[ca], B[cb]); out_existing may be null (empty output)
+  void contract_out(int a, bool ca_conj, const std::vector<int>& ca, int b, bool cb_conj, const std::vector<int>& cb,
+                    const Layout* out_existing, double alpha = 1.0) {
+    View va{&P_.slot_layout[a], ca_conj}, vb{&P_.slot_layout[b], cb_conj};
+    ContractSym sym = contract_symbolic(va, ca, vb, cb, out_existing);
+    const int id = add_output(sym.C);
+    P_.steps.push_back(make_contract_step(sym, a, b, id, 1, alpha));
+    P_.flops += sym.flops;
+    P_.bytes += sym.C.nelem * 8;
+  }
+
+  int tie(int x, const std::vector<int>& t) {
+    std::vector<CopySeg> segs;
+    Layout Y = tie_layout(P_.slot_layout[x], t, segs);
+    const int id = add_inter(Y, 0);
+    P_.steps.push_back(make_copy_step(segs, x, id, 1.0, 0.0));
+    return id;
+  }
+
+  // output := up-casting copy of slot x onto (q, qshape) of `target` (btas::Copy(x, y, 1): y's blocks are cleared
+  // first, then every x block allowed in y is copied; STBLAS.h:374-388, 48-60)
+  void upcast_out(int x, const Layout& target) {
+    const Layout& X = P_.slot_layout[x];
+    Layout Y;
+    Y.set_shape(target.rank, target.qsym, target.q, target.qs, X.ds);
+    if (X.rank != target.rank) throw Error(MPSXX_ERR_SHAPE, "Copy: rank mismatch");
+    for (int i = 0; i < X.rank; ++i)
+      if (X.ds[i].size() != target.ds[i].size())
+        throw Error(MPSXX_ERR_SHAPE, "Copy(SPARSE): x and y must have the same shape as long as up-casting is specified.");
+    for (int64_t b = 0; b < X.nblocks(); ++b)
+      if (Y.allowed(X.index(b))) Y.add_block(X.index(b));
+    Y.finalize();
+    std::vector<CopySeg> segs;
+    for (int64_t b = 0; b < X.nblocks(); ++b) {
+      const int64_t j = Y.find(X.tag[b]);
+      if (j < 0 || X.bsize[b] == 0) continue;
+      segs.push_back({Y.off[j], X.off[b], (int32_t)X.bsize[b], 1});
+    }
+    const int id = add_output(Y);
+    P_.steps.push_back(make_copy_step(segs, x, id, 1.0, 0.0));
+    P_.bytes += Y.nelem * 8;
+  }
+
+  Plan finish() { return std::move(P_); }
+
+ private:
+  int add_inter(const Layout& L, int prec) {
+    const int id = (int)P_.slot_layout.size();
+    P_.slot_layout.push_back(L);
+    P_.slot_input.push_back(-1);
+    P_.slot_prec.push_back(prec);
+    P_.slot_buf.emplace_back(std::max<int64_t>(L.nelem, 1) * (prec ? sizeof(float) : sizeof(double)));
+    return id;
+  }
+  int add_output(const Layout& L) {
+    const int id = (int)P_.slot_layout.size();
+    P_.slot_layout.push_back(L);
+    P_.slot_input.push_back(-2);
+    P_.slot_prec.push_back(0);
+    P_.slot_buf.emplace_back();
+    P_.out = L;
+    return id;
+  }
+  Plan P_;
+};
+
+void require(const Tensor& t, const char* what) {
+  if (t.empty) throw Error(MPSXX_ERR_ARG, std::string(what) + ": input array is empty");
+}
+
+std::vector<uint64_t> make_key(int op, int prec, std::initializer_list<const Tensor*> ins, const Tensor* out,
+                               std::initializer_list<int64_t> extra = {}) {
+  std::vector<uint64_t> k;
+  k.push_back((uint64_t)op);
+  k.push_back((uint64_t)prec);
+  for (const Tensor* t : ins) k.push_back(t->L.hash);
+  k.push_back(out && !out->empty ? out->L.hash : 0ull);
+  for (int64_t e : extra) k.push_back((uint64_t)e);
+  return k;
+}
+
+// Look up or build a plan, shape the output, run.
+template <typename BuildFn>
+void execute(const std::vector<uint64_t>& key, BuildFn build, int out_mode, std::vector<Tensor*> ins, Tensor& out,
+             hipStream_t s) {
+  CachedPlan* cp = cache().find(key);
+  if (!cp) {
+    auto np = std::make_unique<CachedPlan>();
+    np->plan = build();
+    np->out_mode = out_mode;
+    cp = cache().insert(key, std::move(np));
+  }
+  Plan& P = cp->plan;
+  if (out_mode == OUT_OVERWRITE || out.empty) out.set_layout(P.out, s);
+  else if (out.L.hash != P.out.hash) out.relayout(P.out, s);
+  std::vector<const double*> ptrs;
+  for (Tensor* t : ins) ptrs.push_back(t->ptr());
+  run_plan(P, ptrs, out.ptr(), s);
+  last_stats().flops = P.flops;
+  last_stats().bytes = P.bytes;
+}
+
+const Layout* existing(const Tensor& t) { return t.empty ? nullptr : &t.L; }
+
+}  // namespace
+
+void clear_plan_cache() { cache().clear(); }
+
+// ---------------------------------------------------------------------------------------------------------------------
+// generic machinery
+
+void op_contract(double alpha, Tensor& a, bool conj_a, const std::vector<int>& ca, Tensor& b, bool conj_b,
+                 const std::vector<int>& cb, double beta, Tensor& c, hipStream_t s) {
+  require(a, "Contract");
+  require(b, "Contract");
+  if (!c.empty && beta != 1.0) op_scal(beta, c, s);  // QSTBLAS.h:206 (Scal(beta, c))
+  auto key = make_key(OP_CONTRACT, 0, {&a, &b}, &c, {conj_a, conj_b, (int64_t)ca.size()});
+  for (int x : ca) key.push_back(x);
+  for (int x : cb) key.push_back(x + 100);
+  uint64_t ab;
+  std::memcpy(&ab, &alpha, 8);
+  key.push_back(ab);
+  const Layout* ex = existing(c);
+  execute(key, [&] {
+    PlanBuilder B(MPSXX_FP64);
+    const int sa = B.input(a.L, 0), sb = B.input(b.L, 1);
+    B.contract_out(sa, conj_a, ca, sb, conj_b, cb, ex, alpha);
+    return B.finish();
+  }, OUT_ACCUMULATE, {&a, &b}, c, s);
+}
+
+void op_gemm(int ta, int tb, double alpha, Tensor& a, bool conj_a, Tensor& b, bool conj_b, double beta, Tensor& c,
+             hipStream_t s) {
+  require(a, "Gemm");
+  require(b, "Gemm");
+  int NC;
+  if (!c.empty) NC = c.L.rank;
+  else throw Error(MPSXX_ERR_ARG, "Gemm: the output rank must be known (create the output with its structure)");
+  std::vector<int> ca, cb;
+  gemm_indices(ta, tb, a.L.rank, b.L.rank, NC, ca, cb);
+  op_contract(alpha, a, conj_a, ca, b, conj_b, cb, beta, c, s);
+}
+
+void op_tie(Tensor& x, const std::vector<int>& t, Tensor& y, hipStream_t s) {
+  require(x, "Tie");
+  auto key = make_key(OP_TIE, 0, {&x}, nullptr, {(int64_t)t.size()});
+  for (int v : t) key.push_back(v);
+  execute(key, [&] {
+    PlanBuilder B(MPSXX_FP64);
+    const int sx = B.input(x.L, 0);
+    std::vector<CopySeg> segs;
+    Layout Y = tie_layout(x.L, t, segs);
+    (void)sx;
+    // a one-step plan whose output is the tie result
+    Plan P = B.finish();
+    P.slot_layout.push_back(Y);
+    P.slot_input.push_back(-2);
+    P.slot_prec.push_back(0);
+    P.slot_buf.emplace_back();
+    P.out = Y;
+    P.steps.push_back(make_copy_step(segs, 0, 1, 1.0, 0.0));
+    return P;
+  }, OUT_OVERWRITE, {&x}, y, s);
+}
+
+void op_copy_upcast(Tensor& x, Tensor& y, hipStream_t s) {
+  require(x, "Copy");
+  if (y.empty) throw Error(MPSXX_ERR_ARG, "Copy(UpCast): target must carry its quantum numbers");
+  auto key = make_key(OP_UPCAST, 0, {&x}, &y);
+  Layout target = y.L;
+  execute(key, [&] {
+    PlanBuilder B(MPSXX_FP64);
+    const int sx = B.input(x.L, 0);
+    B.upcast_out(sx, target);
+    return B.finish();
+  }, OUT_OVERWRITE, {&x}, y, s);
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// BLAS-1 (QSTBLAS.h:40-93, STBLAS.h:94-145)
+
+namespace {
+
+struct SegList {
+  std::vector<CopySeg> segs;
+  std::vector<Chunk> chunks;
+};
+
+SegList match_segments(const Layout& x, const Layout& y) {
+  SegList r;
+  for (int64_t b = 0; b < x.nblocks(); ++b) {
+    const int64_t j = y.find(x.tag[b]);
+    if (j < 0 || x.bsize[b] == 0) continue;
+    r.segs.push_back({y.off[j], x.off[b], (int32_t)x.bsize[b], 1});
+  }
+  for (size_t i = 0; i < r.segs.size(); ++i)
+    for (int s = 0; s < r.segs[i].len; s += kAxpyChunk) r.chunks.push_back({(int32_t)i, s});
+  return r;
+}
+
+struct Scratch {
+  DevBuf partial, result;
+  size_t npart = 0;
+  double* get_partial(size_t n) {
+    if (n > npart) {
+      npart = std::max<size_t>(n, 4096);
+      partial.alloc(npart * sizeof(double));
+    }
+    if (!result.p) result.alloc(64 * sizeof(double));
+    return partial.as<double>();
+  }
+};
+
+Scratch& scratch() {
+  static Scratch s;
+  return s;
+}
+
+void check_same_q(const Tensor& x, const Tensor& y, const char* what) {
+  if (x.L.qsym && y.L.qsym) {
+    if (x.L.q != y.L.q) throw Error(MPSXX_ERR_QN, std::string(what) + "(QSPARSE): quantum numbers of x and y must be the same.");
+    if (!x.L.same_qshape(y.L)) throw Error(MPSXX_ERR_QN, std::string(what) + "(QSPARSE): quantum shapes of x and y must be the same.");
+  }
+}
+
+}  // namespace
+
+double op_dotc(Tensor& x, Tensor& y, hipStream_t s) {
+  if (x.empty || y.empty) return 0.0;
+  check_same_q(x, y, "Dotc");
+  SegList m = match_segments(y.L, x.L);  // dst = x offset, src = y offset
+  double* part = scratch().get_partial(m.chunks.size() + 1);
+  double* d_out = scratch().result.as<double>();
+  DevBuf dsegs = upload_vec(m.segs), dch = upload_vec(m.chunks);
+  launch_dot(dch.as<Chunk>(), (int)m.chunks.size(), dsegs.as<CopySeg>(), x.ptr(), y.ptr(), part, d_out, s);
+  double h = 0.0;
+  hip_check(hipMemcpyAsync(&h, d_out, sizeof(double), hipMemcpyDeviceToHost, s), "dot result");
+  hip_check(hipStreamSynchronize(s), "dot sync");
+  return h;
+}
+
+double op_nrm2(Tensor& x, hipStream_t s) { return std::sqrt(op_dotc(x, x, s)); }
+
+void op_axpy(double alpha, Tensor& x, Tensor& y, hipStream_t s) {
+  if (x.empty) return;
+  if (y.empty) {
+    Layout l = x.L;
+    l.tag.clear();
+    l.bidx.clear();
+    l.finalize();
+    y.set_layout(l, s);
+  } else {
+    check_same_q(x, y, "Axpy");
+  }
+  // y.reserve(each x block) (STBLAS.h:135-145)
+  bool missing = false;
+  for (int64_t b = 0; b < x.L.nblocks() && !missing; ++b) missing = y.L.find(x.L.tag[b]) < 0;
+  if (missing) {
+    Layout u = y.L;
+    for (int64_t b = 0; b < x.L.nblocks(); ++b) u.add_block(x.L.index(b));
+    u.finalize();
+    y.relayout(u, s);
+  }
+  SegList m = match_segments(x.L, y.L);
+  DevBuf dsegs = upload_vec(m.segs), dch = upload_vec(m.chunks);
+  launch_copy<double, double>(dch.as<Chunk>(), (int)m.chunks.size(), dsegs.as<CopySeg>(), x.ptr(), y.ptr(), alpha,
+                              1.0, s);
+  hip_check(hipStreamSynchronize(s), "axpy sync");
+}
+
+void op_scal(double alpha, Tensor& x, hipStream_t s) {
+  if (x.empty) return;
+  launch_scale(x.ptr(), x.L.nelem, alpha, s);
+}
+
+void op_copy(Tensor& x, Tensor& y, hipStream_t s) {
+  if (x.empty) {
+    y.empty = true;
+    y.data.release();
+    y.L = Layout();
+    return;
+  }
+  y.set_layout(x.L, s);
+  hip_check(hipMemcpyAsync(y.ptr(), x.ptr(), x.L.nelem * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// sigma vectors (src/compute_sigma_vector.hpp)
+
+Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
+                         const Layout& wfn, const Layout* out, int prec) {
+  std::vector<uint64_t> key{OP_SIGMA2, (uint64_t)prec, lmpo.hash, rmpo.hash, lopr.hash, ropr.hash, wfn.hash,
+                            out ? out->hash : 0ull};
+  CachedPlan* cp = cache().find(key);
+  if (cp) return cp->plan;
+  PlanBuilder B(prec);
+  const int s_lmpo = B.input(lmpo, 0), s_rmpo = B.input(rmpo, 1), s_lopr = B.input(lopr, 2);
+  const int s_ropr = B.input(ropr, 3), s_wfn = B.input(wfn, 4);
+  // compute_sigma_vector.hpp:63-69
+  const int s1 = B.contract(s_lopr, false, {2}, s_wfn, false, {0});
+  const int s2 = B.contract(s1, false, {1, 2}, s_lmpo, false, {0, 2});
+  const int s3 = B.contract(s2, false, {4, 1}, s_rmpo, false, {0, 2});
+  B.contract_out(s3, false, {4, 1}, s_ropr, false, {1, 2}, out);
+  auto np = std::make_unique<CachedPlan>();
+  np->plan = B.finish();
+  return cache().insert(key, std::move(np))->plan;
+}
+
+void op_sigma_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec,
+                      hipStream_t s) {
+  for (Tensor* t : {&lmpo, &rmpo, &lopr, &ropr, &wfn}) require(*t, "compute_sigma_vector");
+  Plan& P = plan_sigma_twosite(lmpo.L, rmpo.L, lopr.L, ropr.L, wfn.L, existing(sgv), prec);
+  if (sgv.empty) sgv.set_layout(P.out, s);
+  else if (sgv.L.hash != P.out.hash) sgv.relayout(P.out, s);
+  run_plan(P, {lmpo.ptr(), rmpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s);
+  last_stats().flops = P.flops;
+  last_stats().bytes = P.bytes;
+}
+
+void op_sigma_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s) {
+  for (Tensor* t : {&mpo, &lopr, &ropr, &wfn}) require(*t, "compute_sigma_vector");
+  const Layout* ex = existing(sgv);
+  execute(make_key(OP_SIGMA1, prec, {&mpo, &lopr, &ropr, &wfn}, &sgv), [&] {
+    PlanBuilder B(prec);
+    const int sm = B.input(mpo.L, 0), sl = B.input(lopr.L, 1), sr = B.input(ropr.L, 2), sw = B.input(wfn.L, 3);
+    // compute_sigma_vector.hpp:17-21: Gemm(N,N,lopr,wfn0) -> Contract(mpo) -> Contract(ropr)
+    const int s1 = B.contract(sl, false, {2}, sw, false, {0});
+    const int s2 = B.contract(s1, false, {1, 2}, sm, false, {0, 2});
+    B.contract_out(s2, false, {3, 1}, sr, false, {1, 2}, ex);
+    return B.finish();
+  }, OUT_ACCUMULATE, {&mpo, &lopr, &ropr, &wfn}, sgv, s);
+}
+
+void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s) {
+  for (Tensor* t : {&lopr, &ropr, &wfn}) require(*t, "compute_sigma_vector");
+  if (lopr.L.rank != 3 || ropr.L.rank != 3 || wfn.L.rank != 2)
+    throw Error(MPSXX_ERR_ARG, "merged sigma: expects 3-index operators and a 2-index wavefunction");
+  const Layout* ex = existing(sgv);
+  execute(make_key(OP_SIGMAM, prec, {&lopr, &ropr, &wfn}, &sgv), [&] {
+    PlanBuilder B(prec);
+    const int sl = B.input(lopr.L, 0), sr = B.input(ropr.L, 1), sw = B.input(wfn.L, 2);
+    // compute_sigma_vector.hpp:94-95: Gemm(N,N,lopr,wfn0) (K=1), Gemm(N,T,scr1,ropr) (K=2)
+    const int s1 = B.contract(sl, false, {2}, sw, false, {0});
+    B.contract_out(s1, false, {1, 2}, sr, false, {1, 2}, ex);
+    return B.finish();
+  }, OUT_ACCUMULATE, {&lopr, &ropr, &wfn}, sgv, s);
+}
+
+void op_sigma_overlap(Tensor& lsov, Tensor& rsov, Tensor& wfn, Tensor& sgv, hipStream_t s) {
+  for (Tensor* t : {&lsov, &rsov, &wfn}) require(*t, "compute_sigma_vector");
+  if (lsov.L.rank != 2 || rsov.L.rank != 2) throw Error(MPSXX_ERR_ARG, "overlap sigma: expects 2-index overlaps");
+  const Layout* ex = existing(sgv);
+  const int N = wfn.L.rank;
+  execute(make_key(OP_SIGMA_OVL, 0, {&lsov, &rsov, &wfn}, &sgv), [&] {
+    PlanBuilder B(MPSXX_FP64);
+    const int sl = B.input(lsov.L, 0), sr = B.input(rsov.L, 1), sw = B.input(wfn.L, 2);
+    // compute_sigma_vector.hpp:32-33, 80-81, 106-107: Gemm(N,N,lopr,wfn0), Gemm(N,T,scr1,ropr)
+    const int s1 = B.contract(sl, false, {1}, sw, false, {0});
+    B.contract_out(s1, false, {N - 1}, sr, false, {1}, ex);
+    return B.finish();
+  }, OUT_ACCUMULATE, {&lsov, &rsov, &wfn}, sgv, s);
+}
+
+void op_sigma_conj(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, hipStream_t s) {
+  for (Tensor* t : {&mpo, &lopr, &ropr, &wfn}) require(*t, "compute_sigma_vector_conj");
+  // compute_sigma_vector.hpp:45-50.  The output is the conjugated view of sgv0: its quantum numbers are checked
+  // negated, and an empty sgv0 receives the data with the conjugate's quantum numbers flipped back.
+  Layout exneg;
+  const Layout* ex = nullptr;
+  if (!sgv.empty) {
+    exneg = sgv.L;
+    exneg.q = -exneg.q;
+    for (auto& v : exneg.qs)
+      for (QN& x : v) x = -x;
+    exneg.finalize();
+    ex = &exneg;
+  }
+  auto key = make_key(OP_SIGMA_CONJ, 0, {&mpo, &lopr, &ropr, &wfn}, &sgv);
+  CachedPlan* cp = cache().find(key);
+  if (!cp) {
+    PlanBuilder B(MPSXX_FP64);
+    const int sm = B.input(mpo.L, 0), sl = B.input(lopr.L, 1), sr = B.input(ropr.L, 2), sw = B.input(wfn.L, 3);
+    // Gemm(Trans, ConjTrans, lopr, wfn0): A contracts its first index, B (conjugated) its last
+    const int s1 = B.contract(sl, false, {0}, sw, true, {2});
+    const int s2 = B.contract(s1, false, {0, 2}, sm, false, {0, 1});
+    B.contract_out(s2, false, {1, 3}, sr, false, {0, 1}, ex);
+    auto np = std::make_unique<CachedPlan>();
+    np->plan = B.finish();
+    cp = cache().insert(key, std::move(np));
+  }
+  Plan& P = cp->plan;
+  Layout outL = P.out;
+  outL.q = -outL.q;
+  for (auto& v : outL.qs)
+    for (QN& x : v) x = -x;
+  outL.finalize();
+  if (sgv.empty) sgv.set_layout(outL, s);
+  else if (sgv.L.hash != outL.hash) sgv.relayout(outL, s);
+  run_plan(P, {mpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s);
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// diagonal (src/compute_diagonal_elements.hpp)
+
+void op_diag_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Tensor& diag, hipStream_t s) {
+  for (Tensor* t : {&lmpo, &rmpo, &lopr, &ropr}) require(*t, "compute_diagonal_elements");
+  if (diag.empty) throw Error(MPSXX_ERR_ARG, "compute_diagonal_elements: diag must carry the wavefunction's q/qshape");
+  Layout target = diag.L;
+  execute(make_key(OP_DIAG2, 0, {&lmpo, &rmpo, &lopr, &ropr}, &diag), [&] {
+    PlanBuilder B(MPSXX_FP64);
+    const int a = B.input(lmpo.L, 0), b = B.input(rmpo.L, 1), c = B.input(lopr.L, 2), d = B.input(ropr.L, 3);
+    // compute_diagonal_elements.hpp:47-60
+    const int lm = B.tie(a, {1, 2}), rm = B.tie(b, {1, 2}), lo = B.tie(c, {0, 2}), ro = B.tie(d, {0, 2});
+    const int s1 = B.contract(lo, false, {1}, lm, false, {0});
+    const int s2 = B.contract(s1, false, {2}, rm, false, {0});
+    const int s3 = B.contract(s2, false, {3}, ro, false, {1});
+    B.upcast_out(s3, target);
+    return B.finish();
+  }, OUT_OVERWRITE, {&lmpo, &rmpo, &lopr, &ropr}, diag, s);
+}
+
+void op_diag_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& diag, hipStream_t s) {
+  for (Tensor* t : {&mpo, &lopr, &ropr}) require(*t, "compute_diagonal_elements");
+  if (diag.empty) throw Error(MPSXX_ERR_ARG, "compute_diagonal_elements: diag must carry the wavefunction's q/qshape");
+  Layout target = diag.L;
+  execute(make_key(OP_DIAG1, 0, {&mpo, &lopr, &ropr}, &diag), [&] {
+    PlanBuilder B(MPSXX_FP64);
+    const int a = B.input(mpo.L, 0), c = B.input(lopr.L, 1), d = B.input(ropr.L, 2);
+    // compute_diagonal_elements.hpp:21-31
+    const int m = B.tie(a, {1, 2}), lo = B.tie(c, {0, 2}), ro = B.tie(d, {0, 2});
+    const int s1 = B.contract(lo, false, {1}, m, false, {0});
+    const int s2 = B.contract(s1, false, {2}, ro, false, {1});
+    B.upcast_out(s2, target);
+    return B.finish();
+  }, OUT_OVERWRITE, {&mpo, &lopr, &ropr}, diag, s);
+}
+
+void op_diag_merged(Tensor& lopr, Tensor& ropr, Tensor& diag, hipStream_t s) {
+  for (Tensor* t : {&lopr, &ropr}) require(*t, "compute_diagonal_elements");
+  if (diag.empty) throw Error(MPSXX_ERR_ARG, "compute_diagonal_elements: diag must carry the wavefunction's q/qshape");
+  Layout target = diag.L;
+  execute(make_key(OP_DIAGM, 0, {&lopr, &ropr}, &diag), [&] {
+    PlanBuilder B(MPSXX_FP64);
+    const int c = B.input(lopr.L, 0), d = B.input(ropr.L, 1);
+    // compute_diagonal_elements.hpp:72-79: Gemm(N,T,lopr_diag,ropr_diag)
+    const int lo = B.tie(c, {0, 2}), ro = B.tie(d, {0, 2});
+    const int s1 = B.contract(lo, false, {1}, ro, false, {1});
+    B.upcast_out(s1, target);
+    return B.finish();
+  }, OUT_OVERWRITE, {&lopr, &ropr}, diag, s);
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// renormalization (src/renormalize.hpp)
+
+void op_renormalize(bool fwd, Tensor& mpo, Tensor& opr0, Tensor& bra, Tensor& ket, Tensor& opr1, int prec,
+                    hipStream_t s) {
+  for (Tensor* t : {&mpo, &opr0, &bra, &ket}) require(*t, "renormalize");
+  const Layout* ex = existing(opr1);
+  execute(make_key(fwd ? OP_RENORM_F : OP_RENORM_B, prec, {&mpo, &opr0, &bra, &ket}, &opr1), [&] {
+    PlanBuilder B(prec);
+    const int sm = B.input(mpo.L, 0), so = B.input(opr0.L, 1), sb = B.input(bra.L, 2), sk = B.input(ket.L, 3);
+    if (fwd) {  // renormalize.hpp:20-23
+      const int s1 = B.contract(so, false, {0}, sb, true, {0});
+      const int s2 = B.contract(s1, false, {0, 2}, sm, false, {0, 1});
+      B.contract_out(s2, false, {0, 2}, sk, false, {0, 1}, ex);
+    } else {    // renormalize.hpp:27-30
+      const int s1 = B.contract(sb, true, {2}, so, false, {0});
+      const int s2 = B.contract(s1, false, {1, 2}, sm, false, {1, 3});
+      B.contract_out(s2, false, {3, 1}, sk, false, {1, 2}, ex);
+    }
+    return B.finish();
+  }, OUT_ACCUMULATE, {&mpo, &opr0, &bra, &ket}, opr1, s);
+}
+
+void op_renormalize_overlap(bool fwd, Tensor& sov0, Tensor& bra, Tensor& ket, Tensor& sov1, hipStream_t s) {
+  for (Tensor* t : {&sov0, &bra, &ket}) require(*t, "renormalize");
+  const Layout* ex = existing(sov1);
+  const int N = bra.L.rank;
+  if (N != 2 && N != 3) throw Error(MPSXX_ERR_ARG, "renormalize(overlap): bra must have 2 or 3 indices");
+  execute(make_key(fwd ? OP_RENORM_OVL_F : OP_RENORM_OVL_B, 0, {&sov0, &bra, &ket}, &sov1), [&] {
+    PlanBuilder B(MPSXX_FP64);
+    const int so = B.input(sov0.L, 0), sb = B.input(bra.L, 1), sk = B.input(ket.L, 2);
+    if (fwd) {
+      // renormalize.hpp:45-46 / 87-88: Gemm(T,N,sov0,bra*), Gemm(T,N,scr1,ket0)
+      const int s1 = B.contract(so, false, {0}, sb, true, {0});
+      std::vector<int> c1, c2;
+      for (int k = 0; k < N - 1; ++k) { c1.push_back(k); c2.push_back(k); }
+      B.contract_out(s1, false, c1, sk, false, c2, ex);
+    } else {
+      // renormalize.hpp:50-51 / 92-93: Gemm(N,N,bra*,sov0), Gemm(N,T,scr1,ket0)
+      const int s1 = B.contract(sb, true, {N - 1}, so, false, {0});
+      std::vector<int> c1, c2;
+      for (int k = 1; k < N; ++k) { c1.push_back(k); c2.push_back(k); }
+      B.contract_out(s1, false, c1, sk, false, c2, ex);
+    }
+    return B.finish();
+  }, OUT_ACCUMULATE, {&sov0, &bra, &ket}, sov1, s);
+}
+
+void op_renormalize_merged(bool fwd, Tensor& opr0, Tensor& bra, Tensor& ket, Tensor& opr1, int prec,
+                           hipStream_t s) {
+  for (Tensor* t : {&opr0, &bra, &ket}) require(*t, "renormalize");
+  if (bra.L.rank != 2 || ket.L.rank != 2 || opr0.L.rank != 3)
+    throw Error(MPSXX_ERR_ARG, "renormalize(merged): expects a 3-index operator and 2-index bra/ket");
+  const Layout* ex = existing(opr1);
+  execute(make_key(fwd ? OP_RENORM_M_F : OP_RENORM_M_B, prec, {&opr0, &bra, &ket}, &opr1), [&] {
+    PlanBuilder B(prec);
+    const int so = B.input(opr0.L, 0), sb = B.input(bra.L, 1), sk = B.input(ket.L, 2);
+    if (fwd) {  // renormalize.hpp:66-67: Gemm(T,N,bra*,opr0), Gemm(N,N,scr1,ket0)
+      const int s1 = B.contract(sb, true, {0}, so, false, {0});
+      B.contract_out(s1, false, {2}, sk, false, {0}, ex);
+    } else {    // renormalize.hpp:71-72: Gemm(N,N,bra*,opr0), Gemm(N,T,scr1,ket0)
+      const int s1 = B.contract(sb, true, {1}, so, false, {0});
+      B.contract_out(s1, false, {2}, sk, false, {1}, ex);
+    }
+    return B.finish();
+  }, OUT_ACCUMULATE, {&opr0, &bra, &ket}, opr1, s);
+}
+
+}  // namespace mpx

@@ -1,0 +1,67 @@
+// Composite hot-path operations on device tensors (C++ layer under the C ABI).
+#pragma once
+
+#include <vector>
+
+#include "engine.hpp"
+
+namespace mpx {
+
+struct OpStats {
+  int64_t flops = 0;   // algorithmic flops of the last plan executed (reference contraction order)
+  int64_t bytes = 0;   // algorithmic bytes (inputs + output, FP64 storage)
+};
+
+// Last plan statistics (per thread).
+OpStats& last_stats();
+
+// Generic BTAS machinery
+void op_contract(double alpha, Tensor& a, bool conj_a, const std::vector<int>& ca, Tensor& b, bool conj_b,
+                 const std::vector<int>& cb, double beta, Tensor& c, hipStream_t s);
+void op_gemm(int ta, int tb, double alpha, Tensor& a, bool conj_a, Tensor& b, bool conj_b, double beta, Tensor& c,
+             hipStream_t s);
+void op_tie(Tensor& x, const std::vector<int>& tie, Tensor& y, hipStream_t s);
+void op_copy_upcast(Tensor& x, Tensor& y, hipStream_t s);
+
+// BLAS-1
+double op_dotc(Tensor& x, Tensor& y, hipStream_t s);
+double op_nrm2(Tensor& x, hipStream_t s);
+void op_axpy(double alpha, Tensor& x, Tensor& y, hipStream_t s);
+void op_scal(double alpha, Tensor& x, hipStream_t s);
+void op_copy(Tensor& x, Tensor& y, hipStream_t s);
+
+// Hot path (reference: src/compute_sigma_vector.hpp, compute_diagonal_elements.hpp, renormalize.hpp)
+void op_sigma_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec,
+                      hipStream_t s);
+void op_sigma_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s);
+void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s);
+void op_sigma_overlap(Tensor& lsov, Tensor& rsov, Tensor& wfn, Tensor& sgv, hipStream_t s);
+void op_sigma_conj(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, hipStream_t s);
+void op_diag_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Tensor& diag, hipStream_t s);
+void op_diag_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& diag, hipStream_t s);
+void op_diag_merged(Tensor& lopr, Tensor& ropr, Tensor& diag, hipStream_t s);
+void op_renormalize(bool fwd, Tensor& mpo, Tensor& opr0, Tensor& bra, Tensor& ket, Tensor& opr1, int prec,
+                    hipStream_t s);
+void op_renormalize_overlap(bool fwd, Tensor& sov0, Tensor& bra, Tensor& ket, Tensor& sov1, hipStream_t s);
+void op_renormalize_merged(bool fwd, Tensor& opr0, Tensor& bra, Tensor& ket, Tensor& opr1, int prec,
+                           hipStream_t s);
+
+// Plan for a two-site sigma vector (exposed for the Davidson driver, which binds device pointers itself).
+Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
+                         const Layout& wfn, const Layout* out, int prec);
+
+void clear_plan_cache();
+
+// Davidson (src/diagonalize.hpp:310-449) over operator groups of the two-site form
+struct DavGroup {
+  Tensor *lmpo, *rmpo, *lopr, *ropr;
+};
+struct DavidsonStats {
+  int nsigma = 0;
+  double sigma_ms = 0.0;  // device time spent in sigma vectors
+};
+Layout davidson_layout(const std::vector<DavGroup>& groups, const Layout& psi);
+double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tensor& wfnc, double tol, int prec,
+                        DavidsonStats& st, hipStream_t s);
+
+}  // namespace mpx

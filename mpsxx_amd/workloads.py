@@ -1,0 +1,82 @@
+"""Synthetic hot-path workloads (inputs already shaped like a converged DMRG state).
+
+``hubbard_twosite(L, M, U)``: the two-site sigma-vector problem at the middle bond of the open Hubbard chain at half
+filling and 2Sz = 0, with bond dimension M distributed over the quantum-number sectors of the bond like the reduced
+density matrix spectrum of a converged chain (Gaussian in particle-number and spin deviation from the mean, width
+sigma = 1.5), and pattern-preserving random entries N(0,1)/sqrt(m) in every quantum-number-allowed block.
+"""
+import math
+
+import numpy as np
+
+from . import mps
+from .mpo import hubbard_mpo
+from .qst import QST
+
+
+def gaussian_dims(qns, center, M, width=1.5, min_weight=0.02):
+    """Distribute M states over the sectors `qns` with weight exp(-(dp^2 + ds^2) / (2 width^2))."""
+    w = []
+    for (p, s) in qns:
+        dp = p - center[0]
+        ds = s - center[1]
+        w.append(math.exp(-(dp * dp + ds * ds) / (2.0 * width * width)))
+    w = np.array(w)
+    keep = w >= min_weight * w.max()
+    w = np.where(keep, w, 0.0)
+    raw = w / w.sum() * M
+    dims = np.floor(raw).astype(int)
+    dims = np.where(keep & (dims == 0), 1, dims)
+    # hand the remainder to the largest fractional parts
+    rem = M - int(dims.sum())
+    order = np.argsort(-(raw - np.floor(raw)))
+    k = 0
+    while rem > 0:
+        j = order[k % len(order)]
+        if keep[j]:
+            dims[j] += 1
+            rem -= 1
+        k += 1
+    while rem < 0:
+        j = int(np.argmax(dims))
+        dims[j] -= 1
+        rem += 1
+    return [int(d) for d in dims]
+
+
+def _prune(qs, ds):
+    q2 = [q for q, d in zip(qs, ds) if d > 0]
+    d2 = [d for d in ds if d > 0]
+    return q2, d2
+
+
+def hubbard_twosite(L=32, M=512, U=4.0, t=1.0, seed=0, width=1.5):
+    """(lmpo, rmpo, lenv, renv, psi) QSTs of the middle two-site problem; see the module docstring."""
+    rng = np.random.default_rng(seed)
+    qt = (L, 0)
+    i = L // 2 - 1  # two-site centre (i, i+1)
+    mpos = hubbard_mpo(L, t, U)
+    lq = mps.left_bond_qns(L, i, qt)
+    rq = mps.right_bond_qns(L, i + 2, qt)
+    nl = i             # sites on the left
+    nr = L - (i + 2)   # sites on the right
+    ld = gaussian_dims(lq, (nl * qt[0] / L, 0), M, width)
+    rd = gaussian_dims(rq, (-nr * qt[0] / L, 0), M, width)
+    lq, ld = _prune(lq, ld)
+    rq, rd = _prune(rq, rd)
+
+    def fill(a):
+        import itertools
+        for idx in itertools.product(*[range(a.nsec(k)) for k in range(a.rank)]):
+            if a.allowed(idx):
+                shp = a.block_shape(idx)
+                m = max(1, max(shp))
+                a.blocks[tuple(idx)] = rng.standard_normal(shp) / math.sqrt(m)
+        return a
+
+    qw = mpos[i].qshape[0]
+    lenv = fill(QST((0, 0), [lq, [(-a, -b) for a, b in qw], [(-a, -b) for a, b in lq]], [ld, [1] * len(qw), ld]))
+    qwr = [(-a, -b) for a, b in mpos[i + 1].qshape[3]]
+    renv = fill(QST((0, 0), [[(-a, -b) for a, b in rq], qwr, rq], [rd, [1] * len(qwr), rd]))
+    psi = fill(mps.twosite_wavefunction(lq, ld, rq, rd, qt))
+    return mpos[i], mpos[i + 1], lenv, renv, psi

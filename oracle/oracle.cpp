@@ -1,0 +1,764 @@
+// ORACLE -- test infrastructure only (CPU, FP64).  Imported by tests/, __graft_entry__.smoke() and bench.py's CPU
+// baseline leg as the checker; never part of the product path.
+//
+// A clean-room CPU restatement of the reference's block-sparse algorithm (naokin/mpsxx), following the reference
+// step by step so that block patterns and FP64 values can be compared with the device path:
+//   * QSTArray model: blocks keyed by row-major tag, allowed iff sum of sector QNs == total
+//     (include/btas/QSPARSE/QSTArray.h:34-36; include/btas/SPARSE/STArray.h:338-348), conjugate() negates QNs
+//     (QSTArray.h:293-300);
+//   * Contract: get_contract_jobs (include/btas/common/btas_contract_shape.h:164-204) -> explicit Permute copies of
+//     A and B (QSTREINDEX.h:22-50, reindex.h:20-42) -> block-pair GEMM over (row, col) with at least one matching
+//     contracted block, C block zero-initialised then accumulated pair by pair (STBLAS.h:298-359,
+//     T_arguments.h:517-577), C's qshape/q from gemm_contract_qshape (btas_contract_qshape.h:60-108);
+//   * Tie (STREINDEX.h:95-147), Copy with up-cast (STBLAS.h:48-60, 374-388), Dotc/Axpy/Scal/Nrm2 (QSTBLAS.h:40-93,
+//     STBLAS.h:94-145);
+//   * the hot-path compositions of src/compute_sigma_vector.hpp, src/compute_diagonal_elements.hpp and
+//     src/renormalize.hpp, with the same index lists;
+//   * Davidson (src/diagonalize.hpp:283-449) with a Jacobi eigen-solver standing in for LAPACKE_dsyev.
+// Dense kernels are a plain cache-friendly row-major dgemm (the reference calls cblas_dgemm, blas/gemm_impl.h:63).
+// OpenMP over output blocks mirrors parallel_call (T_arguments.h:584-599).
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mpsxx_amd.h"  // descriptor struct only
+
+namespace orc {
+
+struct QN {
+  int p = 0, s = 0;
+  QN() = default;
+  QN(int a, int b) : p(a), s(b) {}
+  QN operator+(const QN& o) const { return QN(p + o.p, s + o.s); }
+  QN operator-() const { return QN(-p, -s); }
+  bool operator==(const QN& o) const { return p == o.p && s == o.s; }
+  bool operator!=(const QN& o) const { return !(*this == o); }
+};
+
+typedef std::vector<double> Block;
+
+struct QST {
+  int rank = 0;
+  bool qsym = true;
+  QN q;
+  std::vector<std::vector<QN>> qs;
+  std::vector<std::vector<int>> ds;
+  std::map<long, std::shared_ptr<Block>> blocks;
+  bool empty() const { return rank == 0; }
+
+  long nsec(int i) const { return (long)ds[i].size(); }
+  long tag(const std::vector<int>& idx) const {
+    long t = 0;
+    for (int i = 0; i < rank; ++i) t = t * nsec(i) + idx[i];
+    return t;
+  }
+  std::vector<int> index(long t) const {
+    std::vector<int> idx(rank);
+    for (int i = rank - 1; i >= 0; --i) {
+      idx[i] = (int)(t % nsec(i));
+      t /= nsec(i);
+    }
+    return idx;
+  }
+  bool allowed(const std::vector<int>& idx) const {
+    if (!qsym) return true;
+    QN s;
+    for (int i = 0; i < rank; ++i) s = s + qs[i][idx[i]];
+    return s == q;
+  }
+  long bsize(const std::vector<int>& idx) const {
+    long n = 1;
+    for (int i = 0; i < rank; ++i) n *= ds[i][idx[i]];
+    return n;
+  }
+  // STArray::reserve: existing block, or a new zero block when allowed, else nullptr (STArray.h:448-471)
+  Block* reserve(long t) {
+    auto it = blocks.find(t);
+    std::vector<int> idx = index(t);
+    if (!allowed(idx)) return nullptr;
+    if (it != blocks.end()) return it->second.get();
+    auto b = std::make_shared<Block>(bsize(idx), 0.0);
+    blocks[t] = b;
+    return b.get();
+  }
+  void set_shape(int r, bool qs_, QN q_, std::vector<std::vector<QN>> qv, std::vector<std::vector<int>> dv) {
+    rank = r;
+    qsym = qs_;
+    q = q_;
+    qs = qv;
+    ds = dv;
+    if (!qsym) {
+      qs.assign(rank, {});
+      for (int i = 0; i < rank; ++i) qs[i].assign(ds[i].size(), QN());
+    }
+    blocks.clear();
+  }
+};
+
+QST conjugate(const QST& a) {
+  QST c = a;  // shares block pointers
+  c.q = -a.q;
+  for (auto& v : c.qs)
+    for (QN& x : v) x = -x;
+  return c;
+}
+
+// dense row-major reindex: out index i takes input index perm[i]
+QST permute(const QST& a, const std::vector<int>& perm) {
+  QST c;
+  std::vector<std::vector<QN>> qs(a.rank);
+  std::vector<std::vector<int>> ds(a.rank);
+  for (int i = 0; i < a.rank; ++i) {
+    qs[i] = a.qs[perm[i]];
+    ds[i] = a.ds[perm[i]];
+  }
+  c.set_shape(a.rank, a.qsym, a.q, qs, ds);
+  for (const auto& kv : a.blocks) {
+    std::vector<int> ia = a.index(kv.first);
+    std::vector<int> ic(a.rank);
+    for (int i = 0; i < a.rank; ++i) ic[i] = ia[perm[i]];
+    std::vector<long> da(a.rank), sa(a.rank), dc(a.rank);
+    long s = 1;
+    for (int i = a.rank - 1; i >= 0; --i) {
+      da[i] = a.ds[i][ia[i]];
+      sa[i] = s;
+      s *= da[i];
+    }
+    for (int i = 0; i < a.rank; ++i) dc[i] = da[perm[i]];
+    auto out = std::make_shared<Block>(s, 0.0);
+    std::vector<long> cnt(a.rank, 0);
+    for (long e = 0; e < s; ++e) {
+      long src = 0;
+      for (int i = 0; i < a.rank; ++i) src += cnt[i] * sa[perm[i]];
+      (*out)[e] = (*kv.second)[src];
+      for (int i = a.rank - 1; i >= 0; --i) {
+        if (++cnt[i] < dc[i]) break;
+        cnt[i] = 0;
+      }
+    }
+    c.blocks[c.tag(ic)] = out;
+  }
+  return c;
+}
+
+// C(m x n) += alpha * A(m x k) * B(k x n), row-major
+void dgemm(long m, long n, long k, double alpha, const double* A, const double* B, double* C) {
+  for (long i = 0; i < m; ++i) {
+    double* c = C + i * n;
+    for (long p = 0; p < k; ++p) {
+      const double a = alpha * A[i * k + p];
+      const double* b = B + p * n;
+      for (long j = 0; j < n; ++j) c[j] += a * b[j];
+    }
+  }
+}
+
+struct Task {
+  Block* c;
+  long m, n;
+  std::vector<std::pair<const Block*, const Block*>> ab;
+  std::vector<long> ks;
+  double flops;
+};
+
+int g_threads = 1;
+
+// Contract (QSTCONTRACT.h:16-57); only 0 < K < NA, NB (every call on the hot path)
+void contract(double alpha, const QST& a, const std::vector<int>& ca, const QST& b, const std::vector<int>& cb,
+              double beta, QST& c) {
+  const int NA = a.rank, NB = b.rank, K = (int)ca.size();
+  if (K <= 0 || K >= NA || K >= NB || (int)cb.size() != K) throw std::runtime_error("oracle contract: unsupported K");
+  // get_contract_jobs: A -> (uncontracted..., contracted...), B -> (contracted..., uncontracted...)
+  std::vector<int> pa, pb;
+  for (int i = 0; i < NA; ++i)
+    if (std::find(ca.begin(), ca.end(), i) == ca.end()) pa.push_back(i);
+  for (int x : ca) pa.push_back(x);
+  for (int x : cb) pb.push_back(x);
+  for (int i = 0; i < NB; ++i)
+    if (std::find(cb.begin(), cb.end(), i) == cb.end()) pb.push_back(i);
+  bool ida = true, idb = true;
+  for (int i = 0; i < NA; ++i) ida &= (pa[i] == i);
+  for (int i = 0; i < NB; ++i) idb &= (pb[i] == i);
+  const QST ap = ida ? a : permute(a, pa);
+  const QST bp = idb ? b : permute(b, pb);
+  // gemm_contract_qshape (NoTrans, NoTrans)
+  const bool qsym = a.qsym && b.qsym;
+  for (int k = 0; k < K; ++k) {
+    const auto& qa = ap.qs[NA - K + k];
+    const auto& qb = bp.qs[k];
+    if (qa.size() != qb.size()) throw std::runtime_error("oracle: contracted sector counts differ");
+    for (size_t s = 0; s < qa.size(); ++s)
+      if (qsym && qa[s] != -qb[s]) throw std::runtime_error("btas::gemm_contract_qshape contraction of quantum numbers failed");
+  }
+  std::vector<std::vector<QN>> qs;
+  std::vector<std::vector<int>> ds;
+  for (int i = 0; i < NA - K; ++i) { qs.push_back(ap.qs[i]); ds.push_back(ap.ds[i]); }
+  for (int i = K; i < NB; ++i) { qs.push_back(bp.qs[i]); ds.push_back(bp.ds[i]); }
+  const QN qc = ap.q + bp.q;
+  if (!c.empty()) {
+    if (qsym && c.q != qc) throw std::runtime_error("Gemm(QSPARSE): quantum number of c must equal to a.q() + b.q().");
+    if (qsym && c.qs != qs) throw std::runtime_error("Gemm(QSPARSE): c must have the same quantum shape of [ a * b ].");
+    for (auto& kv : c.blocks) {  // Scal(beta, c)
+      auto nb = std::make_shared<Block>(*kv.second);
+      for (double& x : *nb) x *= beta;
+      kv.second = nb;
+    }
+  } else {
+    c.set_shape(NA + NB - 2 * K, qsym, qc, qs, ds);
+  }
+  // rows/cols/K-tags of the permuted operands
+  long colsA = 1, rowsA = 1, colsB = 1;
+  for (int i = 0; i < NA - K; ++i) rowsA *= ap.nsec(i);
+  for (int i = NA - K; i < NA; ++i) colsA *= ap.nsec(i);
+  for (int i = K; i < NB; ++i) colsB *= bp.nsec(i);
+  std::map<long, std::vector<std::pair<long, const Block*>>> arow, bcol;
+  for (const auto& kv : ap.blocks) arow[kv.first / colsA].push_back({kv.first % colsA, kv.second.get()});
+  for (const auto& kv : bp.blocks) bcol[kv.first % colsB].push_back({kv.first / colsB, kv.second.get()});
+  std::vector<Task> tasks;
+  for (const auto& ra : arow) {
+    const long i = ra.first;
+    for (const auto& cbk : bcol) {
+      const long j = cbk.first;
+      const long ij = i * colsB + j;
+      std::vector<int> cidx = c.index(ij);
+      if (!c.allowed(cidx)) continue;  // STBLAS.h:331
+      Task t;
+      for (const auto& x : ra.second)
+        for (const auto& y : cbk.second)
+          if (x.first == y.first) {
+            t.ab.push_back({x.second, y.second});
+            // K extent from A's contracted sectors
+            std::vector<int> ia = ap.index(i * colsA + x.first);
+            long kk = 1;
+            for (int q = NA - K; q < NA; ++q) kk *= ap.ds[q][ia[q]];
+            t.ks.push_back(kk);
+          }
+      if (t.ab.empty()) continue;
+      long m = 1, n = 1;
+      for (int q = 0; q < NA - K; ++q) m *= c.ds[q][cidx[q]];
+      for (int q = NA - K; q < c.rank; ++q) n *= c.ds[q][cidx[q]];
+      t.m = m;
+      t.n = n;
+      t.c = c.reserve(ij);
+      if (!t.c) continue;
+      // copy-on-write: C blocks may share storage with a conjugated reference
+      t.flops = 0;
+      for (long kk : t.ks) t.flops += (double)m * n * kk;
+      tasks.push_back(std::move(t));
+    }
+  }
+  std::sort(tasks.begin(), tasks.end(), [](const Task& x, const Task& y) { return x.flops > y.flops; });
+#pragma omp parallel for schedule(guided) num_threads(g_threads)
+  for (long t = 0; t < (long)tasks.size(); ++t) {
+    Task& T = tasks[t];
+    for (size_t p = 0; p < T.ab.size(); ++p)
+      dgemm(T.m, T.n, T.ks[p], alpha, T.ab[p].first->data(), T.ab[p].second->data(), T.c->data());
+  }
+}
+
+// btas::Gemm index lists
+void gemm(int ta, int tb, double alpha, const QST& a, const QST& b, double beta, QST& c, int NC) {
+  const int K = (a.rank + b.rank - NC) / 2;
+  std::vector<int> ca, cb;
+  for (int k = 0; k < K; ++k) ca.push_back(ta ? k : a.rank - K + k);
+  for (int k = 0; k < K; ++k) cb.push_back(tb ? b.rank - K + k : k);
+  contract(alpha, a, ca, b, cb, beta, c);
+}
+
+QST tie(const QST& x, const std::vector<int>& t) {
+  const int N = x.rank, K = (int)t.size();
+  std::vector<int> uniq;
+  for (int i = 0; i < N; ++i) {
+    const bool tied = std::find(t.begin(), t.end(), i) != t.end();
+    if (!tied || i == t[0]) uniq.push_back(i);
+  }
+  QST y;
+  std::vector<std::vector<int>> ds;
+  for (int i : uniq) ds.push_back(x.ds[i]);
+  y.set_shape((int)uniq.size(), false, QN(), {}, ds);
+  for (const auto& kv : x.blocks) {
+    std::vector<int> ix = x.index(kv.first);
+    bool ok = true;
+    for (int k = 1; k < K; ++k) ok &= ix[t[k]] == ix[t[0]];
+    if (!ok) continue;
+    std::vector<int> iy;
+    for (int i : uniq) iy.push_back(ix[i]);
+    std::vector<long> dx(N), sx(N);
+    long s = 1;
+    for (int i = N - 1; i >= 0; --i) { dx[i] = x.ds[i][ix[i]]; sx[i] = s; s *= dx[i]; }
+    std::vector<long> dy, sy;
+    for (int i : uniq) {
+      dy.push_back(dx[i]);
+      long st = sx[i];
+      if (i == t[0]) { st = 0; for (int k = 0; k < K; ++k) st += sx[t[k]]; }
+      sy.push_back(st);
+    }
+    long ny = 1;
+    for (long d : dy) ny *= d;
+    auto out = std::make_shared<Block>(ny, 0.0);
+    std::vector<long> cnt(dy.size(), 0);
+    for (long e = 0; e < ny; ++e) {
+      long src = 0;
+      for (size_t i = 0; i < dy.size(); ++i) src += cnt[i] * sy[i];
+      (*out)[e] = (*kv.second)[src];
+      for (int i = (int)dy.size() - 1; i >= 0; --i) {
+        if (++cnt[i] < dy[i]) break;
+        cnt[i] = 0;
+      }
+    }
+    y.blocks[y.tag(iy)] = out;
+  }
+  return y;
+}
+
+// Copy(x, y, UpCast=1): y keeps q/qshape, its blocks are cleared, allowed x blocks are copied
+void copy_upcast(const QST& x, QST& y) {
+  y.ds = x.ds;
+  y.blocks.clear();
+  for (const auto& kv : x.blocks) {
+    Block* b = y.reserve(kv.first);
+    if (!b) continue;
+    *b = *kv.second;
+  }
+}
+
+double dotc(const QST& x, const QST& y) {
+  double v = 0.0;
+  for (const auto& kv : x.blocks) {
+    auto it = y.blocks.find(kv.first);
+    if (it == y.blocks.end()) continue;
+    const Block& a = *kv.second;
+    const Block& b = *it->second;
+    for (size_t i = 0; i < a.size(); ++i) v += a[i] * b[i];
+  }
+  return v;
+}
+
+void scal(double alpha, QST& x) {
+  for (auto& kv : x.blocks) {
+    auto nb = std::make_shared<Block>(*kv.second);
+    for (double& v : *nb) v *= alpha;
+    kv.second = nb;
+  }
+}
+
+void axpy(double alpha, const QST& x, QST& y) {
+  if (y.empty()) { y.set_shape(x.rank, x.qsym, x.q, x.qs, x.ds); }
+  for (const auto& kv : x.blocks) {
+    Block* b = y.reserve(kv.first);
+    if (!b) throw std::runtime_error("ST_Axpy_serial: reservation failed");
+    // copy-on-write when shared
+    auto it = y.blocks.find(kv.first);
+    if (it->second.use_count() > 1) { it->second = std::make_shared<Block>(*it->second); b = it->second.get(); }
+    for (size_t i = 0; i < b->size(); ++i) (*b)[i] += alpha * (*kv.second)[i];
+  }
+}
+
+QST deep_copy(const QST& x) {
+  QST y = x;
+  for (auto& kv : y.blocks) kv.second = std::make_shared<Block>(*kv.second);
+  return y;
+}
+
+// ---- hot path compositions ---------------------------------------------------------------------------------------
+
+void sigma_twosite(const QST& lmpo, const QST& rmpo, const QST& lopr, const QST& ropr, const QST& wfn, QST& sgv) {
+  QST s1, s2, s3;  // compute_sigma_vector.hpp:63-69
+  contract(1.0, lopr, {2}, wfn, {0}, 1.0, s1);
+  contract(1.0, s1, {1, 2}, lmpo, {0, 2}, 1.0, s2);
+  contract(1.0, s2, {4, 1}, rmpo, {0, 2}, 1.0, s3);
+  contract(1.0, s3, {4, 1}, ropr, {1, 2}, 1.0, sgv);
+}
+
+void sigma_onesite(const QST& mpo, const QST& lopr, const QST& ropr, const QST& wfn, QST& sgv) {
+  QST s1, s2;  // compute_sigma_vector.hpp:17-21
+  gemm(0, 0, 1.0, lopr, wfn, 1.0, s1, 4);
+  contract(1.0, s1, {1, 2}, mpo, {0, 2}, 1.0, s2);
+  contract(1.0, s2, {3, 1}, ropr, {1, 2}, 1.0, sgv);
+}
+
+void sigma_merged(const QST& lopr, const QST& ropr, const QST& wfn, QST& sgv) {
+  QST s1;  // compute_sigma_vector.hpp:94-95
+  gemm(0, 0, 1.0, lopr, wfn, 1.0, s1, 3);
+  gemm(0, 1, 1.0, s1, ropr, 1.0, sgv, 2);
+}
+
+void sigma_overlap(const QST& lsov, const QST& rsov, const QST& wfn, QST& sgv) {
+  QST s1;  // compute_sigma_vector.hpp:32-33, 80-81, 106-107
+  gemm(0, 0, 1.0, lsov, wfn, 1.0, s1, wfn.rank);
+  gemm(0, 1, 1.0, s1, rsov, 1.0, sgv, wfn.rank);
+}
+
+void diag_twosite(const QST& lmpo, const QST& rmpo, const QST& lopr, const QST& ropr, QST& diag) {
+  QST lm = tie(lmpo, {1, 2}), rm = tie(rmpo, {1, 2}), lo = tie(lopr, {0, 2}), ro = tie(ropr, {0, 2});
+  QST s1, s2, s3;  // compute_diagonal_elements.hpp:47-60
+  contract(1.0, lo, {1}, lm, {0}, 1.0, s1);
+  contract(1.0, s1, {2}, rm, {0}, 1.0, s2);
+  contract(1.0, s2, {3}, ro, {1}, 1.0, s3);
+  copy_upcast(s3, diag);
+}
+
+void diag_onesite(const QST& mpo, const QST& lopr, const QST& ropr, QST& diag) {
+  QST m = tie(mpo, {1, 2}), lo = tie(lopr, {0, 2}), ro = tie(ropr, {0, 2});
+  QST s1, s2;  // compute_diagonal_elements.hpp:21-31
+  contract(1.0, lo, {1}, m, {0}, 1.0, s1);
+  contract(1.0, s1, {2}, ro, {1}, 1.0, s2);
+  copy_upcast(s2, diag);
+}
+
+void diag_merged(const QST& lopr, const QST& ropr, QST& diag) {
+  QST lo = tie(lopr, {0, 2}), ro = tie(ropr, {0, 2});
+  QST s1;  // compute_diagonal_elements.hpp:72-79
+  gemm(0, 1, 1.0, lo, ro, 1.0, s1, 2);
+  copy_upcast(s1, diag);
+}
+
+void renormalize(bool fwd, const QST& mpo, const QST& opr0, const QST& bra, const QST& ket, QST& opr1) {
+  QST s1, s2;  // renormalize.hpp:18-31
+  if (fwd) {
+    contract(1.0, opr0, {0}, conjugate(bra), {0}, 1.0, s1);
+    contract(1.0, s1, {0, 2}, mpo, {0, 1}, 1.0, s2);
+    contract(1.0, s2, {0, 2}, ket, {0, 1}, 1.0, opr1);
+  } else {
+    contract(1.0, conjugate(bra), {2}, opr0, {0}, 1.0, s1);
+    contract(1.0, s1, {1, 2}, mpo, {1, 3}, 1.0, s2);
+    contract(1.0, s2, {3, 1}, ket, {1, 2}, 1.0, opr1);
+  }
+}
+
+void renormalize_overlap(bool fwd, const QST& sov0, const QST& bra, const QST& ket, QST& sov1) {
+  QST s1;  // renormalize.hpp:43-52, 85-94
+  if (fwd) {
+    gemm(1, 0, 1.0, sov0, conjugate(bra), 1.0, s1, bra.rank);
+    gemm(1, 0, 1.0, s1, ket, 1.0, sov1, 2);
+  } else {
+    gemm(0, 0, 1.0, conjugate(bra), sov0, 1.0, s1, bra.rank);
+    gemm(0, 1, 1.0, s1, ket, 1.0, sov1, 2);
+  }
+}
+
+void renormalize_merged(bool fwd, const QST& opr0, const QST& bra, const QST& ket, QST& opr1) {
+  QST s1;  // renormalize.hpp:64-73
+  if (fwd) {
+    gemm(1, 0, 1.0, conjugate(bra), opr0, 1.0, s1, 3);
+    gemm(0, 0, 1.0, s1, ket, 1.0, opr1, 3);
+  } else {
+    gemm(0, 0, 1.0, conjugate(bra), opr0, 1.0, s1, 3);
+    gemm(0, 1, 1.0, s1, ket, 1.0, opr1, 3);
+  }
+}
+
+// ---- Davidson (src/diagonalize.hpp:283-449), one root, group loop inside multH ----------------------------------
+
+// Jacobi eigen-decomposition of a small symmetric matrix (row-major n x n); eigenvalues ascending, vectors in columns
+// (the convention of LAPACKE_dsyev with row-major storage, TLAPACK.h:23-47)
+void jacobi_eig(int n, std::vector<double> a, std::vector<double>& w, std::vector<double>& v) {
+  v.assign(n * n, 0.0);
+  for (int i = 0; i < n; ++i) v[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) off += a[i * n + j] * a[i * n + j];
+    if (off < 1e-30) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double theta = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = c * akp - s * akq;
+          a[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = c * apk - s * aqk;
+          a[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = v[k * n + p], vkq = v[k * n + q];
+          v[k * n + p] = c * vkp - s * vkq;
+          v[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) { return a[x * n + x] < a[y * n + y]; });
+  w.resize(n);
+  std::vector<double> v2(n * n);
+  for (int j = 0; j < n; ++j) {
+    w[j] = a[ord[j] * n + ord[j]];
+    for (int i = 0; i < n; ++i) v2[i * n + j] = v[i * n + ord[j]];
+  }
+  v.swap(v2);
+}
+
+void normalize(QST& x) { scal(1.0 / std::sqrt(dotc(x, x)), x); }
+
+void precondition(double eval, const QST& diag, QST& errv) {
+  for (auto& kv : errv.blocks) {
+    auto nb = std::make_shared<Block>(*kv.second);
+    auto id = diag.blocks.find(kv.first);
+    if (id != diag.blocks.end()) {
+      for (size_t i = 0; i < nb->size(); ++i) {
+        double denm = eval - (*id->second)[i];
+        if (std::fabs(denm) < 1.0e-8) denm = 1.0e-8;
+        (*nb)[i] /= denm;
+      }
+    } else {
+      for (double& v : *nb) v *= 1.0 / eval;
+    }
+    kv.second = nb;
+  }
+}
+
+struct Functor2 {
+  std::vector<const QST*> lmpo, rmpo, lopr, ropr;
+  int nsigma = 0;
+  void operator()(const QST& c, QST& v) {
+    for (size_t g = 0; g < lmpo.size(); ++g) sigma_twosite(*lmpo[g], *rmpo[g], *lopr[g], *ropr[g], c, v);
+    ++nsigma;
+  }
+};
+
+double davidson(Functor2& multH, const QST& diag, QST& wfnc, double tolerance) {
+  const int max_ritz = 10;
+  double eval = 0.0;
+  std::vector<QST> trial(max_ritz), sigma(max_ritz);
+  trial[0] = deep_copy(wfnc);
+  int niter = 0, iconv = 0;
+  while (iconv < 1 && niter < 4) {
+    normalize(trial[0]);
+    sigma[0] = QST();
+    multH(trial[0], sigma[0]);
+    for (int m = 1; m <= max_ritz; ++m) {
+      std::vector<double> heff(m * m), ovlp(m * m);
+      for (int i = 0; i < m; ++i) {
+        heff[i * m + i] = dotc(trial[i], sigma[i]);
+        ovlp[i * m + i] = dotc(trial[i], trial[i]);
+        for (int j = 0; j < i; ++j) {
+          const double hij = dotc(trial[i], sigma[j]);
+          heff[i * m + j] = heff[j * m + i] = hij;
+          const double sij = dotc(trial[i], trial[j]);
+          ovlp[i * m + j] = ovlp[j * m + i] = sij;
+        }
+      }
+      std::vector<double> rval, rvec;
+      jacobi_eig(m, heff, rval, rvec);
+      eval = rval[0];
+      std::vector<QST> ts(m), ss(m);
+      for (int i = 0; i < m; ++i) {
+        ts[i] = deep_copy(trial[i]);
+        ss[i] = deep_copy(sigma[i]);
+        scal(rvec[i * m + i], trial[i]);
+        scal(rvec[i * m + i], sigma[i]);
+      }
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j)
+          if (i != j) {
+            axpy(rvec[i * m + j], ts[i], trial[j]);
+            axpy(rvec[i * m + j], ss[i], sigma[j]);
+          }
+      QST errv = deep_copy(sigma[0]);
+      axpy(-eval, trial[0], errv);
+      const double rnorm = dotc(errv, errv);
+      if (rnorm < tolerance) { ++iconv; break; }
+      if (m < max_ritz) {
+        precondition(eval, diag, errv);
+        for (int i = 0; i < m; ++i) {
+          normalize(errv);
+          axpy(-dotc(trial[i], errv), trial[i], errv);
+        }
+        normalize(errv);
+        trial[m] = deep_copy(errv);
+        sigma[m] = QST();
+        multH(trial[m], sigma[m]);
+      }
+    }
+    ++niter;
+  }
+  wfnc = deep_copy(trial[0]);
+  normalize(wfnc);  // Functor::orthonormalize with no lower roots
+  return eval;
+}
+
+}  // namespace orc
+
+// ---- C interface (ctypes) -------------------------------------------------------------------------------------------
+
+struct orc_tensor {
+  orc::QST a;
+};
+
+static thread_local std::string g_err;
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+void orc_set_threads(int n) { orc::g_threads = n > 0 ? n : 1; }
+
+orc_tensor* orc_create(const mpsxx_qst_desc* d, const double* data) {
+  auto* t = new orc_tensor;
+  std::vector<std::vector<orc::QN>> qs(d->rank);
+  std::vector<std::vector<int>> ds(d->rank);
+  long k = 0;
+  for (int i = 0; i < d->rank; ++i)
+    for (int j = 0; j < d->nsec[i]; ++j, ++k) {
+      ds[i].push_back(d->sec_dim[k]);
+      if (d->qsym) qs[i].push_back(orc::QN(d->sec_q[2 * k], d->sec_q[2 * k + 1]));
+    }
+  t->a.set_shape(d->rank, d->qsym != 0, orc::QN(d->q_total[0], d->q_total[1]), qs, ds);
+  std::vector<long> tags;
+  for (long b = 0; b < d->nblocks; ++b) {
+    std::vector<int> idx(d->block_idx + b * d->rank, d->block_idx + (b + 1) * d->rank);
+    tags.push_back(t->a.tag(idx));
+  }
+  std::sort(tags.begin(), tags.end());
+  tags.erase(std::unique(tags.begin(), tags.end()), tags.end());
+  long off = 0;
+  for (long tg : tags) {
+    std::vector<int> idx = t->a.index(tg);
+    const long n = t->a.bsize(idx);
+    auto blk = std::make_shared<orc::Block>(data ? std::vector<double>(data + off, data + off + n)
+                                                 : std::vector<double>(n, 0.0));
+    t->a.blocks[tg] = blk;
+    off += n;
+  }
+  return t;
+}
+
+orc_tensor* orc_empty(void) { return new orc_tensor; }
+
+void orc_free(orc_tensor* t) { delete t; }
+
+void orc_info(orc_tensor* t, int32_t* rank, int32_t* qsym, int32_t* q, int64_t* nsec_total, int64_t* nblocks,
+              int64_t* nelem) {
+  const orc::QST& a = t->a;
+  *rank = a.rank;
+  *qsym = a.qsym;
+  q[0] = a.q.p;
+  q[1] = a.q.s;
+  long ns = 0;
+  for (int i = 0; i < a.rank; ++i) ns += a.nsec(i);
+  *nsec_total = ns;
+  *nblocks = (int64_t)a.blocks.size();
+  long ne = 0;
+  for (const auto& kv : a.blocks) ne += (long)kv.second->size();
+  *nelem = ne;
+}
+
+void orc_layout(orc_tensor* t, int32_t* nsec, int32_t* sec_q, int32_t* sec_dim, int32_t* block_idx, double* data) {
+  const orc::QST& a = t->a;
+  long k = 0;
+  for (int i = 0; i < a.rank; ++i) {
+    nsec[i] = (int32_t)a.nsec(i);
+    for (int j = 0; j < a.nsec(i); ++j, ++k) {
+      sec_dim[k] = a.ds[i][j];
+      sec_q[2 * k] = a.qs[i][j].p;
+      sec_q[2 * k + 1] = a.qs[i][j].s;
+    }
+  }
+  long b = 0, off = 0;
+  for (const auto& kv : a.blocks) {
+    std::vector<int> idx = a.index(kv.first);
+    for (int i = 0; i < a.rank; ++i) block_idx[b * a.rank + i] = idx[i];
+    std::memcpy(data + off, kv.second->data(), kv.second->size() * sizeof(double));
+    off += (long)kv.second->size();
+    ++b;
+  }
+}
+
+#define ORC_GUARD(...)                \
+  try {                               \
+    __VA_ARGS__;                      \
+    return 0;                         \
+  } catch (const std::exception& e) { \
+    g_err = e.what();                 \
+    return 1;                         \
+  }
+
+int orc_sigma_twosite(orc_tensor* lmpo, orc_tensor* rmpo, orc_tensor* lopr, orc_tensor* ropr, orc_tensor* wfn,
+                      orc_tensor* sgv) {
+  ORC_GUARD(orc::sigma_twosite(lmpo->a, rmpo->a, lopr->a, ropr->a, wfn->a, sgv->a))
+}
+int orc_sigma_onesite(orc_tensor* mpo, orc_tensor* lopr, orc_tensor* ropr, orc_tensor* wfn, orc_tensor* sgv) {
+  ORC_GUARD(orc::sigma_onesite(mpo->a, lopr->a, ropr->a, wfn->a, sgv->a))
+}
+int orc_sigma_merged(orc_tensor* lopr, orc_tensor* ropr, orc_tensor* wfn, orc_tensor* sgv) {
+  ORC_GUARD(orc::sigma_merged(lopr->a, ropr->a, wfn->a, sgv->a))
+}
+int orc_sigma_overlap(orc_tensor* lsov, orc_tensor* rsov, orc_tensor* wfn, orc_tensor* sgv) {
+  ORC_GUARD(orc::sigma_overlap(lsov->a, rsov->a, wfn->a, sgv->a))
+}
+int orc_diag_twosite(orc_tensor* lmpo, orc_tensor* rmpo, orc_tensor* lopr, orc_tensor* ropr, orc_tensor* diag) {
+  ORC_GUARD(orc::diag_twosite(lmpo->a, rmpo->a, lopr->a, ropr->a, diag->a))
+}
+int orc_diag_onesite(orc_tensor* mpo, orc_tensor* lopr, orc_tensor* ropr, orc_tensor* diag) {
+  ORC_GUARD(orc::diag_onesite(mpo->a, lopr->a, ropr->a, diag->a))
+}
+int orc_diag_merged(orc_tensor* lopr, orc_tensor* ropr, orc_tensor* diag) {
+  ORC_GUARD(orc::diag_merged(lopr->a, ropr->a, diag->a))
+}
+int orc_renormalize(int fwd, orc_tensor* mpo, orc_tensor* opr0, orc_tensor* bra, orc_tensor* ket, orc_tensor* opr1) {
+  ORC_GUARD(orc::renormalize(fwd != 0, mpo->a, opr0->a, bra->a, ket->a, opr1->a))
+}
+int orc_renormalize_overlap(int fwd, orc_tensor* sov0, orc_tensor* bra, orc_tensor* ket, orc_tensor* sov1) {
+  ORC_GUARD(orc::renormalize_overlap(fwd != 0, sov0->a, bra->a, ket->a, sov1->a))
+}
+int orc_renormalize_merged(int fwd, orc_tensor* opr0, orc_tensor* bra, orc_tensor* ket, orc_tensor* opr1) {
+  ORC_GUARD(orc::renormalize_merged(fwd != 0, opr0->a, bra->a, ket->a, opr1->a))
+}
+int orc_contract(double alpha, orc_tensor* a, int conj_a, const int32_t* ca, orc_tensor* b, int conj_b,
+                 const int32_t* cb, int32_t K, double beta, orc_tensor* c) {
+  ORC_GUARD({
+    std::vector<int> va(ca, ca + K), vb(cb, cb + K);
+    orc::contract(alpha, conj_a ? orc::conjugate(a->a) : a->a, va, conj_b ? orc::conjugate(b->a) : b->a, vb, beta,
+                  c->a);
+  })
+}
+int orc_tie(orc_tensor* x, const int32_t* t, int32_t K, orc_tensor* y) {
+  ORC_GUARD(y->a = orc::tie(x->a, std::vector<int>(t, t + K)))
+}
+int orc_copy_upcast(orc_tensor* x, orc_tensor* y) { ORC_GUARD(orc::copy_upcast(x->a, y->a)) }
+double orc_dotc(orc_tensor* x, orc_tensor* y) { return orc::dotc(x->a, y->a); }
+int orc_axpy(double alpha, orc_tensor* x, orc_tensor* y) { ORC_GUARD(orc::axpy(alpha, x->a, y->a)) }
+
+// Davidson over ngroups operator groups (two-site form)
+int orc_davidson_twosite(int ngroups, orc_tensor** lmpo, orc_tensor** rmpo, orc_tensor** lopr, orc_tensor** ropr,
+                         orc_tensor* diag, orc_tensor* wfnc, double tol, double* energy, int32_t* nsigma) {
+  ORC_GUARD({
+    orc::Functor2 F;
+    for (int g = 0; g < ngroups; ++g) {
+      F.lmpo.push_back(&lmpo[g]->a);
+      F.rmpo.push_back(&rmpo[g]->a);
+      F.lopr.push_back(&lopr[g]->a);
+      F.ropr.push_back(&ropr[g]->a);
+    }
+    *energy = orc::davidson(F, diag->a, wfnc->a, tol);
+    *nsigma = F.nsigma;
+  })
+}
+
+int orc_jacobi_eig(int n, const double* a, double* w, double* v) {
+  ORC_GUARD({
+    std::vector<double> A(a, a + n * n), W, V;
+    orc::jacobi_eig(n, A, W, V);
+    std::memcpy(w, W.data(), n * sizeof(double));
+    std::memcpy(v, V.data(), n * n * sizeof(double));
+  })
+}
+
+}  // extern "C"

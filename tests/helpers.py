@@ -1,0 +1,83 @@
+"""Shared builders for the tests: random chains, environments from the oracle, dense projections."""
+import numpy as np
+
+import oracle
+from mpsxx_amd import mps
+from mpsxx_amd.mpo import hubbard_mpo
+
+
+def chain_envs(mpos, ch, i, L):
+    """Left environment at site i and right environment at site i+2 (oracle renormalize chains)."""
+    lenv = mps.boundary_env()
+    for k in range(i):
+        lenv = oracle.renormalize(True, mpos[k], lenv, ch["A"][k], ch["A"][k])
+    renv = mps.boundary_env()
+    for k in range(L - 1, i + 1, -1):
+        renv = oracle.renormalize(False, mpos[k], renv, ch["B"][k], ch["B"][k])
+    return lenv, renv
+
+
+def chain_overlaps(ch, i, L):
+    lsov = mps.boundary_overlap()
+    for k in range(i):
+        lsov = oracle.renormalize_overlap(True, lsov, ch["A"][k], ch["A"][k])
+    rsov = mps.boundary_overlap()
+    for k in range(L - 1, i + 1, -1):
+        rsov = oracle.renormalize_overlap(False, rsov, ch["B"][k], ch["B"][k])
+    return lsov, rsov
+
+
+def projectors(ch, i, L):
+    """Dense maps P_L (4^i x D_l) and P_R (D_r x 4^(L-i-2)) of the random chain."""
+    PL = np.ones((1, 1))
+    for k in range(i):
+        Ad = ch["A"][k].dense()
+        x = np.einsum("xa,anb->xnb", PL, Ad)
+        PL = x.reshape(-1, x.shape[2])
+    PR = np.ones((1, 1))
+    for k in range(L - 1, i + 1, -1):
+        Bd = ch["B"][k].dense()
+        x = np.einsum("anb,by->any", Bd, PR)
+        PR = x.reshape(x.shape[0], -1)
+    return PL, PR
+
+
+def hubbard_problem(L, i, rng, dmax=3, t=1.0, U=4.0, qt=None):
+    if qt is None:
+        qt = (L, L % 2)
+    mpos = hubbard_mpo(L, t, U)
+    ch = mps.random_chain(L, i, qt, rng, dmax=dmax)
+    lenv, renv = chain_envs(mpos, ch, i, L)
+    return mpos, ch, lenv, renv
+
+
+def random_env_left(lq, ld, lmpo, rng, scale=1.0):
+    """Pattern-preserving random fill of a left environment L(l_bra, w, l_ket): quantum shape (lq, -q_w, -lq)."""
+    from mpsxx_amd.qst import QST
+    qw = lmpo.qshape[0]
+    e = QST((0, 0), [lq, [(-a, -b) for (a, b) in qw], [(-a, -b) for (a, b) in lq]],
+            [ld, [1] * len(qw), ld])
+    return mps.fill_allowed(e, rng, scale)
+
+
+def random_env_right(rq, rd, rmpo, rng, scale=1.0):
+    """Random right environment R(r_bra, w, r_ket): quantum shape (-rq, q_w, rq) with q_w = -rmpo.qshape[3]."""
+    from mpsxx_amd.qst import QST
+    qw = [(-a, -b) for (a, b) in rmpo.qshape[3]]
+    e = QST((0, 0), [[(-a, -b) for (a, b) in rq], qw, rq], [rd, [1] * len(qw), rd])
+    return mps.fill_allowed(e, rng, scale)
+
+
+def random_twosite_problem(L, i, rng, dims, U=4.0, qt=None):
+    """Two-site problem at bond sizes given by dims(q) with pattern-preserving random environments."""
+    if qt is None:
+        qt = (L, L % 2)
+    mpos = hubbard_mpo(L, 1.0, U)
+    lq = mps.left_bond_qns(L, i, qt)
+    rq = mps.right_bond_qns(L, i + 2, qt)
+    ld = [dims(q) for q in lq]
+    rd = [dims(q) for q in rq]
+    lenv = random_env_left(lq, ld, mpos[i], rng)
+    renv = random_env_right(rq, rd, mpos[i + 1], rng)
+    psi = mps.fill_allowed(mps.twosite_wavefunction(lq, ld, rq, rd, qt), rng)
+    return mpos[i], mpos[i + 1], lenv, renv, psi
