@@ -131,8 +131,16 @@ int mpsxx_renormalize_merged(int forward, mpsxx_tensor opr0, mpsxx_tensor bra0, 
                              mpsxx_tensor opr1, int precision, void* stream);                       /* :57-74  */
 
 /* ---- merged operator (src/optimize.hpp:273-368) ------------------------------------------------------------------- */
+/* forward: L~(Lb,w',Lk) from L(lb,w,lk) and W(w,nb,nk,w'); backward: R~(Rb,w,Rk) from W and R(rb,w',rk); one call per
+ * operator group (the reference loops over groups, optimize.hpp:318-322, 355-359).  The output is overwritten. */
 int mpsxx_compute_merged_operator(int forward, mpsxx_tensor xmpo, mpsxx_tensor xopr, mpsxx_tensor mopr,
                                   void* stream);
+/* merged overlap S~ = merge(S (x) I_site) (optimize.hpp:326-328, 363-365); output overwritten */
+int mpsxx_compute_merged_overlap(int forward, mpsxx_tensor xmpo, mpsxx_tensor xsov, mpsxx_tensor msov,
+                                 void* stream);
+/* btas::QSTmerge(rows, a, cols, b) merging the first nrow and the last ncol indices (include/btas/QSPARSE/QSTmerge.h,
+ * QSTmergeInfo.h:44-95); nrow or ncol = 1 leaves that side as is.  Output overwritten. */
+int mpsxx_qst_merge(mpsxx_tensor x, int32_t nrow, int32_t ncol, mpsxx_tensor y, void* stream);
 
 /* ---- Davidson (src/diagonalize.hpp:310-449) ----------------------------------------------------------------------- */
 /* Ground state of sum_g H_g with H_g given by the two-site operator groups; wfnc is the initial guess and receives the
@@ -141,6 +149,32 @@ int mpsxx_compute_merged_operator(int forward, mpsxx_tensor xmpo, mpsxx_tensor x
 int mpsxx_davidson_twosite(int ngroups, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
                            const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, mpsxx_tensor diag, mpsxx_tensor wfnc,
                            double tolerance, int precision, double* energy, int32_t* n_sigma, void* stream);
+
+/* ---- two-site canonicalization (src/canonicalize.hpp:27-46 -> QSTLAPACK.h:122-288) -------------------------------- */
+/* forward (LeftArrow): lmps = U (total QN 0), rmps = s*Vt; backward (RightArrow): lmps = U*s, rmps = Vt (total 0).
+ * Keeps every singular value >= the M-th largest (M <= 0: all >= 1e-16); returns the discarded weight. */
+int mpsxx_canonicalize_twosite(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmps, mpsxx_tensor rmps, int max_states,
+                               double* discarded, void* stream);
+
+/* ---- device-resident two-site sweeps (dmrg/make_sweep.cpp:12-254, src/make_random_mpss.cpp:97-272) --------------- */
+typedef struct mpsxx_dmrg_s* mpsxx_dmrg;
+typedef struct mpsxx_sweep_stats {
+  double seconds;          /* wall time of the sweep                                                              */
+  double sigma_ms;         /* device time in sigma vectors (HIP events)                                           */
+  double davidson_s, svd_s, renorm_s, diag_s;
+  int32_t nsigma, max_bond;
+  double discarded_weight;
+} mpsxx_sweep_stats;
+/* init_mps: site tensors A(l,n,r) with total QN 0 except the last site, which carries the target (N, 2Sz); the driver
+ * right-canonicalizes them (truncating to max_states) and builds every right environment on the device. */
+int mpsxx_dmrg_create(int nsites, const mpsxx_tensor* mpo, const mpsxx_tensor* init_mps, int max_states,
+                      mpsxx_dmrg* out, void* stream);
+/* one forward + one backward two-site sweep; *energy = minimum site energy (make_sweep.cpp:115-118) */
+int mpsxx_dmrg_sweep(mpsxx_dmrg d, int max_states, double tolerance, int precision, double* energy,
+                     mpsxx_sweep_stats* stats, void* stream);
+int mpsxx_dmrg_site(mpsxx_dmrg d, int site, mpsxx_tensor out, void* stream);
+int mpsxx_dmrg_free(mpsxx_dmrg d);
+const char* mpsxx_dmrg_last_error(void);
 
 /* ---- plan introspection (measurement) ---------------------------------------------------------------------------- */
 /* Algorithmic cost of the two-site sigma vector for these inputs in the reference contraction order: flops = 2 m n k

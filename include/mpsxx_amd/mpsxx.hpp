@@ -1,0 +1,191 @@
+// Drop-in C++ boundary: the reference's hot-path function templates (namespace mpsxx, same names, argument order and
+// accumulate semantics) implemented over the C ABI in mpsxx_amd.h.
+//
+// Works with any block-sparse container offering the btas::QSTArray API subset the reference drivers use
+// (include/btas/QSPARSE/QSTArray.h, include/btas/SPARSE/STArray.h): q(), qshape(i), dshape(i), begin()/end() over
+// (tag, shared_ptr<dense block>) pairs in ascending tag order, index(tag), size(), resize(q, qshape, dshape, false) and
+// reserve(tag); dense blocks provide data() and size().  Quantum numbers provide p(), s() and a (p, s) constructor
+// (src/fermion.h).  Including this header in place of src/compute_sigma_vector.hpp, src/compute_diagonal_elements.hpp
+// and src/renormalize.hpp routes Davidson (src/diagonalize.hpp) and the site optimizers (src/optimize.hpp) through the
+// device path with unchanged call sites.
+//
+// Each call uploads its inputs, runs on the device and downloads the output (the output is uploaded first when it is
+// non-empty, to keep the reference's accumulate semantics).  Callers that keep operands resident use the C ABI
+// directly (mpsxx_tensor handles).  Errors from the C ABI are rethrown as std::runtime_error, as BTAS_THROW does.
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mpsxx_amd.h"
+
+namespace mpsxx_amd {
+
+inline void check(int rc) {
+  if (rc != MPSXX_OK) throw std::runtime_error(std::string("mpsxx_amd: ") + mpsxx_last_error());
+}
+
+// RAII handle
+class Handle {
+ public:
+  Handle() { check(mpsxx_tensor_create_empty(&h_)); }
+  explicit Handle(mpsxx_tensor h) : h_(h) {}
+  ~Handle() {
+    if (h_) mpsxx_tensor_free(h_);
+  }
+  Handle(const Handle&) = delete;
+  Handle& operator=(const Handle&) = delete;
+  Handle(Handle&& o) noexcept : h_(o.h_) { o.h_ = nullptr; }
+  mpsxx_tensor get() const { return h_; }
+
+ private:
+  mpsxx_tensor h_ = nullptr;
+};
+
+// host container -> device tensor
+template <class Arr>
+Handle upload(const Arr& a, bool conj = false) {
+  if (a.size() == 0) return Handle();
+  const int N = (int)a.qshape().size();
+  std::vector<int32_t> nsec(N), secq, secd, bidx;
+  for (int i = 0; i < N; ++i) {
+    const auto& qs = a.qshape(i);
+    const auto& ds = a.dshape(i);
+    nsec[i] = (int32_t)ds.size();
+    for (size_t k = 0; k < ds.size(); ++k) {
+      secq.push_back(conj ? -qs[k].p() : qs[k].p());
+      secq.push_back(conj ? -qs[k].s() : qs[k].s());
+      secd.push_back((int32_t)ds[k]);
+    }
+  }
+  std::vector<double> data;
+  for (auto it = a.begin(); it != a.end(); ++it) {
+    const auto idx = a.index(it->first);
+    for (int i = 0; i < N; ++i) bidx.push_back((int32_t)idx[i]);
+    const double* p = it->second->data();
+    data.insert(data.end(), p, p + it->second->size());
+  }
+  mpsxx_qst_desc d;
+  d.rank = N;
+  d.qsym = 1;
+  d.q_total[0] = conj ? -a.q().p() : a.q().p();
+  d.q_total[1] = conj ? -a.q().s() : a.q().s();
+  d.nsec = nsec.data();
+  d.sec_q = secq.data();
+  d.sec_dim = secd.data();
+  d.nblocks = (int64_t)(bidx.size() / (size_t)N);
+  d.block_idx = bidx.data();
+  mpsxx_tensor h;
+  check(mpsxx_tensor_create(&d, &h));
+  Handle out(h);
+  check(mpsxx_tensor_upload(h, data.data(), (int64_t)data.size(), nullptr));
+  return out;
+}
+
+// device tensor -> host container (replaces its content)
+template <class Arr>
+void download(const Handle& t, Arr& a) {
+  int32_t rank, qsym, q[2];
+  int64_t nsec_total, nb, ne;
+  check(mpsxx_tensor_info(t.get(), &rank, &qsym, q, &nsec_total, &nb, &ne));
+  if (rank == 0) {
+    a.clear();
+    return;
+  }
+  std::vector<int32_t> nsec(rank), secq(2 * nsec_total), secd(nsec_total), bidx(nb * rank);
+  std::vector<int64_t> boff(nb);
+  check(mpsxx_tensor_layout(t.get(), nsec.data(), secq.data(), secd.data(), bidx.data(), boff.data()));
+  std::vector<double> data(ne);
+  check(mpsxx_tensor_download(t.get(), data.data(), ne, nullptr));
+  typedef typename std::decay<decltype(a.q())>::type Q;
+  auto qshape = a.qshape();  // TVector<Qshapes<Q>, N>
+  auto dshape = a.dshape();  // TVector<Dshapes, N>
+  int64_t k = 0;
+  for (int i = 0; i < rank; ++i) {
+    qshape[i].clear();
+    dshape[i].clear();
+    for (int j = 0; j < nsec[i]; ++j, ++k) {
+      qshape[i].push_back(Q(secq[2 * k], secq[2 * k + 1]));
+      dshape[i].push_back(secd[k]);
+    }
+  }
+  a.resize(Q(q[0], q[1]), qshape, dshape, false);
+  for (int64_t b = 0; b < nb; ++b) {
+    int64_t tag = 0;
+    for (int i = 0; i < rank; ++i) tag = tag * nsec[i] + bidx[b * rank + i];
+    auto it = a.reserve((int)tag);
+    if (it == a.end()) throw std::runtime_error("mpsxx_amd: downloaded block not allowed by the container");
+    std::memcpy(it->second->data(), data.data() + boff[b], it->second->size() * sizeof(double));
+  }
+}
+
+}  // namespace mpsxx_amd
+
+namespace mpsxx {
+
+//! Compute sigmavector for two-site algorithm: | phi > = H | psi >   (reference src/compute_sigma_vector.hpp:54-70)
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_sigma_vector(const Arr<double, 4, Q>& lmpo, const Arr<double, 4, Q>& rmpo, const Arr<double, 3, Q>& lopr,
+                          const Arr<double, 3, Q>& ropr, const Arr<double, 4, Q>& wfn0, Arr<double, 4, Q>& sgv0) {
+  using namespace mpsxx_amd;
+  Handle a = upload(lmpo), b = upload(rmpo), c = upload(lopr), d = upload(ropr), e = upload(wfn0), f = upload(sgv0);
+  check(mpsxx_compute_sigma_vector_twosite(a.get(), b.get(), c.get(), d.get(), e.get(), f.get(), MPSXX_FP64, nullptr));
+  download(f, sgv0);
+}
+
+//! Compute sigmavector: | phi > = H | psi >   (one-site, src/compute_sigma_vector.hpp:10-22)
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_sigma_vector(const Arr<double, 4, Q>& mpo0, const Arr<double, 3, Q>& lopr, const Arr<double, 3, Q>& ropr,
+                          const Arr<double, 3, Q>& wfn0, Arr<double, 3, Q>& sgv0) {
+  using namespace mpsxx_amd;
+  Handle a = upload(mpo0), c = upload(lopr), d = upload(ropr), e = upload(wfn0), f = upload(sgv0);
+  check(mpsxx_compute_sigma_vector_onesite(a.get(), c.get(), d.get(), e.get(), f.get(), MPSXX_FP64, nullptr));
+  download(f, sgv0);
+}
+
+//! Compute sigmavector for merged algorithm (src/compute_sigma_vector.hpp:86-96)
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_sigma_vector(const Arr<double, 3, Q>& lopr, const Arr<double, 3, Q>& ropr, const Arr<double, 2, Q>& wfn0,
+                          Arr<double, 2, Q>& sgv0) {
+  using namespace mpsxx_amd;
+  Handle c = upload(lopr), d = upload(ropr), e = upload(wfn0), f = upload(sgv0);
+  check(mpsxx_compute_sigma_vector_merged(c.get(), d.get(), e.get(), f.get(), MPSXX_FP64, nullptr));
+  download(f, sgv0);
+}
+
+//! Compute sigmavector: | phi > = S | psi >  (src/compute_sigma_vector.hpp:25-35, 73-83, 99-109)
+template <template <typename, size_t, class> class Arr, class Q, size_t N>
+void compute_sigma_vector(const Arr<double, 2, Q>& lopr, const Arr<double, 2, Q>& ropr, const Arr<double, N, Q>& wfn0,
+                          Arr<double, N, Q>& sgv0) {
+  using namespace mpsxx_amd;
+  Handle c = upload(lopr), d = upload(ropr), e = upload(wfn0), f = upload(sgv0);
+  check(mpsxx_compute_sigma_vector_overlap(c.get(), d.get(), e.get(), f.get(), nullptr));
+  download(f, sgv0);
+}
+
+//! Compute diagonal H elements for two-site algorithm (src/compute_diagonal_elements.hpp:35-61)
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_diagonal_elements(const Arr<double, 4, Q>& lmpo, const Arr<double, 4, Q>& rmpo,
+                               const Arr<double, 3, Q>& lopr, const Arr<double, 3, Q>& ropr, Arr<double, 4, Q>& diag) {
+  using namespace mpsxx_amd;
+  Handle a = upload(lmpo), b = upload(rmpo), c = upload(lopr), d = upload(ropr);
+  // diag carries q/qshape but may hold no blocks: create it with its structure only
+  Handle f = upload(diag);
+  check(mpsxx_compute_diagonal_elements_twosite(a.get(), b.get(), c.get(), d.get(), f.get(), nullptr));
+  download(f, diag);
+}
+
+//! Renormalize operator tensor (src/renormalize.hpp:10-32)
+template <template <typename, size_t, class> class Arr, class Q>
+void renormalize(bool forward, const Arr<double, 4, Q>& mpo0, const Arr<double, 3, Q>& opr0,
+                 const Arr<double, 3, Q>& bra0, const Arr<double, 3, Q>& ket0, Arr<double, 3, Q>& opr1) {
+  using namespace mpsxx_amd;
+  Handle a = upload(mpo0), b = upload(opr0), c = upload(bra0), d = upload(ket0), f = upload(opr1);
+  check(mpsxx_renormalize(forward ? 1 : 0, a.get(), b.get(), c.get(), d.get(), f.get(), MPSXX_FP64, nullptr));
+  download(f, opr1);
+}
+
+}  // namespace mpsxx

@@ -302,10 +302,24 @@ int mpsxx_renormalize_merged(int forward, mpsxx_tensor opr0, mpsxx_tensor bra0, 
 
 int mpsxx_compute_merged_operator(int forward, mpsxx_tensor xmpo, mpsxx_tensor xopr, mpsxx_tensor mopr,
                                   void* stream) {
+  return guard([&] { mpx::op_merged_operator(forward != 0, T(xmpo), T(xopr), T(mopr), S(stream)); });
+}
+
+int mpsxx_canonicalize_twosite(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmps, mpsxx_tensor rmps, int max_states,
+                               double* discarded, void* stream) {
   return guard([&] {
-    (void)forward; (void)xmpo; (void)xopr; (void)mopr; (void)stream;
-    throw mpx::Error(MPSXX_ERR_UNSUPPORTED, "compute_merged_operator: quantum-number merge not available yet");
+    const double d = mpx::op_canonicalize_twosite(forward != 0, T(wfnc), T(lmps), T(rmps), max_states, S(stream));
+    if (discarded) *discarded = d;
   });
+}
+
+int mpsxx_compute_merged_overlap(int forward, mpsxx_tensor xmpo, mpsxx_tensor xsov, mpsxx_tensor msov,
+                                 void* stream) {
+  return guard([&] { mpx::op_merged_overlap(forward != 0, T(xmpo), T(xsov), T(msov), S(stream)); });
+}
+
+int mpsxx_qst_merge(mpsxx_tensor x, int32_t nrow, int32_t ncol, mpsxx_tensor y, void* stream) {
+  return guard([&] { mpx::op_qst_merge(T(x), nrow, ncol, T(y), S(stream)); });
 }
 
 int mpsxx_davidson_twosite(int ngroups, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,

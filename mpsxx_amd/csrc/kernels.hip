@@ -305,6 +305,23 @@ void launch_fill(double* x, int64_t n, double v, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, n, v);
 }
 
+__global__ void svd_scale_kernel(double* X, int batch, int rows, int cols, int64_t stride, const double* __restrict__ S,
+                                 int kmax, int by_col) {
+  const int64_t per = (int64_t)rows * cols, n = per * batch;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / per, rem = e % per;
+    const int i = (int)(rem / cols), c = (int)(rem % cols);
+    X[b * stride + rem] *= S[b * kmax + (by_col ? c : i)];
+  }
+}
+
+void launch_svd_scale(double* X, int batch, int rows, int cols, int64_t stride, const double* S, int kmax, int by_col,
+                      hipStream_t s) {
+  const int64_t n = (int64_t)rows * cols * batch;
+  if (n > 0) hipLaunchKernelGGL(svd_scale_kernel, dim3(grid_for(n)), dim3(256), 0, s, X, batch, rows, cols, stride, S,
+                                kmax, by_col);
+}
+
 void launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, s, a, x, b, y, n);
 }

@@ -86,6 +86,9 @@ void launch_convert(const TX* x, TY* y, int64_t n, hipStream_t s);
 void launch_fill(double* x, int64_t n, double v, hipStream_t s);
 // y = a*x + b*y
 void launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s);
+// X[b*stride + i*cols + c] *= S[b*kmax + (by_col ? c : i)] for a batch of row-major (rows x cols) matrices
+void launch_svd_scale(double* X, int batch, int rows, int cols, int64_t stride, const double* S, int kmax, int by_col,
+                      hipStream_t s);
 
 // Davidson helpers on flat vectors of one layout (length n).
 // y = sum_i c[i] * X_i  with X_i = base + i*ld  (FP64; c lives in device memory)

@@ -1,0 +1,293 @@
+// Quantum-number merging of block-sparse indices and the merged block operators.
+//
+// MergeInfo follows btas::QSTmergeInfo (include/btas/QSPARSE/QSTmergeInfo.h:44-95): the merged indices are packed
+// row-major (first index slowest), each distinct packed quantum number becomes one merged sector in order of first
+// appearance, and the packed sectors of one merged sector are stacked in packed order.  QSTmerge
+// (QSTmerge.h:31-140) creates a merged block when at least one of its sub-blocks is stored and zero-fills the rest.
+#include <algorithm>
+#include <map>
+
+#include "mpsxx_amd.h"
+#include "ops.hpp"
+
+namespace mpx {
+
+
+MergeInfo merge_info(const std::vector<std::vector<QN>>& qs, const std::vector<std::vector<int>>& ds) {
+  MergeInfo M;
+  std::vector<QN> qp{QN()};
+  std::vector<int> dp{1};
+  for (size_t i = 0; i < qs.size(); ++i) {
+    if (qs[i].size() != ds[i].size()) throw Error(MPSXX_ERR_ARG, "QSTmergeInfo: qshape and dshape have different block size");
+    std::vector<QN> q2;
+    std::vector<int> d2;
+    for (size_t a = 0; a < qp.size(); ++a)
+      for (size_t b = 0; b < qs[i].size(); ++b) {
+        q2.push_back(qp[a] + qs[i][b]);
+        d2.push_back(dp[a] * ds[i][b]);
+      }
+    qp.swap(q2);
+    dp.swap(d2);
+    M.radix.push_back((int)qs[i].size());
+  }
+  std::map<std::pair<int, int>, int> idx;
+  for (size_t k = 0; k < qp.size(); ++k) {
+    auto key = std::make_pair(qp[k].p, qp[k].s);
+    auto it = idx.find(key);
+    int j;
+    if (it == idx.end()) {
+      j = (int)M.qm.size();
+      idx[key] = j;
+      M.qm.push_back(qp[k]);
+      M.dm.push_back(0);
+    } else {
+      j = it->second;
+    }
+    M.p2m.push_back(j);
+    M.poff.push_back(M.dm[j]);
+    M.pdim.push_back(dp[k]);
+    M.dm[j] += dp[k];
+  }
+  return M;
+}
+
+namespace {
+
+int packed_index(const MergeInfo& M, const int32_t* idx) {
+  int p = 0;
+  for (size_t i = 0; i < M.radix.size(); ++i) p = p * M.radix[i] + idx[i];
+  return p;
+}
+
+// Sub-rectangle copy segments: a source block viewed as (R x C) row-major placed at (ro, co) of a target block with
+// row length ldc.
+void rect_segments(int64_t src_off, int64_t R, int64_t C, int64_t dst_off, int64_t ro, int64_t co, int64_t ldc,
+                   std::vector<CopySeg>& segs) {
+  if (R == 0 || C == 0) return;
+  if (co == 0 && C == ldc) {
+    segs.push_back({dst_off + ro * ldc, src_off, (int32_t)(R * C), 1});
+    return;
+  }
+  for (int64_t r = 0; r < R; ++r) segs.push_back({dst_off + (ro + r) * ldc + co, src_off + r * C, (int32_t)C, 1});
+}
+
+}  // namespace
+
+// Structure and copy segments of QSTmerge(rows_info, a, cols_info): merge the first nr and the last nc indices of X
+// (nr or nc may be 1, which leaves that side unchanged).
+Layout merge_layout(const Layout& X, int nr, int nc, std::vector<CopySeg>& segs) {
+  const int N = X.rank;
+  if (nr < 1 || nc < 1 || nr + nc > N) throw Error(MPSXX_ERR_ARG, "QSTmerge: bad index counts");
+  std::vector<std::vector<QN>> rq(X.qs.begin(), X.qs.begin() + nr), cq(X.qs.end() - nc, X.qs.end());
+  std::vector<std::vector<int>> rd(X.ds.begin(), X.ds.begin() + nr), cd(X.ds.end() - nc, X.ds.end());
+  // a single index is left as is (no renumbering of sectors that share a quantum number)
+  auto identity = [](const std::vector<QN>& q, const std::vector<int>& d) {
+    MergeInfo M;
+    M.qm = q;
+    M.dm = d;
+    M.radix = {(int)q.size()};
+    for (size_t k = 0; k < q.size(); ++k) {
+      M.p2m.push_back((int)k);
+      M.poff.push_back(0);
+      M.pdim.push_back(d[k]);
+    }
+    return M;
+  };
+  MergeInfo MR = nr > 1 ? merge_info(rq, rd) : identity(rq[0], rd[0]);
+  MergeInfo MC = nc > 1 ? merge_info(cq, cd) : identity(cq[0], cd[0]);
+  std::vector<std::vector<QN>> qs;
+  std::vector<std::vector<int>> ds;
+  qs.push_back(MR.qm);
+  ds.push_back(MR.dm);
+  for (int i = nr; i < N - nc; ++i) { qs.push_back(X.qs[i]); ds.push_back(X.ds[i]); }
+  qs.push_back(MC.qm);
+  ds.push_back(MC.dm);
+  Layout Y;
+  Y.set_shape((int)qs.size(), X.qsym, X.q, qs, ds);
+  std::vector<int32_t> yi(Y.rank);
+  auto target = [&](const int32_t* xi) {
+    const int pr = packed_index(MR, xi), pc = packed_index(MC, xi + (N - nc));
+    yi[0] = MR.p2m[pr];
+    for (int i = nr; i < N - nc; ++i) yi[1 + i - nr] = xi[i];
+    yi[Y.rank - 1] = MC.p2m[pc];
+  };
+  for (int64_t b = 0; b < X.nblocks(); ++b) {
+    target(X.index(b));
+    if (Y.allowed(yi.data())) Y.add_block(yi.data());
+  }
+  Y.finalize();
+  segs.clear();
+  for (int64_t b = 0; b < X.nblocks(); ++b) {
+    const int32_t* xi = X.index(b);
+    target(xi);
+    const int64_t yb = Y.find(Y.compute_tag(yi.data()));
+    if (yb < 0) continue;
+    const int pr = packed_index(MR, xi), pc = packed_index(MC, xi + (N - nc));
+    int64_t mid = 1;
+    for (int i = nr; i < N - nc; ++i) mid *= X.ds[i][xi[i]];
+    const int64_t dr = MR.pdim[pr], dc = MC.pdim[pc];
+    const int64_t Dc = MC.dm[MC.p2m[pc]];
+    // target element (r, m, c) at ((ro + r) * mid + m) * Dc + co + c; source (r, m, c) at (r * mid + m) * dc + c
+    rect_segments(X.off[b], dr * mid, dc, Y.off[yb], (int64_t)MR.poff[pr] * mid, MC.poff[pc], Dc, segs);
+  }
+  return Y;
+}
+
+// Merged block operator (reference src/optimize.hpp:273-368), one operator group:
+//   forward : L~(Lb, w', Lk) = merge_{(lb,nb)} merge_{(lk,nk)} sum_w L(lb,w,lk) W(w,nb,nk,w')
+//   backward: R~(Rb, w, Rk)  = merge_{(nb,rb)} merge_{(nk,rk)} sum_w' W(w,nb,nk,w') R(rb,w',rk)
+// The contraction is the reference's Contract-by-symbols (one K=1 coefficient step), the permutation and both merges
+// become one placement copy.
+void op_merged_operator(bool fwd, Tensor& xmpo, Tensor& xopr, Tensor& mopr, hipStream_t s) {
+  if (xmpo.empty || xopr.empty) throw Error(MPSXX_ERR_ARG, "compute_merged_operator: empty input");
+  const Layout& W = xmpo.L;
+  const Layout& E = xopr.L;
+  // symbolic contraction: forward Contract(L, {1}, W, {0}) -> (lb, lk, nb, nk, w');
+  //                       backward Contract(W, {3}, R, {1}) -> (w, nb, nk, rb, rk)
+  View vw{&W, false}, ve{&E, false};
+  ContractSym sym = fwd ? contract_symbolic(ve, {1}, vw, {0}, nullptr) : contract_symbolic(vw, {3}, ve, {1}, nullptr);
+  const Layout& X = sym.C;
+  MergeInfo MR, MC;
+  if (fwd) {
+    MR = merge_info({E.qs[0], W.qs[1]}, {E.ds[0], W.ds[1]});
+    MC = merge_info({E.qs[2], W.qs[2]}, {E.ds[2], W.ds[2]});
+  } else {
+    MR = merge_info({W.qs[1], E.qs[0]}, {W.ds[1], E.ds[0]});
+    MC = merge_info({W.qs[2], E.qs[2]}, {W.ds[1], E.ds[0]});
+  }
+  Layout Y;
+  Y.set_shape(3, true, X.q, {MR.qm, fwd ? W.qs[3] : W.qs[0], MC.qm}, {MR.dm, fwd ? W.ds[3] : W.ds[0], MC.dm});
+  int32_t yi[3];
+  // (row sector index pair, col sector index pair, operator) of an X block
+  auto decode = [&](const int32_t* xi, int& pr, int& pc, int& w, int64_t& R, int64_t& C) {
+    if (fwd) {  // (lb, lk, nb, nk, w')
+      pr = xi[0] * MR.radix[1] + xi[2];
+      pc = xi[1] * MC.radix[1] + xi[3];
+      w = xi[4];
+      R = X.ds[0][xi[0]];
+      C = X.ds[1][xi[1]];
+    } else {    // (w, nb, nk, rb, rk)
+      pr = xi[1] * MR.radix[1] + xi[3];
+      pc = xi[2] * MC.radix[1] + xi[4];
+      w = xi[0];
+      R = X.ds[3][xi[3]];
+      C = X.ds[4][xi[4]];
+    }
+  };
+  for (int64_t b = 0; b < X.nblocks(); ++b) {
+    int pr, pc, w;
+    int64_t R, C;
+    decode(X.index(b), pr, pc, w, R, C);
+    yi[0] = MR.p2m[pr];
+    yi[1] = w;
+    yi[2] = MC.p2m[pc];
+    if (Y.allowed(yi)) Y.add_block(yi);
+  }
+  Y.finalize();
+  std::vector<CopySeg> segs;
+  for (int64_t b = 0; b < X.nblocks(); ++b) {
+    int pr, pc, w;
+    int64_t R, C;
+    decode(X.index(b), pr, pc, w, R, C);
+    yi[0] = MR.p2m[pr];
+    yi[1] = w;
+    yi[2] = MC.p2m[pc];
+    const int64_t yb = Y.find(Y.compute_tag(yi));
+    if (yb < 0) continue;
+    rect_segments(X.off[b], R, C, Y.off[yb], MR.poff[pr], MC.poff[pc], MC.dm[MC.p2m[pc]], segs);
+  }
+  // plan: [0] mpo, [1] operator, [2] contraction intermediate, [3] output
+  Plan P;
+  P.slot_layout = {W, E, X, Y};
+  P.slot_input = {0, 1, -1, -2};
+  P.slot_prec = {0, 0, 0, 0};
+  P.slot_buf.resize(4);
+  P.slot_buf[2].alloc(std::max<int64_t>(X.nelem, 1) * sizeof(double));
+  P.out = Y;
+  P.steps.push_back(fwd ? make_contract_step(sym, 1, 0, 2, 0, 1.0) : make_contract_step(sym, 0, 1, 2, 0, 1.0));
+  P.steps.push_back(make_copy_step(segs, 2, 3, 1.0, 0.0));
+  P.flops = sym.flops;
+  mopr.set_layout(Y, s);
+  run_plan(P, {xmpo.ptr(), xopr.ptr()}, mopr.ptr(), s);
+  hip_check(hipStreamSynchronize(s), "merged operator");
+}
+
+// Merged overlap (reference src/optimize.hpp:326-328, 363-365): S (x) I on the site, merged like the operators.
+//   forward : S~(Lb, Lk) = merge_{(lb,n)} merge_{(lk,n')} S(lb,lk) I(n,n')
+//   backward: S~(Rb, Rk) = merge_{(n,rb)} merge_{(n',rk)} I(n,n') S(rb,rk)
+// I is the site identity (the QSTArray filled with 1.0 on its allowed blocks, all diagonal for distinct site QNs).
+void op_merged_overlap(bool fwd, Tensor& xmpo, Tensor& xsov, Tensor& msov, hipStream_t s) {
+  if (xmpo.empty || xsov.empty) throw Error(MPSXX_ERR_ARG, "compute_merged_operator: empty input");
+  const Layout& W = xmpo.L;
+  const Layout& S = xsov.L;
+  const std::vector<QN>& qn = W.qs[1];
+  const std::vector<QN>& qk = W.qs[2];
+  MergeInfo MR, MC;
+  if (fwd) {
+    MR = merge_info({S.qs[0], qn}, {S.ds[0], W.ds[1]});
+    MC = merge_info({S.qs[1], qk}, {S.ds[1], W.ds[2]});
+  } else {
+    MR = merge_info({qn, S.qs[0]}, {W.ds[1], S.ds[0]});
+    MC = merge_info({qk, S.qs[1]}, {W.ds[2], S.ds[1]});
+  }
+  // identity blocks (a, b) allowed: qn[a] + qk[b] == 0
+  std::vector<std::pair<int, int>> eye;
+  for (size_t a = 0; a < qn.size(); ++a)
+    for (size_t b = 0; b < qk.size(); ++b)
+      if (qn[a] + qk[b] == QN() && W.ds[1][a] == 1 && W.ds[2][b] == 1) eye.push_back({(int)a, (int)b});
+  Layout Y;
+  Y.set_shape(2, true, S.q, {MR.qm, MC.qm}, {MR.dm, MC.dm});
+  struct Place { int64_t src, R, C; int pr, pc; };
+  std::vector<Place> pl;
+  for (int64_t b = 0; b < S.nblocks(); ++b) {
+    const int32_t* si = S.index(b);
+    for (auto& e : eye) {
+      Place p;
+      p.src = S.off[b];
+      p.R = S.ds[0][si[0]];
+      p.C = S.ds[1][si[1]];
+      p.pr = fwd ? si[0] * MR.radix[1] + e.first : e.first * MR.radix[1] + si[0];
+      p.pc = fwd ? si[1] * MC.radix[1] + e.second : e.second * MC.radix[1] + si[1];
+      int32_t yi[2] = {MR.p2m[p.pr], MC.p2m[p.pc]};
+      if (!Y.allowed(yi)) continue;
+      Y.add_block(yi);
+      pl.push_back(p);
+    }
+  }
+  Y.finalize();
+  std::vector<CopySeg> segs;
+  for (const Place& p : pl) {
+    int32_t yi[2] = {MR.p2m[p.pr], MC.p2m[p.pc]};
+    const int64_t yb = Y.find(Y.compute_tag(yi));
+    rect_segments(p.src, p.R, p.C, Y.off[yb], MR.poff[p.pr], MC.poff[p.pc], MC.dm[MC.p2m[p.pc]], segs);
+  }
+  Plan P;
+  P.slot_layout = {S, Y};
+  P.slot_input = {0, -2};
+  P.slot_prec = {0, 0};
+  P.slot_buf.resize(2);
+  P.out = Y;
+  P.steps.push_back(make_copy_step(segs, 0, 1, 1.0, 0.0));
+  msov.set_layout(Y, s);
+  run_plan(P, {xsov.ptr()}, msov.ptr(), s);
+  hip_check(hipStreamSynchronize(s), "merged overlap");
+}
+
+// QSTmerge of the first nr and last nc indices (nr, nc >= 1).
+void op_qst_merge(Tensor& x, int nr, int nc, Tensor& y, hipStream_t s) {
+  if (x.empty) throw Error(MPSXX_ERR_ARG, "QSTmerge: empty input");
+  std::vector<CopySeg> segs;
+  Layout Y = merge_layout(x.L, nr, nc, segs);
+  Plan P;
+  P.slot_layout = {x.L, Y};
+  P.slot_input = {0, -2};
+  P.slot_prec = {0, 0};
+  P.slot_buf.resize(2);
+  P.out = Y;
+  P.steps.push_back(make_copy_step(segs, 0, 1, 1.0, 0.0));
+  y.set_layout(Y, s);
+  run_plan(P, {x.ptr()}, y.ptr(), s);
+  hip_check(hipStreamSynchronize(s), "merge");
+}
+
+}  // namespace mpx

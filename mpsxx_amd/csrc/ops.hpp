@@ -52,6 +52,24 @@ Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& l
 
 void clear_plan_cache();
 
+// Quantum-number merging (include/btas/QSPARSE/QSTmerge.h) and merged operators (src/optimize.hpp:273-368)
+struct MergeInfo {
+  std::vector<QN> qm;     // merged sector quantum numbers (first-appearance order)
+  std::vector<int> dm;    // merged sector dims
+  std::vector<int> p2m;   // packed index -> merged sector
+  std::vector<int> poff;  // packed index -> offset inside its merged sector
+  std::vector<int> pdim;  // packed index -> dim
+  std::vector<int> radix; // sector counts of the merged indices (packing radix, first index slowest)
+};
+MergeInfo merge_info(const std::vector<std::vector<QN>>& qs, const std::vector<std::vector<int>>& ds);
+Layout merge_layout(const Layout& X, int nr, int nc, std::vector<CopySeg>& segs);
+void op_qst_merge(Tensor& x, int nr, int nc, Tensor& y, hipStream_t s);
+void op_merged_operator(bool fwd, Tensor& xmpo, Tensor& xopr, Tensor& mopr, hipStream_t s);
+void op_merged_overlap(bool fwd, Tensor& xmpo, Tensor& xsov, Tensor& msov, hipStream_t s);
+
+// Two-site canonicalization with truncation to M states (src/canonicalize.hpp:27-46); returns the discarded weight.
+double op_canonicalize_twosite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor& rmps, int M, hipStream_t s);
+
 // Davidson (src/diagonalize.hpp:310-449) over operator groups of the two-site form
 struct DavGroup {
   Tensor *lmpo, *rmpo, *lopr, *ropr;

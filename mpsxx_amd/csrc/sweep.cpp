@@ -1,0 +1,238 @@
+// Device-resident two-site DMRG sweeps (reference: dmrg/make_sweep.cpp:40-254 with src/optimize.hpp:164-265 and the
+// random-MPS initialisation of src/make_random_mpss.cpp:97-272).
+//
+// Everything stays in HBM across the sweep: MPO site tensors, MPS site tensors, every left/right environment.  The
+// reference instead writes each object to a per-site Boost archive and reloads it (src/fileio.h:9-25).  Per site:
+// two-site wavefunction (Gemm), diagonal, Davidson (tolerance 0.1*T as make_sweep.cpp:112 passes), canonicalize with
+// truncation to M, renormalize.  The sweep energy is the minimum of the site energies (make_sweep.cpp:115).
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "mpsxx_amd.h"
+#include "ops.hpp"
+
+namespace mpx {
+
+struct SweepStats {
+  double seconds = 0.0, sigma_ms = 0.0, davidson_s = 0.0, svd_s = 0.0, renorm_s = 0.0, diag_s = 0.0;
+  int nsigma = 0, max_bond = 0;
+  double dweight = 0.0;
+};
+
+struct Dmrg {
+  int L = 0;
+  std::vector<Tensor*> mpo;                 // borrowed
+  std::vector<std::unique_ptr<Tensor>> mps; // site tensors (A left of the centre, wave at the centre, B right)
+  std::vector<std::unique_ptr<Tensor>> lenv, renv;  // lenv[i]: left of site i; renv[i]: right of site i-1
+  int center = 0;
+};
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+double since(clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); }
+
+std::unique_ptr<Tensor> boundary_env(hipStream_t s) {
+  Layout L;
+  L.set_shape(3, true, QN(), {{QN()}, {QN()}, {QN()}}, {{1}, {1}, {1}});
+  int32_t z[3] = {0, 0, 0};
+  L.add_block(z);
+  L.finalize();
+  auto t = std::make_unique<Tensor>();
+  t->set_layout(L, s);
+  launch_fill(t->ptr(), 1, 1.0, s);
+  return t;
+}
+
+// two-site psi = Gemm(N, N, a(l,n,r), b(r,n',r')) (optimize.hpp:180-182)
+void twosite(Tensor& a, Tensor& b, Tensor& psi, hipStream_t s) {
+  psi = Tensor();
+  op_contract(1.0, a, false, {2}, b, false, {0}, 1.0, psi, s);
+}
+
+Tensor shape_only(const Layout& L) {
+  Layout e = L;
+  e.tag.clear();
+  e.bidx.clear();
+  e.finalize();
+  Tensor t;
+  t.set_layout(e, nullptr);
+  return t;
+}
+
+int bond_dim(const Layout& L, int idx) {
+  int d = 0;
+  for (int x : L.ds[idx]) d += x;
+  return d;
+}
+
+}  // namespace
+
+Dmrg* dmrg_create(int L, const std::vector<Tensor*>& mpo, const std::vector<Tensor*>& init, int M, hipStream_t s) {
+  if (L < 2 || (int)mpo.size() != L || (int)init.size() != L) throw Error(MPSXX_ERR_ARG, "dmrg: need L >= 2 sites");
+  auto D = std::make_unique<Dmrg>();
+  D->L = L;
+  D->mpo = mpo;
+  D->mps.resize(L);
+  for (int i = 0; i < L; ++i) {
+    D->mps[i] = std::make_unique<Tensor>();
+    op_copy(*init[i], *D->mps[i], s);
+  }
+  // right-canonicalise from the right end: psi = T_{i-1} T_i, RightArrow SVD -> (U s, B_i)
+  // (make_random_mpss.cpp:204-251 does the same with one-site canonicalize(0, ...))
+  for (int i = L - 1; i >= 1; --i) {
+    Tensor psi, U, B;
+    twosite(*D->mps[i - 1], *D->mps[i], psi, s);
+    op_canonicalize_twosite(false, psi, U, B, M, s);
+    const double nrm = op_nrm2(U, s);
+    op_scal(1.0 / nrm, U, s);
+    *D->mps[i - 1] = std::move(U);
+    *D->mps[i] = std::move(B);
+  }
+  D->lenv.resize(L + 1);
+  D->renv.resize(L + 1);
+  D->lenv[0] = boundary_env(s);
+  D->renv[L] = boundary_env(s);
+  for (int i = L - 1; i >= 1; --i) {
+    D->renv[i] = std::make_unique<Tensor>();
+    op_renormalize(false, *mpo[i], *D->renv[i + 1], *D->mps[i], *D->mps[i], *D->renv[i], MPSXX_FP64, s);
+  }
+  D->center = 0;
+  hip_check(hipStreamSynchronize(s), "dmrg init");
+  return D.release();
+}
+
+// One forward + one backward two-site sweep; returns the minimum site energy.
+double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStream_t s) {
+  const int L = D.L;
+  const auto t0 = clk::now();
+  double emin = 1.0e8;
+  auto optimize = [&](int i, bool forward) {
+    // sites (i, i+1); forward: mps[i] is the wave, mps[i+1] right-canonical; backward: mps[i] left, mps[i+1] wave
+    Tensor psi;
+    twosite(*D.mps[i], *D.mps[i + 1], psi, s);
+    auto t = clk::now();
+    Tensor diag = shape_only(psi.L);
+    op_diag_twosite(*D.mpo[i], *D.mpo[i + 1], *D.lenv[i], *D.renv[i + 2], diag, s);
+    hip_check(hipStreamSynchronize(s), "diag");
+    st.diag_s += since(t);
+    t = clk::now();
+    DavidsonStats ds;
+    std::vector<DavGroup> g{{D.mpo[i], D.mpo[i + 1], D.lenv[i].get(), D.renv[i + 2].get()}};
+    const double e = davidson_twosite(g, diag, psi, 0.1 * tol, prec, ds, s);
+    st.davidson_s += since(t);
+    st.nsigma += ds.nsigma;
+    st.sigma_ms += ds.sigma_ms;
+    if (e < emin) emin = e;
+    t = clk::now();
+    Tensor U, V;
+    st.dweight += op_canonicalize_twosite(forward, psi, U, V, M, s);
+    st.svd_s += since(t);
+    st.max_bond = std::max(st.max_bond, bond_dim(U.L, 2));
+    *D.mps[i] = std::move(U);
+    *D.mps[i + 1] = std::move(V);
+    t = clk::now();
+    if (forward) {
+      D.lenv[i + 1] = std::make_unique<Tensor>();
+      op_renormalize(true, *D.mpo[i], *D.lenv[i], *D.mps[i], *D.mps[i], *D.lenv[i + 1], MPSXX_FP64, s);
+    } else {
+      D.renv[i + 1] = std::make_unique<Tensor>();
+      op_renormalize(false, *D.mpo[i + 1], *D.renv[i + 2], *D.mps[i + 1], *D.mps[i + 1], *D.renv[i + 1], MPSXX_FP64,
+                     s);
+    }
+    hip_check(hipStreamSynchronize(s), "renormalize");
+    st.renorm_s += since(t);
+  };
+  for (int i = 0; i < L - 1; ++i) optimize(i, true);
+  for (int i = L - 2; i >= 0; --i) optimize(i, false);
+  D.center = 0;
+  st.seconds += since(t0);
+  return emin;
+}
+
+void dmrg_free(Dmrg* D) { delete D; }
+
+}  // namespace mpx
+
+struct mpsxx_tensor_s {
+  mpx::Tensor t;
+};
+
+struct mpsxx_dmrg_s {
+  mpx::Dmrg* d = nullptr;
+};
+
+namespace {
+thread_local std::string g_dmrg_err;
+template <typename F>
+int dguard(F f) {
+  try {
+    f();
+    return MPSXX_OK;
+  } catch (const mpx::Error& e) {
+    g_dmrg_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_dmrg_err = e.what();
+    return MPSXX_ERR_ARG;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* mpsxx_dmrg_last_error(void) { return g_dmrg_err.c_str(); }
+
+int mpsxx_dmrg_create(int nsites, const mpsxx_tensor* mpo, const mpsxx_tensor* init_mps, int max_states,
+                      mpsxx_dmrg* out, void* stream) {
+  return dguard([&] {
+    std::vector<mpx::Tensor*> m, i;
+    for (int k = 0; k < nsites; ++k) {
+      m.push_back(&mpo[k]->t);
+      i.push_back(&init_mps[k]->t);
+    }
+    auto* h = new mpsxx_dmrg_s;
+    h->d = mpx::dmrg_create(nsites, m, i, max_states, reinterpret_cast<hipStream_t>(stream));
+    *out = h;
+  });
+}
+
+int mpsxx_dmrg_sweep(mpsxx_dmrg d, int max_states, double tolerance, int precision, double* energy,
+                     mpsxx_sweep_stats* stats, void* stream) {
+  return dguard([&] {
+    mpx::SweepStats st;
+    const double e = mpx::dmrg_sweep(*d->d, max_states, tolerance, precision, st, reinterpret_cast<hipStream_t>(stream));
+    if (energy) *energy = e;
+    if (stats) {
+      stats->seconds = st.seconds;
+      stats->sigma_ms = st.sigma_ms;
+      stats->davidson_s = st.davidson_s;
+      stats->svd_s = st.svd_s;
+      stats->renorm_s = st.renorm_s;
+      stats->diag_s = st.diag_s;
+      stats->nsigma = st.nsigma;
+      stats->max_bond = st.max_bond;
+      stats->discarded_weight = st.dweight;
+    }
+  });
+}
+
+int mpsxx_dmrg_site(mpsxx_dmrg d, int site, mpsxx_tensor out, void* stream) {
+  return dguard([&] {
+    if (site < 0 || site >= d->d->L) throw mpx::Error(MPSXX_ERR_ARG, "site out of range");
+    mpx::op_copy(*d->d->mps[site], out->t, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int mpsxx_dmrg_free(mpsxx_dmrg d) {
+  return dguard([&] {
+    if (d) {
+      mpx::dmrg_free(d->d);
+      delete d;
+    }
+  });
+}
+
+}  // extern "C"

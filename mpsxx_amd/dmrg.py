@@ -1,0 +1,107 @@
+"""Two-site DMRG driver on the device (Python side of the mpsxx_dmrg_* C ABI).
+
+Initial MPS (reference src/make_random_mpss.cpp:20-94, 120-171): bond quantum numbers from
+``generate_quantum_states`` (all products of site quantum numbers with particle number <= N, pruned to those that
+can still reach the target, optionally capped at M sectors around the middle), every sector of dimension 1, total
+QN 0 on every site except the last, which carries (N, 2Sz).  Entries: N(0,1) per element of every allowed block
+from numpy's default_rng(seed) (the configuration's seeded initialisation).
+"""
+import ctypes
+
+import numpy as np
+
+from . import DTensor, _check, lib
+from .mpo import SITE_Q
+from .qst import QST
+
+
+def generate_quantum_states(L, qt, max_blocks=0):
+    """Bond quantum-number lists qb[0..L-1] (qb[i] = bond right of site i, left convention)."""
+    site = list(SITE_Q)
+    qb = [None] * L
+    qb[0] = list(site)
+    for i in range(1, L - 1):
+        bare = sorted({(a[0] + b[0], a[1] + b[1]) for a in qb[i - 1] for b in site})
+        qb[i] = [q for q in bare if q[0] <= qt[0]]
+    qb[L - 1] = [tuple(qt)]
+    for i in range(L - 1, 0, -1):
+        qr = qb[i]
+        qb[i - 1] = [q for q in qb[i - 1] if any((q[0] + n[0], q[1] + n[1]) in qr for n in site)]
+        assert qb[i - 1]
+    for i in range(L - 1):
+        ql, qr = qb[i], qb[i + 1]
+        if max_blocks > 0 and len(ql) > max_blocks:
+            offs = (len(ql) - max_blocks) // 2
+            qb[i] = ql[offs:offs + max_blocks]
+            qb[i + 1] = [q for q in qr if any((l[0] + n[0], l[1] + n[1]) == q for l in qb[i] for n in site)]
+            assert qb[i + 1]
+    return qb
+
+
+def initial_mps(L, qt, M, seed=0):
+    rng = np.random.default_rng(seed)
+    qb = generate_quantum_states(L, qt, M)
+    mps = []
+    ql = [(0, 0)]
+    for i in range(L):
+        qr = qb[i] if i < L - 1 else [(0, 0)]
+        total = (0, 0) if i < L - 1 else tuple(qt)
+        a = QST(total, [ql, SITE_Q, [(-p, -s) for (p, s) in qr]], [[1] * len(ql), [1] * 4, [1] * len(qr)])
+        import itertools
+        for idx in itertools.product(range(len(ql)), range(4), range(len(qr))):
+            if a.allowed(idx):
+                a.blocks[idx] = rng.standard_normal((1, 1, 1))
+        nrm = a.norm()
+        if nrm > 0:
+            for k in a.blocks:
+                a.blocks[k] /= nrm
+        mps.append(a)
+        ql = qr
+    return mps
+
+
+class SweepStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("sigma_ms", ctypes.c_double), ("davidson_s", ctypes.c_double),
+                ("svd_s", ctypes.c_double), ("renorm_s", ctypes.c_double), ("diag_s", ctypes.c_double),
+                ("nsigma", ctypes.c_int32), ("max_bond", ctypes.c_int32), ("discarded_weight", ctypes.c_double)]
+
+
+class DeviceDMRG:
+    """Device-resident two-site DMRG over one operator group."""
+
+    def __init__(self, mpos, init_mps, M):
+        L = len(mpos)
+        self.L = L
+        self._mpo = [DTensor(w) for w in mpos]
+        init = [DTensor(a) for a in init_mps]
+        arr_m = (ctypes.c_void_p * L)(*[t.h.value for t in self._mpo])
+        arr_i = (ctypes.c_void_p * L)(*[t.h.value for t in init])
+        self.h = ctypes.c_void_p()
+        lib().mpsxx_dmrg_last_error.restype = ctypes.c_char_p
+        rc = lib().mpsxx_dmrg_create(ctypes.c_int(L), arr_m, arr_i, ctypes.c_int(M), ctypes.byref(self.h), None)
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_create: " + lib().mpsxx_dmrg_last_error().decode())
+
+    def sweep(self, M, tol, precision=0):
+        e = ctypes.c_double()
+        st = SweepStats()
+        rc = lib().mpsxx_dmrg_sweep(self.h, ctypes.c_int(M), ctypes.c_double(tol), ctypes.c_int(precision),
+                                    ctypes.byref(e), ctypes.byref(st), None)
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_sweep: " + lib().mpsxx_dmrg_last_error().decode())
+        return e.value, {k: getattr(st, k) for k, _ in SweepStats._fields_}
+
+    def site(self, i):
+        out = DTensor()
+        rc = lib().mpsxx_dmrg_site(self.h, ctypes.c_int(i), out.h, None)
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_site: " + lib().mpsxx_dmrg_last_error().decode())
+        return out.to_qst()
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().mpsxx_dmrg_free(self.h)
+                self.h = ctypes.c_void_p()
+        except Exception:
+            pass

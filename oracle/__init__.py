@@ -200,3 +200,27 @@ def jacobi_eig(a):
     v = np.zeros((n, n))
     _run(lib().orc_jacobi_eig, ctypes.c_int(n), a.ctypes.data_as(VP), w.ctypes.data_as(VP), v.ctypes.data_as(VP))
     return w, v
+
+
+def canonicalize_twosite(forward, wfn, M):
+    W = _o(wfn)
+    L, R = OTensor(), OTensor()
+    d = ctypes.c_double()
+    _run(lib().orc_canonicalize_twosite, ctypes.c_int(1 if forward else 0), W.h, L.h, R.h, ctypes.c_int(M),
+         ctypes.byref(d))
+    return L.to_qst(), R.to_qst(), d.value
+
+
+def dmrg(mpos, init_mps, Ms, tol):
+    """Two-site DMRG on the CPU (one forward + one backward sweep per entry of Ms).  Returns (energies, n_sigma)."""
+    L = len(mpos)
+    mo = [_o(x) for x in mpos]
+    io = [_o(x) for x in init_mps]
+    am = (VP * L)(*[x.h for x in mo])
+    ai = (VP * L)(*[x.h for x in io])
+    Ms = np.array(Ms, dtype=np.int32)
+    e = np.zeros(len(Ms))
+    ns = ctypes.c_int32()
+    _run(lib().orc_dmrg, ctypes.c_int(L), am, ai, ctypes.c_int(len(Ms)), Ms.ctypes.data_as(VP), ctypes.c_double(tol),
+         e.ctypes.data_as(VP), ctypes.byref(ns))
+    return e, ns.value

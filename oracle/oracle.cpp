@@ -369,6 +369,350 @@ QST deep_copy(const QST& x) {
   return y;
 }
 
+// ---- quantum-number merge (QSTmergeInfo.h:44-95, QSTmerge.h:31-140) ---------------------------------------------
+
+struct MergeInfo {
+  std::vector<QN> qm;
+  std::vector<int> dm;
+  std::multimap<int, int> m2p;  // merged -> packed (ascending packed order)
+  std::vector<int> pdim;
+};
+
+MergeInfo merge_info(const std::vector<std::vector<QN>>& qs, const std::vector<std::vector<int>>& ds) {
+  MergeInfo M;
+  std::vector<QN> qp{QN()};
+  std::vector<int> dp{1};
+  for (size_t i = 0; i < qs.size(); ++i) {
+    std::vector<QN> q2;
+    std::vector<int> d2;
+    for (size_t a = 0; a < qp.size(); ++a)
+      for (size_t b = 0; b < qs[i].size(); ++b) {
+        q2.push_back(qp[a] + qs[i][b]);
+        d2.push_back(dp[a] * ds[i][b]);
+      }
+    qp = q2;
+    dp = d2;
+  }
+  // first-appearance numbering of distinct packed quantum numbers
+  std::vector<QN> seen;
+  for (size_t k = 0; k < qp.size(); ++k) {
+    int j = -1;
+    for (size_t t = 0; t < seen.size(); ++t)
+      if (seen[t] == qp[k]) j = (int)t;
+    if (j < 0) {
+      j = (int)seen.size();
+      seen.push_back(qp[k]);
+      M.dm.push_back(0);
+    }
+    M.m2p.insert({j, (int)k});
+    M.dm[j] += dp[k];
+  }
+  M.qm = seen;
+  M.pdim = dp;
+  return M;
+}
+
+// merge the first MR indices (row merge)
+QST merge_rows(const MergeInfo& R, int MR, const QST& a) {
+  const int N = a.rank, MC = N - MR;
+  QST b;
+  std::vector<std::vector<QN>> qs{R.qm};
+  std::vector<std::vector<int>> ds{R.dm};
+  for (int i = 0; i < MC; ++i) { qs.push_back(a.qs[MR + i]); ds.push_back(a.ds[MR + i]); }
+  b.set_shape(1 + MC, a.qsym, a.q, qs, ds);
+  long a_stride = 1, b_stride = 1;
+  for (int i = MR; i < N; ++i) a_stride *= a.nsec(i);
+  for (int i = 1; i < b.rank; ++i) b_stride *= b.nsec(i);
+  for (int i = 0; i < b.nsec(0); ++i) {
+    auto range = R.m2p.equal_range(i);
+    if (range.first == range.second) continue;
+    for (long j = 0; j < b_stride; ++j) {
+      std::vector<int> bi = b.index(i * b_stride + j);
+      if (!b.allowed(bi)) continue;
+      const long ncol = b.bsize(bi) / b.ds[0][bi[0]];
+      Block blk(b.bsize(bi), 0.0);
+      bool nz = false;
+      long row = 0;
+      for (auto it = range.first; it != range.second; ++it) {
+        const int irow = it->second, drow = R.pdim[irow];
+        auto f = a.blocks.find(irow * a_stride + j);
+        if (f != a.blocks.end()) {
+          nz = true;
+          std::copy(f->second->begin(), f->second->end(), blk.begin() + row * ncol);
+        }
+        row += drow;
+      }
+      if (nz) b.blocks[i * b_stride + j] = std::make_shared<Block>(blk);
+    }
+  }
+  return b;
+}
+
+// merge the last MC indices (column merge)
+QST merge_cols(const QST& a, const MergeInfo& C, int MC) {
+  const int N = a.rank, MR = N - MC;
+  QST b;
+  std::vector<std::vector<QN>> qs;
+  std::vector<std::vector<int>> ds;
+  for (int i = 0; i < MR; ++i) { qs.push_back(a.qs[i]); ds.push_back(a.ds[i]); }
+  qs.push_back(C.qm);
+  ds.push_back(C.dm);
+  b.set_shape(MR + 1, a.qsym, a.q, qs, ds);
+  long a_stride = 1, b_stride = b.nsec(MR);
+  for (int i = MR; i < N; ++i) a_stride *= a.nsec(i);
+  long nrows = 1;
+  for (int i = 0; i < MR; ++i) nrows *= b.nsec(i);
+  for (long i = 0; i < nrows; ++i)
+    for (long j = 0; j < b_stride; ++j) {
+      auto range = C.m2p.equal_range((int)j);
+      if (range.first == range.second) continue;
+      std::vector<int> bi = b.index(i * b_stride + j);
+      if (!b.allowed(bi)) continue;
+      const long Dc = b.ds[MR][bi[MR]];
+      const long R = b.bsize(bi) / Dc;
+      Block blk(b.bsize(bi), 0.0);
+      bool nz = false;
+      long col = 0;
+      for (auto it = range.first; it != range.second; ++it) {
+        const int jcol = it->second, dcol = C.pdim[jcol];
+        auto f = a.blocks.find(i * a_stride + jcol);
+        if (f != a.blocks.end()) {
+          nz = true;
+          for (long r = 0; r < R; ++r)
+            for (long c = 0; c < dcol; ++c) blk[r * Dc + col + c] = (*f->second)[r * dcol + c];
+        }
+        col += dcol;
+      }
+      if (nz) b.blocks[i * b_stride + j] = std::make_shared<Block>(blk);
+    }
+  return b;
+}
+
+// Contract by symbols (QSTCONTRACT.h:61-89): contract, then permute into the requested symbol order
+QST contract_symbols(const QST& a, const std::vector<int>& sa, const QST& b, const std::vector<int>& sb,
+                     const std::vector<int>& sc) {
+  std::vector<int> ca, cb, axb;
+  for (size_t i = 0; i < sa.size(); ++i) {
+    auto it = std::find(sb.begin(), sb.end(), sa[i]);
+    if (it != sb.end()) {
+      ca.push_back((int)i);
+      cb.push_back((int)(it - sb.begin()));
+    } else {
+      axb.push_back(sa[i]);
+    }
+  }
+  for (int s : sb)
+    if (std::find(sa.begin(), sa.end(), s) == sa.end()) axb.push_back(s);
+  QST c;
+  contract(1.0, a, ca, b, cb, 1.0, c);
+  if (axb == sc) return c;
+  std::vector<int> perm;
+  for (int s : sc) perm.push_back((int)(std::find(axb.begin(), axb.end(), s) - axb.begin()));
+  return permute(c, perm);
+}
+
+// compute_merged_operator for one group (src/optimize.hpp:296-368)
+QST merged_operator(bool fwd, const QST& mpo, const QST& opr) {
+  if (fwd) {
+    MergeInfo Mb = merge_info({opr.qs[0], mpo.qs[1]}, {opr.ds[0], mpo.ds[1]});
+    MergeInfo Mk = merge_info({opr.qs[2], mpo.qs[2]}, {opr.ds[2], mpo.ds[2]});
+    QST l = contract_symbols(opr, {0, 1, 2}, mpo, {1, 3, 4, 5}, {0, 3, 5, 2, 4});
+    QST s = merge_rows(Mb, 2, l);
+    return merge_cols(s, Mk, 2);
+  }
+  MergeInfo Mb = merge_info({mpo.qs[1], opr.qs[0]}, {mpo.ds[1], opr.ds[0]});
+  MergeInfo Mk = merge_info({mpo.qs[2], opr.qs[2]}, {mpo.ds[1], opr.ds[0]});
+  QST r = contract_symbols(mpo, {0, 1, 2, 3}, opr, {4, 3, 5}, {1, 4, 0, 2, 5});
+  QST s = merge_rows(Mb, 2, r);
+  return merge_cols(s, Mk, 2);
+}
+
+// ---- QN-blocked SVD with truncation (QSTLAPACK.h:122-288) and two-site canonicalize (canonicalize.hpp:27-46) ------
+
+// One-sided Jacobi SVD of a row-major m x n matrix: U (m x k), s (k, descending), Vt (k x n), k = min(m, n).
+// (The reference calls LAPACKE_dgesvd('S','S'); singular values agree to rounding, vectors up to sign/rotation in
+// degenerate subspaces.)
+void jacobi_svd(int m, int n, const double* A, std::vector<double>& U, std::vector<double>& s, std::vector<double>& Vt) {
+  const bool tr = m < n;
+  const int R = tr ? n : m, C = tr ? m : n;  // work on W (R x C), R >= C, columns orthogonalised
+  std::vector<double> W((size_t)R * C), V((size_t)C * C, 0.0);
+  for (int i = 0; i < R; ++i)
+    for (int j = 0; j < C; ++j) W[(size_t)i * C + j] = tr ? A[(size_t)j * n + i] : A[(size_t)i * n + j];
+  for (int j = 0; j < C; ++j) V[(size_t)j * C + j] = 1.0;
+  for (int sweep = 0; sweep < 80; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < C; ++p)
+      for (int q = p + 1; q < C; ++q) {
+        double a = 0, b = 0, g = 0;
+        for (int i = 0; i < R; ++i) {
+          const double x = W[(size_t)i * C + p], y = W[(size_t)i * C + q];
+          a += x * x;
+          b += y * y;
+          g += x * y;
+        }
+        if (std::fabs(g) <= 1e-15 * std::sqrt(a * b) || g == 0.0) continue;
+        off = std::max(off, std::fabs(g) / std::sqrt(a * b));
+        const double zeta = (b - a) / (2.0 * g);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = c * t;
+        for (int i = 0; i < R; ++i) {
+          const double x = W[(size_t)i * C + p], y = W[(size_t)i * C + q];
+          W[(size_t)i * C + p] = c * x - sn * y;
+          W[(size_t)i * C + q] = sn * x + c * y;
+        }
+        for (int i = 0; i < C; ++i) {
+          const double x = V[(size_t)i * C + p], y = V[(size_t)i * C + q];
+          V[(size_t)i * C + p] = c * x - sn * y;
+          V[(size_t)i * C + q] = sn * x + c * y;
+        }
+      }
+    if (off < 1e-15) break;
+  }
+  std::vector<double> sv(C);
+  for (int j = 0; j < C; ++j) {
+    double a = 0;
+    for (int i = 0; i < R; ++i) a += W[(size_t)i * C + j] * W[(size_t)i * C + j];
+    sv[j] = std::sqrt(a);
+  }
+  std::vector<int> ord(C);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return sv[x] > sv[y]; });
+  const int k = C;
+  s.resize(k);
+  // left vectors of W: columns / s ; right vectors: columns of V
+  std::vector<double> Lw((size_t)R * k, 0.0), Rw((size_t)C * k);
+  for (int j = 0; j < k; ++j) {
+    const int o = ord[j];
+    s[j] = sv[o];
+    for (int i = 0; i < R; ++i) Lw[(size_t)i * k + j] = sv[o] > 0 ? W[(size_t)i * C + o] / sv[o] : 0.0;
+    for (int i = 0; i < C; ++i) Rw[(size_t)i * k + j] = V[(size_t)i * C + o];
+  }
+  if (!tr) {  // A = Lw s Rw^T
+    U = Lw;
+    Vt.assign((size_t)k * n, 0.0);
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < n; ++i) Vt[(size_t)j * n + i] = Rw[(size_t)i * k + j];
+  } else {    // A^T = Lw s Rw^T -> A = Rw s Lw^T
+    U = Rw;
+    Vt.assign((size_t)k * n, 0.0);
+    for (int j = 0; j < k; ++j)
+      for (int i = 0; i < n; ++i) Vt[(size_t)j * n + i] = Lw[(size_t)i * k + j];
+  }
+}
+
+// QSTexpand of the rows of a (R, s) matrix into (l, n, s) and of the columns of (s, C) into (s, n, r)
+QST expand_rows(const MergeInfo& Mr, const std::vector<std::vector<QN>>& qs, const std::vector<std::vector<int>>& ds,
+                const QST& a) {
+  QST b;
+  b.set_shape(3, true, a.q, {qs[0], qs[1], a.qs[1]}, {ds[0], ds[1], a.ds[1]});
+  for (const auto& kv : a.blocks) {
+    std::vector<int> ai = a.index(kv.first);
+    const long ncol = a.ds[1][ai[1]];
+    long row = 0;
+    auto range = Mr.m2p.equal_range(ai[0]);
+    for (auto it = range.first; it != range.second; ++it) {
+      const int p = it->second, dr = Mr.pdim[p];
+      if (dr == 0) continue;
+      std::vector<int> bi{p / (int)qs[1].size(), p % (int)qs[1].size(), ai[1]};
+      Block* blk = b.reserve(b.tag(bi));
+      if (blk) std::copy(kv.second->begin() + row * ncol, kv.second->begin() + (row + dr) * ncol, blk->begin());
+      row += dr;
+    }
+  }
+  return b;
+}
+
+QST expand_cols(const QST& a, const MergeInfo& Mc, const std::vector<std::vector<QN>>& qs,
+                const std::vector<std::vector<int>>& ds) {
+  QST b;
+  b.set_shape(3, true, a.q, {a.qs[0], qs[0], qs[1]}, {a.ds[0], ds[0], ds[1]});
+  for (const auto& kv : a.blocks) {
+    std::vector<int> ai = a.index(kv.first);
+    const long nrow = a.ds[0][ai[0]], Dc = a.ds[1][ai[1]];
+    long col = 0;
+    auto range = Mc.m2p.equal_range(ai[1]);
+    for (auto it = range.first; it != range.second; ++it) {
+      const int p = it->second, dc = Mc.pdim[p];
+      if (dc == 0) continue;
+      std::vector<int> bi{ai[0], p / (int)qs[1].size(), p % (int)qs[1].size()};
+      Block* blk = b.reserve(b.tag(bi));
+      if (blk)
+        for (long r = 0; r < nrow; ++r)
+          for (long c = 0; c < dc; ++c) (*blk)[r * dc + c] = (*kv.second)[r * Dc + col + c];
+      col += dc;
+    }
+  }
+  return b;
+}
+
+// Tie rule at the truncation cutoff: values within a relative 1e-12 of the M-th largest count as equal to it (so
+// exact degeneracies split by rounding are kept together on every backend).
+double truncation_cutoff(std::vector<double> all, int M) {
+  std::sort(all.rbegin(), all.rend());
+  double cutoff = 1.0e-16;
+  if (M > 0 && M < (int)all.size()) cutoff = all[M - 1] * (1.0 - 1e-12);
+  return cutoff;
+}
+
+double canonicalize_twosite(bool forward, const QST& wfnc, QST& lmps, QST& rmps, int M) {
+  MergeInfo Mr = merge_info({wfnc.qs[0], wfnc.qs[1]}, {wfnc.ds[0], wfnc.ds[1]});
+  MergeInfo Mc = merge_info({wfnc.qs[2], wfnc.qs[3]}, {wfnc.ds[2], wfnc.ds[3]});
+  QST a = merge_cols(merge_rows(Mr, 2, wfnc), Mc, 2);
+  struct Dec { int i, j, m, n; std::vector<double> U, s, Vt; };
+  std::vector<Dec> dec;
+  for (const auto& kv : a.blocks) {
+    std::vector<int> ai = a.index(kv.first);
+    Dec d;
+    d.i = ai[0];
+    d.j = ai[1];
+    d.m = a.ds[0][ai[0]];
+    d.n = a.ds[1][ai[1]];
+    jacobi_svd(d.m, d.n, kv.second->data(), d.U, d.s, d.Vt);
+    dec.push_back(std::move(d));
+  }
+  std::vector<double> all;
+  for (auto& d : dec) all.insert(all.end(), d.s.begin(), d.s.end());
+  const double cutoff = truncation_cutoff(all, M);
+  double dnorm = 0.0;
+  // kept sectors in sector order (rows for LeftArrow, columns for RightArrow)
+  const int nsec = forward ? a.nsec(0) : a.nsec(1);
+  std::vector<QN> qnz;
+  std::vector<int> dnz;
+  std::vector<int> secmap(dec.size(), -1), kept(dec.size(), 0);
+  for (size_t k = 0; k < dec.size(); ++k) {
+    int r = 0;
+    while (r < (int)dec[k].s.size() && dec[k].s[r] >= cutoff) ++r;
+    kept[k] = r;
+    for (int t = r; t < (int)dec[k].s.size(); ++t) dnorm += dec[k].s[t] * dec[k].s[t];
+  }
+  for (int q = 0; q < nsec; ++q)
+    for (size_t k = 0; k < dec.size(); ++k)
+      if ((forward ? dec[k].i : dec[k].j) == q && kept[k] > 0) {
+        secmap[k] = (int)qnz.size();
+        qnz.push_back(forward ? a.qs[0][q] : -a.qs[1][q]);
+        dnz.push_back(kept[k]);
+      }
+  std::vector<QN> nq;
+  for (const QN& q : qnz) nq.push_back(-q);
+  QST um, vm;
+  um.set_shape(2, true, forward ? QN() : a.q, {a.qs[0], nq}, {a.ds[0], dnz});
+  vm.set_shape(2, true, forward ? a.q : QN(), {qnz, a.qs[1]}, {dnz, a.ds[1]});
+  for (size_t k = 0; k < dec.size(); ++k) {
+    if (secmap[k] < 0) continue;
+    const Dec& d = dec[k];
+    const int kk = (int)d.s.size(), r = kept[k];
+    Block* ub = um.reserve(um.tag({d.i, secmap[k]}));
+    Block* vb = vm.reserve(vm.tag({secmap[k], d.j}));
+    for (int i = 0; i < d.m; ++i)
+      for (int c = 0; c < r; ++c) (*ub)[(size_t)i * r + c] = d.U[(size_t)i * kk + c] * (forward ? 1.0 : d.s[c]);
+    for (int c = 0; c < r; ++c)
+      for (int j = 0; j < d.n; ++j) (*vb)[(size_t)c * d.n + j] = d.Vt[(size_t)c * d.n + j] * (forward ? d.s[c] : 1.0);
+  }
+  lmps = expand_rows(Mr, {wfnc.qs[0], wfnc.qs[1]}, {wfnc.ds[0], wfnc.ds[1]}, um);
+  rmps = expand_cols(vm, Mc, {wfnc.qs[2], wfnc.qs[3]}, {wfnc.ds[2], wfnc.ds[3]});
+  return dnorm;
+}
+
 // ---- hot path compositions ---------------------------------------------------------------------------------------
 
 void sigma_twosite(const QST& lmpo, const QST& rmpo, const QST& lopr, const QST& ropr, const QST& wfn, QST& sgv) {
@@ -594,6 +938,74 @@ double davidson(Functor2& multH, const QST& diag, QST& wfnc, double tolerance) {
   return eval;
 }
 
+// ---- two-site sweeps (dmrg/make_sweep.cpp:12-254; initial right-canonicalisation as make_random_mpss.cpp:204-251) ---
+
+struct Sweeper {
+  int L;
+  std::vector<QST> mpo, mps, lenv, renv;
+  int nsigma = 0;
+};
+
+QST boundary_env() {
+  QST e;
+  e.set_shape(3, true, QN(), {{QN()}, {QN()}, {QN()}}, {{1}, {1}, {1}});
+  e.blocks[0] = std::make_shared<Block>(1, 1.0);
+  return e;
+}
+
+void sweeper_init(Sweeper& S, int M) {
+  const int L = S.L;
+  for (int i = L - 1; i >= 1; --i) {
+    QST psi, U, B;
+    contract(1.0, S.mps[i - 1], {2}, S.mps[i], {0}, 1.0, psi);
+    canonicalize_twosite(false, psi, U, B, M);
+    normalize(U);
+    S.mps[i - 1] = U;
+    S.mps[i] = B;
+  }
+  S.lenv.assign(L + 1, QST());
+  S.renv.assign(L + 1, QST());
+  S.lenv[0] = boundary_env();
+  S.renv[L] = boundary_env();
+  for (int i = L - 1; i >= 1; --i) renormalize(false, S.mpo[i], S.renv[i + 1], S.mps[i], S.mps[i], S.renv[i]);
+}
+
+double sweeper_sweep(Sweeper& S, int M, double tol) {
+  const int L = S.L;
+  double emin = 1.0e8;
+  auto opt = [&](int i, bool fwd) {
+    QST psi;
+    contract(1.0, S.mps[i], {2}, S.mps[i + 1], {0}, 1.0, psi);
+    QST diag;
+    diag.set_shape(4, true, psi.q, psi.qs, psi.ds);
+    diag_twosite(S.mpo[i], S.mpo[i + 1], S.lenv[i], S.renv[i + 2], diag);
+    Functor2 F;
+    F.lmpo = {&S.mpo[i]};
+    F.rmpo = {&S.mpo[i + 1]};
+    F.lopr = {&S.lenv[i]};
+    F.ropr = {&S.renv[i + 2]};
+    const double e = davidson(F, diag, psi, 0.1 * tol);
+    S.nsigma += F.nsigma;
+    emin = std::min(emin, e);
+    QST U, V;
+    canonicalize_twosite(fwd, psi, U, V, M);
+    S.mps[i] = U;
+    S.mps[i + 1] = V;
+    if (fwd) {
+      QST e1;
+      renormalize(true, S.mpo[i], S.lenv[i], S.mps[i], S.mps[i], e1);
+      S.lenv[i + 1] = e1;
+    } else {
+      QST e1;
+      renormalize(false, S.mpo[i + 1], S.renv[i + 2], S.mps[i + 1], S.mps[i + 1], e1);
+      S.renv[i + 1] = e1;
+    }
+  };
+  for (int i = 0; i < L - 1; ++i) opt(i, true);
+  for (int i = L - 2; i >= 0; --i) opt(i, false);
+  return emin;
+}
+
 }  // namespace orc
 
 // ---- C interface (ctypes) -------------------------------------------------------------------------------------------
@@ -733,6 +1145,18 @@ int orc_tie(orc_tensor* x, const int32_t* t, int32_t K, orc_tensor* y) {
   ORC_GUARD(y->a = orc::tie(x->a, std::vector<int>(t, t + K)))
 }
 int orc_copy_upcast(orc_tensor* x, orc_tensor* y) { ORC_GUARD(orc::copy_upcast(x->a, y->a)) }
+int orc_merged_operator(int fwd, orc_tensor* mpo, orc_tensor* opr, orc_tensor* out) {
+  ORC_GUARD(out->a = orc::merged_operator(fwd != 0, mpo->a, opr->a))
+}
+int orc_qst_merge(orc_tensor* x, int nr, int nc, orc_tensor* y) {
+  ORC_GUARD({
+    const orc::QST& a = x->a;
+    std::vector<std::vector<orc::QN>> rq(a.qs.begin(), a.qs.begin() + nr), cq(a.qs.end() - nc, a.qs.end());
+    std::vector<std::vector<int>> rd(a.ds.begin(), a.ds.begin() + nr), cd(a.ds.end() - nc, a.ds.end());
+    orc::QST r = nr > 1 ? orc::merge_rows(orc::merge_info(rq, rd), nr, a) : a;
+    y->a = nc > 1 ? orc::merge_cols(r, orc::merge_info(cq, cd), nc) : r;
+  })
+}
 double orc_dotc(orc_tensor* x, orc_tensor* y) { return orc::dotc(x->a, y->a); }
 int orc_axpy(double alpha, orc_tensor* x, orc_tensor* y) { ORC_GUARD(orc::axpy(alpha, x->a, y->a)) }
 
@@ -749,6 +1173,26 @@ int orc_davidson_twosite(int ngroups, orc_tensor** lmpo, orc_tensor** rmpo, orc_
     }
     *energy = orc::davidson(F, diag->a, wfnc->a, tol);
     *nsigma = F.nsigma;
+  })
+}
+
+int orc_canonicalize_twosite(int fwd, orc_tensor* wfnc, orc_tensor* lmps, orc_tensor* rmps, int M, double* dnorm) {
+  ORC_GUARD(*dnorm = orc::canonicalize_twosite(fwd != 0, wfnc->a, lmps->a, rmps->a, M))
+}
+
+// Full two-site DMRG on the CPU: nsweeps sweeps at bond dimension Ms[k]; energies[k] = sweep energy.
+int orc_dmrg(int L, orc_tensor** mpo, orc_tensor** init, int nsweeps, const int32_t* Ms, double tol, double* energies,
+             int32_t* nsigma) {
+  ORC_GUARD({
+    orc::Sweeper S;
+    S.L = L;
+    for (int i = 0; i < L; ++i) {
+      S.mpo.push_back(mpo[i]->a);
+      S.mps.push_back(orc::deep_copy(init[i]->a));
+    }
+    orc::sweeper_init(S, Ms[0]);
+    for (int k = 0; k < nsweeps; ++k) energies[k] = orc::sweeper_sweep(S, Ms[k], tol);
+    *nsigma = S.nsigma;
   })
 }
 

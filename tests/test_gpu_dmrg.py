@@ -1,0 +1,89 @@
+"""Device canonicalization, merged operators and two-site sweeps vs the oracle (needs an MI355X)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import ed
+import mpsxx_amd as mx
+import oracle
+from mpsxx_amd import mps
+from mpsxx_amd.dmrg import DeviceDMRG, initial_mps
+from mpsxx_amd.mpo import hubbard_mpo
+
+pytestmark = pytest.mark.gpu
+
+
+def _canon(forward, psi, M):
+    U, V = mx.DTensor(), mx.DTensor()
+    d = ctypes.c_double()
+    mx._check(mx.lib().mpsxx_canonicalize_twosite(ctypes.c_int(1 if forward else 0), mx.DTensor(psi).h, U.h, V.h,
+                                                  ctypes.c_int(M), ctypes.byref(d), None))
+    return U.to_qst(), V.to_qst(), d.value
+
+
+@pytest.mark.parametrize("forward,M", [(True, 0), (False, 0), (True, 12), (False, 12)])
+def test_canonicalize_matches_oracle(forward, M):
+    print("canonicalize", forward, M, flush=True)
+    rng = np.random.default_rng(5)
+    ch = mps.random_chain(8, 3, (8, 0), rng, dmax=5)
+    psi = ch["psi"]
+    U, V, dw = _canon(forward, psi, M)
+    Ur, Vr, dwr = oracle.canonicalize_twosite(forward, psi, M)
+    # same kept sectors and dimensions, same discarded weight, same reconstructed (truncated) wavefunction
+    assert U.qshape == Ur.qshape and U.dshape == Ur.dshape and U.block_set() == Ur.block_set()
+    assert V.qshape == Vr.qshape and V.dshape == Vr.dshape and V.block_set() == Vr.block_set()
+    assert abs(dw - dwr) < 1e-12 * max(1.0, psi.norm() ** 2)
+    rec = oracle.contract(1.0, U, [2], V, [0])
+    recr = oracle.contract(1.0, Ur, [2], Vr, [0])
+    scale = max(float(np.max(np.abs(b))) for b in psi.blocks.values())
+    assert rec.max_abs_diff(recr) < 1e-11 * scale
+
+
+def test_merged_operator_matches_oracle():
+    rng = np.random.default_rng(6)
+    L, i = 8, 3
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    qt = (L, 0)
+    lq = mps.left_bond_qns(L, i, qt)
+    ld = [int(rng.integers(1, 8)) for _ in lq]
+    from helpers import random_env_left, random_env_right
+    le = random_env_left(lq, ld, mpos[i], rng)
+    lib = mx.lib()
+    out = mx.DTensor()
+    mx._check(lib.mpsxx_compute_merged_operator(ctypes.c_int(1), mx.DTensor(mpos[i]).h, mx.DTensor(le).h, out.h, None))
+    got = out.to_qst()
+    o = oracle.OTensor()
+    oracle._run(oracle.lib().orc_merged_operator, ctypes.c_int(1), oracle.OTensor(mpos[i]).h, oracle.OTensor(le).h, o.h)
+    ref = o.to_qst()
+    assert got.qshape == ref.qshape and got.dshape == ref.dshape and got.block_set() == ref.block_set()
+    assert got.max_abs_diff(ref) < 1e-13
+    rq = mps.right_bond_qns(L, i + 1, qt)
+    rd = [int(rng.integers(1, 8)) for _ in rq]
+    re_ = random_env_right(rq, rd, mpos[i], rng)
+    out2 = mx.DTensor()
+    mx._check(lib.mpsxx_compute_merged_operator(ctypes.c_int(0), mx.DTensor(mpos[i]).h, mx.DTensor(re_).h, out2.h, None))
+    got2 = out2.to_qst()
+    o2 = oracle.OTensor()
+    oracle._run(oracle.lib().orc_merged_operator, ctypes.c_int(0), oracle.OTensor(mpos[i]).h, oracle.OTensor(re_).h, o2.h)
+    ref2 = o2.to_qst()
+    assert got2.qshape == ref2.qshape and got2.block_set() == ref2.block_set()
+    assert got2.max_abs_diff(ref2) < 1e-13
+
+
+def test_device_dmrg_config1_matches_oracle_and_ed():
+    """Config 1: L=8, t=1, U=4, N=8, Sz=0, M=64, 4 sweeps, tol 1e-10; |dE| GPU vs CPU <= 1e-10."""
+    print("dmrg config1", flush=True)
+    L, M = 8, 64
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    init = initial_mps(L, (L, 0), M, 0)
+    dm = DeviceDMRG(mpos, init, M)
+    eg = []
+    for _ in range(4):
+        e, st = dm.sweep(M, 1e-10)
+        eg.append(e)
+    er, _ = oracle.dmrg(mpos, init, [M] * 4, 1e-10)
+    h, v = ed.hubbard_integrals(L, 1.0, 4.0)
+    e0 = ed.ground_energy(h, v, L, L, 0)
+    assert abs(eg[-1] - er[-1]) <= 1e-10, (eg, er)
+    assert abs(eg[-1] - e0) <= 1e-9, (eg, e0)
