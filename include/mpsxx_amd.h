@@ -164,6 +164,8 @@ typedef struct mpsxx_sweep_stats {
   double davidson_s, svd_s, renorm_s, diag_s;
   int32_t nsigma, max_bond;
   double discarded_weight;
+  int32_t nsigma_fp32;     /* sigma vectors applied in FP32 (precision MPSXX_FP32: FP32 phase + FP64 refinement)    */
+  int32_t pad;
 } mpsxx_sweep_stats;
 /* init_mps: site tensors A(l,n,r) with total QN 0 except the last site, which carries the target (N, 2Sz); the driver
  * right-canonicalizes them (truncating to max_states) and builds every right environment on the device. */

@@ -145,17 +145,20 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   auto axpy = [&](double a, const double* x, double* y) { launch_axpby(a, x, 1.0, y, n, s); };
 
   std::vector<Plan*> plans;
-  for (const DavGroup& g : groups)
-    plans.push_back(&plan_sigma_twosite(g.lmpo->L, g.rmpo->L, g.lopr->L, g.ropr->L, U, &U, prec));
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "event");
   hip_check(hipEventCreate(&e1), "event");
+  // FP32 plans convert the environments and MPO tensors once per Davidson call (they do not change inside it) and
+  // the trial vector on every application
+  unsigned convert_mask = ~0u;
   auto multH = [&](const double* c, double* v) {
     hip_check(hipEventRecord(e0, s), "event");
     hip_check(hipMemsetAsync(v, 0, n * sizeof(double), s), "memset");
     for (size_t g = 0; g < groups.size(); ++g)
-      run_plan(*plans[g], {groups[g].lmpo->ptr(), groups[g].rmpo->ptr(), groups[g].lopr->ptr(), groups[g].ropr->ptr(), c},
-               v, s);
+      run_plan_ex(*plans[g],
+                  {groups[g].lmpo->ptr(), groups[g].rmpo->ptr(), groups[g].lopr->ptr(), groups[g].ropr->ptr(), c}, v, s,
+                  {}, convert_mask);
+    convert_mask = 1u << 4;
     hip_check(hipEventRecord(e1, s), "event");
     hip_check(hipEventSynchronize(e1), "event sync");
     float ms = 0.f;
@@ -165,9 +168,15 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   };
 
   double eval = 0.0;
-  int niter = 0, iconv = 0;
   std::vector<double> G(max_ritz * max_ritz), R;
-  while (iconv < 1 && niter < 4) {
+  // one Davidson solve from the current trial_0 with the sigma arithmetic of `p`
+  auto solve = [&](int p, double tol_p) {
+    plans.clear();
+    for (const DavGroup& g : groups)
+      plans.push_back(&plan_sigma_twosite(g.lmpo->L, g.rmpo->L, g.lopr->L, g.ropr->L, U, &U, p));
+    convert_mask = ~0u;
+    int niter = 0, iconv = 0;
+    while (iconv < 1 && niter < 4) {
     normalize(T(0));
     multH(T(0), Sg(0));
     for (int m = 1; m <= max_ritz; ++m) {
@@ -200,7 +209,8 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
       hip_check(hipMemcpyAsync(errv, Sg(0), n * sizeof(double), hipMemcpyDeviceToDevice, s), "errv");
       axpy(-eval, T(0), errv);
       const double rnorm = dot(errv, errv);
-      if (rnorm < tol) {
+      st.last_rnorm = rnorm;
+      if (rnorm < tol_p) {
         ++iconv;
         break;
       }
@@ -216,6 +226,16 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
       }
     }
     ++niter;
+    }
+  };
+  if (prec == MPSXX_FP32) {
+    // mixed precision: FP32 sigma vectors (FP32 MFMA, FP32 accumulation inside each block GEMM) until the FP32 noise
+    // floor, then FP64 refinement from that vector; Gram-Schmidt, the subspace problem and residuals stay FP64
+    solve(MPSXX_FP32, std::max(tol, kMixedTolerance));
+    st.nsigma_fp32 = st.nsigma;
+    solve(MPSXX_FP64, tol);
+  } else {
+    solve(MPSXX_FP64, tol);
   }
   // wfnc <- normalized trial_0 on layout U (Functor::orthonormalize with no lower roots)
   normalize(T(0));

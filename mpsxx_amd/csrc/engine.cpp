@@ -533,13 +533,19 @@ void run_plan(Plan& P, const std::vector<const double*>& inputs, double* out, hi
 
 void run_plan_timed(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
                     const std::vector<hipEvent_t>& events) {
+  run_plan_ex(P, inputs, out, s, events, ~0u);
+}
+
+void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
+                 const std::vector<hipEvent_t>& events, unsigned convert_mask) {
   const int ns = (int)P.slot_layout.size();
   std::vector<void*> ptr(ns, nullptr);
   for (int i = 0; i < ns; ++i) {
     if (P.slot_input[i] >= 0) {
       const double* src = inputs[P.slot_input[i]];
       if (P.slot_prec[i] == 1) {
-        launch_convert<double, float>(src, P.slot_buf[i].as<float>(), P.slot_layout[i].nelem, s);
+        if (convert_mask & (1u << P.slot_input[i]))
+          launch_convert<double, float>(src, P.slot_buf[i].as<float>(), P.slot_layout[i].nelem, s);
         ptr[i] = P.slot_buf[i].p;
       } else {
         ptr[i] = const_cast<double*>(src);

@@ -110,6 +110,10 @@ void run_plan(Plan& P, const std::vector<const double*>& inputs, double* out, hi
 // Same, recording an event before each step and after the last one (events.size() >= steps + 1).
 void run_plan_timed(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
                     const std::vector<hipEvent_t>& events);
+// Full form: FP32 plans convert only the inputs whose bit is set in convert_mask (the others reuse the FP32 copy
+// made by an earlier call with the same input contents).
+void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
+                 const std::vector<hipEvent_t>& events, unsigned convert_mask);
 
 // Workspace pool for FP32 input copies (per plan slot, grown on demand).
 const char* kind_name(int k);

@@ -76,8 +76,12 @@ struct DavGroup {
 };
 struct DavidsonStats {
   int nsigma = 0;
+  int nsigma_fp32 = 0;    // sigma vectors applied in FP32 (mixed precision)
   double sigma_ms = 0.0;  // device time spent in sigma vectors
+  double last_rnorm = 0.0;
 };
+// Residual tolerance (||r||^2) of the FP32 phase of a mixed-precision Davidson solve.
+constexpr double kMixedTolerance = 1.0e-9;
 Layout davidson_layout(const std::vector<DavGroup>& groups, const Layout& psi);
 double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tensor& wfnc, double tol, int prec,
                         DavidsonStats& st, hipStream_t s);

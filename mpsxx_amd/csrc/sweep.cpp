@@ -17,7 +17,7 @@ namespace mpx {
 
 struct SweepStats {
   double seconds = 0.0, sigma_ms = 0.0, davidson_s = 0.0, svd_s = 0.0, renorm_s = 0.0, diag_s = 0.0;
-  int nsigma = 0, max_bond = 0;
+  int nsigma = 0, max_bond = 0, nsigma_fp32 = 0;
   double dweight = 0.0;
 };
 
@@ -124,6 +124,7 @@ double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStrea
     const double e = davidson_twosite(g, diag, psi, 0.1 * tol, prec, ds, s);
     st.davidson_s += since(t);
     st.nsigma += ds.nsigma;
+    st.nsigma_fp32 += ds.nsigma_fp32;
     st.sigma_ms += ds.sigma_ms;
     if (e < emin) emin = e;
     t = clk::now();
@@ -215,6 +216,8 @@ int mpsxx_dmrg_sweep(mpsxx_dmrg d, int max_states, double tolerance, int precisi
       stats->nsigma = st.nsigma;
       stats->max_bond = st.max_bond;
       stats->discarded_weight = st.dweight;
+      stats->nsigma_fp32 = st.nsigma_fp32;
+      stats->pad = 0;
     }
   });
 }

@@ -63,7 +63,8 @@ def initial_mps(L, qt, M, seed=0):
 class SweepStats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("sigma_ms", ctypes.c_double), ("davidson_s", ctypes.c_double),
                 ("svd_s", ctypes.c_double), ("renorm_s", ctypes.c_double), ("diag_s", ctypes.c_double),
-                ("nsigma", ctypes.c_int32), ("max_bond", ctypes.c_int32), ("discarded_weight", ctypes.c_double)]
+                ("nsigma", ctypes.c_int32), ("max_bond", ctypes.c_int32), ("discarded_weight", ctypes.c_double),
+                ("nsigma_fp32", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class DeviceDMRG:
