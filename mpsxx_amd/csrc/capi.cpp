@@ -8,10 +8,6 @@
 #include "mpsxx_amd.h"
 #include "ops.hpp"
 
-struct mpsxx_tensor_s {
-  mpx::Tensor t;
-};
-
 namespace {
 
 thread_local std::string g_err;
@@ -44,6 +40,7 @@ int guard(F f) {
 
 mpx::Tensor& T(mpsxx_tensor h) {
   if (!h) throw mpx::Error(MPSXX_ERR_ARG, "null tensor handle");
+  if (h->magic != mpsxx_tensor_s::kMagic) throw mpx::Error(MPSXX_ERR_ARG, "invalid or freed tensor handle");
   return h->t;
 }
 

@@ -115,7 +115,14 @@ void run_plan_timed(Plan& P, const std::vector<const double*>& inputs, double* o
 void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
                  const std::vector<hipEvent_t>& events, unsigned convert_mask);
 
-// Workspace pool for FP32 input copies (per plan slot, grown on demand).
 const char* kind_name(int k);
 
 }  // namespace mpx
+
+// Opaque handle behind the C ABI's mpsxx_tensor; the tag lets entry points reject freed or foreign handles.
+struct mpsxx_tensor_s {
+  static constexpr uint64_t kMagic = 0x4d50535854454e53ull;
+  uint64_t magic = kMagic;
+  mpx::Tensor t;
+  ~mpsxx_tensor_s() { magic = 0; }
+};

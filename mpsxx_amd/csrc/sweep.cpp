@@ -157,10 +157,6 @@ void dmrg_free(Dmrg* D) { delete D; }
 
 }  // namespace mpx
 
-struct mpsxx_tensor_s {
-  mpx::Tensor t;
-};
-
 struct mpsxx_dmrg_s {
   mpx::Dmrg* d = nullptr;
 };

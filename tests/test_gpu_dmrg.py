@@ -15,9 +15,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _canon(forward, psi, M):
-    U, V = mx.DTensor(), mx.DTensor()
+    U, V, P = mx.DTensor(), mx.DTensor(), mx.DTensor(psi)  # handles must outlive the call
     d = ctypes.c_double()
-    mx._check(mx.lib().mpsxx_canonicalize_twosite(ctypes.c_int(1 if forward else 0), mx.DTensor(psi).h, U.h, V.h,
+    mx._check(mx.lib().mpsxx_canonicalize_twosite(ctypes.c_int(1 if forward else 0), P.h, U.h, V.h,
                                                   ctypes.c_int(M), ctypes.byref(d), None))
     return U.to_qst(), V.to_qst(), d.value
 
@@ -50,22 +50,22 @@ def test_merged_operator_matches_oracle():
     from helpers import random_env_left, random_env_right
     le = random_env_left(lq, ld, mpos[i], rng)
     lib = mx.lib()
-    out = mx.DTensor()
-    mx._check(lib.mpsxx_compute_merged_operator(ctypes.c_int(1), mx.DTensor(mpos[i]).h, mx.DTensor(le).h, out.h, None))
+    out, dw, dl = mx.DTensor(), mx.DTensor(mpos[i]), mx.DTensor(le)
+    mx._check(lib.mpsxx_compute_merged_operator(ctypes.c_int(1), dw.h, dl.h, out.h, None))
     got = out.to_qst()
-    o = oracle.OTensor()
-    oracle._run(oracle.lib().orc_merged_operator, ctypes.c_int(1), oracle.OTensor(mpos[i]).h, oracle.OTensor(le).h, o.h)
+    o, ow, ol = oracle.OTensor(), oracle.OTensor(mpos[i]), oracle.OTensor(le)
+    oracle._run(oracle.lib().orc_merged_operator, ctypes.c_int(1), ow.h, ol.h, o.h)
     ref = o.to_qst()
     assert got.qshape == ref.qshape and got.dshape == ref.dshape and got.block_set() == ref.block_set()
     assert got.max_abs_diff(ref) < 1e-13
     rq = mps.right_bond_qns(L, i + 1, qt)
     rd = [int(rng.integers(1, 8)) for _ in rq]
     re_ = random_env_right(rq, rd, mpos[i], rng)
-    out2 = mx.DTensor()
-    mx._check(lib.mpsxx_compute_merged_operator(ctypes.c_int(0), mx.DTensor(mpos[i]).h, mx.DTensor(re_).h, out2.h, None))
+    out2, dr = mx.DTensor(), mx.DTensor(re_)
+    mx._check(lib.mpsxx_compute_merged_operator(ctypes.c_int(0), dw.h, dr.h, out2.h, None))
     got2 = out2.to_qst()
-    o2 = oracle.OTensor()
-    oracle._run(oracle.lib().orc_merged_operator, ctypes.c_int(0), oracle.OTensor(mpos[i]).h, oracle.OTensor(re_).h, o2.h)
+    o2, orr = oracle.OTensor(), oracle.OTensor(re_)
+    oracle._run(oracle.lib().orc_merged_operator, ctypes.c_int(0), ow.h, orr.h, o2.h)
     ref2 = o2.to_qst()
     assert got2.qshape == ref2.qshape and got2.block_set() == ref2.block_set()
     assert got2.max_abs_diff(ref2) < 1e-13
