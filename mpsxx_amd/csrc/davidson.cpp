@@ -168,7 +168,7 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   };
 
   double eval = 0.0;
-  std::vector<double> G(max_ritz * max_ritz), R;
+  std::vector<double> G(max_ritz * max_ritz), R, coef_h(max_ritz * max_ritz, 0.0);
   // one Davidson solve from the current trial_0 with the sigma arithmetic of `p`
   auto solve = [&](int p, double tol_p) {
     plans.clear();
@@ -190,14 +190,13 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
       std::vector<double> rval;
       sym_eig(m, heff, rval, R);
       eval = rval[0];
-      // rotate: trial_j <- sum_i rvec(i,j) trial_i (and sigma), diagonalize.hpp:378-394
-      for (int j = 0; j < m; ++j) {
-        std::vector<double> c(m);
-        for (int i = 0; i < m; ++i) c[i] = R[i * m + j];
-        hip_check(hipMemcpyAsync(coef.as<double>() + j * max_ritz, c.data(), m * sizeof(double),
-                                 hipMemcpyHostToDevice, s),
-                  "coef");
-      }
+      // rotate: trial_j <- sum_i rvec(i,j) trial_i (and sigma), diagonalize.hpp:378-394.  The host staging array
+      // outlives the asynchronous copy (the next write to it follows several stream synchronisations).
+      for (int j = 0; j < m; ++j)
+        for (int i = 0; i < m; ++i) coef_h[j * max_ritz + i] = R[i * m + j];
+      hip_check(hipMemcpyAsync(coef.as<double>(), coef_h.data(), max_ritz * max_ritz * sizeof(double),
+                               hipMemcpyHostToDevice, s),
+                "coef");
       // stage the rotated vectors in T2/S2, then copy back
       for (int j = 0; j < m; ++j) {
         launch_combine(T(0), n, m, coef.as<double>() + j * max_ritz, T2 + (int64_t)j * n, n, s);

@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <unordered_map>
@@ -499,13 +500,27 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
     cost[ti] = ksum;
   }
   std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  // blocks with both sides <= small_limit go to the one-wave direct-load kernel in 32x32 tiles, the rest to the
+  // LDS-staged 64x64 kernel
+  static const int small_limit = [] {
+    const char* e = std::getenv("MPSXX_SMALL_GEMM_LIMIT");
+    return e ? std::atoi(e) : 64;
+  }();
+  std::vector<GemmTile> small;
   for (size_t ti : order) {
     const GemmTask& t = sym.tasks[ti];
+    if (t.m <= small_limit && t.n <= small_limit) {
+      for (int r = 0; r < t.m; r += kSmallTile)
+        for (int c = 0; c < t.n; c += kSmallTile) small.push_back({(int32_t)ti, r, c, 0});
+      continue;
+    }
     for (int r = 0; r < t.m; r += kGemmTile)
       for (int c = 0; c < t.n; c += kGemmTile) tiles.push_back({(int32_t)ti, r, c, 0});
   }
   st.nwork = (int)tiles.size();
   st.work = upload_vec(tiles);
+  st.nwork_small = (int)small.size();
+  st.work_small = upload_vec(small);
   st.tasks = upload_vec(sym.tasks);
   st.pairs = upload_vec(sym.pairs);
   return st;
@@ -563,18 +578,28 @@ void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out,
     const bool cf32 = P.slot_prec[st.c] == 1;
     switch (st.kind) {
       case Step::GEMM:
-        if (!f32)
+        if (!f32) {
           launch_gemm<double, double>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
                                       st.pairs.as<GemmPair>(), (const double*)ptr[st.a], (const double*)ptr[st.b],
                                       (double*)ptr[st.c], st.alpha, st.accumulate, s);
-        else if (cf32)
+          launch_gemm_small<double, double>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
+                                            st.pairs.as<GemmPair>(), (const double*)ptr[st.a],
+                                            (const double*)ptr[st.b], (double*)ptr[st.c], st.alpha, st.accumulate, s);
+        } else if (cf32) {
           launch_gemm<float, float>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
                                     st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
                                     (float*)ptr[st.c], st.alpha, st.accumulate, s);
-        else
+          launch_gemm_small<float, float>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
+                                          st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
+                                          (float*)ptr[st.c], st.alpha, st.accumulate, s);
+        } else {
           launch_gemm<float, double>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
                                      st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
                                      (double*)ptr[st.c], st.alpha, st.accumulate, s);
+          launch_gemm_small<float, double>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
+                                           st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
+                                           (double*)ptr[st.c], st.alpha, st.accumulate, s);
+        }
         break;
       case Step::AXPY:
         if (!f32)

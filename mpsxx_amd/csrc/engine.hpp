@@ -85,6 +85,8 @@ struct Step {
   double alpha = 1.0, beta = 0.0;
   int nwork = 0;               // tiles or chunks
   DevBuf work, tasks, pairs;   // device copies of the task lists
+  int nwork_small = 0;         // GEMM: 32x32 tiles of the small blocks (gemm_small_kernel)
+  DevBuf work_small;
   int64_t flops = 0;
 };
 

@@ -124,6 +124,71 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __res
       }
 }
 
+// Small-block variant: one wave per 32x32 C tile (2x2 MFMA 16x16 blocks), operands read straight from global memory
+// into the MFMA source registers (lane l takes A[row + l%16][k + l/16] and B[k + l/16][col + l%16]); no LDS and no
+// barriers, so thousands of tiny blocks run at full occupancy.  Four independent waves per work-group.
+template <typename T, typename TC>
+__global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restrict__ tiles, int ntiles,
+                                                         const GemmTask* __restrict__ tasks,
+                                                         const GemmPair* __restrict__ pairs, const T* __restrict__ A,
+                                                         const T* __restrict__ B, TC* __restrict__ C, double alpha,
+                                                         int accumulate) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int wave = threadIdx.x >> 6;
+  const int ti = blockIdx.x * 4 + wave;
+  if (ti >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const GemmTile tl = tiles[ti];
+  const GemmTask tk = tasks[tl.task];
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  const int r0 = tl.row0 + lr, r1 = tl.row0 + 16 + lr;
+  const int c0 = tl.col0 + lr, c1 = tl.col0 + 16 + lr;
+  const bool vr0 = r0 < tk.m, vr1 = r1 < tk.m, vc0 = c0 < tk.n, vc1 = c1 < tk.n;
+  const int pend = tk.pair_begin + tk.pair_count;
+  for (int p = tk.pair_begin; p < pend; ++p) {
+    const GemmPair pr = pairs[p];
+    const T* Ap = A + pr.a_off;
+    const T* Bp = B + pr.b_off;
+    const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
+    const T* a0p = Ap + r0 * sam;
+    const T* a1p = Ap + r1 * sam;
+    const T* b0p = Bp + c0 * sbn;
+    const T* b1p = Bp + c1 * sbn;
+    for (int k0 = 0; k0 < pr.k; k0 += 4) {
+      const int gk = k0 + lk;
+      const bool vk = gk < pr.k;
+      const int64_t ao = gk * sak, bo = gk * sbk;
+      const T a0 = (vk && vr0) ? a0p[ao] : T(0);
+      const T a1 = (vk && vr1) ? a1p[ao] : T(0);
+      const T b0 = (vk && vc0) ? b0p[bo] : T(0);
+      const T b1 = (vk && vc1) ? b1p[bo] : T(0);
+      acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+      acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+      acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+    }
+  }
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = tl.row0 + mi * 16 + 4 * lk + r;
+        const int col = tl.col0 + ni * 16 + lr;
+        if (row < tk.m && col < tk.n) {
+          const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+          const double v = alpha * (double)acc[mi][ni][r];
+          C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+        }
+      }
+}
+
 template <typename T, typename TC>
 __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restrict__ chunks,
                                                            const AxpyTask* __restrict__ tasks,
@@ -349,6 +414,20 @@ void launch_gram(const double* X, int64_t ldx, int mx, const double* Y, int64_t 
   hipLaunchKernelGGL(gram_reduce_kernel, dim3(mx * my), dim3(256), 0, s, partial, npart, G);
 }
 
+template <typename T, typename TC>
+void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
+                       const T* B, TC* C, double alpha, int accumulate, hipStream_t s) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL((gemm_small_kernel<T, TC>), dim3((ntiles + 3) / 4), dim3(256), 0, s, tiles, ntiles, tasks, pairs, A,
+                     B, C, alpha, accumulate);
+}
+
+template void launch_gemm_small<double, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const double*,
+                                                const double*, double*, double, int, hipStream_t);
+template void launch_gemm_small<float, float>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
+                                              const float*, float*, double, int, hipStream_t);
+template void launch_gemm_small<float, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
+                                               const float*, double*, double, int, hipStream_t);
 template void launch_gemm<double, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const double*,
                                           const double*, double*, double, int, hipStream_t);
 template void launch_gemm<float, float>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,

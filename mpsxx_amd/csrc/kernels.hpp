@@ -66,6 +66,12 @@ template <typename T, typename TC>
 void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
                  const T* B, TC* C, double alpha, int accumulate, hipStream_t s);
 
+// 32x32 tiles, one wave each, operands loaded straight into the MFMA registers (blocks up to ~64 on a side).
+constexpr int kSmallTile = 32;
+template <typename T, typename TC>
+void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
+                       const T* B, TC* C, double alpha, int accumulate, hipStream_t s);
+
 template <typename T, typename TC>
 void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, const T* V,
                  const T* S, TC* C, double alpha, int accumulate, hipStream_t s);
