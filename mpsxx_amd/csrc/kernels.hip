@@ -13,18 +13,23 @@ namespace mpx {
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
+// Operand layout of both 16x16x4 instructions: lane l supplies A[l%16][l/16] and B[l/16][l%16].
+// Result layout (measured on gfx950): FP32 lane l, register r holds D[4*(l/16) + r][l%16];
+//                                     FP64 lane l, register r holds D[(l/16) + 4*r][l%16].
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
   typedef dbl4 acc_t;
   static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
 };
 template <> struct Mfma<float> {
   typedef flt4 acc_t;
   static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
 constexpr int BM = 64, BN = 64, BK = 16, LDP = 64 + 1;
@@ -114,7 +119,7 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __res
     for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = tl.row0 + wr * 32 + mi * 16 + 4 * lk + r;
+        const int row = tl.row0 + wr * 32 + mi * 16 + Mfma<T>::row(lk, r);
         const int col = tl.col0 + wc * 32 + ni * 16 + lr;
         if (row < tk.m && col < tk.n) {
           const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
     for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = tl.row0 + mi * 16 + 4 * lk + r;
+        const int row = tl.row0 + mi * 16 + Mfma<T>::row(lk, r);
         const int col = tl.col0 + ni * 16 + lr;
         if (row < tk.m && col < tk.n) {
           const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
