@@ -475,6 +475,7 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
           ap.v_stride = g.sb_n;
           ap.s_off = g.a_off;
         }
+        ap.s2_off = -1;
         ap.pad = 0;
         pairs.push_back(ap);
       }
@@ -524,6 +525,65 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
   st.tasks = upload_vec(sym.tasks);
   st.pairs = upload_vec(sym.pairs);
   return st;
+}
+
+bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, int y, int w1, int w2, int z,
+                          int accumulate, Step& out) {
+  auto scalar_b = [](const ContractSym& s) {
+    for (const GemmTask& t : s.tasks) {
+      if (t.n != 1) return false;
+      for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p)
+        if (s.pairs[p].k != 1) return false;
+    }
+    return true;
+  };
+  if (!scalar_b(first) || !scalar_b(second)) return false;
+  std::unordered_map<int64_t, size_t> xtask;  // X block offset -> task of `first` that writes it
+  for (size_t i = 0; i < first.tasks.size(); ++i) xtask[first.tasks[i].c_off] = i;
+  std::vector<AxpyTask> tasks;
+  std::vector<AxpyPair> pairs;
+  std::vector<Chunk> chunks;
+  for (size_t ti = 0; ti < second.tasks.size(); ++ti) {
+    const GemmTask& t = second.tasks[ti];
+    AxpyTask at;
+    at.c_off = t.c_off;
+    at.len = t.m;
+    at.pair_begin = (int32_t)pairs.size();
+    at.pad = 0;
+    for (int q = t.pair_begin; q < t.pair_begin + t.pair_count; ++q) {
+      const GemmPair& g2 = second.pairs[q];
+      auto it = xtask.find(g2.a_off);
+      if (it == xtask.end()) return false;  // X read from inside a block: not a plain chain
+      const GemmTask& t1 = first.tasks[it->second];
+      for (int p = t1.pair_begin; p < t1.pair_begin + t1.pair_count; ++p) {
+        const GemmPair& g1 = first.pairs[p];
+        AxpyPair ap;
+        ap.v_off = g1.a_off;
+        ap.v_stride = g1.sa_m * g2.sa_m;
+        ap.s_off = g1.b_off;
+        ap.s2_off = g2.b_off;
+        ap.pad = 0;
+        pairs.push_back(ap);
+      }
+    }
+    at.pair_count = (int32_t)pairs.size() - at.pair_begin;
+    for (int s = 0; s < at.len; s += kAxpyChunk) chunks.push_back({(int32_t)ti, s});
+    tasks.push_back(at);
+  }
+  out = Step();
+  out.kind = Step::AXPY;
+  out.a = y;
+  out.b = w1;
+  out.b2 = w2;
+  out.c = z;
+  out.accumulate = accumulate;
+  out.alpha = 1.0;
+  out.flops = first.flops + second.flops;
+  out.nwork = (int)chunks.size();
+  out.work = upload_vec(chunks);
+  out.tasks = upload_vec(tasks);
+  out.pairs = upload_vec(pairs);
+  return true;
 }
 
 Step make_copy_step(const std::vector<CopySeg>& segs, int x, int y, double alpha, double beta) {
@@ -605,15 +665,18 @@ void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out,
         if (!f32)
           launch_axpy<double, double>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(),
                                       st.pairs.as<AxpyPair>(), (const double*)ptr[st.a], (const double*)ptr[st.b],
-                                      (double*)ptr[st.c], st.alpha, st.accumulate, s);
+                                      (const double*)ptr[st.b2 >= 0 ? st.b2 : st.b], (double*)ptr[st.c], st.alpha,
+                                      st.accumulate, s);
         else if (cf32)
           launch_axpy<float, float>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(), st.pairs.as<AxpyPair>(),
-                                    (const float*)ptr[st.a], (const float*)ptr[st.b], (float*)ptr[st.c], st.alpha,
+                                    (const float*)ptr[st.a], (const float*)ptr[st.b],
+                                    (const float*)ptr[st.b2 >= 0 ? st.b2 : st.b], (float*)ptr[st.c], st.alpha,
                                     st.accumulate, s);
         else
           launch_axpy<float, double>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(),
                                      st.pairs.as<AxpyPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
-                                     (double*)ptr[st.c], st.alpha, st.accumulate, s);
+                                     (const float*)ptr[st.b2 >= 0 ? st.b2 : st.b], (double*)ptr[st.c], st.alpha,
+                                     st.accumulate, s);
         break;
       case Step::COPY:
         if (!f32 && !cf32)

@@ -81,6 +81,7 @@ Layout tie_layout(const Layout& X, const std::vector<int>& tie, std::vector<Copy
 struct Step {
   enum Kind { GEMM, AXPY, COPY } kind = GEMM;
   int a = -1, b = -1, c = -1;  // slots: GEMM (A,B,C); AXPY (V,S,C); COPY (x,-,y)
+  int b2 = -1;                 // fused AXPY: slot of the second coefficient tensor
   int accumulate = 0;
   double alpha = 1.0, beta = 0.0;
   int nwork = 0;               // tiles or chunks
@@ -106,6 +107,11 @@ struct Plan {
 // Builders of device steps from symbolic results.
 Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumulate, double alpha);
 Step make_copy_step(const std::vector<CopySeg>& segs, int x, int y, double alpha, double beta);
+// Two consecutive coefficient contractions X = Contract(Y, W1), Z = Contract(X, W2) (both with a scalar B side)
+// executed as one pass Z[r] += sum W1*W2*Y[..] without materialising X; returns false (no step) when either
+// contraction is not of that form.
+bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, int y, int w1, int w2, int z,
+                          int accumulate, Step& out);
 
 // Execute a plan: inputs[i] are the FP64 device pointers of the bound inputs, out the FP64 output pointer.
 void run_plan(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s);

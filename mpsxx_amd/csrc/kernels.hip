@@ -155,6 +155,7 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
   const int c0 = tl.col0 + lr, c1 = tl.col0 + 16 + lr;
   const bool vr0 = r0 < tk.m, vr1 = r1 < tk.m, vc0 = c0 < tk.n, vc1 = c1 < tk.n;
   const int pend = tk.pair_begin + tk.pair_count;
+  constexpr int KU = 4;  // k-panel of 16: all 16 operand loads of a lane are issued before the 16 MFMAs
   for (int p = tk.pair_begin; p < pend; ++p) {
     const GemmPair pr = pairs[p];
     const T* Ap = A + pr.a_off;
@@ -164,18 +165,26 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
     const T* a1p = Ap + r1 * sam;
     const T* b0p = Bp + c0 * sbn;
     const T* b1p = Bp + c1 * sbn;
-    for (int k0 = 0; k0 < pr.k; k0 += 4) {
-      const int gk = k0 + lk;
-      const bool vk = gk < pr.k;
-      const int64_t ao = gk * sak, bo = gk * sbk;
-      const T a0 = (vk && vr0) ? a0p[ao] : T(0);
-      const T a1 = (vk && vr1) ? a1p[ao] : T(0);
-      const T b0 = (vk && vc0) ? b0p[bo] : T(0);
-      const T b1 = (vk && vc1) ? b1p[bo] : T(0);
-      acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
-      acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
-      acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
-      acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+    for (int k0 = 0; k0 < pr.k; k0 += 4 * KU) {
+      T a0[KU], a1[KU], b0[KU], b1[KU];
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        const int gk = k0 + 4 * u + lk;
+        const bool vk = gk < pr.k;
+        const int64_t ao = gk * sak, bo = gk * sbk;
+        a0[u] = (vk && vr0) ? a0p[ao] : T(0);
+        a1[u] = (vk && vr1) ? a1p[ao] : T(0);
+        b0[u] = (vk && vc0) ? b0p[bo] : T(0);
+        b1[u] = (vk && vc1) ? b1p[bo] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        if (k0 + 4 * u >= pr.k) break;  // uniform across the wave
+        acc[0][0] = Mfma<T>::op(a0[u], b0[u], acc[0][0]);
+        acc[0][1] = Mfma<T>::op(a0[u], b1[u], acc[0][1]);
+        acc[1][0] = Mfma<T>::op(a1[u], b0[u], acc[1][0]);
+        acc[1][1] = Mfma<T>::op(a1[u], b1[u], acc[1][1]);
+      }
     }
   }
 #pragma unroll
@@ -199,7 +208,8 @@ __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restri
                                                            const AxpyTask* __restrict__ tasks,
                                                            const AxpyPair* __restrict__ pairs,
                                                            const T* __restrict__ V, const T* __restrict__ S,
-                                                           TC* __restrict__ C, double alpha, int accumulate) {
+                                                           const T* __restrict__ S2, TC* __restrict__ C,
+                                                           double alpha, int accumulate) {
   const Chunk ch = chunks[blockIdx.x];
   const AxpyTask tk = tasks[ch.task];
   const int pend = tk.pair_begin + tk.pair_count;
@@ -207,7 +217,8 @@ __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restri
     T acc = T(0);
     for (int p = tk.pair_begin; p < pend; ++p) {
       const AxpyPair pr = pairs[p];
-      acc += S[pr.s_off] * V[pr.v_off + (int64_t)r * pr.v_stride];
+      const T coef = pr.s2_off >= 0 ? S[pr.s_off] * S2[pr.s2_off] : S[pr.s_off];
+      acc += coef * V[pr.v_off + (int64_t)r * pr.v_stride];
     }
     const int64_t idx = tk.c_off + r;
     const double v = alpha * (double)acc;
@@ -347,9 +358,9 @@ void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const
 
 template <typename T, typename TC>
 void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, const T* V,
-                 const T* S, TC* C, double alpha, int accumulate, hipStream_t s) {
+                 const T* S, const T* S2, TC* C, double alpha, int accumulate, hipStream_t s) {
   if (nchunks <= 0) return;
-  hipLaunchKernelGGL((axpy_grouped_kernel<T, TC>), dim3(nchunks), dim3(256), 0, s, chunks, tasks, pairs, V, S, C,
+  hipLaunchKernelGGL((axpy_grouped_kernel<T, TC>), dim3(nchunks), dim3(256), 0, s, chunks, tasks, pairs, V, S, S2, C,
                      alpha, accumulate);
 }
 
@@ -439,12 +450,12 @@ template void launch_gemm<float, float>(const GemmTile*, int, const GemmTask*, c
                                         const float*, float*, double, int, hipStream_t);
 template void launch_gemm<float, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
                                          const float*, double*, double, int, hipStream_t);
-template void launch_axpy<double, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const double*,
-                                          const double*, double*, double, int, hipStream_t);
-template void launch_axpy<float, float>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const float*,
-                                        const float*, float*, double, int, hipStream_t);
-template void launch_axpy<float, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const float*,
-                                         const float*, double*, double, int, hipStream_t);
+template void launch_axpy<double, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const double*, const double*,
+                                 const double*, double*, double, int, hipStream_t);
+template void launch_axpy<float, float>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const float*, const float*,
+                                 const float*, float*, double, int, hipStream_t);
+template void launch_axpy<float, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const float*, const float*,
+                                 const float*, double*, double, int, hipStream_t);
 template void launch_copy<double, double>(const Chunk*, int, const CopySeg*, const double*, double*, double, double,
                                           hipStream_t);
 template void launch_copy<float, float>(const Chunk*, int, const CopySeg*, const float*, float*, double, double,

@@ -35,8 +35,9 @@ struct GemmTile {
   int32_t pad;
 };
 
+// s2_off >= 0 marks a fused pair of two consecutive coefficient contractions: C[r] += S[s_off] * S2[s2_off] * V[..]
 struct AxpyPair {
-  int64_t v_off, s_off;
+  int64_t v_off, s_off, s2_off;
   int32_t v_stride;
   int32_t pad;
 };
@@ -74,7 +75,7 @@ void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks,
 
 template <typename T, typename TC>
 void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, const T* V,
-                 const T* S, TC* C, double alpha, int accumulate, hipStream_t s);
+                 const T* S, const T* S2, TC* C, double alpha, int accumulate, hipStream_t s);
 
 template <typename TX, typename TY>
 void launch_copy(const Chunk* chunks, int nchunks, const CopySeg* segs, const TX* x, TY* y, double alpha,

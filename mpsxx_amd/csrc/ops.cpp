@@ -101,6 +101,32 @@ class PlanBuilder {
     P_.bytes += sym.C.nelem * 8;
   }
 
+  // intermediate = Contract(Contract(Y[ca1], W1[cb1])[ca2], W2[cb2]) as one fused coefficient pass when both are
+  // contractions with a scalar (MPO) block; the inner intermediate's structure is still derived (it fixes the
+  // reference's block pattern) but never stored.  Falls back to two steps otherwise.
+  int contract2(int y, const std::vector<int>& ca1, int w1, const std::vector<int>& cb1, const std::vector<int>& ca2,
+                int w2, const std::vector<int>& cb2) {
+    static const bool nofuse = std::getenv("MPSXX_NO_FUSE") != nullptr;
+    View vy{&P_.slot_layout[y], false}, vw1{&P_.slot_layout[w1], false};
+    ContractSym s1 = contract_symbolic(vy, ca1, vw1, cb1, nullptr);
+    View vx{&s1.C, false}, vw2{&P_.slot_layout[w2], false};
+    ContractSym s2 = contract_symbolic(vx, ca2, vw2, cb2, nullptr);
+    Step fused;
+    const int prec = P_.precision == MPSXX_FP32 ? 1 : 0;
+    if (!nofuse && make_fused_axpy_step(s1, s2, y, w1, w2, (int)P_.slot_layout.size(), 0, fused)) {
+      const int id = add_inter(s2.C, prec);
+      P_.steps.push_back(std::move(fused));
+      P_.flops += s1.flops + s2.flops;
+      return id;
+    }
+    const int x = add_inter(s1.C, prec);
+    P_.steps.push_back(make_contract_step(s1, y, w1, x, 0, 1.0));
+    const int z = add_inter(s2.C, prec);
+    P_.steps.push_back(make_contract_step(s2, x, w2, z, 0, 1.0));
+    P_.flops += s1.flops + s2.flops;
+    return z;
+  }
+
   int tie(int x, const std::vector<int>& t) {
     std::vector<CopySeg> segs;
     Layout Y = tie_layout(P_.slot_layout[x], t, segs);
@@ -390,8 +416,8 @@ Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& l
   const int s_ropr = B.input(ropr, 3), s_wfn = B.input(wfn, 4);
   // compute_sigma_vector.hpp:63-69
   const int s1 = B.contract(s_lopr, false, {2}, s_wfn, false, {0});
-  const int s2 = B.contract(s1, false, {1, 2}, s_lmpo, false, {0, 2});
-  const int s3 = B.contract(s2, false, {4, 1}, s_rmpo, false, {0, 2});
+  // the two MPO-coefficient contractions (scalar MPO blocks) run as one fused pass
+  const int s3 = B.contract2(s1, {1, 2}, s_lmpo, {0, 2}, {4, 1}, s_rmpo, {0, 2});
   B.contract_out(s3, false, {4, 1}, s_ropr, false, {1, 2}, out);
   auto np = std::make_unique<CachedPlan>();
   np->plan = B.finish();
