@@ -507,12 +507,20 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
     const char* e = std::getenv("MPSXX_SMALL_GEMM_LIMIT");
     return e ? std::atoi(e) : 64;
   }();
-  std::vector<GemmTile> small;
+  // a reduction of >= split_units (pair, 16-wide k panel) units is split over the 4 waves of a work-group
+  static const int split_units = [] {
+    const char* e = std::getenv("MPSXX_SPLIT_UNITS");
+    return e ? std::atoi(e) : 4;
+  }();
+  std::vector<GemmTile> small, split;
   for (size_t ti : order) {
     const GemmTask& t = sym.tasks[ti];
     if (t.m <= small_limit && t.n <= small_limit) {
+      int units = 0;
+      for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) units += (sym.pairs[p].k + 15) / 16;
+      std::vector<GemmTile>& dst = units >= split_units ? split : small;
       for (int r = 0; r < t.m; r += kSmallTile)
-        for (int c = 0; c < t.n; c += kSmallTile) small.push_back({(int32_t)ti, r, c, 0});
+        for (int c = 0; c < t.n; c += kSmallTile) dst.push_back({(int32_t)ti, r, c, 0});
       continue;
     }
     for (int r = 0; r < t.m; r += kGemmTile)
@@ -522,6 +530,8 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
   st.work = upload_vec(tiles);
   st.nwork_small = (int)small.size();
   st.work_small = upload_vec(small);
+  st.nwork_split = (int)split.size();
+  st.work_split = upload_vec(split);
   st.tasks = upload_vec(sym.tasks);
   st.pairs = upload_vec(sym.pairs);
   return st;
@@ -645,6 +655,9 @@ void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out,
           launch_gemm_small<double, double>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
                                             st.pairs.as<GemmPair>(), (const double*)ptr[st.a],
                                             (const double*)ptr[st.b], (double*)ptr[st.c], st.alpha, st.accumulate, s);
+          launch_gemm_split<double, double>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
+                                            st.pairs.as<GemmPair>(), (const double*)ptr[st.a],
+                                            (const double*)ptr[st.b], (double*)ptr[st.c], st.alpha, st.accumulate, s);
         } else if (cf32) {
           launch_gemm<float, float>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
                                     st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
@@ -652,11 +665,17 @@ void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out,
           launch_gemm_small<float, float>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
                                           st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
                                           (float*)ptr[st.c], st.alpha, st.accumulate, s);
+          launch_gemm_split<float, float>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
+                                          st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
+                                          (float*)ptr[st.c], st.alpha, st.accumulate, s);
         } else {
           launch_gemm<float, double>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
                                      st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
                                      (double*)ptr[st.c], st.alpha, st.accumulate, s);
           launch_gemm_small<float, double>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
+                                           st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
+                                           (double*)ptr[st.c], st.alpha, st.accumulate, s);
+          launch_gemm_split<float, double>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
                                            st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
                                            (double*)ptr[st.c], st.alpha, st.accumulate, s);
         }

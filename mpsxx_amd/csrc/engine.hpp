@@ -88,6 +88,8 @@ struct Step {
   DevBuf work, tasks, pairs;   // device copies of the task lists
   int nwork_small = 0;         // GEMM: 32x32 tiles of the small blocks (gemm_small_kernel)
   DevBuf work_small;
+  int nwork_split = 0;         // GEMM: 32x32 tiles with long reductions (gemm_split_kernel)
+  DevBuf work_split;
   int64_t flops = 0;
 };
 

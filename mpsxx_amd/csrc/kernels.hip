@@ -203,6 +203,97 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
       }
 }
 
+// K-split variant for 32x32 tiles with a long reduction (many pairs and/or large k): the four waves of a work-group
+// take every fourth (pair, 16-wide k panel) unit of the same tile, then the partial tiles are summed in LDS in a
+// fixed order (deterministic) and stored once.
+template <typename T, typename TC>
+__global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restrict__ tiles,
+                                                         const GemmTask* __restrict__ tasks,
+                                                         const GemmPair* __restrict__ pairs, const T* __restrict__ A,
+                                                         const T* __restrict__ B, TC* __restrict__ C, double alpha,
+                                                         int accumulate) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T red[3][4][4][64];  // partial tiles of waves 1..3: [wave-1][mi*2+ni][r][lane]
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const GemmTile tl = tiles[blockIdx.x];
+  const GemmTask tk = tasks[tl.task];
+  acc_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  const int r0 = tl.row0 + lr, r1 = tl.row0 + 16 + lr;
+  const int c0 = tl.col0 + lr, c1 = tl.col0 + 16 + lr;
+  const bool vr0 = r0 < tk.m, vr1 = r1 < tk.m, vc0 = c0 < tk.n, vc1 = c1 < tk.n;
+  const int pend = tk.pair_begin + tk.pair_count;
+  constexpr int KU = 4;
+  int unit = 0;  // running (pair, panel) index; this wave takes unit % 4 == wave
+  for (int p = tk.pair_begin; p < pend; ++p) {
+    const GemmPair pr = pairs[p];
+    const int npanel = (pr.k + 4 * KU - 1) / (4 * KU);
+    const int first = (wave - unit % 4 + 4) % 4;  // first panel of this pair owned by this wave
+    unit += npanel;
+    if (first >= npanel) continue;
+    const T* Ap = A + pr.a_off;
+    const T* Bp = B + pr.b_off;
+    const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
+    const T* a0p = Ap + r0 * sam;
+    const T* a1p = Ap + r1 * sam;
+    const T* b0p = Bp + c0 * sbn;
+    const T* b1p = Bp + c1 * sbn;
+    for (int panel = first; panel < npanel; panel += 4) {
+      const int k0 = panel * 4 * KU;
+      T a0[KU], a1[KU], b0[KU], b1[KU];
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        const int gk = k0 + 4 * u + lk;
+        const bool vk = gk < pr.k;
+        const int64_t ao = gk * sak, bo = gk * sbk;
+        a0[u] = (vk && vr0) ? a0p[ao] : T(0);
+        a1[u] = (vk && vr1) ? a1p[ao] : T(0);
+        b0[u] = (vk && vc0) ? b0p[bo] : T(0);
+        b1[u] = (vk && vc1) ? b1p[bo] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        if (k0 + 4 * u >= pr.k) break;
+        acc[0][0] = Mfma<T>::op(a0[u], b0[u], acc[0][0]);
+        acc[0][1] = Mfma<T>::op(a0[u], b1[u], acc[0][1]);
+        acc[1][0] = Mfma<T>::op(a1[u], b0[u], acc[1][0]);
+        acc[1][1] = Mfma<T>::op(a1[u], b1[u], acc[1][1]);
+      }
+    }
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave - 1][mi * 2 + ni][r][lane] = acc[mi][ni][r];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const T v4 = acc[mi][ni][r] + red[0][mi * 2 + ni][r][lane] + red[1][mi * 2 + ni][r][lane] +
+                     red[2][mi * 2 + ni][r][lane];
+        const int row = tl.row0 + mi * 16 + Mfma<T>::row(lk, r);
+        const int col = tl.col0 + ni * 16 + lr;
+        if (row < tk.m && col < tk.n) {
+          const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+          const double v = alpha * (double)v4;
+          C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+        }
+      }
+}
+
 template <typename T, typename TC>
 __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restrict__ chunks,
                                                            const AxpyTask* __restrict__ tasks,
@@ -213,16 +304,29 @@ __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restri
   const Chunk ch = chunks[blockIdx.x];
   const AxpyTask tk = tasks[ch.task];
   const int pend = tk.pair_begin + tk.pair_count;
-  for (int r = ch.start + threadIdx.x; r < tk.len && r < ch.start + kAxpyChunk; r += 256) {
-    T acc = T(0);
-    for (int p = tk.pair_begin; p < pend; ++p) {
-      const AxpyPair pr = pairs[p];
-      const T coef = pr.s2_off >= 0 ? S[pr.s_off] * S2[pr.s2_off] : S[pr.s_off];
-      acc += coef * V[pr.v_off + (int64_t)r * pr.v_stride];
+  constexpr int E = kAxpyChunk / 256;  // elements per thread; the E loads of one pair are independent
+  const int end = min(tk.len, ch.start + kAxpyChunk);
+  T acc[E];
+#pragma unroll
+  for (int j = 0; j < E; ++j) acc[j] = T(0);
+  for (int p = tk.pair_begin; p < pend; ++p) {
+    const AxpyPair pr = pairs[p];
+    const T coef = pr.s2_off >= 0 ? S[pr.s_off] * S2[pr.s2_off] : S[pr.s_off];
+    const T* vp = V + pr.v_off;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int r = ch.start + threadIdx.x + 256 * j;
+      if (r < end) acc[j] += coef * vp[(int64_t)r * pr.v_stride];
     }
-    const int64_t idx = tk.c_off + r;
-    const double v = alpha * (double)acc;
-    C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+  }
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int r = ch.start + threadIdx.x + 256 * j;
+    if (r < end) {
+      const int64_t idx = tk.c_off + r;
+      const double v = alpha * (double)acc[j];
+      C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+    }
   }
 }
 
@@ -438,6 +542,20 @@ void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks,
                      B, C, alpha, accumulate);
 }
 
+template <typename T, typename TC>
+void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
+                       const T* B, TC* C, double alpha, int accumulate, hipStream_t s) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL((gemm_split_kernel<T, TC>), dim3(ntiles), dim3(256), 0, s, tiles, tasks, pairs, A, B, C, alpha,
+                     accumulate);
+}
+
+template void launch_gemm_split<double, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const double*,
+                                                const double*, double*, double, int, hipStream_t);
+template void launch_gemm_split<float, float>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
+                                              const float*, float*, double, int, hipStream_t);
+template void launch_gemm_split<float, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
+                                               const float*, double*, double, int, hipStream_t);
 template void launch_gemm_small<double, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const double*,
                                                 const double*, double*, double, int, hipStream_t);
 template void launch_gemm_small<float, float>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,

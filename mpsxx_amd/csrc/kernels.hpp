@@ -73,6 +73,11 @@ template <typename T, typename TC>
 void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
                        const T* B, TC* C, double alpha, int accumulate, hipStream_t s);
 
+// 32x32 tiles with the (pair, k-panel) reduction split over the 4 waves of a work-group, LDS sum at the end.
+template <typename T, typename TC>
+void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
+                       const T* B, TC* C, double alpha, int accumulate, hipStream_t s);
+
 template <typename T, typename TC>
 void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, const T* V,
                  const T* S, const T* S2, TC* C, double alpha, int accumulate, hipStream_t s);
