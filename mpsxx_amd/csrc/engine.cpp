@@ -507,10 +507,12 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
     const char* e = std::getenv("MPSXX_SMALL_GEMM_LIMIT");
     return e ? std::atoi(e) : 64;
   }();
-  // a reduction of >= split_units (pair, 16-wide k panel) units is split over the 4 waves of a work-group
+  // a reduction over several pairs with >= split_units (pair, 16-wide k panel) units is split over the 4 waves of a
+  // work-group (measured on the L=32 M=512 Hubbard sigma: single-pair blocks run best one wave per tile, multi-pair
+  // blocks best split from 2 units on)
   static const int split_units = [] {
     const char* e = std::getenv("MPSXX_SPLIT_UNITS");
-    return e ? std::atoi(e) : 4;
+    return e ? std::atoi(e) : 2;
   }();
   std::vector<GemmTile> small, split;
   for (size_t ti : order) {
@@ -518,7 +520,7 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
     if (t.m <= small_limit && t.n <= small_limit) {
       int units = 0;
       for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) units += (sym.pairs[p].k + 15) / 16;
-      std::vector<GemmTile>& dst = units >= split_units ? split : small;
+      std::vector<GemmTile>& dst = (t.pair_count >= 2 && units >= split_units) ? split : small;
       for (int r = 0; r < t.m; r += kSmallTile)
         for (int c = 0; c < t.n; c += kSmallTile) dst.push_back({(int32_t)ti, r, c, 0});
       continue;
