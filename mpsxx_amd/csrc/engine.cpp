@@ -322,6 +322,8 @@ ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const 
     t.c_off = C.off[cb_];
     t.pair_begin = (int32_t)R.pairs.size();
     t.pair_count = (int32_t)(q - p);
+    t.inst = 0;
+    t.pad = 0;
     int64_t m = -1, n = -1;
     for (size_t r = p; r < q; ++r) {
       const int32_t* ai = a.index(con[r].ia);
@@ -462,7 +464,7 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
       at.len = scalarB ? t.m : t.n;
       at.pair_begin = (int32_t)pairs.size();
       at.pair_count = t.pair_count;
-      at.pad = 0;
+      at.inst = 0;
       for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) {
         const GemmPair& g = sym.pairs[p];
         AxpyPair ap;
@@ -561,7 +563,7 @@ bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, i
     at.c_off = t.c_off;
     at.len = t.m;
     at.pair_begin = (int32_t)pairs.size();
-    at.pad = 0;
+    at.inst = 0;
     for (int q = t.pair_begin; q < t.pair_begin + t.pair_count; ++q) {
       const GemmPair& g2 = second.pairs[q];
       auto it = xtask.find(g2.a_off);
@@ -614,6 +616,29 @@ Step make_copy_step(const std::vector<CopySeg>& segs, int x, int y, double alpha
   return st;
 }
 
+namespace {
+template <typename T, typename TC>
+void launch_step_t(const Step& st, const Operands& ops, hipStream_t s) {
+  if (st.kind == Step::GEMM) {
+    launch_gemm<T, TC>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops,
+                       st.alpha, st.accumulate, s);
+    launch_gemm_small<T, TC>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
+                             st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s);
+    launch_gemm_split<T, TC>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
+                             st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s);
+  } else {
+    launch_axpy<T, TC>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(), st.pairs.as<AxpyPair>(), ops,
+                       st.alpha, st.accumulate, s);
+  }
+}
+}  // namespace
+
+void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s) {
+  if (!f32) launch_step_t<double, double>(st, ops, s);
+  else if (cf32) launch_step_t<float, float>(st, ops, s);
+  else launch_step_t<float, double>(st, ops, s);
+}
+
 void run_plan(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s) {
   run_plan_timed(P, inputs, out, s, {});
 }
@@ -648,56 +673,12 @@ void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out,
     if (!events.empty()) hip_check(hipEventRecord(events[si], s), "event");
     const bool f32 = P.slot_prec[st.a] == 1;
     const bool cf32 = P.slot_prec[st.c] == 1;
+    Operands ops{ptr[st.a], st.b >= 0 ? ptr[st.b] : nullptr, ptr[st.b2 >= 0 ? st.b2 : (st.b >= 0 ? st.b : st.a)],
+                 ptr[st.c], 0};
     switch (st.kind) {
       case Step::GEMM:
-        if (!f32) {
-          launch_gemm<double, double>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
-                                      st.pairs.as<GemmPair>(), (const double*)ptr[st.a], (const double*)ptr[st.b],
-                                      (double*)ptr[st.c], st.alpha, st.accumulate, s);
-          launch_gemm_small<double, double>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
-                                            st.pairs.as<GemmPair>(), (const double*)ptr[st.a],
-                                            (const double*)ptr[st.b], (double*)ptr[st.c], st.alpha, st.accumulate, s);
-          launch_gemm_split<double, double>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
-                                            st.pairs.as<GemmPair>(), (const double*)ptr[st.a],
-                                            (const double*)ptr[st.b], (double*)ptr[st.c], st.alpha, st.accumulate, s);
-        } else if (cf32) {
-          launch_gemm<float, float>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
-                                    st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
-                                    (float*)ptr[st.c], st.alpha, st.accumulate, s);
-          launch_gemm_small<float, float>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
-                                          st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
-                                          (float*)ptr[st.c], st.alpha, st.accumulate, s);
-          launch_gemm_split<float, float>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
-                                          st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
-                                          (float*)ptr[st.c], st.alpha, st.accumulate, s);
-        } else {
-          launch_gemm<float, double>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(),
-                                     st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
-                                     (double*)ptr[st.c], st.alpha, st.accumulate, s);
-          launch_gemm_small<float, double>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
-                                           st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
-                                           (double*)ptr[st.c], st.alpha, st.accumulate, s);
-          launch_gemm_split<float, double>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
-                                           st.pairs.as<GemmPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
-                                           (double*)ptr[st.c], st.alpha, st.accumulate, s);
-        }
-        break;
       case Step::AXPY:
-        if (!f32)
-          launch_axpy<double, double>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(),
-                                      st.pairs.as<AxpyPair>(), (const double*)ptr[st.a], (const double*)ptr[st.b],
-                                      (const double*)ptr[st.b2 >= 0 ? st.b2 : st.b], (double*)ptr[st.c], st.alpha,
-                                      st.accumulate, s);
-        else if (cf32)
-          launch_axpy<float, float>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(), st.pairs.as<AxpyPair>(),
-                                    (const float*)ptr[st.a], (const float*)ptr[st.b],
-                                    (const float*)ptr[st.b2 >= 0 ? st.b2 : st.b], (float*)ptr[st.c], st.alpha,
-                                    st.accumulate, s);
-        else
-          launch_axpy<float, double>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(),
-                                     st.pairs.as<AxpyPair>(), (const float*)ptr[st.a], (const float*)ptr[st.b],
-                                     (const float*)ptr[st.b2 >= 0 ? st.b2 : st.b], (double*)ptr[st.c], st.alpha,
-                                     st.accumulate, s);
+        launch_step(st, ops, f32, cf32, s);
         break;
       case Step::COPY:
         if (!f32 && !cf32)

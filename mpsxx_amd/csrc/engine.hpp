@@ -115,6 +115,9 @@ Step make_copy_step(const std::vector<CopySeg>& segs, int x, int y, double alpha
 bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, int y, int w1, int w2, int z,
                           int accumulate, Step& out);
 
+// Launch the kernels of one GEMM or AXPY step (FP64, FP32, or FP32 operands into an FP64 output).
+void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s);
+
 // Execute a plan: inputs[i] are the FP64 device pointers of the bound inputs, out the FP64 output pointer.
 void run_plan(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s);
 // Same, recording an event before each step and after the last one (events.size() >= steps + 1).

@@ -34,18 +34,26 @@ template <> struct Mfma<float> {
 
 constexpr int BM = 64, BN = 64, BK = 16, LDP = 64 + 1;
 
+// operand base of the task's instance
+template <typename T>
+__device__ __forceinline__ T* opnd(const void* p, int table, int inst) {
+  return table ? reinterpret_cast<T* const*>(p)[inst] : reinterpret_cast<T*>(const_cast<void*>(p));
+}
+
 template <typename T, typename TC>
 __global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __restrict__ tiles,
                                                            const GemmTask* __restrict__ tasks,
-                                                           const GemmPair* __restrict__ pairs,
-                                                           const T* __restrict__ A, const T* __restrict__ B,
-                                                           TC* __restrict__ C, double alpha, int accumulate) {
+                                                           const GemmPair* __restrict__ pairs, Operands ops,
+                                                           double alpha, int accumulate) {
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T As[BK][LDP];
   __shared__ T Bs[BK][LDP];
 
   const GemmTile tl = tiles[blockIdx.x];
   const GemmTask tk = tasks[tl.task];
+  const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
+  const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
+  TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -135,9 +143,8 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __res
 template <typename T, typename TC>
 __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restrict__ tiles, int ntiles,
                                                          const GemmTask* __restrict__ tasks,
-                                                         const GemmPair* __restrict__ pairs, const T* __restrict__ A,
-                                                         const T* __restrict__ B, TC* __restrict__ C, double alpha,
-                                                         int accumulate) {
+                                                         const GemmPair* __restrict__ pairs, Operands ops,
+                                                         double alpha, int accumulate) {
   typedef typename Mfma<T>::acc_t acc_t;
   const int wave = threadIdx.x >> 6;
   const int ti = blockIdx.x * 4 + wave;
@@ -146,6 +153,9 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
   const int lr = lane & 15, lk = lane >> 4;
   const GemmTile tl = tiles[ti];
   const GemmTask tk = tasks[tl.task];
+  const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
+  const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
+  TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
   acc_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -209,9 +219,8 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
 template <typename T, typename TC>
 __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restrict__ tiles,
                                                          const GemmTask* __restrict__ tasks,
-                                                         const GemmPair* __restrict__ pairs, const T* __restrict__ A,
-                                                         const T* __restrict__ B, TC* __restrict__ C, double alpha,
-                                                         int accumulate) {
+                                                         const GemmPair* __restrict__ pairs, Operands ops,
+                                                         double alpha, int accumulate) {
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T red[3][4][4][64];  // partial tiles of waves 1..3: [wave-1][mi*2+ni][r][lane]
   const int wave = threadIdx.x >> 6;
@@ -219,6 +228,9 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
   const int lr = lane & 15, lk = lane >> 4;
   const GemmTile tl = tiles[blockIdx.x];
   const GemmTask tk = tasks[tl.task];
+  const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
+  const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
+  TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
   acc_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -297,12 +309,14 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
 template <typename T, typename TC>
 __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restrict__ chunks,
                                                            const AxpyTask* __restrict__ tasks,
-                                                           const AxpyPair* __restrict__ pairs,
-                                                           const T* __restrict__ V, const T* __restrict__ S,
-                                                           const T* __restrict__ S2, TC* __restrict__ C,
+                                                           const AxpyPair* __restrict__ pairs, Operands ops,
                                                            double alpha, int accumulate) {
   const Chunk ch = chunks[blockIdx.x];
   const AxpyTask tk = tasks[ch.task];
+  const T* __restrict__ V = opnd<const T>(ops.a, ops.table, tk.inst);
+  const T* __restrict__ S = opnd<const T>(ops.b, ops.table, tk.inst);
+  const T* __restrict__ S2 = opnd<const T>(ops.b2, ops.table, tk.inst);
+  TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
   const int pend = tk.pair_begin + tk.pair_count;
   constexpr int E = kAxpyChunk / 256;  // elements per thread; the E loads of one pair are independent
   const int end = min(tk.len, ch.start + kAxpyChunk);
@@ -453,19 +467,19 @@ static inline int grid_for(int64_t n) {
 }
 
 template <typename T, typename TC>
-void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
-                 const T* B, TC* C, double alpha, int accumulate, hipStream_t s) {
+void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
+                 double alpha, int accumulate, hipStream_t s) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((gemm_grouped_kernel<T, TC>), dim3(ntiles), dim3(256), 0, s, tiles, tasks, pairs, A, B, C, alpha,
+  hipLaunchKernelGGL((gemm_grouped_kernel<T, TC>), dim3(ntiles), dim3(256), 0, s, tiles, tasks, pairs, ops, alpha,
                      accumulate);
 }
 
 template <typename T, typename TC>
-void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, const T* V,
-                 const T* S, const T* S2, TC* C, double alpha, int accumulate, hipStream_t s) {
+void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, Operands ops,
+                 double alpha, int accumulate, hipStream_t s) {
   if (nchunks <= 0) return;
-  hipLaunchKernelGGL((axpy_grouped_kernel<T, TC>), dim3(nchunks), dim3(256), 0, s, chunks, tasks, pairs, V, S, S2, C,
-                     alpha, accumulate);
+  hipLaunchKernelGGL((axpy_grouped_kernel<T, TC>), dim3(nchunks), dim3(256), 0, s, chunks, tasks, pairs, ops, alpha,
+                     accumulate);
 }
 
 template <typename TX, typename TY>
@@ -535,45 +549,39 @@ void launch_gram(const double* X, int64_t ldx, int mx, const double* Y, int64_t 
 }
 
 template <typename T, typename TC>
-void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
-                       const T* B, TC* C, double alpha, int accumulate, hipStream_t s) {
+void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
+                       double alpha, int accumulate, hipStream_t s) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((gemm_small_kernel<T, TC>), dim3((ntiles + 3) / 4), dim3(256), 0, s, tiles, ntiles, tasks, pairs, A,
-                     B, C, alpha, accumulate);
+  hipLaunchKernelGGL((gemm_small_kernel<T, TC>), dim3((ntiles + 3) / 4), dim3(256), 0, s, tiles, ntiles, tasks, pairs,
+                     ops, alpha, accumulate);
 }
 
 template <typename T, typename TC>
-void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
-                       const T* B, TC* C, double alpha, int accumulate, hipStream_t s) {
+void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
+                       double alpha, int accumulate, hipStream_t s) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((gemm_split_kernel<T, TC>), dim3(ntiles), dim3(256), 0, s, tiles, tasks, pairs, A, B, C, alpha,
+  hipLaunchKernelGGL((gemm_split_kernel<T, TC>), dim3(ntiles), dim3(256), 0, s, tiles, tasks, pairs, ops, alpha,
                      accumulate);
 }
 
-template void launch_gemm_split<double, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const double*,
-                                                const double*, double*, double, int, hipStream_t);
-template void launch_gemm_split<float, float>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
-                                              const float*, float*, double, int, hipStream_t);
-template void launch_gemm_split<float, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
-                                               const float*, double*, double, int, hipStream_t);
-template void launch_gemm_small<double, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const double*,
-                                                const double*, double*, double, int, hipStream_t);
-template void launch_gemm_small<float, float>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
-                                              const float*, float*, double, int, hipStream_t);
-template void launch_gemm_small<float, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
-                                               const float*, double*, double, int, hipStream_t);
-template void launch_gemm<double, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const double*,
-                                          const double*, double*, double, int, hipStream_t);
-template void launch_gemm<float, float>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
-                                        const float*, float*, double, int, hipStream_t);
-template void launch_gemm<float, double>(const GemmTile*, int, const GemmTask*, const GemmPair*, const float*,
-                                         const float*, double*, double, int, hipStream_t);
-template void launch_axpy<double, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const double*, const double*,
-                                 const double*, double*, double, int, hipStream_t);
-template void launch_axpy<float, float>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const float*, const float*,
-                                 const float*, float*, double, int, hipStream_t);
-template void launch_axpy<float, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, const float*, const float*,
-                                 const float*, double*, double, int, hipStream_t);
+#define MPX_INST_GEMM(FN, T, TC) \
+  template void FN<T, TC>(const GemmTile*, int, const GemmTask*, const GemmPair*, Operands, double, int, hipStream_t);
+MPX_INST_GEMM(launch_gemm, double, double)
+MPX_INST_GEMM(launch_gemm, float, float)
+MPX_INST_GEMM(launch_gemm, float, double)
+MPX_INST_GEMM(launch_gemm_small, double, double)
+MPX_INST_GEMM(launch_gemm_small, float, float)
+MPX_INST_GEMM(launch_gemm_small, float, double)
+MPX_INST_GEMM(launch_gemm_split, double, double)
+MPX_INST_GEMM(launch_gemm_split, float, float)
+MPX_INST_GEMM(launch_gemm_split, float, double)
+#undef MPX_INST_GEMM
+template void launch_axpy<double, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, Operands, double, int,
+                                          hipStream_t);
+template void launch_axpy<float, float>(const Chunk*, int, const AxpyTask*, const AxpyPair*, Operands, double, int,
+                                        hipStream_t);
+template void launch_axpy<float, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, Operands, double, int,
+                                         hipStream_t);
 template void launch_copy<double, double>(const Chunk*, int, const CopySeg*, const double*, double*, double, double,
                                           hipStream_t);
 template void launch_copy<float, float>(const Chunk*, int, const CopySeg*, const float*, float*, double, double,

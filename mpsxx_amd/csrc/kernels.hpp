@@ -27,6 +27,18 @@ struct GemmTask {
   int64_t c_off;
   int32_t m, n;
   int32_t pair_begin, pair_count;
+  int32_t inst;  // instance of a batched launch (index into the operand pointer tables); 0 otherwise
+  int32_t pad;
+};
+
+// Operand bases of one launch: direct pointers (table == 0), or device tables of per-instance pointers indexed by the
+// task's `inst` (table == 1), so one grouped launch covers many independent problems.
+struct Operands {
+  const void* a;
+  const void* b;
+  const void* b2;
+  void* c;
+  int table;
 };
 
 struct GemmTile {
@@ -46,7 +58,7 @@ struct AxpyTask {
   int64_t c_off;
   int32_t len;
   int32_t pair_begin, pair_count;
-  int32_t pad;
+  int32_t inst;
 };
 
 struct Chunk {
@@ -63,24 +75,26 @@ constexpr int kGemmTile = 64;     // C tile edge of the grouped GEMM
 constexpr int kAxpyChunk = 1024;  // elements per work-group in the AXPY / copy kernels
 
 // Launchers.  T = storage/arithmetic type of A and B (double or float), TC = type of C.
+// 64x64 tiles, 4 waves, LDS-staged (large blocks).
 template <typename T, typename TC>
-void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
-                 const T* B, TC* C, double alpha, int accumulate, hipStream_t s);
+void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
+                 double alpha, int accumulate, hipStream_t s);
 
 // 32x32 tiles, one wave each, operands loaded straight into the MFMA registers (blocks up to ~64 on a side).
 constexpr int kSmallTile = 32;
 template <typename T, typename TC>
-void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
-                       const T* B, TC* C, double alpha, int accumulate, hipStream_t s);
+void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
+                       double alpha, int accumulate, hipStream_t s);
 
 // 32x32 tiles with the (pair, k-panel) reduction split over the 4 waves of a work-group, LDS sum at the end.
 template <typename T, typename TC>
-void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, const T* A,
-                       const T* B, TC* C, double alpha, int accumulate, hipStream_t s);
+void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
+                       double alpha, int accumulate, hipStream_t s);
 
+// ops.a = V (vector side), ops.b = S (coefficients), ops.b2 = second coefficients of fused pairs, ops.c = C
 template <typename T, typename TC>
-void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, const T* V,
-                 const T* S, const T* S2, TC* C, double alpha, int accumulate, hipStream_t s);
+void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, Operands ops,
+                 double alpha, int accumulate, hipStream_t s);
 
 template <typename TX, typename TY>
 void launch_copy(const Chunk* chunks, int nchunks, const CopySeg* segs, const TX* x, TY* y, double alpha,
