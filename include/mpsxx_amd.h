@@ -190,6 +190,31 @@ int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tens
                                 mpsxx_tensor wfn0, int precision, int reps, int32_t max_steps, double* step_ms,
                                 int64_t* step_flops, int32_t* step_kind, int32_t* nsteps, void* stream);
 
+/* ---- batched two-site sigma vectors (many independent instances per launch) ------------------------------------- */
+/* No single reference counterpart: it runs compute_sigma_vector (src/compute_sigma_vector.hpp:54-70) for n independent
+ * problems (different chains, bond dimensions and block structures) with ONE grouped launch per contraction step and
+ * optional hipGraph replay.  The batch keeps pointers to the caller's tensors, which must outlive it; their block
+ * structures are fixed at creation (a changed input structure makes run fail with MPSXX_ERR_SHAPE).  Not re-entrant. */
+typedef struct mpsxx_batch_s* mpsxx_batch;
+int mpsxx_batch_sigma_twosite_create(int32_t n, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
+                                     const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, const mpsxx_tensor* wfn0,
+                                     const mpsxx_tensor* sgv0, int precision, mpsxx_batch* out, void* stream);
+/* sgv0[i] += H_i wfn0[i] for all i (accumulate, as compute_sigma_vector).  FP32 batches refresh the FP32 copies of
+ * the inputs whose bit is set in convert_mask (1 lmpo, 2 rmpo, 4 lopr, 8 ropr, 16 wfn0; 31 = all). */
+int mpsxx_batch_sigma_twosite_run(mpsxx_batch b, uint32_t convert_mask, int use_graph, void* stream);
+/* mean device time (HIP events on `stream`) of one run over `reps` runs; sgv0 accumulates every run */
+int mpsxx_batch_time(mpsxx_batch b, int32_t reps, uint32_t convert_mask, int use_graph, double* ms, void* stream);
+/* per launch group (0 = FP32 input conversion, 1.. = contraction steps): mean device time over `reps` eager runs and
+ * algorithmic flops; sgv0 accumulates every run */
+int mpsxx_batch_profile(mpsxx_batch b, int32_t reps, int32_t max_steps, double* step_ms, int64_t* step_flops,
+                        int32_t* nsteps, void* stream);
+/* Autotuner: times candidate tile policies per step on the batch's own task lists (outputs redirected to scratch) and
+ * keeps the fastest; policy[3*k..3*k+2] = {small-block limit, split threshold, grid cap} of step k. */
+int mpsxx_batch_tune(mpsxx_batch b, int32_t reps, int32_t max_steps, int32_t* policy, void* stream);
+int mpsxx_batch_set_policy(mpsxx_batch b, int32_t nsteps, const int32_t* policy);
+int mpsxx_batch_info(mpsxx_batch b, int64_t* flops, int64_t* bytes, int32_t* nsteps, int64_t* work_items);
+int mpsxx_batch_free(mpsxx_batch b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,80 @@
+"""Batched two-site sigma vectors over the C ABI (``mpsxx_batch_*`` in include/mpsxx_amd.h).
+
+One ``SigmaBatch`` binds n independent (lmpo, rmpo, lopr, ropr, wfn0, sgv0) instances -- different chains, bond
+dimensions and block structures -- and applies ``sgv0[i] += H_i wfn0[i]`` for all of them with one grouped launch per
+contraction step (optionally replayed from a captured hipGraph).
+"""
+import ctypes
+
+import numpy as np
+
+from . import FP64, VP, DTensor, _check, lib
+
+ALL_INPUTS = 31   # convert mask: refresh the FP32 copies of every input
+WFN_ONLY = 16     # only the wavefunction changed (Davidson iterations)
+
+
+class SigmaBatch:
+    def __init__(self, instances, precision=FP64):
+        """instances: sequence of (lmpo, rmpo, lopr, ropr, wfn0, sgv0) DTensors; empty sgv0 are shaped here."""
+        self.inst = [tuple(x) for x in instances]  # keeps the tensors alive as long as the batch
+        n = len(self.inst)
+        arr = [(VP * n)(*[self.inst[i][k].h for i in range(n)]) for k in range(6)]
+        self.h = VP()
+        _check(lib().mpsxx_batch_sigma_twosite_create(ctypes.c_int32(n), *arr, ctypes.c_int(precision),
+                                                      ctypes.byref(self.h), None))
+        self.precision = precision
+
+    def run(self, convert_mask=ALL_INPUTS, use_graph=True):
+        _check(lib().mpsxx_batch_sigma_twosite_run(self.h, ctypes.c_uint32(convert_mask), ctypes.c_int(int(use_graph)),
+                                                   None))
+
+    def time(self, reps, convert_mask=ALL_INPUTS, use_graph=True):
+        ms = ctypes.c_double()
+        _check(lib().mpsxx_batch_time(self.h, ctypes.c_int32(reps), ctypes.c_uint32(convert_mask),
+                                      ctypes.c_int(int(use_graph)), ctypes.byref(ms), None))
+        return ms.value
+
+    def profile(self, reps):
+        """[(ms, flops)] per launch group: FP32 input conversion first, then each contraction step."""
+        n = 16
+        ms = np.zeros(n)
+        fl = np.zeros(n, dtype=np.int64)
+        ns = ctypes.c_int32()
+        _check(lib().mpsxx_batch_profile(self.h, ctypes.c_int32(reps), ctypes.c_int32(n), ms.ctypes.data_as(VP),
+                                         fl.ctypes.data_as(VP), ctypes.byref(ns), None))
+        return [(float(ms[k]), int(fl[k])) for k in range(ns.value)]
+
+    def tune(self, reps=5):
+        n = 16
+        pol = np.zeros(3 * n, dtype=np.int32)
+        _check(lib().mpsxx_batch_tune(self.h, ctypes.c_int32(reps), ctypes.c_int32(n), pol.ctypes.data_as(VP), None))
+        return [tuple(int(v) for v in pol[3 * k:3 * k + 3]) for k in range(self.info()["nsteps"])]
+
+    def set_policy(self, policies):
+        pol = np.ascontiguousarray(np.array(policies, dtype=np.int32).reshape(-1))
+        _check(lib().mpsxx_batch_set_policy(self.h, ctypes.c_int32(len(policies)), pol.ctypes.data_as(VP)))
+
+    def info(self):
+        fl = ctypes.c_int64()
+        by = ctypes.c_int64()
+        ns = ctypes.c_int32()
+        wi = ctypes.c_int64()
+        _check(lib().mpsxx_batch_info(self.h, ctypes.byref(fl), ctypes.byref(by), ctypes.byref(ns), ctypes.byref(wi)))
+        return {"flops": fl.value, "bytes": by.value, "nsteps": ns.value, "work_items": wi.value}
+
+    def free(self):
+        if self.h:
+            lib().mpsxx_batch_free(self.h)
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def upload_instance(w):
+    """Device tensors (lmpo, rmpo, lopr, ropr, wfn0, empty sgv0) of a workload dict with QST entries."""
+    return tuple(DTensor(w[k]) for k in ("lmpo", "rmpo", "lopr", "ropr", "wfn")) + (DTensor(),)

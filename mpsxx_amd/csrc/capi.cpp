@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include "batch.hpp"
 #include "mpsxx_amd.h"
 #include "ops.hpp"
 
@@ -42,6 +43,11 @@ mpx::Tensor& T(mpsxx_tensor h) {
   if (!h) throw mpx::Error(MPSXX_ERR_ARG, "null tensor handle");
   if (h->magic != mpsxx_tensor_s::kMagic) throw mpx::Error(MPSXX_ERR_ARG, "invalid or freed tensor handle");
   return h->t;
+}
+
+mpx::BatchSigma& B(mpsxx_batch h) {
+  if (!h || h->magic != mpsxx_batch_s::kMagic || !h->b) throw mpx::Error(MPSXX_ERR_ARG, "invalid or freed batch handle");
+  return *h->b;
 }
 
 hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -371,6 +377,100 @@ int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tens
       if (step_flops) step_flops[i] = P.steps[i].flops;
       if (step_kind) step_kind[i] = (int32_t)P.steps[i].kind;
     }
+  });
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// batched sigma vectors
+
+int mpsxx_batch_sigma_twosite_create(int32_t n, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
+                                     const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, const mpsxx_tensor* wfn0,
+                                     const mpsxx_tensor* sgv0, int precision, mpsxx_batch* out, void* stream) {
+  return guard([&] {
+    if (!out || n <= 0 || !lmpo || !rmpo || !lopr || !ropr || !wfn0 || !sgv0)
+      throw mpx::Error(MPSXX_ERR_ARG, "batch create: bad arguments");
+    std::vector<mpx::BatchSigma::Operand6> inst(n);
+    for (int i = 0; i < n; ++i)
+      inst[i] = {&T(lmpo[i]), &T(rmpo[i]), &T(lopr[i]), &T(ropr[i]), &T(wfn0[i]), &T(sgv0[i])};
+    auto* b = new mpsxx_batch_s;
+    try {
+      b->b = std::make_unique<mpx::BatchSigma>(inst, precision, S(stream));
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int mpsxx_batch_sigma_twosite_run(mpsxx_batch b, uint32_t convert_mask, int use_graph, void* stream) {
+  return guard([&] { B(b).run(convert_mask, use_graph != 0, S(stream)); });
+}
+
+int mpsxx_batch_time(mpsxx_batch b, int32_t reps, uint32_t convert_mask, int use_graph, double* ms, void* stream) {
+  return guard([&] {
+    const double t = B(b).time(reps, convert_mask, use_graph != 0, S(stream));
+    if (ms) *ms = t;
+  });
+}
+
+int mpsxx_batch_profile(mpsxx_batch b, int32_t reps, int32_t max_steps, double* step_ms, int64_t* step_flops,
+                        int32_t* nsteps, void* stream) {
+  return guard([&] {
+    mpx::BatchSigma& x = B(b);
+    const int ns = x.nsteps() + 1;
+    if (nsteps) *nsteps = ns;
+    if (ns > max_steps) throw mpx::Error(MPSXX_ERR_ARG, "batch profile: max_steps too small");
+    const std::vector<double> t = x.profile(reps, S(stream));
+    for (int k = 0; k < ns; ++k) {
+      if (step_ms) step_ms[k] = t[k];
+      if (step_flops) step_flops[k] = k == 0 ? 0 : x.step_flops(k - 1);
+    }
+  });
+}
+
+int mpsxx_batch_tune(mpsxx_batch b, int32_t reps, int32_t max_steps, int32_t* policy, void* stream) {
+  return guard([&] {
+    mpx::BatchSigma& x = B(b);
+    if (x.nsteps() > max_steps) throw mpx::Error(MPSXX_ERR_ARG, "batch tune: max_steps too small");
+    const std::vector<mpx::GemmPolicy> p = x.tune(reps, S(stream));
+    for (size_t k = 0; policy && k < p.size(); ++k) {
+      policy[3 * k] = p[k].small_limit;
+      policy[3 * k + 1] = p[k].split_units;
+      policy[3 * k + 2] = p[k].max_groups;
+    }
+  });
+}
+
+int mpsxx_batch_set_policy(mpsxx_batch b, int32_t nsteps, const int32_t* policy) {
+  return guard([&] {
+    mpx::BatchSigma& x = B(b);
+    if (nsteps != x.nsteps() || !policy) throw mpx::Error(MPSXX_ERR_ARG, "batch set_policy: one policy per step");
+    std::vector<mpx::GemmPolicy> p(nsteps);
+    for (int k = 0; k < nsteps; ++k) {
+      if (policy[3 * k] < 0 || policy[3 * k + 1] < 1 || policy[3 * k + 2] < 0)
+        throw mpx::Error(MPSXX_ERR_ARG, "batch set_policy: negative limit, threshold or grid cap");
+      p[k] = {policy[3 * k], policy[3 * k + 1], policy[3 * k + 2]};
+    }
+    x.set_policies(p);
+  });
+}
+
+int mpsxx_batch_info(mpsxx_batch b, int64_t* flops, int64_t* bytes, int32_t* nsteps, int64_t* work_items) {
+  return guard([&] {
+    mpx::BatchSigma& x = B(b);
+    if (flops) *flops = x.flops();
+    if (bytes) *bytes = x.bytes();
+    if (nsteps) *nsteps = x.nsteps();
+    if (work_items) *work_items = x.work_items();
+  });
+}
+
+int mpsxx_batch_free(mpsxx_batch b) {
+  return guard([&] {
+    if (!b) return;
+    B(b);
+    delete b;
   });
 }
 

@@ -488,48 +488,16 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
     st.work = upload_vec(chunks);
     st.tasks = upload_vec(tasks);
     st.pairs = upload_vec(pairs);
+    st.host.atasks = std::move(tasks);
+    st.host.apairs = std::move(pairs);
+    st.host.chunks = std::move(chunks);
     return st;
   }
   st.kind = Step::GEMM;
-  std::vector<GemmTile> tiles;
-  // heavy tasks first so the tail of the launch is made of small tiles
-  std::vector<size_t> order(sym.tasks.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::vector<int64_t> cost(sym.tasks.size(), 0);
-  for (size_t ti = 0; ti < sym.tasks.size(); ++ti) {
-    const GemmTask& t = sym.tasks[ti];
-    int64_t ksum = 0;
-    for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) ksum += sym.pairs[p].k;
-    cost[ti] = ksum;
-  }
-  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
-  // blocks with both sides <= small_limit go to the one-wave direct-load kernel in 32x32 tiles, the rest to the
-  // LDS-staged 64x64 kernel
-  static const int small_limit = [] {
-    const char* e = std::getenv("MPSXX_SMALL_GEMM_LIMIT");
-    return e ? std::atoi(e) : 64;
-  }();
-  // a reduction over several pairs with >= split_units (pair, 16-wide k panel) units is split over the 4 waves of a
-  // work-group (measured on the L=32 M=512 Hubbard sigma: single-pair blocks run best one wave per tile, multi-pair
-  // blocks best split from 2 units on)
-  static const int split_units = [] {
-    const char* e = std::getenv("MPSXX_SPLIT_UNITS");
-    return e ? std::atoi(e) : 2;
-  }();
-  std::vector<GemmTile> small, split;
-  for (size_t ti : order) {
-    const GemmTask& t = sym.tasks[ti];
-    if (t.m <= small_limit && t.n <= small_limit) {
-      int units = 0;
-      for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) units += (sym.pairs[p].k + 15) / 16;
-      std::vector<GemmTile>& dst = (t.pair_count >= 2 && units >= split_units) ? split : small;
-      for (int r = 0; r < t.m; r += kSmallTile)
-        for (int c = 0; c < t.n; c += kSmallTile) dst.push_back({(int32_t)ti, r, c, 0});
-      continue;
-    }
-    for (int r = 0; r < t.m; r += kGemmTile)
-      for (int c = 0; c < t.n; c += kGemmTile) tiles.push_back({(int32_t)ti, r, c, 0});
-  }
+  std::vector<GemmTile> tiles, small, split;
+  const GemmPolicy pol = default_policy();
+  gemm_tiles(sym.tasks, sym.pairs, pol, tiles, small, split);
+  st.max_groups = pol.max_groups;
   st.nwork = (int)tiles.size();
   st.work = upload_vec(tiles);
   st.nwork_small = (int)small.size();
@@ -538,7 +506,52 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
   st.work_split = upload_vec(split);
   st.tasks = upload_vec(sym.tasks);
   st.pairs = upload_vec(sym.pairs);
+  st.host.gtasks = sym.tasks;
+  st.host.gpairs = sym.pairs;
   return st;
+}
+
+GemmPolicy default_policy() {
+  // measured on the L=32 M=512 Hubbard sigma: single-pair blocks run best one wave per tile, multi-pair blocks best
+  // split from 2 units on
+  static const GemmPolicy p = [] {
+    GemmPolicy q;
+    if (const char* e = std::getenv("MPSXX_SMALL_GEMM_LIMIT")) q.small_limit = std::atoi(e);
+    if (const char* e = std::getenv("MPSXX_SPLIT_UNITS")) q.split_units = std::atoi(e);
+    return q;
+  }();
+  return p;
+}
+
+void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>& pairs, const GemmPolicy& pol,
+                std::vector<GemmTile>& big, std::vector<GemmTile>& small, std::vector<GemmTile>& split) {
+  big.clear();
+  small.clear();
+  split.clear();
+  // heavy tasks first so the tail of each launch is made of small tiles
+  std::vector<size_t> order(tasks.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::vector<int64_t> cost(tasks.size(), 0);
+  for (size_t ti = 0; ti < tasks.size(); ++ti) {
+    const GemmTask& t = tasks[ti];
+    int64_t ksum = 0;
+    for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) ksum += pairs[p].k;
+    cost[ti] = ksum;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  for (size_t ti : order) {
+    const GemmTask& t = tasks[ti];
+    if (t.m <= pol.small_limit && t.n <= pol.small_limit) {
+      int units = 0;
+      for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) units += (pairs[p].k + 15) / 16;
+      std::vector<GemmTile>& dst = (t.pair_count >= 2 && units >= pol.split_units) ? split : small;
+      for (int r = 0; r < t.m; r += kSmallTile)
+        for (int c = 0; c < t.n; c += kSmallTile) dst.push_back({(int32_t)ti, r, c, 0});
+      continue;
+    }
+    for (int r = 0; r < t.m; r += kGemmTile)
+      for (int c = 0; c < t.n; c += kGemmTile) big.push_back({(int32_t)ti, r, c, 0});
+  }
 }
 
 bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, int y, int w1, int w2, int z,
@@ -597,6 +610,9 @@ bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, i
   out.work = upload_vec(chunks);
   out.tasks = upload_vec(tasks);
   out.pairs = upload_vec(pairs);
+  out.host.atasks = std::move(tasks);
+  out.host.apairs = std::move(pairs);
+  out.host.chunks = std::move(chunks);
   return true;
 }
 
@@ -621,14 +637,14 @@ template <typename T, typename TC>
 void launch_step_t(const Step& st, const Operands& ops, hipStream_t s) {
   if (st.kind == Step::GEMM) {
     launch_gemm<T, TC>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops,
-                       st.alpha, st.accumulate, s);
+                       st.alpha, st.accumulate, s, st.max_groups);
     launch_gemm_small<T, TC>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
-                             st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s);
+                             st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
     launch_gemm_split<T, TC>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
-                             st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s);
+                             st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
   } else {
     launch_axpy<T, TC>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(), st.pairs.as<AxpyPair>(), ops,
-                       st.alpha, st.accumulate, s);
+                       st.alpha, st.accumulate, s, st.max_groups);
   }
 }
 }  // namespace

@@ -77,6 +77,29 @@ void gemm_indices(int transa, int transb, int NA, int NB, int NC, std::vector<in
 // Structure of btas::Tie (STREINDEX.h:95-147) and its gather segments.
 Layout tie_layout(const Layout& X, const std::vector<int>& tie, std::vector<CopySeg>& segs);
 
+// Host copies of a step's task lists (batched launches concatenate them across instances).
+struct HostLists {
+  std::vector<GemmTask> gtasks;
+  std::vector<GemmPair> gpairs;
+  std::vector<AxpyTask> atasks;
+  std::vector<AxpyPair> apairs;
+  std::vector<Chunk> chunks;
+};
+
+// How the blocks of a GEMM step are mapped to kernels: blocks with both sides <= small_limit run as 32x32 tiles, split
+// over the 4 waves of a work-group when they reduce over >= 2 pairs and >= split_units (pair, 16-wide k panel) units,
+// one wave per tile otherwise; larger blocks run as 64x64 LDS tiles.  max_groups > 0 caps the grid of every launch.
+struct GemmPolicy {
+  int small_limit = 64;
+  int split_units = 2;
+  int max_groups = 0;
+};
+GemmPolicy default_policy();  // the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT / MPSXX_SPLIT_UNITS
+
+// Tile lists of one GEMM step under a policy (heaviest blocks first in each list).
+void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>& pairs, const GemmPolicy& pol,
+                std::vector<GemmTile>& big, std::vector<GemmTile>& small, std::vector<GemmTile>& split);
+
 // One device step of a plan.
 struct Step {
   enum Kind { GEMM, AXPY, COPY } kind = GEMM;
@@ -90,7 +113,9 @@ struct Step {
   DevBuf work_small;
   int nwork_split = 0;         // GEMM: 32x32 tiles with long reductions (gemm_split_kernel)
   DevBuf work_split;
+  int max_groups = 0;          // grid cap of the step's launches (0: one work-group per work item)
   int64_t flops = 0;
+  HostLists host;
 };
 
 // Slot table: inputs are bound per call; intermediates are owned by the plan.

@@ -41,7 +41,7 @@ __device__ __forceinline__ T* opnd(const void* p, int table, int inst) {
 }
 
 template <typename T, typename TC>
-__global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __restrict__ tiles,
+__global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __restrict__ tiles, int ntiles,
                                                            const GemmTask* __restrict__ tasks,
                                                            const GemmPair* __restrict__ pairs, Operands ops,
                                                            double alpha, int accumulate) {
@@ -49,92 +49,94 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __res
   __shared__ T As[BK][LDP];
   __shared__ T Bs[BK][LDP];
 
-  const GemmTile tl = tiles[blockIdx.x];
-  const GemmTask tk = tasks[tl.task];
-  const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
-  const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
-  TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  const int lr = lane & 15, lk = lane >> 4;
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const GemmTile tl = tiles[w];
+    const GemmTask tk = tasks[tl.task];
+    const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
+    const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
+    TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int lr = lane & 15, lk = lane >> 4;
 
-  acc_t acc[2][2];
+    acc_t acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+      for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  const int pend = tk.pair_begin + tk.pair_count;
-  for (int p = tk.pair_begin; p < pend; ++p) {
-    const GemmPair pr = pairs[p];
-    const T* Ap = A + pr.a_off;
-    const T* Bp = B + pr.b_off;
-    const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
-    for (int k0 = 0; k0 < pr.k; k0 += BK) {
-      if (pr.sa_k == 1) {
-        // contiguous along k: 16 consecutive threads read one row segment
+    const int pend = tk.pair_begin + tk.pair_count;
+    for (int p = tk.pair_begin; p < pend; ++p) {
+      const GemmPair pr = pairs[p];
+      const T* Ap = A + pr.a_off;
+      const T* Bp = B + pr.b_off;
+      const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
+      for (int k0 = 0; k0 < pr.k; k0 += BK) {
+        if (pr.sa_k == 1) {
+          // contiguous along k: 16 consecutive threads read one row segment
 #pragma unroll
-        for (int e = tid; e < BM * BK; e += 256) {
-          const int kk = e & 15, i = e >> 4;
-          const int gi = tl.row0 + i, gk = k0 + kk;
-          As[kk][i] = (gi < tk.m && gk < pr.k) ? Ap[gi * sam + gk] : T(0);
-        }
-      } else {
+          for (int e = tid; e < BM * BK; e += 256) {
+            const int kk = e & 15, i = e >> 4;
+            const int gi = tl.row0 + i, gk = k0 + kk;
+            As[kk][i] = (gi < tk.m && gk < pr.k) ? Ap[gi * sam + gk] : T(0);
+          }
+        } else {
 #pragma unroll
-        for (int e = tid; e < BM * BK; e += 256) {
-          const int i = e & 63, kk = e >> 6;
-          const int gi = tl.row0 + i, gk = k0 + kk;
-          As[kk][i] = (gi < tk.m && gk < pr.k) ? Ap[gi * sam + gk * sak] : T(0);
+          for (int e = tid; e < BM * BK; e += 256) {
+            const int i = e & 63, kk = e >> 6;
+            const int gi = tl.row0 + i, gk = k0 + kk;
+            As[kk][i] = (gi < tk.m && gk < pr.k) ? Ap[gi * sam + gk * sak] : T(0);
+          }
         }
+        if (pr.sb_n == 1) {
+#pragma unroll
+          for (int e = tid; e < BK * BN; e += 256) {
+            const int j = e & 63, kk = e >> 6;
+            const int gj = tl.col0 + j, gk = k0 + kk;
+            Bs[kk][j] = (gj < tk.n && gk < pr.k) ? Bp[gk * sbk + gj] : T(0);
+          }
+        } else {
+#pragma unroll
+          for (int e = tid; e < BK * BN; e += 256) {
+            const int kk = e & 15, j = e >> 4;
+            const int gj = tl.col0 + j, gk = k0 + kk;
+            Bs[kk][j] = (gj < tk.n && gk < pr.k) ? Bp[gk * sbk + gj * sbn] : T(0);
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+          const T a0 = As[kk + lk][wr * 32 + lr];
+          const T a1 = As[kk + lk][wr * 32 + 16 + lr];
+          const T b0 = Bs[kk + lk][wc * 32 + lr];
+          const T b1 = Bs[kk + lk][wc * 32 + 16 + lr];
+          acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+          acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+          acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+          acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
       }
-      if (pr.sb_n == 1) {
-#pragma unroll
-        for (int e = tid; e < BK * BN; e += 256) {
-          const int j = e & 63, kk = e >> 6;
-          const int gj = tl.col0 + j, gk = k0 + kk;
-          Bs[kk][j] = (gj < tk.n && gk < pr.k) ? Bp[gk * sbk + gj] : T(0);
-        }
-      } else {
-#pragma unroll
-        for (int e = tid; e < BK * BN; e += 256) {
-          const int kk = e & 15, j = e >> 4;
-          const int gj = tl.col0 + j, gk = k0 + kk;
-          Bs[kk][j] = (gj < tk.n && gk < pr.k) ? Bp[gk * sbk + gj * sbn] : T(0);
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < BK; kk += 4) {
-        const T a0 = As[kk + lk][wr * 32 + lr];
-        const T a1 = As[kk + lk][wr * 32 + 16 + lr];
-        const T b0 = Bs[kk + lk][wc * 32 + lr];
-        const T b1 = Bs[kk + lk][wc * 32 + 16 + lr];
-        acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
-        acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
-        acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
-        acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
-      }
-      __syncthreads();
     }
-  }
 
-  // D layout of 16x16x4: lane l holds rows 4*(l/16)+r, column l%16.
+    // D layout of 16x16x4: lane l holds rows 4*(l/16)+r, column l%16.
 #pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
+    for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
+      for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = tl.row0 + wr * 32 + mi * 16 + Mfma<T>::row(lk, r);
-        const int col = tl.col0 + wc * 32 + ni * 16 + lr;
-        if (row < tk.m && col < tk.n) {
-          const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
-          const double v = alpha * (double)acc[mi][ni][r];
-          C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+        for (int r = 0; r < 4; ++r) {
+          const int row = tl.row0 + wr * 32 + mi * 16 + Mfma<T>::row(lk, r);
+          const int col = tl.col0 + wc * 32 + ni * 16 + lr;
+          if (row < tk.m && col < tk.n) {
+            const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+            const double v = alpha * (double)acc[mi][ni][r];
+            C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+          }
         }
-      }
+  }
 }
 
 // Small-block variant: one wave per 32x32 C tile (2x2 MFMA 16x16 blocks), operands read straight from global memory
@@ -147,77 +149,77 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
                                                          double alpha, int accumulate) {
   typedef typename Mfma<T>::acc_t acc_t;
   const int wave = threadIdx.x >> 6;
-  const int ti = blockIdx.x * 4 + wave;
-  if (ti >= ntiles) return;
   const int lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
-  const GemmTile tl = tiles[ti];
-  const GemmTask tk = tasks[tl.task];
-  const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
-  const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
-  TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
-  acc_t acc[2][2];
+  for (int ti = blockIdx.x * 4 + wave; ti < ntiles; ti += gridDim.x * 4) {
+    const GemmTile tl = tiles[ti];
+    const GemmTask tk = tasks[tl.task];
+    const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
+    const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
+    TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
+    acc_t acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-  const int r0 = tl.row0 + lr, r1 = tl.row0 + 16 + lr;
-  const int c0 = tl.col0 + lr, c1 = tl.col0 + 16 + lr;
-  const bool vr0 = r0 < tk.m, vr1 = r1 < tk.m, vc0 = c0 < tk.n, vc1 = c1 < tk.n;
-  const int pend = tk.pair_begin + tk.pair_count;
-  constexpr int KU = 4;  // k-panel of 16: all 16 operand loads of a lane are issued before the 16 MFMAs
-  for (int p = tk.pair_begin; p < pend; ++p) {
-    const GemmPair pr = pairs[p];
-    const T* Ap = A + pr.a_off;
-    const T* Bp = B + pr.b_off;
-    const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
-    const T* a0p = Ap + r0 * sam;
-    const T* a1p = Ap + r1 * sam;
-    const T* b0p = Bp + c0 * sbn;
-    const T* b1p = Bp + c1 * sbn;
-    for (int k0 = 0; k0 < pr.k; k0 += 4 * KU) {
-      T a0[KU], a1[KU], b0[KU], b1[KU];
+      for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    const int r0 = tl.row0 + lr, r1 = tl.row0 + 16 + lr;
+    const int c0 = tl.col0 + lr, c1 = tl.col0 + 16 + lr;
+    const bool vr0 = r0 < tk.m, vr1 = r1 < tk.m, vc0 = c0 < tk.n, vc1 = c1 < tk.n;
+    const int pend = tk.pair_begin + tk.pair_count;
+    constexpr int KU = 4;  // k-panel of 16: all 16 operand loads of a lane are issued before the 16 MFMAs
+    for (int p = tk.pair_begin; p < pend; ++p) {
+      const GemmPair pr = pairs[p];
+      const T* Ap = A + pr.a_off;
+      const T* Bp = B + pr.b_off;
+      const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
+      const T* a0p = Ap + r0 * sam;
+      const T* a1p = Ap + r1 * sam;
+      const T* b0p = Bp + c0 * sbn;
+      const T* b1p = Bp + c1 * sbn;
+      for (int k0 = 0; k0 < pr.k; k0 += 4 * KU) {
+        T a0[KU], a1[KU], b0[KU], b1[KU];
 #pragma unroll
-      for (int u = 0; u < KU; ++u) {
-        const int gk = k0 + 4 * u + lk;
-        const bool vk = gk < pr.k;
-        const int64_t ao = gk * sak, bo = gk * sbk;
-        a0[u] = (vk && vr0) ? a0p[ao] : T(0);
-        a1[u] = (vk && vr1) ? a1p[ao] : T(0);
-        b0[u] = (vk && vc0) ? b0p[bo] : T(0);
-        b1[u] = (vk && vc1) ? b1p[bo] : T(0);
-      }
+        for (int u = 0; u < KU; ++u) {
+          const int gk = k0 + 4 * u + lk;
+          const bool vk = gk < pr.k;
+          const int64_t ao = gk * sak, bo = gk * sbk;
+          a0[u] = (vk && vr0) ? a0p[ao] : T(0);
+          a1[u] = (vk && vr1) ? a1p[ao] : T(0);
+          b0[u] = (vk && vc0) ? b0p[bo] : T(0);
+          b1[u] = (vk && vc1) ? b1p[bo] : T(0);
+        }
 #pragma unroll
-      for (int u = 0; u < KU; ++u) {
-        if (k0 + 4 * u >= pr.k) break;  // uniform across the wave
-        acc[0][0] = Mfma<T>::op(a0[u], b0[u], acc[0][0]);
-        acc[0][1] = Mfma<T>::op(a0[u], b1[u], acc[0][1]);
-        acc[1][0] = Mfma<T>::op(a1[u], b0[u], acc[1][0]);
-        acc[1][1] = Mfma<T>::op(a1[u], b1[u], acc[1][1]);
-      }
-    }
-  }
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = tl.row0 + mi * 16 + Mfma<T>::row(lk, r);
-        const int col = tl.col0 + ni * 16 + lr;
-        if (row < tk.m && col < tk.n) {
-          const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
-          const double v = alpha * (double)acc[mi][ni][r];
-          C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+        for (int u = 0; u < KU; ++u) {
+          if (k0 + 4 * u >= pr.k) break;  // uniform across the wave
+          acc[0][0] = Mfma<T>::op(a0[u], b0[u], acc[0][0]);
+          acc[0][1] = Mfma<T>::op(a0[u], b1[u], acc[0][1]);
+          acc[1][0] = Mfma<T>::op(a1[u], b0[u], acc[1][0]);
+          acc[1][1] = Mfma<T>::op(a1[u], b1[u], acc[1][1]);
         }
       }
+    }
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = tl.row0 + mi * 16 + Mfma<T>::row(lk, r);
+          const int col = tl.col0 + ni * 16 + lr;
+          if (row < tk.m && col < tk.n) {
+            const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+            const double v = alpha * (double)acc[mi][ni][r];
+            C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+          }
+        }
+  }
 }
 
 // K-split variant for 32x32 tiles with a long reduction (many pairs and/or large k): the four waves of a work-group
 // take every fourth (pair, 16-wide k panel) unit of the same tile, then the partial tiles are summed in LDS in a
 // fixed order (deterministic) and stored once.
 template <typename T, typename TC>
-__global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restrict__ tiles,
+__global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restrict__ tiles, int ntiles,
                                                          const GemmTask* __restrict__ tasks,
                                                          const GemmPair* __restrict__ pairs, Operands ops,
                                                          double alpha, int accumulate) {
@@ -226,120 +228,126 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
-  const GemmTile tl = tiles[blockIdx.x];
-  const GemmTask tk = tasks[tl.task];
-  const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
-  const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
-  TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
-  acc_t acc[2][2];
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const GemmTile tl = tiles[w];
+    const GemmTask tk = tasks[tl.task];
+    const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
+    const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
+    TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
+    acc_t acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-  const int r0 = tl.row0 + lr, r1 = tl.row0 + 16 + lr;
-  const int c0 = tl.col0 + lr, c1 = tl.col0 + 16 + lr;
-  const bool vr0 = r0 < tk.m, vr1 = r1 < tk.m, vc0 = c0 < tk.n, vc1 = c1 < tk.n;
-  const int pend = tk.pair_begin + tk.pair_count;
-  constexpr int KU = 4;
-  int unit = 0;  // running (pair, panel) index; this wave takes unit % 4 == wave
-  for (int p = tk.pair_begin; p < pend; ++p) {
-    const GemmPair pr = pairs[p];
-    const int npanel = (pr.k + 4 * KU - 1) / (4 * KU);
-    const int first = (wave - unit % 4 + 4) % 4;  // first panel of this pair owned by this wave
-    unit += npanel;
-    if (first >= npanel) continue;
-    const T* Ap = A + pr.a_off;
-    const T* Bp = B + pr.b_off;
-    const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
-    const T* a0p = Ap + r0 * sam;
-    const T* a1p = Ap + r1 * sam;
-    const T* b0p = Bp + c0 * sbn;
-    const T* b1p = Bp + c1 * sbn;
-    for (int panel = first; panel < npanel; panel += 4) {
-      const int k0 = panel * 4 * KU;
-      T a0[KU], a1[KU], b0[KU], b1[KU];
+      for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    const int r0 = tl.row0 + lr, r1 = tl.row0 + 16 + lr;
+    const int c0 = tl.col0 + lr, c1 = tl.col0 + 16 + lr;
+    const bool vr0 = r0 < tk.m, vr1 = r1 < tk.m, vc0 = c0 < tk.n, vc1 = c1 < tk.n;
+    const int pend = tk.pair_begin + tk.pair_count;
+    constexpr int KU = 4;
+    int unit = 0;  // running (pair, panel) index; this wave takes unit % 4 == wave
+    for (int p = tk.pair_begin; p < pend; ++p) {
+      const GemmPair pr = pairs[p];
+      const int npanel = (pr.k + 4 * KU - 1) / (4 * KU);
+      const int first = (wave - unit % 4 + 4) % 4;  // first panel of this pair owned by this wave
+      unit += npanel;
+      if (first >= npanel) continue;
+      const T* Ap = A + pr.a_off;
+      const T* Bp = B + pr.b_off;
+      const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
+      const T* a0p = Ap + r0 * sam;
+      const T* a1p = Ap + r1 * sam;
+      const T* b0p = Bp + c0 * sbn;
+      const T* b1p = Bp + c1 * sbn;
+      for (int panel = first; panel < npanel; panel += 4) {
+        const int k0 = panel * 4 * KU;
+        T a0[KU], a1[KU], b0[KU], b1[KU];
 #pragma unroll
-      for (int u = 0; u < KU; ++u) {
-        const int gk = k0 + 4 * u + lk;
-        const bool vk = gk < pr.k;
-        const int64_t ao = gk * sak, bo = gk * sbk;
-        a0[u] = (vk && vr0) ? a0p[ao] : T(0);
-        a1[u] = (vk && vr1) ? a1p[ao] : T(0);
-        b0[u] = (vk && vc0) ? b0p[bo] : T(0);
-        b1[u] = (vk && vc1) ? b1p[bo] : T(0);
-      }
+        for (int u = 0; u < KU; ++u) {
+          const int gk = k0 + 4 * u + lk;
+          const bool vk = gk < pr.k;
+          const int64_t ao = gk * sak, bo = gk * sbk;
+          a0[u] = (vk && vr0) ? a0p[ao] : T(0);
+          a1[u] = (vk && vr1) ? a1p[ao] : T(0);
+          b0[u] = (vk && vc0) ? b0p[bo] : T(0);
+          b1[u] = (vk && vc1) ? b1p[bo] : T(0);
+        }
 #pragma unroll
-      for (int u = 0; u < KU; ++u) {
-        if (k0 + 4 * u >= pr.k) break;
-        acc[0][0] = Mfma<T>::op(a0[u], b0[u], acc[0][0]);
-        acc[0][1] = Mfma<T>::op(a0[u], b1[u], acc[0][1]);
-        acc[1][0] = Mfma<T>::op(a1[u], b0[u], acc[1][0]);
-        acc[1][1] = Mfma<T>::op(a1[u], b1[u], acc[1][1]);
-      }
-    }
-  }
-  if (wave > 0) {
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-      for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) red[wave - 1][mi * 2 + ni][r][lane] = acc[mi][ni][r];
-  }
-  __syncthreads();
-  if (wave != 0) return;
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const T v4 = acc[mi][ni][r] + red[0][mi * 2 + ni][r][lane] + red[1][mi * 2 + ni][r][lane] +
-                     red[2][mi * 2 + ni][r][lane];
-        const int row = tl.row0 + mi * 16 + Mfma<T>::row(lk, r);
-        const int col = tl.col0 + ni * 16 + lr;
-        if (row < tk.m && col < tk.n) {
-          const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
-          const double v = alpha * (double)v4;
-          C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+        for (int u = 0; u < KU; ++u) {
+          if (k0 + 4 * u >= pr.k) break;
+          acc[0][0] = Mfma<T>::op(a0[u], b0[u], acc[0][0]);
+          acc[0][1] = Mfma<T>::op(a0[u], b1[u], acc[0][1]);
+          acc[1][0] = Mfma<T>::op(a1[u], b0[u], acc[1][0]);
+          acc[1][1] = Mfma<T>::op(a1[u], b1[u], acc[1][1]);
         }
       }
+    }
+    if (wave > 0) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[wave - 1][mi * 2 + ni][r][lane] = acc[mi][ni][r];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const T v4 = acc[mi][ni][r] + red[0][mi * 2 + ni][r][lane] + red[1][mi * 2 + ni][r][lane] +
+                         red[2][mi * 2 + ni][r][lane];
+            const int row = tl.row0 + mi * 16 + Mfma<T>::row(lk, r);
+            const int col = tl.col0 + ni * 16 + lr;
+            if (row < tk.m && col < tk.n) {
+              const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+              const double v = alpha * (double)v4;
+              C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+            }
+          }
+    }
+    __syncthreads();  // red is rewritten by the next tile
+  }
 }
 
 template <typename T, typename TC>
-__global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restrict__ chunks,
+__global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restrict__ chunks, int nchunks,
                                                            const AxpyTask* __restrict__ tasks,
                                                            const AxpyPair* __restrict__ pairs, Operands ops,
                                                            double alpha, int accumulate) {
-  const Chunk ch = chunks[blockIdx.x];
-  const AxpyTask tk = tasks[ch.task];
-  const T* __restrict__ V = opnd<const T>(ops.a, ops.table, tk.inst);
-  const T* __restrict__ S = opnd<const T>(ops.b, ops.table, tk.inst);
-  const T* __restrict__ S2 = opnd<const T>(ops.b2, ops.table, tk.inst);
-  TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
-  const int pend = tk.pair_begin + tk.pair_count;
-  constexpr int E = kAxpyChunk / 256;  // elements per thread; the E loads of one pair are independent
-  const int end = min(tk.len, ch.start + kAxpyChunk);
-  T acc[E];
+  for (int w = blockIdx.x; w < nchunks; w += gridDim.x) {
+    const Chunk ch = chunks[w];
+    const AxpyTask tk = tasks[ch.task];
+    const T* __restrict__ V = opnd<const T>(ops.a, ops.table, tk.inst);
+    const T* __restrict__ S = opnd<const T>(ops.b, ops.table, tk.inst);
+    const T* __restrict__ S2 = opnd<const T>(ops.b2, ops.table, tk.inst);
+    TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
+    const int pend = tk.pair_begin + tk.pair_count;
+    constexpr int E = kAxpyChunk / 256;  // elements per thread; the E loads of one pair are independent
+    const int end = min(tk.len, ch.start + kAxpyChunk);
+    T acc[E];
 #pragma unroll
-  for (int j = 0; j < E; ++j) acc[j] = T(0);
-  for (int p = tk.pair_begin; p < pend; ++p) {
-    const AxpyPair pr = pairs[p];
-    const T coef = pr.s2_off >= 0 ? S[pr.s_off] * S2[pr.s2_off] : S[pr.s_off];
-    const T* vp = V + pr.v_off;
+    for (int j = 0; j < E; ++j) acc[j] = T(0);
+    for (int p = tk.pair_begin; p < pend; ++p) {
+      const AxpyPair pr = pairs[p];
+      const T coef = pr.s2_off >= 0 ? S[pr.s_off] * S2[pr.s2_off] : S[pr.s_off];
+      const T* vp = V + pr.v_off;
+#pragma unroll
+      for (int j = 0; j < E; ++j) {
+        const int r = ch.start + threadIdx.x + 256 * j;
+        if (r < end) acc[j] += coef * vp[(int64_t)r * pr.v_stride];
+      }
+    }
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int r = ch.start + threadIdx.x + 256 * j;
-      if (r < end) acc[j] += coef * vp[(int64_t)r * pr.v_stride];
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const int r = ch.start + threadIdx.x + 256 * j;
-    if (r < end) {
-      const int64_t idx = tk.c_off + r;
-      const double v = alpha * (double)acc[j];
-      C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+      if (r < end) {
+        const int64_t idx = tk.c_off + r;
+        const double v = alpha * (double)acc[j];
+        C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+      }
     }
   }
 }
@@ -354,6 +362,16 @@ __global__ __launch_bounds__(256) void copy_kernel(const Chunk* __restrict__ chu
     const int64_t d = sg.dst_off + r;
     const double v = alpha * (double)x[sg.src_off + (int64_t)r * sg.stride];
     y[d] = (beta == 0.0) ? (TY)v : (TY)(beta * (double)y[d] + v);
+  }
+}
+
+__global__ __launch_bounds__(256) void convert_segments_kernel(const Chunk* __restrict__ chunks, int nchunks,
+                                                               const ConvSeg* __restrict__ segs) {
+  for (int w = blockIdx.x; w < nchunks; w += gridDim.x) {
+    const Chunk ch = chunks[w];
+    const ConvSeg sg = segs[ch.task];
+    const int64_t end = min(sg.n, (int64_t)ch.start + kAxpyChunk);
+    for (int64_t r = ch.start + threadIdx.x; r < end; r += 256) sg.dst[r] = (float)sg.src[r];
   }
 }
 
@@ -459,6 +477,9 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restri
   if (threadIdx.x == 0) G[pair] = acc;
 }
 
+// work-groups of a task-list launch: one per work item, or at most max_groups (> 0) looping over the list
+static inline int groups(int items, int max_groups) { return max_groups > 0 && items > max_groups ? max_groups : items; }
+
 static inline int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 8192) g = 8192;
@@ -468,18 +489,18 @@ static inline int grid_for(int64_t n) {
 
 template <typename T, typename TC>
 void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
-                 double alpha, int accumulate, hipStream_t s) {
+                 double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((gemm_grouped_kernel<T, TC>), dim3(ntiles), dim3(256), 0, s, tiles, tasks, pairs, ops, alpha,
-                     accumulate);
+  hipLaunchKernelGGL((gemm_grouped_kernel<T, TC>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles, ntiles,
+                     tasks, pairs, ops, alpha, accumulate);
 }
 
 template <typename T, typename TC>
 void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, Operands ops,
-                 double alpha, int accumulate, hipStream_t s) {
+                 double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (nchunks <= 0) return;
-  hipLaunchKernelGGL((axpy_grouped_kernel<T, TC>), dim3(nchunks), dim3(256), 0, s, chunks, tasks, pairs, ops, alpha,
-                     accumulate);
+  hipLaunchKernelGGL((axpy_grouped_kernel<T, TC>), dim3(groups(nchunks, max_groups)), dim3(256), 0, s, chunks,
+                     nchunks, tasks, pairs, ops, alpha, accumulate);
 }
 
 template <typename TX, typename TY>
@@ -487,6 +508,12 @@ void launch_copy(const Chunk* chunks, int nchunks, const CopySeg* segs, const TX
                  double beta, hipStream_t s) {
   if (nchunks <= 0) return;
   hipLaunchKernelGGL((copy_kernel<TX, TY>), dim3(nchunks), dim3(256), 0, s, chunks, segs, x, y, alpha, beta);
+}
+
+void launch_convert_segments(const Chunk* chunks, int nchunks, const ConvSeg* segs, hipStream_t s, int max_groups) {
+  if (nchunks <= 0) return;
+  hipLaunchKernelGGL(convert_segments_kernel, dim3(groups(nchunks, max_groups)), dim3(256), 0, s, chunks, nchunks,
+                     segs);
 }
 
 void launch_dot(const Chunk* chunks, int nchunks, const CopySeg* segs, const double* x, const double* y,
@@ -550,22 +577,23 @@ void launch_gram(const double* X, int64_t ldx, int mx, const double* Y, int64_t 
 
 template <typename T, typename TC>
 void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
-                       double alpha, int accumulate, hipStream_t s) {
+                       double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((gemm_small_kernel<T, TC>), dim3((ntiles + 3) / 4), dim3(256), 0, s, tiles, ntiles, tasks, pairs,
-                     ops, alpha, accumulate);
+  hipLaunchKernelGGL((gemm_small_kernel<T, TC>), dim3(groups((ntiles + 3) / 4, max_groups)), dim3(256), 0, s, tiles,
+                     ntiles, tasks, pairs, ops, alpha, accumulate);
 }
 
 template <typename T, typename TC>
 void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
-                       double alpha, int accumulate, hipStream_t s) {
+                       double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((gemm_split_kernel<T, TC>), dim3(ntiles), dim3(256), 0, s, tiles, tasks, pairs, ops, alpha,
-                     accumulate);
+  hipLaunchKernelGGL((gemm_split_kernel<T, TC>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles, ntiles,
+                     tasks, pairs, ops, alpha, accumulate);
 }
 
 #define MPX_INST_GEMM(FN, T, TC) \
-  template void FN<T, TC>(const GemmTile*, int, const GemmTask*, const GemmPair*, Operands, double, int, hipStream_t);
+  template void FN<T, TC>(const GemmTile*, int, const GemmTask*, const GemmPair*, Operands, double, int, hipStream_t, \
+                           int);
 MPX_INST_GEMM(launch_gemm, double, double)
 MPX_INST_GEMM(launch_gemm, float, float)
 MPX_INST_GEMM(launch_gemm, float, double)
@@ -577,11 +605,11 @@ MPX_INST_GEMM(launch_gemm_split, float, float)
 MPX_INST_GEMM(launch_gemm_split, float, double)
 #undef MPX_INST_GEMM
 template void launch_axpy<double, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, Operands, double, int,
-                                          hipStream_t);
+                                          hipStream_t, int);
 template void launch_axpy<float, float>(const Chunk*, int, const AxpyTask*, const AxpyPair*, Operands, double, int,
-                                        hipStream_t);
+                                        hipStream_t, int);
 template void launch_axpy<float, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, Operands, double, int,
-                                         hipStream_t);
+                                         hipStream_t, int);
 template void launch_copy<double, double>(const Chunk*, int, const CopySeg*, const double*, double*, double, double,
                                           hipStream_t);
 template void launch_copy<float, float>(const Chunk*, int, const CopySeg*, const float*, float*, double, double,

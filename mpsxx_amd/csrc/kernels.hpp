@@ -74,27 +74,37 @@ struct CopySeg {
 constexpr int kGemmTile = 64;     // C tile edge of the grouped GEMM
 constexpr int kAxpyChunk = 1024;  // elements per work-group in the AXPY / copy kernels
 
-// Launchers.  T = storage/arithmetic type of A and B (double or float), TC = type of C.
+// Launchers.  T = storage/arithmetic type of A and B (double or float), TC = type of C.  max_groups > 0 caps the grid
+// (persistent form: each work-group loops over the task list with a grid stride); 0 launches one per work item.
 // 64x64 tiles, 4 waves, LDS-staged (large blocks).
 template <typename T, typename TC>
 void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
-                 double alpha, int accumulate, hipStream_t s);
+                 double alpha, int accumulate, hipStream_t s, int max_groups = 0);
 
 // 32x32 tiles, one wave each, operands loaded straight into the MFMA registers (blocks up to ~64 on a side).
 constexpr int kSmallTile = 32;
 template <typename T, typename TC>
 void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
-                       double alpha, int accumulate, hipStream_t s);
+                       double alpha, int accumulate, hipStream_t s, int max_groups = 0);
 
 // 32x32 tiles with the (pair, k-panel) reduction split over the 4 waves of a work-group, LDS sum at the end.
 template <typename T, typename TC>
 void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
-                       double alpha, int accumulate, hipStream_t s);
+                       double alpha, int accumulate, hipStream_t s, int max_groups = 0);
 
 // ops.a = V (vector side), ops.b = S (coefficients), ops.b2 = second coefficients of fused pairs, ops.c = C
 template <typename T, typename TC>
 void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, Operands ops,
-                 double alpha, int accumulate, hipStream_t s);
+                 double alpha, int accumulate, hipStream_t s, int max_groups = 0);
+
+// FP64 -> FP32 copies of many buffers in one launch (mixed-precision inputs of a batched launch); the chunks are
+// (segment, start) pairs of kAxpyChunk elements.
+struct ConvSeg {
+  const double* src;
+  float* dst;
+  int64_t n;
+};
+void launch_convert_segments(const Chunk* chunks, int nchunks, const ConvSeg* segs, hipStream_t s, int max_groups = 0);
 
 template <typename TX, typename TY>
 void launch_copy(const Chunk* chunks, int nchunks, const CopySeg* segs, const TX* x, TY* y, double alpha,

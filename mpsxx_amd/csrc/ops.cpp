@@ -405,12 +405,8 @@ void op_copy(Tensor& x, Tensor& y, hipStream_t s) {
 // ---------------------------------------------------------------------------------------------------------------------
 // sigma vectors (src/compute_sigma_vector.hpp)
 
-Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
-                         const Layout& wfn, const Layout* out, int prec) {
-  std::vector<uint64_t> key{OP_SIGMA2, (uint64_t)prec, lmpo.hash, rmpo.hash, lopr.hash, ropr.hash, wfn.hash,
-                            out ? out->hash : 0ull};
-  CachedPlan* cp = cache().find(key);
-  if (cp) return cp->plan;
+Plan build_plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
+                              const Layout& wfn, const Layout* out, int prec) {
   PlanBuilder B(prec);
   const int s_lmpo = B.input(lmpo, 0), s_rmpo = B.input(rmpo, 1), s_lopr = B.input(lopr, 2);
   const int s_ropr = B.input(ropr, 3), s_wfn = B.input(wfn, 4);
@@ -419,8 +415,17 @@ Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& l
   // the two MPO-coefficient contractions (scalar MPO blocks) run as one fused pass
   const int s3 = B.contract2(s1, {1, 2}, s_lmpo, {0, 2}, {4, 1}, s_rmpo, {0, 2});
   B.contract_out(s3, false, {4, 1}, s_ropr, false, {1, 2}, out);
+  return B.finish();
+}
+
+Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
+                         const Layout& wfn, const Layout* out, int prec) {
+  std::vector<uint64_t> key{OP_SIGMA2, (uint64_t)prec, lmpo.hash, rmpo.hash, lopr.hash, ropr.hash, wfn.hash,
+                            out ? out->hash : 0ull};
+  CachedPlan* cp = cache().find(key);
+  if (cp) return cp->plan;
   auto np = std::make_unique<CachedPlan>();
-  np->plan = B.finish();
+  np->plan = build_plan_sigma_twosite(lmpo, rmpo, lopr, ropr, wfn, out, prec);
   return cache().insert(key, std::move(np))->plan;
 }
 

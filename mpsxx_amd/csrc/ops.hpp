@@ -49,6 +49,10 @@ void op_renormalize_merged(bool fwd, Tensor& opr0, Tensor& bra, Tensor& ket, Ten
 // Plan for a two-site sigma vector (exposed for the Davidson driver, which binds device pointers itself).
 Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
                          const Layout& wfn, const Layout* out, int prec);
+// The same plan, uncached and owned by the caller (batched launches keep one per instance: its intermediates are
+// that instance's scratch).
+Plan build_plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
+                              const Layout& wfn, const Layout* out, int prec);
 
 void clear_plan_cache();
 

@@ -174,6 +174,24 @@ class QST:
     def norm(self):
         return float(np.sqrt(sum(float(np.sum(b * b)) for b in self.blocks.values())))
 
+    def scaled(self, alpha):
+        c = self.copy()
+        c.blocks = {k: alpha * v for k, v in c.blocks.items()}
+        return c
+
+    def diff_norm(self, other):
+        """Frobenius norm of (self - other) over the union of blocks (absent = zero)."""
+        t = 0.0
+        for k in set(self.blocks) | set(other.blocks):
+            a = self.blocks.get(k)
+            b = other.blocks.get(k)
+            if a is None:
+                a = np.zeros_like(b)
+            if b is None:
+                b = np.zeros_like(a)
+            t += float(np.sum((a - b) ** 2))
+        return float(np.sqrt(t))
+
 
 def random_fill(a, rng, scale=1.0, all_allowed=True, density=1.0):
     """Fill every allowed block (or a random subset of them) with N(0,1)*scale entries."""

@@ -3,7 +3,11 @@
 ``hubbard_twosite(L, M, U)``: the two-site sigma-vector problem at the middle bond of the open Hubbard chain at half
 filling and 2Sz = 0, with bond dimension M distributed over the quantum-number sectors of the bond like the reduced
 density matrix spectrum of a converged chain (Gaussian in particle-number and spin deviation from the mean, width
-sigma = 1.5), and pattern-preserving random entries N(0,1)/sqrt(m) in every quantum-number-allowed block.
+sigma = 1.5; no sector holds more states than the sites on its side can carry), and pattern-preserving random entries
+N(0,1)/sqrt(m) in every quantum-number-allowed block.
+
+``batch_specs(n, seed)``: the (L, U, M) sample of a batch of independent chains: L uniform over the even lengths
+16..64 (half filling with 2Sz = 0 needs N = L even), U from {2, 4, 8}, M from {128, 256, 512}.
 """
 import math
 
@@ -14,8 +18,39 @@ from .mpo import hubbard_mpo
 from .qst import QST
 
 
-def gaussian_dims(qns, center, M, width=1.5, min_weight=0.02):
-    """Distribute M states over the sectors `qns` with weight exp(-(dp^2 + ds^2) / (2 width^2))."""
+def sector_capacity(nsites, q):
+    """Number of states of `nsites` Hubbard sites with particle number q[0] and 2Sz q[1]."""
+    p, s = q
+    if (p + s) % 2 or p < 0:
+        return 0
+    nu, nd = (p + s) // 2, (p - s) // 2
+    if nu < 0 or nd < 0 or nu > nsites or nd > nsites:
+        return 0
+    return math.comb(nsites, nu) * math.comb(nsites, nd)
+
+
+def _cap(dims, caps, w):
+    """Clip dims to the sector capacities and hand the excess to the heaviest sectors with room left."""
+    dims = list(dims)
+    excess = 0
+    for j, c in enumerate(caps):
+        if dims[j] > c:
+            excess += dims[j] - c
+            dims[j] = c
+    for j in np.argsort(-np.asarray(w), kind="stable"):
+        if excess == 0:
+            break
+        room = caps[j] - dims[j]
+        if room > 0 and w[j] > 0:
+            take = min(room, excess)
+            dims[j] += take
+            excess -= take
+    return dims
+
+
+def gaussian_dims(qns, center, M, width=1.5, min_weight=0.02, caps=None):
+    """Distribute M states over the sectors `qns` with weight exp(-(dp^2 + ds^2) / (2 width^2)), at most caps[j]
+    states in sector j when capacities are given."""
     w = []
     for (p, s) in qns:
         dp = p - center[0]
@@ -41,6 +76,8 @@ def gaussian_dims(qns, center, M, width=1.5, min_weight=0.02):
         j = int(np.argmax(dims))
         dims[j] -= 1
         rem += 1
+    if caps is not None:
+        dims = _cap(dims, caps, w)
     return [int(d) for d in dims]
 
 
@@ -60,8 +97,8 @@ def hubbard_twosite(L=32, M=512, U=4.0, t=1.0, seed=0, width=1.5):
     rq = mps.right_bond_qns(L, i + 2, qt)
     nl = i             # sites on the left
     nr = L - (i + 2)   # sites on the right
-    ld = gaussian_dims(lq, (nl * qt[0] / L, 0), M, width)
-    rd = gaussian_dims(rq, (-nr * qt[0] / L, 0), M, width)
+    ld = gaussian_dims(lq, (nl * qt[0] / L, 0), M, width, caps=[sector_capacity(nl, q) for q in lq])
+    rd = gaussian_dims(rq, (-nr * qt[0] / L, 0), M, width, caps=[sector_capacity(nr, (-q[0], -q[1])) for q in rq])
     lq, ld = _prune(lq, ld)
     rq, rd = _prune(rq, rd)
 
@@ -80,3 +117,15 @@ def hubbard_twosite(L=32, M=512, U=4.0, t=1.0, seed=0, width=1.5):
     renv = fill(QST((0, 0), [[(-a, -b) for a, b in rq], qwr, rq], [rd, [1] * len(qwr), rd]))
     psi = fill(mps.twosite_wavefunction(lq, ld, rq, rd, qt))
     return mpos[i], mpos[i + 1], lenv, renv, psi
+
+
+def batch_specs(n=64, seed=4):
+    """(L, U, M) of n independent half-filled chains; see the module docstring."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        L = 2 * int(rng.integers(8, 33))
+        U = float(rng.choice([2.0, 4.0, 8.0]))
+        M = int(rng.choice([128, 256, 512]))
+        out.append((L, U, M))
+    return out
