@@ -1,14 +1,17 @@
 #!/usr/bin/env python3
-"""Benchmark: two-site sigma vector H_eff|psi> on one MI355X (FP64 block GEMMs on MFMA).
+"""Benchmark: two-site sigma vector H_eff|psi> on one MI355X (block GEMMs on MFMA).
 
-Workload (the headline configuration): open Hubbard chain L=32, t=1, U=4, half filling, 2Sz=0, bond dimension M=512,
-two-site centre at the middle bond, compact nearest-neighbour MPO.  Sector dimensions follow a converged chain's
-spectrum shape and every quantum-number-allowed block holds random N(0,1)/sqrt(m) entries (synthetic data, see
-mpsxx_amd/workloads.py).  One step = one sigma-vector application with inputs resident in HBM.
+Headline workload: open Hubbard chain L=32, t=1, U=4, half filling, 2Sz=0, bond dimension M=512, two-site centre at
+the middle bond, compact nearest-neighbour MPO.  Sector dimensions follow a converged chain's spectrum shape and every
+quantum-number-allowed block holds random N(0,1)/sqrt(m) entries (synthetic data, see mpsxx_amd/workloads.py).  One
+step = one sigma-vector application with inputs resident in HBM.
 
 Prints one JSON line.  `value` = algorithmic TFLOP/s of the sigma vector (reference contraction order, flops counted
-by the planner), `roofline` = the dominant kernel against its bound, `cpu_baseline` = the oracle (FP64 CPU
-restatement of the reference algorithm) timed on the host on the same inputs.
+by the planner); `roofline` = the kernel with the most device time per sigma (all its launches: algorithmic flops /
+device time, HIP events on the launch stream) against its bound; `cpu_baseline` = the oracle (FP64 CPU restatement of
+the reference algorithm) on the same inputs on one host core.  Beside it: the same sigma with FP32 block GEMMs, and a
+batch of 64 independent chains (L in 16..64, U in {2,4,8}, M in {128,256,512}) applied sequentially in FP64 and as one
+grouped launch per step (FP64 and FP32, hipGraph replay, autotuned tiles).
 """
 import argparse
 import ctypes
@@ -23,11 +26,103 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import mpsxx_amd as mx  # noqa: E402
-from mpsxx_amd.workloads import hubbard_twosite  # noqa: E402
+from mpsxx_amd import workloads  # noqa: E402
+from mpsxx_amd.batch import ALL_INPUTS, WFN_ONLY, SigmaBatch, dominant_kernel, sigma_twosite_profile  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+
+
+def sigma_stats(dl, prec):
+    flops = ctypes.c_int64()
+    nbytes = ctypes.c_int64()
+    nsteps = ctypes.c_int32()
+    mx._check(mx.lib().mpsxx_sigma_twosite_stats(*[d.h for d in dl], ctypes.c_int(prec), ctypes.byref(flops),
+                                                 ctypes.byref(nbytes), ctypes.byref(nsteps)))
+    return flops.value, nbytes.value
+
+
+def time_sigma(dl, prec, steps, warmup):
+    """Wall time per sigma application (inputs in HBM, output accumulating as in Davidson's multH)."""
+    out = mx.DTensor()
+    mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
+    for _ in range(max(warmup, 1)):
+        mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
+    mx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
+    mx.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def roofline_of(entries, prec):
+    name, agg, allk = dominant_kernel(entries)
+    per_kernel = {k: dict(ms=v["ms"], launches=v["launches"], flops=v["flops"], bytes=v["bytes"],
+                          avg_launch_us=1e3 * v["ms"] / v["launches"]) for k, v in allk.items()}
+    sec = agg["ms"] * 1e-3
+    if name == "axpy_grouped_kernel":
+        achieved = agg["bytes"] / sec / 1e9
+        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS)
+    else:
+        peak = FP64_PEAK_TFLOPS if prec == mx.FP64 else FP32_PEAK_TFLOPS
+        achieved = agg["flops"] / sec / 1e12
+        roof = dict(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak)
+    roof.update(traffic=None, kernel=name, launches_per_sigma=agg["launches"],
+                avg_launch_us=1e3 * agg["ms"] / agg["launches"], algorithmic_flops=agg["flops"],
+                algorithmic_bytes=agg["bytes"])
+    return roof, per_kernel
+
+
+def rel_diff(a, b):
+    return a.diff_norm(b) / b.norm()
+
+
+def batched_leg(n, reps):
+    specs = workloads.batch_specs(n, 4)
+    probs = [workloads.hubbard_twosite(L=L, M=M, U=U, seed=k) for k, (L, U, M) in enumerate(specs)]
+    dev = [tuple(mx.DTensor(x) for x in p) for p in probs]
+    # plain FP64 path: one compute_sigma_vector call per instance
+    outs = [mx.DTensor() for _ in dev]
+    for d, o in zip(dev, outs):
+        mx.compute_sigma_vector_twosite(*d, sgv0=o)
+    mx.synchronize()
+    ref = [o.to_qst() for o in outs]
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for d, o in zip(dev, outs):
+            mx.compute_sigma_vector_twosite(*d, sgv0=o)
+    mx.synchronize()
+    seq = (time.perf_counter() - t0) / reps
+    res = dict(instances=n, sample_seed=4, specs=[list(s) for s in specs], seq_fp64_ms=seq * 1e3)
+    for prec, name in ((mx.FP64, "fp64"), (mx.FP32, "fp32")):
+        inst = [d + (mx.DTensor(),) for d in dev]
+        b = SigmaBatch(inst, precision=prec)
+        info = b.info()
+        res["flops_per_batch"] = info["flops"]
+        res[name + "_default_ms"] = b.time(reps, ALL_INPUTS, True)
+        res[name + "_policy"] = b.tune(3)
+        res[name + "_graph_ms"] = b.time(reps, ALL_INPUTS, True)
+        if prec == mx.FP32:
+            res[name + "_graph_wfn_ms"] = b.time(reps, WFN_ONLY, True)
+        prof = b.profile(5)
+        res[name + "_roofline"], _ = roofline_of(prof, prec)
+        # accuracy of one application against the plain FP64 path
+        for t in inst:
+            t[5].clear()
+        b.run(ALL_INPUTS, True)
+        mx.synchronize()
+        res[name + "_max_rel_err"] = max(rel_diff(t[5].to_qst(), r) for t, r in zip(inst, ref))
+        b.free()
+    fl = res["flops_per_batch"]
+    for k in list(res):
+        if k.endswith("_ms"):
+            res[k[:-3] + "_tflops"] = fl / (res[k] * 1e-3) / 1e12
+            res[k[:-3] + "_sigma_per_s"] = n / (res[k] * 1e-3)
+    res["speedup_fp32_batched_vs_fp64_sequential"] = res["seq_fp64_ms"] / res["fp32_graph_wfn_ms"]
+    res["speedup_fp64_batched_vs_fp64_sequential"] = res["seq_fp64_ms"] / res["fp64_graph_ms"]
+    return res
 
 
 def main():
@@ -41,6 +136,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--batch", type=int, default=64, help="instances of the batched leg (0: skip it)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -49,59 +145,34 @@ def main():
         sys.exit(2)
 
     prec = mx.FP64 if args.precision == "fp64" else mx.FP32
-    lm, rm, le, re_, psi = hubbard_twosite(args.L, args.M, args.U)
+    lm, rm, le, re_, psi = workloads.hubbard_twosite(args.L, args.M, args.U)
     dl = [mx.DTensor(x) for x in (lm, rm, le, re_, psi)]
-    lib = mx.lib()
+    print("headline sigma: L=%d M=%d %s" % (args.L, args.M, args.precision), file=sys.stderr, flush=True)
+    flops, nbytes = sigma_stats(dl, prec)
+    sec = time_sigma(dl, prec, args.steps, args.warmup)
+    tflops = flops / sec / 1e12
+    entries = sigma_twosite_profile(*dl, precision=prec, reps=50)
+    roof, per_kernel = roofline_of(entries, prec)
 
-    flops = ctypes.c_int64()
-    nbytes = ctypes.c_int64()
-    nsteps = ctypes.c_int32()
-    mx._check(lib.mpsxx_sigma_twosite_stats(*[d.h for d in dl], ctypes.c_int(prec), ctypes.byref(flops),
-                                            ctypes.byref(nbytes), ctypes.byref(nsteps)))
+    # the other precision on the same inputs
+    other = mx.FP32 if prec == mx.FP64 else mx.FP64
+    sec_o = time_sigma(dl, other, args.steps, args.warmup)
+    one = mx.compute_sigma_vector_twosite(*dl, precision=prec).to_qst()
+    one_o = mx.compute_sigma_vector_twosite(*dl, precision=other).to_qst()
+    other_leg = dict(precision="fp32" if other == mx.FP32 else "fp64", ms_per_sigma=sec_o * 1e3,
+                     tflops=flops / sec_o / 1e12, rel_diff_vs_headline=rel_diff(one_o, one))
 
-    # the first application shapes the output (plan built, storage allocated); warmup then runs the accumulate plan
-    out = mx.DTensor()
-    mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
-    for _ in range(max(args.warmup, 1)):
-        mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
-    mx.synchronize()
-    # timed region: K sigma applications (output cleared as Davidson does before each multH)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
-    mx.synchronize()
-    t1 = time.perf_counter()
-    sec = (t1 - t0) / args.steps
-    tflops = flops.value / sec / 1e12
+    batched = None
+    if args.batch > 0:
+        print("batched leg: %d instances" % args.batch, file=sys.stderr, flush=True)
+        batched = batched_leg(args.batch, 20)
 
-    # per-step device times (HIP events on the launch stream)
-    ms = (ctypes.c_double * 16)()
-    sf = (ctypes.c_int64 * 16)()
-    sk = (ctypes.c_int32 * 16)()
-    ns = ctypes.c_int32()
-    mx._check(lib.mpsxx_sigma_twosite_profile(*[d.h for d in dl], ctypes.c_int(prec), ctypes.c_int(50),
-                                              ctypes.c_int32(16), ms, sf, sk, ctypes.byref(ns), None))
-    steps = [dict(kind=["gemm", "axpy", "copy"][sk[i]], ms=ms[i], flops=int(sf[i])) for i in range(ns.value)]
-    dom = max(range(ns.value), key=lambda i: ms[i])
-    dk = steps[dom]
-    peak = FP64_PEAK_TFLOPS if prec == mx.FP64 else FP32_PEAK_TFLOPS
-    if dk["kind"] == "gemm":
-        achieved = dk["flops"] / (dk["ms"] * 1e-3) / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak, traffic=None,
-                    kernel="gemm_grouped_kernel", step=dom)
-    else:
-        achieved = dk["flops"] * 4 / (dk["ms"] * 1e-3) / 1e9  # 2 flops per 8-byte read+... see DESIGN
-        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, kernel="axpy_grouped_kernel", step=dom)
-
-    # parity of this very run against the oracle + CPU baseline (oracle, 1 thread)
     cpu = None
     parity = None
     if not args.no_cpu:
+        print("cpu baseline", file=sys.stderr, flush=True)
         import oracle
         oracle.set_threads(1)
-        got = out.to_qst()
-        ref = None
         reps = 0
         t0 = time.perf_counter()
         while True:
@@ -110,19 +181,14 @@ def main():
             if time.perf_counter() - t0 > args.cpu_seconds or reps >= 200:
                 break
         tc = (time.perf_counter() - t0) / reps
-        cpu = dict(value=flops.value / tc / 1e12, unit="TFLOP/s", cores=1, kind="port",
-                   sample="%d sigma applications of the same workload (%.3f s each)" % (reps, tc),
+        cpu = dict(value=flops / tc / 1e12, unit="TFLOP/s", cores=1, kind="port",
+                   sample="%d sigma applications of the headline workload (%.3f s each)" % (reps, tc),
                    sigma_per_s=1.0 / tc)
-        # the timed output accumulated warmup+steps sigmas onto a cleared tensor: compare one application
-        one = mx.DTensor()
-        mx.compute_sigma_vector_twosite(*dl, sgv0=one, precision=prec)
-        g1 = one.to_qst()
         scale = max(float(np.max(np.abs(b))) for b in ref.blocks.values())
-        parity = dict(same_blocks=g1.block_set() == ref.block_set(), max_abs_diff=g1.max_abs_diff(ref) / scale)
-        del got
+        parity = dict(same_blocks=one.block_set() == ref.block_set(), max_abs_diff=one.max_abs_diff(ref) / scale)
 
     res = {
-        "metric": "H_psi TFLOP/s (two-site sigma vector, FP64 MFMA)",
+        "metric": "H_psi TFLOP/s (two-site sigma vector, block GEMMs on MFMA)",
         "value": tflops,
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -138,12 +204,14 @@ def main():
                    "N": args.L, "twoSz": 0, "mpo": "compact nearest-neighbour (D_w=6)",
                    "psi_blocks": len(psi.blocks), "psi_elems": psi.nelem(), "precision": args.precision},
         "sigma_per_s": 1.0 / sec,
-        "flops_per_sigma": flops.value,
-        "bytes_per_sigma": nbytes.value,
-        "steps_detail": steps,
+        "flops_per_sigma": flops,
+        "bytes_per_sigma": nbytes,
+        "kernels": per_kernel,
         "roofline": roof,
         "cpu_baseline": cpu,
         "parity": parity,
+        "other_precision": other_leg,
+        "batched": batched,
     }
     print(json.dumps(res))
 

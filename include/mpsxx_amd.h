@@ -184,11 +184,14 @@ const char* mpsxx_dmrg_last_error(void);
  * counted over allocated blocks; nsteps = number of device steps of the plan. */
 int mpsxx_sigma_twosite_stats(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
                               mpsxx_tensor wfn0, int precision, int64_t* flops, int64_t* bytes, int32_t* nsteps);
-/* Runs the two-site sigma plan `reps` times into a scratch output and returns, per device step, the mean device time
- * (HIP events on the launch stream), the algorithmic flops and the kind (0 grouped GEMM, 1 coefficient AXPY, 2 copy). */
+/* Times every kernel launch of the two-site sigma plan separately (HIP events on the launch stream around each launch,
+ * mean over `reps`, scratch output) and returns per launch: device ms, its share of the algorithmic flops and bytes,
+ * the kernel (0 gemm_grouped_kernel 64x64, 1 gemm_small_kernel, 2 gemm_split_kernel, 3 axpy_grouped_kernel) and the
+ * plan step it belongs to. */
 int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
-                                mpsxx_tensor wfn0, int precision, int reps, int32_t max_steps, double* step_ms,
-                                int64_t* step_flops, int32_t* step_kind, int32_t* nsteps, void* stream);
+                                mpsxx_tensor wfn0, int precision, int reps, int32_t max_entries, double* ms,
+                                int64_t* flops, int64_t* bytes, int32_t* kernel, int32_t* step, int32_t* nentries,
+                                void* stream);
 
 /* ---- batched two-site sigma vectors (many independent instances per launch) ------------------------------------- */
 /* No single reference counterpart: it runs compute_sigma_vector (src/compute_sigma_vector.hpp:54-70) for n independent
@@ -204,10 +207,10 @@ int mpsxx_batch_sigma_twosite_create(int32_t n, const mpsxx_tensor* lmpo, const 
 int mpsxx_batch_sigma_twosite_run(mpsxx_batch b, uint32_t convert_mask, int use_graph, void* stream);
 /* mean device time (HIP events on `stream`) of one run over `reps` runs; sgv0 accumulates every run */
 int mpsxx_batch_time(mpsxx_batch b, int32_t reps, uint32_t convert_mask, int use_graph, double* ms, void* stream);
-/* per launch group (0 = FP32 input conversion, 1.. = contraction steps): mean device time over `reps` eager runs and
- * algorithmic flops; sgv0 accumulates every run */
-int mpsxx_batch_profile(mpsxx_batch b, int32_t reps, int32_t max_steps, double* step_ms, int64_t* step_flops,
-                        int32_t* nsteps, void* stream);
+/* Times every kernel launch separately (as mpsxx_sigma_twosite_profile; outputs redirected to scratch): per launch the
+ * device ms, algorithmic flops and bytes, kernel id and launch-group index (FP32 input conversion excluded). */
+int mpsxx_batch_profile(mpsxx_batch b, int32_t reps, int32_t max_entries, double* ms, int64_t* flops, int64_t* bytes,
+                        int32_t* kernel, int32_t* step, int32_t* nentries, void* stream);
 /* Autotuner: times candidate tile policies per step on the batch's own task lists (outputs redirected to scratch) and
  * keeps the fastest; policy[3*k..3*k+2] = {small-block limit, split threshold, grid cap} of step k. */
 int mpsxx_batch_tune(mpsxx_batch b, int32_t reps, int32_t max_steps, int32_t* policy, void* stream);

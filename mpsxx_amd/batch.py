@@ -36,14 +36,8 @@ class SigmaBatch:
         return ms.value
 
     def profile(self, reps):
-        """[(ms, flops)] per launch group: FP32 input conversion first, then each contraction step."""
-        n = 16
-        ms = np.zeros(n)
-        fl = np.zeros(n, dtype=np.int64)
-        ns = ctypes.c_int32()
-        _check(lib().mpsxx_batch_profile(self.h, ctypes.c_int32(reps), ctypes.c_int32(n), ms.ctypes.data_as(VP),
-                                         fl.ctypes.data_as(VP), ctypes.byref(ns), None))
-        return [(float(ms[k]), int(fl[k])) for k in range(ns.value)]
+        """Per kernel launch: dict(kernel, step, ms, flops, bytes) (outputs redirected to scratch)."""
+        return _profile_entries(lambda *a: lib().mpsxx_batch_profile(self.h, ctypes.c_int32(reps), *a, None))
 
     def tune(self, reps=5):
         n = 16
@@ -75,6 +69,37 @@ class SigmaBatch:
             pass
 
 
-def upload_instance(w):
-    """Device tensors (lmpo, rmpo, lopr, ropr, wfn0, empty sgv0) of a workload dict with QST entries."""
-    return tuple(DTensor(w[k]) for k in ("lmpo", "rmpo", "lopr", "ropr", "wfn")) + (DTensor(),)
+KERNELS = ["gemm_grouped_kernel", "gemm_small_kernel", "gemm_split_kernel", "axpy_grouped_kernel"]
+
+
+def _profile_entries(call, cap=256):
+    ms = np.zeros(cap)
+    fl = np.zeros(cap, dtype=np.int64)
+    by = np.zeros(cap, dtype=np.int64)
+    kn = np.zeros(cap, dtype=np.int32)
+    st = np.zeros(cap, dtype=np.int32)
+    n = ctypes.c_int32()
+    _check(call(ctypes.c_int32(cap), ms.ctypes.data_as(VP), fl.ctypes.data_as(VP), by.ctypes.data_as(VP),
+                kn.ctypes.data_as(VP), st.ctypes.data_as(VP), ctypes.byref(n)))
+    return [dict(kernel=KERNELS[kn[i]], step=int(st[i]), ms=float(ms[i]), flops=int(fl[i]), bytes=int(by[i]))
+            for i in range(n.value)]
+
+
+def sigma_twosite_profile(lmpo, rmpo, lopr, ropr, wfn0, precision=FP64, reps=50):
+    """Per kernel launch of the single-instance two-site sigma plan (see mpsxx_sigma_twosite_profile)."""
+    hs = [t.h for t in (lmpo, rmpo, lopr, ropr, wfn0)]
+    return _profile_entries(lambda *a: lib().mpsxx_sigma_twosite_profile(*hs, ctypes.c_int(precision),
+                                                                         ctypes.c_int(reps), *a, None))
+
+
+def dominant_kernel(entries):
+    """Aggregate launches per kernel; return (name, dict(ms, flops, bytes, launches)) of the one with most time."""
+    agg = {}
+    for e in entries:
+        a = agg.setdefault(e["kernel"], dict(ms=0.0, flops=0, bytes=0, launches=0))
+        a["ms"] += e["ms"]
+        a["flops"] += e["flops"]
+        a["bytes"] += e["bytes"]
+        a["launches"] += 1
+    name = max(agg, key=lambda k: agg[k]["ms"])
+    return name, agg[name], agg

@@ -184,6 +184,7 @@ void BatchSigma::drop_graphs() {
 void BatchSigma::build_step_lists(int k) {
   Step& st = launches_[k].st;
   st.max_groups = pol_[k].max_groups;
+  st.policy = pol_[k];
   if (st.kind != Step::GEMM) return;
   std::vector<GemmTile> big, small, split;
   gemm_tiles(st.host.gtasks, st.host.gpairs, pol_[k], big, small, split);
@@ -266,7 +267,7 @@ void BatchSigma::upload_tables(hipStream_t s, const std::vector<void*>* out_over
   shadow_.swap(buf);
 }
 
-void BatchSigma::launch_one(int k, hipStream_t s) {
+void BatchSigma::launch_one(int k, hipStream_t s, unsigned parts) {
   const Launch& L = launches_[k];
   const Step& st = L.st;
   const Group& g = groups_[L.group];
@@ -275,7 +276,7 @@ void BatchSigma::launch_one(int k, hipStream_t s) {
   const int b2 = st.b2 >= 0 ? st.b2 : (st.b >= 0 ? st.b : st.a);
   Operands ops{T + (size_t)st.a * nm, st.b >= 0 ? T + (size_t)st.b * nm : nullptr, T + (size_t)b2 * nm,
                T + (size_t)st.c * nm, 1};
-  launch_step(st, ops, L.f32, L.cf32, s);
+  launch_step(st, ops, L.f32, L.cf32, s, parts);
 }
 
 void BatchSigma::enqueue(unsigned convert_mask, hipStream_t s, const std::vector<hipEvent_t>* events) {
@@ -327,42 +328,31 @@ void BatchSigma::run(unsigned convert_mask, bool use_graph, hipStream_t s) {
 
 double BatchSigma::time(int reps, unsigned convert_mask, bool use_graph, hipStream_t s) {
   if (reps <= 0) throw Error(MPSXX_ERR_ARG, "batch time: reps must be positive");
-  run(convert_mask, use_graph, s);  // warm-up (and graph capture)
-  hipEvent_t e0, e1;
-  hip_check(hipEventCreate(&e0), "event");
-  hip_check(hipEventCreate(&e1), "event");
-  hip_check(hipEventRecord(e0, s), "event");
-  for (int r = 0; r < reps; ++r) run(convert_mask, use_graph, s);
-  hip_check(hipEventRecord(e1, s), "event");
-  hip_check(hipEventSynchronize(e1), "event sync");
-  float ms = 0.f;
-  hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return ms / reps;
+  // the untimed first call also captures the graph
+  return time_launches([&] { run(convert_mask, use_graph, s); }, reps, s);
 }
 
-std::vector<double> BatchSigma::profile(int reps, hipStream_t s) {
+std::vector<PartTiming> BatchSigma::profile(int reps, hipStream_t s) {
   if (reps <= 0) throw Error(MPSXX_ERR_ARG, "batch profile: reps must be positive");
-  const int ne = (int)launches_.size() + 2;
-  std::vector<hipEvent_t> ev(ne);
-  for (auto& e : ev) hip_check(hipEventCreate(&e), "event");
-  std::vector<double> acc(ne - 1, 0.0);
   check_structures(s);
-  upload_tables(s, nullptr);
-  enqueue(~0u, s, nullptr);
-  for (int r = 0; r < reps; ++r) {
-    enqueue(~0u, s, &ev);
-    hip_check(hipEventSynchronize(ev[ne - 1]), "event sync");
-    for (int k = 0; k + 1 < ne; ++k) {
-      float ms = 0.f;
-      hip_check(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "elapsed");
-      acc[k] += ms;
-    }
+  std::vector<DevBuf> scratch(n_);
+  std::vector<void*> outp(n_);
+  for (int i = 0; i < n_; ++i) {
+    scratch[i].alloc(std::max<int64_t>(plans_[i].out.nelem, 1) * sizeof(double));
+    outp[i] = scratch[i].p;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  for (double& a : acc) a /= reps;
-  return acc;
+  upload_tables(s, &outp);
+  enqueue(~0u, s, nullptr);
+  std::vector<PartTiming> res;
+  for (int k = 0; k < (int)launches_.size(); ++k)
+    for (const KernelPart& kp : step_parts(launches_[k].st)) {
+      const double ms = time_launches([&] { launch_one(k, s, kp.mask); }, reps, s);
+      res.push_back({k, kp, ms});
+    }
+  hip_check(hipStreamSynchronize(s), "profile sync");
+  upload_tables(s, nullptr);
+  hip_check(hipStreamSynchronize(s), "profile sync");
+  return res;
 }
 
 std::vector<GemmPolicy> BatchSigma::tune(int reps, hipStream_t s) {
@@ -377,19 +367,7 @@ std::vector<GemmPolicy> BatchSigma::tune(int reps, hipStream_t s) {
   }
   upload_tables(s, &outp);
   enqueue(~0u, s, nullptr);  // every intermediate now holds real data
-  hipEvent_t e0, e1;
-  hip_check(hipEventCreate(&e0), "event");
-  hip_check(hipEventCreate(&e1), "event");
-  auto time_step = [&](int k) {
-    launch_one(k, s);
-    hip_check(hipEventRecord(e0, s), "event");
-    for (int r = 0; r < reps; ++r) launch_one(k, s);
-    hip_check(hipEventRecord(e1, s), "event");
-    hip_check(hipEventSynchronize(e1), "event sync");
-    float ms = 0.f;
-    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-    return (double)ms / reps;
-  };
+  auto time_step = [&](int k) { return time_launches([&] { launch_one(k, s); }, reps, s); };
   const int grid_caps[] = {0, 1024, 2048, 4096, 8192};
   for (int k = 0; k < (int)launches_.size(); ++k) {
     std::vector<GemmPolicy> cand;
@@ -418,8 +396,6 @@ std::vector<GemmPolicy> BatchSigma::tune(int reps, hipStream_t s) {
     pol_[k] = bp;
     build_step_lists(k);
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   hip_check(hipStreamSynchronize(s), "tune sync");
   upload_tables(s, nullptr);
   hip_check(hipStreamSynchronize(s), "tune sync");

@@ -31,8 +31,8 @@ class BatchSigma {
 
   // Average device time of one run over `reps` runs (HIP events on s).
   double time(int reps, unsigned convert_mask, bool use_graph, hipStream_t s);
-  // Device time of each launch group (conversion, then each step) averaged over `reps` eager runs.
-  std::vector<double> profile(int reps, hipStream_t s);
+  // Device time and algorithmic cost of every kernel launch (outputs redirected to scratch).
+  std::vector<PartTiming> profile(int reps, hipStream_t s);
 
   // Autotuner: for every step, time each candidate tile policy (block-size cut between the 32x32 and 64x64 kernels,
   // K-split threshold, grid cap) on the real task lists and keep the fastest.  Outputs are redirected to scratch while
@@ -66,7 +66,7 @@ class BatchSigma {
   void check_structures(hipStream_t s);
   void upload_tables(hipStream_t s, const std::vector<void*>* out_override);
   void enqueue(unsigned convert_mask, hipStream_t s, const std::vector<hipEvent_t>* events);
-  void launch_one(int k, hipStream_t s);
+  void launch_one(int k, hipStream_t s, unsigned parts = 7u);
   void build_step_lists(int k);
   void drop_graphs();
   std::vector<uint64_t> structure_key() const;

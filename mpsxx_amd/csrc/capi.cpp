@@ -350,32 +350,25 @@ int mpsxx_sigma_twosite_stats(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor
 }
 
 int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
-                                mpsxx_tensor wfn0, int precision, int reps, int32_t max_steps, double* step_ms,
-                                int64_t* step_flops, int32_t* step_kind, int32_t* nsteps, void* stream) {
+                                mpsxx_tensor wfn0, int precision, int reps, int32_t max_entries, double* ms,
+                                int64_t* flops, int64_t* bytes, int32_t* kernel, int32_t* step, int32_t* nentries,
+                                void* stream) {
   return guard([&] {
+    if (reps <= 0) throw mpx::Error(MPSXX_ERR_ARG, "profile: reps must be positive");
     mpx::Plan& P = mpx::plan_sigma_twosite(T(lmpo).L, T(rmpo).L, T(lopr).L, T(ropr).L, T(wfn0).L, nullptr, precision);
-    const int ns = (int)P.steps.size();
-    if (nsteps) *nsteps = ns;
-    if (ns > max_steps) throw mpx::Error(MPSXX_ERR_ARG, "profile: max_steps too small");
     mpx::DevBuf out(std::max<int64_t>(P.out.nelem, 1) * sizeof(double));
-    std::vector<hipEvent_t> ev(ns + 1);
-    for (auto& e : ev) mpx::hip_check(hipEventCreate(&e), "event");
-    std::vector<double> acc(ns, 0.0);
     std::vector<const double*> in{T(lmpo).ptr(), T(rmpo).ptr(), T(lopr).ptr(), T(ropr).ptr(), T(wfn0).ptr()};
-    for (int r = 0; r < reps; ++r) {
-      mpx::run_plan_timed(P, in, out.as<double>(), S(stream), ev);
-      mpx::hip_check(hipEventSynchronize(ev[ns]), "event sync");
-      for (int i = 0; i < ns; ++i) {
-        float ms = 0.f;
-        mpx::hip_check(hipEventElapsedTime(&ms, ev[i], ev[i + 1]), "elapsed");
-        acc[i] += ms;
-      }
-    }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    for (int i = 0; i < ns; ++i) {
-      if (step_ms) step_ms[i] = acc[i] / std::max(reps, 1);
-      if (step_flops) step_flops[i] = P.steps[i].flops;
-      if (step_kind) step_kind[i] = (int32_t)P.steps[i].kind;
+    // one full run converts the FP32 inputs and fills every intermediate
+    mpx::run_plan(P, in, out.as<double>(), S(stream));
+    std::vector<mpx::PartTiming> t = mpx::profile_plan_parts(P, in, out.as<double>(), reps, S(stream));
+    if (nentries) *nentries = (int32_t)t.size();
+    if ((int)t.size() > max_entries) throw mpx::Error(MPSXX_ERR_ARG, "profile: max_entries too small");
+    for (size_t i = 0; i < t.size(); ++i) {
+      if (ms) ms[i] = t[i].ms;
+      if (flops) flops[i] = t[i].part.flops;
+      if (bytes) bytes[i] = t[i].part.bytes;
+      if (kernel) kernel[i] = t[i].part.kind;
+      if (step) step[i] = t[i].step;
     }
   });
 }
@@ -414,17 +407,18 @@ int mpsxx_batch_time(mpsxx_batch b, int32_t reps, uint32_t convert_mask, int use
   });
 }
 
-int mpsxx_batch_profile(mpsxx_batch b, int32_t reps, int32_t max_steps, double* step_ms, int64_t* step_flops,
-                        int32_t* nsteps, void* stream) {
+int mpsxx_batch_profile(mpsxx_batch b, int32_t reps, int32_t max_entries, double* ms, int64_t* flops, int64_t* bytes,
+                        int32_t* kernel, int32_t* step, int32_t* nentries, void* stream) {
   return guard([&] {
-    mpx::BatchSigma& x = B(b);
-    const int ns = x.nsteps() + 1;
-    if (nsteps) *nsteps = ns;
-    if (ns > max_steps) throw mpx::Error(MPSXX_ERR_ARG, "batch profile: max_steps too small");
-    const std::vector<double> t = x.profile(reps, S(stream));
-    for (int k = 0; k < ns; ++k) {
-      if (step_ms) step_ms[k] = t[k];
-      if (step_flops) step_flops[k] = k == 0 ? 0 : x.step_flops(k - 1);
+    const std::vector<mpx::PartTiming> t = B(b).profile(reps, S(stream));
+    if (nentries) *nentries = (int32_t)t.size();
+    if ((int)t.size() > max_entries) throw mpx::Error(MPSXX_ERR_ARG, "batch profile: max_entries too small");
+    for (size_t i = 0; i < t.size(); ++i) {
+      if (ms) ms[i] = t[i].ms;
+      if (flops) flops[i] = t[i].part.flops;
+      if (bytes) bytes[i] = t[i].part.bytes;
+      if (kernel) kernel[i] = t[i].part.kind;
+      if (step) step[i] = t[i].step;
     }
   });
 }

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <unordered_map>
 
@@ -497,6 +498,7 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
   std::vector<GemmTile> tiles, small, split;
   const GemmPolicy pol = default_policy();
   gemm_tiles(sym.tasks, sym.pairs, pol, tiles, small, split);
+  st.policy = pol;
   st.max_groups = pol.max_groups;
   st.nwork = (int)tiles.size();
   st.work = upload_vec(tiles);
@@ -634,25 +636,74 @@ Step make_copy_step(const std::vector<CopySeg>& segs, int x, int y, double alpha
 
 namespace {
 template <typename T, typename TC>
-void launch_step_t(const Step& st, const Operands& ops, hipStream_t s) {
+void launch_step_t(const Step& st, const Operands& ops, unsigned parts, hipStream_t s) {
   if (st.kind == Step::GEMM) {
-    launch_gemm<T, TC>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops,
-                       st.alpha, st.accumulate, s, st.max_groups);
-    launch_gemm_small<T, TC>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
-                             st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
-    launch_gemm_split<T, TC>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
-                             st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
-  } else {
+    if (parts & 1u)
+      launch_gemm<T, TC>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops,
+                         st.alpha, st.accumulate, s, st.max_groups);
+    if (parts & 2u)
+      launch_gemm_small<T, TC>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
+                               st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
+    if (parts & 4u)
+      launch_gemm_split<T, TC>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
+                               st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
+  } else if (parts & 1u) {
     launch_axpy<T, TC>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(), st.pairs.as<AxpyPair>(), ops,
                        st.alpha, st.accumulate, s, st.max_groups);
   }
 }
 }  // namespace
 
-void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s) {
-  if (!f32) launch_step_t<double, double>(st, ops, s);
-  else if (cf32) launch_step_t<float, float>(st, ops, s);
-  else launch_step_t<float, double>(st, ops, s);
+void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s, unsigned parts) {
+  if (!f32) launch_step_t<double, double>(st, ops, parts, s);
+  else if (cf32) launch_step_t<float, float>(st, ops, parts, s);
+  else launch_step_t<float, double>(st, ops, parts, s);
+}
+
+std::vector<KernelPart> step_parts(const Step& st) {
+  std::vector<KernelPart> out;
+  if (st.kind == Step::AXPY) {
+    int64_t fl = 0, by = 0;
+    for (const AxpyTask& t : st.host.atasks) {
+      fl += 2ll * t.len * t.pair_count;
+      by += 8ll * t.len * (t.pair_count + 1);
+    }
+    if (st.nwork > 0) out.push_back({KernelPart::AXPY, 1u, st.nwork, fl, by});
+    return out;
+  }
+  if (st.kind != Step::GEMM) return out;
+  // exact split of the step's flops over the three tile lists: a tile covers rows x cols of its block
+  auto list_cost = [&](const std::vector<GemmTile>& tiles, int edge, int64_t& fl, int64_t& by) {
+    fl = by = 0;
+    for (const GemmTile& tl : tiles) {
+      const GemmTask& t = st.host.gtasks[tl.task];
+      const int64_t r = std::min(edge, t.m - tl.row0), c = std::min(edge, t.n - tl.col0);
+      int64_t ks = 0;
+      for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) ks += st.host.gpairs[p].k;
+      fl += 2 * r * c * ks;
+      by += 8 * ((r + c) * ks + r * c);
+    }
+  };
+  std::vector<GemmTile> big, small, split;
+  gemm_tiles(st.host.gtasks, st.host.gpairs, st.policy, big, small, split);
+  int64_t fl, by;
+  list_cost(big, kGemmTile, fl, by);
+  if (!big.empty()) out.push_back({KernelPart::GEMM64, 1u, (int)big.size(), fl, by});
+  list_cost(small, kSmallTile, fl, by);
+  if (!small.empty()) out.push_back({KernelPart::GEMM_SMALL, 2u, (int)small.size(), fl, by});
+  list_cost(split, kSmallTile, fl, by);
+  if (!split.empty()) out.push_back({KernelPart::GEMM_SPLIT, 4u, (int)split.size(), fl, by});
+  return out;
+}
+
+const char* part_kernel_name(int kind) {
+  switch (kind) {
+    case KernelPart::GEMM64: return "gemm_grouped_kernel";
+    case KernelPart::GEMM_SMALL: return "gemm_small_kernel";
+    case KernelPart::GEMM_SPLIT: return "gemm_split_kernel";
+    case KernelPart::AXPY: return "axpy_grouped_kernel";
+    default: return "copy_kernel";
+  }
 }
 
 void run_plan(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s) {
@@ -664,8 +715,8 @@ void run_plan_timed(Plan& P, const std::vector<const double*>& inputs, double* o
   run_plan_ex(P, inputs, out, s, events, ~0u);
 }
 
-void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
-                 const std::vector<hipEvent_t>& events, unsigned convert_mask) {
+std::vector<void*> slot_pointers(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
+                                 unsigned convert_mask) {
   const int ns = (int)P.slot_layout.size();
   std::vector<void*> ptr(ns, nullptr);
   for (int i = 0; i < ns; ++i) {
@@ -684,17 +735,28 @@ void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out,
       ptr[i] = P.slot_buf[i].p;
     }
   }
+  return ptr;
+}
+
+namespace {
+Operands step_operands(const Step& st, const std::vector<void*>& ptr) {
+  return Operands{ptr[st.a], st.b >= 0 ? ptr[st.b] : nullptr, ptr[st.b2 >= 0 ? st.b2 : (st.b >= 0 ? st.b : st.a)],
+                  ptr[st.c], 0};
+}
+}  // namespace
+
+void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
+                 const std::vector<hipEvent_t>& events, unsigned convert_mask) {
+  std::vector<void*> ptr = slot_pointers(P, inputs, out, s, convert_mask);
   for (size_t si = 0; si < P.steps.size(); ++si) {
     Step& st = P.steps[si];
     if (!events.empty()) hip_check(hipEventRecord(events[si], s), "event");
     const bool f32 = P.slot_prec[st.a] == 1;
     const bool cf32 = P.slot_prec[st.c] == 1;
-    Operands ops{ptr[st.a], st.b >= 0 ? ptr[st.b] : nullptr, ptr[st.b2 >= 0 ? st.b2 : (st.b >= 0 ? st.b : st.a)],
-                 ptr[st.c], 0};
     switch (st.kind) {
       case Step::GEMM:
       case Step::AXPY:
-        launch_step(st, ops, f32, cf32, s);
+        launch_step(st, step_operands(st, ptr), f32, cf32, s);
         break;
       case Step::COPY:
         if (!f32 && !cf32)
@@ -714,6 +776,39 @@ void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out,
   }
   if (!events.empty()) hip_check(hipEventRecord(events[P.steps.size()], s), "event");
   hip_check(hipGetLastError(), "kernel launch");
+}
+
+double time_launches(const std::function<void()>& launch, int reps, hipStream_t s) {
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  launch();  // warm
+  hip_check(hipEventRecord(e0, s), "event");
+  for (int r = 0; r < reps; ++r) launch();
+  hip_check(hipEventRecord(e1, s), "event");
+  hip_check(hipEventSynchronize(e1), "event sync");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return (double)ms / reps;
+}
+
+std::vector<PartTiming> profile_plan_parts(Plan& P, const std::vector<const double*>& inputs, double* out, int reps,
+                                           hipStream_t s) {
+  std::vector<void*> ptr = slot_pointers(P, inputs, out, s, 0u);
+  std::vector<PartTiming> res;
+  for (size_t si = 0; si < P.steps.size(); ++si) {
+    const Step& st = P.steps[si];
+    if (st.kind == Step::COPY) continue;
+    const bool f32 = P.slot_prec[st.a] == 1, cf32 = P.slot_prec[st.c] == 1;
+    const Operands ops = step_operands(st, ptr);
+    for (const KernelPart& kp : step_parts(st)) {
+      const double ms = time_launches([&] { launch_step(st, ops, f32, cf32, s, kp.mask); }, reps, s);
+      res.push_back({(int)si, kp, ms});
+    }
+  }
+  return res;
 }
 
 const char* kind_name(int k) { return k == Step::GEMM ? "gemm" : k == Step::AXPY ? "axpy" : "copy"; }

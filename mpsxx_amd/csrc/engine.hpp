@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -114,9 +115,22 @@ struct Step {
   int nwork_split = 0;         // GEMM: 32x32 tiles with long reductions (gemm_split_kernel)
   DevBuf work_split;
   int max_groups = 0;          // grid cap of the step's launches (0: one work-group per work item)
+  GemmPolicy policy;           // GEMM: how the blocks were mapped to the tile lists
   int64_t flops = 0;
   HostLists host;
 };
+
+// One kernel launch of a step, with its share of the step's algorithmic cost: flops 2*rows*cols*k summed over the
+// tiles of the launch (GEMM) or 2*len per pair (AXPY); bytes = 8 * (operand elements read once per tile + outputs).
+struct KernelPart {
+  enum Kind { GEMM64 = 0, GEMM_SMALL = 1, GEMM_SPLIT = 2, AXPY = 3 };
+  int kind;
+  unsigned mask;  // `parts` bit of launch_step
+  int items;
+  int64_t flops, bytes;
+};
+std::vector<KernelPart> step_parts(const Step& st);
+const char* part_kernel_name(int kind);
 
 // Slot table: inputs are bound per call; intermediates are owned by the plan.
 struct Plan {
@@ -140,8 +154,9 @@ Step make_copy_step(const std::vector<CopySeg>& segs, int x, int y, double alpha
 bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, int y, int w1, int w2, int z,
                           int accumulate, Step& out);
 
-// Launch the kernels of one GEMM or AXPY step (FP64, FP32, or FP32 operands into an FP64 output).
-void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s);
+// Launch the kernels of one GEMM or AXPY step (FP64, FP32, or FP32 operands into an FP64 output); `parts` selects
+// the tile lists (bit 0: 64x64 tiles or the AXPY chunks, bit 1: small tiles, bit 2: split tiles).
+void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s, unsigned parts = 7u);
 
 // Execute a plan: inputs[i] are the FP64 device pointers of the bound inputs, out the FP64 output pointer.
 void run_plan(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s);
@@ -152,6 +167,22 @@ void run_plan_timed(Plan& P, const std::vector<const double*>& inputs, double* o
 // made by an earlier call with the same input contents).
 void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
                  const std::vector<hipEvent_t>& events, unsigned convert_mask);
+
+// Device pointers of every slot of a plan for one call (FP32 plans convert the inputs whose bit is set).
+std::vector<void*> slot_pointers(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
+                                 unsigned convert_mask);
+
+// Mean device time (HIP events on s) of `launch` over reps calls, after one untimed call.
+double time_launches(const std::function<void()>& launch, int reps, hipStream_t s);
+
+// Per kernel launch of a plan: device time and algorithmic cost (the slots must hold data from an earlier run).
+struct PartTiming {
+  int step;
+  KernelPart part;
+  double ms;
+};
+std::vector<PartTiming> profile_plan_parts(Plan& P, const std::vector<const double*>& inputs, double* out, int reps,
+                                           hipStream_t s);
 
 const char* kind_name(int k);
 

@@ -34,13 +34,15 @@ template <> struct Mfma<float> {
 
 constexpr int BM = 64, BN = 64, BK = 16, LDP = 64 + 1;
 
-// operand base of the task's instance
-template <typename T>
-__device__ __forceinline__ T* opnd(const void* p, int table, int inst) {
-  return table ? reinterpret_cast<T* const*>(p)[inst] : reinterpret_cast<T*>(const_cast<void*>(p));
+// operand base of the task's instance: Tab = the launch passes per-instance pointer tables (batched launches, which
+// are distinct kernel symbols in profiles)
+template <typename T, bool Tab>
+__device__ __forceinline__ T* opnd(const void* p, int inst) {
+  if constexpr (Tab) return reinterpret_cast<T* const*>(p)[inst];
+  else return reinterpret_cast<T*>(const_cast<void*>(p));
 }
 
-template <typename T, typename TC>
+template <typename T, typename TC, bool Tab>
 __global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __restrict__ tiles, int ntiles,
                                                            const GemmTask* __restrict__ tasks,
                                                            const GemmPair* __restrict__ pairs, Operands ops,
@@ -52,9 +54,9 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __res
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const GemmTile tl = tiles[w];
     const GemmTask tk = tasks[tl.task];
-    const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
-    const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
-    TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
+    const T* __restrict__ A = opnd<const T, Tab>(ops.a, tk.inst);
+    const T* __restrict__ B = opnd<const T, Tab>(ops.b, tk.inst);
+    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __res
 // Small-block variant: one wave per 32x32 C tile (2x2 MFMA 16x16 blocks), operands read straight from global memory
 // into the MFMA source registers (lane l takes A[row + l%16][k + l/16] and B[k + l/16][col + l%16]); no LDS and no
 // barriers, so thousands of tiny blocks run at full occupancy.  Four independent waves per work-group.
-template <typename T, typename TC>
+template <typename T, typename TC, bool Tab>
 __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restrict__ tiles, int ntiles,
                                                          const GemmTask* __restrict__ tasks,
                                                          const GemmPair* __restrict__ pairs, Operands ops,
@@ -154,9 +156,9 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
   for (int ti = blockIdx.x * 4 + wave; ti < ntiles; ti += gridDim.x * 4) {
     const GemmTile tl = tiles[ti];
     const GemmTask tk = tasks[tl.task];
-    const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
-    const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
-    TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
+    const T* __restrict__ A = opnd<const T, Tab>(ops.a, tk.inst);
+    const T* __restrict__ B = opnd<const T, Tab>(ops.b, tk.inst);
+    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
     acc_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
 // K-split variant for 32x32 tiles with a long reduction (many pairs and/or large k): the four waves of a work-group
 // take every fourth (pair, 16-wide k panel) unit of the same tile, then the partial tiles are summed in LDS in a
 // fixed order (deterministic) and stored once.
-template <typename T, typename TC>
+template <typename T, typename TC, bool Tab>
 __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restrict__ tiles, int ntiles,
                                                          const GemmTask* __restrict__ tasks,
                                                          const GemmPair* __restrict__ pairs, Operands ops,
@@ -231,9 +233,9 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
     const GemmTile tl = tiles[w];
     const GemmTask tk = tasks[tl.task];
-    const T* __restrict__ A = opnd<const T>(ops.a, ops.table, tk.inst);
-    const T* __restrict__ B = opnd<const T>(ops.b, ops.table, tk.inst);
-    TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
+    const T* __restrict__ A = opnd<const T, Tab>(ops.a, tk.inst);
+    const T* __restrict__ B = opnd<const T, Tab>(ops.b, tk.inst);
+    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
     acc_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
   }
 }
 
-template <typename T, typename TC>
+template <typename T, typename TC, bool Tab>
 __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restrict__ chunks, int nchunks,
                                                            const AxpyTask* __restrict__ tasks,
                                                            const AxpyPair* __restrict__ pairs, Operands ops,
@@ -320,10 +322,10 @@ __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restri
   for (int w = blockIdx.x; w < nchunks; w += gridDim.x) {
     const Chunk ch = chunks[w];
     const AxpyTask tk = tasks[ch.task];
-    const T* __restrict__ V = opnd<const T>(ops.a, ops.table, tk.inst);
-    const T* __restrict__ S = opnd<const T>(ops.b, ops.table, tk.inst);
-    const T* __restrict__ S2 = opnd<const T>(ops.b2, ops.table, tk.inst);
-    TC* __restrict__ C = opnd<TC>(ops.c, ops.table, tk.inst);
+    const T* __restrict__ V = opnd<const T, Tab>(ops.a, tk.inst);
+    const T* __restrict__ S = opnd<const T, Tab>(ops.b, tk.inst);
+    const T* __restrict__ S2 = opnd<const T, Tab>(ops.b2, tk.inst);
+    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
     const int pend = tk.pair_begin + tk.pair_count;
     constexpr int E = kAxpyChunk / 256;  // elements per thread; the E loads of one pair are independent
     const int end = min(tk.len, ch.start + kAxpyChunk);
@@ -491,16 +493,24 @@ template <typename T, typename TC>
 void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
                  double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((gemm_grouped_kernel<T, TC>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles, ntiles,
-                     tasks, pairs, ops, alpha, accumulate);
+  if (ops.table)
+    hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
+                       ntiles, tasks, pairs, ops, alpha, accumulate);
+  else
+    hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
+                       ntiles, tasks, pairs, ops, alpha, accumulate);
 }
 
 template <typename T, typename TC>
 void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, Operands ops,
                  double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (nchunks <= 0) return;
-  hipLaunchKernelGGL((axpy_grouped_kernel<T, TC>), dim3(groups(nchunks, max_groups)), dim3(256), 0, s, chunks,
-                     nchunks, tasks, pairs, ops, alpha, accumulate);
+  if (ops.table)
+    hipLaunchKernelGGL((axpy_grouped_kernel<T, TC, true>), dim3(groups(nchunks, max_groups)), dim3(256), 0, s, chunks,
+                       nchunks, tasks, pairs, ops, alpha, accumulate);
+  else
+    hipLaunchKernelGGL((axpy_grouped_kernel<T, TC, false>), dim3(groups(nchunks, max_groups)), dim3(256), 0, s, chunks,
+                       nchunks, tasks, pairs, ops, alpha, accumulate);
 }
 
 template <typename TX, typename TY>
@@ -579,16 +589,24 @@ template <typename T, typename TC>
 void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
                        double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((gemm_small_kernel<T, TC>), dim3(groups((ntiles + 3) / 4, max_groups)), dim3(256), 0, s, tiles,
-                     ntiles, tasks, pairs, ops, alpha, accumulate);
+  if (ops.table)
+    hipLaunchKernelGGL((gemm_small_kernel<T, TC, true>), dim3(groups((ntiles + 3) / 4, max_groups)), dim3(256), 0, s,
+                       tiles, ntiles, tasks, pairs, ops, alpha, accumulate);
+  else
+    hipLaunchKernelGGL((gemm_small_kernel<T, TC, false>), dim3(groups((ntiles + 3) / 4, max_groups)), dim3(256), 0, s,
+                       tiles, ntiles, tasks, pairs, ops, alpha, accumulate);
 }
 
 template <typename T, typename TC>
 void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
                        double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL((gemm_split_kernel<T, TC>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles, ntiles,
-                     tasks, pairs, ops, alpha, accumulate);
+  if (ops.table)
+    hipLaunchKernelGGL((gemm_split_kernel<T, TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
+                       ntiles, tasks, pairs, ops, alpha, accumulate);
+  else
+    hipLaunchKernelGGL((gemm_split_kernel<T, TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
+                       ntiles, tasks, pairs, ops, alpha, accumulate);
 }
 
 #define MPX_INST_GEMM(FN, T, TC) \
