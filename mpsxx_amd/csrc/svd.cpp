@@ -10,6 +10,7 @@
 // total 0 (backward sweep).  Blocks of the row (column) sectors keep the merged-sector order of QSTmergeInfo.
 #include <algorithm>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include <rocblas/rocblas.h>
@@ -95,6 +96,9 @@ double op_canonicalize_twosite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor&
   // row-major A (m x n) is column-major A^T (n x m): its left vectors are the rows of Vt, its right ones give U
   rocblas_handle h = solver_handle();
   rocblas_set_stream(h, s);
+  // gesvd destroys A; keep the input for the (rare) blocks whose bidiagonal QR iteration does not converge
+  DevBuf A0(nb * sa * sizeof(double));
+  hip_check(hipMemcpyAsync(A0.p, A.p, nb * sa * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
   rocblas_status st = rocsolver_dgesvd_strided_batched(
       h, rocblas_svect_singular, rocblas_svect_singular, nmax, mmax, A.as<double>(), nmax, sa, Sv.as<double>(), kmax,
       U.as<double>(), nmax, su, V.as<double>(), kmax, sv, E.as<double>(), std::max(kmax, 2), rocblas_outofplace,
@@ -105,8 +109,40 @@ double op_canonicalize_twosite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor&
   hip_check(hipMemcpyAsync(sv_h.data(), Sv.p, sv_h.size() * sizeof(double), hipMemcpyDeviceToHost, s), "S");
   hip_check(hipMemcpyAsync(info_h.data(), info.p, nb * sizeof(rocblas_int), hipMemcpyDeviceToHost, s), "info");
   hip_check(hipStreamSynchronize(s), "svd sync");
+  // blocks gesvd did not converge on (seen on small rank-deficient blocks of unconverged early-sweep states) are
+  // decomposed again with the one-sided Jacobi driver, in place in their batch slots
+  for (int k = 0; k < nb; ++k) {
+    if (info_h[k] == 0) continue;
+    DevBuf res(sizeof(double)), nsw(sizeof(rocblas_int)), inf(sizeof(rocblas_int));
+    st = rocsolver_dgesvdj_strided_batched(h, rocblas_svect_singular, rocblas_svect_singular, nmax, mmax,
+                                           A0.as<double>() + k * sa, nmax, sa, 0.0, res.as<double>(), 100,
+                                           nsw.as<rocblas_int>(), Sv.as<double>() + (int64_t)k * kmax, kmax,
+                                           U.as<double>() + k * su, nmax, su, V.as<double>() + k * sv, kmax, sv,
+                                           inf.as<rocblas_int>(), 1);
+    if (st != rocblas_status_success) throw Error(MPSXX_ERR_HIP, "rocsolver_dgesvdj_strided_batched failed");
+    rocblas_int info_j = 0;
+    hip_check(hipMemcpyAsync(&info_j, inf.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s), "info");
+    hip_check(hipMemcpyAsync(sv_h.data() + (size_t)k * kmax, Sv.as<double>() + (int64_t)k * kmax,
+                             kmax * sizeof(double), hipMemcpyDeviceToHost, s), "S");
+    hip_check(hipStreamSynchronize(s), "svd sync");
+    if (info_j == 0) info_h[k] = 0;
+  }
   for (int k = 0; k < nb; ++k)
-    if (info_h[k] != 0) throw Error(MPSXX_ERR_HIP, "gesvd did not converge");
+    if (info_h[k] != 0) {
+      std::vector<double> x(X.nelem);
+      hip_check(hipMemcpy(x.data(), wfnc.ptr(), X.nelem * sizeof(double), hipMemcpyDeviceToHost), "download");
+      int64_t bad = 0;
+      double amax = 0.0;
+      for (double v : x) {
+        if (!std::isfinite(v)) ++bad;
+        else amax = std::max(amax, std::fabs(v));
+      }
+      throw Error(MPSXX_ERR_HIP, "gesvd did not converge (block " + std::to_string(k) + " of " + std::to_string(nb) +
+                                     ", " + std::to_string(blks[k].m) + "x" + std::to_string(blks[k].n) +
+                                     " padded to " + std::to_string(mmax) + "x" + std::to_string(nmax) + ", info " +
+                                     std::to_string(info_h[k]) + "; input has " + std::to_string(bad) +
+                                     " non-finite values, max |x| " + std::to_string(amax) + ")");
+    }
   // truncation (QSTLAPACK.h:190-232)
   std::vector<double> all;
   for (int k = 0; k < nb; ++k)
