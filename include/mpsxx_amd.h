@@ -193,6 +193,25 @@ int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tens
                                 int64_t* flops, int64_t* bytes, int32_t* kernel, int32_t* step, int32_t* nentries,
                                 void* stream);
 
+/* ---- quantum-chemistry MPO (src/gen_qc_operators.cpp:18-202, src/boundary_opinfo.cpp:47-140) ------------------- */
+/* H = sum_{pq,s} h[p*K+q] a+_ps a_qs + 1/2 sum_{pqrs,st} v[((p*K+q)*K+r)*K+s] a+_ps a+_rt a_st a_qs + ecore, with v the
+ * chemist-notation (pq|rs) (the reference's twoint(p,r,q,s), src/parsing_integral.cpp:139-147).  Bond operators are
+ * the reference's (identity, H, normal singles of the left sites, complementary singles of the right sites, pair
+ * operators of the smaller side when swap_sweep != 0, else of the left side).  Host-side: create, info and
+ * site_entries need no GPU. */
+typedef struct mpsxx_qcmpo_s* mpsxx_qcmpo;
+int mpsxx_qc_mpo_create(int32_t norb, const double* oneint, const double* twoint, double ecore, int swap_sweep,
+                        mpsxx_qcmpo* out);
+/* bond_dims[0..norb]: number of bond operators left of site 0 (1), between sites, right of the last site (1) */
+int mpsxx_qc_mpo_info(mpsxx_qcmpo m, int32_t* bond_dims, int64_t* nentries_total);
+/* Site tensor W(wl, n_bra, n_ket, wr) as host arrays: two calls, the first with null buffers returns the counts.
+ * lq/rq: (particle, 2Sz) of every left/right bond operator; per entry: wl, wr and the 4x4 (bra, ket) matrix. */
+int mpsxx_qc_mpo_site_entries(mpsxx_qcmpo m, int32_t site, int32_t* nl, int32_t* nr, int64_t* nentries, int32_t* lq,
+                              int32_t* rq, int32_t* wl, int32_t* wr, double* mats);
+/* Site tensor as a device array (quantum shape (q_wl, q_n, -q_n, -q_wr), every non-zero element a 1x1x1x1 block) */
+int mpsxx_qc_mpo_site(mpsxx_qcmpo m, int32_t site, mpsxx_tensor out, void* stream);
+int mpsxx_qc_mpo_free(mpsxx_qcmpo m);
+
 /* ---- batched two-site sigma vectors (many independent instances per launch) ------------------------------------- */
 /* No single reference counterpart: it runs compute_sigma_vector (src/compute_sigma_vector.hpp:54-70) for n independent
  * problems (different chains, bond dimensions and block structures) with ONE grouped launch per contraction step and

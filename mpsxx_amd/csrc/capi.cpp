@@ -8,6 +8,7 @@
 #include "batch.hpp"
 #include "mpsxx_amd.h"
 #include "ops.hpp"
+#include "qcmpo.hpp"
 
 namespace {
 
@@ -37,6 +38,29 @@ int guard(F f) {
     g_err = e.what();
     return MPSXX_ERR_ARG;
   }
+}
+
+// host-only entry points: no device initialisation
+template <typename F>
+int guard_host(F f) {
+  try {
+    f();
+    return MPSXX_OK;
+  } catch (const mpx::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return MPSXX_ERR_OOM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return MPSXX_ERR_ARG;
+  }
+}
+
+mpx::QcMpo& QM(mpsxx_qcmpo h) {
+  if (!h || h->magic != mpsxx_qcmpo_s::kMagic) throw mpx::Error(MPSXX_ERR_ARG, "invalid or freed qc mpo handle");
+  return h->m;
 }
 
 mpx::Tensor& T(mpsxx_tensor h) {
@@ -370,6 +394,72 @@ int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tens
       if (kernel) kernel[i] = t[i].part.kind;
       if (step) step[i] = t[i].step;
     }
+  });
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// quantum-chemistry MPO
+
+int mpsxx_qc_mpo_create(int32_t norb, const double* oneint, const double* twoint, double ecore, int swap_sweep,
+                        mpsxx_qcmpo* out) {
+  return guard_host([&] {
+    if (!out || !oneint || !twoint) throw mpx::Error(MPSXX_ERR_ARG, "qc mpo: null argument");
+    auto* m = new mpsxx_qcmpo_s;
+    try {
+      m->m = mpx::build_qc_mpo(norb, oneint, twoint, ecore, swap_sweep != 0);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int mpsxx_qc_mpo_info(mpsxx_qcmpo m, int32_t* bond_dims, int64_t* nentries_total) {
+  return guard_host([&] {
+    const mpx::QcMpo& x = QM(m);
+    int64_t tot = 0;
+    for (int k = 0; k < x.K; ++k) {
+      if (bond_dims) bond_dims[k] = (int32_t)x.sites[k].lq.size();
+      tot += (int64_t)x.sites[k].mat.size();
+    }
+    if (bond_dims) bond_dims[x.K] = (int32_t)x.sites[x.K - 1].rq.size();
+    if (nentries_total) *nentries_total = tot;
+  });
+}
+
+int mpsxx_qc_mpo_site_entries(mpsxx_qcmpo m, int32_t site, int32_t* nl, int32_t* nr, int64_t* nentries, int32_t* lq,
+                              int32_t* rq, int32_t* wl, int32_t* wr, double* mats) {
+  return guard_host([&] {
+    const mpx::QcMpo& x = QM(m);
+    if (site < 0 || site >= x.K) throw mpx::Error(MPSXX_ERR_ARG, "qc mpo: site out of range");
+    const mpx::QcMpoSite& w = x.sites[site];
+    if (nl) *nl = (int32_t)w.lq.size();
+    if (nr) *nr = (int32_t)w.rq.size();
+    if (nentries) *nentries = (int64_t)w.mat.size();
+    for (size_t i = 0; lq && i < w.lq.size(); ++i) { lq[2 * i] = w.lq[i].p; lq[2 * i + 1] = w.lq[i].s; }
+    for (size_t i = 0; rq && i < w.rq.size(); ++i) { rq[2 * i] = w.rq[i].p; rq[2 * i + 1] = w.rq[i].s; }
+    for (size_t e = 0; e < w.mat.size(); ++e) {
+      if (wl) wl[e] = w.wl[e];
+      if (wr) wr[e] = w.wr[e];
+      if (mats) std::memcpy(mats + 16 * e, w.mat[e].data(), 16 * sizeof(double));
+    }
+  });
+}
+
+int mpsxx_qc_mpo_site(mpsxx_qcmpo m, int32_t site, mpsxx_tensor out, void* stream) {
+  return guard([&] {
+    const mpx::QcMpo& x = QM(m);
+    if (site < 0 || site >= x.K) throw mpx::Error(MPSXX_ERR_ARG, "qc mpo: site out of range");
+    mpx::qc_mpo_tensor(x.sites[site], T(out), S(stream));
+  });
+}
+
+int mpsxx_qc_mpo_free(mpsxx_qcmpo m) {
+  return guard_host([&] {
+    if (!m) return;
+    QM(m);
+    delete m;
   });
 }
 

@@ -1,0 +1,78 @@
+"""Quantum-chemistry MPO on the device path (needs an MI355X).
+
+* sigma two-site with the QC MPO at the middle of a 10-orbital chain (D_w = 232 on both bonds) against the oracle;
+* device two-site sweeps with the QC MPO of random 6-orbital integrals against the oracle's sweeps and the FCI energy
+  (exact diagonalization, sector dimension 400);
+* Hubbard L=12, U=1 read from an FCIDUMP (tests/golden/hubbard12.FCIDUMP, the reference's dmrg/tests/HUBBARD12 model)
+  through the QC MPO: E0 = -11.8406372775523305 (dmrg/tests/hub12_dmrg.out:15536)."""
+import os
+
+import numpy as np
+import pytest
+
+import ed
+import mpsxx_amd as mx
+import oracle
+from helpers import random_env_left, random_env_right
+from mpsxx_amd import mps, qc
+from mpsxx_amd.dmrg import DeviceDMRG, initial_mps
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_sigma_twosite_with_qc_mpo_matches_oracle():
+    print("qc sigma K=10", flush=True)
+    K, i = 10, 4
+    rng = np.random.default_rng(21)
+    h, v = qc.synthetic_integrals(K, seed=4)
+    m = qc.QcMpo(h, v)
+    W1, W2 = m.site_qst(i), m.site_qst(i + 1)
+    assert len(W1.qshape[0]) == m.bond_dims()[i] and len(W2.qshape[3]) == m.bond_dims()[i + 2]
+    qt = (K, 0)
+    lq = mps.left_bond_qns(K, i, qt)
+    rq = mps.right_bond_qns(K, i + 2, qt)
+    ld = [int(rng.integers(1, 6)) if abs(q[1]) <= 2 else 0 for q in lq]
+    rd = [int(rng.integers(1, 6)) if abs(q[1]) <= 2 else 0 for q in rq]
+    lenv = random_env_left(lq, ld, W1, rng)
+    renv = random_env_right(rq, rd, W2, rng)
+    psi = mps.fill_allowed(mps.twosite_wavefunction(lq, ld, rq, rd, qt), rng)
+    ref = oracle.sigma_twosite(W1, W2, lenv, renv, psi)
+    got = mx.compute_sigma_vector_twosite(*[mx.DTensor(x) for x in (W1, W2, lenv, renv, psi)]).to_qst()
+    assert got.block_set() == ref.block_set()
+    scale = max(float(np.max(np.abs(b))) for b in ref.blocks.values())
+    assert got.max_abs_diff(ref) <= 1e-12 * scale
+
+
+def test_device_dmrg_with_qc_mpo_matches_oracle_and_fci():
+    print("qc dmrg K=6", flush=True)
+    K, M = 6, 64
+    h, v = qc.synthetic_integrals(K, seed=5, nvec=8, hsig=0.2, lsig=0.25)
+    e_fci = ed.ground_energy(h, v, K, K, 0)
+    mpos = qc.QcMpo(h, v).qsts()
+    init = initial_mps(K, (K, 0), M, 0)
+    es, _ = oracle.dmrg(mpos, init, [M] * 4, 1e-10)
+    dm = DeviceDMRG(mpos, init, M)
+    e = None
+    for _ in range(4):
+        e, st = dm.sweep(M, 1e-10)
+    assert abs(e - es[-1]) < 1e-10, (e, es)
+    assert abs(e - e_fci) < 1e-10, (e, e_fci)
+
+
+def test_hubbard12_from_fcidump_through_qc_mpo_matches_reference_log():
+    print("hub12 fcidump", flush=True)
+    norb, nelec, ms2, h, v, ecore = qc.read_fcidump(os.path.join(HERE, "golden", "hubbard12.FCIDUMP"))
+    assert (norb, nelec, ms2) == (12, 12, 0)
+    mpos = qc.QcMpo(h, v, ecore).qsts()
+    M = 200
+    dm = DeviceDMRG(mpos, initial_mps(norb, (nelec, ms2), M, 0), M)
+    e_prev, e = None, None
+    for sweep in range(8):
+        e, st = dm.sweep(M, 1e-8)
+        print("sweep", sweep, e, st["seconds"], st["max_bond"], flush=True)
+        if e_prev is not None and abs(e - e_prev) < 1e-10:
+            break
+        e_prev = e
+    assert abs(e - (-11.8406372775523305)) < 2e-7, e
