@@ -193,6 +193,12 @@ int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tens
                                 int64_t* flops, int64_t* bytes, int32_t* kernel, int32_t* step, int32_t* nentries,
                                 void* stream);
 
+/* Same per-launch profile for the plan of the most recent hot-path call on this thread (any form: merged, one-site,
+ * renormalize, ...), re-run on the same inputs with the output redirected to scratch; the caller must still hold the
+ * input tensors.  plan_flops = the plan's algorithmic flops. */
+int mpsxx_last_plan_profile(int reps, int32_t max_entries, double* ms, int64_t* flops, int64_t* bytes,
+                            int32_t* kernel, int32_t* step, int32_t* nentries, int64_t* plan_flops, void* stream);
+
 /* ---- quantum-chemistry MPO (src/gen_qc_operators.cpp:18-202, src/boundary_opinfo.cpp:47-140) ------------------- */
 /* H = sum_{pq,s} h[p*K+q] a+_ps a_qs + 1/2 sum_{pqrs,st} v[((p*K+q)*K+r)*K+s] a+_ps a+_rt a_st a_qs + ecore, with v the
  * chemist-notation (pq|rs) (the reference's twoint(p,r,q,s), src/parsing_integral.cpp:139-147).  Bond operators are

@@ -231,3 +231,18 @@ def copy(x, y=None):
     y = _out(y)
     _check(lib().mpsxx_tensor_copy(_h(x), _h(y), None))
     return y
+
+
+def compute_merged_operator(forward, mpo0, opr0, out=None):
+    """Merged environment operator (src/optimize.hpp:273-368): forward merges (l, n) of Contract(L, W); backward
+    merges (n, r) of Contract(W, R)."""
+    out = _out(out)
+    _check(lib().mpsxx_compute_merged_operator(ctypes.c_int(1 if forward else 0), _h(mpo0), _h(opr0), _h(out), None))
+    return out
+
+
+def qst_merge(x, nrow, ncol, out=None):
+    """QSTmerge of the first nrow and the last ncol indices (include/btas/QSPARSE/QSTmerge.h)."""
+    out = _out(out)
+    _check(lib().mpsxx_qst_merge(_h(x), ctypes.c_int32(nrow), ctypes.c_int32(ncol), _h(out), None))
+    return out

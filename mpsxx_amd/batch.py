@@ -103,3 +103,10 @@ def dominant_kernel(entries):
         a["launches"] += 1
     name = max(agg, key=lambda k: agg[k]["ms"])
     return name, agg[name], agg
+
+
+def last_plan_profile(reps=20):
+    """Per kernel launch of the plan of the most recent hot-path call (see mpsxx_last_plan_profile)."""
+    pf = ctypes.c_int64()
+    ent = _profile_entries(lambda *a: lib().mpsxx_last_plan_profile(ctypes.c_int(reps), *a, ctypes.byref(pf), None))
+    return ent, pf.value

@@ -397,6 +397,29 @@ int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tens
   });
 }
 
+int mpsxx_last_plan_profile(int reps, int32_t max_entries, double* ms, int64_t* flops, int64_t* bytes,
+                            int32_t* kernel, int32_t* step, int32_t* nentries, int64_t* plan_flops, void* stream) {
+  return guard([&] {
+    if (reps <= 0) throw mpx::Error(MPSXX_ERR_ARG, "profile: reps must be positive");
+    mpx::LastRun& r = mpx::last_run();
+    if (!r.plan) throw mpx::Error(MPSXX_ERR_ARG, "profile: no hot-path call to profile on this thread");
+    mpx::Plan& P = *r.plan;
+    if (plan_flops) *plan_flops = P.flops;
+    mpx::DevBuf out(std::max<int64_t>(P.out.nelem, 1) * sizeof(double));
+    mpx::run_plan(P, r.inputs, out.as<double>(), S(stream));
+    std::vector<mpx::PartTiming> t = mpx::profile_plan_parts(P, r.inputs, out.as<double>(), reps, S(stream));
+    if (nentries) *nentries = (int32_t)t.size();
+    if ((int)t.size() > max_entries) throw mpx::Error(MPSXX_ERR_ARG, "profile: max_entries too small");
+    for (size_t i = 0; i < t.size(); ++i) {
+      if (ms) ms[i] = t[i].ms;
+      if (flops) flops[i] = t[i].part.flops;
+      if (bytes) bytes[i] = t[i].part.bytes;
+      if (kernel) kernel[i] = t[i].part.kind;
+      if (step) step[i] = t[i].step;
+    }
+  });
+}
+
 // ---------------------------------------------------------------------------------------------------------------------
 // quantum-chemistry MPO
 

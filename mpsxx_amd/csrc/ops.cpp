@@ -17,6 +17,11 @@ OpStats& last_stats() {
   return st;
 }
 
+LastRun& last_run() {
+  static thread_local LastRun r;
+  return r;
+}
+
 namespace {
 
 enum OpId {
@@ -216,13 +221,18 @@ void execute(const std::vector<uint64_t>& key, BuildFn build, int out_mode, std:
   run_plan(P, ptrs, out.ptr(), s);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
+  last_run().plan = &P;
+  last_run().inputs = ptrs;
 }
 
 const Layout* existing(const Tensor& t) { return t.empty ? nullptr : &t.L; }
 
 }  // namespace
 
-void clear_plan_cache() { cache().clear(); }
+void clear_plan_cache() {
+  cache().clear();
+  last_run() = LastRun();
+}
 
 // ---------------------------------------------------------------------------------------------------------------------
 // generic machinery
@@ -438,6 +448,8 @@ void op_sigma_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Te
   run_plan(P, {lmpo.ptr(), rmpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
+  last_run().plan = &P;
+  last_run().inputs = {lmpo.ptr(), rmpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()};
 }
 
 void op_sigma_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s) {

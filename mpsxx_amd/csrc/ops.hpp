@@ -15,6 +15,14 @@ struct OpStats {
 // Last plan statistics (per thread).
 OpStats& last_stats();
 
+// The plan and input pointers of the last hot-path call on this thread (for per-kernel profiling; the plan stays
+// valid while it is cached, the inputs while the caller keeps the tensors).
+struct LastRun {
+  Plan* plan = nullptr;
+  std::vector<const double*> inputs;
+};
+LastRun& last_run();
+
 // Generic BTAS machinery
 void op_contract(double alpha, Tensor& a, bool conj_a, const std::vector<int>& ca, Tensor& b, bool conj_b,
                  const std::vector<int>& cb, double beta, Tensor& c, hipStream_t s);

@@ -6,6 +6,11 @@ density matrix spectrum of a converged chain (Gaussian in particle-number and sp
 sigma = 1.5; no sector holds more states than the sites on its side can carry), and pattern-preserving random entries
 N(0,1)/sqrt(m) in every quantum-number-allowed block.
 
+``qc_merged_problem(norb, M, seed)``: the merged-form sigma problem at the middle of a quantum-chemistry chain
+(synthetic integrals, QC MPO with swap-sweep pair operators): left/right environments with the bond operators of the
+two middle MPO sites, sector dimensions as above (centre N*k/norb, capacities of k sites), random entries; merged
+operators and wavefunction built on the device.
+
 ``batch_specs(n, seed)``: the (L, U, M) sample of a batch of independent chains: L uniform over the even lengths
 16..64 (half filling with 2Sz = 0 needs N = L even), U from {2, 4, 8}, M from {128, 256, 512}.
 """
@@ -129,3 +134,56 @@ def batch_specs(n=64, seed=4):
         M = int(rng.choice([128, 256, 512]))
         out.append((L, U, M))
     return out
+
+
+def _env_blocks(rows_q, rows_d, wq, cols_q, cols_d, rng, left):
+    """Random environment QST over (rows, w, cols) with every quantum-number-allowed block filled."""
+    if left:   # L(l_bra, w, l_ket): quantum shape (lq, -q_w, -lq), total 0
+        qs = [list(rows_q), [(-a, -b) for (a, b) in wq], [(-a, -b) for (a, b) in cols_q]]
+    else:      # R(r_bra, w, r_ket): quantum shape (-rq, q_w, rq)
+        qs = [[(-a, -b) for (a, b) in rows_q], list(wq), list(cols_q)]
+    e = QST((0, 0), qs, [list(rows_d), [1] * len(wq), list(cols_d)])
+    index = {}
+    for a, q in enumerate(qs[0]):
+        index.setdefault(q, []).append(a)
+    for w, qw in enumerate(qs[1]):
+        for c, qc_ in enumerate(qs[2]):
+            need = (-(qw[0] + qc_[0]), -(qw[1] + qc_[1]))
+            for a in index.get(need, []):
+                shp = (rows_d[a], 1, cols_d[c])
+                e.blocks[(a, w, c)] = rng.standard_normal(shp) / math.sqrt(max(shp))
+    return e
+
+
+def qc_merged_problem(norb=24, M=1024, seed=1, width=1.5):
+    """Device tensors (Lm, Rm, psim) of the merged sigma at the middle of a norb-orbital chain; see the docstring."""
+    from . import DTensor, compute_merged_operator, qst_merge
+    from .qc import QcMpo, synthetic_integrals
+    rng = np.random.default_rng(seed)
+    h, v = synthetic_integrals(norb, seed)
+    m = QcMpo(h, v, swap_sweep=True)
+    K = norb
+    qt = (K, 0)
+    i = K // 2 - 1
+    lq = mps.left_bond_qns(K, i, qt)
+    rq = mps.right_bond_qns(K, i + 2, qt)
+    nl, nr = i, K - (i + 2)
+    ld = gaussian_dims(lq, (nl * qt[0] / K, 0), M, width, caps=[sector_capacity(nl, q) for q in lq])
+    rd = gaussian_dims(rq, (-nr * qt[0] / K, 0), M, width, caps=[sector_capacity(nr, (-q[0], -q[1])) for q in rq])
+    lq, ld = _prune(lq, ld)
+    rq, rd = _prune(rq, rd)
+    wl = m.site_entries(i)[0]        # left bond operators of site i
+    wr = m.site_entries(i + 1)[1]    # right bond operators of site i+1
+    lenv = _env_blocks(lq, ld, wl, lq, ld, rng, True)
+    renv = _env_blocks(rq, rd, wr, rq, rd, rng, False)
+    psi = mps.twosite_wavefunction(lq, ld, rq, rd, qt)
+    psi = mps.fill_allowed(psi, rng, 1.0)
+    W1, W2 = m.site_device(i), m.site_device(i + 1)
+    dl, dr, dpsi = DTensor(lenv), DTensor(renv), DTensor(psi)
+    Lm = compute_merged_operator(True, W1, dl)
+    Rm = compute_merged_operator(False, W2, dr)
+    psim = qst_merge(dpsi, 2, 2)
+    info = dict(norb=norb, M=M, seed=seed, site=i, D_w=m.bond_dims()[i + 1], D_w_left=len(wl), D_w_right=len(wr),
+                left_sectors=len(lq), right_sectors=len(rq), psi_blocks=len(psi.blocks), psi_elems=psi.nelem(),
+                env_elems=lenv.nelem() + renv.nelem())
+    return Lm, Rm, psim, info
