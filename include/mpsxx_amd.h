@@ -237,7 +237,8 @@ int mpsxx_batch_time(mpsxx_batch b, int32_t reps, uint32_t convert_mask, int use
 int mpsxx_batch_profile(mpsxx_batch b, int32_t reps, int32_t max_entries, double* ms, int64_t* flops, int64_t* bytes,
                         int32_t* kernel, int32_t* step, int32_t* nentries, void* stream);
 /* Autotuner: times candidate tile policies per step on the batch's own task lists (outputs redirected to scratch) and
- * keeps the fastest; policy[3*k..3*k+2] = {small-block limit, split threshold, grid cap} of step k. */
+ * keeps the fastest; policy[4*k..4*k+3] = {small-block limit, split threshold, grid cap, 64x64 split target} of
+ * launch k (mpsxx_batch_set_policy takes the same layout). */
 int mpsxx_batch_tune(mpsxx_batch b, int32_t reps, int32_t max_steps, int32_t* policy, void* stream);
 int mpsxx_batch_set_policy(mpsxx_batch b, int32_t nsteps, const int32_t* policy);
 int mpsxx_batch_info(mpsxx_batch b, int64_t* flops, int64_t* bytes, int32_t* nsteps, int64_t* work_items);

@@ -41,11 +41,13 @@ class SigmaBatch:
 
     def tune(self, reps=5):
         n = 16
-        pol = np.zeros(3 * n, dtype=np.int32)
+        pol = np.zeros(4 * n, dtype=np.int32)
         _check(lib().mpsxx_batch_tune(self.h, ctypes.c_int32(reps), ctypes.c_int32(n), pol.ctypes.data_as(VP), None))
-        return [tuple(int(v) for v in pol[3 * k:3 * k + 3]) for k in range(self.info()["nsteps"])]
+        return [tuple(int(v) for v in pol[4 * k:4 * k + 4]) for k in range(self.info()["nsteps"])]
 
     def set_policy(self, policies):
+        """One (small-block limit, split threshold, grid cap[, big-tile split target]) tuple per launch."""
+        policies = [tuple(p) + (2048,) if len(p) == 3 else tuple(p) for p in policies]
         pol = np.ascontiguousarray(np.array(policies, dtype=np.int32).reshape(-1))
         _check(lib().mpsxx_batch_set_policy(self.h, ctypes.c_int32(len(policies)), pol.ctypes.data_as(VP)))
 

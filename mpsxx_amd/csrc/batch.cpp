@@ -183,17 +183,12 @@ void BatchSigma::drop_graphs() {
 
 void BatchSigma::build_step_lists(int k) {
   Step& st = launches_[k].st;
-  st.max_groups = pol_[k].max_groups;
-  st.policy = pol_[k];
-  if (st.kind != Step::GEMM) return;
-  std::vector<GemmTile> big, small, split;
-  gemm_tiles(st.host.gtasks, st.host.gpairs, pol_[k], big, small, split);
-  st.nwork = (int)big.size();
-  st.work = upload_vec(big);
-  st.nwork_small = (int)small.size();
-  st.work_small = upload_vec(small);
-  st.nwork_split = (int)split.size();
-  st.work_split = upload_vec(split);
+  if (st.kind == Step::GEMM) {
+    set_gemm_policy(st, pol_[k]);
+  } else {
+    st.max_groups = pol_[k].max_groups;
+    st.policy = pol_[k];
+  }
 }
 
 void BatchSigma::set_policies(const std::vector<GemmPolicy>& pol) {
@@ -375,10 +370,10 @@ std::vector<GemmPolicy> BatchSigma::tune(int reps, hipStream_t s) {
       for (int lim : {0, 32, 64, 96, 128, 192})
         for (int su : {2, 4, 8, INT_MAX}) {
           if (lim == 0 && su != 2) continue;  // no small blocks: the split threshold is irrelevant
-          for (int cap : grid_caps) cand.push_back({lim, su, cap});
+          for (int cap : grid_caps) cand.push_back({lim, su, cap, pol_[k].split_big});
         }
     } else {
-      for (int cap : grid_caps) cand.push_back({0, 0, cap});
+      for (int cap : grid_caps) cand.push_back({0, 0, cap, 0});
     }
     // the current policy first: a candidate must beat it by 1 % to replace it
     cand.insert(cand.begin(), pol_[k]);
@@ -391,6 +386,20 @@ std::vector<GemmPolicy> BatchSigma::tune(int reps, hipStream_t s) {
       if (t < (c == 0 ? best : 0.99 * best)) {
         best = t;
         bp = cand[c];
+      }
+    }
+    // second pass: the split threshold of the 64x64 tiles, with the other choices fixed
+    if (launches_[k].st.kind == Step::GEMM) {
+      for (int sb : {0, 512, 2048, 8192}) {
+        if (sb == bp.split_big) continue;
+        pol_[k] = bp;
+        pol_[k].split_big = sb;
+        build_step_lists(k);
+        const double t = time_step(k);
+        if (t < 0.99 * best) {
+          best = t;
+          bp = pol_[k];
+        }
       }
     }
     pol_[k] = bp;

@@ -542,9 +542,10 @@ int mpsxx_batch_tune(mpsxx_batch b, int32_t reps, int32_t max_steps, int32_t* po
     if (x.nsteps() > max_steps) throw mpx::Error(MPSXX_ERR_ARG, "batch tune: max_steps too small");
     const std::vector<mpx::GemmPolicy> p = x.tune(reps, S(stream));
     for (size_t k = 0; policy && k < p.size(); ++k) {
-      policy[3 * k] = p[k].small_limit;
-      policy[3 * k + 1] = p[k].split_units;
-      policy[3 * k + 2] = p[k].max_groups;
+      policy[4 * k] = p[k].small_limit;
+      policy[4 * k + 1] = p[k].split_units;
+      policy[4 * k + 2] = p[k].max_groups;
+      policy[4 * k + 3] = p[k].split_big;
     }
   });
 }
@@ -555,9 +556,10 @@ int mpsxx_batch_set_policy(mpsxx_batch b, int32_t nsteps, const int32_t* policy)
     if (nsteps != x.nsteps() || !policy) throw mpx::Error(MPSXX_ERR_ARG, "batch set_policy: one policy per step");
     std::vector<mpx::GemmPolicy> p(nsteps);
     for (int k = 0; k < nsteps; ++k) {
-      if (policy[3 * k] < 0 || policy[3 * k + 1] < 1 || policy[3 * k + 2] < 0)
-        throw mpx::Error(MPSXX_ERR_ARG, "batch set_policy: negative limit, threshold or grid cap");
-      p[k] = {policy[3 * k], policy[3 * k + 1], policy[3 * k + 2]};
+      const int32_t* q = policy + 4 * k;
+      if (q[0] < 0 || q[1] < 1 || q[2] < 0 || q[3] < 0)
+        throw mpx::Error(MPSXX_ERR_ARG, "batch set_policy: negative limit, threshold, grid cap or split target");
+      p[k] = {q[0], q[1], q[2], q[3]};
     }
     x.set_policies(p);
   });

@@ -495,21 +495,11 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
     return st;
   }
   st.kind = Step::GEMM;
-  std::vector<GemmTile> tiles, small, split;
-  const GemmPolicy pol = default_policy();
-  gemm_tiles(sym.tasks, sym.pairs, pol, tiles, small, split);
-  st.policy = pol;
-  st.max_groups = pol.max_groups;
-  st.nwork = (int)tiles.size();
-  st.work = upload_vec(tiles);
-  st.nwork_small = (int)small.size();
-  st.work_small = upload_vec(small);
-  st.nwork_split = (int)split.size();
-  st.work_split = upload_vec(split);
   st.tasks = upload_vec(sym.tasks);
   st.pairs = upload_vec(sym.pairs);
   st.host.gtasks = sym.tasks;
   st.host.gpairs = sym.pairs;
+  set_gemm_policy(st, default_policy());
   return st;
 }
 
@@ -526,14 +516,17 @@ GemmPolicy default_policy() {
 }
 
 void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>& pairs, const GemmPolicy& pol,
-                std::vector<GemmTile>& big, std::vector<GemmTile>& small, std::vector<GemmTile>& split) {
-  big.clear();
-  small.clear();
-  split.clear();
+                GemmTiling& out) {
+  out = GemmTiling();
   // heavy tasks first so the tail of each launch is made of small tiles
   std::vector<size_t> order(tasks.size());
   std::iota(order.begin(), order.end(), 0);
   std::vector<int64_t> cost(tasks.size(), 0);
+  auto units_of = [&](int lo, int hi) {
+    int64_t u = 0;
+    for (int p = lo; p < hi; ++p) u += (pairs[p].k + 15) / 16;
+    return u;
+  };
   for (size_t ti = 0; ti < tasks.size(); ++ti) {
     const GemmTask& t = tasks[ti];
     int64_t ksum = 0;
@@ -541,19 +534,73 @@ void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>&
     cost[ti] = ksum;
   }
   std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  std::vector<size_t> bigtasks;
+  int64_t nbig = 0, ubig = 0;
   for (size_t ti : order) {
     const GemmTask& t = tasks[ti];
     if (t.m <= pol.small_limit && t.n <= pol.small_limit) {
-      int units = 0;
-      for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) units += (pairs[p].k + 15) / 16;
-      std::vector<GemmTile>& dst = (t.pair_count >= 2 && units >= pol.split_units) ? split : small;
+      const int64_t units = units_of(t.pair_begin, t.pair_begin + t.pair_count);
+      std::vector<GemmTile>& dst = (t.pair_count >= 2 && units >= pol.split_units) ? out.split : out.small;
       for (int r = 0; r < t.m; r += kSmallTile)
         for (int c = 0; c < t.n; c += kSmallTile) dst.push_back({(int32_t)ti, r, c, 0});
       continue;
     }
-    for (int r = 0; r < t.m; r += kGemmTile)
-      for (int c = 0; c < t.n; c += kGemmTile) big.push_back({(int32_t)ti, r, c, 0});
+    bigtasks.push_back(ti);
+    const int64_t nt = (int64_t)((t.m + kGemmTile - 1) / kGemmTile) * ((t.n + kGemmTile - 1) / kGemmTile);
+    nbig += nt;
+    ubig += nt * units_of(t.pair_begin, t.pair_begin + t.pair_count);
   }
+  // split the pair lists of long reductions when the big tiles alone cannot fill the device
+  const bool split = pol.split_big > 0 && nbig > 0 && nbig < pol.split_big;
+  const int64_t target = split ? std::max<int64_t>(8, (ubig + pol.split_big - 1) / pol.split_big) : 0;
+  for (size_t ti : bigtasks) {
+    const GemmTask& t = tasks[ti];
+    const int lo = t.pair_begin, hi = t.pair_begin + t.pair_count;
+    const int64_t units = units_of(lo, hi);
+    int nseg = 1;
+    if (split) nseg = (int)std::min<int64_t>({64, (int64_t)t.pair_count, std::max<int64_t>(1, (units + target - 1) / target)});
+    // segment boundaries with about units/nseg units each
+    std::vector<int> cut{lo};
+    if (nseg > 1) {
+      int64_t acc = 0;
+      int made = 1;
+      for (int p = lo; p < hi && made < nseg; ++p) {
+        acc += (pairs[p].k + 15) / 16;
+        if (acc * nseg >= units * made && p + 1 < hi) {
+          cut.push_back(p + 1);
+          ++made;
+        }
+      }
+    }
+    cut.push_back(hi);
+    const int nparts = (int)cut.size() - 1;
+    for (int r = 0; r < t.m; r += kGemmTile)
+      for (int c = 0; c < t.n; c += kGemmTile) {
+        if (nparts == 1) {
+          out.big.push_back({(int32_t)ti, r, c, lo, hi, -1});
+          continue;
+        }
+        out.red.push_back({(int32_t)ti, r, c, out.nslots, nparts, 0});
+        for (int q = 0; q < nparts; ++q) out.big.push_back({(int32_t)ti, r, c, cut[q], cut[q + 1], out.nslots + q});
+        out.nslots += nparts;
+      }
+  }
+}
+
+void set_gemm_policy(Step& st, const GemmPolicy& pol) {
+  GemmTiling g;
+  gemm_tiles(st.host.gtasks, st.host.gpairs, pol, g);
+  st.policy = pol;
+  st.max_groups = pol.max_groups;
+  st.nwork = (int)g.big.size();
+  st.work = upload_vec(g.big);
+  st.nwork_red = (int)g.red.size();
+  st.work_red = upload_vec(g.red);
+  st.partial.alloc((size_t)g.nslots * kGemmTile * kGemmTile * sizeof(double));
+  st.nwork_small = (int)g.small.size();
+  st.work_small = upload_vec(g.small);
+  st.nwork_split = (int)g.split.size();
+  st.work_split = upload_vec(g.split);
 }
 
 bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, int y, int w1, int w2, int z,
@@ -639,8 +686,9 @@ template <typename T, typename TC>
 void launch_step_t(const Step& st, const Operands& ops, unsigned parts, hipStream_t s) {
   if (st.kind == Step::GEMM) {
     if (parts & 1u)
-      launch_gemm<T, TC>(st.work.as<GemmTile>(), st.nwork, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops,
-                         st.alpha, st.accumulate, s, st.max_groups);
+      launch_gemm<T, TC>(st.work.as<BigTile>(), st.nwork, st.work_red.as<RedTile>(), st.nwork_red, st.partial.p,
+                         st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s,
+                         st.max_groups);
     if (parts & 2u)
       launch_gemm_small<T, TC>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
                                st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
@@ -673,26 +721,42 @@ std::vector<KernelPart> step_parts(const Step& st) {
   }
   if (st.kind != Step::GEMM) return out;
   // exact split of the step's flops over the three tile lists: a tile covers rows x cols of its block
-  auto list_cost = [&](const std::vector<GemmTile>& tiles, int edge, int64_t& fl, int64_t& by) {
-    fl = by = 0;
-    for (const GemmTile& tl : tiles) {
-      const GemmTask& t = st.host.gtasks[tl.task];
-      const int64_t r = std::min(edge, t.m - tl.row0), c = std::min(edge, t.n - tl.col0);
-      int64_t ks = 0;
-      for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) ks += st.host.gpairs[p].k;
-      fl += 2 * r * c * ks;
-      by += 8 * ((r + c) * ks + r * c);
-    }
+  auto tile_cost = [&](int task, int row0, int col0, int lo, int hi, int edge, int64_t& fl, int64_t& by) {
+    const GemmTask& t = st.host.gtasks[task];
+    const int64_t r = std::min(edge, t.m - row0), c = std::min(edge, t.n - col0);
+    int64_t ks = 0;
+    for (int p = lo; p < hi; ++p) ks += st.host.gpairs[p].k;
+    fl += 2 * r * c * ks;
+    by += 8 * ((r + c) * ks);
   };
-  std::vector<GemmTile> big, small, split;
-  gemm_tiles(st.host.gtasks, st.host.gpairs, st.policy, big, small, split);
-  int64_t fl, by;
-  list_cost(big, kGemmTile, fl, by);
-  if (!big.empty()) out.push_back({KernelPart::GEMM64, 1u, (int)big.size(), fl, by});
-  list_cost(small, kSmallTile, fl, by);
-  if (!small.empty()) out.push_back({KernelPart::GEMM_SMALL, 2u, (int)small.size(), fl, by});
-  list_cost(split, kSmallTile, fl, by);
-  if (!split.empty()) out.push_back({KernelPart::GEMM_SPLIT, 4u, (int)split.size(), fl, by});
+  GemmTiling g;
+  gemm_tiles(st.host.gtasks, st.host.gpairs, st.policy, g);
+  int64_t fl = 0, by = 0;
+  for (const BigTile& tl : g.big) tile_cost(tl.task, tl.row0, tl.col0, tl.pair_lo, tl.pair_hi, kGemmTile, fl, by);
+  for (const BigTile& tl : g.big) {
+    if (tl.slot >= 0) continue;
+    const GemmTask& t = st.host.gtasks[tl.task];
+    by += 8 * (int64_t)std::min(kGemmTile, t.m - tl.row0) * std::min(kGemmTile, t.n - tl.col0);
+  }
+  for (const RedTile& rt : g.red) {
+    const GemmTask& t = st.host.gtasks[rt.task];
+    by += 8 * (int64_t)std::min(kGemmTile, t.m - rt.row0) * std::min(kGemmTile, t.n - rt.col0);
+  }
+  if (!g.big.empty()) out.push_back({KernelPart::GEMM64, 1u, (int)g.big.size(), fl, by});
+  fl = by = 0;
+  for (const GemmTile& tl : g.small) {
+    const GemmTask& t = st.host.gtasks[tl.task];
+    tile_cost(tl.task, tl.row0, tl.col0, t.pair_begin, t.pair_begin + t.pair_count, kSmallTile, fl, by);
+    by += 8 * (int64_t)std::min(kSmallTile, t.m - tl.row0) * std::min(kSmallTile, t.n - tl.col0);
+  }
+  if (!g.small.empty()) out.push_back({KernelPart::GEMM_SMALL, 2u, (int)g.small.size(), fl, by});
+  fl = by = 0;
+  for (const GemmTile& tl : g.split) {
+    const GemmTask& t = st.host.gtasks[tl.task];
+    tile_cost(tl.task, tl.row0, tl.col0, t.pair_begin, t.pair_begin + t.pair_count, kSmallTile, fl, by);
+    by += 8 * (int64_t)std::min(kSmallTile, t.m - tl.row0) * std::min(kSmallTile, t.n - tl.col0);
+  }
+  if (!g.split.empty()) out.push_back({KernelPart::GEMM_SPLIT, 4u, (int)g.split.size(), fl, by});
   return out;
 }
 

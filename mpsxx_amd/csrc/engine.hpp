@@ -89,17 +89,26 @@ struct HostLists {
 
 // How the blocks of a GEMM step are mapped to kernels: blocks with both sides <= small_limit run as 32x32 tiles, split
 // over the 4 waves of a work-group when they reduce over >= 2 pairs and >= split_units (pair, 16-wide k panel) units,
-// one wave per tile otherwise; larger blocks run as 64x64 LDS tiles.  max_groups > 0 caps the grid of every launch.
+// one wave per tile otherwise; larger blocks run as 64x64 LDS tiles, whose pair lists are split over several
+// work-groups (partial tiles + an ordered sum) while the step has fewer than split_big such tiles (0: never).
+// max_groups > 0 caps the grid of every launch.
 struct GemmPolicy {
   int small_limit = 64;
   int split_units = 2;
   int max_groups = 0;
+  int split_big = 2048;
 };
 GemmPolicy default_policy();  // the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT / MPSXX_SPLIT_UNITS
 
-// Tile lists of one GEMM step under a policy (heaviest blocks first in each list).
+// Tile lists of one GEMM step (heaviest blocks first in each list).
+struct GemmTiling {
+  std::vector<BigTile> big;
+  std::vector<RedTile> red;
+  int nslots = 0;  // partial 64x64 tiles of the split big reductions
+  std::vector<GemmTile> small, split;
+};
 void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>& pairs, const GemmPolicy& pol,
-                std::vector<GemmTile>& big, std::vector<GemmTile>& small, std::vector<GemmTile>& split);
+                GemmTiling& out);
 
 // One device step of a plan.
 struct Step {
@@ -108,8 +117,10 @@ struct Step {
   int b2 = -1;                 // fused AXPY: slot of the second coefficient tensor
   int accumulate = 0;
   double alpha = 1.0, beta = 0.0;
-  int nwork = 0;               // tiles or chunks
+  int nwork = 0;               // GEMM: 64x64 tiles (BigTile); AXPY: chunks
   DevBuf work, tasks, pairs;   // device copies of the task lists
+  int nwork_red = 0;           // GEMM: tiles of split reductions (RedTile) and their partial buffer
+  DevBuf work_red, partial;
   int nwork_small = 0;         // GEMM: 32x32 tiles of the small blocks (gemm_small_kernel)
   DevBuf work_small;
   int nwork_split = 0;         // GEMM: 32x32 tiles with long reductions (gemm_split_kernel)
@@ -130,6 +141,8 @@ struct KernelPart {
   int64_t flops, bytes;
 };
 std::vector<KernelPart> step_parts(const Step& st);
+// (Re)build the device tile lists of a GEMM step from its host task lists.
+void set_gemm_policy(Step& st, const GemmPolicy& pol);
 const char* part_kernel_name(int kind);
 
 // Slot table: inputs are bound per call; intermediates are owned by the plan.

@@ -42,26 +42,89 @@ __device__ __forceinline__ T* opnd(const void* p, int inst) {
   else return reinterpret_cast<T*>(const_cast<void*>(p));
 }
 
+// Panel staging of the 64x64 kernel: each of the 256 threads moves 4 elements of the 64x16 A panel and 4 of the 16x64
+// B panel.  The element -> thread map follows the operand's unit-stride direction (k for A stored as used, rows for A
+// used transposed; columns for B as stored, k for B transposed) so that global loads are coalesced either way.
+template <typename T>
+__device__ __forceinline__ void load_a(T (&r)[4], const T* __restrict__ Ap, const GemmPair& pr, int row0, int m, int k0,
+                                       int tid) {
+  if (pr.sa_k == 1) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j, kk = e & 15, i = e >> 4;
+      const int gi = row0 + i, gk = k0 + kk;
+      r[j] = (gi < m && gk < pr.k) ? Ap[(int64_t)gi * pr.sa_m + gk] : T(0);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j, i = e & 63, kk = e >> 6;
+      const int gi = row0 + i, gk = k0 + kk;
+      r[j] = (gi < m && gk < pr.k) ? Ap[(int64_t)gi * pr.sa_m + (int64_t)gk * pr.sa_k] : T(0);
+    }
+  }
+}
+template <typename T>
+__device__ __forceinline__ void store_a(T (*As)[LDP], const T (&r)[4], bool contig_k, int tid) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = tid + 256 * j;
+    if (contig_k) As[e & 15][e >> 4] = r[j];
+    else As[e >> 6][e & 63] = r[j];
+  }
+}
+template <typename T>
+__device__ __forceinline__ void load_b(T (&r)[4], const T* __restrict__ Bp, const GemmPair& pr, int col0, int n, int k0,
+                                       int tid) {
+  if (pr.sb_n == 1) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j, jj = e & 63, kk = e >> 6;
+      const int gj = col0 + jj, gk = k0 + kk;
+      r[j] = (gj < n && gk < pr.k) ? Bp[(int64_t)gk * pr.sb_k + gj] : T(0);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j, kk = e & 15, jj = e >> 4;
+      const int gj = col0 + jj, gk = k0 + kk;
+      r[j] = (gj < n && gk < pr.k) ? Bp[(int64_t)gk * pr.sb_k + (int64_t)gj * pr.sb_n] : T(0);
+    }
+  }
+}
+template <typename T>
+__device__ __forceinline__ void store_b(T (*Bs)[LDP], const T (&r)[4], bool contig_n, int tid) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = tid + 256 * j;
+    if (contig_n) Bs[e >> 6][e & 63] = r[j];
+    else Bs[e & 15][e >> 4] = r[j];
+  }
+}
+
+// 64x64 tiles of large blocks.  The (pair, 16-wide k panel) sequence of a tile (or of its pair range, when the
+// reduction is split over several work-groups) is software-pipelined: the next panel is loaded into registers while
+// the MFMAs run on the current one from one of two LDS buffers, one barrier per panel.  A tile with slot >= 0 writes
+// its unscaled partial sum to partial[slot] (64x64, row-major); gemm_reduce_kernel adds the slots of a tile in order.
 template <typename T, typename TC, bool Tab>
-__global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(256) void gemm_grouped_kernel(const BigTile* __restrict__ tiles, int ntiles,
                                                            const GemmTask* __restrict__ tasks,
                                                            const GemmPair* __restrict__ pairs, Operands ops,
-                                                           double alpha, int accumulate) {
+                                                           double alpha, int accumulate, T* __restrict__ partial) {
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T As[BK][LDP];
-  __shared__ T Bs[BK][LDP];
+  __shared__ T As[2][BK][LDP];
+  __shared__ T Bs[2][BK][LDP];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
 
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    const GemmTile tl = tiles[w];
+    const BigTile tl = tiles[w];
     const GemmTask tk = tasks[tl.task];
     const T* __restrict__ A = opnd<const T, Tab>(ops.a, tk.inst);
     const T* __restrict__ B = opnd<const T, Tab>(ops.b, tk.inst);
-    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int lr = lane & 15, lk = lane >> 4;
 
     acc_t acc[2][2];
 #pragma unroll
@@ -69,75 +132,102 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmTile* __res
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    const int pend = tk.pair_begin + tk.pair_count;
-    for (int p = tk.pair_begin; p < pend; ++p) {
-      const GemmPair pr = pairs[p];
-      const T* Ap = A + pr.a_off;
-      const T* Bp = B + pr.b_off;
-      const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
-      for (int k0 = 0; k0 < pr.k; k0 += BK) {
-        if (pr.sa_k == 1) {
-          // contiguous along k: 16 consecutive threads read one row segment
-#pragma unroll
-          for (int e = tid; e < BM * BK; e += 256) {
-            const int kk = e & 15, i = e >> 4;
-            const int gi = tl.row0 + i, gk = k0 + kk;
-            As[kk][i] = (gi < tk.m && gk < pr.k) ? Ap[gi * sam + gk] : T(0);
-          }
-        } else {
-#pragma unroll
-          for (int e = tid; e < BM * BK; e += 256) {
-            const int i = e & 63, kk = e >> 6;
-            const int gi = tl.row0 + i, gk = k0 + kk;
-            As[kk][i] = (gi < tk.m && gk < pr.k) ? Ap[gi * sam + gk * sak] : T(0);
-          }
-        }
-        if (pr.sb_n == 1) {
-#pragma unroll
-          for (int e = tid; e < BK * BN; e += 256) {
-            const int j = e & 63, kk = e >> 6;
-            const int gj = tl.col0 + j, gk = k0 + kk;
-            Bs[kk][j] = (gj < tk.n && gk < pr.k) ? Bp[gk * sbk + gj] : T(0);
-          }
-        } else {
-#pragma unroll
-          for (int e = tid; e < BK * BN; e += 256) {
-            const int kk = e & 15, j = e >> 4;
-            const int gj = tl.col0 + j, gk = k0 + kk;
-            Bs[kk][j] = (gj < tk.n && gk < pr.k) ? Bp[gk * sbk + gj * sbn] : T(0);
-          }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-          const T a0 = As[kk + lk][wr * 32 + lr];
-          const T a1 = As[kk + lk][wr * 32 + 16 + lr];
-          const T b0 = Bs[kk + lk][wc * 32 + lr];
-          const T b1 = Bs[kk + lk][wc * 32 + 16 + lr];
-          acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
-          acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
-          acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
-          acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
-        }
-        __syncthreads();
+    // prologue: first panel into buffer 0
+    int p = tl.pair_lo, k0 = 0;
+    GemmPair pr = pairs[p];
+    T ra[4], rb[4];
+    load_a(ra, A + pr.a_off, pr, tl.row0, tk.m, 0, tid);
+    load_b(rb, B + pr.b_off, pr, tl.col0, tk.n, 0, tid);
+    store_a(As[0], ra, pr.sa_k == 1, tid);
+    store_b(Bs[0], rb, pr.sb_n == 1, tid);
+    __syncthreads();
+    int buf = 0;
+    while (true) {
+      int np = p, nk0 = k0 + BK;
+      GemmPair npr = pr;
+      if (nk0 >= pr.k) {
+        np = p + 1;
+        nk0 = 0;
+        if (np < tl.pair_hi) npr = pairs[np];
       }
+      const bool more = np < tl.pair_hi;  // uniform over the work-group
+      if (more) {
+        load_a(ra, A + npr.a_off, npr, tl.row0, tk.m, nk0, tid);
+        load_b(rb, B + npr.b_off, npr, tl.col0, tk.n, nk0, tid);
+      }
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const T a0 = As[buf][kk + lk][wr * 32 + lr];
+        const T a1 = As[buf][kk + lk][wr * 32 + 16 + lr];
+        const T b0 = Bs[buf][kk + lk][wc * 32 + lr];
+        const T b1 = Bs[buf][kk + lk][wc * 32 + 16 + lr];
+        acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+        acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+        acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+        acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+      }
+      if (!more) break;
+      store_a(As[buf ^ 1], ra, npr.sa_k == 1, tid);
+      store_b(Bs[buf ^ 1], rb, npr.sb_n == 1, tid);
+      __syncthreads();
+      buf ^= 1;
+      p = np;
+      k0 = nk0;
+      pr = npr;
     }
 
-    // D layout of 16x16x4: lane l holds rows 4*(l/16)+r, column l%16.
+    if (tl.slot < 0) {
+      TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
+      for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-      for (int ni = 0; ni < 2; ++ni)
+        for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = tl.row0 + wr * 32 + mi * 16 + Mfma<T>::row(lk, r);
-          const int col = tl.col0 + wc * 32 + ni * 16 + lr;
-          if (row < tk.m && col < tk.n) {
-            const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
-            const double v = alpha * (double)acc[mi][ni][r];
-            C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+          for (int r = 0; r < 4; ++r) {
+            const int row = tl.row0 + wr * 32 + mi * 16 + Mfma<T>::row(lk, r);
+            const int col = tl.col0 + wc * 32 + ni * 16 + lr;
+            if (row < tk.m && col < tk.n) {
+              const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+              const double v = alpha * (double)acc[mi][ni][r];
+              C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+            }
           }
-        }
+    } else {
+      T* __restrict__ P = partial + (int64_t)tl.slot * (BM * BN);
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = wr * 32 + mi * 16 + Mfma<T>::row(lk, r);
+            const int col = wc * 32 + ni * 16 + lr;
+            P[row * BN + col] = acc[mi][ni][r];
+          }
+    }
+    __syncthreads();  // every wave is done with the LDS buffers before the next tile's prologue
+  }
+}
+
+// Sum of the partial tiles of split reductions, in slot order (deterministic), into C.
+template <typename T, typename TC, bool Tab>
+__global__ __launch_bounds__(256) void gemm_reduce_kernel(const RedTile* __restrict__ red, int nred,
+                                                          const GemmTask* __restrict__ tasks,
+                                                          const T* __restrict__ partial, Operands ops, double alpha,
+                                                          int accumulate) {
+  for (int w = blockIdx.x; w < nred; w += gridDim.x) {
+    const RedTile rt = red[w];
+    const GemmTask tk = tasks[rt.task];
+    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
+    for (int e = threadIdx.x; e < BM * BN; e += 256) {
+      const int row = rt.row0 + (e >> 6), col = rt.col0 + (e & 63);
+      if (row >= tk.m || col >= tk.n) continue;
+      double sum = 0.0;
+      for (int q = 0; q < rt.nslots; ++q) sum += (double)partial[(int64_t)(rt.slot0 + q) * (BM * BN) + e];
+      const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+      const double v = alpha * sum;
+      C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+    }
   }
 }
 
@@ -490,15 +580,23 @@ static inline int grid_for(int64_t n) {
 }
 
 template <typename T, typename TC>
-void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
-                 double alpha, int accumulate, hipStream_t s, int max_groups) {
+void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
+                 const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
+  T* part = static_cast<T*>(partial);
   if (ops.table)
     hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate);
+                       ntiles, tasks, pairs, ops, alpha, accumulate, part);
   else
     hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate);
+                       ntiles, tasks, pairs, ops, alpha, accumulate, part);
+  if (nred <= 0) return;
+  if (ops.table)
+    hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, true>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red, nred,
+                       tasks, part, ops, alpha, accumulate);
+  else
+    hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, false>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red, nred,
+                       tasks, part, ops, alpha, accumulate);
 }
 
 template <typename T, typename TC>
@@ -609,12 +707,16 @@ void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks,
                        ntiles, tasks, pairs, ops, alpha, accumulate);
 }
 
+#define MPX_INST_BIG(T, TC)                                                                                   \
+  template void launch_gemm<T, TC>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*, const GemmPair*, \
+                                   Operands, double, int, hipStream_t, int);
+MPX_INST_BIG(double, double)
+MPX_INST_BIG(float, float)
+MPX_INST_BIG(float, double)
+#undef MPX_INST_BIG
 #define MPX_INST_GEMM(FN, T, TC) \
   template void FN<T, TC>(const GemmTile*, int, const GemmTask*, const GemmPair*, Operands, double, int, hipStream_t, \
                            int);
-MPX_INST_GEMM(launch_gemm, double, double)
-MPX_INST_GEMM(launch_gemm, float, float)
-MPX_INST_GEMM(launch_gemm, float, double)
 MPX_INST_GEMM(launch_gemm_small, double, double)
 MPX_INST_GEMM(launch_gemm_small, float, float)
 MPX_INST_GEMM(launch_gemm_small, float, double)

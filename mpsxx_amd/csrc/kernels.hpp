@@ -47,6 +47,23 @@ struct GemmTile {
   int32_t pad;
 };
 
+// A 64x64 tile of a large block over the pair range [pair_lo, pair_hi) of its task; slot >= 0: the reduction is split
+// and this work-group writes its partial tile to slot `slot` of the step's partial buffer.
+struct BigTile {
+  int32_t task;
+  int32_t row0, col0;
+  int32_t pair_lo, pair_hi;
+  int32_t slot;
+};
+
+// Tile whose partial sums occupy slots [slot0, slot0 + nslots) (summed in that order).
+struct RedTile {
+  int32_t task;
+  int32_t row0, col0;
+  int32_t slot0, nslots;
+  int32_t pad;
+};
+
 // s2_off >= 0 marks a fused pair of two consecutive coefficient contractions: C[r] += S[s_off] * S2[s2_off] * V[..]
 struct AxpyPair {
   int64_t v_off, s_off, s2_off;
@@ -76,10 +93,11 @@ constexpr int kAxpyChunk = 1024;  // elements per work-group in the AXPY / copy 
 
 // Launchers.  T = storage/arithmetic type of A and B (double or float), TC = type of C.  max_groups > 0 caps the grid
 // (persistent form: each work-group loops over the task list with a grid stride); 0 launches one per work item.
-// 64x64 tiles, 4 waves, LDS-staged (large blocks).
+// 64x64 tiles, 4 waves, LDS-staged and software-pipelined (large blocks); split reductions write partial tiles
+// (type T, 64*64 per slot) that a second launch sums in slot order.
 template <typename T, typename TC>
-void launch_gemm(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
-                 double alpha, int accumulate, hipStream_t s, int max_groups = 0);
+void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
+                 const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups = 0);
 
 // 32x32 tiles, one wave each, operands loaded straight into the MFMA registers (blocks up to ~64 on a side).
 constexpr int kSmallTile = 32;

@@ -76,3 +76,21 @@ def test_hubbard12_from_fcidump_through_qc_mpo_matches_reference_log():
             break
         e_prev = e
     assert abs(e - (-11.8406372775523305)) < 2e-7, e
+
+
+@pytest.mark.parametrize("norb,M", [(8, 48), (12, 128)])
+def test_merged_sigma_qc_matches_oracle(norb, M):
+    """Merged-form sigma (src/compute_sigma_vector.hpp:86-96) with the merged QC operators at the middle bond: the
+    second GEMM reduces over every (bond operator, right sector) pair of an output block, which the planner splits
+    over several work-groups (partial tiles summed in a fixed order)."""
+    from mpsxx_amd import workloads
+    print("qc merged sigma", norb, M, flush=True)
+    Lm, Rm, psim, info = workloads.qc_merged_problem(norb, M, seed=3)
+    Lq, Rq, pq = Lm.to_qst(), Rm.to_qst(), psim.to_qst()
+    ref = oracle.sigma_merged(Lq, Rq, pq)
+    got = mx.compute_sigma_vector_merged(Lm, Rm, psim).to_qst()
+    assert got.block_set() == ref.block_set()
+    scale = max(float(np.max(np.abs(b))) for b in ref.blocks.values() if b.size)
+    assert got.max_abs_diff(ref) <= 1e-12 * scale
+    got32 = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=mx.FP32).to_qst()
+    assert got32.diff_norm(ref) <= 2e-5 * ref.norm()
