@@ -38,7 +38,9 @@ enum {
 /* ---- arithmetic of the block contractions ------------------------------------------------------------------------- */
 enum {
   MPSXX_FP64 = 0,   /* FP64 storage and MFMA (v_mfma_f64_16x16x4_f64)                                                */
-  MPSXX_FP32 = 1    /* FP32 storage and MFMA with FP32 accumulation; outputs accumulated into the FP64 tensor       */
+  MPSXX_FP32 = 1,   /* FP32 storage and MFMA with FP32 accumulation; outputs accumulated into the FP64 tensor       */
+  MPSXX_BF16X3 = 2  /* FP32 storage; blocks above the small-tile size multiply as split-BF16 pairs x = hi + lo, three */
+                    /* v_mfma_f32_16x16x32_bf16 per product (hi*hi + hi*lo + lo*hi), FP32 accumulation               */
 };
 
 /* ---- structure descriptor ----------------------------------------------------------------------------------------- */
@@ -186,8 +188,8 @@ int mpsxx_sigma_twosite_stats(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor
                               mpsxx_tensor wfn0, int precision, int64_t* flops, int64_t* bytes, int32_t* nsteps);
 /* Times every kernel launch of the two-site sigma plan separately (HIP events on the launch stream around each launch,
  * mean over `reps`, scratch output) and returns per launch: device ms, its share of the algorithmic flops and bytes,
- * the kernel (0 gemm_grouped_kernel 64x64, 1 gemm_small_kernel, 2 gemm_split_kernel, 3 axpy_grouped_kernel) and the
- * plan step it belongs to. */
+ * the kernel (0 gemm_grouped_kernel 64x64, 1 gemm_small_kernel, 2 gemm_split_kernel, 3 axpy_grouped_kernel,
+ * 4 gemm_x3_kernel) and the plan step it belongs to. */
 int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
                                 mpsxx_tensor wfn0, int precision, int reps, int32_t max_entries, double* ms,
                                 int64_t* flops, int64_t* bytes, int32_t* kernel, int32_t* step, int32_t* nentries,

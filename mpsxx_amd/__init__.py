@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmpsxx_amd.so")
 
 FP64 = 0
 FP32 = 1
+BF16X3 = 2
 
 _lib = None
 

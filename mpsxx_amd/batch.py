@@ -71,7 +71,7 @@ class SigmaBatch:
             pass
 
 
-KERNELS = ["gemm_grouped_kernel", "gemm_small_kernel", "gemm_split_kernel", "axpy_grouped_kernel"]
+KERNELS = ["gemm_grouped_kernel", "gemm_small_kernel", "gemm_split_kernel", "axpy_grouped_kernel", "gemm_x3_kernel"]
 
 
 def _profile_entries(call, cap=256):

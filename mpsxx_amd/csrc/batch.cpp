@@ -25,7 +25,8 @@ bool same_shape(const Layout& a, const Layout& b) { return a.rank == b.rank && a
 BatchSigma::BatchSigma(const std::vector<Operand6>& inst, int prec, hipStream_t s) : inst_(inst), prec_(prec) {
   n_ = (int)inst_.size();
   if (n_ <= 0) throw Error(MPSXX_ERR_ARG, "batch: no instances");
-  if (prec != MPSXX_FP64 && prec != MPSXX_FP32) throw Error(MPSXX_ERR_ARG, "batch: unknown precision");
+  if (prec != MPSXX_FP64 && prec != MPSXX_FP32 && prec != MPSXX_BF16X3)
+    throw Error(MPSXX_ERR_ARG, "batch: unknown precision");
   plans_.reserve(n_);
   for (int i = 0; i < n_; ++i) {
     const Operand6& o = inst_[i];
@@ -128,7 +129,7 @@ BatchSigma::BatchSigma(const std::vector<Operand6>& inst, int prec, hipStream_t 
   // FP32 input copies
   conv_chunks_.resize(5);
   nconv_chunks_.assign(5, 0);
-  if (prec_ == MPSXX_FP32) {
+  if (prec_ != MPSXX_FP64) {
     for (int idx = 0; idx < 5; ++idx) {
       std::vector<Chunk> ch;
       for (int i = 0; i < n_; ++i)
@@ -271,13 +272,13 @@ void BatchSigma::launch_one(int k, hipStream_t s, unsigned parts) {
   const int b2 = st.b2 >= 0 ? st.b2 : (st.b >= 0 ? st.b : st.a);
   Operands ops{T + (size_t)st.a * nm, st.b >= 0 ? T + (size_t)st.b * nm : nullptr, T + (size_t)b2 * nm,
                T + (size_t)st.c * nm, 1};
-  launch_step(st, ops, L.f32, L.cf32, s, parts);
+  launch_step(st, ops, L.f32, L.cf32, s, parts, prec_ == MPSXX_BF16X3);
 }
 
 void BatchSigma::enqueue(unsigned convert_mask, hipStream_t s, const std::vector<hipEvent_t>* events) {
   int e = 0;
   if (events) hip_check(hipEventRecord((*events)[e++], s), "event");
-  if (prec_ == MPSXX_FP32) {
+  if (prec_ != MPSXX_FP64) {
     const ConvSeg* segs =
         reinterpret_cast<const ConvSeg*>(table_.as<char>() + table_ptrs_ * sizeof(void*));
     for (int idx = 0; idx < 5; ++idx)
@@ -340,8 +341,9 @@ std::vector<PartTiming> BatchSigma::profile(int reps, hipStream_t s) {
   enqueue(~0u, s, nullptr);
   std::vector<PartTiming> res;
   for (int k = 0; k < (int)launches_.size(); ++k)
-    for (const KernelPart& kp : step_parts(launches_[k].st)) {
+    for (KernelPart kp : step_parts(launches_[k].st)) {
       const double ms = time_launches([&] { launch_one(k, s, kp.mask); }, reps, s);
+      if (prec_ == MPSXX_BF16X3 && kp.kind == KernelPart::GEMM64) kp.kind = KernelPart::GEMM_X3;
       res.push_back({k, kp, ms});
     }
   hip_check(hipStreamSynchronize(s), "profile sync");

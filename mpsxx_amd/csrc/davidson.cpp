@@ -227,10 +227,10 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
     ++niter;
     }
   };
-  if (prec == MPSXX_FP32) {
-    // mixed precision: FP32 sigma vectors (FP32 MFMA, FP32 accumulation inside each block GEMM) until the FP32 noise
-    // floor, then FP64 refinement from that vector; Gram-Schmidt, the subspace problem and residuals stay FP64
-    solve(MPSXX_FP32, std::max(tol, kMixedTolerance));
+  if (prec == MPSXX_FP32 || prec == MPSXX_BF16X3) {
+    // mixed precision: FP32 (or split-BF16) sigma vectors with FP32 accumulation inside each block GEMM until their
+    // noise floor, then FP64 refinement from that vector; Gram-Schmidt, the subspace problem and residuals stay FP64
+    solve(prec, std::max(tol, prec == MPSXX_FP32 ? kMixedTolerance : kMixedToleranceX3));
     st.nsigma_fp32 = st.nsigma;
     solve(MPSXX_FP64, tol);
   } else {

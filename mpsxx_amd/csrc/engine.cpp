@@ -683,9 +683,13 @@ Step make_copy_step(const std::vector<CopySeg>& segs, int x, int y, double alpha
 
 namespace {
 template <typename T, typename TC>
-void launch_step_t(const Step& st, const Operands& ops, unsigned parts, hipStream_t s) {
+void launch_step_t(const Step& st, const Operands& ops, unsigned parts, bool x3, hipStream_t s) {
   if (st.kind == Step::GEMM) {
-    if (parts & 1u)
+    if ((parts & 1u) && x3)
+      launch_gemm_x3<TC>(st.work.as<BigTile>(), st.nwork, st.work_red.as<RedTile>(), st.nwork_red, st.partial.p,
+                         st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s,
+                         st.max_groups);
+    else if (parts & 1u)
       launch_gemm<T, TC>(st.work.as<BigTile>(), st.nwork, st.work_red.as<RedTile>(), st.nwork_red, st.partial.p,
                          st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s,
                          st.max_groups);
@@ -702,10 +706,10 @@ void launch_step_t(const Step& st, const Operands& ops, unsigned parts, hipStrea
 }
 }  // namespace
 
-void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s, unsigned parts) {
-  if (!f32) launch_step_t<double, double>(st, ops, parts, s);
-  else if (cf32) launch_step_t<float, float>(st, ops, parts, s);
-  else launch_step_t<float, double>(st, ops, parts, s);
+void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s, unsigned parts, bool x3) {
+  if (!f32) launch_step_t<double, double>(st, ops, parts, false, s);
+  else if (cf32) launch_step_t<float, float>(st, ops, parts, x3, s);
+  else launch_step_t<float, double>(st, ops, parts, x3, s);
 }
 
 std::vector<KernelPart> step_parts(const Step& st) {
@@ -763,6 +767,7 @@ std::vector<KernelPart> step_parts(const Step& st) {
 const char* part_kernel_name(int kind) {
   switch (kind) {
     case KernelPart::GEMM64: return "gemm_grouped_kernel";
+    case KernelPart::GEMM_X3: return "gemm_x3_kernel";
     case KernelPart::GEMM_SMALL: return "gemm_small_kernel";
     case KernelPart::GEMM_SPLIT: return "gemm_split_kernel";
     case KernelPart::AXPY: return "axpy_grouped_kernel";
@@ -820,7 +825,7 @@ void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out,
     switch (st.kind) {
       case Step::GEMM:
       case Step::AXPY:
-        launch_step(st, step_operands(st, ptr), f32, cf32, s);
+        launch_step(st, step_operands(st, ptr), f32, cf32, s, 7u, P.precision == MPSXX_BF16X3);
         break;
       case Step::COPY:
         if (!f32 && !cf32)
@@ -867,8 +872,10 @@ std::vector<PartTiming> profile_plan_parts(Plan& P, const std::vector<const doub
     if (st.kind == Step::COPY) continue;
     const bool f32 = P.slot_prec[st.a] == 1, cf32 = P.slot_prec[st.c] == 1;
     const Operands ops = step_operands(st, ptr);
-    for (const KernelPart& kp : step_parts(st)) {
-      const double ms = time_launches([&] { launch_step(st, ops, f32, cf32, s, kp.mask); }, reps, s);
+    const bool x3 = P.precision == MPSXX_BF16X3;
+    for (KernelPart kp : step_parts(st)) {
+      const double ms = time_launches([&] { launch_step(st, ops, f32, cf32, s, kp.mask, x3); }, reps, s);
+      if (x3 && kp.kind == KernelPart::GEMM64) kp.kind = KernelPart::GEMM_X3;
       res.push_back({(int)si, kp, ms});
     }
   }

@@ -134,7 +134,7 @@ struct Step {
 // One kernel launch of a step, with its share of the step's algorithmic cost: flops 2*rows*cols*k summed over the
 // tiles of the launch (GEMM) or 2*len per pair (AXPY); bytes = 8 * (operand elements read once per tile + outputs).
 struct KernelPart {
-  enum Kind { GEMM64 = 0, GEMM_SMALL = 1, GEMM_SPLIT = 2, AXPY = 3 };
+  enum Kind { GEMM64 = 0, GEMM_SMALL = 1, GEMM_SPLIT = 2, AXPY = 3, GEMM_X3 = 4 };
   int kind;
   unsigned mask;  // `parts` bit of launch_step
   int items;
@@ -169,7 +169,9 @@ bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, i
 
 // Launch the kernels of one GEMM or AXPY step (FP64, FP32, or FP32 operands into an FP64 output); `parts` selects
 // the tile lists (bit 0: 64x64 tiles or the AXPY chunks, bit 1: small tiles, bit 2: split tiles).
-void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s, unsigned parts = 7u);
+// x3: FP32 operands of the 64x64 tiles multiplied as split-BF16 pairs (precision MPSXX_BF16X3).
+void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s, unsigned parts = 7u,
+                 bool x3 = false);
 
 // Execute a plan: inputs[i] are the FP64 device pointers of the bound inputs, out the FP64 output pointer.
 void run_plan(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s);

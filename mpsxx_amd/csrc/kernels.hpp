@@ -99,6 +99,12 @@ template <typename T, typename TC>
 void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
                  const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups = 0);
 
+// Same tiles with FP32 operands multiplied as split-BF16 pairs (three bf16 MFMAs per product, FP32 accumulation).
+template <typename TC>
+void launch_gemm_x3(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
+                    const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
+                    hipStream_t s, int max_groups = 0);
+
 // 32x32 tiles, one wave each, operands loaded straight into the MFMA registers (blocks up to ~64 on a side).
 constexpr int kSmallTile = 32;
 template <typename T, typename TC>

@@ -72,15 +72,18 @@ PlanCache& cache() {
 
 class PlanBuilder {
  public:
-  explicit PlanBuilder(int prec) { P_.precision = prec; }
+  explicit PlanBuilder(int prec) {
+    if (prec != MPSXX_FP64 && prec != MPSXX_FP32 && prec != MPSXX_BF16X3) throw Error(MPSXX_ERR_ARG, "unknown precision");
+    P_.precision = prec;
+  }
 
   int input(const Layout& L, int which) {
     const int id = (int)P_.slot_layout.size();
     P_.slot_layout.push_back(L);
     P_.slot_input.push_back(which);
-    P_.slot_prec.push_back(P_.precision == MPSXX_FP32 ? 1 : 0);
+    P_.slot_prec.push_back(P_.precision != MPSXX_FP64 ? 1 : 0);
     P_.slot_buf.emplace_back();
-    if (P_.precision == MPSXX_FP32) P_.slot_buf.back().alloc(std::max<int64_t>(L.nelem, 1) * sizeof(float));
+    if (P_.precision != MPSXX_FP64) P_.slot_buf.back().alloc(std::max<int64_t>(L.nelem, 1) * sizeof(float));
     P_.bytes += L.nelem * 8;
     return id;
   }
@@ -89,7 +92,7 @@ class PlanBuilder {
   int contract(int a, bool ca_conj, const std::vector<int>& ca, int b, bool cb_conj, const std::vector<int>& cb) {
     View va{&P_.slot_layout[a], ca_conj}, vb{&P_.slot_layout[b], cb_conj};
     ContractSym sym = contract_symbolic(va, ca, vb, cb, nullptr);
-    const int id = add_inter(sym.C, P_.precision == MPSXX_FP32 ? 1 : 0);
+    const int id = add_inter(sym.C, P_.precision != MPSXX_FP64 ? 1 : 0);
     P_.steps.push_back(make_contract_step(sym, a, b, id, 0, 1.0));
     P_.flops += sym.flops;
     return id;
@@ -117,7 +120,7 @@ class PlanBuilder {
     View vx{&s1.C, false}, vw2{&P_.slot_layout[w2], false};
     ContractSym s2 = contract_symbolic(vx, ca2, vw2, cb2, nullptr);
     Step fused;
-    const int prec = P_.precision == MPSXX_FP32 ? 1 : 0;
+    const int prec = P_.precision != MPSXX_FP64 ? 1 : 0;
     if (!nofuse && make_fused_axpy_step(s1, s2, y, w1, w2, (int)P_.slot_layout.size(), 0, fused)) {
       const int id = add_inter(s2.C, prec);
       P_.steps.push_back(std::move(fused));

@@ -92,8 +92,10 @@ struct DavidsonStats {
   double sigma_ms = 0.0;  // device time spent in sigma vectors
   double last_rnorm = 0.0;
 };
-// Residual tolerance (||r||^2) of the FP32 phase of a mixed-precision Davidson solve.
+// Residual tolerance (||r||^2) of the FP32 phase of a mixed-precision Davidson solve, and of the split-BF16 phase
+// (products good to ~2^-16 relative).
 constexpr double kMixedTolerance = 1.0e-9;
+constexpr double kMixedToleranceX3 = 1.0e-6;
 Layout davidson_layout(const std::vector<DavGroup>& groups, const Layout& psi);
 double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tensor& wfnc, double tol, int prec,
                         DavidsonStats& st, hipStream_t s);

@@ -273,3 +273,14 @@ def test_davidson_matches_oracle_and_dense():
     s = mx.compute_sigma_vector_twosite(g[0], g[1], g[2], g[3], dw).to_qst()
     rq = sum(float(np.sum(w.blocks[k] * s.blocks.get(k, 0))) for k in w.blocks)
     assert abs(rq - e.value) < 1e-8 * max(1.0, abs(e.value))
+
+
+@pytest.mark.parametrize("seed,dlo,dhi", [(12, 60, 150)])
+def test_sigma_twosite_split_bf16(seed, dlo, dhi):
+    """MPSXX_BF16X3 on blocks above the small-tile size: 1e-4 relative Frobenius norm against the oracle."""
+    rng = np.random.default_rng(seed)
+    lm, rm, le, re_, psi = random_twosite_problem(10, 4, rng, lambda q: int(rng.integers(dlo, dhi + 1)) if abs(q[1]) <= 2 else 0)
+    ref = oracle.sigma_twosite(lm, rm, le, re_, psi)
+    got = mx.compute_sigma_vector_twosite(D(lm), D(rm), D(le), D(re_), D(psi), precision=mx.BF16X3).to_qst()
+    assert got.block_set() == ref.block_set()
+    assert got.diff_norm(ref) <= 1e-4 * ref.norm()

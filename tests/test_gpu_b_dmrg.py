@@ -87,3 +87,24 @@ def test_device_dmrg_config1_matches_oracle_and_ed():
     e0 = ed.ground_energy(h, v, L, L, 0)
     assert abs(eg[-1] - er[-1]) <= 1e-10, (eg, er)
     assert abs(eg[-1] - e0) <= 1e-9, (eg, e0)
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_mixed_precision_sweeps_match_fp64(prec):
+    """Mixed precision (FP32 or split-BF16 sigma vectors, then FP64 refinement; FP64 Gram-Schmidt, subspace problem,
+    residuals and SVD): the energy after 4 sweeps of the open Hubbard chain L=16, M=128 agrees with the FP64 sweeps to
+    1e-8 (the bar stated for mixed precision)."""
+    print("mixed precision sweeps", prec, flush=True)
+    L, M = 16, 128
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    init = initial_mps(L, (L, 0), M, 0)
+    e64 = e_mx = None
+    d64, dmx = DeviceDMRG(mpos, init, M), DeviceDMRG(mpos, init, M)
+    nfp = 0
+    for _ in range(4):
+        e64, _ = d64.sweep(M, 1e-10, 0)
+        e_mx, st = dmx.sweep(M, 1e-10, prec)
+        nfp += st["nsigma_fp32"]
+    print("E fp64", e64, "mixed", e_mx, "low-precision sigma", nfp, flush=True)
+    assert nfp > 0
+    assert abs(e_mx - e64) <= 1e-8, (e_mx, e64)

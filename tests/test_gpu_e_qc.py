@@ -94,3 +94,19 @@ def test_merged_sigma_qc_matches_oracle(norb, M):
     assert got.max_abs_diff(ref) <= 1e-12 * scale
     got32 = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=mx.FP32).to_qst()
     assert got32.diff_norm(ref) <= 2e-5 * ref.norm()
+
+
+def test_merged_sigma_qc_split_bf16_close_to_fp64():
+    """MPSXX_BF16X3: the large blocks multiply as split-BF16 pairs (three bf16 MFMAs per product, ~2^-16 relative per
+    product); bar 1e-4 relative Frobenius norm against the FP64 device result (itself pinned to the oracle above)."""
+    from mpsxx_amd import workloads
+    print("qc merged sigma bf16x3", flush=True)
+    Lm, Rm, psim, info = workloads.qc_merged_problem(12, 128, seed=3)
+    ref = mx.compute_sigma_vector_merged(Lm, Rm, psim).to_qst()
+    got = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=mx.BF16X3).to_qst()
+    assert got.block_set() == ref.block_set()
+    rel = got.diff_norm(ref) / ref.norm()
+    print("bf16x3 rel", rel, flush=True)
+    assert rel <= 1e-4, rel
+    got32 = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=mx.FP32).to_qst()
+    assert rel >= 0.5 * got32.diff_norm(ref) / ref.norm()  # not more accurate than FP32: the split path really ran
