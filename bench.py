@@ -11,7 +11,9 @@ by the planner); `roofline` = the kernel with the most device time per sigma (al
 device time, HIP events on the launch stream) against its bound; `cpu_baseline` = the oracle (FP64 CPU restatement of
 the reference algorithm) on the same inputs on one host core.  Beside it: the same sigma with FP32 block GEMMs, and a
 batch of 64 independent chains (L in 16..64, U in {2,4,8}, M in {128,256,512}) applied sequentially in FP64 and as one
-grouped launch per step (FP64 and FP32, hipGraph replay, autotuned tiles).
+grouped launch per step (FP64 and FP32, hipGraph replay, autotuned tiles), and the merged-form sigma at the middle of
+a 24-orbital quantum-chemistry chain at M=1024 (synthetic integrals from seed 1, QC MPO with D_w=1226) in FP64, FP32 and
+split-BF16x3 with the operators resident.
 """
 import argparse
 import ctypes
@@ -27,7 +29,8 @@ sys.path.insert(0, ROOT)
 
 import mpsxx_amd as mx  # noqa: E402
 from mpsxx_amd import workloads  # noqa: E402
-from mpsxx_amd.batch import ALL_INPUTS, WFN_ONLY, SigmaBatch, dominant_kernel, sigma_twosite_profile  # noqa: E402
+from mpsxx_amd.batch import (ALL_INPUTS, WFN_ONLY, SigmaBatch, dominant_kernel, last_plan_profile,  # noqa: E402
+                             sigma_twosite_profile)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
@@ -125,6 +128,34 @@ def batched_leg(n, reps):
     return res
 
 
+def qc_leg(norb, M, reps):
+    """Merged-form sigma (src/compute_sigma_vector.hpp:86-96) at the middle of a QC chain, FP64 / FP32 / split-BF16."""
+    Lm, Rm, psim, info = workloads.qc_merged_problem(norb, M, 1)
+    res = dict(info)
+    ref = None
+    for prec, name in ((mx.FP64, "fp64"), (mx.FP32, "fp32"), (mx.BF16X3, "bf16x3")):
+        out = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=prec)  # converts every input once
+        mx.synchronize()
+        one = out.to_qst()
+        if ref is None:
+            ref = one
+        ent, pf = last_plan_profile(3)
+        roof, _ = roofline_of(ent, prec if prec != mx.BF16X3 else mx.FP32)
+        # operators resident (FP32 copies kept): only the wavefunction is refreshed per call, as inside Davidson
+        mask = 1 << 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            mx.compute_sigma_vector_merged_ex(Lm, Rm, psim, out, prec, mask)
+        mx.synchronize()
+        sec = (time.perf_counter() - t0) / reps
+        res[name] = dict(ms=sec * 1e3, tflops=pf / sec / 1e12, rel_diff_vs_fp64=rel_diff(one, ref), roofline=roof)
+        res["flops_per_sigma"] = pf
+        del out
+    res["speedup_fp32_vs_fp64"] = res["fp64"]["ms"] / res["fp32"]["ms"]
+    res["speedup_bf16x3_vs_fp64"] = res["fp64"]["ms"] / res["bf16x3"]["ms"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +168,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched leg (0: skip it)")
+    ap.add_argument("--qc-norb", type=int, default=24, help="orbitals of the quantum-chemistry leg (0: skip it)")
+    ap.add_argument("--qc-M", type=int, default=1024)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -166,6 +199,11 @@ def main():
     if args.batch > 0:
         print("batched leg: %d instances" % args.batch, file=sys.stderr, flush=True)
         batched = batched_leg(args.batch, 20)
+
+    qcres = None
+    if args.qc_norb > 0:
+        print("quantum-chemistry leg: norb=%d M=%d" % (args.qc_norb, args.qc_M), file=sys.stderr, flush=True)
+        qcres = qc_leg(args.qc_norb, args.qc_M, 5)
 
     cpu = None
     parity = None
@@ -212,6 +250,7 @@ def main():
         "parity": parity,
         "other_precision": other_leg,
         "batched": batched,
+        "qc_merged": qcres,
     }
     print(json.dumps(res))
 

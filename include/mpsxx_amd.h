@@ -108,6 +108,15 @@ int mpsxx_compute_sigma_vector_onesite(mpsxx_tensor mpo0, mpsxx_tensor lopr, mps
  * compute_sigma_vector.hpp:86-96 */
 int mpsxx_compute_sigma_vector_merged(mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor wfn0, mpsxx_tensor sgv0,
                                       int precision, void* stream);
+/* Reduced-precision variants for iterative callers whose operators stay fixed between calls (no reference
+ * counterpart; the reference recomputes nothing per precision): bit i of convert_mask set = refresh the FP32 copy of
+ * input i (argument order, e.g. 1 << 4 = the wavefunction of the two-site form, 1 << 2 of the merged form); a clear
+ * bit reuses the copy made by the previous call with the same structures and precision.  FP64 ignores the mask. */
+int mpsxx_compute_sigma_vector_twosite_ex(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
+                                          mpsxx_tensor wfn0, mpsxx_tensor sgv0, int precision, uint32_t convert_mask,
+                                          void* stream);
+int mpsxx_compute_sigma_vector_merged_ex(mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor wfn0, mpsxx_tensor sgv0,
+                                         int precision, uint32_t convert_mask, void* stream);
 /* overlap forms S|psi> for 3-, 4- and 2-index wavefunctions: compute_sigma_vector.hpp:25-35, 73-83, 99-109 */
 int mpsxx_compute_sigma_vector_overlap(mpsxx_tensor lsov, mpsxx_tensor rsov, mpsxx_tensor wfn0, mpsxx_tensor sgv0,
                                        void* stream);

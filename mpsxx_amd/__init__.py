@@ -119,6 +119,22 @@ def compute_sigma_vector_twosite(lmpo, rmpo, lopr, ropr, wfn0, sgv0=None, precis
     return sgv0
 
 
+def compute_sigma_vector_twosite_ex(lmpo, rmpo, lopr, ropr, wfn0, sgv0, precision, convert_mask):
+    """Reduced-precision two-site sigma refreshing only the FP32 copies of the inputs whose bit is set."""
+    sgv0 = _out(sgv0)
+    _check(lib().mpsxx_compute_sigma_vector_twosite_ex(_h(lmpo), _h(rmpo), _h(lopr), _h(ropr), _h(wfn0), _h(sgv0),
+                                                       ctypes.c_int(precision), ctypes.c_uint32(convert_mask), None))
+    return sgv0
+
+
+def compute_sigma_vector_merged_ex(lopr, ropr, wfn0, sgv0, precision, convert_mask):
+    """Reduced-precision merged sigma refreshing only the FP32 copies of the inputs whose bit is set."""
+    sgv0 = _out(sgv0)
+    _check(lib().mpsxx_compute_sigma_vector_merged_ex(_h(lopr), _h(ropr), _h(wfn0), _h(sgv0), ctypes.c_int(precision),
+                                                      ctypes.c_uint32(convert_mask), None))
+    return sgv0
+
+
 def compute_sigma_vector_onesite(mpo0, lopr, ropr, wfn0, sgv0=None, precision=FP64):
     """One-site form (src/compute_sigma_vector.hpp:10-22)."""
     sgv0 = _out(sgv0)

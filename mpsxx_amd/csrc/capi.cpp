@@ -279,6 +279,21 @@ int mpsxx_compute_sigma_vector_onesite(mpsxx_tensor mpo0, mpsxx_tensor lopr, mps
   return guard([&] { mpx::op_sigma_onesite(T(mpo0), T(lopr), T(ropr), T(wfn0), T(sgv0), precision, S(stream)); });
 }
 
+int mpsxx_compute_sigma_vector_twosite_ex(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
+                                          mpsxx_tensor wfn0, mpsxx_tensor sgv0, int precision, uint32_t convert_mask,
+                                          void* stream) {
+  return guard([&] {
+    mpx::op_sigma_twosite(T(lmpo), T(rmpo), T(lopr), T(ropr), T(wfn0), T(sgv0), precision, S(stream), convert_mask);
+  });
+}
+
+int mpsxx_compute_sigma_vector_merged_ex(mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor wfn0, mpsxx_tensor sgv0,
+                                         int precision, uint32_t convert_mask, void* stream) {
+  return guard([&] {
+    mpx::op_sigma_merged(T(lopr), T(ropr), T(wfn0), T(sgv0), precision, S(stream), convert_mask);
+  });
+}
+
 int mpsxx_compute_sigma_vector_merged(mpsxx_tensor lopr, mpsxx_tensor ropr, mpsxx_tensor wfn0, mpsxx_tensor sgv0,
                                       int precision, void* stream) {
   return guard([&] { mpx::op_sigma_merged(T(lopr), T(ropr), T(wfn0), T(sgv0), precision, S(stream)); });

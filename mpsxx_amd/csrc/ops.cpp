@@ -208,7 +208,7 @@ std::vector<uint64_t> make_key(int op, int prec, std::initializer_list<const Ten
 // Look up or build a plan, shape the output, run.
 template <typename BuildFn>
 void execute(const std::vector<uint64_t>& key, BuildFn build, int out_mode, std::vector<Tensor*> ins, Tensor& out,
-             hipStream_t s) {
+             hipStream_t s, unsigned convert_mask = ~0u) {
   CachedPlan* cp = cache().find(key);
   if (!cp) {
     auto np = std::make_unique<CachedPlan>();
@@ -221,7 +221,7 @@ void execute(const std::vector<uint64_t>& key, BuildFn build, int out_mode, std:
   else if (out.L.hash != P.out.hash) out.relayout(P.out, s);
   std::vector<const double*> ptrs;
   for (Tensor* t : ins) ptrs.push_back(t->ptr());
-  run_plan(P, ptrs, out.ptr(), s);
+  run_plan_ex(P, ptrs, out.ptr(), s, {}, convert_mask);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
   last_run().plan = &P;
@@ -443,12 +443,12 @@ Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& l
 }
 
 void op_sigma_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec,
-                      hipStream_t s) {
+                      hipStream_t s, unsigned convert_mask) {
   for (Tensor* t : {&lmpo, &rmpo, &lopr, &ropr, &wfn}) require(*t, "compute_sigma_vector");
   Plan& P = plan_sigma_twosite(lmpo.L, rmpo.L, lopr.L, ropr.L, wfn.L, existing(sgv), prec);
   if (sgv.empty) sgv.set_layout(P.out, s);
   else if (sgv.L.hash != P.out.hash) sgv.relayout(P.out, s);
-  run_plan(P, {lmpo.ptr(), rmpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s);
+  run_plan_ex(P, {lmpo.ptr(), rmpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s, {}, convert_mask);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
   last_run().plan = &P;
@@ -469,7 +469,8 @@ void op_sigma_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tens
   }, OUT_ACCUMULATE, {&mpo, &lopr, &ropr, &wfn}, sgv, s);
 }
 
-void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s) {
+void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s,
+                     unsigned convert_mask) {
   for (Tensor* t : {&lopr, &ropr, &wfn}) require(*t, "compute_sigma_vector");
   if (lopr.L.rank != 3 || ropr.L.rank != 3 || wfn.L.rank != 2)
     throw Error(MPSXX_ERR_ARG, "merged sigma: expects 3-index operators and a 2-index wavefunction");
@@ -481,7 +482,7 @@ void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int p
     const int s1 = B.contract(sl, false, {2}, sw, false, {0});
     B.contract_out(s1, false, {1, 2}, sr, false, {1, 2}, ex);
     return B.finish();
-  }, OUT_ACCUMULATE, {&lopr, &ropr, &wfn}, sgv, s);
+  }, OUT_ACCUMULATE, {&lopr, &ropr, &wfn}, sgv, s, convert_mask);
 }
 
 void op_sigma_overlap(Tensor& lsov, Tensor& rsov, Tensor& wfn, Tensor& sgv, hipStream_t s) {

@@ -39,10 +39,13 @@ void op_scal(double alpha, Tensor& x, hipStream_t s);
 void op_copy(Tensor& x, Tensor& y, hipStream_t s);
 
 // Hot path (reference: src/compute_sigma_vector.hpp, compute_diagonal_elements.hpp, renormalize.hpp)
+// convert_mask (reduced-precision plans): bit i set = refresh the FP32 copy of input i (in argument order); a clear bit
+// reuses the copy made by the previous call of the same plan
 void op_sigma_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec,
-                      hipStream_t s);
+                      hipStream_t s, unsigned convert_mask = ~0u);
 void op_sigma_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s);
-void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s);
+void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s,
+                     unsigned convert_mask = ~0u);
 void op_sigma_overlap(Tensor& lsov, Tensor& rsov, Tensor& wfn, Tensor& sgv, hipStream_t s);
 void op_sigma_conj(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, hipStream_t s);
 void op_diag_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Tensor& diag, hipStream_t s);
