@@ -225,54 +225,74 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(x - (float)hi);
 }
 
-__device__ __forceinline__ void x3_load_a(float (&r)[8], const float* __restrict__ Ap, const GemmPair& pr, int row0,
-                                          int m, int k0, int tid) {
-  if (pr.sa_k == 1) {
+// Staging of one 64x32 panel of A (rows x k) or of B (columns x k), 8 elements per thread, chosen so that global loads
+// stay coalesced and LDS stores are packed:
+//   k is the operand's unit-stride direction: thread t holds k pair 2*(t & 15) + {0, 1} of rows (t >> 4) + 16 j,
+//     j = 0..3 (16 lanes read 32 consecutive k of a row), stored as one 4-byte hi pair and one lo pair per row;
+//   otherwise: thread t holds k = 8*(t >> 6) + 0..7 of row t & 63 (a wave reads 64 consecutive rows), stored as one
+//     16-byte hi vector and one lo vector.
+__device__ __forceinline__ void x3_load(float (&r)[8], const float* __restrict__ P, int64_t s_rc, int64_t s_k, int klen,
+                                        int rc0, int nrc, int k0, bool k_fast, int tid) {
+  if (k_fast) {
+    const int kp = 2 * (tid & 15);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = tid + 256 * j, kk = e & 31, i = e >> 5;
-      const int gi = row0 + i, gk = k0 + kk;
-      r[j] = (gi < m && gk < pr.k) ? Ap[(int64_t)gi * pr.sa_m + gk] : 0.f;
+    for (int j = 0; j < 4; ++j) {
+      const int g = rc0 + (tid >> 4) + 16 * j;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int gk = k0 + kp + c;
+        r[2 * j + c] = (g < nrc && gk < klen) ? P[(int64_t)g * s_rc + (int64_t)gk * s_k] : 0.f;
+      }
     }
   } else {
+    const int g = rc0 + (tid & 63), k8 = 8 * (tid >> 6);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = tid + 256 * j, i = e & 63, kk = e >> 6;
-      const int gi = row0 + i, gk = k0 + kk;
-      r[j] = (gi < m && gk < pr.k) ? Ap[(int64_t)gi * pr.sa_m + (int64_t)gk * pr.sa_k] : 0.f;
+    for (int c = 0; c < 8; ++c) {
+      const int gk = k0 + k8 + c;
+      r[c] = (g < nrc && gk < klen) ? P[(int64_t)g * s_rc + (int64_t)gk * s_k] : 0.f;
     }
   }
+}
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void x3_store(__bf16 (*Hi)[XLD], __bf16 (*Lo)[XLD], const float (&r)[8], bool k_fast,
+                                         int tid) {
+  if (k_fast) {
+    const int kp = 2 * (tid & 15);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int rc = (tid >> 4) + 16 * j;
+      bf16x2 h, l;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        __bf16 hc, lc;
+        split_bf16(r[2 * j + c], hc, lc);
+        h[c] = hc;
+        l[c] = lc;
+      }
+      *reinterpret_cast<bf16x2*>(&Hi[rc][kp]) = h;
+      *reinterpret_cast<bf16x2*>(&Lo[rc][kp]) = l;
+    }
+  } else {
+    const int rc = tid & 63, k8 = 8 * (tid >> 6);
+    bf16x8 h, l;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      __bf16 hc, lc;
+      split_bf16(r[c], hc, lc);
+      h[c] = hc;
+      l[c] = lc;
+    }
+    *reinterpret_cast<bf16x8*>(&Hi[rc][k8]) = h;
+    *reinterpret_cast<bf16x8*>(&Lo[rc][k8]) = l;
+  }
+}
+__device__ __forceinline__ void x3_load_a(float (&r)[8], const float* __restrict__ Ap, const GemmPair& pr, int row0,
+                                          int m, int k0, int tid) {
+  x3_load(r, Ap, pr.sa_m, pr.sa_k, pr.k, row0, m, k0, pr.sa_k == 1, tid);
 }
 __device__ __forceinline__ void x3_load_b(float (&r)[8], const float* __restrict__ Bp, const GemmPair& pr, int col0,
                                           int n, int k0, int tid) {
-  if (pr.sb_n == 1) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = tid + 256 * j, jj = e & 63, kk = e >> 6;
-      const int gj = col0 + jj, gk = k0 + kk;
-      r[j] = (gj < n && gk < pr.k) ? Bp[(int64_t)gk * pr.sb_k + gj] : 0.f;
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = tid + 256 * j, kk = e & 31, jj = e >> 5;
-      const int gj = col0 + jj, gk = k0 + kk;
-      r[j] = (gj < n && gk < pr.k) ? Bp[(int64_t)gk * pr.sb_k + (int64_t)gj * pr.sb_n] : 0.f;
-    }
-  }
-}
-// (row or column, k) of element j of this thread, for the map its load used
-__device__ __forceinline__ void x3_store(__bf16 (*Hi)[XLD], __bf16 (*Lo)[XLD], const float (&r)[8], bool k_fast,
-                                         int tid) {
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int e = tid + 256 * j;
-    const int rc = k_fast ? (e >> 5) : (e & 63), kk = k_fast ? (e & 31) : (e >> 6);
-    __bf16 h, l;
-    split_bf16(r[j], h, l);
-    Hi[rc][kk] = h;
-    Lo[rc][kk] = l;
-  }
+  x3_load(r, Bp, pr.sb_n, pr.sb_k, pr.k, col0, n, k0, pr.sb_n != 1, tid);
 }
 
 template <typename TC, bool Tab>
