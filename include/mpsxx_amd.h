@@ -186,6 +186,12 @@ int mpsxx_dmrg_create(int nsites, const mpsxx_tensor* mpo, const mpsxx_tensor* i
 int mpsxx_dmrg_sweep(mpsxx_dmrg d, int max_states, double tolerance, int precision, double* energy,
                      mpsxx_sweep_stats* stats, void* stream);
 int mpsxx_dmrg_site(mpsxx_dmrg d, int site, mpsxx_tensor out, void* stream);
+/* Adjoint of the sigma-vector / renormalization contractions with respect to the MPO: grads[k] = <psi|dH/dW_k|psi>
+ * for the current state (W_k's block indices, quantum numbers negated).  The centre moves across the chain without
+ * truncation and back; sweeps can continue afterwards.  No reference counterpart. */
+int mpsxx_dmrg_mpo_gradient(mpsxx_dmrg d, const mpsxx_tensor* grads, void* stream);
+/* <psi|O|psi>/<psi|psi> for psi = prod_i mps[i] by left-to-right environment chains (src/expectation.hpp:55-114) */
+int mpsxx_expectation(int nsites, const mpsxx_tensor* mpo, const mpsxx_tensor* mps, double* value, void* stream);
 int mpsxx_dmrg_free(mpsxx_dmrg d);
 const char* mpsxx_dmrg_last_error(void);
 
@@ -227,6 +233,12 @@ int mpsxx_qc_mpo_site_entries(mpsxx_qcmpo m, int32_t site, int32_t* nl, int32_t*
                               int32_t* rq, int32_t* wl, int32_t* wr, double* mats);
 /* Site tensor as a device array (quantum shape (q_wl, q_n, -q_n, -q_wr), every non-zero element a 1x1x1x1 block) */
 int mpsxx_qc_mpo_site(mpsxx_qcmpo m, int32_t site, mpsxx_tensor out, void* stream);
+/* Energy derivatives from MPO gradients (mpsxx_dmrg_mpo_gradient): dh[p*K+q] = dE/dh_pq with h_pq and h_qp taken as
+ * independent (= sum_s <a+_ps a_qs>, the 1-RDM), *dlambda = dE/dlambda for v -> lambda v at lambda = 1 (= <V>), *energy
+ * = <H> rebuilt from the same derivatives (a consistency check).  No reference counterpart (the reference has no
+ * derivatives); the 1-RDM it gives is the one src/opex.cpp:72-111 computes by expectation values. */
+int mpsxx_qc_mpo_gradient(mpsxx_qcmpo m, int32_t nsites, const mpsxx_tensor* grads, double* dh, double* dlambda,
+                          double* energy, void* stream);
 int mpsxx_qc_mpo_free(mpsxx_qcmpo m);
 
 /* ---- batched two-site sigma vectors (many independent instances per launch) ------------------------------------- */

@@ -493,6 +493,17 @@ int mpsxx_qc_mpo_site(mpsxx_qcmpo m, int32_t site, mpsxx_tensor out, void* strea
   });
 }
 
+int mpsxx_qc_mpo_gradient(mpsxx_qcmpo m, int32_t nsites, const mpsxx_tensor* grads, double* dh, double* dlambda,
+                          double* energy, void* stream) {
+  return guard([&] {
+    const mpx::QcMpo& x = QM(m);
+    if (nsites != x.K || !grads || !dh) throw mpx::Error(MPSXX_ERR_ARG, "qc gradient: one tensor per site, dh required");
+    std::vector<const mpx::Tensor*> g;
+    for (int k = 0; k < nsites; ++k) g.push_back(&T(grads[k]));
+    mpx::qc_mpo_gradient(x, g, dh, dlambda, energy, S(stream));
+  });
+}
+
 int mpsxx_qc_mpo_free(mpsxx_qcmpo m) {
   return guard_host([&] {
     if (!m) return;

@@ -72,6 +72,7 @@ class MatTable {
     return {(int)mats_.size() - 1, sign};
   }
   const Mat& operator[](int id) const { return mats_[id]; }
+  int size() const { return (int)mats_.size(); }
 
  private:
   std::vector<Mat> mats_;
@@ -79,6 +80,7 @@ class MatTable {
 
 struct Term {
   double coeff;
+  int p1, q1;            // one-body term: orbitals (p, q); two-body: -1
   int n;                 // 2 or 4 operators
   int nsite;             // distinct operator sites, ascending
   int site[4];
@@ -178,6 +180,8 @@ QcMpo build_qc_mpo(int K, const double* h, const double* v, double ecore, bool s
   auto add_term = [&](double coeff, int n, const int* sos, const bool* dag) {
     Term t;
     t.coeff = coeff;
+    t.p1 = n == 2 ? sos[0] / 2 : -1;
+    t.q1 = n == 2 ? sos[1] / 2 : -1;
     t.n = n;
     t.nsite = 0;
     int sites[4];
@@ -269,7 +273,9 @@ QcMpo build_qc_mpo(int K, const double* h, const double* v, double ecore, bool s
 
   QcMpo out;
   out.K = K;
+  out.ecore = ecore;
   out.sites.resize(K);
+  const int idP = tab.intern(P).first, idI = tab.intern(I).first;
   for (int k = 0; k < K; ++k) {
     QcMpoSite& site = out.sites[k];
     const auto& cl = chans[k];
@@ -305,29 +311,42 @@ QcMpo build_qc_mpo(int K, const double* h, const double* v, double ecore, bool s
       if (l < 0 || r < 0) throw Error(MPSXX_ERR_ARG, "qc mpo: missing channel");
       if (cp.cat == CAT_I && cn.cat == CAT_I) continue;  // identity transition set above
       if (cp.cat == CAT_H) continue;
-      // the term's factor on site k and its canonical (sign-free) form
-      Mat Ok = I, canon = I;
+      // the term's factor on site k: Ok = ok_sgn * canon, canon = table[ok_id]
+      int ok_id = idI, ok_sgn = 1;
       int j = 0;
       while (j < t.nsite && t.site[j] < k) ++j;
       if (j < t.nsite && t.site[j] == k) {
-        canon = tab[t.id[j]];
-        Ok = canon;
-        if (t.sgn[j] < 0) for (double& x : Ok) x = -x;
+        ok_id = t.id[j];
+        ok_sgn = t.sgn[j];
       } else {
         int nl = 0;
         for (int i = 0; i < t.nsite; ++i)
           if (t.site[i] < k) nl += t.cnt[i];
-        if (nl % 2) { Ok = P; canon = P; }
+        if (nl % 2) ok_id = idP;
       }
+      const Mat& canon = tab[ok_id];
+      Mat Ok = canon;
+      if (ok_sgn < 0) for (double& x : Ok) x = -x;
       const int sprev = (cp.cat == CAT_N) ? cp.sign : 1;
+      auto record = [&](double mult) {
+        out.apps.push_back({k, l, r, ok_id, mult * ok_sgn, t.p1, t.q1, t.coeff});
+      };
       if (cn.cat == CAT_N) {
         set(l, r, canon);
       } else if (cn.cat == CAT_C) {
-        if (cp.cat == CAT_C) set(l, r, canon);
-        else add(l, r, t.coeff * cn.sign * sprev, Ok);
+        if (cp.cat == CAT_C) {
+          set(l, r, canon);
+        } else {
+          add(l, r, t.coeff * cn.sign * sprev, Ok);
+          record(cn.sign * sprev);
+        }
       } else {  // H
-        if (cp.cat == CAT_C) set(l, r, canon);
-        else add(l, r, t.coeff * sprev, Ok);
+        if (cp.cat == CAT_C) {
+          set(l, r, canon);
+        } else {
+          add(l, r, t.coeff * sprev, Ok);
+          record(sprev);
+        }
       }
     }
     std::vector<std::pair<uint64_t, const Mat*>> ent;
@@ -345,7 +364,47 @@ QcMpo build_qc_mpo(int K, const double* h, const double* v, double ecore, bool s
       site.mat.push_back(*e.second);
     }
   }
+  for (int i = 0; i < tab.size(); ++i) out.mats.push_back(tab[i]);
   return out;
+}
+
+void qc_mpo_gradient(const QcMpo& m, const std::vector<const Tensor*>& G, double* dh, double* dlambda, double* energy,
+                     hipStream_t s) {
+  const int K = m.K;
+  if ((int)G.size() != K) throw Error(MPSXX_ERR_ARG, "qc gradient: one gradient tensor per site expected");
+  // host copies of the gradient tensors: (wl, wr) -> 4x4 (bra, ket)
+  std::vector<std::unordered_map<uint64_t, Mat>> g(K);
+  for (int k = 0; k < K; ++k) {
+    const Tensor& t = *G[k];
+    if (t.empty) continue;
+    if (t.L.rank != 4) throw Error(MPSXX_ERR_ARG, "qc gradient: gradient tensors are 4-index");
+    std::vector<double> x(std::max<int64_t>(t.L.nelem, 1));
+    hip_check(hipMemcpyAsync(x.data(), t.ptr(), t.L.nelem * sizeof(double), hipMemcpyDeviceToHost, s), "download");
+    hip_check(hipStreamSynchronize(s), "download sync");
+    for (int64_t b = 0; b < t.L.nblocks(); ++b) {
+      const int32_t* idx = t.L.index(b);
+      if (t.L.bsize[b] != 1) throw Error(MPSXX_ERR_SHAPE, "qc gradient: blocks of an MPO gradient are 1x1x1x1");
+      const uint64_t key = ((uint64_t)(uint32_t)idx[0] << 32) | (uint32_t)idx[3];
+      auto it = g[k].find(key);
+      if (it == g[k].end()) it = g[k].emplace(key, mat_zero()).first;
+      it->second[idx[1] * 4 + idx[2]] = x[t.L.off[b]];
+    }
+  }
+  std::fill(dh, dh + (size_t)K * K, 0.0);
+  double dl = 0.0, e = m.ecore;
+  for (const QcTermApp& a : m.apps) {
+    auto it = g[a.site].find(((uint64_t)(uint32_t)a.wl << 32) | (uint32_t)a.wr);
+    if (it == g[a.site].end()) continue;
+    double d = 0.0;
+    const Mat& f = m.mats[a.mat];
+    for (int e16 = 0; e16 < 16; ++e16) d += it->second[e16] * f[e16];
+    d *= a.mult;
+    if (a.p >= 0) dh[(size_t)a.p * K + a.q] += d;
+    else dl += a.coeff * d;
+    e += a.coeff * d;
+  }
+  if (dlambda) *dlambda = dl;
+  if (energy) *energy = e;
 }
 
 void qc_mpo_tensor(const QcMpoSite& site, Tensor& out, hipStream_t s) {

@@ -153,6 +153,87 @@ double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStrea
   return emin;
 }
 
+// <psi| dH/dW_k |psi> for every site k (the adjoint of the sigma-vector contraction with respect to the MPO tensor),
+// from the state a sweep leaves (centre at site 0, right-canonical sites, right environments valid):
+//   sites 0..L-2, centre moved right site by site without truncation:
+//     G_i(wl,n1b,n1k,w') = sum L_i(lb,wl,lk) psi*(lb,n1b,n2b,rb) psi(lk,n1k,n2k,rk) W_{i+1}(w',n2b,n2k,wr) R(rb,wr,rk)
+//   last site, centre there:  G(wl,nb,nk,wr) = sum L(lb,wl,lk) c*(lb,nb,rb) c(lk,nk,rk) R(rb,wr,rk)
+// then the centre is moved back to site 0 (environments recomputed, as renormalize does), so sweeps can continue.
+// G_k has W_k's block indices with every quantum number negated (the dual space of W_k).
+void dmrg_mpo_gradient(Dmrg& D, const std::vector<Tensor*>& G, hipStream_t s) {
+  const int L = D.L;
+  if ((int)G.size() != L) throw Error(MPSXX_ERR_ARG, "dmrg gradient: one output tensor per site expected");
+  for (int i = 0; i < L - 1; ++i) {
+    Tensor psi;
+    twosite(*D.mps[i], *D.mps[i + 1], psi, s);
+    Tensor X, Y, Z;
+    op_contract(1.0, *D.mpo[i + 1], false, {3}, *D.renv[i + 2], false, {1}, 1.0, X, s);  // (w', n2b, n2k, rb, rk)
+    op_contract(1.0, psi, false, {2, 3}, X, false, {2, 4}, 1.0, Y, s);                  // (lk, n1k, w', n2b, rb)
+    op_contract(1.0, psi, true, {2, 3}, Y, false, {3, 4}, 1.0, Z, s);                   // (lb, n1b, lk, n1k, w')
+    *G[i] = Tensor();
+    op_contract(1.0, *D.lenv[i], false, {0, 2}, Z, false, {0, 2}, 1.0, *G[i], s);        // (wl, n1b, n1k, w')
+    Tensor U, V;
+    op_canonicalize_twosite(true, psi, U, V, 0, s);
+    *D.mps[i] = std::move(U);
+    *D.mps[i + 1] = std::move(V);
+    D.lenv[i + 1] = std::make_unique<Tensor>();
+    op_renormalize(true, *D.mpo[i], *D.lenv[i], *D.mps[i], *D.mps[i], *D.lenv[i + 1], MPSXX_FP64, s);
+  }
+  {
+    Tensor& c = *D.mps[L - 1];
+    Tensor X, Y;
+    op_contract(1.0, *D.lenv[L - 1], false, {0}, c, true, {0}, 1.0, X, s);           // (wl, lk, nb, rb)
+    op_contract(1.0, X, false, {1}, c, false, {0}, 1.0, Y, s);                       // (wl, nb, rb, nk, rk)
+    *G[L - 1] = Tensor();
+    op_contract(1.0, Y, false, {2, 4}, *D.renv[L], false, {0, 2}, 1.0, *G[L - 1], s);  // (wl, nb, nk, wr)
+  }
+  for (int i = L - 2; i >= 0; --i) {
+    Tensor psi;
+    twosite(*D.mps[i], *D.mps[i + 1], psi, s);
+    Tensor U, V;
+    op_canonicalize_twosite(false, psi, U, V, 0, s);
+    *D.mps[i] = std::move(U);
+    *D.mps[i + 1] = std::move(V);
+    D.renv[i + 1] = std::make_unique<Tensor>();
+    op_renormalize(false, *D.mpo[i + 1], *D.renv[i + 2], *D.mps[i + 1], *D.mps[i + 1], *D.renv[i + 1], MPSXX_FP64, s);
+  }
+  D.center = 0;
+  hip_check(hipStreamSynchronize(s), "gradient");
+}
+
+// <psi|O|psi> / <psi|psi> of the MPS psi = prod_i T_i by full left-to-right environment chains
+// (src/expectation.hpp:55-114 runs the same renormalize chain).
+double expectation(const std::vector<Tensor*>& mpo, const std::vector<Tensor*>& mps, hipStream_t s) {
+  const int L = (int)mps.size();
+  if (L < 1 || (int)mpo.size() != L) throw Error(MPSXX_ERR_ARG, "expectation: one MPO tensor per site expected");
+  std::unique_ptr<Tensor> E = boundary_env(s);
+  Tensor S;
+  {
+    Layout l;
+    l.set_shape(2, true, QN(), {{QN()}, {QN()}}, {{1}, {1}});
+    int32_t z[2] = {0, 0};
+    l.add_block(z);
+    l.finalize();
+    S.set_layout(l, s);
+    launch_fill(S.ptr(), 1, 1.0, s);
+  }
+  for (int i = 0; i < L; ++i) {
+    auto E1 = std::make_unique<Tensor>();
+    op_renormalize(true, *mpo[i], *E, *mps[i], *mps[i], *E1, MPSXX_FP64, s);
+    E = std::move(E1);
+    Tensor S1;
+    op_renormalize_overlap(true, S, *mps[i], *mps[i], S1, s);
+    S = std::move(S1);
+  }
+  if (E->L.nelem != 1 || S.L.nelem != 1)
+    throw Error(MPSXX_ERR_SHAPE, "expectation: the chain does not close on one scalar (MPO total quantum number?)");
+  double e = 0.0, n = 0.0;
+  hip_check(hipMemcpyAsync(&e, E->ptr(), sizeof(double), hipMemcpyDeviceToHost, s), "download");
+  hip_check(hipMemcpyAsync(&n, S.ptr(), sizeof(double), hipMemcpyDeviceToHost, s), "download");
+  hip_check(hipStreamSynchronize(s), "expectation");
+  return e / n;
+}
+
 void dmrg_free(Dmrg* D) { delete D; }
 
 }  // namespace mpx
@@ -215,6 +296,27 @@ int mpsxx_dmrg_sweep(mpsxx_dmrg d, int max_states, double tolerance, int precisi
       stats->nsigma_fp32 = st.nsigma_fp32;
       stats->pad = 0;
     }
+  });
+}
+
+int mpsxx_dmrg_mpo_gradient(mpsxx_dmrg d, const mpsxx_tensor* grads, void* stream) {
+  return dguard([&] {
+    if (!d || !grads) throw mpx::Error(MPSXX_ERR_ARG, "dmrg gradient: null argument");
+    std::vector<mpx::Tensor*> g;
+    for (int k = 0; k < d->d->L; ++k) g.push_back(&grads[k]->t);
+    mpx::dmrg_mpo_gradient(*d->d, g, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int mpsxx_expectation(int nsites, const mpsxx_tensor* mpo, const mpsxx_tensor* mps, double* value, void* stream) {
+  return dguard([&] {
+    if (!mpo || !mps || !value) throw mpx::Error(MPSXX_ERR_ARG, "expectation: null argument");
+    std::vector<mpx::Tensor*> m, a;
+    for (int k = 0; k < nsites; ++k) {
+      m.push_back(&mpo[k]->t);
+      a.push_back(&mps[k]->t);
+    }
+    *value = mpx::expectation(m, a, reinterpret_cast<hipStream_t>(stream));
   });
 }
 

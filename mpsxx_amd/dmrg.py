@@ -92,6 +92,22 @@ class DeviceDMRG:
             raise RuntimeError("mpsxx_dmrg_sweep: " + lib().mpsxx_dmrg_last_error().decode())
         return e.value, {k: getattr(st, k) for k, _ in SweepStats._fields_}
 
+    def mpo_gradient(self):
+        """[<psi| dH/dW_k |psi>] for every site (DTensors; W_k's block indices with quantum numbers negated)."""
+        grads = [DTensor() for _ in range(self.L)]
+        arr = (ctypes.c_void_p * self.L)(*[g.h.value for g in grads])
+        rc = lib().mpsxx_dmrg_mpo_gradient(self.h, arr, None)
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_mpo_gradient: " + lib().mpsxx_dmrg_last_error().decode())
+        return grads
+
+    def site_device(self, i):
+        out = DTensor()
+        rc = lib().mpsxx_dmrg_site(self.h, ctypes.c_int(i), out.h, None)
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_site: " + lib().mpsxx_dmrg_last_error().decode())
+        return out
+
     def site(self, i):
         out = DTensor()
         rc = lib().mpsxx_dmrg_site(self.h, ctypes.c_int(i), out.h, None)
@@ -106,3 +122,15 @@ class DeviceDMRG:
                 self.h = ctypes.c_void_p()
         except Exception:
             pass
+
+
+def expectation(mpos, mps):
+    """<psi|O|psi>/<psi|psi> for device MPO and MPS site tensors (DTensors), src/expectation.hpp:55-114."""
+    L = len(mps)
+    am = (ctypes.c_void_p * L)(*[t.h.value for t in mpos])
+    aa = (ctypes.c_void_p * L)(*[t.h.value for t in mps])
+    v = ctypes.c_double()
+    rc = lib().mpsxx_expectation(ctypes.c_int(L), am, aa, ctypes.byref(v), None)
+    if rc != 0:
+        raise RuntimeError("mpsxx_expectation: " + lib().mpsxx_dmrg_last_error().decode())
+    return v.value

@@ -139,6 +139,21 @@ class QcMpo:
         _check(lib().mpsxx_qc_mpo_site(self.h, ctypes.c_int32(k), t.h, None))
         return t
 
+    def devices(self):
+        return [self.site_device(k) for k in range(self.norb)]
+
+    def gradient(self, grads):
+        """(dh, dlambda, energy) from the MPO gradients of a state: dh[p, q] = sum_s <a+_ps a_qs>, dlambda = <V>,
+        energy = <H> (see mpsxx_qc_mpo_gradient)."""
+        K = self.norb
+        arr = (VP * K)(*[g.h.value for g in grads])
+        dh = np.zeros((K, K))
+        dl = ctypes.c_double()
+        e = ctypes.c_double()
+        _check(lib().mpsxx_qc_mpo_gradient(self.h, ctypes.c_int32(K), arr, dh.ctypes.data_as(VP), ctypes.byref(dl),
+                                           ctypes.byref(e), None))
+        return dh, dl.value, e.value
+
     def free(self):
         if self.h:
             lib().mpsxx_qc_mpo_free(self.h)

@@ -113,3 +113,17 @@ def ground_energy(h, v, L, nelec, twosz, ecore=0.0):
         return float(np.linalg.eigvalsh(H.toarray())[0])
     import scipy.sparse.linalg as sla
     return float(sla.eigsh(H, k=1, which="SA", tol=1e-12)[0][0])
+
+
+def ground_state(h, v, L, nelec, twosz, ecore=0.0):
+    """(E0, vector, states) of the (nelec, 2Sz) sector (dense diagonalization; small sectors only)."""
+    st = sector_states(L, nelec, twosz)
+    H = hamiltonian(h, v, L, ecore, st).toarray()
+    w, U = np.linalg.eigh(H)
+    return float(w[0]), U[:, 0], st
+
+
+def expect(h, v, L, vec, states):
+    """<vec| H(h, v) |vec> on the given sector states."""
+    H = hamiltonian(h, v, L, 0.0, states)
+    return float(vec @ (H @ vec))
