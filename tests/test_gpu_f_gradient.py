@@ -48,8 +48,10 @@ def test_gradient_matches_exact_rdm_and_expectation():
             one = np.zeros((K, K))
             one[p, q] = 1.0
             gamma[p, q] = ed.expect(one, None, K, vec, st)
-    assert np.max(np.abs(dh - gamma)) < 1e-8, np.max(np.abs(dh - gamma))
-    assert abs(dl - ed.expect(np.zeros((K, K)), v, K, vec, st)) < 1e-8
+    # the state's own error is first order here: Davidson stops at ||r||^2 < 1e-13, i.e. |psi - psi_exact| ~ 3e-7 / gap
+    # (gap 1.1), while the energy error is second order (< 1e-10)
+    assert np.max(np.abs(dh - gamma)) < 1e-6, np.max(np.abs(dh - gamma))
+    assert abs(dl - ed.expect(np.zeros((K, K)), v, K, vec, st)) < 1e-6
     # Hellmann-Feynman through the expectation routine on one-term MPOs, on the state the sweeps left
     mps = [dm.site_device(i) for i in range(K)]
     for (p, q) in [(0, 0), (1, 3), (2, 5), (4, 4)]:
