@@ -48,8 +48,12 @@ class Handle {
 // host container -> device tensor
 template <class Arr>
 Handle upload(const Arr& a, bool conj = false) {
-  if (a.size() == 0) return Handle();
   const int N = (int)a.qshape().size();
+  // a default-constructed container (no sectors on any index) is the empty tensor; one that carries q/qshape/dshape
+  // but no blocks (the reference's diag, built from the wavefunction's quantum numbers) keeps that structure
+  bool shaped = false;
+  for (int i = 0; i < N; ++i) shaped = shaped || !a.dshape(i).empty();
+  if (!shaped) return Handle();
   std::vector<int32_t> nsec(N), secq, secd, bidx;
   for (int i = 0; i < N; ++i) {
     const auto& qs = a.qshape(i);

@@ -427,9 +427,86 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const RedTile* __restr
   }
 }
 
+// Operand fragment of one (pair, 16-wide k panel) unit of a 32x32 tile, straight from global memory into MFMA source
+// registers: lane l takes A[row + l%16][k + l/16] and B[k + l/16][col + l%16] for four 4-wide k steps.
+template <typename T>
+struct Frag {
+  T a0[4], a1[4], b0[4], b1[4];
+};
+
+struct TileLanes {
+  int r0, r1, c0, c1;
+  bool vr0, vr1, vc0, vc1;
+  int lk;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_frag(Frag<T>& f, const T* __restrict__ A, const T* __restrict__ B,
+                                          const GemmPair& pr, int k0, const TileLanes& t) {
+  const T* a0p = A + pr.a_off + (int64_t)t.r0 * pr.sa_m;
+  const T* a1p = A + pr.a_off + (int64_t)t.r1 * pr.sa_m;
+  const T* b0p = B + pr.b_off + (int64_t)t.c0 * pr.sb_n;
+  const T* b1p = B + pr.b_off + (int64_t)t.c1 * pr.sb_n;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int gk = k0 + 4 * u + t.lk;
+    const bool vk = gk < pr.k;
+    const int64_t ao = (int64_t)gk * pr.sa_k, bo = (int64_t)gk * pr.sb_k;
+    f.a0[u] = (vk && t.vr0) ? a0p[ao] : T(0);
+    f.a1[u] = (vk && t.vr1) ? a1p[ao] : T(0);
+    f.b0[u] = (vk && t.vc0) ? b0p[bo] : T(0);
+    f.b1[u] = (vk && t.vc1) ? b1p[bo] : T(0);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void mma_frag(typename Mfma<T>::acc_t (&acc)[2][2], const Frag<T>& f, int kleft) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (4 * u >= kleft) break;  // uniform across the wave
+    acc[0][0] = Mfma<T>::op(f.a0[u], f.b0[u], acc[0][0]);
+    acc[0][1] = Mfma<T>::op(f.a0[u], f.b1[u], acc[0][1]);
+    acc[1][0] = Mfma<T>::op(f.a1[u], f.b0[u], acc[1][0]);
+    acc[1][1] = Mfma<T>::op(f.a1[u], f.b1[u], acc[1][1]);
+  }
+}
+
+// advance a (pair, 16-wide panel) position by `steps` panels across pair boundaries; false past the last pair
+__device__ __forceinline__ bool advance_units(const GemmPair* __restrict__ pairs, int pend, int& p, int& panel,
+                                              GemmPair& pr, int steps) {
+  panel += steps;
+  while (true) {
+    const int np = (pr.k + 15) / 16;
+    if (panel < np) return true;
+    panel -= np;
+    if (++p >= pend) return false;
+    pr = pairs[p];
+  }
+}
+
+template <typename T, typename TC>
+__device__ __forceinline__ void store_tile(TC* __restrict__ C, const GemmTask& tk, const GemmTile& tl,
+                                           const typename Mfma<T>::acc_t (&acc)[2][2], int lr, int lk, double alpha,
+                                           int accumulate) {
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = tl.row0 + mi * 16 + Mfma<T>::row(lk, r);
+        const int col = tl.col0 + ni * 16 + lr;
+        if (row < tk.m && col < tk.n) {
+          const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+          const double v = alpha * (double)acc[mi][ni][r];
+          C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+        }
+      }
+}
+
 // Small-block variant: one wave per 32x32 C tile (2x2 MFMA 16x16 blocks), operands read straight from global memory
-// into the MFMA source registers (lane l takes A[row + l%16][k + l/16] and B[k + l/16][col + l%16]); no LDS and no
-// barriers, so thousands of tiny blocks run at full occupancy.  Four independent waves per work-group.
+// into the MFMA source registers; the next (pair, panel) unit is loaded while the MFMAs of the current one run.  No
+// LDS and no barriers, so thousands of tiny blocks run at full occupancy.  Four independent waves per work-group.
 template <typename T, typename TC, bool Tab>
 __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restrict__ tiles, int ntiles,
                                                          const GemmTask* __restrict__ tasks,
@@ -444,68 +521,46 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
     const GemmTask tk = tasks[tl.task];
     const T* __restrict__ A = opnd<const T, Tab>(ops.a, tk.inst);
     const T* __restrict__ B = opnd<const T, Tab>(ops.b, tk.inst);
-    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
     acc_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-    const int r0 = tl.row0 + lr, r1 = tl.row0 + 16 + lr;
-    const int c0 = tl.col0 + lr, c1 = tl.col0 + 16 + lr;
-    const bool vr0 = r0 < tk.m, vr1 = r1 < tk.m, vc0 = c0 < tk.n, vc1 = c1 < tk.n;
+    TileLanes tlanes;
+    tlanes.r0 = tl.row0 + lr;
+    tlanes.r1 = tl.row0 + 16 + lr;
+    tlanes.c0 = tl.col0 + lr;
+    tlanes.c1 = tl.col0 + 16 + lr;
+    tlanes.vr0 = tlanes.r0 < tk.m;
+    tlanes.vr1 = tlanes.r1 < tk.m;
+    tlanes.vc0 = tlanes.c0 < tk.n;
+    tlanes.vc1 = tlanes.c1 < tk.n;
+    tlanes.lk = lk;
     const int pend = tk.pair_begin + tk.pair_count;
-    constexpr int KU = 4;  // k-panel of 16: all 16 operand loads of a lane are issued before the 16 MFMAs
-    for (int p = tk.pair_begin; p < pend; ++p) {
-      const GemmPair pr = pairs[p];
-      const T* Ap = A + pr.a_off;
-      const T* Bp = B + pr.b_off;
-      const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
-      const T* a0p = Ap + r0 * sam;
-      const T* a1p = Ap + r1 * sam;
-      const T* b0p = Bp + c0 * sbn;
-      const T* b1p = Bp + c1 * sbn;
-      for (int k0 = 0; k0 < pr.k; k0 += 4 * KU) {
-        T a0[KU], a1[KU], b0[KU], b1[KU];
-#pragma unroll
-        for (int u = 0; u < KU; ++u) {
-          const int gk = k0 + 4 * u + lk;
-          const bool vk = gk < pr.k;
-          const int64_t ao = gk * sak, bo = gk * sbk;
-          a0[u] = (vk && vr0) ? a0p[ao] : T(0);
-          a1[u] = (vk && vr1) ? a1p[ao] : T(0);
-          b0[u] = (vk && vc0) ? b0p[bo] : T(0);
-          b1[u] = (vk && vc1) ? b1p[bo] : T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < KU; ++u) {
-          if (k0 + 4 * u >= pr.k) break;  // uniform across the wave
-          acc[0][0] = Mfma<T>::op(a0[u], b0[u], acc[0][0]);
-          acc[0][1] = Mfma<T>::op(a0[u], b1[u], acc[0][1]);
-          acc[1][0] = Mfma<T>::op(a1[u], b0[u], acc[1][0]);
-          acc[1][1] = Mfma<T>::op(a1[u], b1[u], acc[1][1]);
-        }
-      }
+    int p = tk.pair_begin, panel = 0;
+    GemmPair pr = pairs[p];
+    Frag<T> cur;
+    load_frag(cur, A, B, pr, 0, tlanes);
+    while (true) {
+      int np = p, npanel = panel;
+      GemmPair npr = pr;
+      const bool more = advance_units(pairs, pend, np, npanel, npr, 1);  // uniform across the wave
+      Frag<T> nxt;
+      if (more) load_frag(nxt, A, B, npr, 16 * npanel, tlanes);
+      mma_frag<T>(acc, cur, pr.k - 16 * panel);
+      if (!more) break;
+      cur = nxt;
+      p = np;
+      panel = npanel;
+      pr = npr;
     }
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-      for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = tl.row0 + mi * 16 + Mfma<T>::row(lk, r);
-          const int col = tl.col0 + ni * 16 + lr;
-          if (row < tk.m && col < tk.n) {
-            const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
-            const double v = alpha * (double)acc[mi][ni][r];
-            C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
-          }
-        }
+    store_tile<T, TC>(opnd<TC, Tab>(ops.c, tk.inst), tk, tl, acc, lr, lk, alpha, accumulate);
   }
 }
 
 // K-split variant for 32x32 tiles with a long reduction (many pairs and/or large k): the four waves of a work-group
-// take every fourth (pair, 16-wide k panel) unit of the same tile, then the partial tiles are summed in LDS in a
-// fixed order (deterministic) and stored once.
+// take every fourth (pair, 16-wide k panel) unit of the same tile (each wave loading its next unit while the MFMAs of
+// the current one run), then the partial tiles are summed in LDS in a fixed order (deterministic) and stored once.
 template <typename T, typename TC, bool Tab>
 __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restrict__ tiles, int ntiles,
                                                          const GemmTask* __restrict__ tasks,
@@ -521,53 +576,40 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
     const GemmTask tk = tasks[tl.task];
     const T* __restrict__ A = opnd<const T, Tab>(ops.a, tk.inst);
     const T* __restrict__ B = opnd<const T, Tab>(ops.b, tk.inst);
-    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
     acc_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-    const int r0 = tl.row0 + lr, r1 = tl.row0 + 16 + lr;
-    const int c0 = tl.col0 + lr, c1 = tl.col0 + 16 + lr;
-    const bool vr0 = r0 < tk.m, vr1 = r1 < tk.m, vc0 = c0 < tk.n, vc1 = c1 < tk.n;
+    TileLanes tlanes;
+    tlanes.r0 = tl.row0 + lr;
+    tlanes.r1 = tl.row0 + 16 + lr;
+    tlanes.c0 = tl.col0 + lr;
+    tlanes.c1 = tl.col0 + 16 + lr;
+    tlanes.vr0 = tlanes.r0 < tk.m;
+    tlanes.vr1 = tlanes.r1 < tk.m;
+    tlanes.vc0 = tlanes.c0 < tk.n;
+    tlanes.vc1 = tlanes.c1 < tk.n;
+    tlanes.lk = lk;
     const int pend = tk.pair_begin + tk.pair_count;
-    constexpr int KU = 4;
-    int unit = 0;  // running (pair, panel) index; this wave takes unit % 4 == wave
-    for (int p = tk.pair_begin; p < pend; ++p) {
-      const GemmPair pr = pairs[p];
-      const int npanel = (pr.k + 4 * KU - 1) / (4 * KU);
-      const int first = (wave - unit % 4 + 4) % 4;  // first panel of this pair owned by this wave
-      unit += npanel;
-      if (first >= npanel) continue;
-      const T* Ap = A + pr.a_off;
-      const T* Bp = B + pr.b_off;
-      const int64_t sam = pr.sa_m, sak = pr.sa_k, sbk = pr.sb_k, sbn = pr.sb_n;
-      const T* a0p = Ap + r0 * sam;
-      const T* a1p = Ap + r1 * sam;
-      const T* b0p = Bp + c0 * sbn;
-      const T* b1p = Bp + c1 * sbn;
-      for (int panel = first; panel < npanel; panel += 4) {
-        const int k0 = panel * 4 * KU;
-        T a0[KU], a1[KU], b0[KU], b1[KU];
-#pragma unroll
-        for (int u = 0; u < KU; ++u) {
-          const int gk = k0 + 4 * u + lk;
-          const bool vk = gk < pr.k;
-          const int64_t ao = gk * sak, bo = gk * sbk;
-          a0[u] = (vk && vr0) ? a0p[ao] : T(0);
-          a1[u] = (vk && vr1) ? a1p[ao] : T(0);
-          b0[u] = (vk && vc0) ? b0p[bo] : T(0);
-          b1[u] = (vk && vc1) ? b1p[bo] : T(0);
-        }
-#pragma unroll
-        for (int u = 0; u < KU; ++u) {
-          if (k0 + 4 * u >= pr.k) break;
-          acc[0][0] = Mfma<T>::op(a0[u], b0[u], acc[0][0]);
-          acc[0][1] = Mfma<T>::op(a0[u], b1[u], acc[0][1]);
-          acc[1][0] = Mfma<T>::op(a1[u], b0[u], acc[1][0]);
-          acc[1][1] = Mfma<T>::op(a1[u], b1[u], acc[1][1]);
-        }
-      }
+    // this wave's first unit: the wave-th (pair, panel) of the task
+    int p = tk.pair_begin, panel = 0;
+    GemmPair pr = pairs[p];
+    bool have = advance_units(pairs, pend, p, panel, pr, wave);
+    Frag<T> cur;
+    if (have) load_frag(cur, A, B, pr, 16 * panel, tlanes);
+    while (have) {
+      int np = p, npanel = panel;
+      GemmPair npr = pr;
+      const bool more = advance_units(pairs, pend, np, npanel, npr, 4);
+      Frag<T> nxt;
+      if (more) load_frag(nxt, A, B, npr, 16 * npanel, tlanes);
+      mma_frag<T>(acc, cur, pr.k - 16 * panel);
+      if (!more) break;
+      cur = nxt;
+      p = np;
+      panel = npanel;
+      pr = npr;
     }
     if (wave > 0) {
 #pragma unroll
@@ -584,17 +626,10 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const T v4 = acc[mi][ni][r] + red[0][mi * 2 + ni][r][lane] + red[1][mi * 2 + ni][r][lane] +
-                         red[2][mi * 2 + ni][r][lane];
-            const int row = tl.row0 + mi * 16 + Mfma<T>::row(lk, r);
-            const int col = tl.col0 + ni * 16 + lr;
-            if (row < tk.m && col < tk.n) {
-              const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
-              const double v = alpha * (double)v4;
-              C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
-            }
-          }
+          for (int r = 0; r < 4; ++r)
+            acc[mi][ni][r] = acc[mi][ni][r] + red[0][mi * 2 + ni][r][lane] + red[1][mi * 2 + ni][r][lane] +
+                             red[2][mi * 2 + ni][r][lane];
+      store_tile<T, TC>(opnd<TC, Tab>(ops.c, tk.inst), tk, tl, acc, lr, lk, alpha, accumulate);
     }
     __syncthreads();  // red is rewritten by the next tile
   }
