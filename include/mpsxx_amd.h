@@ -186,6 +186,13 @@ int mpsxx_dmrg_create(int nsites, const mpsxx_tensor* mpo, const mpsxx_tensor* i
 int mpsxx_dmrg_sweep(mpsxx_dmrg d, int max_states, double tolerance, int precision, double* energy,
                      mpsxx_sweep_stats* stats, void* stream);
 int mpsxx_dmrg_site(mpsxx_dmrg d, int site, mpsxx_tensor out, void* stream);
+/* Form of the two-site step used by later sweeps: MPSXX_SWEEP_FOURINDEX (default; dmrg/make_sweep.cpp:113 with
+ * src/optimize.hpp:164-265: 4-index sigma, diag and renormalize) or MPSXX_SWEEP_MERGED (src/make_sweep.cpp:112 with
+ * optimize_twosite_merged, src/optimize.hpp:495-630: merged operators built once per site, merged sigma and diag,
+ * SVD of the merged wavefunction, environment = U~^+ L~ U~).  Same energies; the merged form is the reference's
+ * quantum-chemistry path and avoids the per-site planning of the 4-index MPO-coefficient steps for large D_w. */
+enum { MPSXX_SWEEP_FOURINDEX = 0, MPSXX_SWEEP_MERGED = 1 };
+int mpsxx_dmrg_set_form(mpsxx_dmrg d, int form);
 /* Adjoint of the sigma-vector / renormalization contractions with respect to the MPO: grads[k] = <psi|dH/dW_k|psi>
  * for the current state (W_k's block indices, quantum numbers negated).  The centre moves across the chain without
  * truncation and back; sweeps can continue afterwards.  No reference counterpart. */

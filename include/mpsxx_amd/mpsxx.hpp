@@ -18,6 +18,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mpsxx_amd.h"
@@ -45,38 +46,42 @@ class Handle {
   mpsxx_tensor h_ = nullptr;
 };
 
-// host container -> device tensor
+// host container -> device tensor.  A default-constructed container (no sectors on any index) is the empty tensor;
+// one that carries q/qshape but no blocks (the reference's diag, built as QSTArray(q, qshape) from the wavefunction,
+// optimize.hpp:67) keeps that structure.  with_blocks = false uploads the structure only.  Indices whose dense shape
+// is not set (QSTArray(q, qshape) leaves it empty) get zero-sized sectors; the ops take dims from their inputs.
 template <class Arr>
-Handle upload(const Arr& a, bool conj = false) {
+Handle upload(const Arr& a, bool with_blocks = true) {
   const int N = (int)a.qshape().size();
-  // a default-constructed container (no sectors on any index) is the empty tensor; one that carries q/qshape/dshape
-  // but no blocks (the reference's diag, built from the wavefunction's quantum numbers) keeps that structure
   bool shaped = false;
-  for (int i = 0; i < N; ++i) shaped = shaped || !a.dshape(i).empty();
+  for (int i = 0; i < N; ++i) shaped = shaped || !a.qshape(i).empty();
   if (!shaped) return Handle();
   std::vector<int32_t> nsec(N), secq, secd, bidx;
   for (int i = 0; i < N; ++i) {
     const auto& qs = a.qshape(i);
     const auto& ds = a.dshape(i);
-    nsec[i] = (int32_t)ds.size();
-    for (size_t k = 0; k < ds.size(); ++k) {
-      secq.push_back(conj ? -qs[k].p() : qs[k].p());
-      secq.push_back(conj ? -qs[k].s() : qs[k].s());
-      secd.push_back((int32_t)ds[k]);
+    const bool has_dims = ds.size() == qs.size();
+    nsec[i] = (int32_t)qs.size();
+    for (size_t k = 0; k < qs.size(); ++k) {
+      secq.push_back(qs[k].p());
+      secq.push_back(qs[k].s());
+      secd.push_back(has_dims ? (int32_t)ds[k] : 0);
     }
   }
   std::vector<double> data;
-  for (auto it = a.begin(); it != a.end(); ++it) {
-    const auto idx = a.index(it->first);
-    for (int i = 0; i < N; ++i) bidx.push_back((int32_t)idx[i]);
-    const double* p = it->second->data();
-    data.insert(data.end(), p, p + it->second->size());
+  if (with_blocks) {
+    for (auto it = a.begin(); it != a.end(); ++it) {
+      const auto idx = a.index(it->first);
+      for (int i = 0; i < N; ++i) bidx.push_back((int32_t)idx[i]);
+      const double* p = it->second->data();
+      data.insert(data.end(), p, p + it->second->size());
+    }
   }
   mpsxx_qst_desc d;
   d.rank = N;
   d.qsym = 1;
-  d.q_total[0] = conj ? -a.q().p() : a.q().p();
-  d.q_total[1] = conj ? -a.q().s() : a.q().s();
+  d.q_total[0] = a.q().p();
+  d.q_total[1] = a.q().s();
   d.nsec = nsec.data();
   d.sec_q = secq.data();
   d.sec_dim = secd.data();
@@ -87,6 +92,14 @@ Handle upload(const Arr& a, bool conj = false) {
   Handle out(h);
   check(mpsxx_tensor_upload(h, data.data(), (int64_t)data.size(), nullptr));
   return out;
+}
+
+// Target of Axpy(1.0, x, diag) (STBLAS.h:94-145): a container without blocks is shaped from the first x (its dense
+// shapes may not be set yet), one with blocks is accumulated into.
+template <class Arr>
+Handle accumulator(const Arr& a) {
+  if (a.begin() == a.end()) return Handle();
+  return upload(a);
 }
 
 // device tensor -> host container (replaces its content)
@@ -170,16 +183,96 @@ void compute_sigma_vector(const Arr<double, 2, Q>& lopr, const Arr<double, 2, Q>
   download(f, sgv0);
 }
 
+//! Compute sigmavector: < phi | = < psi | H^*   (src/compute_sigma_vector.hpp:38-51).  The reference contracts into
+//! the temporary sgv0.conjugate(), so its result is discarded; here it lands in sgv0 (quantum numbers of the conjugate
+//! flipped back), which is what a caller of this form wants.
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_sigma_vector_conj(const Arr<double, 4, Q>& mpo0, const Arr<double, 3, Q>& lopr,
+                               const Arr<double, 3, Q>& ropr, const Arr<double, 3, Q>& wfn0, Arr<double, 3, Q>& sgv0) {
+  using namespace mpsxx_amd;
+  Handle a = upload(mpo0), c = upload(lopr), d = upload(ropr), e = upload(wfn0), f = upload(sgv0);
+  check(mpsxx_compute_sigma_vector_conj(a.get(), c.get(), d.get(), e.get(), f.get(), nullptr));
+  download(f, sgv0);
+}
+
+//! Compute diagonal H elements for one-site algorithm (src/compute_diagonal_elements.hpp:11-32); diag carries the
+//! wavefunction's q/qshape (blocks, if any, are replaced, as Copy(scr, diag, 1) does)
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_diagonal_elements(const Arr<double, 4, Q>& mpo0, const Arr<double, 3, Q>& lopr,
+                               const Arr<double, 3, Q>& ropr, Arr<double, 3, Q>& diag) {
+  using namespace mpsxx_amd;
+  Handle a = upload(mpo0), c = upload(lopr), d = upload(ropr), f = upload(diag, false);
+  check(mpsxx_compute_diagonal_elements_onesite(a.get(), c.get(), d.get(), f.get(), nullptr));
+  download(f, diag);
+}
+
 //! Compute diagonal H elements for two-site algorithm (src/compute_diagonal_elements.hpp:35-61)
 template <template <typename, size_t, class> class Arr, class Q>
 void compute_diagonal_elements(const Arr<double, 4, Q>& lmpo, const Arr<double, 4, Q>& rmpo,
                                const Arr<double, 3, Q>& lopr, const Arr<double, 3, Q>& ropr, Arr<double, 4, Q>& diag) {
   using namespace mpsxx_amd;
-  Handle a = upload(lmpo), b = upload(rmpo), c = upload(lopr), d = upload(ropr);
-  // diag carries q/qshape but may hold no blocks: create it with its structure only
-  Handle f = upload(diag);
+  Handle a = upload(lmpo), b = upload(rmpo), c = upload(lopr), d = upload(ropr), f = upload(diag, false);
   check(mpsxx_compute_diagonal_elements_twosite(a.get(), b.get(), c.get(), d.get(), f.get(), nullptr));
   download(f, diag);
+}
+
+//! Compute diagonal H elements for merged algorithm (src/compute_diagonal_elements.hpp:64-80)
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_diagonal_elements(const Arr<double, 3, Q>& lopr, const Arr<double, 3, Q>& ropr, Arr<double, 2, Q>& diag) {
+  using namespace mpsxx_amd;
+  Handle c = upload(lopr), d = upload(ropr), f = upload(diag, false);
+  check(mpsxx_compute_diagonal_elements_merged(c.get(), d.get(), f.get(), nullptr));
+  download(f, diag);
+}
+
+//! Group sums (src/compute_diagonal_elements.hpp:83-179): diag += sum over operator groups g of the group's diagonal.
+//! One process holds every group here, so the reference's MPI reduce + broadcast has nothing left to combine.
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_diagonal_elements(const std::vector<Arr<double, 4, Q>>& mpo0, const std::vector<Arr<double, 3, Q>>& lopr,
+                               const std::vector<Arr<double, 3, Q>>& ropr, Arr<double, 3, Q>& diag) {
+  using namespace mpsxx_amd;
+  if (mpo0.size() != lopr.size() || mpo0.size() != ropr.size())
+    throw std::runtime_error("compute_diagonal_elements: operator group counts differ");
+  if (mpo0.empty()) return;
+  Handle out = accumulator(diag);
+  for (size_t g = 0; g < mpo0.size(); ++g) {
+    Handle a = upload(mpo0[g]), c = upload(lopr[g]), d = upload(ropr[g]), t = upload(diag, false);
+    check(mpsxx_compute_diagonal_elements_onesite(a.get(), c.get(), d.get(), t.get(), nullptr));
+    check(mpsxx_axpy(1.0, t.get(), out.get(), nullptr));
+  }
+  download(out, diag);
+}
+
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_diagonal_elements(const std::vector<Arr<double, 4, Q>>& lmpo, const std::vector<Arr<double, 4, Q>>& rmpo,
+                               const std::vector<Arr<double, 3, Q>>& lopr, const std::vector<Arr<double, 3, Q>>& ropr,
+                               Arr<double, 4, Q>& diag) {
+  using namespace mpsxx_amd;
+  if (lmpo.size() != rmpo.size() || lmpo.size() != lopr.size() || rmpo.size() != ropr.size())
+    throw std::runtime_error("compute_diagonal_elements: operator group counts differ");
+  if (lmpo.empty()) return;
+  Handle out = accumulator(diag);
+  for (size_t g = 0; g < lmpo.size(); ++g) {
+    Handle a = upload(lmpo[g]), b = upload(rmpo[g]), c = upload(lopr[g]), d = upload(ropr[g]), t = upload(diag, false);
+    check(mpsxx_compute_diagonal_elements_twosite(a.get(), b.get(), c.get(), d.get(), t.get(), nullptr));
+    check(mpsxx_axpy(1.0, t.get(), out.get(), nullptr));
+  }
+  download(out, diag);
+}
+
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_diagonal_elements(const std::vector<Arr<double, 3, Q>>& lopr, const std::vector<Arr<double, 3, Q>>& ropr,
+                               Arr<double, 2, Q>& diag) {
+  using namespace mpsxx_amd;
+  if (lopr.size() != ropr.size()) throw std::runtime_error("compute_diagonal_elements: operator group counts differ");
+  if (lopr.empty()) return;
+  Handle out = accumulator(diag);
+  for (size_t g = 0; g < lopr.size(); ++g) {
+    Handle c = upload(lopr[g]), d = upload(ropr[g]), t = upload(diag, false);
+    check(mpsxx_compute_diagonal_elements_merged(c.get(), d.get(), t.get(), nullptr));
+    check(mpsxx_axpy(1.0, t.get(), out.get(), nullptr));
+  }
+  download(out, diag);
 }
 
 //! Renormalize operator tensor (src/renormalize.hpp:10-32)
@@ -189,6 +282,27 @@ void renormalize(bool forward, const Arr<double, 4, Q>& mpo0, const Arr<double, 
   using namespace mpsxx_amd;
   Handle a = upload(mpo0), b = upload(opr0), c = upload(bra0), d = upload(ket0), f = upload(opr1);
   check(mpsxx_renormalize(forward ? 1 : 0, a.get(), b.get(), c.get(), d.get(), f.get(), MPSXX_FP64, nullptr));
+  download(f, opr1);
+}
+
+//! Renormalize overlap matrix (src/renormalize.hpp:36-53) and merged overlap matrix (:78-95)
+template <template <typename, size_t, class> class Arr, class Q, size_t N>
+void renormalize(bool forward, const Arr<double, 2, Q>& sov0, const Arr<double, N, Q>& bra0,
+                 const Arr<double, N, Q>& ket0, Arr<double, 2, Q>& sov1) {
+  static_assert(N == 2 || N == 3, "renormalize(overlap): bra/ket have 3 indices (MPS site) or 2 (merged)");
+  using namespace mpsxx_amd;
+  Handle a = upload(sov0), c = upload(bra0), d = upload(ket0), f = upload(sov1);
+  check(mpsxx_renormalize_overlap(forward ? 1 : 0, a.get(), c.get(), d.get(), f.get(), nullptr));
+  download(f, sov1);
+}
+
+//! Renormalize merged operator matrix (src/renormalize.hpp:57-74)
+template <template <typename, size_t, class> class Arr, class Q>
+void renormalize(bool forward, const Arr<double, 3, Q>& opr0, const Arr<double, 2, Q>& bra0,
+                 const Arr<double, 2, Q>& ket0, Arr<double, 3, Q>& opr1) {
+  using namespace mpsxx_amd;
+  Handle a = upload(opr0), c = upload(bra0), d = upload(ket0), f = upload(opr1);
+  check(mpsxx_renormalize_merged(forward ? 1 : 0, a.get(), c.get(), d.get(), f.get(), MPSXX_FP64, nullptr));
   download(f, opr1);
 }
 

@@ -68,13 +68,22 @@ void sym_eig(int n, std::vector<double> a, std::vector<double>& w, std::vector<d
 
 }  // namespace
 
+namespace {
+
+Plan& group_plan(const DavGroup& g, const Layout& U, int prec) {
+  if (g.merged()) return plan_sigma_merged(g.lopr->L, g.ropr->L, U, &U, prec);
+  return plan_sigma_twosite(g.lmpo->L, g.rmpo->L, g.lopr->L, g.ropr->L, U, &U, prec);
+}
+
+}  // namespace
+
 // Closure of a block layout under the sigma pattern maps of all groups.
 Layout davidson_layout(const std::vector<DavGroup>& groups, const Layout& psi) {
   Layout U = psi;
   for (int iter = 0; iter < 16; ++iter) {
     Layout next = U;
     for (const DavGroup& g : groups) {
-      Plan& P = plan_sigma_twosite(g.lmpo->L, g.rmpo->L, g.lopr->L, g.ropr->L, U, &U, MPSXX_FP64);
+      Plan& P = group_plan(g, U, MPSXX_FP64);
       for (int64_t b = 0; b < P.out.nblocks(); ++b)
         if (next.find(P.out.tag[b]) < 0) next.add_block(P.out.index(b));
     }
@@ -88,6 +97,11 @@ Layout davidson_layout(const std::vector<DavGroup>& groups, const Layout& psi) {
 double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tensor& wfnc, double tol, int prec,
                         DavidsonStats& st, hipStream_t s) {
   if (wfnc.empty) throw Error(MPSXX_ERR_ARG, "Davidson: empty initial wavefunction");
+  if (groups.empty()) throw Error(MPSXX_ERR_ARG, "Davidson: no operator groups");
+  const bool merged = groups[0].merged();
+  for (const DavGroup& g : groups)
+    if (g.merged() != merged) throw Error(MPSXX_ERR_ARG, "Davidson: 4-index and merged groups mixed");
+  if (wfnc.L.rank != (merged ? 2 : 4)) throw Error(MPSXX_ERR_ARG, "Davidson: wavefunction rank does not fit the groups");
   const int max_ritz = 10;
   Layout U = davidson_layout(groups, wfnc.L);
   const int64_t n = U.nelem;
@@ -154,11 +168,15 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   auto multH = [&](const double* c, double* v) {
     hip_check(hipEventRecord(e0, s), "event");
     hip_check(hipMemsetAsync(v, 0, n * sizeof(double), s), "memset");
-    for (size_t g = 0; g < groups.size(); ++g)
-      run_plan_ex(*plans[g],
-                  {groups[g].lmpo->ptr(), groups[g].rmpo->ptr(), groups[g].lopr->ptr(), groups[g].ropr->ptr(), c}, v, s,
-                  {}, convert_mask);
-    convert_mask = 1u << 4;
+    for (size_t g = 0; g < groups.size(); ++g) {
+      const DavGroup& gr = groups[g];
+      if (merged)
+        run_plan_ex(*plans[g], {gr.lopr->ptr(), gr.ropr->ptr(), c}, v, s, {}, convert_mask);
+      else
+        run_plan_ex(*plans[g], {gr.lmpo->ptr(), gr.rmpo->ptr(), gr.lopr->ptr(), gr.ropr->ptr(), c}, v, s, {},
+                    convert_mask);
+    }
+    convert_mask = merged ? 1u << 2 : 1u << 4;  // from now on only the trial vector changes
     hip_check(hipEventRecord(e1, s), "event");
     hip_check(hipEventSynchronize(e1), "event sync");
     float ms = 0.f;
@@ -172,8 +190,7 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   // one Davidson solve from the current trial_0 with the sigma arithmetic of `p`
   auto solve = [&](int p, double tol_p) {
     plans.clear();
-    for (const DavGroup& g : groups)
-      plans.push_back(&plan_sigma_twosite(g.lmpo->L, g.rmpo->L, g.lopr->L, g.ropr->L, U, &U, p));
+    for (const DavGroup& g : groups) plans.push_back(&group_plan(g, U, p));
     convert_mask = ~0u;
     int niter = 0, iconv = 0;
     while (iconv < 1 && niter < 4) {

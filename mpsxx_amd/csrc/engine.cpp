@@ -5,7 +5,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <climits>
+#include <exception>
 #include <numeric>
+#include <thread>
 #include <unordered_map>
 
 #include "mpsxx_amd.h"
@@ -181,6 +184,43 @@ bool group_view(const std::vector<int>& g, const int64_t* dims, const int64_t* s
   return true;
 }
 
+// Host threads for the planner: MPSXX_HOST_THREADS if set, else the hardware concurrency, at most 16.
+int host_threads() {
+  static const int n = [] {
+    if (const char* e = std::getenv("MPSXX_HOST_THREADS")) return std::max(1, std::atoi(e));
+    const unsigned h = std::thread::hardware_concurrency();
+    return (int)std::min(16u, std::max(1u, h));
+  }();
+  return n;
+}
+
+// Run f(0..n-1), f(0) on the calling thread; the first exception is rethrown after all have finished.
+void parallel_for(int n, const std::function<void(int)>& f) {
+  if (n <= 1) {
+    if (n == 1) f(0);
+    return;
+  }
+  std::vector<std::exception_ptr> err(n);
+  std::vector<std::thread> th;
+  th.reserve(n - 1);
+  for (int c = 1; c < n; ++c)
+    th.emplace_back([&, c] {
+      try {
+        f(c);
+      } catch (...) {
+        err[c] = std::current_exception();
+      }
+    });
+  try {
+    f(0);
+  } catch (...) {
+    err[0] = std::current_exception();
+  }
+  for (auto& t : th) t.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
+
 }  // namespace
 
 void gemm_indices(int transa, int transb, int NA, int NB, int NC, std::vector<int>& ca, std::vector<int>& cb) {
@@ -254,108 +294,242 @@ ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const 
     if (!existing->same_dshape(C)) throw Error(MPSXX_ERR_SHAPE, "output dense shape mismatch");
   }
 
-  // group blocks by their contracted sector tuple
-  auto keyA = [&](const int32_t* idx) {
+  // Every contributing (A block, B block) pair shares the contracted sector tuple ("key").  The output tag is
+  // free_A * (sector count of B's free part) + free_B (row-major over C = A's free indices then B's), so walking the
+  // A blocks grouped by their free tuple in ascending order, and each group's contributions sorted by (free_B, key),
+  // yields the output blocks in ascending tag order and every block's pairs in ascending key order: the order of a
+  // sort of all contributions by (output tag, key), without one.
+  const int64_t nA = a.nblocks(), nB = b.nblocks();
+  auto tag_over = [](const Layout& L, const std::vector<int>& ids, const int32_t* idx) {
     int64_t t = 0;
-    for (int k = 0; k < K; ++k) t = t * a.nsec(ca[k]) + idx[ca[k]];
+    for (int x : ids) t = t * L.nsec(x) + idx[x];
     return t;
   };
-  auto keyB = [&](const int32_t* idx) {
-    int64_t t = 0;
-    for (int k = 0; k < K; ++k) t = t * b.nsec(cb[k]) + idx[cb[k]];
-    return t;
-  };
-  std::unordered_map<int64_t, std::vector<int64_t>> bmap;
-  bmap.reserve(b.nblocks() * 2 + 1);
-  for (int64_t j = 0; j < b.nblocks(); ++j) bmap[keyB(b.index(j))].push_back(j);
-
-  struct Contrib {
-    int64_t ctag, key, ia, ib;
-  };
-  std::vector<Contrib> con;
-  std::vector<int32_t> cidx(C.rank);
-  // sum of the C QNs for the allowed check
-  for (int64_t i = 0; i < a.nblocks(); ++i) {
-    const int32_t* ai = a.index(i);
-    const int64_t kk = keyA(ai);
-    auto it = bmap.find(kk);
-    if (it == bmap.end()) continue;
-    for (int64_t j : it->second) {
+  // per-block strided 2-D views and quantum numbers of the free parts
+  struct SideA { int64_t key, free; int64_t m, sm, k, sk; QN q; };
+  struct SideB { int64_t key, free; int64_t k, sk, n, sn; QN q; };
+  std::vector<SideA> A(nA);
+  std::vector<SideB> Bs(nB);
+  {
+    int64_t dims[kMaxRank], st[kMaxRank];
+    for (int64_t i = 0; i < nA; ++i) {
+      const int32_t* ai = a.index(i);
+      SideA& x = A[i];
+      x.key = tag_over(a, ca, ai);
+      x.free = tag_over(a, ua, ai);
+      block_strides(a, ai, dims, st);
+      if (!group_view(ua, dims, st, x.m, x.sm) || !group_view(ca, dims, st, x.k, x.sk))
+        x.m = -1;  // only an error if the block takes part in a pair
+      QN q;
+      for (int u : ua) q = q + Av.qs(u, ai[u]);
+      x.q = q;
+    }
+    for (int64_t j = 0; j < nB; ++j) {
       const int32_t* bj = b.index(j);
-      int n = 0;
-      for (int x : ua) cidx[n++] = ai[x];
-      for (int x : ub) cidx[n++] = bj[x];
-      if (!C.allowed(cidx.data())) continue;  // STBLAS.h:331 (c.allowed)
-      con.push_back({C.compute_tag(cidx.data()), kk, i, j});
+      SideB& y = Bs[j];
+      y.key = tag_over(b, cb, bj);
+      y.free = tag_over(b, ub, bj);
+      block_strides(b, bj, dims, st);
+      if (!group_view(cb, dims, st, y.k, y.sk) || !group_view(ub, dims, st, y.n, y.sn)) y.n = -1;
+      QN q;
+      for (int u : ub) q = q + Bv.qs(u, bj[u]);
+      y.q = q;
     }
   }
-  std::sort(con.begin(), con.end(), [](const Contrib& x, const Contrib& y) {
-    return x.ctag != y.ctag ? x.ctag < y.ctag : x.key < y.key;
+  int64_t nfreeB = 1;
+  for (int u : ub) nfreeB *= b.nsec(u);
+  // B blocks by (key, quantum number of the free part), ascending free_B inside a list: an A block finds exactly
+  // the B blocks whose pairing with it gives an allowed output block (STBLAS.h:331, c.allowed)
+  struct KQ {
+    int64_t key;
+    int32_t p, s;
+    bool operator==(const KQ& o) const { return key == o.key && p == o.p && s == o.s; }
+  };
+  struct KQHash {
+    size_t operator()(const KQ& x) const {
+      return std::hash<int64_t>()(x.key * 1000003 + (int64_t)x.p * 4099 + x.s);
+    }
+  };
+  std::unordered_map<KQ, std::vector<int64_t>, KQHash> bmap;
+  bmap.reserve(nB * 2 + 1);
+  {
+    std::vector<int64_t> ord(nB);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return Bs[x].free < Bs[y].free; });
+    for (int64_t j : ord) {
+      const QN q = qsym ? Bs[j].q : QN();
+      bmap[KQ{Bs[j].key, q.p, q.s}].push_back(j);
+    }
+  }
+  const QN cq = C.q;
+  auto partners = [&](int64_t i) -> const std::vector<int64_t>* {
+    const QN need = qsym ? cq - A[i].q : QN();
+    auto it = bmap.find(KQ{A[i].key, need.p, need.s});
+    return it == bmap.end() ? nullptr : &it->second;
+  };
+  // A blocks by (free_A, key), then the boundaries of the free_A groups
+  std::vector<int64_t> aord(nA);
+  std::iota(aord.begin(), aord.end(), 0);
+  std::sort(aord.begin(), aord.end(), [&](int64_t x, int64_t y) {
+    return A[x].free != A[y].free ? A[x].free < A[y].free : A[x].key < A[y].key;
+  });
+  std::vector<size_t> gbeg;
+  for (size_t r = 0; r < aord.size(); ++r)
+    if (r == 0 || A[aord[r]].free != A[aord[r - 1]].free) gbeg.push_back(r);
+  const size_t ngroups = gbeg.size();
+  gbeg.push_back(aord.size());
+  // Per chunk of consecutive groups: output tags (ascending), their first pair, and the pairs.  Chunks run on
+  // several host threads when the contraction is large; their results are concatenated in order, so the plan is
+  // the same for any thread count.
+  struct Chunk {
+    std::vector<int64_t> tag;
+    std::vector<int64_t> first;  // chunk-local index of each output block's first pair
+    std::vector<GemmPair> pairs;
+    std::vector<int64_t> src;    // A block and B block of each output block's first pair
+    int64_t flops = 0;
+  };
+  std::vector<int64_t> gwork(ngroups, 0);
+  int64_t work = 0;
+  for (size_t g = 0; g < ngroups; ++g) {
+    for (size_t r = gbeg[g]; r < gbeg[g + 1]; ++r) {
+      const std::vector<int64_t>* pl = partners(aord[r]);
+      if (pl) gwork[g] += (int64_t)pl->size();
+    }
+    work += gwork[g];
+  }
+  const int nthreads = (int)std::max<int64_t>(
+      1, std::min<int64_t>({(int64_t)host_threads(), work / 200000, (int64_t)ngroups}));
+  std::vector<Chunk> chunks(nthreads);
+  // nthreads ranges of consecutive groups with about equal candidate-pair counts
+  std::vector<size_t> cut(nthreads + 1, ngroups);
+  cut[0] = 0;
+  {
+    int64_t acc = 0;
+    int c = 1;
+    for (size_t g = 0; g < ngroups && c < nthreads; ++g) {
+      acc += gwork[g];
+      if (acc * nthreads >= work * c) cut[c++] = g + 1;
+    }
+  }
+  // Order inside a group: the A blocks are walked in ascending key and each partner list is in ascending free_B, so a
+  // stable counting sort by free_B gives (free_B, key) order; a comparison sort is the fallback for huge free_B ranges.
+  const bool counting = nfreeB <= (int64_t(1) << 24);
+  parallel_for(nthreads, [&](int c) {
+    Chunk& out = chunks[c];
+    struct Contrib { int64_t bfree, key; int32_t ia, ib; };
+    std::vector<Contrib> loc, sorted;
+    std::vector<int32_t> cnt(counting ? (size_t)nfreeB : 0, 0);
+    std::vector<int64_t> seen;
+    int64_t cand = 0;
+    for (size_t g = cut[c]; g < cut[c + 1]; ++g) cand += gwork[g];
+    out.pairs.reserve(cand);
+    for (size_t g = cut[c]; g < cut[c + 1]; ++g) {
+      const int64_t af = A[aord[gbeg[g]]].free;
+      loc.clear();
+      for (size_t r = gbeg[g]; r < gbeg[g + 1]; ++r) {
+        const int64_t i = aord[r];
+        const std::vector<int64_t>* pl = partners(i);
+        if (!pl) continue;
+        for (int64_t j : *pl) loc.push_back({Bs[j].free, A[i].key, (int32_t)i, (int32_t)j});
+      }
+      if (counting) {
+        seen.clear();
+        for (const Contrib& x : loc)
+          if (cnt[x.bfree]++ == 0) seen.push_back(x.bfree);
+        std::sort(seen.begin(), seen.end());
+        int32_t pos = 0;
+        for (int64_t v : seen) {
+          const int32_t n = cnt[v];
+          cnt[v] = pos;
+          pos += n;
+        }
+        sorted.resize(loc.size());
+        for (const Contrib& x : loc) sorted[cnt[x.bfree]++] = x;
+        for (int64_t v : seen) cnt[v] = 0;
+        loc.swap(sorted);
+      } else {
+        std::sort(loc.begin(), loc.end(), [](const Contrib& x, const Contrib& y) {
+          return x.bfree != y.bfree ? x.bfree < y.bfree : x.key < y.key;
+        });
+      }
+      for (size_t r = 0; r < loc.size(); ++r) {
+        const SideA& x = A[loc[r].ia];
+        const SideB& y = Bs[loc[r].ib];
+        if (r == 0 || loc[r].bfree != loc[r - 1].bfree) {
+          out.tag.push_back(af * nfreeB + loc[r].bfree);
+          out.first.push_back((int64_t)out.pairs.size());
+          out.src.push_back(loc[r].ia);
+          out.src.push_back(loc[r].ib);
+        }
+        if (x.m < 0 || y.n < 0)
+          throw Error(MPSXX_ERR_UNSUPPORTED, "Contract: block permutation is not a strided 2-D view");
+        if (x.k != y.k) throw Error(MPSXX_ERR_SHAPE, "Contract: contracted block dims differ");
+        GemmPair gp;
+        gp.a_off = a.off[loc[r].ia];
+        gp.b_off = b.off[loc[r].ib];
+        gp.sa_m = (int32_t)x.sm;
+        gp.sa_k = (int32_t)x.sk;
+        gp.sb_k = (int32_t)y.sk;
+        gp.sb_n = (int32_t)y.sn;
+        gp.k = (int32_t)x.k;
+        gp.pad = 0;
+        // a unit group carries stride 0; the kernel only ever touches index 0 of it, and the stride choice merely
+        // selects which global->LDS thread mapping is used
+        out.pairs.push_back(gp);
+        out.flops += 2 * x.m * y.n * x.k;
+      }
+    }
   });
 
   // output structure: existing blocks U contributed blocks
   if (existing && existing->rank > 0) {
     for (int64_t k = 0; k < existing->nblocks(); ++k) C.add_block(existing->index(k));
   }
+  std::vector<int64_t> out_base(nthreads + 1, 0), pair_base(nthreads + 1, 0);
+  for (int c = 0; c < nthreads; ++c) {
+    out_base[c + 1] = out_base[c] + (int64_t)chunks[c].tag.size();
+    pair_base[c + 1] = pair_base[c] + (int64_t)chunks[c].pairs.size();
+    R.flops += chunks[c].flops;
+  }
+  const int64_t nout = out_base[nthreads], npairs = pair_base[nthreads];
+  if (npairs > INT32_MAX) throw Error(MPSXX_ERR_UNSUPPORTED, "Contract: more than 2^31 block pairs");
   {
-    int64_t last = -1;
-    for (const Contrib& x : con) {
-      if (x.ctag == last) continue;
-      last = x.ctag;
-      const int32_t* ai = a.index(x.ia);
-      const int32_t* bj = b.index(x.ib);
-      int n = 0;
-      for (int y : ua) cidx[n++] = ai[y];
-      for (int y : ub) cidx[n++] = bj[y];
-      C.add_block(cidx.data());
-    }
+    std::vector<int32_t> cidx(C.rank);
+    for (int c = 0; c < nthreads; ++c)
+      for (size_t t = 0; t < chunks[c].tag.size(); ++t) {
+        const int32_t* ai = a.index(chunks[c].src[2 * t]);
+        const int32_t* bj = b.index(chunks[c].src[2 * t + 1]);
+        int n = 0;
+        for (int y : ua) cidx[n++] = ai[y];
+        for (int y : ub) cidx[n++] = bj[y];
+        C.add_block(cidx.data());
+      }
   }
   C.finalize();
 
-  // tasks and strided pair views
-  int64_t dimsA[kMaxRank], stA[kMaxRank], dimsB[kMaxRank], stB[kMaxRank];
-  size_t p = 0;
-  while (p < con.size()) {
-    size_t q = p;
-    while (q < con.size() && con[q].ctag == con[p].ctag) ++q;
-    const int64_t cb_ = C.find(con[p].ctag);
-    GemmTask t;
-    t.c_off = C.off[cb_];
-    t.pair_begin = (int32_t)R.pairs.size();
-    t.pair_count = (int32_t)(q - p);
-    t.inst = 0;
-    t.pad = 0;
-    int64_t m = -1, n = -1;
-    for (size_t r = p; r < q; ++r) {
-      const int32_t* ai = a.index(con[r].ia);
-      const int32_t* bj = b.index(con[r].ib);
-      block_strides(a, ai, dimsA, stA);
-      block_strides(b, bj, dimsB, stB);
-      int64_t mm, sm, kk, sk, kb, skb, nn, sn;
-      if (!group_view(ua, dimsA, stA, mm, sm) || !group_view(ca, dimsA, stA, kk, sk) ||
-          !group_view(cb, dimsB, stB, kb, skb) || !group_view(ub, dimsB, stB, nn, sn))
-        throw Error(MPSXX_ERR_UNSUPPORTED, "Contract: block permutation is not a strided 2-D view");
-      if (kk != kb) throw Error(MPSXX_ERR_SHAPE, "Contract: contracted block dims differ");
-      if (m < 0) { m = mm; n = nn; }
-      GemmPair gp;
-      gp.a_off = a.off[con[r].ia];
-      gp.b_off = b.off[con[r].ib];
-      gp.sa_m = (int32_t)sm;
-      gp.sa_k = (int32_t)sk;
-      gp.sb_k = (int32_t)skb;
-      gp.sb_n = (int32_t)sn;
-      gp.k = (int32_t)kk;
-      gp.pad = 0;
-      // a unit group carries stride 0; the kernel only ever touches index 0 of it, and the stride choice merely
-      // selects which global->LDS thread mapping is used
-      R.pairs.push_back(gp);
-      R.flops += 2 * mm * nn * kk;
+  // tasks; without existing blocks the output blocks are exactly the contributed ones, in the same order
+  R.tasks.resize(nout);
+  R.pairs.resize(npairs);
+  const bool only_new = !(existing && existing->rank > 0);
+  parallel_for(nthreads, [&](int c) {
+    Chunk& ch = chunks[c];
+    std::copy(ch.pairs.begin(), ch.pairs.end(), R.pairs.begin() + pair_base[c]);
+    std::vector<GemmPair>().swap(ch.pairs);
+    for (size_t t = 0; t < ch.tag.size(); ++t) {
+      const int64_t o = out_base[c] + (int64_t)t;
+      const int64_t cb_ = only_new ? o : C.find(ch.tag[t]);
+      const int64_t lo = ch.first[t];
+      const int64_t hi = t + 1 < ch.tag.size() ? ch.first[t + 1] : pair_base[c + 1] - pair_base[c];
+      GemmTask& task = R.tasks[o];
+      task.c_off = C.off[cb_];
+      task.pair_begin = (int32_t)(pair_base[c] + lo);
+      task.pair_count = (int32_t)(hi - lo);
+      task.inst = 0;
+      task.pad = 0;
+      task.m = (int32_t)A[ch.src[2 * t]].m;
+      task.n = (int32_t)Bs[ch.src[2 * t + 1]].n;
     }
-    t.m = (int32_t)m;
-    t.n = (int32_t)n;
-    R.tasks.push_back(t);
-    p = q;
-  }
+  });
   return R;
 }
 

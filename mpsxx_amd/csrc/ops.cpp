@@ -469,20 +469,33 @@ void op_sigma_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tens
   }, OUT_ACCUMULATE, {&mpo, &lopr, &ropr, &wfn}, sgv, s);
 }
 
+Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wfn, const Layout* out, int prec) {
+  std::vector<uint64_t> key{OP_SIGMAM, (uint64_t)prec, lopr.hash, ropr.hash, wfn.hash, out ? out->hash : 0ull};
+  CachedPlan* cp = cache().find(key);
+  if (cp) return cp->plan;
+  auto np = std::make_unique<CachedPlan>();
+  PlanBuilder B(prec);
+  const int sl = B.input(lopr, 0), sr = B.input(ropr, 1), sw = B.input(wfn, 2);
+  // compute_sigma_vector.hpp:94-95: Gemm(N,N,lopr,wfn0) (K=1), Gemm(N,T,scr1,ropr) (K=2)
+  const int s1 = B.contract(sl, false, {2}, sw, false, {0});
+  B.contract_out(s1, false, {1, 2}, sr, false, {1, 2}, out);
+  np->plan = B.finish();
+  return cache().insert(key, std::move(np))->plan;
+}
+
 void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s,
                      unsigned convert_mask) {
   for (Tensor* t : {&lopr, &ropr, &wfn}) require(*t, "compute_sigma_vector");
   if (lopr.L.rank != 3 || ropr.L.rank != 3 || wfn.L.rank != 2)
     throw Error(MPSXX_ERR_ARG, "merged sigma: expects 3-index operators and a 2-index wavefunction");
-  const Layout* ex = existing(sgv);
-  execute(make_key(OP_SIGMAM, prec, {&lopr, &ropr, &wfn}, &sgv), [&] {
-    PlanBuilder B(prec);
-    const int sl = B.input(lopr.L, 0), sr = B.input(ropr.L, 1), sw = B.input(wfn.L, 2);
-    // compute_sigma_vector.hpp:94-95: Gemm(N,N,lopr,wfn0) (K=1), Gemm(N,T,scr1,ropr) (K=2)
-    const int s1 = B.contract(sl, false, {2}, sw, false, {0});
-    B.contract_out(s1, false, {1, 2}, sr, false, {1, 2}, ex);
-    return B.finish();
-  }, OUT_ACCUMULATE, {&lopr, &ropr, &wfn}, sgv, s, convert_mask);
+  Plan& P = plan_sigma_merged(lopr.L, ropr.L, wfn.L, existing(sgv), prec);
+  if (sgv.empty) sgv.set_layout(P.out, s);
+  else if (sgv.L.hash != P.out.hash) sgv.relayout(P.out, s);
+  run_plan_ex(P, {lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s, {}, convert_mask);
+  last_stats().flops = P.flops;
+  last_stats().bytes = P.bytes;
+  last_run().plan = &P;
+  last_run().inputs = {lopr.ptr(), ropr.ptr(), wfn.ptr()};
 }
 
 void op_sigma_overlap(Tensor& lsov, Tensor& rsov, Tensor& wfn, Tensor& sgv, hipStream_t s) {

@@ -49,24 +49,75 @@ void run_copy(const std::vector<CopySeg>& segs, const double* x, double* y, hipS
 
 }  // namespace
 
+namespace {
+
+struct Blk { int i, j, m, n; };  // merged (row sector, col sector) and its dims
+// A dense source sub-rectangle (R x C, row length ld) placed at (ro, co) of the padded slot of block k.
+struct Rect { int k; int64_t src, ld, R, C, ro, co; };
+
+double canonicalize_core(bool forward, const Layout& X, const MergeInfo& MR, const MergeInfo& MC,
+                         const std::vector<Blk>& blks, const std::vector<Rect>& rects, const double* src,
+                         int64_t src_elems, Tensor& lmps, Tensor& rmps, int M, hipStream_t s);
+
+}  // namespace
+
 double op_canonicalize_twosite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor& rmps, int M, hipStream_t s) {
   if (wfnc.empty || wfnc.L.rank != 4) throw Error(MPSXX_ERR_ARG, "canonicalize: expects a 4-index wavefunction");
   const Layout& X = wfnc.L;
   MergeInfo MR = merge_info({X.qs[0], X.qs[1]}, {X.ds[0], X.ds[1]});
   MergeInfo MC = merge_info({X.qs[2], X.qs[3]}, {X.ds[2], X.ds[3]});
-  // merged matrix blocks (row sector, col sector) and copy of psi into zero-padded batch slots
-  struct Blk { int i, j, m, n; };
+  // merged matrix blocks (row sector, col sector) that hold at least one psi block
   std::vector<Blk> blks;
   {
     std::vector<std::pair<int, int>> seen;
     for (int64_t b = 0; b < X.nblocks(); ++b) {
       const int32_t* xi = X.index(b);
       const int i = MR.p2m[xi[0] * MR.radix[1] + xi[1]], j = MC.p2m[xi[2] * MC.radix[1] + xi[3]];
-      if (std::find(seen.begin(), seen.end(), std::make_pair(i, j)) == seen.end()) seen.push_back({i, j});
+      seen.push_back({i, j});
     }
     std::sort(seen.begin(), seen.end());
+    seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
     for (auto& p : seen) blks.push_back({p.first, p.second, MR.dm[p.first], MC.dm[p.second]});
   }
+  std::vector<Rect> rects;
+  for (int64_t b = 0; b < X.nblocks(); ++b) {
+    const int32_t* xi = X.index(b);
+    const int pr = xi[0] * MR.radix[1] + xi[1], pc = xi[2] * MC.radix[1] + xi[3];
+    const Blk key{MR.p2m[pr], MC.p2m[pc], 0, 0};
+    const int k = (int)(std::lower_bound(blks.begin(), blks.end(), key, [](const Blk& x, const Blk& y) {
+                          return x.i != y.i ? x.i < y.i : x.j < y.j;
+                        }) - blks.begin());
+    // sub-block (pdim_r x pdim_c) at (poff_r, poff_c) of its merged block
+    rects.push_back({k, X.off[b], MC.pdim[pc], MR.pdim[pr], MC.pdim[pc], MR.poff[pr], MC.poff[pc]});
+  }
+  return canonicalize_core(forward, X, MR, MC, blks, rects, wfnc.ptr(), X.nelem, lmps, rmps, M, s);
+}
+
+double op_canonicalize_merged(bool forward, Tensor& wm, const Layout& shape4, Tensor& lmps, Tensor& rmps, int M,
+                              hipStream_t s) {
+  if (wm.empty || wm.L.rank != 2) throw Error(MPSXX_ERR_ARG, "canonicalize: expects a merged 2-index wavefunction");
+  if (shape4.rank != 4) throw Error(MPSXX_ERR_ARG, "canonicalize: the expanded shape must have 4 indices");
+  const Layout& X = shape4;
+  MergeInfo MR = merge_info({X.qs[0], X.qs[1]}, {X.ds[0], X.ds[1]});
+  MergeInfo MC = merge_info({X.qs[2], X.qs[3]}, {X.ds[2], X.ds[3]});
+  const Layout& W = wm.L;
+  if (W.qs[0] != MR.qm || W.qs[1] != MC.qm || W.ds[0] != MR.dm || W.ds[1] != MC.dm || W.q != X.q)
+    throw Error(MPSXX_ERR_SHAPE, "canonicalize: the merged wavefunction is not the merge of the given shape");
+  std::vector<Blk> blks;
+  std::vector<Rect> rects;
+  for (int64_t b = 0; b < W.nblocks(); ++b) {
+    const int32_t* wi = W.index(b);
+    blks.push_back({wi[0], wi[1], MR.dm[wi[0]], MC.dm[wi[1]]});
+    rects.push_back({(int)b, W.off[b], MC.dm[wi[1]], MR.dm[wi[0]], MC.dm[wi[1]], 0, 0});
+  }
+  return canonicalize_core(forward, X, MR, MC, blks, rects, wm.ptr(), W.nelem, lmps, rmps, M, s);
+}
+
+namespace {
+
+double canonicalize_core(bool forward, const Layout& X, const MergeInfo& MR, const MergeInfo& MC,
+                         const std::vector<Blk>& blks, const std::vector<Rect>& rects, const double* src,
+                         int64_t src_elems, Tensor& lmps, Tensor& rmps, int M, hipStream_t s) {
   const int nb = (int)blks.size();
   if (nb == 0) throw Error(MPSXX_ERR_ARG, "canonicalize: empty wavefunction");
   int mmax = 1, nmax = 1;
@@ -76,22 +127,12 @@ double op_canonicalize_twosite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor&
   DevBuf A(nb * sa * sizeof(double)), Sv(nb * (int64_t)kmax * sizeof(double)), U(nb * su * sizeof(double)),
       V(nb * sv * sizeof(double)), E(nb * (int64_t)std::max(kmax, 2) * sizeof(double)), info(nb * sizeof(rocblas_int));
   hip_check(hipMemsetAsync(A.p, 0, nb * sa * sizeof(double), s), "memset");
-  auto blk_of = [&](int i, int j) {
-    for (int k = 0; k < nb; ++k)
-      if (blks[k].i == i && blks[k].j == j) return k;
-    return -1;
-  };
   {
+    // each source rectangle into the zero-padded row-major (mmax x nmax) slot of its block
     std::vector<CopySeg> segs;
-    for (int64_t b = 0; b < X.nblocks(); ++b) {
-      const int32_t* xi = X.index(b);
-      const int pr = xi[0] * MR.radix[1] + xi[1], pc = xi[2] * MC.radix[1] + xi[3];
-      const int k = blk_of(MR.p2m[pr], MC.p2m[pc]);
-      // sub-block (pdim_r x pdim_c) at (poff_r, poff_c) of the padded row-major (mmax x nmax) slot k
-      rect_copy_segments(X.off[b], MC.pdim[pc], MR.pdim[pr], MC.pdim[pc],
-                         k * sa + (int64_t)MR.poff[pr] * nmax + MC.poff[pc], nmax, segs);
-    }
-    run_copy(segs, wfnc.ptr(), A.as<double>(), s);
+    for (const Rect& r : rects)
+      rect_copy_segments(r.src, r.ld, r.R, r.C, r.k * sa + r.ro * nmax + r.co, nmax, segs);
+    run_copy(segs, src, A.as<double>(), s);
   }
   // row-major A (m x n) is column-major A^T (n x m): its left vectors are the rows of Vt, its right ones give U
   rocblas_handle h = solver_handle();
@@ -129,8 +170,8 @@ double op_canonicalize_twosite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor&
   }
   for (int k = 0; k < nb; ++k)
     if (info_h[k] != 0) {
-      std::vector<double> x(X.nelem);
-      hip_check(hipMemcpy(x.data(), wfnc.ptr(), X.nelem * sizeof(double), hipMemcpyDeviceToHost), "download");
+      std::vector<double> x(src_elems);
+      hip_check(hipMemcpy(x.data(), src, src_elems * sizeof(double), hipMemcpyDeviceToHost), "download");
       int64_t bad = 0;
       double amax = 0.0;
       for (double v : x) {
@@ -241,5 +282,7 @@ double op_canonicalize_twosite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor&
   run_copy(vseg, U.as<double>(), rmps.ptr(), s);
   return dnorm;
 }
+
+}  // namespace
 
 }  // namespace mpx

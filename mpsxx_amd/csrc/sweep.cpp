@@ -27,6 +27,7 @@ struct Dmrg {
   std::vector<std::unique_ptr<Tensor>> mps; // site tensors (A left of the centre, wave at the centre, B right)
   std::vector<std::unique_ptr<Tensor>> lenv, renv;  // lenv[i]: left of site i; renv[i]: right of site i-1
   int center = 0;
+  bool merged = false;  // optimize_twosite_merged (src/make_sweep.cpp:112) instead of the 4-index form
 };
 
 namespace {
@@ -104,6 +105,56 @@ Dmrg* dmrg_create(int L, const std::vector<Tensor*>& mpo, const std::vector<Tens
   return D.release();
 }
 
+// One two-site step in the merged form (src/optimize.hpp:495-630, optimize_twosite_merged): the merged block
+// operators L~ = merge(L W_i) and R~ = merge(W_{i+1} R) are built once (compute_merged_operator, :273-368), psi is
+// merged to psi~((l,n1),(n2,r)), Davidson runs on the merged sigma vector (two GEMM families), the SVD splits psi~,
+// and the environment of the next site is U~^+ L~ U~ (renormalize of the merged operator, renormalize.hpp:57-74).
+// The merged-operator build is counted in renorm_s.
+void optimize_merged(Dmrg& D, int i, bool forward, Tensor& psi, int M, double tol, int prec, SweepStats& st,
+                     double& emin, hipStream_t s) {
+  auto t = clk::now();
+  Tensor Lm, Rm, wm;
+  op_merged_operator(true, *D.mpo[i], *D.lenv[i], Lm, s);
+  op_merged_operator(false, *D.mpo[i + 1], *D.renv[i + 2], Rm, s);
+  op_qst_merge(psi, 2, 2, wm, s);
+  st.renorm_s += since(t);
+  t = clk::now();
+  Tensor diag = shape_only(wm.L);
+  op_diag_merged(Lm, Rm, diag, s);
+  hip_check(hipStreamSynchronize(s), "diag");
+  st.diag_s += since(t);
+  t = clk::now();
+  DavidsonStats ds;
+  std::vector<DavGroup> g{{nullptr, nullptr, &Lm, &Rm}};
+  const double e = davidson_twosite(g, diag, wm, 0.1 * tol, prec, ds, s);
+  st.davidson_s += since(t);
+  st.nsigma += ds.nsigma;
+  st.nsigma_fp32 += ds.nsigma_fp32;
+  st.sigma_ms += ds.sigma_ms;
+  if (e < emin) emin = e;
+  t = clk::now();
+  Tensor U, V;
+  st.dweight += op_canonicalize_merged(forward, wm, psi.L, U, V, M, s);
+  st.svd_s += since(t);
+  st.max_bond = std::max(st.max_bond, bond_dim(U.L, 2));
+  t = clk::now();
+  if (forward) {
+    Tensor Um;
+    op_qst_merge(U, 2, 1, Um, s);  // U~((l,n1), s)
+    D.lenv[i + 1] = std::make_unique<Tensor>();
+    op_renormalize_merged(true, Lm, Um, Um, *D.lenv[i + 1], MPSXX_FP64, s);
+  } else {
+    Tensor Vm;
+    op_qst_merge(V, 1, 2, Vm, s);  // V~(s, (n2,r))
+    D.renv[i + 1] = std::make_unique<Tensor>();
+    op_renormalize_merged(false, Rm, Vm, Vm, *D.renv[i + 1], MPSXX_FP64, s);
+  }
+  hip_check(hipStreamSynchronize(s), "renormalize");
+  st.renorm_s += since(t);
+  *D.mps[i] = std::move(U);
+  *D.mps[i + 1] = std::move(V);
+}
+
 // One forward + one backward two-site sweep; returns the minimum site energy.
 double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStream_t s) {
   const int L = D.L;
@@ -113,6 +164,10 @@ double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStrea
     // sites (i, i+1); forward: mps[i] is the wave, mps[i+1] right-canonical; backward: mps[i] left, mps[i+1] wave
     Tensor psi;
     twosite(*D.mps[i], *D.mps[i + 1], psi, s);
+    if (D.merged) {
+      optimize_merged(D, i, forward, psi, M, tol, prec, st, emin, s);
+      return;
+    }
     auto t = clk::now();
     Tensor diag = shape_only(psi.L);
     op_diag_twosite(*D.mpo[i], *D.mpo[i + 1], *D.lenv[i], *D.renv[i + 2], diag, s);
@@ -296,6 +351,14 @@ int mpsxx_dmrg_sweep(mpsxx_dmrg d, int max_states, double tolerance, int precisi
       stats->nsigma_fp32 = st.nsigma_fp32;
       stats->pad = 0;
     }
+  });
+}
+
+int mpsxx_dmrg_set_form(mpsxx_dmrg d, int form) {
+  return dguard([&] {
+    if (!d || !d->d) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: null handle");
+    if (form != MPSXX_SWEEP_FOURINDEX && form != MPSXX_SWEEP_MERGED) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: unknown form");
+    d->d->merged = form == MPSXX_SWEEP_MERGED;
   });
 }
 

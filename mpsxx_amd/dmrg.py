@@ -67,13 +67,19 @@ class SweepStats(ctypes.Structure):
                 ("nsigma_fp32", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
-class DeviceDMRG:
-    """Device-resident two-site DMRG over one operator group."""
+SWEEP_FOURINDEX, SWEEP_MERGED = 0, 1
 
-    def __init__(self, mpos, init_mps, M):
+
+class DeviceDMRG:
+    """Device-resident two-site DMRG over one operator group.
+
+    mpos: host QSTs or device tensors (e.g. ``QcMpo.devices()``); form: SWEEP_FOURINDEX (dmrg/make_sweep.cpp) or
+    SWEEP_MERGED (src/make_sweep.cpp, optimize_twosite_merged: the reference's quantum-chemistry path)."""
+
+    def __init__(self, mpos, init_mps, M, form=SWEEP_FOURINDEX):
         L = len(mpos)
         self.L = L
-        self._mpo = [DTensor(w) for w in mpos]
+        self._mpo = [w if isinstance(w, DTensor) else DTensor(w) for w in mpos]
         init = [DTensor(a) for a in init_mps]
         arr_m = (ctypes.c_void_p * L)(*[t.h.value for t in self._mpo])
         arr_i = (ctypes.c_void_p * L)(*[t.h.value for t in init])
@@ -82,6 +88,10 @@ class DeviceDMRG:
         rc = lib().mpsxx_dmrg_create(ctypes.c_int(L), arr_m, arr_i, ctypes.c_int(M), ctypes.byref(self.h), None)
         if rc != 0:
             raise RuntimeError("mpsxx_dmrg_create: " + lib().mpsxx_dmrg_last_error().decode())
+        if form != SWEEP_FOURINDEX:
+            rc = lib().mpsxx_dmrg_set_form(self.h, ctypes.c_int(form))
+            if rc != 0:
+                raise RuntimeError("mpsxx_dmrg_set_form: " + lib().mpsxx_dmrg_last_error().decode())
 
     def sweep(self, M, tol, precision=0):
         e = ctypes.c_double()

@@ -1,5 +1,6 @@
 // Minimal stand-in for the btas::QSTArray API subset the drivers use (q, qshape, dshape, begin/end over
-// (tag, shared_ptr<block>) in ascending tag order, index, size, resize, reserve, clear), used to compile and exercise
+// (tag, shared_ptr<block>) in ascending tag order, index, size, resize, reserve, clear; tags run over the sector
+// counts of qshape, as STArray.h:338-348), used to compile and exercise
 // include/mpsxx_amd/mpsxx.hpp without the reference tree.  Quantum numbers: (particle, 2Sz) with p(), s().
 #pragma once
 
@@ -49,14 +50,14 @@ class QSTArray {
   std::array<int, N> index(int tag) const {
     std::array<int, N> idx;
     for (int i = (int)N - 1; i >= 0; --i) {
-      idx[i] = tag % (int)ds_[i].size();
-      tag /= (int)ds_[i].size();
+      idx[i] = tag % (int)qs_[i].size();
+      tag /= (int)qs_[i].size();
     }
     return idx;
   }
   int tag(const std::array<int, N>& idx) const {
     int t = 0;
-    for (size_t i = 0; i < N; ++i) t = t * (int)ds_[i].size() + idx[i];
+    for (size_t i = 0; i < N; ++i) t = t * (int)qs_[i].size() + idx[i];
     return t;
   }
   bool allowed(const std::array<int, N>& idx) const {
@@ -66,6 +67,13 @@ class QSTArray {
       s += qs_[i][idx[i]].s();
     }
     return p == q_.p() && s == q_.s();
+  }
+  // quantum numbers only, dense shapes unset (QSTArray(q, qshape))
+  void resize(const Q& q, const std::array<std::vector<Q>, N>& qs) {
+    q_ = q;
+    qs_ = qs;
+    for (auto& x : ds_) x.clear();
+    blocks_.clear();
   }
   void resize(const Q& q, const std::array<std::vector<Q>, N>& qs, const std::array<std::vector<int>, N>& ds, bool) {
     q_ = q;

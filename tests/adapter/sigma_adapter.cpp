@@ -63,6 +63,29 @@ void write_array(std::ostream& out, const mock::QSTArray<double, N, Q>& a) {
   }
 }
 
+// Every other drop-in overload, instantiated so that the compile test checks it against the container API (their
+// device paths are covered through the C ABI by the other tests).
+[[maybe_unused]] void instantiate_other_forms() {
+  mock::QSTArray<double, 4, Q> w4, s4;
+  mock::QSTArray<double, 3, Q> o3, a3, b3, c3;
+  mock::QSTArray<double, 2, Q> s2, t2, u2, v2;
+  mpsxx::compute_sigma_vector(w4, o3, a3, b3, c3);                  // one-site
+  mpsxx::compute_sigma_vector(o3, a3, s2, t2);                      // merged
+  mpsxx::compute_sigma_vector(s2, t2, b3, c3);                      // overlap, 3-index
+  mpsxx::compute_sigma_vector(s2, t2, w4, s4);                      // overlap, 4-index
+  mpsxx::compute_sigma_vector(s2, t2, u2, v2);                      // overlap, 2-index
+  mpsxx::compute_sigma_vector_conj(w4, o3, a3, b3, c3);
+  mpsxx::compute_diagonal_elements(w4, o3, a3, b3);                 // one-site
+  mpsxx::compute_diagonal_elements(o3, a3, s2);                     // merged
+  std::vector<mock::QSTArray<double, 4, Q>> wg;
+  std::vector<mock::QSTArray<double, 3, Q>> og, ag;
+  mpsxx::compute_diagonal_elements(wg, og, ag, b3);
+  mpsxx::compute_diagonal_elements(og, ag, s2);
+  mpsxx::renormalize(true, s2, a3, b3, t2);                         // overlap
+  mpsxx::renormalize(false, o3, s2, t2, a3);                        // merged operator
+  mpsxx::renormalize(true, s2, t2, u2, v2);                         // merged overlap
+}
+
 int main(int argc, char** argv) {
   if (argc != 3) {
     std::cerr << "usage: sigma_adapter in.txt out.txt\n";
@@ -70,7 +93,7 @@ int main(int argc, char** argv) {
   }
   std::ifstream in(argv[1]);
   mock::QSTArray<double, 4, Q> lmpo, rmpo, wfn, sgv;
-  mock::QSTArray<double, 4, Q> diag;
+  mock::QSTArray<double, 4, Q> diag, diag2;
   mock::QSTArray<double, 3, Q> lopr, ropr, ket, lnext;
   if (!read_array(in, lmpo) || !read_array(in, rmpo) || !read_array(in, lopr) || !read_array(in, ropr) ||
       !read_array(in, wfn) || !read_array(in, ket)) {
@@ -83,6 +106,12 @@ int main(int argc, char** argv) {
     diag.resize(wfn.q(), wfn.qshape(), wfn.dshape(), false);         // structure only, no blocks
     mpsxx::compute_diagonal_elements(lmpo, rmpo, lopr, ropr, diag);
     mpsxx::renormalize(true, lmpo, lopr, ket, ket, lnext);
+    // group-sum form with the same operators in two groups: 2 x diag
+    std::vector<mock::QSTArray<double, 4, Q>> lg{lmpo, lmpo}, rg{rmpo, rmpo};
+    std::vector<mock::QSTArray<double, 3, Q>> log{lopr, lopr}, rog{ropr, ropr};
+    diag2.resize(wfn.q(), wfn.qshape());  // as QSTArray(q, qshape): no dense shapes
+
+    mpsxx::compute_diagonal_elements(lg, rg, log, rog, diag2);
   } catch (const std::exception& e) {
     std::cerr << e.what() << "\n";
     return 4;
@@ -91,5 +120,6 @@ int main(int argc, char** argv) {
   write_array(out, sgv);
   write_array(out, diag);
   write_array(out, lnext);
+  write_array(out, diag2);
   return 0;
 }

@@ -3,8 +3,9 @@ container).  tests/adapter/mock_qstarray.hpp stands in for btas::QSTArray (the A
 tests/adapter/sigma_adapter.cpp calls the templates as src/optimize.hpp:60-75 does.
 
 * CPU: the header compiles against the container API and links against the library's exported symbols;
-* GPU: sigma (twice, accumulating), diagonal elements into a structure-only container and a forward renormalization
-  through the header agree with the oracle (identical block sets, 1e-12 of the output scale)."""
+* GPU: sigma (twice, accumulating), diagonal elements into a structure-only container, the two-group diagonal sum and
+  a forward renormalization through the header agree with the oracle (identical block sets, 1e-12 of the output
+  scale)."""
 import os
 import subprocess
 
@@ -88,13 +89,15 @@ def test_header_calls_match_oracle(tmp_path):
     r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     tok = iter(open(out).read().split())
-    sgv, diag, lnext = _read(tok), _read(tok), _read(tok)
+    sgv, diag, lnext, diag2 = _read(tok), _read(tok), _read(tok), _read(tok)
 
     ref = oracle.sigma_twosite(mpos[i], mpos[i + 1], lenv, renv, psi)
     ref2 = oracle.sigma_twosite(mpos[i], mpos[i + 1], lenv, renv, psi, sgv=ref.copy())
     _same(sgv, ref2)
     d0 = QST(psi.q, psi.qshape, psi.dshape)
-    _same(diag, oracle.diag_twosite(mpos[i], mpos[i + 1], lenv, renv, d0))
+    dref = oracle.diag_twosite(mpos[i], mpos[i + 1], lenv, renv, d0)
+    _same(diag, dref)
+    _same(diag2, dref.scaled(2.0))
     _same(lnext, oracle.renormalize(True, mpos[i], lenv, ket, ket))
     # the library's own ctypes path gives the same sigma
     dev = mx.compute_sigma_vector_twosite(*[mx.DTensor(x) for x in (mpos[i], mpos[i + 1], lenv, renv, psi)]).to_qst()

@@ -61,6 +61,48 @@ def test_device_dmrg_with_qc_mpo_matches_oracle_and_fci():
     assert abs(e - e_fci) < 1e-10, (e, e_fci)
 
 
+@pytest.mark.parametrize("prec", [mx.FP64, mx.FP32, mx.BF16X3])
+def test_device_dmrg_merged_form_matches_fourindex_and_fci(prec):
+    """Merged two-site sweeps (src/make_sweep.cpp:112, optimize_twosite_merged): merged operators, merged sigma and
+    diag, SVD of the merged wavefunction, merged renormalization.  Same inputs as the 4-index test above: the final
+    energy equals the oracle's 4-index sweeps and FCI to 1e-10 (FP64) / 1e-8 (mixed precision, FP64 refinement)."""
+    from mpsxx_amd.dmrg import SWEEP_MERGED
+    print("qc dmrg merged K=6 prec", prec, flush=True)
+    K, M = 6, 64
+    h, v = qc.synthetic_integrals(K, seed=5, nvec=8, hsig=0.2, lsig=0.25)
+    e_fci = ed.ground_energy(h, v, K, K, 0)
+    m = qc.QcMpo(h, v)
+    init = initial_mps(K, (K, 0), M, 0)
+    dm = DeviceDMRG(m.devices(), init, M, form=SWEEP_MERGED)
+    e = None
+    for _ in range(4):
+        e, st = dm.sweep(M, 1e-10, precision=prec)
+    tol = 1e-10 if prec == mx.FP64 else 1e-8
+    assert abs(e - e_fci) < tol, (e, e_fci)
+    if prec != mx.FP64:
+        assert st["nsigma_fp32"] > 0
+
+
+def test_device_dmrg_merged_form_hubbard_matches_fourindex():
+    """Hubbard L=8, U=4, M=64 (the compact MPO): merged and 4-index sweeps reach the same energy (ED to 1e-9)."""
+    from mpsxx_amd.dmrg import SWEEP_MERGED
+    from mpsxx_amd.mpo import hubbard_mpo
+    print("hubbard merged L=8", flush=True)
+    L, M = 8, 64
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    init = initial_mps(L, (L, 0), M, 0)
+    es = []
+    for form in (0, SWEEP_MERGED):
+        dm = DeviceDMRG(mpos, init, M, form=form)
+        for _ in range(4):
+            e, _ = dm.sweep(M, 1e-10)
+        es.append(e)
+    h, v = ed.hubbard_integrals(L, 1.0, 4.0)
+    e0 = ed.ground_energy(h, v, L, L, 0)
+    assert abs(es[0] - es[1]) < 1e-10, es
+    assert abs(es[1] - e0) < 1e-9, (es, e0)
+
+
 def test_hubbard12_from_fcidump_through_qc_mpo_matches_reference_log():
     print("hub12 fcidump", flush=True)
     norb, nelec, ms2, h, v, ecore = qc.read_fcidump(os.path.join(HERE, "golden", "hubbard12.FCIDUMP"))
