@@ -13,7 +13,9 @@ the reference algorithm) on the same inputs on one host core.  Beside it: the sa
 batch of 64 independent chains (L in 16..64, U in {2,4,8}, M in {128,256,512}) applied sequentially in FP64 and as one
 grouped launch per step (FP64 and FP32, hipGraph replay, autotuned tiles), and the merged-form sigma at the middle of
 a 24-orbital quantum-chemistry chain at M=1024 (synthetic integrals from seed 1, QC MPO with D_w=1226) in FP64, FP32 and
-split-BF16x3 with the operators resident.
+split-BF16x3 with the operators resident; and the sweep half of the metric: device-resident two-site sweeps of the
+L=32 chain over M 128x3, 256x3, 512x2 in FP64 and in mixed precision (FP32 sigma vectors + FP64 refinement), with
+the energy difference between them.
 """
 import argparse
 import ctypes
@@ -82,6 +84,11 @@ def rel_diff(a, b):
     return a.diff_norm(b) / b.norm()
 
 
+def qdot(a, b):
+    """<a|b> over the common blocks of two host block-sparse arrays."""
+    return float(sum(np.vdot(a.blocks[k], b.blocks[k]) for k in a.blocks if k in b.blocks))
+
+
 def batched_leg(n, reps):
     specs = workloads.batch_specs(n, 4)
     probs = [workloads.hubbard_twosite(L=L, M=M, U=U, seed=k) for k, (L, U, M) in enumerate(specs)]
@@ -116,7 +123,18 @@ def batched_leg(n, reps):
             t[5].clear()
         b.run(ALL_INPUTS, True)
         mx.synchronize()
-        res[name + "_max_rel_err"] = max(rel_diff(t[5].to_qst(), r) for t, r in zip(inst, ref))
+        outs_p = [t[5].to_qst() for t in inst]
+        res[name + "_max_rel_err"] = max(rel_diff(o, r) for o, r in zip(outs_p, ref))
+        if prec == mx.FP32:
+            # per-instance energy functional <psi|H psi>/<psi|psi> from this launch vs the plain FP64 path
+            errs = []
+            for pr, o, r in zip(probs, outs_p, ref):
+                psi_q = pr[4]
+                nn = qdot(psi_q, psi_q)
+                e32, e64 = qdot(psi_q, o) / nn, qdot(psi_q, r) / nn
+                errs.append(abs(e32 - e64) / max(abs(e64), 1e-300))
+            res["fp32_energy_rel_err_max"] = max(errs)
+            res["fp32_energy_rel_err_median"] = float(np.median(errs))
         b.free()
     fl = res["flops_per_batch"]
     for k in list(res):
@@ -156,6 +174,36 @@ def qc_leg(norb, M, reps):
     return res
 
 
+def sweep_leg(L, schedule, precisions):
+    """Device-resident two-site sweeps of the open Hubbard chain (t=1, U=4, half filling, 2Sz=0) from the seeded
+    random initial MPS over the M schedule; per precision: total and per-sweep seconds, final energy, and for the
+    mixed-precision runs |E - E_fp64|."""
+    from mpsxx_amd.dmrg import DeviceDMRG, initial_mps
+    from mpsxx_amd.mpo import hubbard_mpo
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    init = initial_mps(L, (L, 0), schedule[0], 0)
+    out = {"L": L, "schedule": schedule, "tol": 1e-10}
+    e64 = None
+    for prec in precisions:
+        name = {mx.FP64: "fp64", mx.FP32: "fp32_mixed", mx.BF16X3: "bf16x3_mixed"}[prec]
+        print("sweep leg: %s" % name, file=sys.stderr, flush=True)
+        dm = DeviceDMRG(mpos, init, schedule[0])
+        secs, e, st = [], None, None
+        t0 = time.perf_counter()
+        for M in schedule:
+            e, st = dm.sweep(M, 1e-10, precision=prec)
+            secs.append(round(st["seconds"], 3))
+        total = time.perf_counter() - t0
+        r = {"seconds_total": total, "seconds_per_sweep": secs, "energy": e, "last_sweep_svd_s": st["svd_s"],
+             "last_sweep_davidson_s": st["davidson_s"], "last_sweep_sigma_ms": st["sigma_ms"]}
+        if prec == mx.FP64:
+            e64 = e
+        elif e64 is not None:
+            r["abs_dE_vs_fp64"] = abs(e - e64)
+        out[name] = r
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +218,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched leg (0: skip it)")
     ap.add_argument("--qc-norb", type=int, default=24, help="orbitals of the quantum-chemistry leg (0: skip it)")
     ap.add_argument("--qc-M", type=int, default=1024)
+    ap.add_argument("--sweeps", type=int, default=1, help="run the sweep leg (L=32, M 128x3, 256x3, 512x2; 0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -204,6 +253,10 @@ def main():
     if args.qc_norb > 0:
         print("quantum-chemistry leg: norb=%d M=%d" % (args.qc_norb, args.qc_M), file=sys.stderr, flush=True)
         qcres = qc_leg(args.qc_norb, args.qc_M, 5)
+
+    sweeps = None
+    if args.sweeps > 0:
+        sweeps = sweep_leg(32, [128] * 3 + [256] * 3 + [512] * 2, [mx.FP64, mx.FP32])
 
     cpu = None
     parity = None
@@ -251,6 +304,7 @@ def main():
         "other_precision": other_leg,
         "batched": batched,
         "qc_merged": qcres,
+        "sweeps": sweeps,
     }
     print(json.dumps(res))
 

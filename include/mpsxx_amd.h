@@ -202,6 +202,21 @@ int mpsxx_expectation(int nsites, const mpsxx_tensor* mpo, const mpsxx_tensor* m
 int mpsxx_dmrg_free(mpsxx_dmrg d);
 const char* mpsxx_dmrg_last_error(void);
 
+/* ---- planner introspection (host only, no device work) ---------------------------------------------------------- */
+/* Host threads of the contraction planner (threads <= 0: MPSXX_HOST_THREADS or the hardware concurrency, at most 16)
+ * and the candidate pairs per thread below which a contraction stays on fewer threads.  Plans are identical for any
+ * setting.  No reference counterpart (the reference enumerates pairs serially, STBLAS.h:316-356). */
+int mpsxx_planner_threads(int threads, int64_t min_pairs_per_thread);
+/* Block structure of btas::Contract(a[ca], b[cb]) (QSTCONTRACT.h:16-57 + STBLAS.h:298-359): the output blocks (indices
+ * over a's free then b's free indices, ascending tag; a block exists iff it is quantum-number allowed and receives at
+ * least one pair) and, per output block x, its (a block, b block) pairs pair_begin[x]..pair_begin[x+1] in accumulation
+ * (contracted-tag) order.  Block numbers refer to each input's blocks in ascending tag order.  Call with block_idx =
+ * NULL for the counts, then with arrays of nblocks*rank(c), nblocks+1 and npairs entries.  Blocks of zero size are
+ * not supported in the pair numbering. */
+int mpsxx_contract_structure(const mpsxx_qst_desc* a, int conj_a, const int32_t* ca, const mpsxx_qst_desc* b,
+                             int conj_b, const int32_t* cb, int32_t k, int64_t* nblocks, int64_t* npairs,
+                             int32_t* block_idx, int64_t* pair_begin, int64_t* pair_a, int64_t* pair_b);
+
 /* ---- plan introspection (measurement) ---------------------------------------------------------------------------- */
 /* Algorithmic cost of the two-site sigma vector for these inputs in the reference contraction order: flops = 2 m n k
  * summed over every accepted (A-block, B-block) pair of the four Contract steps, bytes = 8 * (|L|+|R|+|W|+|psi|+|sigma|)

@@ -1,4 +1,5 @@
 // extern "C" boundary: opaque tensor handles, status codes, thread-local error text.
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -450,6 +451,45 @@ int mpsxx_qc_mpo_create(int32_t norb, const double* oneint, const double* twoint
       throw;
     }
     *out = m;
+  });
+}
+
+int mpsxx_planner_threads(int threads, int64_t min_pairs_per_thread) {
+  return guard_host([&] { mpx::set_planner_threads(threads, min_pairs_per_thread); });
+}
+
+int mpsxx_contract_structure(const mpsxx_qst_desc* a, int conj_a, const int32_t* ca, const mpsxx_qst_desc* b,
+                             int conj_b, const int32_t* cb, int32_t k, int64_t* nblocks, int64_t* npairs,
+                             int32_t* block_idx, int64_t* pair_begin, int64_t* pair_a, int64_t* pair_b) {
+  return guard_host([&] {
+    if (!ca || !cb || k < 1 || !nblocks || !npairs) throw mpx::Error(MPSXX_ERR_ARG, "contract_structure: bad arguments");
+    const mpx::Layout A = layout_from_desc(a), B = layout_from_desc(b);
+    std::vector<int> ia(ca, ca + k), ib(cb, cb + k);
+    mpx::ContractSym sym = mpx::contract_symbolic(mpx::View{&A, conj_a != 0}, ia, mpx::View{&B, conj_b != 0}, ib, nullptr);
+    const mpx::Layout& C = sym.C;
+    *nblocks = C.nblocks();
+    *npairs = (int64_t)sym.pairs.size();
+    if (!block_idx) return;
+    for (int64_t x = 0; x < C.nblocks(); ++x)
+      for (int i = 0; i < C.rank; ++i) block_idx[x * C.rank + i] = C.index(x)[i];
+    // without an existing output, the planner's tasks are the output blocks in order
+    if ((int64_t)sym.tasks.size() != C.nblocks()) throw mpx::Error(MPSXX_ERR_ARG, "contract_structure: task mismatch");
+    // the block holding element offset `off` (offsets ascend with the block number; size-0 blocks share the offset
+    // of the next block and take part in no pair)
+    auto block_at = [](const mpx::Layout& L, int64_t off) {
+      int64_t x = std::lower_bound(L.off.begin(), L.off.end(), off) - L.off.begin();
+      while (x < L.nblocks() && L.off[x] == off && L.bsize[x] == 0) ++x;
+      return x < L.nblocks() && L.off[x] == off ? x : (int64_t)-1;
+    };
+    for (int64_t x = 0; x < C.nblocks(); ++x) {
+      const mpx::GemmTask& t = sym.tasks[x];
+      pair_begin[x] = t.pair_begin;
+      for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) {
+        pair_a[p] = block_at(A, sym.pairs[p].a_off);
+        pair_b[p] = block_at(B, sym.pairs[p].b_off);
+      }
+    }
+    pair_begin[C.nblocks()] = (int64_t)sym.pairs.size();
   });
 }
 

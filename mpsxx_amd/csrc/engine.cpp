@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <atomic>
 #include <climits>
 #include <exception>
 #include <numeric>
@@ -184,8 +185,24 @@ bool group_view(const std::vector<int>& g, const int64_t* dims, const int64_t* s
   return true;
 }
 
-// Host threads for the planner: MPSXX_HOST_THREADS if set, else the hardware concurrency, at most 16.
+}  // namespace
+
+// Host threads of the planner: set_planner_threads, else MPSXX_HOST_THREADS, else the hardware concurrency (at most
+// 16); a contraction uses at most one thread per min_pairs candidate pairs.
+namespace {
+std::atomic<int> g_threads{0};
+std::atomic<int64_t> g_grain{200000};
+}  // namespace
+
+void set_planner_threads(int threads, int64_t min_pairs) {
+  g_threads = std::max(0, threads);
+  g_grain = std::max<int64_t>(1, min_pairs);
+}
+
+namespace {
+
 int host_threads() {
+  if (g_threads > 0) return g_threads;
   static const int n = [] {
     if (const char* e = std::getenv("MPSXX_HOST_THREADS")) return std::max(1, std::atoi(e));
     const unsigned h = std::thread::hardware_concurrency();
@@ -398,7 +415,7 @@ ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const 
     work += gwork[g];
   }
   const int nthreads = (int)std::max<int64_t>(
-      1, std::min<int64_t>({(int64_t)host_threads(), work / 200000, (int64_t)ngroups}));
+      1, std::min<int64_t>({(int64_t)host_threads(), work / g_grain.load(), (int64_t)ngroups}));
   std::vector<Chunk> chunks(nthreads);
   // nthreads ranges of consecutive groups with about equal candidate-pair counts
   std::vector<size_t> cut(nthreads + 1, ngroups);
