@@ -558,9 +558,31 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(const GemmTile* __restr
   }
 }
 
+// advance a (pair, 16-wide panel) position by `steps` panels across the pair descriptors staged in LDS
+__device__ __forceinline__ bool advance_staged(const GemmPair* sp, int n, int& p, int& panel, GemmPair& pr, int steps) {
+  panel += steps;
+  while (true) {
+    const int np = (pr.k + 15) / 16;
+    if (panel < np) return true;
+    panel -= np;
+    if (++p >= n) return false;
+    pr = sp[p];
+  }
+}
+
+__device__ __forceinline__ int wave_isum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+constexpr int kSplitPairs = 128;  // pair descriptors of one task staged in LDS per round
+
 // K-split variant for 32x32 tiles with a long reduction (many pairs and/or large k): the four waves of a work-group
 // take every fourth (pair, 16-wide k panel) unit of the same tile (each wave loading its next unit while the MFMAs of
 // the current one run), then the partial tiles are summed in LDS in a fixed order (deterministic) and stored once.
+// The task's pair descriptors are first staged in LDS by the whole work-group (rounds of kSplitPairs), so the unit
+// loop waits only on operand loads, not on a descriptor load before each operand load.
 template <typename T, typename TC, bool Tab>
 __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restrict__ tiles, int ntiles,
                                                          const GemmTask* __restrict__ tasks,
@@ -568,6 +590,7 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
                                                          double alpha, int accumulate) {
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T red[3][4][4][64];  // partial tiles of waves 1..3: [wave-1][mi*2+ni][r][lane]
+  __shared__ GemmPair sp[kSplitPairs];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
@@ -592,24 +615,35 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
     tlanes.vc1 = tlanes.c1 < tk.n;
     tlanes.lk = lk;
     const int pend = tk.pair_begin + tk.pair_count;
-    // this wave's first unit: the wave-th (pair, panel) of the task
-    int p = tk.pair_begin, panel = 0;
-    GemmPair pr = pairs[p];
-    bool have = advance_units(pairs, pend, p, panel, pr, wave);
-    Frag<T> cur;
-    if (have) load_frag(cur, A, B, pr, 16 * panel, tlanes);
-    while (have) {
-      int np = p, npanel = panel;
-      GemmPair npr = pr;
-      const bool more = advance_units(pairs, pend, np, npanel, npr, 4);
-      Frag<T> nxt;
-      if (more) load_frag(nxt, A, B, npr, 16 * npanel, tlanes);
-      mma_frag<T>(acc, cur, pr.k - 16 * panel);
-      if (!more) break;
-      cur = nxt;
-      p = np;
-      panel = npanel;
-      pr = npr;
+    int ubase = 0;  // units of the task before this round, modulo 4
+    for (int c0 = tk.pair_begin; c0 < pend; c0 += kSplitPairs) {  // uniform across the work-group
+      const int nc = min(kSplitPairs, pend - c0);
+      __syncthreads();  // the previous round's descriptors are no longer read
+      for (int i = threadIdx.x; i < nc; i += 256) sp[i] = pairs[c0 + i];
+      __syncthreads();
+      int u = 0;
+      for (int i = lane; i < nc; i += 64) u += (sp[i].k + 15) / 16;
+      const int nunits = wave_isum(u);
+      // the wave takes the units whose index in the whole task is congruent to its number modulo 4
+      int p = 0, panel = 0;
+      GemmPair pr = sp[0];
+      bool have = advance_staged(sp, nc, p, panel, pr, (wave - ubase + 4) & 3);
+      Frag<T> cur;
+      if (have) load_frag(cur, A, B, pr, 16 * panel, tlanes);
+      while (have) {
+        int np = p, npanel = panel;
+        GemmPair npr = pr;
+        const bool more = advance_staged(sp, nc, np, npanel, npr, 4);
+        Frag<T> nxt;
+        if (more) load_frag(nxt, A, B, npr, 16 * npanel, tlanes);
+        mma_frag<T>(acc, cur, pr.k - 16 * panel);
+        if (!more) break;
+        cur = nxt;
+        p = np;
+        panel = npanel;
+        pr = npr;
+      }
+      ubase = (ubase + nunits) & 3;
     }
     if (wave > 0) {
 #pragma unroll
@@ -635,11 +669,18 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
   }
 }
 
+// Scaled accumulation C[r] += sum over pairs of coef * V[r * stride] for contractions with scalar (MPO) blocks.  The
+// pair descriptors and their coefficients (S, or the fused S * S2 of two MPO steps) are staged in LDS by the whole
+// work-group, 256 pairs per round, so the element loop issues only the independent V loads; pairs are summed in list
+// order.
 template <typename T, typename TC, bool Tab>
 __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restrict__ chunks, int nchunks,
                                                            const AxpyTask* __restrict__ tasks,
                                                            const AxpyPair* __restrict__ pairs, Operands ops,
                                                            double alpha, int accumulate) {
+  __shared__ int64_t s_voff[256];
+  __shared__ int32_t s_vstr[256];
+  __shared__ T s_coef[256];
   for (int w = blockIdx.x; w < nchunks; w += gridDim.x) {
     const Chunk ch = chunks[w];
     const AxpyTask tk = tasks[ch.task];
@@ -653,14 +694,25 @@ __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restri
     T acc[E];
 #pragma unroll
     for (int j = 0; j < E; ++j) acc[j] = T(0);
-    for (int p = tk.pair_begin; p < pend; ++p) {
-      const AxpyPair pr = pairs[p];
-      const T coef = pr.s2_off >= 0 ? S[pr.s_off] * S2[pr.s2_off] : S[pr.s_off];
-      const T* vp = V + pr.v_off;
+    for (int c0 = tk.pair_begin; c0 < pend; c0 += 256) {  // uniform across the work-group
+      const int nc = min(256, pend - c0);
+      __syncthreads();  // the previous round's entries are no longer read
+      if ((int)threadIdx.x < nc) {
+        const AxpyPair pr = pairs[c0 + threadIdx.x];
+        s_voff[threadIdx.x] = pr.v_off;
+        s_vstr[threadIdx.x] = pr.v_stride;
+        s_coef[threadIdx.x] = pr.s2_off >= 0 ? S[pr.s_off] * S2[pr.s2_off] : S[pr.s_off];
+      }
+      __syncthreads();
+      for (int q = 0; q < nc; ++q) {
+        const T coef = s_coef[q];
+        const T* vp = V + s_voff[q];
+        const int64_t st = s_vstr[q];
 #pragma unroll
-      for (int j = 0; j < E; ++j) {
-        const int r = ch.start + threadIdx.x + 256 * j;
-        if (r < end) acc[j] += coef * vp[(int64_t)r * pr.v_stride];
+        for (int j = 0; j < E; ++j) {
+          const int r = ch.start + threadIdx.x + 256 * j;
+          if (r < end) acc[j] += coef * vp[(int64_t)r * st];
+        }
       }
     }
 #pragma unroll
