@@ -199,8 +199,6 @@ void set_planner_threads(int threads, int64_t min_pairs) {
   g_grain = std::max<int64_t>(1, min_pairs);
 }
 
-namespace {
-
 int host_threads() {
   if (g_threads > 0) return g_threads;
   static const int n = [] {
@@ -237,8 +235,6 @@ void parallel_for(int n, const std::function<void(int)>& f) {
   for (auto& e : err)
     if (e) std::rethrow_exception(e);
 }
-
-}  // namespace
 
 void gemm_indices(int transa, int transb, int NA, int NB, int NC, std::vector<int>& ca, std::vector<int>& cb) {
   const int K = (NA + NB - NC) / 2;

@@ -727,6 +727,58 @@ __global__ __launch_bounds__(256) void axpy_grouped_kernel(const Chunk* __restri
   }
 }
 
+// Merged block operator from a scalar-block MPO, one work-group per (sub-block, 1024-element chunk): the column's MPO
+// entries and the matching environment block offsets are staged in LDS (256 per round), then every thread sums its
+// elements over the entries in order and stores them once at their place in the merged block.
+__global__ __launch_bounds__(256) void merged_operator_kernel(const Chunk* __restrict__ chunks, int nchunks,
+                                                              const MergedOpTask* __restrict__ tasks,
+                                                              const int32_t* __restrict__ slice_blk, int nw,
+                                                              const int64_t* __restrict__ env_off,
+                                                              const int32_t* __restrict__ col_begin,
+                                                              const int32_t* __restrict__ ent_w,
+                                                              const int64_t* __restrict__ ent_off,
+                                                              const double* __restrict__ env,
+                                                              const double* __restrict__ mpo, double* __restrict__ out) {
+  __shared__ int64_t s_off[256];
+  __shared__ double s_coef[256];
+  constexpr int E = kAxpyChunk / 256;
+  for (int w = blockIdx.x; w < nchunks; w += gridDim.x) {
+    const Chunk ch = chunks[w];
+    const MergedOpTask tk = tasks[ch.task];
+    const int end = min(tk.rows * tk.cols, ch.start + kAxpyChunk);
+    double acc[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) acc[j] = 0.0;
+    const int cb = col_begin[tk.col], ce = col_begin[tk.col + 1];
+    for (int c0 = cb; c0 < ce; c0 += 256) {  // uniform across the work-group
+      const int nc = min(256, ce - c0);
+      __syncthreads();  // the previous round's entries are no longer read
+      if ((int)threadIdx.x < nc) {
+        const int blk = slice_blk[(int64_t)tk.slice * nw + ent_w[c0 + threadIdx.x]];
+        s_off[threadIdx.x] = blk >= 0 ? env_off[blk] : -1;
+        s_coef[threadIdx.x] = mpo[ent_off[c0 + threadIdx.x]];
+      }
+      __syncthreads();
+      for (int q = 0; q < nc; ++q) {
+        const int64_t lo = s_off[q];
+        if (lo < 0) continue;  // uniform across the work-group
+        const double coef = s_coef[q];
+        const double* vp = env + lo;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+          const int r = ch.start + threadIdx.x + 256 * j;
+          if (r < end) acc[j] += coef * vp[r];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int r = ch.start + threadIdx.x + 256 * j;
+      if (r < end) out[tk.y_off + (int64_t)(r / tk.cols) * tk.ld + r % tk.cols] = acc[j];
+    }
+  }
+}
+
 template <typename TX, typename TY>
 __global__ __launch_bounds__(256) void copy_kernel(const Chunk* __restrict__ chunks,
                                                    const CopySeg* __restrict__ segs, const TX* __restrict__ x,
@@ -924,6 +976,14 @@ void launch_copy(const Chunk* chunks, int nchunks, const CopySeg* segs, const TX
                  double beta, hipStream_t s) {
   if (nchunks <= 0) return;
   hipLaunchKernelGGL((copy_kernel<TX, TY>), dim3(nchunks), dim3(256), 0, s, chunks, segs, x, y, alpha, beta);
+}
+
+void launch_merged_operator(const Chunk* chunks, int nchunks, const MergedOpTask* tasks, const int32_t* slice_blk,
+                            int nw, const int64_t* env_off, const int32_t* col_begin, const int32_t* ent_w,
+                            const int64_t* ent_off, const double* env, const double* mpo, double* out, hipStream_t s) {
+  if (nchunks <= 0) return;
+  hipLaunchKernelGGL(merged_operator_kernel, dim3(groups(nchunks, 1 << 20)), dim3(256), 0, s, chunks, nchunks, tasks,
+                     slice_blk, nw, env_off, col_begin, ent_w, ent_off, env, mpo, out);
 }
 
 void launch_convert_segments(const Chunk* chunks, int nchunks, const ConvSeg* segs, hipStream_t s, int max_groups) {

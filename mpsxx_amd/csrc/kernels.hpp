@@ -122,6 +122,20 @@ void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const 
                  double alpha, int accumulate, hipStream_t s, int max_groups = 0);
 
 // FP64 -> FP32 copies of many buffers in one launch (mixed-precision inputs of a batched launch); the chunks are
+// One environment-sized sub-block of a merged block operator: its placement in the merged output (row length ld), the
+// environment slice (bra, ket sector pair) it reads and the MPO column (site pair and outgoing operator) it sums over.
+struct MergedOpTask {
+  int64_t y_off;
+  int32_t rows, cols, ld;
+  int32_t slice, col;
+  int32_t pad;
+};
+// out[sub-block] = sum over the column's entries (w, v) in list order of v * env[block of (slice, w)] (entries whose
+// slice block is absent are skipped); slice_blk[slice * nw + w] = environment block or -1, env_off[block] = offset.
+void launch_merged_operator(const Chunk* chunks, int nchunks, const MergedOpTask* tasks, const int32_t* slice_blk,
+                            int nw, const int64_t* env_off, const int32_t* col_begin, const int32_t* ent_w,
+                            const int64_t* ent_off, const double* env, const double* mpo, double* out, hipStream_t s);
+
 // (segment, start) pairs of kAxpyChunk elements.
 struct ConvSeg {
   const double* src;

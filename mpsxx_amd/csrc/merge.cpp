@@ -5,7 +5,9 @@
 // appearance, and the packed sectors of one merged sector are stacked in packed order.  QSTmerge
 // (QSTmerge.h:31-140) creates a merged block when at least one of its sub-blocks is stored and zero-fills the rest.
 #include <algorithm>
+#include <cstdlib>
 #include <map>
+#include <unordered_map>
 
 #include "mpsxx_amd.h"
 #include "ops.hpp"
@@ -133,6 +135,205 @@ Layout merge_layout(const Layout& X, int nr, int nc, std::vector<CopySeg>& segs)
   return Y;
 }
 
+namespace {
+
+bool scalar_blocks(const Layout& W) {
+  for (int64_t b = 0; b < W.nblocks(); ++b)
+    if (W.bsize[b] != 1) return false;
+  return W.nblocks() > 0;
+}
+
+// Merged operator from a scalar-block MPO without materialising the contraction's pair list.  The environment's blocks
+// are grouped into slices (bra sector, ket sector) over the contracted operator index w, the MPO entries into columns
+// (the three other MPO indices) listing (w, entry) in ascending w.  A sub-block (slice, column) of the contraction
+// exists iff the two w sets intersect (Contract's output rule, STBLAS.h:331-350); all w of a slice (of a column) carry
+// one quantum number, so only slices and columns of the same class are intersected, as bit sets.  The kernel sums
+// each sub-block over the column's entries in ascending w, the reference's accumulation order, and writes it straight
+// to its place in the merged block (QSTmerge placement), so the contraction intermediate is never stored.
+}  // namespace
+
+bool merged_operator_plan(bool fwd, const Layout& W, const Layout& E, MergedOpPlan& P) {
+  if (W.rank != 4 || E.rank != 3 || !scalar_blocks(W)) return false;
+  const int wc = fwd ? 0 : 3;  // contracted MPO index (E's is 1)
+  const int nw = E.nsec(1);
+  if (W.nsec(wc) != nw) throw Error(MPSXX_ERR_QN, "Contract: contracted sector counts differ");
+  for (int k = 0; k < nw; ++k) {
+    if (E.qs[1][k] != -W.qs[wc][k]) throw Error(MPSXX_ERR_QN, "contraction of quantum numbers failed");
+    if (E.ds[1][k] != W.ds[wc][k]) throw Error(MPSXX_ERR_SHAPE, "Contract: contracted dense shapes differ");
+    if (E.ds[1][k] != 1) return false;
+  }
+  // the three free MPO indices of a column: forward (nb, nk, w'), backward (w, nb, nk)
+  const int f0 = fwd ? 1 : 0, f1 = fwd ? 2 : 1, f2 = fwd ? 3 : 2;
+  const int n1 = E.nsec(0), n2 = E.nsec(2);
+  // slices
+  std::vector<int32_t> slice_of((size_t)n1 * n2, -1);
+  std::vector<int32_t> sl_e0, sl_e2;
+  std::vector<QN> sl_q;
+  for (int64_t b = 0; b < E.nblocks(); ++b) {
+    const int32_t* e = E.index(b);
+    int32_t& id = slice_of[(size_t)e[0] * n2 + e[2]];
+    if (id < 0) {
+      id = (int32_t)sl_e0.size();
+      sl_e0.push_back(e[0]);
+      sl_e2.push_back(e[2]);
+      sl_q.push_back(E.qs[1][e[1]]);
+    }
+  }
+  const int nsl = (int)sl_e0.size();
+  const int words = (nw + 63) / 64;
+  std::vector<int32_t> slice_blk((size_t)std::max(nsl, 1) * nw, -1);
+  std::vector<uint64_t> sl_bits((size_t)std::max(nsl, 1) * words, 0);
+  for (int64_t b = 0; b < E.nblocks(); ++b) {
+    const int32_t* e = E.index(b);
+    const int32_t id = slice_of[(size_t)e[0] * n2 + e[2]];
+    slice_blk[(size_t)id * nw + e[1]] = (int32_t)b;
+    sl_bits[(size_t)id * words + e[1] / 64] |= uint64_t(1) << (e[1] % 64);
+  }
+  // columns, entries in MPO tag order (ascending w within a column: w is the slowest index forward, the fastest
+  // backward, and the column key leaves it out)
+  const int64_t nc1 = W.nsec(f1), nc2 = W.nsec(f2);
+  std::unordered_map<int64_t, int32_t> col_of;
+  std::vector<int64_t> col_key;
+  std::vector<int32_t> cnt;
+  std::vector<QN> col_q;
+  for (int64_t b = 0; b < W.nblocks(); ++b) {
+    const int32_t* x = W.index(b);
+    const int64_t key = ((int64_t)x[f0] * nc1 + x[f1]) * nc2 + x[f2];
+    auto it = col_of.find(key);
+    if (it == col_of.end()) {
+      it = col_of.emplace(key, (int32_t)col_key.size()).first;
+      col_key.push_back(key);
+      cnt.push_back(0);
+      col_q.push_back(-W.qs[wc][x[wc]]);
+    }
+    ++cnt[it->second];
+  }
+  const int ncol = (int)col_key.size();
+  std::vector<int32_t> col_begin(ncol + 1, 0);
+  for (int c = 0; c < ncol; ++c) col_begin[c + 1] = col_begin[c] + cnt[c];
+  std::vector<int32_t> ent_w(W.nblocks());
+  std::vector<int64_t> ent_off(W.nblocks());
+  std::vector<uint64_t> col_bits((size_t)std::max(ncol, 1) * words, 0);
+  {
+    std::vector<int32_t> fill(col_begin.begin(), col_begin.end() - 1);
+    for (int64_t b = 0; b < W.nblocks(); ++b) {
+      const int32_t* x = W.index(b);
+      const int32_t c = col_of[((int64_t)x[f0] * nc1 + x[f1]) * nc2 + x[f2]];
+      ent_w[fill[c]] = x[wc];
+      ent_off[fill[c]] = W.off[b];
+      ++fill[c];
+      col_bits[(size_t)c * words + x[wc] / 64] |= uint64_t(1) << (x[wc] % 64);
+    }
+  }
+  // classes by the contracted quantum number
+  std::map<std::pair<int, int>, std::pair<std::vector<int32_t>, std::vector<int32_t>>> cls;
+  for (int i = 0; i < nsl; ++i) cls[{sl_q[i].p, sl_q[i].s}].first.push_back(i);
+  for (int c = 0; c < ncol; ++c) cls[{col_q[c].p, col_q[c].s}].second.push_back(c);
+  std::vector<std::pair<const std::vector<int32_t>*, int32_t>> work;  // (columns of the class, slice)
+  for (auto& kv : cls)
+    for (int32_t i : kv.second.first) work.push_back({&kv.second.second, i});
+  // merged row / column sectors
+  MergeInfo MR, MC;
+  if (fwd) {
+    MR = merge_info({E.qs[0], W.qs[1]}, {E.ds[0], W.ds[1]});
+    MC = merge_info({E.qs[2], W.qs[2]}, {E.ds[2], W.ds[2]});
+  } else {
+    MR = merge_info({W.qs[1], E.qs[0]}, {W.ds[1], E.ds[0]});
+    MC = merge_info({W.qs[2], E.qs[2]}, {W.ds[2], E.ds[2]});
+  }
+  // (slice, column) pairs with intersecting w sets, on several host threads in contiguous ranges of the work list
+  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)host_threads(), (int64_t)work.size() / 64));
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> part(nth);
+  parallel_for(nth, [&](int t) {
+    const size_t lo = work.size() * t / nth, hi = work.size() * (t + 1) / nth;
+    for (size_t k = lo; k < hi; ++k) {
+      const int32_t i = work[k].second;
+      const uint64_t* a = &sl_bits[(size_t)i * words];
+      for (int32_t c : *work[k].first) {
+        const uint64_t* bb = &col_bits[(size_t)c * words];
+        for (int x = 0; x < words; ++x)
+          if (a[x] & bb[x]) {
+            part[t].push_back({i, c});
+            break;
+          }
+      }
+    }
+  });
+  Layout Y;
+  Y.set_shape(3, true, E.q + W.q, {MR.qm, W.qs[fwd ? 3 : 0], MC.qm}, {MR.dm, W.ds[fwd ? 3 : 0], MC.dm});
+  struct Place { int32_t slice, col, pr, pc, w; };
+  std::vector<Place> pl;
+  for (auto& v : part)
+    for (auto& sc : v) {
+      const int64_t key = col_key[sc.second];
+      const int32_t x0 = (int32_t)(key / (nc1 * nc2)), x1 = (int32_t)(key / nc2 % nc1), x2 = (int32_t)(key % nc2);
+      Place p;
+      p.slice = sc.first;
+      p.col = sc.second;
+      if (fwd) {  // column (nb, nk, w'): rows (lb, nb), cols (lk, nk)
+        p.pr = sl_e0[sc.first] * MR.radix[1] + x0;
+        p.pc = sl_e2[sc.first] * MC.radix[1] + x1;
+        p.w = x2;
+      } else {    // column (w, nb, nk): rows (nb, rb), cols (nk, rk)
+        p.pr = x1 * MR.radix[1] + sl_e0[sc.first];
+        p.pc = x2 * MC.radix[1] + sl_e2[sc.first];
+        p.w = x0;
+      }
+      int32_t yi[3] = {MR.p2m[p.pr], p.w, MC.p2m[p.pc]};
+      if (!Y.allowed(yi)) continue;
+      Y.add_block(yi);
+      pl.push_back(p);
+    }
+  Y.finalize();
+  std::vector<MergedOpTask>& tasks = P.tasks;
+  std::vector<Chunk>& chunks = P.chunks;
+  tasks.clear();
+  chunks.clear();
+  tasks.reserve(pl.size());
+  for (const Place& p : pl) {
+    int32_t yi[3] = {MR.p2m[p.pr], p.w, MC.p2m[p.pc]};
+    const int64_t yb = Y.find(Y.compute_tag(yi));
+    MergedOpTask t;
+    t.rows = E.ds[0][sl_e0[p.slice]];
+    t.cols = E.ds[2][sl_e2[p.slice]];
+    t.ld = MC.dm[yi[2]];
+    t.y_off = Y.off[yb] + (int64_t)MR.poff[p.pr] * t.ld + MC.poff[p.pc];
+    t.slice = p.slice;
+    t.col = p.col;
+    t.pad = 0;
+    for (int st = 0; st < t.rows * t.cols; st += kAxpyChunk) chunks.push_back({(int32_t)tasks.size(), st});
+    tasks.push_back(t);
+  }
+  P.Y = std::move(Y);
+  P.nw = nw;
+  P.slice_blk = std::move(slice_blk);
+  P.col_begin = std::move(col_begin);
+  P.ent_w = std::move(ent_w);
+  P.ent_off = std::move(ent_off);
+  P.env_off = E.off;
+  return true;
+}
+
+namespace {
+
+bool merged_operator_scalar(bool fwd, Tensor& xmpo, Tensor& xopr, Tensor& mopr, hipStream_t s) {
+  MergedOpPlan P;
+  if (!merged_operator_plan(fwd, xmpo.L, xopr.L, P)) return false;
+  mopr.set_layout(P.Y, s);  // zero: sub-blocks no pair reaches stay zero, as QSTmerge leaves them
+  if (!P.chunks.empty()) {
+    DevBuf dt = upload_vec(P.tasks), dc = upload_vec(P.chunks), dsb = upload_vec(P.slice_blk),
+           deo = upload_vec(P.env_off), dcb = upload_vec(P.col_begin), dew = upload_vec(P.ent_w),
+           deff = upload_vec(P.ent_off);
+    launch_merged_operator(dc.as<Chunk>(), (int)P.chunks.size(), dt.as<MergedOpTask>(), dsb.as<int32_t>(), P.nw,
+                           deo.as<int64_t>(), dcb.as<int32_t>(), dew.as<int32_t>(), deff.as<int64_t>(), xopr.ptr(),
+                           xmpo.ptr(), mopr.ptr(), s);
+    hip_check(hipStreamSynchronize(s), "merged operator");
+  }
+  return true;
+}
+
+}  // namespace
+
 // Merged block operator (reference src/optimize.hpp:273-368), one operator group:
 //   forward : L~(Lb, w', Lk) = merge_{(lb,nb)} merge_{(lk,nk)} sum_w L(lb,w,lk) W(w,nb,nk,w')
 //   backward: R~(Rb, w, Rk)  = merge_{(nb,rb)} merge_{(nk,rk)} sum_w' W(w,nb,nk,w') R(rb,w',rk)
@@ -140,6 +341,7 @@ Layout merge_layout(const Layout& X, int nr, int nc, std::vector<CopySeg>& segs)
 // become one placement copy.
 void op_merged_operator(bool fwd, Tensor& xmpo, Tensor& xopr, Tensor& mopr, hipStream_t s) {
   if (xmpo.empty || xopr.empty) throw Error(MPSXX_ERR_ARG, "compute_merged_operator: empty input");
+  if (merged_operator_scalar(fwd, xmpo, xopr, mopr, s)) return;
   const Layout& W = xmpo.L;
   const Layout& E = xopr.L;
   // symbolic contraction: forward Contract(L, {1}, W, {0}) -> (lb, lk, nb, nk, w');
@@ -153,7 +355,7 @@ void op_merged_operator(bool fwd, Tensor& xmpo, Tensor& xopr, Tensor& mopr, hipS
     MC = merge_info({E.qs[2], W.qs[2]}, {E.ds[2], W.ds[2]});
   } else {
     MR = merge_info({W.qs[1], E.qs[0]}, {W.ds[1], E.ds[0]});
-    MC = merge_info({W.qs[2], E.qs[2]}, {W.ds[1], E.ds[0]});
+    MC = merge_info({W.qs[2], E.qs[2]}, {W.ds[2], E.ds[2]});
   }
   Layout Y;
   Y.set_shape(3, true, X.q, {MR.qm, fwd ? W.qs[3] : W.qs[0], MC.qm}, {MR.dm, fwd ? W.ds[3] : W.ds[0], MC.dm});

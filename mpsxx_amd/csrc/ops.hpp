@@ -82,6 +82,17 @@ MergeInfo merge_info(const std::vector<std::vector<QN>>& qs, const std::vector<s
 Layout merge_layout(const Layout& X, int nr, int nc, std::vector<CopySeg>& segs);
 void op_qst_merge(Tensor& x, int nr, int nc, Tensor& y, hipStream_t s);
 void op_merged_operator(bool fwd, Tensor& xmpo, Tensor& xopr, Tensor& mopr, hipStream_t s);
+// Host plan of the merged operator when every MPO block is a scalar (false otherwise): output structure, one task per
+// contributing sub-block (slice x column), the environment slice table and the MPO columns (see merge.cpp).
+struct MergedOpPlan {
+  Layout Y;
+  std::vector<MergedOpTask> tasks;
+  std::vector<Chunk> chunks;
+  std::vector<int32_t> slice_blk, col_begin, ent_w;
+  std::vector<int64_t> ent_off, env_off;
+  int nw = 0;
+};
+bool merged_operator_plan(bool fwd, const Layout& W, const Layout& E, MergedOpPlan& P);
 void op_merged_overlap(bool fwd, Tensor& xmpo, Tensor& xsov, Tensor& msov, hipStream_t s);
 
 // Two-site canonicalization with truncation to M states (src/canonicalize.hpp:27-46); returns the discarded weight.
