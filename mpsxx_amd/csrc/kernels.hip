@@ -577,19 +577,20 @@ __device__ __forceinline__ int wave_isum(int v) {
 }
 
 constexpr int kSplitPairs = 128;  // pair descriptors of one task staged in LDS per round
+constexpr int kSplitWaves = 4;    // waves sharing the reduction of one 32x32 tile
 
 // K-split variant for 32x32 tiles with a long reduction (many pairs and/or large k): the four waves of a work-group
 // take every fourth (pair, 16-wide k panel) unit of the same tile (each wave loading its next unit while the MFMAs of
 // the current one run), then the partial tiles are summed in LDS in a fixed order (deterministic) and stored once.
 // The task's pair descriptors are first staged in LDS by the whole work-group (rounds of kSplitPairs), so the unit
 // loop waits only on operand loads, not on a descriptor load before each operand load.
-template <typename T, typename TC, bool Tab>
-__global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restrict__ tiles, int ntiles,
+template <typename T, typename TC, bool Tab, int NW>
+__global__ __launch_bounds__(64 * NW) void gemm_split_kernel(const GemmTile* __restrict__ tiles, int ntiles,
                                                          const GemmTask* __restrict__ tasks,
                                                          const GemmPair* __restrict__ pairs, Operands ops,
                                                          double alpha, int accumulate) {
   typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T red[3][4][4][64];  // partial tiles of waves 1..3: [wave-1][mi*2+ni][r][lane]
+  __shared__ T red[NW - 1][4][4][64];  // partial tiles of waves 1..NW-1: [wave-1][mi*2+ni][r][lane]
   __shared__ GemmPair sp[kSplitPairs];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -615,25 +616,25 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
     tlanes.vc1 = tlanes.c1 < tk.n;
     tlanes.lk = lk;
     const int pend = tk.pair_begin + tk.pair_count;
-    int ubase = 0;  // units of the task before this round, modulo 4
+    int ubase = 0;  // units of the task before this round, modulo NW
     for (int c0 = tk.pair_begin; c0 < pend; c0 += kSplitPairs) {  // uniform across the work-group
       const int nc = min(kSplitPairs, pend - c0);
       __syncthreads();  // the previous round's descriptors are no longer read
-      for (int i = threadIdx.x; i < nc; i += 256) sp[i] = pairs[c0 + i];
+      for (int i = threadIdx.x; i < nc; i += 64 * NW) sp[i] = pairs[c0 + i];
       __syncthreads();
       int u = 0;
       for (int i = lane; i < nc; i += 64) u += (sp[i].k + 15) / 16;
       const int nunits = wave_isum(u);
-      // the wave takes the units whose index in the whole task is congruent to its number modulo 4
+      // the wave takes the units whose index in the whole task is congruent to its number modulo NW
       int p = 0, panel = 0;
       GemmPair pr = sp[0];
-      bool have = advance_staged(sp, nc, p, panel, pr, (wave - ubase + 4) & 3);
+      bool have = advance_staged(sp, nc, p, panel, pr, (wave - ubase + NW) % NW);
       Frag<T> cur;
       if (have) load_frag(cur, A, B, pr, 16 * panel, tlanes);
       while (have) {
         int np = p, npanel = panel;
         GemmPair npr = pr;
-        const bool more = advance_staged(sp, nc, np, npanel, npr, 4);
+        const bool more = advance_staged(sp, nc, np, npanel, npr, NW);
         Frag<T> nxt;
         if (more) load_frag(nxt, A, B, npr, 16 * npanel, tlanes);
         mma_frag<T>(acc, cur, pr.k - 16 * panel);
@@ -643,7 +644,7 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
         panel = npanel;
         pr = npr;
       }
-      ubase = (ubase + nunits) & 3;
+      ubase = (ubase + nunits) % NW;
     }
     if (wave > 0) {
 #pragma unroll
@@ -660,9 +661,12 @@ __global__ __launch_bounds__(256) void gemm_split_kernel(const GemmTile* __restr
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
-            acc[mi][ni][r] = acc[mi][ni][r] + red[0][mi * 2 + ni][r][lane] + red[1][mi * 2 + ni][r][lane] +
-                             red[2][mi * 2 + ni][r][lane];
+          for (int r = 0; r < 4; ++r) {
+            T v = acc[mi][ni][r];
+#pragma unroll
+            for (int q = 0; q < NW - 1; ++q) v = v + red[q][mi * 2 + ni][r][lane];
+            acc[mi][ni][r] = v;
+          }
       store_tile<T, TC>(opnd<TC, Tab>(ops.c, tk.inst), tk, tl, acc, lr, lk, alpha, accumulate);
     }
     __syncthreads();  // red is rewritten by the next tile
@@ -1068,10 +1072,12 @@ void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks,
                        double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
   if (ops.table)
-    hipLaunchKernelGGL((gemm_split_kernel<T, TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
+    hipLaunchKernelGGL((gemm_split_kernel<T, TC, true, kSplitWaves>), dim3(groups(ntiles, max_groups)),
+                       dim3(64 * kSplitWaves), 0, s, tiles,
                        ntiles, tasks, pairs, ops, alpha, accumulate);
   else
-    hipLaunchKernelGGL((gemm_split_kernel<T, TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
+    hipLaunchKernelGGL((gemm_split_kernel<T, TC, false, kSplitWaves>), dim3(groups(ntiles, max_groups)),
+                       dim3(64 * kSplitWaves), 0, s, tiles,
                        ntiles, tasks, pairs, ops, alpha, accumulate);
 }
 
