@@ -161,6 +161,18 @@ int mpsxx_davidson_twosite(int ngroups, const mpsxx_tensor* lmpo, const mpsxx_te
                            const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, mpsxx_tensor diag, mpsxx_tensor wfnc,
                            double tolerance, int precision, double* energy, int32_t* n_sigma, void* stream);
 
+/* The same with lower roots (excited states, src/diagonalize.hpp:36-89 Functor with more than one wavefunction):
+ * lower_sigma[k] = H|k> (sigma of lower root k with the cross environments <current|H|k>), lower_overlap[k] = S|k>
+ * (compute_sigma_vector with the cross overlaps), lower_energy[k] = E_k; the operator is
+ * v = Hc - sum_k [<H_k|c> S_k + <S_k|c> H_k - E_k <S_k|c> S_k] and the result is orthogonalized against every S_k and
+ * normalized (Functor::orthonormalize).  lmpo = rmpo = NULL selects merged groups (lopr, ropr the merged operators of
+ * compute_merged_operator, wfnc the merged 2-index wavefunction; optimize_twosite_merged). */
+int mpsxx_davidson_twosite_ex(int ngroups, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
+                              const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, mpsxx_tensor diag, mpsxx_tensor wfnc,
+                              int nlower, const mpsxx_tensor* lower_sigma, const mpsxx_tensor* lower_overlap,
+                              const double* lower_energy, double tolerance, int precision, double* energy,
+                              int32_t* n_sigma, void* stream);
+
 /* ---- two-site canonicalization (src/canonicalize.hpp:27-46 -> QSTLAPACK.h:122-288) -------------------------------- */
 /* forward (LeftArrow): lmps = U (total QN 0), rmps = s*Vt; backward (RightArrow): lmps = U*s, rmps = Vt (total 0).
  * Keeps every singular value >= the M-th largest (M <= 0: all >= 1e-16); returns the discarded weight. */

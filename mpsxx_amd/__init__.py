@@ -258,6 +258,27 @@ def compute_merged_operator(forward, mpo0, opr0, out=None):
     return out
 
 
+def davidson_twosite(groups, diag, wfnc, tol, precision=FP64, lower=()):
+    """Davidson on the two-site (or, with groups of (None, None, L~, R~), the merged) effective Hamiltonian
+    (src/diagonalize.hpp:310-449); wfnc is overwritten by the normalized eigenvector.  lower = [(H|k>, S|k>, E_k)]
+    projects out lower roots (excited states, diagonalize.hpp:36-89).  Returns (energy, number of sigma vectors)."""
+    n = len(groups)
+    merged = groups[0][0] is None
+    arr = lambda i: (VP * n)(*[g[i].h.value for g in groups])
+    lm = None if merged else arr(0)
+    rm = None if merged else arr(1)
+    nl = len(lower)
+    ls = (VP * max(nl, 1))(*[x[0].h.value for x in lower]) if nl else None
+    lo = (VP * max(nl, 1))(*[x[1].h.value for x in lower]) if nl else None
+    le = (ctypes.c_double * max(nl, 1))(*[float(x[2]) for x in lower]) if nl else None
+    e = ctypes.c_double()
+    ns = ctypes.c_int32()
+    _check(lib().mpsxx_davidson_twosite_ex(ctypes.c_int(n), lm, rm, arr(2), arr(3), _h(diag), _h(wfnc), ctypes.c_int(nl),
+                                           ls, lo, le, ctypes.c_double(tol), ctypes.c_int(precision), ctypes.byref(e),
+                                           ctypes.byref(ns), None))
+    return e.value, ns.value
+
+
 def qst_merge(x, nrow, ncol, out=None):
     """QSTmerge of the first nrow and the last ncol indices (include/btas/QSPARSE/QSTmerge.h)."""
     out = _out(out)

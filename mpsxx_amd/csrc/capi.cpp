@@ -379,6 +379,28 @@ int mpsxx_davidson_twosite(int ngroups, const mpsxx_tensor* lmpo, const mpsxx_te
   });
 }
 
+int mpsxx_davidson_twosite_ex(int ngroups, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
+                              const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, mpsxx_tensor diag, mpsxx_tensor wfnc,
+                              int nlower, const mpsxx_tensor* lower_sigma, const mpsxx_tensor* lower_overlap,
+                              const double* lower_energy, double tolerance, int precision, double* energy,
+                              int32_t* n_sigma, void* stream) {
+  return guard([&] {
+    if (ngroups <= 0 || !lopr || !ropr) throw mpx::Error(MPSXX_ERR_ARG, "Davidson: need at least one operator group");
+    if ((lmpo == nullptr) != (rmpo == nullptr)) throw mpx::Error(MPSXX_ERR_ARG, "Davidson: lmpo and rmpo both or neither");
+    if (nlower < 0 || (nlower > 0 && (!lower_sigma || !lower_overlap || !lower_energy)))
+      throw mpx::Error(MPSXX_ERR_ARG, "Davidson: lower roots need sigma, overlap and energy arrays");
+    std::vector<mpx::DavGroup> g;
+    for (int i = 0; i < ngroups; ++i)
+      g.push_back({lmpo ? &T(lmpo[i]) : nullptr, rmpo ? &T(rmpo[i]) : nullptr, &T(lopr[i]), &T(ropr[i])});
+    std::vector<mpx::DavLowerRoot> low;
+    for (int k = 0; k < nlower; ++k) low.push_back({&T(lower_sigma[k]), &T(lower_overlap[k]), lower_energy[k]});
+    mpx::DavidsonStats st;
+    const double e = mpx::davidson_twosite(g, T(diag), T(wfnc), tolerance, precision, st, S(stream), low);
+    if (energy) *energy = e;
+    if (n_sigma) *n_sigma = st.nsigma;
+  });
+}
+
 int mpsxx_sigma_twosite_stats(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
                               mpsxx_tensor wfn0, int precision, int64_t* flops, int64_t* bytes, int32_t* nsteps) {
   return guard([&] {

@@ -95,7 +95,7 @@ Layout davidson_layout(const std::vector<DavGroup>& groups, const Layout& psi) {
 }
 
 double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tensor& wfnc, double tol, int prec,
-                        DavidsonStats& st, hipStream_t s) {
+                        DavidsonStats& st, hipStream_t s, const std::vector<DavLowerRoot>& lower) {
   if (wfnc.empty) throw Error(MPSXX_ERR_ARG, "Davidson: empty initial wavefunction");
   if (groups.empty()) throw Error(MPSXX_ERR_ARG, "Davidson: no operator groups");
   const bool merged = groups[0].merged();
@@ -103,7 +103,20 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
     if (g.merged() != merged) throw Error(MPSXX_ERR_ARG, "Davidson: 4-index and merged groups mixed");
   if (wfnc.L.rank != (merged ? 2 : 4)) throw Error(MPSXX_ERR_ARG, "Davidson: wavefunction rank does not fit the groups");
   const int max_ritz = 10;
-  Layout U = davidson_layout(groups, wfnc.L);
+  // the vector space also holds every block of the lower roots' vectors: the projector's Axpy adds them to v
+  // (STBLAS.h:135-145 reserves missing blocks), so they belong to the closure
+  Layout U0 = wfnc.L;
+  for (const DavLowerRoot& r : lower) {
+    for (const Tensor* t : {r.sigma, r.overlap}) {
+      if (!t || t->empty) throw Error(MPSXX_ERR_ARG, "Davidson: empty lower-root vector");
+      if (t->L.rank != U0.rank || !t->L.same_qshape(U0) || !t->L.same_dshape(U0) || t->L.q != U0.q)
+        throw Error(MPSXX_ERR_QN, "Davidson: lower-root vectors must have the wavefunction's quantum shape");
+      for (int64_t b = 0; b < t->L.nblocks(); ++b)
+        if (U0.find(t->L.tag[b]) < 0) U0.add_block(t->L.index(b));
+    }
+  }
+  U0.finalize();
+  Layout U = davidson_layout(groups, U0);
   const int64_t n = U.nelem;
   if (n == 0) throw Error(MPSXX_ERR_ARG, "Davidson: empty vector space");
   // vector slots: T[0..9], S[0..9], T2[0..9], S2[0..9], errv
@@ -124,6 +137,25 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
                              hipMemcpyDeviceToDevice, s),
               "davidson init");
   }
+  // lower-root vectors aligned to U (H_k, S_k for every k)
+  const int nlow = (int)lower.size();
+  DevBuf lowbuf((size_t)std::max(2 * nlow, 1) * n * sizeof(double));
+  if (nlow > 0) hip_check(hipMemsetAsync(lowbuf.p, 0, (size_t)2 * nlow * n * sizeof(double), s), "memset");
+  auto Hk = [&](int k) { return lowbuf.as<double>() + (int64_t)(2 * k) * n; };
+  auto Sk = [&](int k) { return lowbuf.as<double>() + (int64_t)(2 * k + 1) * n; };
+  for (int k = 0; k < nlow; ++k) {
+    for (int which = 0; which < 2; ++which) {
+      const Tensor& t = which ? *lower[k].overlap : *lower[k].sigma;
+      double* dst = which ? Sk(k) : Hk(k);
+      for (int64_t b = 0; b < t.L.nblocks(); ++b) {
+        const int64_t j = U.find(t.L.tag[b]);
+        hip_check(hipMemcpyAsync(dst + U.off[j], t.ptr() + t.L.off[b], t.L.bsize[b] * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s),
+                  "lower root");
+      }
+    }
+  }
+
   // diagonal aligned to U, with the presence mask of diag's blocks (precondition, diagonalize.hpp:289-303)
   DevBuf dvals(n * sizeof(double)), dmask(n);
   {
@@ -177,6 +209,13 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
                     convert_mask);
     }
     convert_mask = merged ? 1u << 2 : 1u << 4;  // from now on only the trial vector changes
+    // lower roots: v -= <H_k|c> S_k + <S_k|c> H_k - E_k <S_k|c> S_k (diagonalize.hpp:72-79)
+    for (int k = 0; k < nlow; ++k) {
+      const double hk = dot(Hk(k), c), sk = dot(Sk(k), c);
+      axpy(-hk, Sk(k), v);
+      axpy(-sk, Hk(k), v);
+      axpy(lower[k].energy * sk, Sk(k), v);
+    }
     hip_check(hipEventRecord(e1, s), "event");
     hip_check(hipEventSynchronize(e1), "event sync");
     float ms = 0.f;
@@ -253,7 +292,8 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   } else {
     solve(MPSXX_FP64, tol);
   }
-  // wfnc <- normalized trial_0 on layout U (Functor::orthonormalize with no lower roots)
+  // wfnc <- trial_0 orthogonalized against the lower roots' S_k and normalized (Functor::orthonormalize)
+  for (int k = 0; k < nlow; ++k) axpy(-dot(Sk(k), T(0)), Sk(k), T(0));
   normalize(T(0));
   wfnc.set_layout(U, s);
   hip_check(hipMemcpyAsync(wfnc.ptr(), T(0), n * sizeof(double), hipMemcpyDeviceToDevice, s), "result");

@@ -120,8 +120,16 @@ struct DavidsonStats {
 // (products good to ~2^-16 relative).
 constexpr double kMixedTolerance = 1.0e-9;
 constexpr double kMixedToleranceX3 = 1.0e-6;
+// A lower root k for excited states (diagonalize.hpp:36-89, Functor with N_ > 1): sigma = H|k> with the cross
+// environments of the current root and root k, overlap = S|k> with the cross overlaps, energy = E_k.  The Davidson
+// operator becomes v = Hc - sum_k [<H_k|c> S_k + <S_k|c> H_k - E_k <S_k|c> S_k] and the result is orthogonalized
+// against every S_k and normalized (Functor::orthonormalize).
+struct DavLowerRoot {
+  Tensor *sigma, *overlap;
+  double energy;
+};
 Layout davidson_layout(const std::vector<DavGroup>& groups, const Layout& psi);
 double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tensor& wfnc, double tol, int prec,
-                        DavidsonStats& st, hipStream_t s);
+                        DavidsonStats& st, hipStream_t s, const std::vector<DavLowerRoot>& lower = {});
 
 }  // namespace mpx
