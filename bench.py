@@ -15,7 +15,8 @@ grouped launch per step (FP64 and FP32, hipGraph replay, autotuned tiles), and t
 a 24-orbital quantum-chemistry chain at M=1024 (synthetic integrals from seed 1, QC MPO with D_w=1226) in FP64, FP32 and
 split-BF16x3 with the operators resident; and the sweep half of the metric: device-resident two-site sweeps of the
 L=32 chain over M 128x3, 256x3, 512x2 in FP64 and in mixed precision (FP32 sigma vectors + FP64 refinement), with
-the energy difference between them.
+the energy difference between them; and merged-form sweeps of the 24-orbital chain over M 256, 512, 1024, 1024 in
+FP64 and mixed precision.
 """
 import argparse
 import ctypes
@@ -221,6 +222,37 @@ def sweep_leg(L, schedule, precisions):
     return out
 
 
+def qc_sweep_leg(norb, schedule, precisions):
+    """Merged two-site sweeps (src/make_sweep.cpp:112) of the QC chain with synthetic integrals from seed 1 over the
+    M schedule: per precision the total and per-sweep seconds, the final energy and |E - E_fp64|."""
+    from mpsxx_amd import qc
+    from mpsxx_amd.dmrg import SWEEP_MERGED, DeviceDMRG, initial_mps
+    h, v = qc.synthetic_integrals(norb, seed=1)
+    m = qc.QcMpo(h, v)
+    mpos = m.devices()
+    out = {"norb": norb, "schedule": schedule, "tol": 1e-10, "D_w_max": max(m.bond_dims())}
+    e64 = None
+    for prec in precisions:
+        name = {mx.FP64: "fp64", mx.FP32: "fp32_mixed", mx.BF16X3: "bf16x3_mixed"}[prec]
+        print("qc sweep leg: %s" % name, file=sys.stderr, flush=True)
+        dm = DeviceDMRG(mpos, initial_mps(norb, (norb, 0), schedule[0], 0), schedule[0], form=SWEEP_MERGED)
+        secs, e = [], None
+        t0 = time.perf_counter()
+        for M in schedule:
+            e, st = dm.sweep(M, 1e-10, precision=prec)
+            secs.append(round(st["seconds"], 3))
+        r = {"seconds_total": time.perf_counter() - t0, "seconds_per_sweep": secs, "energy": e,
+             "last_sweep_svd_s": st["svd_s"], "last_sweep_davidson_s": st["davidson_s"],
+             "last_sweep_renorm_s": st["renorm_s"]}
+        if prec == mx.FP64:
+            e64 = e
+        elif e64 is not None:
+            r["abs_dE_vs_fp64"] = abs(e - e64)
+        out[name] = r
+        del dm
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +268,8 @@ def main():
     ap.add_argument("--qc-norb", type=int, default=24, help="orbitals of the quantum-chemistry leg (0: skip it)")
     ap.add_argument("--qc-M", type=int, default=1024)
     ap.add_argument("--sweeps", type=int, default=1, help="run the sweep leg (L=32, M 128x3, 256x3, 512x2; 0: skip)")
+    ap.add_argument("--qc-sweeps", type=int, default=1,
+                    help="run the QC merged-sweep leg (norb --qc-norb, M 256, 512, 1024, 1024; 0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -280,6 +314,10 @@ def main():
     sweeps = None
     if args.sweeps > 0:
         sweeps = sweep_leg(32, [128] * 3 + [256] * 3 + [512] * 2, [mx.FP64, mx.FP32])
+
+    qc_sweeps = None
+    if args.qc_sweeps > 0 and args.qc_norb > 0:
+        qc_sweeps = qc_sweep_leg(args.qc_norb, [256, 512, 1024, 1024], [mx.FP64, mx.FP32])
 
     cpu = None
     parity = None
@@ -328,6 +366,7 @@ def main():
         "batched": batched,
         "qc_merged": qcres,
         "sweeps": sweeps,
+        "qc_sweeps": qc_sweeps,
     }
     print(json.dumps(res))
 

@@ -202,9 +202,17 @@ int mpsxx_dmrg_site(mpsxx_dmrg d, int site, mpsxx_tensor out, void* stream);
  * src/optimize.hpp:164-265: 4-index sigma, diag and renormalize) or MPSXX_SWEEP_MERGED (src/make_sweep.cpp:112 with
  * optimize_twosite_merged, src/optimize.hpp:495-630: merged operators built once per site, merged sigma and diag,
  * SVD of the merged wavefunction, environment = U~^+ L~ U~).  Same energies; the merged form is the reference's
- * quantum-chemistry path and avoids the per-site planning of the 4-index MPO-coefficient steps for large D_w. */
-enum { MPSXX_SWEEP_FOURINDEX = 0, MPSXX_SWEEP_MERGED = 1 };
+ * quantum-chemistry path and avoids the per-site planning of the 4-index MPO-coefficient steps for large D_w.
+ * MPSXX_SWEEP_ONESITE: one-site sweeps (dmrg/make_sweep.cpp with ONESITE, optimize_onesite, dmrg/optimize.hpp:20-125),
+ * the refinement dmrg/dmrg.cpp:76-114 runs after the two-site sweeps have converged. */
+enum { MPSXX_SWEEP_FOURINDEX = 0, MPSXX_SWEEP_MERGED = 1, MPSXX_SWEEP_ONESITE = 2 };
 int mpsxx_dmrg_set_form(mpsxx_dmrg d, int form);
+/* Excited states (dmrg/dmrg.cpp:36-72 with iroot > 0, dmrg/make_sweep.cpp, dmrg/optimize.hpp:127-230): register the
+ * converged state `root` (energy E_k) as a lower root of `d`.  Both must have the centre at site 0 (as after
+ * mpsxx_dmrg_create or a sweep).  Later sweeps of d project every lower root out of its Davidson solves and keep
+ * the cross environments <d|H|k>, <d|k> up to date; the sweep energy is then root d's.  Two-site 4-index and
+ * one-site forms. */
+int mpsxx_dmrg_add_lower_root(mpsxx_dmrg d, mpsxx_dmrg root, double energy, void* stream);
 /* Adjoint of the sigma-vector / renormalization contractions with respect to the MPO: grads[k] = <psi|dH/dW_k|psi>
  * for the current state (W_k's block indices, quantum numbers negated).  The centre moves across the chain without
  * truncation and back; sweeps can continue afterwards.  No reference counterpart. */

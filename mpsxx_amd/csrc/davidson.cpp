@@ -72,6 +72,7 @@ namespace {
 
 Plan& group_plan(const DavGroup& g, const Layout& U, int prec) {
   if (g.merged()) return plan_sigma_merged(g.lopr->L, g.ropr->L, U, &U, prec);
+  if (g.onesite()) return plan_sigma_onesite(g.lmpo->L, g.lopr->L, g.ropr->L, U, &U, prec);
   return plan_sigma_twosite(g.lmpo->L, g.rmpo->L, g.lopr->L, g.ropr->L, U, &U, prec);
 }
 
@@ -98,10 +99,10 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
                         DavidsonStats& st, hipStream_t s, const std::vector<DavLowerRoot>& lower) {
   if (wfnc.empty) throw Error(MPSXX_ERR_ARG, "Davidson: empty initial wavefunction");
   if (groups.empty()) throw Error(MPSXX_ERR_ARG, "Davidson: no operator groups");
-  const bool merged = groups[0].merged();
+  const int kind = groups[0].wfn_rank();  // 4 two-site, 3 one-site, 2 merged
   for (const DavGroup& g : groups)
-    if (g.merged() != merged) throw Error(MPSXX_ERR_ARG, "Davidson: 4-index and merged groups mixed");
-  if (wfnc.L.rank != (merged ? 2 : 4)) throw Error(MPSXX_ERR_ARG, "Davidson: wavefunction rank does not fit the groups");
+    if (g.wfn_rank() != kind) throw Error(MPSXX_ERR_ARG, "Davidson: operator groups of different forms mixed");
+  if (wfnc.L.rank != kind) throw Error(MPSXX_ERR_ARG, "Davidson: wavefunction rank does not fit the groups");
   const int max_ritz = 10;
   // the vector space also holds every block of the lower roots' vectors: the projector's Axpy adds them to v
   // (STBLAS.h:135-145 reserves missing blocks), so they belong to the closure
@@ -202,13 +203,15 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
     hip_check(hipMemsetAsync(v, 0, n * sizeof(double), s), "memset");
     for (size_t g = 0; g < groups.size(); ++g) {
       const DavGroup& gr = groups[g];
-      if (merged)
+      if (kind == 2)
         run_plan_ex(*plans[g], {gr.lopr->ptr(), gr.ropr->ptr(), c}, v, s, {}, convert_mask);
+      else if (kind == 3)
+        run_plan_ex(*plans[g], {gr.lmpo->ptr(), gr.lopr->ptr(), gr.ropr->ptr(), c}, v, s, {}, convert_mask);
       else
         run_plan_ex(*plans[g], {gr.lmpo->ptr(), gr.rmpo->ptr(), gr.lopr->ptr(), gr.ropr->ptr(), c}, v, s, {},
                     convert_mask);
     }
-    convert_mask = merged ? 1u << 2 : 1u << 4;  // from now on only the trial vector changes
+    convert_mask = 1u << (kind == 2 ? 2 : kind == 3 ? 3 : 4);  // from now on only the trial vector changes
     // lower roots: v -= <H_k|c> S_k + <S_k|c> H_k - E_k <S_k|c> S_k (diagonalize.hpp:72-79)
     for (int k = 0; k < nlow; ++k) {
       const double hk = dot(Hk(k), c), sk = dot(Sk(k), c);

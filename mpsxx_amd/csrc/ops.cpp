@@ -455,18 +455,32 @@ void op_sigma_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Te
   last_run().inputs = {lmpo.ptr(), rmpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()};
 }
 
+Plan& plan_sigma_onesite(const Layout& mpo, const Layout& lopr, const Layout& ropr, const Layout& wfn,
+                         const Layout* out, int prec) {
+  std::vector<uint64_t> key{OP_SIGMA1, (uint64_t)prec, mpo.hash, lopr.hash, ropr.hash, wfn.hash, out ? out->hash : 0ull};
+  CachedPlan* cp = cache().find(key);
+  if (cp) return cp->plan;
+  auto np = std::make_unique<CachedPlan>();
+  PlanBuilder B(prec);
+  const int sm = B.input(mpo, 0), sl = B.input(lopr, 1), sr = B.input(ropr, 2), sw = B.input(wfn, 3);
+  // compute_sigma_vector.hpp:17-21: Gemm(N,N,lopr,wfn0) -> Contract(mpo) -> Contract(ropr)
+  const int s1 = B.contract(sl, false, {2}, sw, false, {0});
+  const int s2 = B.contract(s1, false, {1, 2}, sm, false, {0, 2});
+  B.contract_out(s2, false, {3, 1}, sr, false, {1, 2}, out);
+  np->plan = B.finish();
+  return cache().insert(key, std::move(np))->plan;
+}
+
 void op_sigma_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s) {
   for (Tensor* t : {&mpo, &lopr, &ropr, &wfn}) require(*t, "compute_sigma_vector");
-  const Layout* ex = existing(sgv);
-  execute(make_key(OP_SIGMA1, prec, {&mpo, &lopr, &ropr, &wfn}, &sgv), [&] {
-    PlanBuilder B(prec);
-    const int sm = B.input(mpo.L, 0), sl = B.input(lopr.L, 1), sr = B.input(ropr.L, 2), sw = B.input(wfn.L, 3);
-    // compute_sigma_vector.hpp:17-21: Gemm(N,N,lopr,wfn0) -> Contract(mpo) -> Contract(ropr)
-    const int s1 = B.contract(sl, false, {2}, sw, false, {0});
-    const int s2 = B.contract(s1, false, {1, 2}, sm, false, {0, 2});
-    B.contract_out(s2, false, {3, 1}, sr, false, {1, 2}, ex);
-    return B.finish();
-  }, OUT_ACCUMULATE, {&mpo, &lopr, &ropr, &wfn}, sgv, s);
+  Plan& P = plan_sigma_onesite(mpo.L, lopr.L, ropr.L, wfn.L, existing(sgv), prec);
+  if (sgv.empty) sgv.set_layout(P.out, s);
+  else if (sgv.L.hash != P.out.hash) sgv.relayout(P.out, s);
+  run_plan_ex(P, {mpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s, {}, ~0u);
+  last_stats().flops = P.flops;
+  last_stats().bytes = P.bytes;
+  last_run().plan = &P;
+  last_run().inputs = {mpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()};
 }
 
 Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wfn, const Layout* out, int prec) {

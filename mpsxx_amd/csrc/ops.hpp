@@ -60,6 +60,9 @@ void op_renormalize_merged(bool fwd, Tensor& opr0, Tensor& bra, Tensor& ket, Ten
 // Plan for a two-site sigma vector (exposed for the Davidson driver, which binds device pointers itself).
 Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
                          const Layout& wfn, const Layout* out, int prec);
+// Cached plan of the one-site sigma vector (compute_sigma_vector.hpp:10-22).
+Plan& plan_sigma_onesite(const Layout& mpo, const Layout& lopr, const Layout& ropr, const Layout& wfn,
+                         const Layout* out, int prec);
 // Cached plan of the merged 2-index sigma vector (compute_sigma_vector.hpp:86-96).
 Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wfn, const Layout* out, int prec);
 // The same plan, uncached and owned by the caller (batched launches keep one per instance: its intermediates are
@@ -97,18 +100,25 @@ void op_merged_overlap(bool fwd, Tensor& xmpo, Tensor& xsov, Tensor& msov, hipSt
 
 // Two-site canonicalization with truncation to M states (src/canonicalize.hpp:27-46); returns the discarded weight.
 double op_canonicalize_twosite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor& rmps, int M, hipStream_t s);
+// One-site form (src/canonicalize.hpp:49-85, noise 0): forward (LeftArrow) lmps = U(l,n,s), rmps = s*Vt(s,r);
+// backward (RightArrow) lmps = U*s(l,s), rmps = Vt(s,n,r).  The caller multiplies the 2-index factor into the
+// neighbouring site tensor, as the reference does.
+double op_canonicalize_onesite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor& rmps, int M, hipStream_t s);
 // The same from the merged wavefunction wm((l,n1), (n2,r)) = QSTmerge of a 4-index psi whose index structure is
 // `shape4` (q, qshape, dshape; blocks unused): the SVD blocks are wm's blocks, the factors come out as the 3-index
 // site tensors U(l,n1,s) and V(s,n2,r), exactly as op_canonicalize_twosite gives them for the expanded psi.
 double op_canonicalize_merged(bool forward, Tensor& wm, const Layout& shape4, Tensor& lmps, Tensor& rmps, int M,
                               hipStream_t s);
 
-// Davidson (src/diagonalize.hpp:310-449) over operator groups of the two-site form: 4-index groups (lmpo, rmpo, lopr,
-// ropr) on a 4-index wavefunction, or merged groups (lmpo = rmpo = null; lopr, ropr the merged operators of
-// optimize.hpp:273-368) on the merged 2-index wavefunction, as optimize_twosite_merged (optimize.hpp:495-590) does.
+// Davidson (src/diagonalize.hpp:310-449) over operator groups: two-site groups (lmpo, rmpo, lopr, ropr) on a
+// 4-index wavefunction; one-site groups (lmpo = the site MPO, rmpo = null) on a 3-index wavefunction
+// (optimize_onesite); merged groups (lmpo = rmpo = null; lopr, ropr the merged operators of optimize.hpp:273-368) on
+// the merged 2-index wavefunction (optimize_twosite_merged, optimize.hpp:495-590).
 struct DavGroup {
   Tensor *lmpo, *rmpo, *lopr, *ropr;
   bool merged() const { return lmpo == nullptr; }
+  bool onesite() const { return lmpo != nullptr && rmpo == nullptr; }
+  int wfn_rank() const { return merged() ? 2 : onesite() ? 3 : 4; }
 };
 struct DavidsonStats {
   int nsigma = 0;

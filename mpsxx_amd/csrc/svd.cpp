@@ -55,7 +55,7 @@ struct Blk { int i, j, m, n; };  // merged (row sector, col sector) and its dims
 // A dense source sub-rectangle (R x C, row length ld) placed at (ro, co) of the padded slot of block k.
 struct Rect { int k; int64_t src, ld, R, C, ro, co; };
 
-double canonicalize_core(bool forward, const Layout& X, const MergeInfo& MR, const MergeInfo& MC,
+double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo& MR, const MergeInfo& MC,
                          const std::vector<Blk>& blks, const std::vector<Rect>& rects, const double* src,
                          int64_t src_elems, Tensor& lmps, Tensor& rmps, int M, hipStream_t s);
 
@@ -90,7 +90,7 @@ double op_canonicalize_twosite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor&
     // sub-block (pdim_r x pdim_c) at (poff_r, poff_c) of its merged block
     rects.push_back({k, X.off[b], MC.pdim[pc], MR.pdim[pr], MC.pdim[pc], MR.poff[pr], MC.poff[pc]});
   }
-  return canonicalize_core(forward, X, MR, MC, blks, rects, wfnc.ptr(), X.nelem, lmps, rmps, M, s);
+  return canonicalize_core(forward, X, 2, MR, MC, blks, rects, wfnc.ptr(), X.nelem, lmps, rmps, M, s);
 }
 
 double op_canonicalize_merged(bool forward, Tensor& wm, const Layout& shape4, Tensor& lmps, Tensor& rmps, int M,
@@ -110,12 +110,50 @@ double op_canonicalize_merged(bool forward, Tensor& wm, const Layout& shape4, Te
     blks.push_back({wi[0], wi[1], MR.dm[wi[0]], MC.dm[wi[1]]});
     rects.push_back({(int)b, W.off[b], MC.dm[wi[1]], MR.dm[wi[0]], MC.dm[wi[1]], 0, 0});
   }
-  return canonicalize_core(forward, X, MR, MC, blks, rects, wm.ptr(), W.nelem, lmps, rmps, M, s);
+  return canonicalize_core(forward, X, 2, MR, MC, blks, rects, wm.ptr(), W.nelem, lmps, rmps, M, s);
+}
+
+double op_canonicalize_onesite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor& rmps, int M, hipStream_t s) {
+  if (wfnc.empty || wfnc.L.rank != 3) throw Error(MPSXX_ERR_ARG, "canonicalize: expects a 3-index wavefunction");
+  const Layout& X = wfnc.L;
+  // LeftArrow: rows (l, n), columns r; RightArrow: rows l, columns (n, r)  (canonicalize.hpp:71-84)
+  const int nr = forward ? 2 : 1;
+  auto side = [&](int lo, int hi) {
+    std::vector<std::vector<QN>> qs(X.qs.begin() + lo, X.qs.begin() + hi);
+    std::vector<std::vector<int>> ds(X.ds.begin() + lo, X.ds.begin() + hi);
+    return merge_info(qs, ds);
+  };
+  MergeInfo MR = side(0, nr), MC = side(nr, 3);
+  auto packed = [&](const int32_t* xi, int lo, int hi, const MergeInfo& Mi) {
+    return hi - lo == 1 ? xi[lo] : xi[lo] * Mi.radix[1] + xi[lo + 1];
+  };
+  std::vector<Blk> blks;
+  {
+    std::vector<std::pair<int, int>> seen;
+    for (int64_t b = 0; b < X.nblocks(); ++b) {
+      const int32_t* xi = X.index(b);
+      seen.push_back({MR.p2m[packed(xi, 0, nr, MR)], MC.p2m[packed(xi, nr, 3, MC)]});
+    }
+    std::sort(seen.begin(), seen.end());
+    seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+    for (auto& p : seen) blks.push_back({p.first, p.second, MR.dm[p.first], MC.dm[p.second]});
+  }
+  std::vector<Rect> rects;
+  for (int64_t b = 0; b < X.nblocks(); ++b) {
+    const int32_t* xi = X.index(b);
+    const int pr = packed(xi, 0, nr, MR), pc = packed(xi, nr, 3, MC);
+    const Blk key{MR.p2m[pr], MC.p2m[pc], 0, 0};
+    const int k = (int)(std::lower_bound(blks.begin(), blks.end(), key, [](const Blk& x, const Blk& y) {
+                          return x.i != y.i ? x.i < y.i : x.j < y.j;
+                        }) - blks.begin());
+    rects.push_back({k, X.off[b], MC.pdim[pc], MR.pdim[pr], MC.pdim[pc], MR.poff[pr], MC.poff[pc]});
+  }
+  return canonicalize_core(forward, X, nr, MR, MC, blks, rects, wfnc.ptr(), X.nelem, lmps, rmps, M, s);
 }
 
 namespace {
 
-double canonicalize_core(bool forward, const Layout& X, const MergeInfo& MR, const MergeInfo& MC,
+double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo& MR, const MergeInfo& MC,
                          const std::vector<Blk>& blks, const std::vector<Rect>& rects, const double* src,
                          int64_t src_elems, Tensor& lmps, Tensor& rmps, int M, hipStream_t s) {
   const int nb = (int)blks.size();
@@ -221,28 +259,47 @@ double canonicalize_core(bool forward, const Layout& X, const MergeInfo& MR, con
           dnz.push_back(kept[k]);
         }
   }
-  // U(l, n1, s): quantum shape (q_l, q_n1, -q_s)
+  // U(row indices..., s): quantum shape (q_rows..., -q_s); V(s, column indices...)
+  const int nc = X.rank - nr;
   Layout LU;
   {
     std::vector<QN> nq;
     for (const QN& q : qnz) nq.push_back(-q);
-    LU.set_shape(3, true, forward ? QN() : X.q, {X.qs[0], X.qs[1], nq}, {X.ds[0], X.ds[1], dnz});
+    std::vector<std::vector<QN>> qs(X.qs.begin(), X.qs.begin() + nr);
+    std::vector<std::vector<int>> ds(X.ds.begin(), X.ds.begin() + nr);
+    qs.push_back(nq);
+    ds.push_back(dnz);
+    LU.set_shape(nr + 1, true, forward ? QN() : X.q, qs, ds);
   }
   Layout LV;
-  LV.set_shape(3, true, forward ? X.q : QN(), {qnz, X.qs[2], X.qs[3]}, {dnz, X.ds[2], X.ds[3]});
+  {
+    std::vector<std::vector<QN>> qs{qnz};
+    std::vector<std::vector<int>> ds{dnz};
+    qs.insert(qs.end(), X.qs.begin() + nr, X.qs.end());
+    ds.insert(ds.end(), X.ds.begin() + nr, X.ds.end());
+    LV.set_shape(nc + 1, true, forward ? X.q : QN(), qs, ds);
+  }
+  // packed index of a merged side -> its index tuple (first index slowest)
+  auto unpack = [](const MergeInfo& Mi, int p, int n, int32_t* out) {
+    if (n == 1) {
+      out[0] = p;
+    } else {
+      out[0] = p / Mi.radix[1];
+      out[1] = p % Mi.radix[1];
+    }
+  };
   std::vector<CopySeg> useg, vseg;
   struct Sub { int32_t idx[3]; int64_t src, ld; int R, C; };
   std::vector<Sub> us, vs;
   for (int k = 0; k < nb; ++k) {
     if (sec_of_blk[k] < 0) continue;
     const int d = kept[k];
-    // rows of U: packed (l, n1) of row sector i
+    // rows of U: packed row indices of row sector i
     for (size_t p = 0; p < MR.p2m.size(); ++p) {
       if (MR.p2m[p] != blks[k].i || MR.pdim[p] == 0) continue;
       Sub sb;
-      sb.idx[0] = (int32_t)(p / MR.radix[1]);
-      sb.idx[1] = (int32_t)(p % MR.radix[1]);
-      sb.idx[2] = sec_of_blk[k];
+      unpack(MR, (int)p, nr, sb.idx);
+      sb.idx[nr] = sec_of_blk[k];
       if (!LU.allowed(sb.idx)) continue;
       sb.src = k * sv + (int64_t)MR.poff[p] * kmax;  // U_A row-major (mmax x kmax) in V
       sb.ld = kmax;
@@ -255,8 +312,7 @@ double canonicalize_core(bool forward, const Layout& X, const MergeInfo& MR, con
       if (MC.p2m[p] != blks[k].j || MC.pdim[p] == 0) continue;
       Sub sb;
       sb.idx[0] = sec_of_blk[k];
-      sb.idx[1] = (int32_t)(p / MC.radix[1]);
-      sb.idx[2] = (int32_t)(p % MC.radix[1]);
+      unpack(MC, (int)p, nc, sb.idx + 1);
       if (!LV.allowed(sb.idx)) continue;
       sb.src = k * su + MC.poff[p];  // Vt_A row-major (kmax x nmax) in U
       sb.ld = nmax;

@@ -21,13 +21,24 @@ struct SweepStats {
   double dweight = 0.0;
 };
 
+// A converged lower root k for excited states (dmrg/make_sweep.cpp with iroot > 0, dmrg/optimize.hpp:127-230): its
+// two-site wavefunction at every bond, its left- and right-canonical site tensors, and the cross environments of the
+// current root against it, <cur|H|k> (lH, rH) and <cur|k> (lS, rS), renormalized with bra = current root and
+// ket = root k as the sweep moves.
+struct LowerRoot {
+  double energy = 0.0;
+  std::vector<std::unique_ptr<Tensor>> A, B, psi, C;  // C[i]: one-site wavefunction with the centre at site i
+  std::vector<std::unique_ptr<Tensor>> lH, rH, lS, rS;
+};
+
 struct Dmrg {
   int L = 0;
   std::vector<Tensor*> mpo;                 // borrowed
   std::vector<std::unique_ptr<Tensor>> mps; // site tensors (A left of the centre, wave at the centre, B right)
   std::vector<std::unique_ptr<Tensor>> lenv, renv;  // lenv[i]: left of site i; renv[i]: right of site i-1
   int center = 0;
-  bool merged = false;  // optimize_twosite_merged (src/make_sweep.cpp:112) instead of the 4-index form
+  int form = MPSXX_SWEEP_FOURINDEX;  // two-site 4-index, merged (src/make_sweep.cpp:112) or one-site
+  std::vector<LowerRoot> lower;
 };
 
 namespace {
@@ -44,6 +55,24 @@ std::unique_ptr<Tensor> boundary_env(hipStream_t s) {
   auto t = std::make_unique<Tensor>();
   t->set_layout(L, s);
   launch_fill(t->ptr(), 1, 1.0, s);
+  return t;
+}
+
+std::unique_ptr<Tensor> boundary_overlap(hipStream_t s) {
+  Layout L;
+  L.set_shape(2, true, QN(), {{QN()}, {QN()}}, {{1}, {1}});
+  int32_t z[2] = {0, 0};
+  L.add_block(z);
+  L.finalize();
+  auto t = std::make_unique<Tensor>();
+  t->set_layout(L, s);
+  launch_fill(t->ptr(), 1, 1.0, s);
+  return t;
+}
+
+std::unique_ptr<Tensor> copy_of(Tensor& x, hipStream_t s) {
+  auto t = std::make_unique<Tensor>();
+  op_copy(x, *t, s);
   return t;
 }
 
@@ -155,16 +184,98 @@ void optimize_merged(Dmrg& D, int i, bool forward, Tensor& psi, int M, double to
   *D.mps[i + 1] = std::move(V);
 }
 
+// One forward + one backward one-site sweep (dmrg/make_sweep.cpp with ONESITE, optimize_onesite
+// dmrg/optimize.hpp:20-125, noise 0): the wave at site i is optimized with the one-site sigma vector and diagonal,
+// split by the one-site SVD (LeftArrow forward, RightArrow backward, truncation to M) and the 2-index factor is
+// multiplied into the next site, which becomes the wave.  Forward over sites 0..L-2, backward over L-1..1; the centre
+// ends at site 0.
+double dmrg_sweep_onesite(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStream_t s) {
+  const int L = D.L;
+  const auto t0 = clk::now();
+  double emin = 1.0e8;
+  auto optimize = [&](int i, bool forward) {
+    Tensor psi;
+    op_copy(*D.mps[i], psi, s);
+    auto t = clk::now();
+    Tensor diag = shape_only(psi.L);
+    op_diag_onesite(*D.mpo[i], *D.lenv[i], *D.renv[i + 1], diag, s);
+    hip_check(hipStreamSynchronize(s), "diag");
+    st.diag_s += since(t);
+    t = clk::now();
+    DavidsonStats ds;
+    std::vector<DavGroup> g{{D.mpo[i], nullptr, D.lenv[i].get(), D.renv[i + 1].get()}};
+    const size_t nl = D.lower.size();
+    std::vector<Tensor> Hk(nl), Sk(nl);
+    std::vector<DavLowerRoot> low;
+    for (size_t k = 0; k < nl; ++k) {
+      LowerRoot& R = D.lower[k];
+      op_sigma_onesite(*D.mpo[i], *R.lH[i], *R.rH[i + 1], *R.C[i], Hk[k], MPSXX_FP64, s);
+      op_sigma_overlap(*R.lS[i], *R.rS[i + 1], *R.C[i], Sk[k], s);
+      low.push_back({&Hk[k], &Sk[k], R.energy});
+    }
+    const double e = davidson_twosite(g, diag, psi, 0.1 * tol, prec, ds, s, low);
+    st.davidson_s += since(t);
+    st.nsigma += ds.nsigma;
+    st.nsigma_fp32 += ds.nsigma_fp32;
+    st.sigma_ms += ds.sigma_ms;
+    if (e < emin) emin = e;
+    t = clk::now();
+    Tensor U, V;
+    st.dweight += op_canonicalize_onesite(forward, psi, U, V, M, s);
+    st.svd_s += since(t);
+    t = clk::now();
+    if (forward) {
+      st.max_bond = std::max(st.max_bond, bond_dim(U.L, 2));
+      Tensor next;
+      op_contract(1.0, V, false, {1}, *D.mps[i + 1], false, {0}, 1.0, next, s);  // s*Vt times the next site
+      *D.mps[i] = std::move(U);
+      *D.mps[i + 1] = std::move(next);
+      D.lenv[i + 1] = std::make_unique<Tensor>();
+      op_renormalize(true, *D.mpo[i], *D.lenv[i], *D.mps[i], *D.mps[i], *D.lenv[i + 1], MPSXX_FP64, s);
+      for (LowerRoot& R : D.lower) {
+        R.lH[i + 1] = std::make_unique<Tensor>();
+        op_renormalize(true, *D.mpo[i], *R.lH[i], *D.mps[i], *R.A[i], *R.lH[i + 1], MPSXX_FP64, s);
+        R.lS[i + 1] = std::make_unique<Tensor>();
+        op_renormalize_overlap(true, *R.lS[i], *D.mps[i], *R.A[i], *R.lS[i + 1], s);
+      }
+    } else {
+      st.max_bond = std::max(st.max_bond, bond_dim(V.L, 0));
+      Tensor prev;
+      op_contract(1.0, *D.mps[i - 1], false, {2}, U, false, {0}, 1.0, prev, s);  // previous site times U*s
+      *D.mps[i] = std::move(V);
+      *D.mps[i - 1] = std::move(prev);
+      D.renv[i] = std::make_unique<Tensor>();
+      op_renormalize(false, *D.mpo[i], *D.renv[i + 1], *D.mps[i], *D.mps[i], *D.renv[i], MPSXX_FP64, s);
+      for (LowerRoot& R : D.lower) {
+        R.rH[i] = std::make_unique<Tensor>();
+        op_renormalize(false, *D.mpo[i], *R.rH[i + 1], *D.mps[i], *R.B[i], *R.rH[i], MPSXX_FP64, s);
+        R.rS[i] = std::make_unique<Tensor>();
+        op_renormalize_overlap(false, *R.rS[i + 1], *D.mps[i], *R.B[i], *R.rS[i], s);
+      }
+    }
+    hip_check(hipStreamSynchronize(s), "renormalize");
+    st.renorm_s += since(t);
+  };
+  for (int i = 0; i < L - 1; ++i) optimize(i, true);
+  for (int i = L - 1; i >= 1; --i) optimize(i, false);
+  D.center = 0;
+  st.seconds += since(t0);
+  return emin;
+}
+
 // One forward + one backward two-site sweep; returns the minimum site energy.
 double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStream_t s) {
   const int L = D.L;
+  if (D.form == MPSXX_SWEEP_MERGED && !D.lower.empty())
+    throw Error(MPSXX_ERR_UNSUPPORTED, "dmrg: lower roots are not supported with the merged sweep form");
+  if (D.form == MPSXX_SWEEP_ONESITE) return dmrg_sweep_onesite(D, M, tol, prec, st, s);
   const auto t0 = clk::now();
   double emin = 1.0e8;
   auto optimize = [&](int i, bool forward) {
     // sites (i, i+1); forward: mps[i] is the wave, mps[i+1] right-canonical; backward: mps[i] left, mps[i+1] wave
     Tensor psi;
     twosite(*D.mps[i], *D.mps[i + 1], psi, s);
-    if (D.merged) {
+    if (D.form == MPSXX_SWEEP_MERGED) {
       optimize_merged(D, i, forward, psi, M, tol, prec, st, emin, s);
       return;
     }
@@ -176,7 +287,17 @@ double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStrea
     t = clk::now();
     DavidsonStats ds;
     std::vector<DavGroup> g{{D.mpo[i], D.mpo[i + 1], D.lenv[i].get(), D.renv[i + 2].get()}};
-    const double e = davidson_twosite(g, diag, psi, 0.1 * tol, prec, ds, s);
+    // lower roots: H|k> and S|k> in the current root's basis (Davidson::Functor constructor, diagonalize.hpp:128-137)
+    const size_t nl = D.lower.size();
+    std::vector<Tensor> Hk(nl), Sk(nl);
+    std::vector<DavLowerRoot> low;
+    for (size_t k = 0; k < nl; ++k) {
+      LowerRoot& R = D.lower[k];
+      op_sigma_twosite(*D.mpo[i], *D.mpo[i + 1], *R.lH[i], *R.rH[i + 2], *R.psi[i], Hk[k], MPSXX_FP64, s);
+      op_sigma_overlap(*R.lS[i], *R.rS[i + 2], *R.psi[i], Sk[k], s);
+      low.push_back({&Hk[k], &Sk[k], R.energy});
+    }
+    const double e = davidson_twosite(g, diag, psi, 0.1 * tol, prec, ds, s, low);
     st.davidson_s += since(t);
     st.nsigma += ds.nsigma;
     st.nsigma_fp32 += ds.nsigma_fp32;
@@ -198,6 +319,20 @@ double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStrea
       op_renormalize(false, *D.mpo[i + 1], *D.renv[i + 2], *D.mps[i + 1], *D.mps[i + 1], *D.renv[i + 1], MPSXX_FP64,
                      s);
     }
+    // cross environments, bra = the current root's new site tensor, ket = root k's (dmrg/optimize.hpp:190-225)
+    for (LowerRoot& R : D.lower) {
+      if (forward) {
+        R.lH[i + 1] = std::make_unique<Tensor>();
+        op_renormalize(true, *D.mpo[i], *R.lH[i], *D.mps[i], *R.A[i], *R.lH[i + 1], MPSXX_FP64, s);
+        R.lS[i + 1] = std::make_unique<Tensor>();
+        op_renormalize_overlap(true, *R.lS[i], *D.mps[i], *R.A[i], *R.lS[i + 1], s);
+      } else {
+        R.rH[i + 1] = std::make_unique<Tensor>();
+        op_renormalize(false, *D.mpo[i + 1], *R.rH[i + 2], *D.mps[i + 1], *R.B[i + 1], *R.rH[i + 1], MPSXX_FP64, s);
+        R.rS[i + 1] = std::make_unique<Tensor>();
+        op_renormalize_overlap(false, *R.rS[i + 2], *D.mps[i + 1], *R.B[i + 1], *R.rS[i + 1], s);
+      }
+    }
     hip_check(hipStreamSynchronize(s), "renormalize");
     st.renorm_s += since(t);
   };
@@ -206,6 +341,62 @@ double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStrea
   D.center = 0;
   st.seconds += since(t0);
   return emin;
+}
+
+namespace {
+
+// Right cross environments of the current root (centre at site 0, right-canonical tensors) against lower root R.
+void rebuild_cross_right(Dmrg& D, LowerRoot& R, hipStream_t s) {
+  const int L = D.L;
+  for (int i = L - 1; i >= 1; --i) {
+    R.rH[i] = std::make_unique<Tensor>();
+    op_renormalize(false, *D.mpo[i], *R.rH[i + 1], *D.mps[i], *R.B[i], *R.rH[i], MPSXX_FP64, s);
+    R.rS[i] = std::make_unique<Tensor>();
+    op_renormalize_overlap(false, *R.rS[i + 1], *D.mps[i], *R.B[i], *R.rS[i], s);
+  }
+  hip_check(hipStreamSynchronize(s), "lower root");
+}
+
+}  // namespace
+
+// Register the converged state of `K` (centre at site 0, as a sweep leaves it) as a lower root of `D` (centre at site
+// 0): a forward gauge sweep of K's copy without truncation gives its left-canonical tensors and its two-site
+// wavefunction at every bond (right-canonical tensors are K's own), then the right cross environments of D against it
+// are built from D's current right-canonical tensors, as make_random_mpss.cpp:176-236 builds them for iroot > 0.
+void dmrg_add_lower_root(Dmrg& D, Dmrg& K, double energy, hipStream_t s) {
+  const int L = D.L;
+  if (K.L != L) throw Error(MPSXX_ERR_ARG, "dmrg: lower root has a different number of sites");
+  if (D.center != 0 || K.center != 0) throw Error(MPSXX_ERR_ARG, "dmrg: both states must have the centre at site 0");
+  if (D.form == MPSXX_SWEEP_MERGED)
+    throw Error(MPSXX_ERR_UNSUPPORTED, "dmrg: lower roots are not supported with the merged sweep form");
+  LowerRoot R;
+  R.energy = energy;
+  R.A.resize(L);
+  R.B.resize(L);
+  R.psi.resize(L - 1);
+  R.C.resize(L);
+  for (int i = 1; i < L; ++i) R.B[i] = copy_of(*K.mps[i], s);
+  std::unique_ptr<Tensor> C = copy_of(*K.mps[0], s);
+  for (int i = 0; i < L - 1; ++i) {
+    R.C[i] = copy_of(*C, s);
+    R.psi[i] = std::make_unique<Tensor>();
+    twosite(*C, *R.B[i + 1], *R.psi[i], s);
+    Tensor U, V;
+    op_canonicalize_twosite(true, *R.psi[i], U, V, 0, s);
+    R.A[i] = std::make_unique<Tensor>(std::move(U));
+    C = std::make_unique<Tensor>(std::move(V));
+  }
+  R.C[L - 1] = std::move(C);
+  R.lH.resize(L + 1);
+  R.rH.resize(L + 1);
+  R.lS.resize(L + 1);
+  R.rS.resize(L + 1);
+  R.lH[0] = boundary_env(s);
+  R.lS[0] = boundary_overlap(s);
+  R.rH[L] = boundary_env(s);
+  R.rS[L] = boundary_overlap(s);
+  D.lower.push_back(std::move(R));
+  rebuild_cross_right(D, D.lower.back(), s);
 }
 
 // <psi| dH/dW_k |psi> for every site k (the adjoint of the sigma-vector contraction with respect to the MPO tensor),
@@ -253,6 +444,8 @@ void dmrg_mpo_gradient(Dmrg& D, const std::vector<Tensor*>& G, hipStream_t s) {
     op_renormalize(false, *D.mpo[i + 1], *D.renv[i + 2], *D.mps[i + 1], *D.mps[i + 1], *D.renv[i + 1], MPSXX_FP64, s);
   }
   D.center = 0;
+  // the gauge of the right-canonical tensors changed: rebuild the cross environments of any lower roots
+  for (LowerRoot& R : D.lower) rebuild_cross_right(D, R, s);
   hip_check(hipStreamSynchronize(s), "gradient");
 }
 
@@ -354,11 +547,22 @@ int mpsxx_dmrg_sweep(mpsxx_dmrg d, int max_states, double tolerance, int precisi
   });
 }
 
+int mpsxx_dmrg_add_lower_root(mpsxx_dmrg d, mpsxx_dmrg root, double energy, void* stream) {
+  return dguard([&] {
+    if (!d || !d->d || !root || !root->d) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: null handle");
+    if (d == root) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: a state cannot be its own lower root");
+    mpx::dmrg_add_lower_root(*d->d, *root->d, energy, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
 int mpsxx_dmrg_set_form(mpsxx_dmrg d, int form) {
   return dguard([&] {
     if (!d || !d->d) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: null handle");
-    if (form != MPSXX_SWEEP_FOURINDEX && form != MPSXX_SWEEP_MERGED) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: unknown form");
-    d->d->merged = form == MPSXX_SWEEP_MERGED;
+    if (form != MPSXX_SWEEP_FOURINDEX && form != MPSXX_SWEEP_MERGED && form != MPSXX_SWEEP_ONESITE)
+      throw mpx::Error(MPSXX_ERR_ARG, "dmrg: unknown form");
+    if (form == MPSXX_SWEEP_MERGED && !d->d->lower.empty())
+      throw mpx::Error(MPSXX_ERR_UNSUPPORTED, "dmrg: lower roots are not supported with the merged sweep form");
+    d->d->form = form;
   });
 }
 

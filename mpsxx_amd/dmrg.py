@@ -67,7 +67,7 @@ class SweepStats(ctypes.Structure):
                 ("nsigma_fp32", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
-SWEEP_FOURINDEX, SWEEP_MERGED = 0, 1
+SWEEP_FOURINDEX, SWEEP_MERGED, SWEEP_ONESITE = 0, 1, 2
 
 
 class DeviceDMRG:
@@ -101,6 +101,18 @@ class DeviceDMRG:
         if rc != 0:
             raise RuntimeError("mpsxx_dmrg_sweep: " + lib().mpsxx_dmrg_last_error().decode())
         return e.value, {k: getattr(st, k) for k, _ in SweepStats._fields_}
+
+    def set_form(self, form):
+        """Sweep form of later sweeps: SWEEP_FOURINDEX, SWEEP_MERGED or SWEEP_ONESITE."""
+        rc = lib().mpsxx_dmrg_set_form(self.h, ctypes.c_int(form))
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_set_form: " + lib().mpsxx_dmrg_last_error().decode())
+
+    def add_lower_root(self, root, energy):
+        """Project the converged state of DeviceDMRG `root` (energy E_k) out of later sweeps (excited states)."""
+        rc = lib().mpsxx_dmrg_add_lower_root(self.h, root.h, ctypes.c_double(energy), None)
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_add_lower_root: " + lib().mpsxx_dmrg_last_error().decode())
 
     def mpo_gradient(self):
         """[<psi| dH/dW_k |psi>] for every site (DTensors; W_k's block indices with quantum numbers negated)."""

@@ -108,3 +108,77 @@ def test_mixed_precision_sweeps_match_fp64(prec):
     print("E fp64", e64, "mixed", e_mx, "low-precision sigma", nfp, flush=True)
     assert nfp > 0
     assert abs(e_mx - e64) <= 1e-8, (e_mx, e64)
+
+
+def _roots(mpos, L, qt, M, nroots, sweeps, tol):
+    """Roots 0..nroots-1 one after another, each projected against the converged lower ones (dmrg/dmrg.cpp:36-72)."""
+    done, energies = [], []
+    for k in range(nroots):
+        dm = DeviceDMRG(mpos, initial_mps(L, qt, M, k), M)
+        for r, e_r in zip(done, energies):
+            dm.add_lower_root(r, e_r)
+        e_prev = None
+        for _ in range(sweeps):
+            e, _ = dm.sweep(M, tol)
+            if e_prev is not None and abs(e - e_prev) < tol:
+                break
+            e_prev = e
+        done.append(dm)
+        energies.append(e)
+    return energies
+
+
+def test_excited_states_match_ed():
+    """Excited states by sequential projected sweeps: Hubbard L = 6, U = 4, half filling, 2Sz = 0, M = 64 (exact at
+    this size): roots 0..3 equal the four lowest eigenvalues of the sector Hamiltonian (dimension 400) to 1e-8."""
+    print("excited states L=6", flush=True)
+    L, M = 6, 64
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    es = _roots(mpos, L, (L, 0), M, 4, 10, 1e-11)
+    h, v = ed.hubbard_integrals(L, 1.0, 4.0)
+    H = ed.hamiltonian(h, v, L, 0.0, ed.sector_states(L, L, 0)).toarray()
+    w = np.linalg.eigvalsh(H)[:4]
+    print("dmrg", es, "ed", w, flush=True)
+    assert np.max(np.abs(np.array(es) - w)) < 1e-8, (es, w)
+
+
+def test_onesite_refinement_sweeps_match_ed():
+    """dmrg/dmrg.cpp:76-114: after the two-site sweeps, one-site sweeps (optimize_onesite, one-site sigma vector,
+    diagonal and SVD) refine the state.  Hubbard L = 6, U = 4 at M = 64 (exact bond dimension here): the one-site
+    energy equals ED to 1e-9, for the ground state and for root 1 projected against it.  L = 8 at M = 64 (truncated):
+    a one-site sweep measures the truncated MPS itself, so its energy lies above the two-site value (which is taken
+    before truncation) by the truncation error (< 1e-6 here) and does not rise from one one-site sweep to the next."""
+    from mpsxx_amd.dmrg import SWEEP_ONESITE
+    print("onesite sweeps", flush=True)
+    L, M = 6, 64
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    h, v = ed.hubbard_integrals(L, 1.0, 4.0)
+    w = np.linalg.eigvalsh(ed.hamiltonian(h, v, L, 0.0, ed.sector_states(L, L, 0)).toarray())
+    r0 = DeviceDMRG(mpos, initial_mps(L, (L, 0), M, 0), M)
+    for _ in range(4):
+        ea, _ = r0.sweep(M, 1e-11)
+    r0.set_form(SWEEP_ONESITE)
+    for _ in range(2):
+        ea1, st = r0.sweep(M, 1e-11)
+    assert abs(ea1 - w[0]) < 1e-9 and abs(ea - w[0]) < 1e-9, (ea, ea1, w[0])
+    assert st["max_bond"] <= M
+    r1 = DeviceDMRG(mpos, initial_mps(L, (L, 0), M, 1), M)
+    r1.add_lower_root(r0, ea1)
+    for _ in range(4):
+        eb, _ = r1.sweep(M, 1e-11)
+    r1.set_form(SWEEP_ONESITE)
+    for _ in range(2):
+        eb1, _ = r1.sweep(M, 1e-11)
+    assert abs(eb1 - w[1]) < 1e-8, (eb, eb1, w[:3])
+    # truncated case
+    L, M = 8, 64
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    dm = DeviceDMRG(mpos, initial_mps(L, (L, 0), M, 0), M)
+    for _ in range(4):
+        e2, _ = dm.sweep(M, 1e-10)
+    dm.set_form(SWEEP_ONESITE)
+    e1a, _ = dm.sweep(M, 1e-10)
+    e1b, _ = dm.sweep(M, 1e-10)
+    h, v = ed.hubbard_integrals(L, 1.0, 4.0)
+    e0 = ed.ground_energy(h, v, L, L, 0)
+    assert e2 - 1e-10 <= e1b <= e1a + 1e-10 and e1b - e0 < 1e-6, (e2, e1a, e1b, e0)

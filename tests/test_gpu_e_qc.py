@@ -152,3 +152,29 @@ def test_merged_sigma_qc_split_bf16_close_to_fp64():
     assert rel <= 1e-4, rel
     got32 = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=mx.FP32).to_qst()
     assert rel >= 0.5 * got32.diff_norm(ref) / ref.norm()  # not more accurate than FP32: the split path really ran
+
+
+def test_hubbard12_excited_states_match_reference_log():
+    """Four roots of the reference's Hubbard-12 run (dmrg/tests/twodot.conf: M = 200, nroots 4) from its FCIDUMP
+    through the QC MPO, each root projected against the converged lower ones: the energies in
+    dmrg/tests/hub12_dmrg.out:15536-15539 to 2e-6 (the reference adds one-site sweeps after the two-site ones)."""
+    print("hub12 excited states", flush=True)
+    ref = [-11.8406372775523305, -11.4516084815790968, -11.2058172994693876, -11.0278375085199389]
+    norb, nelec, ms2, h, v, ecore = qc.read_fcidump(os.path.join(HERE, "golden", "hubbard12.FCIDUMP"))
+    mpos = qc.QcMpo(h, v, ecore).devices()
+    M = 200
+    done, es = [], []
+    for k in range(4):
+        dm = DeviceDMRG(mpos, initial_mps(norb, (nelec, ms2), M, k), M)
+        for r, e_r in zip(done, es):
+            dm.add_lower_root(r, e_r)
+        e_prev = None
+        for sweep in range(12):
+            e, st = dm.sweep(M, 1e-8)
+            if e_prev is not None and abs(e - e_prev) < 1e-10:
+                break
+            e_prev = e
+        print("root", k, e, "sweeps", sweep + 1, flush=True)
+        done.append(dm)
+        es.append(e)
+    assert np.max(np.abs(np.array(es) - np.array(ref))) < 2e-6, (es, ref)
