@@ -99,6 +99,7 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
                         DavidsonStats& st, hipStream_t s, const std::vector<DavLowerRoot>& lower) {
   if (wfnc.empty) throw Error(MPSXX_ERR_ARG, "Davidson: empty initial wavefunction");
   if (groups.empty()) throw Error(MPSXX_ERR_ARG, "Davidson: no operator groups");
+  if (groups.size() > kPlanCacheKeep) throw Error(MPSXX_ERR_ARG, "Davidson: more than 16 operator groups");
   const int kind = groups[0].wfn_rank();  // 4 two-site, 3 one-site, 2 merged
   for (const DavGroup& g : groups)
     if (g.wfn_rank() != kind) throw Error(MPSXX_ERR_ARG, "Davidson: operator groups of different forms mixed");

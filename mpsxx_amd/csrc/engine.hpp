@@ -163,6 +163,15 @@ struct Plan {
   int64_t flops = 0;                   // algorithmic flops (reference order)
   int64_t bytes = 0;                   // algorithmic bytes (inputs + output, FP64)
   int precision = 0;
+  // device memory held by the plan (intermediates, FP32 operand copies, task lists)
+  size_t device_bytes() const {
+    size_t n = 0;
+    for (const DevBuf& b : slot_buf) n += b.bytes;
+    for (const Step& st : steps)
+      n += st.work.bytes + st.tasks.bytes + st.pairs.bytes + st.work_red.bytes + st.partial.bytes +
+           st.work_small.bytes + st.work_split.bytes;
+    return n;
+  }
 };
 
 // Builders of device steps from symbolic results.

@@ -2,7 +2,9 @@
 // calls (cited per function), planned once per input structure and replayed as grouped device launches.
 #include "ops.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <map>
@@ -34,9 +36,13 @@ enum OutMode { OUT_ACCUMULATE = 0, OUT_OVERWRITE = 1 };
 struct CachedPlan {
   Plan plan;
   int out_mode = OUT_ACCUMULATE;
+  size_t charged = 0;  // device bytes counted against the cache budget at insertion
 };
 
-// Plan cache keyed by (op, precision, structure hashes, extra ints); least-recently-used eviction.
+// Plan cache keyed by (op, precision, structure hashes, extra ints); least-recently-used eviction when it holds more
+// than kMax plans or more device memory than the budget (MPSXX_PLAN_CACHE_GB, default 64 GiB: plans of large QC
+// problems hold GB-sized intermediates, one per site).  The kKeep most recently used plans are never evicted, so the
+// plans an operation holds at once (a Davidson solve over its operator groups) stay valid.
 class PlanCache {
  public:
   CachedPlan* find(const std::vector<uint64_t>& key) {
@@ -46,20 +52,39 @@ class PlanCache {
     return it->second.first.get();
   }
   CachedPlan* insert(const std::vector<uint64_t>& key, std::unique_ptr<CachedPlan> p) {
-    if (map_.size() >= kMax) {
+    const size_t add = p->plan.device_bytes();
+    while (map_.size() > kKeep && (map_.size() >= kMax || bytes_ + add > budget())) {
       auto victim = lru_.back();
       lru_.pop_back();
-      map_.erase(victim);
+      auto it = map_.find(victim);
+      bytes_ -= it->second.first->charged;
+      map_.erase(it);
     }
     lru_.push_front(key);
     CachedPlan* raw = p.get();
+    raw->charged = add;
     map_[key] = {std::move(p), lru_.begin()};
+    bytes_ += add;
     return raw;
   }
-  void clear() { map_.clear(); lru_.clear(); }
+  void clear() {
+    map_.clear();
+    lru_.clear();
+    bytes_ = 0;
+  }
 
  private:
+  static size_t budget() {
+    static const size_t b = [] {
+      double gb = 64.0;
+      if (const char* e = std::getenv("MPSXX_PLAN_CACHE_GB")) gb = std::max(1.0, std::atof(e));
+      return (size_t)(gb * 1073741824.0);
+    }();
+    return b;
+  }
   static constexpr size_t kMax = 256;
+  static constexpr size_t kKeep = kPlanCacheKeep;
+  size_t bytes_ = 0;
   std::map<std::vector<uint64_t>, std::pair<std::unique_ptr<CachedPlan>, std::list<std::vector<uint64_t>>::iterator>>
       map_;
   std::list<std::vector<uint64_t>> lru_;

@@ -57,6 +57,9 @@ void op_renormalize_overlap(bool fwd, Tensor& sov0, Tensor& bra, Tensor& ket, Te
 void op_renormalize_merged(bool fwd, Tensor& opr0, Tensor& bra, Tensor& ket, Tensor& opr1, int prec,
                            hipStream_t s);
 
+// The most recently used cached plans are never evicted: a caller may hold references to at most this many plans
+// from the accessors below at once (the Davidson driver holds one per operator group).
+constexpr size_t kPlanCacheKeep = 16;
 // Plan for a two-site sigma vector (exposed for the Davidson driver, which binds device pointers itself).
 Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
                          const Layout& wfn, const Layout* out, int prec);
