@@ -1028,6 +1028,609 @@ void launch_svd_scale(double* X, int batch, int rows, int cols, int64_t stride, 
                                 kmax, by_col);
 }
 
+// ---- one-sided Jacobi SVD (Hestenes), blocked over column groups ----
+//
+// A round kernel runs one work-group per task: the task's nc <= COLS columns Y of X are loaded to LDS, their Gram
+// matrix G = Y^T Y is formed with FP64 MFMAs, and one cyclic Jacobi sweep runs on G in the round-robin (circle)
+// order, COLS/2 disjoint pairs per step.  For a pair (p, q) with a = G_pp, b = G_qq, g = G_pq the rotation is applied
+// when |g| > tol sqrt(a b), tol = eps * max(len, 32): zeta = (b - a) / 2g, t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)),
+// c = 1 / sqrt(1 + t^2), s = c t (Rutishauser's formulas), i.e. y_p <- c y_p - s y_q, y_q <- s y_p + c y_q, which
+// zeroes the pair's inner product.  The step's rotations update G two-sidedly (G <- J^T G J, one 2x2 block per
+// thread) and are gathered in Q (Q <- Q J); at the end Y <- Y Q and V <- V Q, again with MFMAs.  G is formed afresh
+// from the rotated columns by every task, so the inner products of the last sweep, which applies no rotation, are
+// exact dot products: a block has converged when one full sweep over all its column pairs applies no rotation.
+
+__global__ void svd_init_kernel(const SvdBlk* __restrict__ blks, double* V) {
+  const SvdBlk b = blks[blockIdx.x];
+  const int64_t n = (int64_t)b.kk * b.kk;
+  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) V[b.v_off + e] = (e / b.kk == e % b.kk) ? 1.0 : 0.0;
+}
+
+// Leading dimension (doubles) of a task's LDS columns of length len; the host sizes LDS with the same formula.
+__host__ __device__ __forceinline__ int svd_lds_ld(int len) { return ((len + 15) & ~15) + 4; }
+
+// Columns gcol(0..nc-1) of a column-major matrix (column length len) into LDS columns of stride ld; rows len..ld-1
+// and columns nc..cols-1 are zeroed.  Eight independent loads per thread are in flight before the LDS writes.
+template <typename F>
+__device__ __forceinline__ void svd_gather_columns(const double* __restrict__ src, int len, int nc, int cols, int ld,
+                                                   F gcol, double* dst, int tid, int nt) {
+  const int total = nc * len;
+  for (int base = tid; base < total; base += 8 * nt) {
+    double tmp[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = base + j * nt;
+      if (e < total) {
+        const int c = e / len;
+        tmp[j] = src[(int64_t)gcol(c) * len + (e - c * len)];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = base + j * nt;
+      if (e < total) {
+        const int c = e / len;
+        dst[c * ld + (e - c * len)] = tmp[j];
+      }
+    }
+  }
+  const int pad = ld - len;
+  for (int e = tid; e < nc * pad; e += nt) dst[(e / pad) * ld + len + e % pad] = 0.0;
+  for (int e = tid; e < (cols - nc) * ld; e += nt) dst[nc * ld + e] = 0.0;
+}
+
+// out(:, gcol(c)) = Y Q(:, c) for c < nc, rows < rows_out: Y (ld-strided LDS columns, COLS of them), Q (COLS x COLS
+// in Qs[k][c]).  Computed as the transposed product Q^T Y^T in 16x16 MFMA tiles, so that consecutive lanes hold
+// consecutive rows of one output column (coalesced stores).
+template <int COLS, typename F>
+__device__ __forceinline__ void svd_apply_q(const double* ys, int ld, int rows_out, const double (*Qs)[33], int nc,
+                                            F gcol, double* out, int64_t col_stride, int w, int l) {
+  constexpr int NCT = (COLS + 15) / 16;
+  const int nrt = (rows_out + 15) / 16;
+  for (int tile = w; tile < nrt * NCT; tile += 4) {
+    const int row0 = (tile / NCT) * 16, col0 = (tile % NCT) * 16;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k0 = 0; k0 < COLS; k0 += 4) {
+      const int k = k0 + l / 16, ci = col0 + l % 16;
+      const double a = ci < COLS ? Qs[k][ci] : 0.0;     // A = Q^T: A[i][k] = Q[k][col0 + i]
+      const double bb = ys[k * ld + row0 + l % 16];       // B = Y^T: B[k][j] = Y[row0 + j][k]
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+    }
+    const int r = row0 + l % 16;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = col0 + l / 16 + 4 * j;  // D[l/16 + 4j][l%16] = (Y Q)[row0 + l%16][col0 + l/16 + 4j]
+      if (c < nc && r < rows_out) out[(int64_t)gcol(c) * col_stride + r] = acc[j];
+    }
+  }
+}
+
+// Pair `t` of step `st` of a task: cross-pairs task (two groups, n0 >= n1): the n1 cross pairs (i, n0 + j) with
+// i = (j - st) mod n0, then the n0 - n1 columns of the first group without a partner this step paired among
+// themselves (and, if odd in number, with the zero slot n0 + n1) with no rotation; otherwise the circle method over
+// all n0 (+ n1) columns rounded up to even.  Every slot is in exactly one pair of a step.
+__device__ __forceinline__ void svd_pair(int n0, int n1, bool cross, int st, int t, int& p, int& q, bool& real) {
+  if (cross) {
+    if (t < n1) {
+      p = (t - st + n0) % n0;
+      q = n0 + t;
+      real = true;
+    } else {
+      const int u = 2 * (t - n1), idle = n0 - n1;
+      p = (n1 + u - st + n0) % n0;
+      q = u + 1 < idle ? (n1 + u + 1 - st + n0) % n0 : n0 + n1;
+      real = false;
+    }
+  } else {
+    const int nc = n0 + n1, P = nc + (nc & 1);
+    p = t == 0 ? P - 1 : (st + t) % (P - 1);
+    q = t == 0 ? st : (st - t + P - 1) % (P - 1);
+    real = true;
+  }
+}
+
+template <int COLS>
+__global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restrict__ tasks,
+                                                        const SvdBlk* __restrict__ blks, double* X, double* V,
+                                                        const int32_t* __restrict__ active, int32_t* rot) {
+  extern __shared__ double ys[];  // COLS columns of stride ld
+  __shared__ double Gs[32][33], Qs[32][33];
+  __shared__ double rcs[16], rsn[16];
+  __shared__ int rp[16], rq[16];
+  __shared__ int scnt;
+  const SvdTask t = tasks[blockIdx.x];
+  if (!active[t.blk]) return;
+  const SvdBlk b = blks[t.blk];
+  const int nc = t.n0 + t.n1, len = b.len, kk = b.kk, ld = svd_lds_ld(len);
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  auto gcol = [&](int c) { return c < t.n0 ? t.c0 + c : t.c1 + (c - t.n0); };
+  // V's columns are only needed at the end: their loads are issued now, into registers, when they fit
+  constexpr int kVpf = 32;
+  const bool vpf = nc * kk <= 256 * kVpf;
+  double vreg[kVpf];
+  if (vpf) {
+#pragma unroll
+    for (int j = 0; j < kVpf; ++j) {
+      const int e = tid + 256 * j;
+      if (e < nc * kk) {
+        const int c = e / kk;
+        vreg[j] = V[b.v_off + (int64_t)gcol(c) * kk + (e - c * kk)];
+      }
+    }
+  }
+  svd_gather_columns(X + b.x_off, len, nc, COLS, ld, gcol, ys, tid, 256);
+  for (int e = tid; e < COLS * COLS; e += 256) Qs[e / COLS][e % COLS] = (e / COLS == e % COLS) ? 1.0 : 0.0;
+  if (tid == 0) scnt = 0;
+  __syncthreads();
+  // G = Y^T Y (zero outside the task's nc columns)
+  {
+    constexpr int NT = (COLS + 15) / 16;
+    if (w < NT * NT) {
+      const int c0 = (w / NT) * 16, c1 = (w % NT) * 16, lenr = (len + 3) & ~3;
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      const int ca = c0 + l % 16, cb = c1 + l % 16;
+      for (int k0 = 0; k0 < lenr; k0 += 4) {
+        const double a = ca < COLS ? ys[ca * ld + k0 + l / 16] : 0.0;
+        const double bb = cb < COLS ? ys[cb * ld + k0 + l / 16] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = c0 + l / 16 + 4 * j, c = c1 + l % 16;
+        if (r < COLS && c < COLS) Gs[r][c] = (r < nc && c < nc) ? acc[j] : 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  const double tol = 2.220446049250313e-16 * (double)max(len, 32), tol2 = tol * tol;
+  const bool cross = t.cross != 0;
+  const int S = nc + (nc & 1), NP = S / 2, nsteps = cross ? t.n0 : S - 1;
+  int myrot = 0;
+  for (int st = 0; st < nsteps; ++st) {
+    if (tid < NP) {
+      int p, q;
+      bool real;
+      svd_pair(t.n0, t.n1, cross, st, tid, p, q, real);
+      double cs = 1.0, sn = 0.0;
+      if (real) {
+        const double sa = Gs[p][p], sb = Gs[q][q], sg = Gs[p][q];
+        // rotate when |g| > tol sqrt(a b); t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)) with zeta = (b - a) / 2g,
+        // written as 2g sign(b - a) / (|b - a| + sqrt((b - a)^2 + 4g^2))
+        if (sg != 0.0 && sg * sg > tol2 * (sa * sb)) {
+          const double d2 = sb - sa, den = fabs(d2) + sqrt(d2 * d2 + 4.0 * sg * sg);
+          if (den > 0.0) {
+            const double tt = copysign(1.0, d2) * (2.0 * sg) / den;
+            cs = 1.0 / sqrt(1.0 + tt * tt);
+            sn = cs * tt;
+            ++myrot;
+          }
+        }
+      }
+      rp[tid] = p;
+      rq[tid] = q;
+      rcs[tid] = cs;
+      rsn[tid] = sn;
+    }
+    __syncthreads();
+    if (tid < NP * NP) {
+      const int i = tid / NP, j = tid % NP;
+      const double ci = rcs[i], si = rsn[i], cj = rcs[j], sj = rsn[j];
+      if (si != 0.0 || sj != 0.0) {
+        const int pi = rp[i], qi = rq[i], pj = rp[j], qj = rq[j];
+        // G block (rows p_i, q_i; cols p_j, q_j) <- J_i^T B J_j, J = [[c, s], [-s, c]]
+        const double b00 = Gs[pi][pj], b01 = Gs[pi][qj], b10 = Gs[qi][pj], b11 = Gs[qi][qj];
+        const double t00 = cj * b00 - sj * b01, t01 = sj * b00 + cj * b01;
+        const double t10 = cj * b10 - sj * b11, t11 = sj * b10 + cj * b11;
+        Gs[pi][pj] = ci * t00 - si * t10;
+        Gs[pi][qj] = ci * t01 - si * t11;
+        Gs[qi][pj] = si * t00 + ci * t10;
+        Gs[qi][qj] = si * t01 + ci * t11;
+      }
+      if (sj != 0.0) {
+        // Q rows 2i, 2i+1, cols p_j, q_j <- B J_j
+        const int pj = rp[j], qj = rq[j];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const double u = Qs[2 * i + h][pj], v = Qs[2 * i + h][qj];
+          Qs[2 * i + h][pj] = cj * u - sj * v;
+          Qs[2 * i + h][qj] = sj * u + cj * v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (myrot) atomicAdd(&scnt, myrot);
+  __syncthreads();
+  const int total = scnt;
+  if (total == 0) return;  // uniform: nothing moved
+  svd_apply_q<COLS>(ys, ld, len, Qs, nc, gcol, X + b.x_off, len, w, l);
+  __syncthreads();
+  if (vpf) {
+    const int pad = ld - kk;
+#pragma unroll
+    for (int j = 0; j < kVpf; ++j) {
+      const int e = tid + 256 * j;
+      if (e < nc * kk) {
+        const int c = e / kk;
+        ys[c * ld + (e - c * kk)] = vreg[j];
+      }
+    }
+    for (int e = tid; e < nc * pad; e += 256) ys[(e / pad) * ld + kk + e % pad] = 0.0;
+    for (int e = tid; e < (COLS - nc) * ld; e += 256) ys[nc * ld + e] = 0.0;
+  } else {
+    svd_gather_columns(V + b.v_off, kk, nc, COLS, ld, gcol, ys, tid, 256);
+  }
+  __syncthreads();
+  svd_apply_q<COLS>(ys, ld, kk, Qs, nc, gcol, V + b.v_off, kk, w, l);
+  if (tid == 0) atomicAdd(rot + t.blk, total);
+}
+
+__global__ __launch_bounds__(256) void svd_finish_kernel(const SvdOut* __restrict__ outs,
+                                                         const double* __restrict__ Xn, const double* __restrict__ Va,
+                                                         double* S, int kmax, double* Ua, int64_t su_a, double* Vta,
+                                                         int64_t svt_a, int nmax) {
+  extern __shared__ double sig[];  // kk values, then kk ranks
+  const SvdOut o = outs[blockIdx.x];
+  const int kk = o.kk, lx = o.u_from_x ? o.m : o.n, lv = o.u_from_x ? o.n : o.m;
+  int* rank = reinterpret_cast<int*>(sig + kk);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const double* xn = Xn + o.xn_off;
+  const double* va = Va + o.va_off;
+  for (int j = w; j < kk; j += 4) {
+    double ss = 0.0;
+    for (int i = lane; i < lx; i += 64) {
+      const double u = xn[(int64_t)j * lx + i];
+      ss += u * u;
+    }
+    ss = wave_sum(ss);
+    if (lane == 0) sig[j] = sqrt(ss);
+  }
+  __syncthreads();
+  for (int j = tid; j < kk; j += 256) {
+    const double v = sig[j];
+    int r = 0;
+    for (int i = 0; i < kk; ++i) r += (sig[i] > v || (sig[i] == v && i < j)) ? 1 : 0;
+    rank[j] = r;
+    S[(int64_t)blockIdx.x * kmax + r] = v;
+  }
+  __syncthreads();
+  double* ua = Ua + (int64_t)blockIdx.x * su_a;
+  double* vt = Vta + (int64_t)blockIdx.x * svt_a;
+  const int64_t nx = (int64_t)lx * kk, nv = (int64_t)lv * kk;
+  for (int64_t e = tid; e < nx; e += 256) {
+    const int j = (int)(e / lx), i = (int)(e - (int64_t)j * lx);
+    const double val = sig[j] > 0.0 ? xn[e] / sig[j] : 0.0;
+    if (o.u_from_x) ua[(int64_t)i * kmax + rank[j]] = val;
+    else vt[(int64_t)rank[j] * nmax + i] = val;
+  }
+  for (int64_t e = tid; e < nv; e += 256) {
+    const int j = (int)(e / lv), i = (int)(e - (int64_t)j * lv);
+    if (o.u_from_x) vt[(int64_t)rank[j] * nmax + i] = va[e];
+    else ua[(int64_t)i * kmax + rank[j]] = va[e];
+  }
+}
+
+// ---- blocked Householder QR of the Jacobi input (one work-group of 256 threads per block) ----
+//
+// Panel p (columns p0 .. p0+15, rows p0 .. len-1, K = len - p0 rows) is factored in LDS column by column (LAPACK's
+// dlarfg: beta = -sign(alpha) |(alpha, x)|, tau = (beta - alpha) / beta, v = x / (alpha - beta), v_0 = 1), its
+// triangular factor T is formed (dlarft, forward columnwise: T_ii = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) V^T v_i),
+// and the trailing columns are updated in chunks of CW through LDS: C <- (I - V T^T V^T) C, the three products with
+// FP64 MFMAs.  Q applied to a matrix Z (Z <- Q Z) runs the panels backwards: Z <- (I - V T V^T) Z.
+
+__host__ __device__ __forceinline__ size_t svd_qr_lds_doubles(int cw, int ld) {
+  return (size_t)ld * (kSvdQrPanel + cw) + 256 + 2 * 16 * cw + 32;
+}
+
+// Panel/chunk gather: dst[c * ld + i] = src[c * col_stride + i] for c < ncols, i < K; zero elsewhere up to
+// ncols_alloc columns of ld rows.
+__device__ __forceinline__ void svd_qr_gather(const double* __restrict__ src, int64_t col_stride, int K, int ncols,
+                                              int ncols_alloc, int ld, double* dst, int tid) {
+  const int total = ncols * K;
+  for (int base = tid; base < total; base += 8 * 256) {
+    double tmp[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = base + j * 256;
+      if (e < total) {
+        const int c = e / K;
+        tmp[j] = src[(int64_t)c * col_stride + (e - c * K)];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = base + j * 256;
+      if (e < total) {
+        const int c = e / K;
+        dst[c * ld + (e - c * K)] = tmp[j];
+      }
+    }
+  }
+  const int pad = ld - K;
+  for (int e = tid; e < ncols * pad; e += 256) dst[(e / pad) * ld + K + e % pad] = 0.0;
+  for (int e = tid; e < (ncols_alloc - ncols) * ld; e += 256) dst[ncols * ld + e] = 0.0;
+}
+
+// C (K x CW in Cs, stride ld) <- C - V (Tt ? T^T : T) (V^T C); V explicit in Vp (16 columns, stride ld).
+template <int CW>
+__device__ __forceinline__ void svd_qr_block_reflect(const double* Vp, double* Cs, const double* Ts, double* Ws,
+                                                     double* W2, int K, int ld, bool transT, int tid) {
+  const int l = tid & 63, w = tid >> 6;
+  constexpr int NCT = CW / 16;
+  const int K4 = (K + 3) & ~3;
+  if (w < NCT) {  // Ws = V^T C
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K4; k0 += 4) {
+      const double a = Vp[(l % 16) * ld + k0 + l / 16];
+      const double bb = Cs[(w * 16 + l % 16) * ld + k0 + l / 16];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Ws[(l / 16 + 4 * j) * CW + w * 16 + l % 16] = acc[j];
+  }
+  __syncthreads();
+  for (int e = tid; e < 16 * CW; e += 256) {  // W2 = T^T Ws (lower) or T Ws (upper)
+    const int i = e / CW, c = e % CW;
+    double acc = 0.0;
+    if (transT) {
+      for (int j = 0; j <= i; ++j) acc += Ts[j * 16 + i] * Ws[j * CW + c];
+    } else {
+      for (int j = i; j < 16; ++j) acc += Ts[i * 16 + j] * Ws[j * CW + c];
+    }
+    W2[e] = acc;
+  }
+  __syncthreads();
+  const int nrt = (K + 15) / 16;
+  for (int tile = w; tile < nrt * NCT; tile += 4) {  // C -= V W2
+    const int row0 = (tile / NCT) * 16, ct = (tile % NCT) * 16;
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += 4) {
+      const double a = Vp[(k0 + l / 16) * ld + row0 + l % 16];
+      const double bb = W2[(k0 + l / 16) * CW + ct + l % 16];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+    }
+    // D[l/16 + 4j][l%16] = (V W2)[row0 + l/16 + 4j][ct + l%16]
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Cs[(ct + l % 16) * ld + row0 + l / 16 + 4 * j] -= acc[j];
+  }
+  __syncthreads();
+}
+
+template <int CW>
+__global__ __launch_bounds__(256) void svd_qr_kernel(const SvdQrBlk* __restrict__ blks, double* X, double* tau,
+                                                     double* T) {
+  extern __shared__ double sm[];
+  const SvdQrBlk b = blks[blockIdx.x];
+  const int len = b.len, kk = b.kk, ld = svd_lds_ld(len);
+  double* Vp = sm;
+  double* Cs = Vp + ld * kSvdQrPanel;
+  double* Ts = Cs + ld * CW;
+  double* Ws = Ts + 256;
+  double* W2 = Ws + 16 * CW;
+  double* red = W2 + 16 * CW;  // 4 partial sums, tau, scale, beta
+  double* tl = red + 16;       // the panel's tau values
+  double* x = X + b.x_off;
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  for (int p0 = 0; p0 < kk; p0 += kSvdQrPanel) {
+    const int nbp = min(kSvdQrPanel, kk - p0), K = len - p0;
+    svd_qr_gather(x + (int64_t)p0 * len + p0, len, K, nbp, kSvdQrPanel, ld, Vp, tid);
+    if (tid < 16) tl[tid] = 0.0;
+    __syncthreads();
+    for (int k = 0; k < nbp; ++k) {
+      double* col = Vp + k * ld;
+      double ss = 0.0;
+      for (int i = k + 1 + tid; i < K; i += 256) ss += col[i] * col[i];
+      ss = wave_sum(ss);
+      if (l == 0) red[w] = ss;
+      __syncthreads();
+      if (tid == 0) {
+        const double xn2 = red[0] + red[1] + red[2] + red[3], alpha = col[k];
+        double tk = 0.0, scale = 0.0, beta = alpha;
+        if (xn2 > 0.0) {
+          beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+          tk = (beta - alpha) / beta;
+          scale = 1.0 / (alpha - beta);
+        }
+        red[4] = tk;
+        red[5] = scale;
+        tl[k] = tk;
+        tau[b.tau_off + p0 + k] = tk;
+        col[k] = beta;
+      }
+      __syncthreads();
+      const double tk = red[4], scale = red[5];
+      for (int i = k + 1 + tid; i < K; i += 256) col[i] *= scale;
+      __syncthreads();
+      if (tk != 0.0) {  // H_k = I - tau v v^T on the panel's later columns, one wave per column
+        for (int j = k + 1 + w; j < nbp; j += 4) {
+          double* cj = Vp + j * ld;
+          double d = 0.0;
+          for (int i = k + 1 + l; i < K; i += 64) d += col[i] * cj[i];
+          d = wave_sum(d) + cj[k];
+          const double f = tk * d;
+          for (int i = k + 1 + l; i < K; i += 64) cj[i] -= f * col[i];
+          if (l == 0) cj[k] -= f;
+        }
+      }
+      __syncthreads();
+    }
+    // the factored panel back to X (R on and above the diagonal, v below), then V made explicit in LDS
+    for (int e = tid; e < nbp * K; e += 256) {
+      const int c = e / K, i = e - c * K;
+      x[(int64_t)(p0 + c) * len + p0 + i] = Vp[c * ld + i];
+    }
+    __syncthreads();
+    for (int e = tid; e < nbp * K; e += 256) {
+      const int c = e / K, i = e - c * K;
+      if (i <= c) Vp[c * ld + i] = (i == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // S = V^T V (one MFMA tile, wave 0) into Ws, then T column by column
+    if (w == 0) {
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      const int K4 = (K + 3) & ~3;
+      for (int k0 = 0; k0 < K4; k0 += 4) {
+        const double a = Vp[(l % 16) * ld + k0 + l / 16];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, a, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) Ws[(l / 16 + 4 * j) * 16 + l % 16] = acc[j];
+    }
+    __syncthreads();
+    for (int i = 0; i < 16; ++i) {
+      if (tid < 16) {
+        const int j = tid;
+        double tv = 0.0;
+        if (j < i) {
+          double acc = 0.0;
+          for (int m = j; m < i; ++m) acc += Ts[j * 16 + m] * Ws[m * 16 + i];
+          tv = -tl[i] * acc;
+        } else if (j == i) {
+          tv = tl[i];
+        }
+        Ts[j * 16 + i] = tv;
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < 256; e += 256) T[b.t_off + (int64_t)(p0 / kSvdQrPanel) * 256 + e] = Ts[e];
+    // trailing columns in chunks of CW: C <- (I - V T^T V^T) C
+    for (int c0 = p0 + nbp; c0 < kk; c0 += CW) {
+      const int cw = min(CW, kk - c0);
+      svd_qr_gather(x + (int64_t)c0 * len + p0, len, K, cw, CW, ld, Cs, tid);
+      __syncthreads();
+      svd_qr_block_reflect<CW>(Vp, Cs, Ts, Ws, W2, K, ld, true, tid);
+      for (int e = tid; e < cw * K; e += 256) {
+        const int c = e / K, i = e - c * K;
+        x[(int64_t)(c0 + c) * len + p0 + i] = Cs[c * ld + i];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void svd_rt_kernel(const SvdQrBlk* __restrict__ blks, const double* __restrict__ X,
+                                                     double* Y) {
+  const SvdQrBlk b = blks[blockIdx.x];
+  const int kk = b.kk;
+  for (int64_t e = threadIdx.x; e < (int64_t)kk * kk; e += 256) {
+    const int j = (int)(e / kk), i = (int)(e - (int64_t)j * kk);  // Y(i, j) = R(j, i), R(j, i) = X(row j, col i)
+    Y[b.y_off + e] = i >= j ? X[b.x_off + (int64_t)i * b.len + j] : 0.0;
+  }
+}
+
+template <int CW>
+__global__ __launch_bounds__(256) void svd_qr_apply_kernel(const SvdQrBlk* __restrict__ blks,
+                                                           const double* __restrict__ X, const double* __restrict__ T,
+                                                           const double* __restrict__ W, double* Z) {
+  extern __shared__ double sm[];
+  const SvdQrBlk b = blks[blockIdx.x];
+  const int len = b.len, kk = b.kk, ld = svd_lds_ld(len);
+  double* Vp = sm;
+  double* Cs = Vp + ld * kSvdQrPanel;
+  double* Ts = Cs + ld * CW;
+  double* Ws = Ts + 256;
+  double* W2 = Ws + 16 * CW;
+  double* z = Z + b.z_off;
+  const int tid = threadIdx.x;
+  for (int64_t e = tid; e < (int64_t)len * kk; e += 256) {  // Z = [W; 0]
+    const int j = (int)(e / len), i = (int)(e - (int64_t)j * len);
+    z[e] = i < kk ? W[b.w_off + (int64_t)j * kk + i] : 0.0;
+  }
+  __syncthreads();
+  const int np = (kk + kSvdQrPanel - 1) / kSvdQrPanel;
+  for (int pi = np - 1; pi >= 0; --pi) {
+    const int p0 = pi * kSvdQrPanel, nbp = min(kSvdQrPanel, kk - p0), K = len - p0;
+    for (int e = tid; e < kSvdQrPanel * ld; e += 256) {
+      const int c = e / ld, i = e - c * ld;
+      double v = 0.0;
+      if (c < nbp && i < K) v = i < c ? 0.0 : (i == c ? 1.0 : X[b.x_off + (int64_t)(p0 + c) * len + p0 + i]);
+      Vp[e] = v;
+    }
+    for (int e = tid; e < 256; e += 256) Ts[e] = T[b.t_off + (int64_t)pi * 256 + e];
+    __syncthreads();
+    for (int c0 = 0; c0 < kk; c0 += CW) {
+      const int cw = min(CW, kk - c0);
+      svd_qr_gather(z + (int64_t)c0 * len + p0, len, K, cw, CW, ld, Cs, tid);
+      __syncthreads();
+      svd_qr_block_reflect<CW>(Vp, Cs, Ts, Ws, W2, K, ld, false, tid);
+      for (int e = tid; e < cw * K; e += 256) {
+        const int c = e / K, i = e - c * K;
+        z[(int64_t)(c0 + c) * len + p0 + i] = Cs[c * ld + i];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+size_t svd_round_lds_limit() {
+  static size_t lim = 0;
+  if (!lim) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                                                  hipSuccess || v <= 0)
+      v = 65536;
+    // the round kernel's static LDS: G and Q (2 x 32 x 33 doubles) and the step's rotations
+    const size_t stat = 2 * 32 * 33 * sizeof(double) + 16 * (2 * sizeof(double) + 2 * sizeof(int)) + 64;
+    lim = (size_t)v - stat;
+    const void* fns[] = {(const void*)svd_round_kernel<8>,     (const void*)svd_round_kernel<16>,
+                         (const void*)svd_round_kernel<32>,    (const void*)svd_finish_kernel,
+                         (const void*)svd_qr_kernel<16>,       (const void*)svd_qr_kernel<32>,
+                         (const void*)svd_qr_apply_kernel<16>, (const void*)svd_qr_apply_kernel<32>};
+    for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+  }
+  return lim;
+}
+
+size_t svd_round_lds_bytes(int cols, int len_max) { return (size_t)cols * svd_lds_ld(len_max) * sizeof(double); }
+
+size_t svd_qr_lds_bytes(int cw, int len_max) { return svd_qr_lds_doubles(cw, svd_lds_ld(len_max)) * sizeof(double); }
+
+void launch_svd_qr(const SvdQrBlk* blks, int nb, double* X, double* tau, double* T, int cw, int len_max,
+                   hipStream_t s) {
+  if (nb <= 0) return;
+  const size_t lds = svd_qr_lds_bytes(cw, len_max);
+  if (cw == 32) hipLaunchKernelGGL(svd_qr_kernel<32>, dim3(nb), dim3(256), lds, s, blks, X, tau, T);
+  else hipLaunchKernelGGL(svd_qr_kernel<16>, dim3(nb), dim3(256), lds, s, blks, X, tau, T);
+}
+
+void launch_svd_rt(const SvdQrBlk* blks, int nb, const double* X, double* Y, hipStream_t s) {
+  if (nb > 0) hipLaunchKernelGGL(svd_rt_kernel, dim3(nb), dim3(256), 0, s, blks, X, Y);
+}
+
+void launch_svd_qr_apply(const SvdQrBlk* blks, int nb, const double* X, const double* T, const double* W, double* Z,
+                         int cw, int len_max, hipStream_t s) {
+  if (nb <= 0) return;
+  const size_t lds = svd_qr_lds_bytes(cw, len_max);
+  if (cw == 32) hipLaunchKernelGGL(svd_qr_apply_kernel<32>, dim3(nb), dim3(256), lds, s, blks, X, T, W, Z);
+  else hipLaunchKernelGGL(svd_qr_apply_kernel<16>, dim3(nb), dim3(256), lds, s, blks, X, T, W, Z);
+}
+
+void launch_svd_init(const SvdBlk* blks, int nb, double* V, hipStream_t s) {
+  if (nb > 0) hipLaunchKernelGGL(svd_init_kernel, dim3(nb), dim3(256), 0, s, blks, V);
+}
+
+void launch_svd_round(const SvdTask* tasks, int ntasks, const SvdBlk* blks, double* X, double* V,
+                      const int32_t* active, int32_t* rot, int cols, int len_max, hipStream_t s) {
+  if (ntasks <= 0) return;
+  const size_t lds = svd_round_lds_bytes(cols, len_max);
+  if (cols == 32)
+    hipLaunchKernelGGL(svd_round_kernel<32>, dim3(ntasks), dim3(256), lds, s, tasks, blks, X, V, active, rot);
+  else if (cols == 16)
+    hipLaunchKernelGGL(svd_round_kernel<16>, dim3(ntasks), dim3(256), lds, s, tasks, blks, X, V, active, rot);
+  else
+    hipLaunchKernelGGL(svd_round_kernel<8>, dim3(ntasks), dim3(256), lds, s, tasks, blks, X, V, active, rot);
+}
+
+void launch_svd_finish(const SvdOut* outs, int nb, const double* Xn, const double* Va, double* S, int kmax, double* Ua,
+                       int64_t su_a, double* Vta, int64_t svt_a, int nmax, int kk_max, hipStream_t s) {
+  if (nb <= 0) return;
+  const size_t lds = (size_t)kk_max * (sizeof(double) + sizeof(int));
+  hipLaunchKernelGGL(svd_finish_kernel, dim3(nb), dim3(256), lds, s, outs, Xn, Va, S, kmax, Ua, su_a, Vta, svt_a,
+                     nmax);
+}
+
 void launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, s, a, x, b, y, n);
 }

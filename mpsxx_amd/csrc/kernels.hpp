@@ -121,7 +121,6 @@ template <typename T, typename TC>
 void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, Operands ops,
                  double alpha, int accumulate, hipStream_t s, int max_groups = 0);
 
-// FP64 -> FP32 copies of many buffers in one launch (mixed-precision inputs of a batched launch); the chunks are
 // One environment-sized sub-block of a merged block operator: its placement in the merged output (row length ld), the
 // environment slice (bra, ket sector pair) it reads and the MPO column (site pair and outgoing operator) it sums over.
 struct MergedOpTask {
@@ -136,6 +135,7 @@ void launch_merged_operator(const Chunk* chunks, int nchunks, const MergedOpTask
                             int nw, const int64_t* env_off, const int32_t* col_begin, const int32_t* ent_w,
                             const int64_t* ent_off, const double* env, const double* mpo, double* out, hipStream_t s);
 
+// FP64 -> FP32 copies of many buffers in one launch (mixed-precision inputs of a batched launch); the chunks are
 // (segment, start) pairs of kAxpyChunk elements.
 struct ConvSeg {
   const double* src;
@@ -163,6 +163,60 @@ void launch_axpby(double a, const double* x, double b, double* y, int64_t n, hip
 // X[b*stride + i*cols + c] *= S[b*kmax + (by_col ? c : i)] for a batch of row-major (rows x cols) matrices
 void launch_svd_scale(double* X, int batch, int rows, int cols, int64_t stride, const double* S, int kmax, int by_col,
                       hipStream_t s);
+
+// Batched one-sided (Hestenes) Jacobi SVD of the quantum-number blocks of a merged wavefunction.  Block A (m x n,
+// row-major) is decomposed through its kk = min(m, n) vectors of length len = max(m, n): the columns of A when
+// n <= m, its rows otherwise.  X holds them column-major (len x kk) and is rotated in place until its columns are
+// mutually orthogonal; V (kk x kk, column-major, starts as the identity) accumulates the rotations, so X = A V
+// (or A^T V) holds U s (or V s) and V the other factor.
+struct SvdBlk {
+  int64_t x_off, v_off;
+  int32_t m, n, len, kk;
+};
+// One work-group of a round: the columns [c0, c0 + n0) and [c1, c1 + n1) of block `blk` (n1 = 0: one group) are
+// swept once through their Gram matrix, every pair of them (cross == 0) or only the n0 x n1 pairs across the two
+// groups (cross == 1, n0 >= n1); the tasks of one round touch disjoint columns.
+struct SvdTask {
+  int32_t blk, c0, n0, c1, n1, cross;
+};
+// Largest dynamic LDS (bytes) a work-group of svd_round may use on the current device, and what a round with tasks
+// of up to `cols` columns (8, 16 or 32) of length up to len_max needs.
+size_t svd_round_lds_limit();
+size_t svd_round_lds_bytes(int cols, int len_max);
+void launch_svd_init(const SvdBlk* blks, int nb, double* V, hipStream_t s);
+// cols = the most columns of any task (8, 16 or 32); len_max = the longest vector.
+// rot[blk] is increased by the number of rotations applied; blocks with active[blk] == 0 are skipped.
+void launch_svd_round(const SvdTask* tasks, int ntasks, const SvdBlk* blks, double* X, double* V,
+                      const int32_t* active, int32_t* rot, int cols, int len_max, hipStream_t s);
+// QR preconditioning of the Jacobi sweeps (blocked Householder, one work-group per block, panels of kSvdQrPanel
+// columns): X (len x kk) = Q R.  The sweeps then run on Y = R^T (kk x kk), whose columns are graded, and the left
+// vectors of X are Z = Q [W; 0] with W the sweeps' accumulated rotations.  In place: R above and the Householder
+// vectors below X's diagonal, tau (kk) and each panel's triangular factor T (kSvdQrPanel^2, block reflector
+// I - V T V^T) per block.
+constexpr int kSvdQrPanel = 16;
+struct SvdQrBlk {
+  int64_t x_off, y_off, w_off, z_off, tau_off, t_off;
+  int32_t len, kk;
+};
+// LDS bytes of the QR / Q-application kernels for trailing chunks of cw (16 or 32) columns of length up to len_max.
+size_t svd_qr_lds_bytes(int cw, int len_max);
+void launch_svd_qr(const SvdQrBlk* blks, int nb, double* X, double* tau, double* T, int cw, int len_max,
+                   hipStream_t s);
+// Y = R^T (zero above the diagonal) and Z = [W; 0] from W (kk x kk), then Z <- Q Z.
+void launch_svd_rt(const SvdQrBlk* blks, int nb, const double* X, double* Y, hipStream_t s);
+void launch_svd_qr_apply(const SvdQrBlk* blks, int nb, const double* X, const double* T, const double* W, double* Z,
+                         int cw, int len_max, hipStream_t s);
+
+// Output of a converged block: sigma_j = |xn_j| (xn: kk columns of length m when u_from_x, else n), sorted descending
+// into S[blk * kmax + r].  U_A (m x kmax row-major, slot stride su_a) and Vt_A (kmax x nmax row-major, slot stride
+// svt_a) are filled as a LAPACK-style gesvd of A returns them: one of them from xn / sigma (U_A when u_from_x), the
+// other from va (kk columns of length n when u_from_x, else m).  Columns of zero singular value are left zero.
+struct SvdOut {
+  int64_t xn_off, va_off;
+  int32_t m, n, kk, u_from_x;
+};
+void launch_svd_finish(const SvdOut* outs, int nb, const double* Xn, const double* Va, double* S, int kmax, double* Ua,
+                       int64_t su_a, double* Vta, int64_t svt_a, int nmax, int kk_max, hipStream_t s);
 
 // Davidson helpers on flat vectors of one layout (length n).
 // y = sum_i c[i] * X_i  with X_i = base + i*ld  (FP64; c lives in device memory)

@@ -2,14 +2,18 @@
 //
 // Reference: src/canonicalize.hpp:27-46 -> btas::Gesvd<.., LeftArrow|RightArrow> (include/btas/QSPARSE/QSTLAPACK.h:
 // 122-288).  psi(l,n1,n2,r) is merged into a matrix over (l,n1) x (n2,r) sectors (QSTmerge), every sector block is
-// decomposed (LAPACKE_dgesvd 'S','S' in the reference; here one rocSOLVER strided-batched gesvd over all blocks,
-// zero-padded to the largest block, which leaves the non-zero spectrum and its vectors unchanged), all singular
+// decomposed (LAPACKE_dgesvd 'S','S' in the reference; here a batched one-sided Jacobi SVD over all blocks at once,
+// or, for vectors longer than one work-group's LDS holds, one rocSOLVER strided-batched gesvd over all blocks
+// zero-padded to the largest, which leaves the non-zero spectrum and its vectors unchanged), all singular
 // values are sorted and cutoff = s[M-1] (M > 0 and fewer than the total, else 1e-16); every value >= cutoff is kept
 // (so ties at the cutoff keep more than M states), and the factors are expanded back (QSTexpand).  LeftArrow gives
 // U with total QN 0 and s*Vt carrying psi's QN (forward sweep); RightArrow gives U*s carrying psi's QN and Vt with
 // total 0 (backward sweep).  Blocks of the row (column) sectors keep the merged-sector order of QSTmergeInfo.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -153,17 +157,187 @@ double op_canonicalize_onesite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor&
 
 namespace {
 
-double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo& MR, const MergeInfo& MC,
-                         const std::vector<Blk>& blks, const std::vector<Rect>& rects, const double* src,
-                         int64_t src_elems, Tensor& lmps, Tensor& rmps, int M, hipStream_t s) {
+constexpr int kJacobiMaxSweeps = 60;
+
+// Batched one-sided Jacobi SVD of every block.  Each block's kk vectors X (columns or rows of A, whichever are
+// fewer) are first QR-factored, X = Q R (svd_qr_kernel), and the Jacobi sweeps (svd_round_kernel) orthogonalize the
+// columns of Y = R^T, which are graded, so the sweeps converge in a few passes: R^T W = U_R s, hence X = (Q W) s U_R^T
+// and Q W comes from applying the Householder reflectors to W (svd_qr_apply_kernel).  Blocks too long for the QR
+// kernels' LDS are swept directly (X V = U_X s).  Writes the descending singular values to Sv (kmax per block), U_A
+// row-major (mmax x kmax) to Ua and Vt_A row-major (kmax x nmax) to Vta, the slots LAPACK-style gesvd fills.  Returns
+// false, having written none of them, when a vector is longer than one work-group's LDS holds or the sweeps do not
+// converge.
+bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, const double* src, int kmax, int nmax,
+                int64_t su, int64_t sv, double* Sv, double* Ua, double* Vta, int* sweeps_out, hipStream_t s) {
   const int nb = (int)blks.size();
-  if (nb == 0) throw Error(MPSXX_ERR_ARG, "canonicalize: empty wavefunction");
-  int mmax = 1, nmax = 1;
-  for (const Blk& b : blks) { mmax = std::max(mmax, b.m); nmax = std::max(nmax, b.n); }
-  const int kmax = std::min(mmax, nmax);
-  const int64_t sa = (int64_t)mmax * nmax, su = (int64_t)nmax * kmax, sv = (int64_t)kmax * mmax;
-  DevBuf A(nb * sa * sizeof(double)), Sv(nb * (int64_t)kmax * sizeof(double)), U(nb * su * sizeof(double)),
-      V(nb * sv * sizeof(double)), E(nb * (int64_t)std::max(kmax, 2) * sizeof(double)), info(nb * sizeof(rocblas_int));
+  std::vector<SvdBlk> sb(nb);
+  int64_t xo = 0, vo = 0;
+  int len_max = 1, kk_max = 1;
+  for (int k = 0; k < nb; ++k) {
+    const int m = blks[k].m, n = blks[k].n;
+    sb[k] = {xo, vo, m, n, std::max(m, n), std::min(m, n)};
+    xo += (int64_t)sb[k].len * sb[k].kk;
+    vo += (int64_t)sb[k].kk * sb[k].kk;
+    len_max = std::max(len_max, sb[k].len);
+    kk_max = std::max(kk_max, sb[k].kk);
+  }
+  const size_t lim = svd_round_lds_limit();
+  const char* mode = std::getenv("MPSXX_SVD");
+  const bool allow_qr = !(mode && std::strcmp(mode, "jacobi_noqr") == 0);
+  const int qcw = !allow_qr ? 0 : svd_qr_lds_bytes(32, len_max) <= lim ? 32 : svd_qr_lds_bytes(16, len_max) <= lim ? 16 : 0;
+  const bool qr = qcw > 0;
+  const int jl_max = qr ? kk_max : len_max;  // length of the swept vectors
+  int cols = 32;
+  while (cols > 8 && svd_round_lds_bytes(cols, jl_max) > lim) cols /= 2;
+  if (svd_round_lds_bytes(cols, jl_max) > lim || (size_t)kk_max * (sizeof(double) + sizeof(int)) > lim) return false;
+  const int g = cols / 2;  // columns per group of a paired task
+
+  // X: the kk vectors of each block, column-major (zero where no psi block lies)
+  DevBuf X(std::max<int64_t>(xo, 1) * sizeof(double)), V(std::max<int64_t>(vo, 1) * sizeof(double));
+  hip_check(hipMemsetAsync(X.p, 0, std::max<int64_t>(xo, 1) * sizeof(double), s), "memset");
+  {
+    std::vector<CopySeg> segs;
+    for (const Rect& r : rects) {
+      const SvdBlk& b = sb[r.k];
+      if (b.n <= b.m) {  // columns of A: column co + c, rows ro ..
+        for (int64_t c = 0; c < r.C; ++c)
+          segs.push_back({b.x_off + (r.co + c) * b.m + r.ro, r.src + c, (int32_t)r.R, (int32_t)r.ld});
+      } else {  // rows of A
+        for (int64_t rr = 0; rr < r.R; ++rr)
+          segs.push_back({b.x_off + (r.ro + rr) * b.n + r.co, r.src + rr * r.ld, (int32_t)r.C, 1});
+      }
+    }
+    run_copy(segs, src, X.as<double>(), s);
+  }
+  // the swept matrices: Y = R^T (kk x kk) with QR, else X itself; V accumulates the rotations
+  std::vector<SvdBlk> jb = sb;
+  std::vector<SvdQrBlk> qb;
+  DevBuf Y, Z, tau, T, dqb;
+  if (qr) {
+    int64_t yo = 0, zo = 0, to = 0, po = 0;
+    for (int k = 0; k < nb; ++k) {
+      const SvdBlk& b = sb[k];
+      qb.push_back({b.x_off, yo, b.v_off, zo, to, po, b.len, b.kk});
+      jb[k].x_off = yo;
+      jb[k].len = b.kk;
+      yo += (int64_t)b.kk * b.kk;
+      zo += (int64_t)b.len * b.kk;
+      to += b.kk;
+      po += (int64_t)((b.kk + kSvdQrPanel - 1) / kSvdQrPanel) * kSvdQrPanel * kSvdQrPanel;
+    }
+    Y = DevBuf(std::max<int64_t>(yo, 1) * sizeof(double));
+    Z = DevBuf(std::max<int64_t>(zo, 1) * sizeof(double));
+    tau = DevBuf(std::max<int64_t>(to, 1) * sizeof(double));
+    T = DevBuf(std::max<int64_t>(po, 1) * sizeof(double));
+    dqb = upload_vec(qb);
+    launch_svd_qr(dqb.as<SvdQrBlk>(), nb, X.as<double>(), tau.as<double>(), T.as<double>(), qcw, len_max, s);
+    launch_svd_rt(dqb.as<SvdQrBlk>(), nb, X.as<double>(), Y.as<double>(), s);
+    hip_check(hipGetLastError(), "svd qr launch");
+  }
+  double* Xj = qr ? Y.as<double>() : X.as<double>();
+  DevBuf dblk = upload_vec(jb);
+  launch_svd_init(dblk.as<SvdBlk>(), nb, V.as<double>(), s);
+
+  // tasks of one sweep, round by round: blocks of at most `cols` vectors are one task of round 0 (all their pairs);
+  // larger ones are cut into groups of g columns paired by the circle method, one task per pair of groups sweeping
+  // all pairs of its 2g columns (the inner pairs of a group are swept again with each partner, which converges in
+  // fewer sweeps than sweeping only the cross pairs, MPSXX_SVD_ORDER=cross, with the inner pairs in a round of their
+  // own)
+  const char* order = std::getenv("MPSXX_SVD_ORDER");
+  const bool cross = order && std::strcmp(order, "cross") == 0;
+  std::vector<std::vector<SvdTask>> rounds(1);
+  for (int k = 0; k < nb; ++k) {
+    const int kk = jb[k].kk;
+    if (kk < 2) continue;
+    if (kk <= cols) {
+      rounds[0].push_back({k, 0, kk, 0, 0, 0});
+      continue;
+    }
+    const int ng = (kk + g - 1) / g, P = ng + (ng & 1), r0 = cross ? 1 : 0;
+    if ((int)rounds.size() < P - 1 + r0) rounds.resize(P - 1 + r0);
+    auto gsize = [&](int x) { return std::min(g, kk - x * g); };
+    if (cross)
+      for (int x = 0; x < ng; ++x)
+        if (gsize(x) > 1) rounds[0].push_back({k, x * g, gsize(x), 0, 0, 0});
+    for (int r = 0; r < P - 1; ++r)
+      for (int w = 0; w < P / 2; ++w) {
+        int p = w == 0 ? P - 1 : (r + w) % (P - 1), q = w == 0 ? r : (r - w + P - 1) % (P - 1);
+        if (p >= ng || q >= ng) continue;
+        if (gsize(p) < gsize(q)) std::swap(p, q);  // the larger group first
+        rounds[r + r0].push_back({k, p * g, gsize(p), q * g, gsize(q), cross ? 1 : 0});
+      }
+  }
+  std::vector<SvdTask> flat;
+  std::vector<int> roff{0};
+  for (const auto& rt : rounds) {
+    for (const SvdTask& t : rt) {
+      const SvdBlk& b = jb[t.blk];
+      if (t.n0 + t.n1 > cols || t.c0 + t.n0 > b.kk || t.c1 + t.n1 > b.kk || t.n0 < 1 || t.n1 < 0 ||
+          (t.cross && (t.n1 < 1 || t.n1 > t.n0)))
+        throw Error(MPSXX_ERR_ARG, "svd: bad Jacobi task");
+      flat.push_back(t);
+    }
+    roff.push_back((int)flat.size());
+  }
+  DevBuf dtask = upload_vec(flat);
+  std::vector<int32_t> act(nb, 1), cnt(nb, 0);
+  DevBuf dact = upload_vec(act), drot = upload_vec(cnt);
+  bool converged = false;
+  int sweep = 0;
+  while (sweep < kJacobiMaxSweeps) {
+    ++sweep;
+    for (size_t r = 0; r + 1 < roff.size(); ++r)
+      launch_svd_round(dtask.as<SvdTask>() + roff[r], roff[r + 1] - roff[r], dblk.as<SvdBlk>(), Xj, V.as<double>(),
+                       dact.as<int32_t>(), drot.as<int32_t>(), cols, jl_max, s);
+    hip_check(hipGetLastError(), "svd round launch");
+    hip_check(hipMemcpyAsync(cnt.data(), drot.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s), "svd count");
+    hip_check(hipStreamSynchronize(s), "svd sweep");
+    bool any = false;
+    for (int k = 0; k < nb; ++k) {
+      act[k] = cnt[k] > 0 ? 1 : 0;
+      any = any || act[k];
+    }
+    if (!any) {
+      converged = true;
+      break;
+    }
+    hip_check(hipMemcpy(dact.p, act.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice), "svd active");
+    hip_check(hipMemsetAsync(drot.p, 0, nb * sizeof(int32_t), s), "memset");
+  }
+  if (sweeps_out) *sweeps_out = sweep;
+  if (std::getenv("MPSXX_SVD_TRACE")) {  // per-call statistics of the batched Jacobi SVD on stderr
+    int64_t work = 0;
+    for (const SvdBlk& b : jb) work += (int64_t)b.len * b.kk * b.kk;
+    fprintf(stderr, "svdtrace nb %d len_max %d kk_max %d qr %d cols %d rounds %d tasks %d sweeps %d conv %d work %lld\n",
+            nb, len_max, kk_max, qcw, cols, (int)roff.size() - 1, (int)flat.size(), sweep, (int)converged,
+            (long long)work);
+  }
+  if (!converged) return false;
+  std::vector<SvdOut> outs(nb);
+  if (qr) {
+    launch_svd_qr_apply(dqb.as<SvdQrBlk>(), nb, X.as<double>(), T.as<double>(), V.as<double>(), Z.as<double>(), qcw,
+                        len_max, s);
+    for (int k = 0; k < nb; ++k) outs[k] = {qb[k].y_off, qb[k].z_off, sb[k].m, sb[k].n, sb[k].kk, sb[k].n > sb[k].m};
+  } else {
+    for (int k = 0; k < nb; ++k) outs[k] = {sb[k].x_off, sb[k].v_off, sb[k].m, sb[k].n, sb[k].kk, sb[k].n <= sb[k].m};
+  }
+  DevBuf douts = upload_vec(outs);
+  hip_check(hipMemsetAsync(Ua, 0, nb * sv * sizeof(double), s), "memset");
+  hip_check(hipMemsetAsync(Vta, 0, nb * su * sizeof(double), s), "memset");
+  launch_svd_finish(douts.as<SvdOut>(), nb, Xj, qr ? Z.as<double>() : V.as<double>(), Sv, kmax, Ua, sv, Vta, su, nmax,
+                    kk_max, s);
+  hip_check(hipGetLastError(), "svd finish launch");
+  hip_check(hipStreamSynchronize(s), "svd finish");
+  return true;
+}
+
+// The rocSOLVER path: every block zero-padded to (mmax x nmax), one strided-batched gesvd (QR iteration) over all.
+void rocsolver_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, const double* src, int64_t src_elems,
+                   int mmax, int nmax, int kmax, int64_t sa, int64_t su, int64_t sv, DevBuf& Sv, DevBuf& U, DevBuf& V,
+                   std::vector<double>& sv_h, hipStream_t s) {
+  const int nb = (int)blks.size();
+  DevBuf A(nb * sa * sizeof(double)), E(nb * (int64_t)std::max(kmax, 2) * sizeof(double)),
+      info(nb * sizeof(rocblas_int));
   hip_check(hipMemsetAsync(A.p, 0, nb * sa * sizeof(double), s), "memset");
   {
     // each source rectangle into the zero-padded row-major (mmax x nmax) slot of its block
@@ -183,7 +357,6 @@ double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo&
       U.as<double>(), nmax, su, V.as<double>(), kmax, sv, E.as<double>(), std::max(kmax, 2), rocblas_outofplace,
       info.as<rocblas_int>(), nb);
   if (st != rocblas_status_success) throw Error(MPSXX_ERR_HIP, "rocsolver_dgesvd_strided_batched failed");
-  std::vector<double> sv_h((size_t)nb * kmax);
   std::vector<rocblas_int> info_h(nb);
   hip_check(hipMemcpyAsync(sv_h.data(), Sv.p, sv_h.size() * sizeof(double), hipMemcpyDeviceToHost, s), "S");
   hip_check(hipMemcpyAsync(info_h.data(), info.p, nb * sizeof(rocblas_int), hipMemcpyDeviceToHost, s), "info");
@@ -222,6 +395,33 @@ double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo&
                                      std::to_string(info_h[k]) + "; input has " + std::to_string(bad) +
                                      " non-finite values, max |x| " + std::to_string(amax) + ")");
     }
+}
+
+double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo& MR, const MergeInfo& MC,
+                         const std::vector<Blk>& blks, const std::vector<Rect>& rects, const double* src,
+                         int64_t src_elems, Tensor& lmps, Tensor& rmps, int M, hipStream_t s) {
+  const int nb = (int)blks.size();
+  if (nb == 0) throw Error(MPSXX_ERR_ARG, "canonicalize: empty wavefunction");
+  int mmax = 1, nmax = 1;
+  for (const Blk& b : blks) { mmax = std::max(mmax, b.m); nmax = std::max(nmax, b.n); }
+  const int kmax = std::min(mmax, nmax);
+  const int64_t sa = (int64_t)mmax * nmax, su = (int64_t)nmax * kmax, sv = (int64_t)kmax * mmax;
+  DevBuf Sv(nb * (int64_t)kmax * sizeof(double)), U(nb * su * sizeof(double)), V(nb * sv * sizeof(double));
+  std::vector<double> sv_h((size_t)nb * kmax);
+  const char* mode = std::getenv("MPSXX_SVD");
+  const bool use_jacobi = !(mode && std::strcmp(mode, "rocsolver") == 0);
+  if (use_jacobi && jacobi_svd(blks, rects, src, kmax, nmax, su, sv, Sv.as<double>(), V.as<double>(), U.as<double>(),
+                               nullptr, s)) {
+    hip_check(hipMemcpyAsync(sv_h.data(), Sv.p, sv_h.size() * sizeof(double), hipMemcpyDeviceToHost, s), "S");
+    hip_check(hipStreamSynchronize(s), "svd sync");
+    for (int k = 0; k < nb; ++k)
+      for (int r = 0; r < std::min(blks[k].m, blks[k].n); ++r)
+        if (!std::isfinite(sv_h[(size_t)k * kmax + r]))
+          throw Error(MPSXX_ERR_HIP, "svd: non-finite singular value in block " + std::to_string(k) +
+                                         " (the wavefunction has non-finite entries)");
+  } else {
+    rocsolver_svd(blks, rects, src, src_elems, mmax, nmax, kmax, sa, su, sv, Sv, U, V, sv_h, s);
+  }
   // truncation (QSTLAPACK.h:190-232)
   std::vector<double> all;
   for (int k = 0; k < nb; ++k)

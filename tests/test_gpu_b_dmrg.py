@@ -40,6 +40,39 @@ def test_canonicalize_matches_oracle(forward, M):
     assert rec.max_abs_diff(recr) < 1e-11 * scale
 
 
+@pytest.mark.parametrize("solver", ["jacobi", "jacobi_noqr", "rocsolver"])
+@pytest.mark.parametrize("forward,M", [(True, 0), (False, 0), (True, 150), (False, 150)])
+def test_canonicalize_large_blocks_matches_oracle(forward, M, solver, monkeypatch):
+    """Merged blocks of up to ~150 vectors (several column groups per Jacobi task, both orientations m < n and
+    m > n, a graded spectrum from a rescaled psi) against the oracle's per-block SVD, for the default batched Jacobi
+    path (QR-preconditioned), the Jacobi sweeps without the QR step (MPSXX_SVD=jacobi_noqr, the path of blocks too
+    long for the QR kernels) and the rocSOLVER path (MPSXX_SVD=rocsolver)."""
+    print("canonicalize large", forward, M, solver, flush=True)
+    monkeypatch.setenv("MPSXX_SVD", solver)
+    rng = np.random.default_rng(11)
+    ch = mps.random_chain(8, 3, (8, 0), rng, dmax=40)
+    psi = ch["psi"]
+    # grade the spectrum: scale the entries of each block by a decaying factor of its row position
+    for k, b in psi.blocks.items():
+        psi.blocks[k] = b * np.exp(-0.05 * np.arange(b.shape[0]))[:, None, None, None]
+    U, V, dw = _canon(forward, psi, M)
+    Ur, Vr, dwr = oracle.canonicalize_twosite(forward, psi, M)
+    assert U.qshape == Ur.qshape and U.dshape == Ur.dshape and U.block_set() == Ur.block_set()
+    assert V.qshape == Vr.qshape and V.dshape == Vr.dshape and V.block_set() == Vr.block_set()
+    assert abs(dw - dwr) < 1e-12 * max(1.0, psi.norm() ** 2)
+    rec = oracle.contract(1.0, U, [2], V, [0])
+    recr = oracle.contract(1.0, Ur, [2], Vr, [0])
+    scale = max(float(np.max(np.abs(b))) for b in psi.blocks.values())
+    assert rec.max_abs_diff(recr) < 1e-11 * scale
+    # the orthonormal factor: U^T U = 1 (forward) or V V^T = 1 (backward) on every kept sector
+    if forward:
+        g = oracle.contract(1.0, U, [0, 1], U, [0, 1], conj_a=True)
+    else:
+        g = oracle.contract(1.0, V, [1, 2], V, [1, 2], conj_b=True)
+    for (i, j), blk in g.blocks.items():
+        assert np.max(np.abs(blk - (np.eye(blk.shape[0]) if i == j else 0.0))) < 1e-12
+
+
 def test_merged_operator_matches_oracle():
     rng = np.random.default_rng(6)
     L, i = 8, 3
