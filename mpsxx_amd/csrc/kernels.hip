@@ -1050,15 +1050,15 @@ __global__ void svd_init_kernel(const SvdBlk* __restrict__ blks, double* V) {
 __host__ __device__ __forceinline__ int svd_lds_ld(int len) { return ((len + 15) & ~15) + 4; }
 
 // Columns gcol(0..nc-1) of a column-major matrix (column length len) into LDS columns of stride ld; rows len..ld-1
-// and columns nc..cols-1 are zeroed.  Eight independent loads per thread are in flight before the LDS writes.
+// and columns nc..cols-1 are zeroed.  Up to 32 independent loads per thread are in flight before the LDS writes.
 template <typename F>
 __device__ __forceinline__ void svd_gather_columns(const double* __restrict__ src, int len, int nc, int cols, int ld,
                                                    F gcol, double* dst, int tid, int nt) {
   const int total = nc * len;
-  for (int base = tid; base < total; base += 8 * nt) {
-    double tmp[8];
+  for (int base = tid; base < total; base += 32 * nt) {
+    double tmp[32];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < 32; ++j) {
       const int e = base + j * nt;
       if (e < total) {
         const int c = e / len;
@@ -1066,7 +1066,7 @@ __device__ __forceinline__ void svd_gather_columns(const double* __restrict__ sr
       }
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < 32; ++j) {
       const int e = base + j * nt;
       if (e < total) {
         const int c = e / len;
@@ -1089,19 +1089,53 @@ __device__ __forceinline__ void svd_apply_q(const double* ys, int ld, int rows_o
   const int nrt = (rows_out + 15) / 16;
   for (int tile = w; tile < nrt * NCT; tile += 4) {
     const int row0 = (tile / NCT) * 16, col0 = (tile % NCT) * 16;
-    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    double a[COLS / 4], bb[COLS / 4];
 #pragma unroll
-    for (int k0 = 0; k0 < COLS; k0 += 4) {
-      const int k = k0 + l / 16, ci = col0 + l % 16;
-      const double a = ci < COLS ? Qs[k][ci] : 0.0;     // A = Q^T: A[i][k] = Q[k][col0 + i]
-      const double bb = ys[k * ld + row0 + l % 16];       // B = Y^T: B[k][j] = Y[row0 + j][k]
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+    for (int u = 0; u < COLS / 4; ++u) {
+      const int k = 4 * u + l / 16, ci = col0 + l % 16;
+      a[u] = ci < COLS ? Qs[k][ci] : 0.0;   // A = Q^T: A[i][k] = Q[k][col0 + i]
+      bb[u] = ys[k * ld + row0 + l % 16];    // B = Y^T: B[k][j] = Y[row0 + j][k]
     }
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < COLS / 4; u += 2) {
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bb[u], acc, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u + 1], bb[u + 1], acc2, 0, 0, 0);
+    }
+    acc += acc2;
     const int r = row0 + l % 16;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int c = col0 + l / 16 + 4 * j;  // D[l/16 + 4j][l%16] = (Y Q)[row0 + l%16][col0 + l/16 + 4j]
       if (c < nc && r < rows_out) out[(int64_t)gcol(c) * col_stride + r] = acc[j];
+    }
+  }
+}
+
+// 1/sqrt(x) and 1/x (x > 0, normal): the hardware estimates refined by one Newton step, y <- y (3 - x y^2) / 2 and
+// y <- y (2 - x y), which doubles their ~2^-29 relative accuracy to the double-precision level.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return y * fma(-0.5 * x * y, y, 1.5);
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return fma(y, fma(-x, y, 1.0), y);
+}
+
+// Rutishauser's rotation of a pair with a = |y_p|^2, b = |y_q|^2, g = y_p.y_q (identity unless real and
+// |g| > tol sqrt(a b)): t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), zeta = (b - a) / 2g, written as
+// 2g sign(b - a) / (|b - a| + sqrt((b - a)^2 + 4g^2)); c = 1 / sqrt(1 + t^2), s = c t.
+__device__ __forceinline__ void svd_rotation(double a, double b, double g, bool real, double tol2, double& cs,
+                                             double& sn) {
+  cs = 1.0;
+  sn = 0.0;
+  if (real && g != 0.0 && g * g > tol2 * (a * b)) {
+    const double d2 = b - a, q2 = d2 * d2 + 4.0 * g * g;
+    if (q2 > 0.0) {
+      const double tt = copysign(1.0, d2) * (2.0 * g) * rcp_nr(fabs(d2) + q2 * rsqrt_nr(q2));
+      cs = rsqrt_nr(1.0 + tt * tt);
+      sn = cs * tt;
     }
   }
 }
@@ -1135,9 +1169,7 @@ __global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restric
                                                         const SvdBlk* __restrict__ blks, double* X, double* V,
                                                         const int32_t* __restrict__ active, int32_t* rot) {
   extern __shared__ double ys[];  // COLS columns of stride ld
-  __shared__ double Gs[32][33], Qs[32][33];
-  __shared__ double rcs[16], rsn[16];
-  __shared__ int rp[16], rq[16];
+  __shared__ double Gs[32][33], Gs2[32][33], Qs[32][33];
   __shared__ int scnt;
   const SvdTask t = tasks[blockIdx.x];
   if (!active[t.blk]) return;
@@ -1167,14 +1199,25 @@ __global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restric
   {
     constexpr int NT = (COLS + 15) / 16;
     if (w < NT * NT) {
-      const int c0 = (w / NT) * 16, c1 = (w % NT) * 16, lenr = (len + 3) & ~3;
-      dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+      const int c0 = (w / NT) * 16, c1 = (w % NT) * 16, len16 = (len + 15) & ~15;
+      dbl4 acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
       const int ca = c0 + l % 16, cb = c1 + l % 16;
-      for (int k0 = 0; k0 < lenr; k0 += 4) {
-        const double a = ca < COLS ? ys[ca * ld + k0 + l / 16] : 0.0;
-        const double bb = cb < COLS ? ys[cb * ld + k0 + l / 16] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+      const double* ya = ys + (ca < COLS ? ca : 0) * ld + l / 16;
+      const double* yb = ys + (cb < COLS ? cb : 0) * ld + l / 16;
+      const double ma = ca < COLS ? 1.0 : 0.0, mb = cb < COLS ? 1.0 : 0.0;
+      for (int k0 = 0; k0 < len16; k0 += 16) {  // rows len .. len16 - 1 are zero in LDS
+        double a[4], bb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          a[u] = ma * ya[k0 + 4 * u];
+          bb[u] = mb * yb[k0 + 4 * u];
+        }
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], bb[0], acc, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[1], bb[1], acc2, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2], bb[2], acc, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[3], bb[3], acc2, 0, 0, 0);
       }
+      acc += acc2;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int r = c0 + l / 16 + 4 * j, c = c1 + l % 16;
@@ -1186,59 +1229,60 @@ __global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restric
   const double tol = 2.220446049250313e-16 * (double)max(len, 32), tol2 = tol * tol;
   const bool cross = t.cross != 0;
   const int S = nc + (nc & 1), NP = S / 2, nsteps = cross ? t.n0 : S - 1;
+  // thread (i, j) of a step owns the 2x2 block of G at the rows of pair i and the columns of pair j and two rows of Q's
+  // columns of pair j.  It computes both pairs' rotations itself from G as it stood before the step (read from one
+  // buffer, its block written to the other, so one barrier per step).  Pair indices are kept in registers and
+  // advanced in place: the circle method moves every slot but the fixed one (P - 1) by one place per step.
+  const int bi = tid / max(NP, 1), bj = tid - bi * max(NP, 1);
+  const bool act = tid < NP * NP;
+  int pi = 0, qi = 0, pj = 0, qj = 0;
+  bool ri = false, rj = false;
+  if (act) {
+    svd_pair(t.n0, t.n1, cross, 0, bi, pi, qi, ri);
+    svd_pair(t.n0, t.n1, cross, 0, bj, pj, qj, rj);
+  }
+  double (*Gc)[33] = Gs;
+  double (*Gn)[33] = Gs2;
   int myrot = 0;
   for (int st = 0; st < nsteps; ++st) {
-    if (tid < NP) {
-      int p, q;
-      bool real;
-      svd_pair(t.n0, t.n1, cross, st, tid, p, q, real);
-      double cs = 1.0, sn = 0.0;
-      if (real) {
-        const double sa = Gs[p][p], sb = Gs[q][q], sg = Gs[p][q];
-        // rotate when |g| > tol sqrt(a b); t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)) with zeta = (b - a) / 2g,
-        // written as 2g sign(b - a) / (|b - a| + sqrt((b - a)^2 + 4g^2))
-        if (sg != 0.0 && sg * sg > tol2 * (sa * sb)) {
-          const double d2 = sb - sa, den = fabs(d2) + sqrt(d2 * d2 + 4.0 * sg * sg);
-          if (den > 0.0) {
-            const double tt = copysign(1.0, d2) * (2.0 * sg) / den;
-            cs = 1.0 / sqrt(1.0 + tt * tt);
-            sn = cs * tt;
-            ++myrot;
-          }
-        }
-      }
-      rp[tid] = p;
-      rq[tid] = q;
-      rcs[tid] = cs;
-      rsn[tid] = sn;
-    }
-    __syncthreads();
-    if (tid < NP * NP) {
-      const int i = tid / NP, j = tid % NP;
-      const double ci = rcs[i], si = rsn[i], cj = rcs[j], sj = rsn[j];
-      if (si != 0.0 || sj != 0.0) {
-        const int pi = rp[i], qi = rq[i], pj = rp[j], qj = rq[j];
-        // G block (rows p_i, q_i; cols p_j, q_j) <- J_i^T B J_j, J = [[c, s], [-s, c]]
-        const double b00 = Gs[pi][pj], b01 = Gs[pi][qj], b10 = Gs[qi][pj], b11 = Gs[qi][qj];
-        const double t00 = cj * b00 - sj * b01, t01 = sj * b00 + cj * b01;
-        const double t10 = cj * b10 - sj * b11, t11 = sj * b10 + cj * b11;
-        Gs[pi][pj] = ci * t00 - si * t10;
-        Gs[pi][qj] = ci * t01 - si * t11;
-        Gs[qi][pj] = si * t00 + ci * t10;
-        Gs[qi][qj] = si * t01 + ci * t11;
-      }
-      if (sj != 0.0) {
-        // Q rows 2i, 2i+1, cols p_j, q_j <- B J_j
-        const int pj = rp[j], qj = rq[j];
+    if (act) {
+      const double ai = Gc[pi][pi], bi2 = Gc[qi][qi], gi = Gc[pi][qi];
+      const double aj = Gc[pj][pj], bj2 = Gc[qj][qj], gj = Gc[pj][qj];
+      const double b00 = Gc[pi][pj], b01 = Gc[pi][qj], b10 = Gc[qi][pj], b11 = Gc[qi][qj];
+      double ci, si, cj, sj;
+      svd_rotation(ai, bi2, gi, ri, tol2, ci, si);
+      svd_rotation(aj, bj2, gj, rj, tol2, cj, sj);
+      if (bi == bj && si != 0.0) ++myrot;
+      // G block (rows p_i, q_i; cols p_j, q_j) <- J_i^T B J_j, J = [[c, s], [-s, c]]
+      const double t00 = cj * b00 - sj * b01, t01 = sj * b00 + cj * b01;
+      const double t10 = cj * b10 - sj * b11, t11 = sj * b10 + cj * b11;
+      Gn[pi][pj] = ci * t00 - si * t10;
+      Gn[pi][qj] = ci * t01 - si * t11;
+      Gn[qi][pj] = si * t00 + ci * t10;
+      Gn[qi][qj] = si * t01 + ci * t11;
+      if (sj != 0.0) {  // Q rows 2i, 2i+1, cols p_j, q_j <- B J_j (each Q element has one owner per step)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          const double u = Qs[2 * i + h][pj], v = Qs[2 * i + h][qj];
-          Qs[2 * i + h][pj] = cj * u - sj * v;
-          Qs[2 * i + h][qj] = sj * u + cj * v;
+          const double u = Qs[2 * bi + h][pj], v = Qs[2 * bi + h][qj];
+          Qs[2 * bi + h][pj] = cj * u - sj * v;
+          Qs[2 * bi + h][qj] = sj * u + cj * v;
         }
+      }
+      if (cross) {
+        svd_pair(t.n0, t.n1, cross, st + 1, bi, pi, qi, ri);
+        svd_pair(t.n0, t.n1, cross, st + 1, bj, pj, qj, rj);
+      } else {
+        const int P1 = S - 1;
+        if (bi != 0) pi = pi + 1 == P1 ? 0 : pi + 1;
+        qi = qi + 1 == P1 ? 0 : qi + 1;
+        if (bj != 0) pj = pj + 1 == P1 ? 0 : pj + 1;
+        qj = qj + 1 == P1 ? 0 : qj + 1;
       }
     }
     __syncthreads();
+    double (*sw)[33] = Gc;
+    Gc = Gn;
+    Gn = sw;
   }
   if (myrot) atomicAdd(&scnt, myrot);
   __syncthreads();
@@ -1571,8 +1615,8 @@ size_t svd_round_lds_limit() {
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
                                                   hipSuccess || v <= 0)
       v = 65536;
-    // the round kernel's static LDS: G and Q (2 x 32 x 33 doubles) and the step's rotations
-    const size_t stat = 2 * 32 * 33 * sizeof(double) + 16 * (2 * sizeof(double) + 2 * sizeof(int)) + 64;
+    // the round kernel's static LDS: two G buffers and Q (3 x 32 x 33 doubles) and a counter
+    const size_t stat = 3 * 32 * 33 * sizeof(double) + 64;
     lim = (size_t)v - stat;
     const void* fns[] = {(const void*)svd_round_kernel<8>,     (const void*)svd_round_kernel<16>,
                          (const void*)svd_round_kernel<32>,    (const void*)svd_finish_kernel,
