@@ -125,10 +125,13 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   const int nslot = 4 * max_ritz + 1;
   DevBuf V((size_t)nslot * n * sizeof(double));
   double* base = V.as<double>();
-  auto T = [&](int i) { return base + (int64_t)i * n; };
-  auto Sg = [&](int i) { return base + (int64_t)(max_ritz + i) * n; };
-  double* T2 = base + (int64_t)(2 * max_ritz) * n;
-  double* S2 = base + (int64_t)(3 * max_ritz) * n;
+  // trial and sigma sets and their rotation targets; the Ritz rotation writes the targets and swaps the roles
+  double* tb = base;
+  double* sb = base + (int64_t)max_ritz * n;
+  double* t2b = base + (int64_t)(2 * max_ritz) * n;
+  double* s2b = base + (int64_t)(3 * max_ritz) * n;
+  auto T = [&](int i) { return tb + (int64_t)i * n; };
+  auto Sg = [&](int i) { return sb + (int64_t)i * n; };
   double* errv = base + (int64_t)(4 * max_ritz) * n;
   hip_check(hipMemsetAsync(base, 0, (size_t)nslot * n * sizeof(double), s), "memset");
 
@@ -190,6 +193,12 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
     return h;
   };
   auto normalize = [&](double* x) { launch_scale(x, n, 1.0 / std::sqrt(dot(x, x)), s); };
+  // the same on the device, with no host round trip (the norm stays in gram[kNrm])
+  const int kNrm = 2 * max_ritz * max_ritz - 1, kProj = max_ritz * max_ritz;
+  auto dnormalize = [&](double* x) {
+    launch_gram(x, n, 1, x, n, 1, n, gram.as<double>() + kNrm, part.as<double>(), npart, s);
+    launch_scale_rnorm(x, n, gram.as<double>() + kNrm, s);
+  };
   auto axpy = [&](double a, const double* x, double* y) { launch_axpby(a, x, 1.0, y, n, s); };
 
   std::vector<Plan*> plans;
@@ -237,7 +246,7 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
     convert_mask = ~0u;
     int niter = 0, iconv = 0;
     while (iconv < 1 && niter < 4) {
-    normalize(T(0));
+    dnormalize(T(0));
     multH(T(0), Sg(0));
     for (int m = 1; m <= max_ritz; ++m) {
       // heff(i,j) = <trial_i|sigma_j>  (diagonalize.hpp:339-365; the overlap matrix is computed but unused there)
@@ -257,29 +266,31 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
       hip_check(hipMemcpyAsync(coef.as<double>(), coef_h.data(), max_ritz * max_ritz * sizeof(double),
                                hipMemcpyHostToDevice, s),
                 "coef");
-      // stage the rotated vectors in T2/S2, then copy back
-      for (int j = 0; j < m; ++j) {
-        launch_combine(T(0), n, m, coef.as<double>() + j * max_ritz, T2 + (int64_t)j * n, n, s);
-        launch_combine(Sg(0), n, m, coef.as<double>() + j * max_ritz, S2 + (int64_t)j * n, n, s);
-      }
-      hip_check(hipMemcpyAsync(T(0), T2, (size_t)m * n * sizeof(double), hipMemcpyDeviceToDevice, s), "rotate");
-      hip_check(hipMemcpyAsync(Sg(0), S2, (size_t)m * n * sizeof(double), hipMemcpyDeviceToDevice, s), "rotate");
+      // both sets rotated in one pass into the other buffers, which then take over
+      launch_rotate2(T(0), Sg(0), n, m, coef.as<double>(), max_ritz, t2b, s2b, n, s);
+      std::swap(tb, t2b);
+      std::swap(sb, s2b);
       // error vector r = sigma_0 - E trial_0 and ||r||^2 < tol (diagonalize.hpp:416-422)
-      hip_check(hipMemcpyAsync(errv, Sg(0), n * sizeof(double), hipMemcpyDeviceToDevice, s), "errv");
-      axpy(-eval, T(0), errv);
-      const double rnorm = dot(errv, errv);
+      double rnorm = 0.0;
+      launch_residual(Sg(0), T(0), eval, errv, n, gram.as<double>() + kNrm, part.as<double>(), npart, s);
+      hip_check(hipMemcpyAsync(&rnorm, gram.as<double>() + kNrm, sizeof(double), hipMemcpyDeviceToHost, s), "rnorm");
+      hip_check(hipStreamSynchronize(s), "rnorm sync");
       st.last_rnorm = rnorm;
       if (rnorm < tol_p) {
         ++iconv;
         break;
       }
       if (m < max_ritz) {
+        // preconditioned residual orthogonalized against the trials and normalized (diagonalize.hpp:427-431
+        // normalizes, then projects out one trial at a time).  Here: classical Gram-Schmidt applied twice, all m
+        // projections of a pass in one Gram launch and one update, on the device -- the same direction up to
+        // rounding, with no host round trip per trial
         launch_precondition(errv, dvals.as<double>(), dmask.as<unsigned char>(), eval, n, s);
-        for (int i = 0; i < m; ++i) {
-          normalize(errv);
-          axpy(-dot(T(i), errv), T(i), errv);
+        for (int pass = 0; pass < 2; ++pass) {
+          launch_gram(T(0), n, m, errv, n, 1, n, gram.as<double>() + kProj, part.as<double>(), npart, s);
+          launch_combine_sub(T(0), n, m, gram.as<double>() + kProj, errv, n, s);
         }
-        normalize(errv);
+        dnormalize(errv);
         hip_check(hipMemcpyAsync(T(m), errv, n * sizeof(double), hipMemcpyDeviceToDevice, s), "trial");
         multH(T(m), Sg(m));
       }

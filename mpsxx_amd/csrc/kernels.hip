@@ -873,6 +873,62 @@ __global__ void combine_kernel(const double* __restrict__ base, int64_t ld, int 
   }
 }
 
+__global__ void combine_sub_kernel(const double* __restrict__ base, int64_t ld, int m, const double* __restrict__ c,
+                                   double* __restrict__ y, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < m; ++j) acc += c[j] * base[j * ld + i];
+    y[i] -= acc;
+  }
+}
+
+__global__ void scale_rnorm_kernel(double* x, int64_t n, const double* __restrict__ nrm2) {
+  const double v = nrm2[0], f = v > 0.0 ? 1.0 / sqrt(v) : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= f;
+}
+
+constexpr int kRotMax = 10;  // the Davidson subspace bound
+__global__ void rotate2_kernel(const double* __restrict__ T, const double* __restrict__ S, int64_t ld, int m,
+                               const double* __restrict__ c, int cld, double* __restrict__ TO,
+                               double* __restrict__ SO, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double t[kRotMax], sv[kRotMax];
+#pragma unroll
+    for (int k = 0; k < kRotMax; ++k) {
+      t[k] = k < m ? T[k * ld + i] : 0.0;
+      sv[k] = k < m ? S[k * ld + i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kRotMax; ++j) {
+      if (j >= m) break;
+      double at = 0.0, as = 0.0;
+#pragma unroll
+      for (int k = 0; k < kRotMax; ++k) {
+        if (k >= m) break;
+        at += c[j * cld + k] * t[k];
+        as += c[j * cld + k] * sv[k];
+      }
+      TO[j * ld + i] = at;
+      SO[j * ld + i] = as;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void residual_partial_kernel(const double* __restrict__ s0,
+                                                               const double* __restrict__ t0, double e,
+                                                               double* __restrict__ r, int64_t n, double* partial,
+                                                               int npart) {
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)npart * 256) {
+    const double v = s0[i] - e * t0[i];
+    r[i] = v;
+    acc += v * v;
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
 __global__ void precondition_kernel(double* r, const double* __restrict__ d, const unsigned char* __restrict__ pres,
                                     double eval, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1692,6 +1748,26 @@ void launch_precondition(double* r, const double* diag, const unsigned char* pre
                          hipStream_t s) {
   if (n > 0)
     hipLaunchKernelGGL(precondition_kernel, dim3(grid_for(n)), dim3(256), 0, s, r, diag, present, eval, n);
+}
+
+void launch_combine_sub(const double* base, int64_t ld, int m, const double* c, double* y, int64_t n, hipStream_t s) {
+  if (n > 0 && m > 0) hipLaunchKernelGGL(combine_sub_kernel, dim3(grid_for(n)), dim3(256), 0, s, base, ld, m, c, y, n);
+}
+
+void launch_scale_rnorm(double* x, int64_t n, const double* nrm2, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(scale_rnorm_kernel, dim3(grid_for(n)), dim3(256), 0, s, x, n, nrm2);
+}
+
+void launch_rotate2(const double* T, const double* S, int64_t ld, int m, const double* c, int cld, double* TO,
+                    double* SO, int64_t n, hipStream_t s) {
+  if (n > 0 && m > 0)
+    hipLaunchKernelGGL(rotate2_kernel, dim3(grid_for(n)), dim3(256), 0, s, T, S, ld, m, c, cld, TO, SO, n);
+}
+
+void launch_residual(const double* s0, const double* t0, double e, double* r, int64_t n, double* G, double* partial,
+                     int npart, hipStream_t s) {
+  hipLaunchKernelGGL(residual_partial_kernel, dim3(npart), dim3(256), 0, s, s0, t0, e, r, n, partial, npart);
+  hipLaunchKernelGGL(gram_reduce_kernel, dim3(1), dim3(256), 0, s, partial, npart, G);
 }
 
 void launch_gram(const double* X, int64_t ldx, int mx, const double* Y, int64_t ldy, int my, int64_t n, double* G,

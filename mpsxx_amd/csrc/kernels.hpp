@@ -227,5 +227,15 @@ void launch_precondition(double* r, const double* diag, const unsigned char* pre
 // G[i*m + j] = <X_i, Y_j> for i < mx, j < my (FP64, deterministic tree reduction)
 void launch_gram(const double* X, int64_t ldx, int mx, const double* Y, int64_t ldy, int my, int64_t n, double* G,
                  double* partial, int npart, hipStream_t s);
+// y -= sum_i c[i] * X_i (c in device memory: Gram-Schmidt projections without a host round trip)
+void launch_combine_sub(const double* base, int64_t ld, int m, const double* c, double* y, int64_t n, hipStream_t s);
+// x *= 1 / sqrt(nrm2[0]) (0 when nrm2[0] == 0), nrm2 in device memory
+void launch_scale_rnorm(double* x, int64_t n, const double* nrm2, hipStream_t s);
+// the Ritz rotation of both vector sets in one pass: TO_j = sum_k c[j*cld + k] T_k, SO_j likewise from S (j, k < m)
+void launch_rotate2(const double* T, const double* S, int64_t ld, int m, const double* c, int cld, double* TO,
+                    double* SO, int64_t n, hipStream_t s);
+// r = s0 - e t0 and G[0] = <r, r> (partial sums, then the tree reduction of launch_gram)
+void launch_residual(const double* s0, const double* t0, double e, double* r, int64_t n, double* G, double* partial,
+                     int npart, hipStream_t s);
 
 }  // namespace mpx
