@@ -44,6 +44,28 @@ def _check(rc):
 VP = ctypes.c_void_p
 
 
+def plan_cache_stats():
+    """Plan cache counters (mpsxx_plan_cache_stats): hits, builds, evictions, build_ms, plans, device_bytes."""
+    i = [ctypes.c_int64() for _ in range(5)]
+    ms = ctypes.c_double()
+    _check(lib().mpsxx_plan_cache_stats(ctypes.byref(i[0]), ctypes.byref(i[1]), ctypes.byref(i[2]), ctypes.byref(ms),
+                                        ctypes.byref(i[3]), ctypes.byref(i[4])))
+    return {"hits": i[0].value, "builds": i[1].value, "evictions": i[2].value, "build_ms": ms.value,
+            "plans": i[3].value, "device_bytes": i[4].value}
+
+
+def device_memory_stats():
+    """Device allocation counters (mpsxx_device_memory_stats): mallocs, frees, malloc_ms, free_ms, reuses,
+    pool_syncs, bytes_live, cached_bytes."""
+    i = [ctypes.c_int64() for _ in range(6)]
+    ma, fr = ctypes.c_double(), ctypes.c_double()
+    _check(lib().mpsxx_device_memory_stats(ctypes.byref(i[0]), ctypes.byref(i[1]), ctypes.byref(ma), ctypes.byref(fr),
+                                           ctypes.byref(i[2]), ctypes.byref(i[3]), ctypes.byref(i[4]),
+                                           ctypes.byref(i[5])))
+    return {"mallocs": i[0].value, "frees": i[1].value, "malloc_ms": ma.value, "free_ms": fr.value,
+            "reuses": i[2].value, "pool_syncs": i[3].value, "bytes_live": i[4].value, "cached_bytes": i[5].value}
+
+
 class DTensor:
     """Device-resident block-sparse array (opaque ``mpsxx_tensor`` handle)."""
 

@@ -476,6 +476,34 @@ int mpsxx_qc_mpo_create(int32_t norb, const double* oneint, const double* twoint
   });
 }
 
+int mpsxx_plan_cache_stats(int64_t* hits, int64_t* builds, int64_t* evictions, double* build_ms, int64_t* plans,
+                           int64_t* device_bytes) {
+  return guard_host([&] {
+    const mpx::PlanCacheStats st = mpx::plan_cache_stats();
+    if (hits) *hits = st.hits;
+    if (builds) *builds = st.builds;
+    if (evictions) *evictions = st.evictions;
+    if (build_ms) *build_ms = st.build_ms;
+    if (plans) *plans = st.plans;
+    if (device_bytes) *device_bytes = st.device_bytes;
+  });
+}
+
+int mpsxx_device_memory_stats(int64_t* mallocs, int64_t* frees, double* malloc_ms, double* free_ms, int64_t* reuses,
+                              int64_t* pool_syncs, int64_t* bytes_live, int64_t* cached_bytes) {
+  return guard_host([&] {
+    const mpx::DeviceMemoryStats st = mpx::device_memory_stats();
+    if (mallocs) *mallocs = st.mallocs;
+    if (frees) *frees = st.frees;
+    if (malloc_ms) *malloc_ms = st.malloc_ms;
+    if (free_ms) *free_ms = st.free_ms;
+    if (reuses) *reuses = st.reuses;
+    if (pool_syncs) *pool_syncs = st.pool_syncs;
+    if (bytes_live) *bytes_live = st.bytes_live;
+    if (cached_bytes) *cached_bytes = st.cached_bytes;
+  });
+}
+
 int mpsxx_planner_threads(int threads, int64_t min_pairs_per_thread) {
   return guard_host([&] { mpx::set_planner_threads(threads, min_pairs_per_thread); });
 }

@@ -2,6 +2,9 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <chrono>
+#include <mutex>
+#include <map>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -105,23 +108,141 @@ DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
     release();
     p = o.p;
     bytes = o.bytes;
+    cls = o.cls;
     o.p = nullptr;
     o.bytes = 0;
+    o.cls = 0;
   }
   return *this;
+}
+
+// Caching allocator behind DevBuf.  hipMalloc of large buffers costs ~1 ms (page mapping) and hipFree synchronizes the
+// device, and a DMRG sweep allocates thousands of them (outputs, scratch, task lists).  Released buffers are kept in a
+// size-class pool (up to MPSXX_DEVICE_CACHE_GB, default 32 GiB) and handed out again.  A buffer released since the last
+// device-wide synchronization may still be read by queued work on any stream, so handing one out is preceded by
+// hipDeviceSynchronize(), after which every pooled buffer is safe; allocations that find no pooled buffer call
+// hipMalloc and, if the device is out of memory, return the pool's buffers to the device and retry.
+namespace {
+
+struct DevicePool {
+  std::mutex mu;
+  std::multimap<size_t, void*> safe, pending;  // size class -> buffer
+  size_t cached = 0;
+  DeviceMemoryStats st;
+
+  static size_t size_class(size_t n) {
+    if (n <= ((size_t)1 << 20)) {
+      size_t c = 256;
+      while (c < n) c <<= 1;
+      return c;
+    }
+    const size_t g = (size_t)2 << 20;
+    return (n + g - 1) / g * g;
+  }
+  static size_t limit() {
+    static const size_t b = [] {
+      double gb = 32.0;
+      if (const char* e = std::getenv("MPSXX_DEVICE_CACHE_GB")) gb = std::max(0.0, std::atof(e));
+      return (size_t)(gb * 1073741824.0);
+    }();
+    return b;
+  }
+  // a pooled buffer of class c (at most 2c), nullptr if none
+  static void* take(std::multimap<size_t, void*>& m, size_t c, size_t& got) {
+    auto it = m.lower_bound(c);
+    if (it == m.end() || it->first > 2 * c) return nullptr;
+    void* p = it->second;
+    got = it->first;
+    m.erase(it);
+    return p;
+  }
+  void drain(std::multimap<size_t, void*>& m) {
+    for (auto& kv : m) {
+      (void)hipFree(kv.second);
+      cached -= kv.first;
+    }
+    m.clear();
+  }
+  void* alloc(size_t n, size_t& got) {
+    std::lock_guard<std::mutex> lk(mu);
+    const size_t c = size_class(n);
+    void* p = take(safe, c, got);
+    if (!p && !pending.empty()) {
+      auto it = pending.lower_bound(c);
+      if (it != pending.end() && it->first <= 2 * c) {
+        hip_check(hipDeviceSynchronize(), "device pool sync");
+        ++st.pool_syncs;
+        safe.insert(pending.begin(), pending.end());
+        pending.clear();
+        p = take(safe, c, got);
+      }
+    }
+    if (p) {
+      cached -= got;
+      ++st.reuses;
+      st.bytes_live += (int64_t)got;
+      return p;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    hipError_t e = hipMalloc(&p, c);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+      (void)hipGetLastError();
+      hip_check(hipDeviceSynchronize(), "device pool sync");
+      drain(safe);
+      drain(pending);
+      e = hipMalloc(&p, c);
+    }
+    hip_check(e, "hipMalloc");
+    st.malloc_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ++st.mallocs;
+    got = c;
+    st.bytes_live += (int64_t)got;
+    return p;
+  }
+  void release(void* p, size_t c) {
+    std::lock_guard<std::mutex> lk(mu);
+    st.bytes_live -= (int64_t)c;
+    if (cached + c <= limit()) {
+      pending.insert({c, p});
+      cached += c;
+      return;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    (void)hipFree(p);
+    st.free_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ++st.frees;
+  }
+};
+
+DevicePool& pool() {
+  static DevicePool* p = new DevicePool();  // never destroyed: buffers may be released during static destruction
+  return *p;
+}
+
+}  // namespace
+
+DeviceMemoryStats device_memory_stats() {
+  DevicePool& P = pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  DeviceMemoryStats st = P.st;
+  st.cached_bytes = (int64_t)P.cached;
+  return st;
 }
 
 void DevBuf::alloc(size_t n) {
   release();
   if (n == 0) return;
-  hip_check(hipMalloc(&p, n), "hipMalloc");
+  size_t got = 0;
+  p = pool().alloc(n, got);
   bytes = n;
+  cls = got;
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p) pool().release(p, cls);
   p = nullptr;
   bytes = 0;
+  cls = 0;
 }
 
 void Tensor::set_layout(const Layout& l, hipStream_t s) {

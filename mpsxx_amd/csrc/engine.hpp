@@ -16,16 +16,29 @@ namespace mpx {
 
 void hip_check(hipError_t e, const char* what);
 
-// Owning device buffer.
+// Device allocations since the library was loaded: hipMalloc/hipFree calls and their host time, buffers handed out
+// again from the pool (and the device synchronizations that took), bytes held by DevBufs and by the pool.
+struct DeviceMemoryStats {
+  int64_t mallocs = 0, frees = 0, reuses = 0, pool_syncs = 0, bytes_live = 0, cached_bytes = 0;
+  double malloc_ms = 0.0, free_ms = 0.0;
+};
+DeviceMemoryStats device_memory_stats();
+
+// Owning device buffer (from the library's caching pool; see engine.cpp).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  size_t cls = 0;  // size of the pooled allocation behind p
   DevBuf() = default;
   explicit DevBuf(size_t n) { alloc(n); }
   ~DevBuf();
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), cls(o.cls) {
+    o.p = nullptr;
+    o.bytes = 0;
+    o.cls = 0;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept;
   void alloc(size_t n);
   void release();

@@ -1387,6 +1387,17 @@ __global__ __launch_bounds__(256) void svd_finish_kernel(const SvdOut* __restric
     if (lane == 0) sig[j] = sqrt(ss);
   }
   __syncthreads();
+  // numerical rank: values below max(m, n) eps |A|_F are at the rounding level of the decomposition (the
+  // backward-error bound of an orthogonal one); they are reported as exactly zero, so noise directions are never kept
+  if (w == 0) {
+    double fro = 0.0;
+    for (int j = lane; j < kk; j += 64) fro += sig[j] * sig[j];
+    fro = wave_sum(fro);
+    const double thr = (double)max(o.m, o.n) * 2.220446049250313e-16 * sqrt(fro);
+    for (int j = lane; j < kk; j += 64)
+      if (sig[j] < thr) sig[j] = 0.0;
+  }
+  __syncthreads();
   for (int j = tid; j < kk; j += 256) {
     const double v = sig[j];
     int r = 0;

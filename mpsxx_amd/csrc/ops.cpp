@@ -3,6 +3,7 @@
 #include "ops.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -47,11 +48,17 @@ class PlanCache {
  public:
   CachedPlan* find(const std::vector<uint64_t>& key) {
     auto it = map_.find(key);
-    if (it == map_.end()) return nullptr;
+    if (it == map_.end()) {
+      miss_t0_ = std::chrono::steady_clock::now();  // a miss is followed by the build and insert()
+      return nullptr;
+    }
+    ++stats_.hits;
     lru_.splice(lru_.begin(), lru_, it->second.second);
     return it->second.first.get();
   }
   CachedPlan* insert(const std::vector<uint64_t>& key, std::unique_ptr<CachedPlan> p) {
+    ++stats_.builds;
+    stats_.build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - miss_t0_).count();
     const size_t add = p->plan.device_bytes();
     while (map_.size() > kKeep && (map_.size() >= kMax || bytes_ + add > budget())) {
       auto victim = lru_.back();
@@ -59,6 +66,7 @@ class PlanCache {
       auto it = map_.find(victim);
       bytes_ -= it->second.first->charged;
       map_.erase(it);
+      ++stats_.evictions;
     }
     lru_.push_front(key);
     CachedPlan* raw = p.get();
@@ -72,8 +80,16 @@ class PlanCache {
     lru_.clear();
     bytes_ = 0;
   }
+  PlanCacheStats stats() const {
+    PlanCacheStats st = stats_;
+    st.plans = (int64_t)map_.size();
+    st.device_bytes = (int64_t)bytes_;
+    return st;
+  }
 
  private:
+  PlanCacheStats stats_;
+  std::chrono::steady_clock::time_point miss_t0_ = std::chrono::steady_clock::now();
   static size_t budget() {
     static const size_t b = [] {
       double gb = 64.0;
@@ -94,6 +110,12 @@ PlanCache& cache() {
   static PlanCache c;
   return c;
 }
+
+}  // namespace
+
+PlanCacheStats plan_cache_stats() { return cache().stats(); }
+
+namespace {
 
 class PlanBuilder {
  public:

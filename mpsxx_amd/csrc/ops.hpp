@@ -57,6 +57,13 @@ void op_renormalize_overlap(bool fwd, Tensor& sov0, Tensor& bra, Tensor& ket, Te
 void op_renormalize_merged(bool fwd, Tensor& opr0, Tensor& bra, Tensor& ket, Tensor& opr1, int prec,
                            hipStream_t s);
 
+// Plan cache counters since the process started: lookups served from the cache, plans built (and the host time
+// spent building them, device allocation included), plans evicted, and what the cache holds now.
+struct PlanCacheStats {
+  int64_t hits = 0, builds = 0, evictions = 0, plans = 0, device_bytes = 0;
+  double build_ms = 0.0;
+};
+PlanCacheStats plan_cache_stats();
 // The most recently used cached plans are never evicted: a caller may hold references to at most this many plans
 // from the accessors below at once (the Davidson driver holds one per operator group).
 constexpr size_t kPlanCacheKeep = 16;

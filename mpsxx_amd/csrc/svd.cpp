@@ -10,6 +10,7 @@
 // U with total QN 0 and s*Vt carrying psi's QN (forward sweep); RightArrow gives U*s carrying psi's QN and Vt with
 // total 0 (backward sweep).  Blocks of the row (column) sectors keep the merged-sector order of QSTmergeInfo.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -170,6 +171,14 @@ constexpr int kJacobiMaxSweeps = 60;
 bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, const double* src, int kmax, int nmax,
                 int64_t su, int64_t sv, double* Sv, double* Ua, double* Vta, int* sweeps_out, hipStream_t s) {
   const int nb = (int)blks.size();
+  const bool trace = std::getenv("MPSXX_SVD_TRACE") != nullptr;
+  std::vector<double> tph;  // phase end times (ms since entry, stream synchronized) when tracing
+  const auto t_in = std::chrono::steady_clock::now();
+  auto phase = [&]() {
+    if (!trace) return;
+    hip_check(hipStreamSynchronize(s), "svd trace");
+    tph.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_in).count());
+  };
   std::vector<SvdBlk> sb(nb);
   int64_t xo = 0, vo = 0;
   int len_max = 1, kk_max = 1;
@@ -209,6 +218,7 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
     }
     run_copy(segs, src, X.as<double>(), s);
   }
+  phase();  // 0: gather
   // the swept matrices: Y = R^T (kk x kk) with QR, else X itself; V accumulates the rotations
   std::vector<SvdBlk> jb = sb;
   std::vector<SvdQrBlk> qb;
@@ -234,6 +244,7 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
     launch_svd_rt(dqb.as<SvdQrBlk>(), nb, X.as<double>(), Y.as<double>(), s);
     hip_check(hipGetLastError(), "svd qr launch");
   }
+  phase();  // 1: QR
   double* Xj = qr ? Y.as<double>() : X.as<double>();
   DevBuf dblk = upload_vec(jb);
   launch_svd_init(dblk.as<SvdBlk>(), nb, V.as<double>(), s);
@@ -280,10 +291,12 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
     roff.push_back((int)flat.size());
   }
   DevBuf dtask = upload_vec(flat);
+  phase();  // 2: schedule
   std::vector<int32_t> act(nb, 1), cnt(nb, 0);
   DevBuf dact = upload_vec(act), drot = upload_vec(cnt);
   bool converged = false;
   int sweep = 0;
+  std::vector<int> last_sweep(nb, 0);  // per block, the last sweep that rotated (tracing)
   while (sweep < kJacobiMaxSweeps) {
     ++sweep;
     for (size_t r = 0; r + 1 < roff.size(); ++r)
@@ -296,6 +309,7 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
     for (int k = 0; k < nb; ++k) {
       act[k] = cnt[k] > 0 ? 1 : 0;
       any = any || act[k];
+      if (trace && act[k]) last_sweep[k] = sweep;
     }
     if (!any) {
       converged = true;
@@ -305,13 +319,7 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
     hip_check(hipMemsetAsync(drot.p, 0, nb * sizeof(int32_t), s), "memset");
   }
   if (sweeps_out) *sweeps_out = sweep;
-  if (std::getenv("MPSXX_SVD_TRACE")) {  // per-call statistics of the batched Jacobi SVD on stderr
-    int64_t work = 0;
-    for (const SvdBlk& b : jb) work += (int64_t)b.len * b.kk * b.kk;
-    fprintf(stderr, "svdtrace nb %d len_max %d kk_max %d qr %d cols %d rounds %d tasks %d sweeps %d conv %d work %lld\n",
-            nb, len_max, kk_max, qcw, cols, (int)roff.size() - 1, (int)flat.size(), sweep, (int)converged,
-            (long long)work);
-  }
+  phase();  // 3: sweeps
   if (!converged) return false;
   std::vector<SvdOut> outs(nb);
   if (qr) {
@@ -328,6 +336,16 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
                     kk_max, s);
   hip_check(hipGetLastError(), "svd finish launch");
   hip_check(hipStreamSynchronize(s), "svd finish");
+  phase();  // 4: Q application and output
+  if (trace) {  // per-call statistics of the batched Jacobi SVD on stderr
+    fprintf(stderr, "svdtrace nb %d len_max %d kk_max %d qr %d cols %d rounds %d tasks %d sweeps %d ms", nb, len_max,
+            kk_max, qcw, cols, (int)roff.size() - 1, (int)flat.size(), sweep);
+    for (double t : tph) fprintf(stderr, " %.3f", t);
+    fprintf(stderr, " | block kk:last-rotating-sweep");
+    for (int k = 0; k < nb; ++k)
+      if (jb[k].kk > 32) fprintf(stderr, " %d:%d", jb[k].kk, last_sweep[k]);
+    fprintf(stderr, "\n");
+  }
   return true;
 }
 
@@ -402,6 +420,7 @@ double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo&
                          int64_t src_elems, Tensor& lmps, Tensor& rmps, int M, hipStream_t s) {
   const int nb = (int)blks.size();
   if (nb == 0) throw Error(MPSXX_ERR_ARG, "canonicalize: empty wavefunction");
+  const auto t_core = std::chrono::steady_clock::now();
   int mmax = 1, nmax = 1;
   for (const Blk& b : blks) { mmax = std::max(mmax, b.m); nmax = std::max(nmax, b.n); }
   const int kmax = std::min(mmax, nmax);
@@ -421,6 +440,15 @@ double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo&
                                          " (the wavefunction has non-finite entries)");
   } else {
     rocsolver_svd(blks, rects, src, src_elems, mmax, nmax, kmax, sa, su, sv, Sv, U, V, sv_h, s);
+    // numerical rank as on the Jacobi path: values below max(m, n) eps |A|_F count as zero
+    for (int k = 0; k < nb; ++k) {
+      const int kk = std::min(blks[k].m, blks[k].n);
+      double fro = 0.0;
+      for (int r = 0; r < kk; ++r) fro += sv_h[(size_t)k * kmax + r] * sv_h[(size_t)k * kmax + r];
+      const double thr = std::max(blks[k].m, blks[k].n) * 2.220446049250313e-16 * std::sqrt(fro);
+      for (int r = 0; r < kk; ++r)
+        if (sv_h[(size_t)k * kmax + r] < thr) sv_h[(size_t)k * kmax + r] = 0.0;
+    }
   }
   // truncation (QSTLAPACK.h:190-232)
   std::vector<double> all;
@@ -429,8 +457,10 @@ double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo&
   std::sort(all.rbegin(), all.rend());
   // tie rule: values within a relative 1e-12 of the M-th largest count as equal to it, so exact degeneracies (e.g.
   // the +Sz/-Sz sector pairs) that rounding splits are kept together on every backend
+  // (never below the default 1e-16: when fewer than M values exceed it, zeros and rounding-level values are not kept
+  // just because the M-th largest is one of them)
   double cutoff = 1.0e-16;
-  if (M > 0 && M < (int)all.size()) cutoff = all[M - 1] * (1.0 - 1e-12);
+  if (M > 0 && M < (int)all.size()) cutoff = std::max(cutoff, all[M - 1] * (1.0 - 1e-12));
   double dnorm = 0.0;
   std::vector<int> kept(nb, 0);
   for (int k = 0; k < nb; ++k) {
@@ -536,6 +566,11 @@ double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo&
   rmps.set_layout(LV, s);
   run_copy(useg, V.as<double>(), lmps.ptr(), s);
   run_copy(vseg, U.as<double>(), rmps.ptr(), s);
+  if (std::getenv("MPSXX_SVD_TRACE")) {
+    hip_check(hipStreamSynchronize(s), "svd trace");
+    fprintf(stderr, "canontrace total %.3f ms\n",
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_core).count());
+  }
   return dnorm;
 }
 

@@ -646,11 +646,13 @@ QST expand_cols(const QST& a, const MergeInfo& Mc, const std::vector<std::vector
 }
 
 // Tie rule at the truncation cutoff: values within a relative 1e-12 of the M-th largest count as equal to it (so
-// exact degeneracies split by rounding are kept together on every backend).
+// exact degeneracies split by rounding are kept together on every backend).  The cutoff never drops below the
+// reference's default 1e-16 (QSTLAPACK.h:210): when fewer than M values exceed it, values at the rounding level are
+// not kept just because the M-th largest is one of them.
 double truncation_cutoff(std::vector<double> all, int M) {
   std::sort(all.rbegin(), all.rend());
   double cutoff = 1.0e-16;
-  if (M > 0 && M < (int)all.size()) cutoff = all[M - 1] * (1.0 - 1e-12);
+  if (M > 0 && M < (int)all.size()) cutoff = std::max(cutoff, all[M - 1] * (1.0 - 1e-12));
   return cutoff;
 }
 
