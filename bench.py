@@ -222,9 +222,11 @@ def sweep_leg(L, schedule, precisions):
     return out
 
 
-def qc_sweep_leg(norb, schedule, precisions):
+def qc_sweep_leg(norb, schedule, precisions, max_final=6):
     """Merged two-site sweeps (src/make_sweep.cpp:112) of the QC chain with synthetic integrals from seed 1 over the
-    M schedule: per precision the total and per-sweep seconds, the final energy and |E - E_fp64|."""
+    M schedule, the last M repeated until the sweep energy changes by less than the tolerance (src/dmrg.cpp:44-78,
+    at most max_final sweeps at that M): per precision the total and per-sweep seconds, the converged energy and
+    |E - E_fp64|."""
     from mpsxx_amd import qc
     from mpsxx_amd.dmrg import SWEEP_MERGED, DeviceDMRG, initial_mps
     h, v = qc.synthetic_integrals(norb, seed=1)
@@ -238,9 +240,15 @@ def qc_sweep_leg(norb, schedule, precisions):
         dm = DeviceDMRG(mpos, initial_mps(norb, (norb, 0), schedule[0], 0), schedule[0], form=SWEEP_MERGED)
         secs, e = [], None
         t0 = time.perf_counter()
-        for M in schedule:
+        for M in schedule[:-1]:
             e, st = dm.sweep(M, 1e-10, precision=prec)
             secs.append(round(st["seconds"], 3))
+        for k in range(max_final):
+            e_prev = e
+            e, st = dm.sweep(schedule[-1], 1e-10, precision=prec)
+            secs.append(round(st["seconds"], 3))
+            if k > 0 and abs(e - e_prev) < 1e-10:
+                break
         r = {"seconds_total": time.perf_counter() - t0, "seconds_per_sweep": secs, "energy": e,
              "last_sweep_svd_s": st["svd_s"], "last_sweep_davidson_s": st["davidson_s"],
              "last_sweep_renorm_s": st["renorm_s"]}
@@ -269,7 +277,7 @@ def main():
     ap.add_argument("--qc-M", type=int, default=1024)
     ap.add_argument("--sweeps", type=int, default=1, help="run the sweep leg (L=32, M 128x3, 256x3, 512x2; 0: skip)")
     ap.add_argument("--qc-sweeps", type=int, default=1,
-                    help="run the QC merged-sweep leg (norb --qc-norb, M 256, 512, 1024, 1024; 0: skip)")
+                    help="run the QC merged-sweep leg (norb --qc-norb, M 256, 512, then 1024 until converged; 0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -317,7 +325,7 @@ def main():
 
     qc_sweeps = None
     if args.qc_sweeps > 0 and args.qc_norb > 0:
-        qc_sweeps = qc_sweep_leg(args.qc_norb, [256, 512, 1024, 1024], [mx.FP64, mx.FP32])
+        qc_sweeps = qc_sweep_leg(args.qc_norb, [256, 512, 1024], [mx.FP64, mx.FP32])
 
     cpu = None
     parity = None
