@@ -23,6 +23,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-I" + os.path.join(ROOT, "include"), "-
 DEVICE = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 HOST = ["-D__HIP_PLATFORM_AMD__"]
+# per-source device options: the FP64 grouped GEMM keeps its MFMA accumulators in VGPRs (kernels_f64.hip)
+EXTRA = {"kernels_f64.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 LIBS = ["-lrocsolver", "-lrocblas"]
 
 
@@ -47,7 +49,7 @@ def _compile(src, force):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
     if not force and os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(src), _headers_mtime()):
         return obj
-    flags = COMMON + (DEVICE if src.endswith(".hip") else HOST)
+    flags = COMMON + (DEVICE if src.endswith(".hip") else HOST) + EXTRA.get(os.path.basename(src), [])
     cmd = [HIPCC] + flags + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -1,0 +1,266 @@
+// The 64x64 grouped-GEMM tiles shared by kernels.hip (FP32 operands) and kernels_f64.hip (FP64 operands, compiled
+// with MFMA accumulators in VGPRs, which keeps them out of per-panel register copies); device code, included only by
+// .hip files.
+//
+// grouped GEMM: one 256-thread work-group per 64x64 C tile; 4 waves, each owning a 32x32 quarter computed with
+// 2x2 v_mfma_{f64,f32}_16x16x4 tiles.  A and B tiles (64x16, 16x64) are staged in LDS k-major so that the MFMA
+// operand of lane l is the LDS word [k + l/16][i + l%16].  The global->LDS mapping picks the unit-stride direction of
+// each operand so loads are coalesced whether the block is used as stored or transposed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace mpx {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Operand layout of both 16x16x4 instructions: lane l supplies A[l%16][l/16] and B[l/16][l%16].
+// Result layout (measured on gfx950): FP32 lane l, register r holds D[4*(l/16) + r][l%16];
+//                                     FP64 lane l, register r holds D[(l/16) + 4*r][l%16].
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef dbl4 acc_t;
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct Mfma<float> {
+  typedef flt4 acc_t;
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
+constexpr int BM = 64, BN = 64, BK = 16, LDP = 64 + 1;
+
+// operand base of the task's instance: Tab = the launch passes per-instance pointer tables (batched launches, which
+// are distinct kernel symbols in profiles)
+template <typename T, bool Tab>
+__device__ __forceinline__ T* opnd(const void* p, int inst) {
+  if constexpr (Tab) return reinterpret_cast<T* const*>(p)[inst];
+  else return reinterpret_cast<T*>(const_cast<void*>(p));
+}
+
+// Panel staging of the 64x64 kernel: each of the 256 threads moves 4 elements of the 64x16 A panel and 4 of the 16x64
+// B panel.  The element -> thread map follows the operand's unit-stride direction (k for A stored as used, rows for A
+// used transposed; columns for B as stored, k for B transposed) so that global loads are coalesced either way.
+template <typename T>
+__device__ __forceinline__ void load_a(T (&r)[4], const T* __restrict__ Ap, const GemmPair& pr, int row0, int m, int k0,
+                                       int tid) {
+  if (pr.sa_k == 1) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j, kk = e & 15, i = e >> 4;
+      const int gi = row0 + i, gk = k0 + kk;
+      r[j] = (gi < m && gk < pr.k) ? Ap[(int64_t)gi * pr.sa_m + gk] : T(0);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j, i = e & 63, kk = e >> 6;
+      const int gi = row0 + i, gk = k0 + kk;
+      r[j] = (gi < m && gk < pr.k) ? Ap[(int64_t)gi * pr.sa_m + (int64_t)gk * pr.sa_k] : T(0);
+    }
+  }
+}
+template <typename T>
+__device__ __forceinline__ void store_a(T (*As)[LDP], const T (&r)[4], bool contig_k, int tid) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = tid + 256 * j;
+    if (contig_k) As[e & 15][e >> 4] = r[j];
+    else As[e >> 6][e & 63] = r[j];
+  }
+}
+template <typename T>
+__device__ __forceinline__ void load_b(T (&r)[4], const T* __restrict__ Bp, const GemmPair& pr, int col0, int n, int k0,
+                                       int tid) {
+  if (pr.sb_n == 1) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j, jj = e & 63, kk = e >> 6;
+      const int gj = col0 + jj, gk = k0 + kk;
+      r[j] = (gj < n && gk < pr.k) ? Bp[(int64_t)gk * pr.sb_k + gj] : T(0);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j, kk = e & 15, jj = e >> 4;
+      const int gj = col0 + jj, gk = k0 + kk;
+      r[j] = (gj < n && gk < pr.k) ? Bp[(int64_t)gk * pr.sb_k + (int64_t)gj * pr.sb_n] : T(0);
+    }
+  }
+}
+template <typename T>
+__device__ __forceinline__ void store_b(T (*Bs)[LDP], const T (&r)[4], bool contig_n, int tid) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int e = tid + 256 * j;
+    if (contig_n) Bs[e >> 6][e & 63] = r[j];
+    else Bs[e & 15][e >> 4] = r[j];
+  }
+}
+
+// 64x64 tiles of large blocks.  The (pair, 16-wide k panel) sequence of a tile (or of its pair range, when the
+// reduction is split over several work-groups) is software-pipelined: the next panel is loaded into registers while
+// the MFMAs run on the current one from one of two LDS buffers, one barrier per panel.  A tile with slot >= 0 writes
+// its unscaled partial sum to partial[slot] (64x64, row-major); gemm_reduce_kernel adds the slots of a tile in order.
+template <typename T, typename TC, bool Tab>
+__global__ __launch_bounds__(256) void gemm_grouped_kernel(const BigTile* __restrict__ tiles, int ntiles,
+                                                           const GemmTask* __restrict__ tasks,
+                                                           const GemmPair* __restrict__ pairs, Operands ops,
+                                                           double alpha, int accumulate, T* __restrict__ partial) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  __shared__ T As[2][BK][LDP];
+  __shared__ T Bs[2][BK][LDP];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
+    const BigTile tl = tiles[w];
+    const GemmTask tk = tasks[tl.task];
+    const T* __restrict__ A = opnd<const T, Tab>(ops.a, tk.inst);
+    const T* __restrict__ B = opnd<const T, Tab>(ops.b, tk.inst);
+
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    // the tile's k panels over its pair range, counted first so that the panel loop is a plain counted loop (the
+    // accumulators then stay in the MFMA registers instead of being copied around a mid-loop exit)
+    int npanel = 0;  // every pair takes at least one panel (k = 0 pairs contribute a zero panel)
+    for (int q = tl.pair_lo; q < tl.pair_hi; ++q) npanel += max(1, (pairs[q].k + BK - 1) / BK);
+    // prologue: first panel into buffer 0
+    int p = tl.pair_lo, k0 = 0;
+    GemmPair pr = pairs[p];
+    T ra[4], rb[4];
+    load_a(ra, A + pr.a_off, pr, tl.row0, tk.m, 0, tid);
+    load_b(rb, B + pr.b_off, pr, tl.col0, tk.n, 0, tid);
+    store_a(As[0], ra, pr.sa_k == 1, tid);
+    store_b(Bs[0], rb, pr.sb_n == 1, tid);
+    __syncthreads();
+    int buf = 0;
+    for (int it = 0; it < npanel; ++it) {
+      int np = p, nk0 = k0 + BK;
+      GemmPair npr = pr;
+      if (nk0 >= pr.k) {
+        np = p + 1;
+        nk0 = 0;
+        if (np < tl.pair_hi) npr = pairs[np];
+      }
+      const bool more = it + 1 < npanel;  // uniform over the work-group
+      if (more) {
+        load_a(ra, A + npr.a_off, npr, tl.row0, tk.m, nk0, tid);
+        load_b(rb, B + npr.b_off, npr, tl.col0, tk.n, nk0, tid);
+      }
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const T a0 = As[buf][kk + lk][wr * 32 + lr];
+        const T a1 = As[buf][kk + lk][wr * 32 + 16 + lr];
+        const T b0 = Bs[buf][kk + lk][wc * 32 + lr];
+        const T b1 = Bs[buf][kk + lk][wc * 32 + 16 + lr];
+        acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+        acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+        acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+        acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+      }
+      if (more) {
+        store_a(As[buf ^ 1], ra, npr.sa_k == 1, tid);
+        store_b(Bs[buf ^ 1], rb, npr.sb_n == 1, tid);
+      }
+      __syncthreads();
+      buf ^= 1;
+      p = np;
+      k0 = nk0;
+      pr = npr;
+    }
+
+    if (tl.slot < 0) {
+      TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = tl.row0 + wr * 32 + mi * 16 + Mfma<T>::row(lk, r);
+            const int col = tl.col0 + wc * 32 + ni * 16 + lr;
+            if (row < tk.m && col < tk.n) {
+              const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+              const double v = alpha * (double)acc[mi][ni][r];
+              C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+            }
+          }
+    } else {
+      T* __restrict__ P = partial + (int64_t)tl.slot * (BM * BN);
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = wr * 32 + mi * 16 + Mfma<T>::row(lk, r);
+            const int col = wc * 32 + ni * 16 + lr;
+            P[row * BN + col] = acc[mi][ni][r];
+          }
+    }
+    __syncthreads();  // every wave is done with the LDS buffers before the next tile's prologue
+  }
+}
+
+// Sum of the partial tiles of split reductions, in slot order (deterministic), into C.
+template <typename T, typename TC, bool Tab>
+__global__ __launch_bounds__(256) void gemm_reduce_kernel(const RedTile* __restrict__ red, int nred,
+                                                          const GemmTask* __restrict__ tasks,
+                                                          const T* __restrict__ partial, Operands ops, double alpha,
+                                                          int accumulate) {
+  for (int w = blockIdx.x; w < nred; w += gridDim.x) {
+    const RedTile rt = red[w];
+    const GemmTask tk = tasks[rt.task];
+    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
+    for (int e = threadIdx.x; e < BM * BN; e += 256) {
+      const int row = rt.row0 + (e >> 6), col = rt.col0 + (e & 63);
+      if (row >= tk.m || col >= tk.n) continue;
+      double sum = 0.0;
+      for (int q = 0; q < rt.nslots; ++q) sum += (double)partial[(int64_t)(rt.slot0 + q) * (BM * BN) + e];
+      const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+      const double v = alpha * sum;
+      C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+    }
+  }
+}
+
+// work-groups of a task-list launch: one per work item, or at most max_groups (> 0) looping over the list
+static inline int groups(int items, int max_groups) { return max_groups > 0 && items > max_groups ? max_groups : items; }
+
+template <typename T, typename TC>
+void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
+                 const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups) {
+  if (ntiles <= 0) return;
+  T* part = static_cast<T*>(partial);
+  if (ops.table)
+    hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
+                       ntiles, tasks, pairs, ops, alpha, accumulate, part);
+  else
+    hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
+                       ntiles, tasks, pairs, ops, alpha, accumulate, part);
+  if (nred <= 0) return;
+  if (ops.table)
+    hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, true>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red, nred,
+                       tasks, part, ops, alpha, accumulate);
+  else
+    hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, false>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red, nred,
+                       tasks, part, ops, alpha, accumulate);
+}
+
+}  // namespace mpx

@@ -1,213 +1,11 @@
-// HIP kernels for gfx950 (CDNA4, wave64).
-//
-// grouped GEMM: one 256-thread work-group per 64x64 C tile; 4 waves, each owning a 32x32 quarter computed with
-// 2x2 v_mfma_{f64,f32}_16x16x4 tiles.  A and B tiles (64x16, 16x64) are staged in LDS k-major so that the MFMA
-// operand of lane l is the LDS word [k + l/16][i + l%16].  The global->LDS mapping picks the unit-stride direction of
-// each operand so loads are coalesced whether the block is used as stored or transposed.
+// HIP kernels for gfx950 (CDNA4, wave64).  The 64x64 grouped-GEMM tiles are in gemm_tiles.hpp (their FP64
+// instantiation in kernels_f64.hip).
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "gemm_tiles.hpp"
 
 namespace mpx {
-
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-typedef float flt4 __attribute__((ext_vector_type(4)));
-
-// Operand layout of both 16x16x4 instructions: lane l supplies A[l%16][l/16] and B[l/16][l%16].
-// Result layout (measured on gfx950): FP32 lane l, register r holds D[4*(l/16) + r][l%16];
-//                                     FP64 lane l, register r holds D[(l/16) + 4*r][l%16].
-template <typename T> struct Mfma;
-template <> struct Mfma<double> {
-  typedef dbl4 acc_t;
-  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
-};
-template <> struct Mfma<float> {
-  typedef flt4 acc_t;
-  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
-};
-
-constexpr int BM = 64, BN = 64, BK = 16, LDP = 64 + 1;
-
-// operand base of the task's instance: Tab = the launch passes per-instance pointer tables (batched launches, which
-// are distinct kernel symbols in profiles)
-template <typename T, bool Tab>
-__device__ __forceinline__ T* opnd(const void* p, int inst) {
-  if constexpr (Tab) return reinterpret_cast<T* const*>(p)[inst];
-  else return reinterpret_cast<T*>(const_cast<void*>(p));
-}
-
-// Panel staging of the 64x64 kernel: each of the 256 threads moves 4 elements of the 64x16 A panel and 4 of the 16x64
-// B panel.  The element -> thread map follows the operand's unit-stride direction (k for A stored as used, rows for A
-// used transposed; columns for B as stored, k for B transposed) so that global loads are coalesced either way.
-template <typename T>
-__device__ __forceinline__ void load_a(T (&r)[4], const T* __restrict__ Ap, const GemmPair& pr, int row0, int m, int k0,
-                                       int tid) {
-  if (pr.sa_k == 1) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j, kk = e & 15, i = e >> 4;
-      const int gi = row0 + i, gk = k0 + kk;
-      r[j] = (gi < m && gk < pr.k) ? Ap[(int64_t)gi * pr.sa_m + gk] : T(0);
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j, i = e & 63, kk = e >> 6;
-      const int gi = row0 + i, gk = k0 + kk;
-      r[j] = (gi < m && gk < pr.k) ? Ap[(int64_t)gi * pr.sa_m + (int64_t)gk * pr.sa_k] : T(0);
-    }
-  }
-}
-template <typename T>
-__device__ __forceinline__ void store_a(T (*As)[LDP], const T (&r)[4], bool contig_k, int tid) {
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int e = tid + 256 * j;
-    if (contig_k) As[e & 15][e >> 4] = r[j];
-    else As[e >> 6][e & 63] = r[j];
-  }
-}
-template <typename T>
-__device__ __forceinline__ void load_b(T (&r)[4], const T* __restrict__ Bp, const GemmPair& pr, int col0, int n, int k0,
-                                       int tid) {
-  if (pr.sb_n == 1) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j, jj = e & 63, kk = e >> 6;
-      const int gj = col0 + jj, gk = k0 + kk;
-      r[j] = (gj < n && gk < pr.k) ? Bp[(int64_t)gk * pr.sb_k + gj] : T(0);
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j, kk = e & 15, jj = e >> 4;
-      const int gj = col0 + jj, gk = k0 + kk;
-      r[j] = (gj < n && gk < pr.k) ? Bp[(int64_t)gk * pr.sb_k + (int64_t)gj * pr.sb_n] : T(0);
-    }
-  }
-}
-template <typename T>
-__device__ __forceinline__ void store_b(T (*Bs)[LDP], const T (&r)[4], bool contig_n, int tid) {
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int e = tid + 256 * j;
-    if (contig_n) Bs[e >> 6][e & 63] = r[j];
-    else Bs[e & 15][e >> 4] = r[j];
-  }
-}
-
-// 64x64 tiles of large blocks.  The (pair, 16-wide k panel) sequence of a tile (or of its pair range, when the
-// reduction is split over several work-groups) is software-pipelined: the next panel is loaded into registers while
-// the MFMAs run on the current one from one of two LDS buffers, one barrier per panel.  A tile with slot >= 0 writes
-// its unscaled partial sum to partial[slot] (64x64, row-major); gemm_reduce_kernel adds the slots of a tile in order.
-template <typename T, typename TC, bool Tab>
-__global__ __launch_bounds__(256) void gemm_grouped_kernel(const BigTile* __restrict__ tiles, int ntiles,
-                                                           const GemmTask* __restrict__ tasks,
-                                                           const GemmPair* __restrict__ pairs, Operands ops,
-                                                           double alpha, int accumulate, T* __restrict__ partial) {
-  typedef typename Mfma<T>::acc_t acc_t;
-  __shared__ T As[2][BK][LDP];
-  __shared__ T Bs[2][BK][LDP];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  const int lr = lane & 15, lk = lane >> 4;
-
-  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    const BigTile tl = tiles[w];
-    const GemmTask tk = tasks[tl.task];
-    const T* __restrict__ A = opnd<const T, Tab>(ops.a, tk.inst);
-    const T* __restrict__ B = opnd<const T, Tab>(ops.b, tk.inst);
-
-    acc_t acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-
-    // prologue: first panel into buffer 0
-    int p = tl.pair_lo, k0 = 0;
-    GemmPair pr = pairs[p];
-    T ra[4], rb[4];
-    load_a(ra, A + pr.a_off, pr, tl.row0, tk.m, 0, tid);
-    load_b(rb, B + pr.b_off, pr, tl.col0, tk.n, 0, tid);
-    store_a(As[0], ra, pr.sa_k == 1, tid);
-    store_b(Bs[0], rb, pr.sb_n == 1, tid);
-    __syncthreads();
-    int buf = 0;
-    while (true) {
-      int np = p, nk0 = k0 + BK;
-      GemmPair npr = pr;
-      if (nk0 >= pr.k) {
-        np = p + 1;
-        nk0 = 0;
-        if (np < tl.pair_hi) npr = pairs[np];
-      }
-      const bool more = np < tl.pair_hi;  // uniform over the work-group
-      if (more) {
-        load_a(ra, A + npr.a_off, npr, tl.row0, tk.m, nk0, tid);
-        load_b(rb, B + npr.b_off, npr, tl.col0, tk.n, nk0, tid);
-      }
-#pragma unroll
-      for (int kk = 0; kk < BK; kk += 4) {
-        const T a0 = As[buf][kk + lk][wr * 32 + lr];
-        const T a1 = As[buf][kk + lk][wr * 32 + 16 + lr];
-        const T b0 = Bs[buf][kk + lk][wc * 32 + lr];
-        const T b1 = Bs[buf][kk + lk][wc * 32 + 16 + lr];
-        acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
-        acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
-        acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
-        acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
-      }
-      if (!more) break;
-      store_a(As[buf ^ 1], ra, npr.sa_k == 1, tid);
-      store_b(Bs[buf ^ 1], rb, npr.sb_n == 1, tid);
-      __syncthreads();
-      buf ^= 1;
-      p = np;
-      k0 = nk0;
-      pr = npr;
-    }
-
-    if (tl.slot < 0) {
-      TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = tl.row0 + wr * 32 + mi * 16 + Mfma<T>::row(lk, r);
-            const int col = tl.col0 + wc * 32 + ni * 16 + lr;
-            if (row < tk.m && col < tk.n) {
-              const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
-              const double v = alpha * (double)acc[mi][ni][r];
-              C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
-            }
-          }
-    } else {
-      T* __restrict__ P = partial + (int64_t)tl.slot * (BM * BN);
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = wr * 32 + mi * 16 + Mfma<T>::row(lk, r);
-            const int col = wc * 32 + ni * 16 + lr;
-            P[row * BN + col] = acc[mi][ni][r];
-          }
-    }
-    __syncthreads();  // every wave is done with the LDS buffers before the next tile's prologue
-  }
-}
 
 // ---------------------------------------------------------------------------------------------------------------------
 // Split-BF16 (x3) 64x64 tiles: FP32 operands are split on the way into LDS as x = hi + lo with hi = bf16(x) and
@@ -402,28 +200,6 @@ __global__ __launch_bounds__(256) void gemm_x3_kernel(const BigTile* __restrict_
           }
     }
     __syncthreads();  // every wave is done with the LDS buffers before the next tile's prologue
-  }
-}
-
-// Sum of the partial tiles of split reductions, in slot order (deterministic), into C.
-template <typename T, typename TC, bool Tab>
-__global__ __launch_bounds__(256) void gemm_reduce_kernel(const RedTile* __restrict__ red, int nred,
-                                                          const GemmTask* __restrict__ tasks,
-                                                          const T* __restrict__ partial, Operands ops, double alpha,
-                                                          int accumulate) {
-  for (int w = blockIdx.x; w < nred; w += gridDim.x) {
-    const RedTile rt = red[w];
-    const GemmTask tk = tasks[rt.task];
-    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
-    for (int e = threadIdx.x; e < BM * BN; e += 256) {
-      const int row = rt.row0 + (e >> 6), col = rt.col0 + (e & 63);
-      if (row >= tk.m || col >= tk.n) continue;
-      double sum = 0.0;
-      for (int q = 0; q < rt.nslots; ++q) sum += (double)partial[(int64_t)(rt.slot0 + q) * (BM * BN) + e];
-      const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
-      const double v = alpha * sum;
-      C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
-    }
   }
 }
 
@@ -901,16 +677,17 @@ __global__ void rotate2_kernel(const double* __restrict__ T, const double* __res
     }
 #pragma unroll
     for (int j = 0; j < kRotMax; ++j) {
-      if (j >= m) break;
-      double at = 0.0, as = 0.0;
+      if (j < m) {
+        double at = 0.0, as = 0.0;
 #pragma unroll
-      for (int k = 0; k < kRotMax; ++k) {
-        if (k >= m) break;
-        at += c[j * cld + k] * t[k];
-        as += c[j * cld + k] * sv[k];
+        for (int k = 0; k < kRotMax; ++k) {
+          const double cf = k < m ? c[j * cld + k] : 0.0;
+          at += cf * t[k];
+          as += cf * sv[k];
+        }
+        TO[j * ld + i] = at;
+        SO[j * ld + i] = as;
       }
-      TO[j * ld + i] = at;
-      SO[j * ld + i] = as;
     }
   }
 }
@@ -964,8 +741,6 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restri
   if (threadIdx.x == 0) G[pair] = acc;
 }
 
-// work-groups of a task-list launch: one per work item, or at most max_groups (> 0) looping over the list
-static inline int groups(int items, int max_groups) { return max_groups > 0 && items > max_groups ? max_groups : items; }
 
 static inline int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
@@ -974,25 +749,6 @@ static inline int grid_for(int64_t n) {
   return (int)g;
 }
 
-template <typename T, typename TC>
-void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
-                 const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups) {
-  if (ntiles <= 0) return;
-  T* part = static_cast<T*>(partial);
-  if (ops.table)
-    hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate, part);
-  else
-    hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate, part);
-  if (nred <= 0) return;
-  if (ops.table)
-    hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, true>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red, nred,
-                       tasks, part, ops, alpha, accumulate);
-  else
-    hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, false>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red, nred,
-                       tasks, part, ops, alpha, accumulate);
-}
 
 template <typename TC>
 void launch_gemm_x3(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
@@ -1818,7 +1574,6 @@ void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks,
 #define MPX_INST_BIG(T, TC)                                                                                   \
   template void launch_gemm<T, TC>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*, const GemmPair*, \
                                    Operands, double, int, hipStream_t, int);
-MPX_INST_BIG(double, double)
 MPX_INST_BIG(float, float)
 MPX_INST_BIG(float, double)
 #undef MPX_INST_BIG
