@@ -135,12 +135,12 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   double* errv = base + (int64_t)(4 * max_ritz) * n;
   hip_check(hipMemsetAsync(base, 0, (size_t)nslot * n * sizeof(double), s), "memset");
 
-  // trial[0] <- wfnc placed into U
-  for (int64_t b = 0; b < wfnc.L.nblocks(); ++b) {
-    const int64_t j = U.find(wfnc.L.tag[b]);
-    hip_check(hipMemcpyAsync(T(0) + U.off[j], wfnc.ptr() + wfnc.L.off[b], wfnc.L.bsize[b] * sizeof(double),
-                             hipMemcpyDeviceToDevice, s),
-              "davidson init");
+  // trial[0] <- wfnc placed into U (one launch for all blocks)
+  {
+    std::vector<CopySeg> segs;
+    for (int64_t b = 0; b < wfnc.L.nblocks(); ++b)
+      if (wfnc.L.bsize[b] > 0) segs.push_back({U.off[U.find(wfnc.L.tag[b])], wfnc.L.off[b], (int32_t)wfnc.L.bsize[b], 1});
+    copy_segments(segs, wfnc.ptr(), T(0), s);
   }
   // lower-root vectors aligned to U (H_k, S_k for every k)
   const int nlow = (int)lower.size();
@@ -152,12 +152,10 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
     for (int which = 0; which < 2; ++which) {
       const Tensor& t = which ? *lower[k].overlap : *lower[k].sigma;
       double* dst = which ? Sk(k) : Hk(k);
-      for (int64_t b = 0; b < t.L.nblocks(); ++b) {
-        const int64_t j = U.find(t.L.tag[b]);
-        hip_check(hipMemcpyAsync(dst + U.off[j], t.ptr() + t.L.off[b], t.L.bsize[b] * sizeof(double),
-                                 hipMemcpyDeviceToDevice, s),
-                  "lower root");
-      }
+      std::vector<CopySeg> segs;
+      for (int64_t b = 0; b < t.L.nblocks(); ++b)
+        if (t.L.bsize[b] > 0) segs.push_back({U.off[U.find(t.L.tag[b])], t.L.off[b], (int32_t)t.L.bsize[b], 1});
+      copy_segments(segs, t.ptr(), dst, s);
     }
   }
 
@@ -167,15 +165,15 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
     std::vector<unsigned char> mask(n, 0);
     hip_check(hipMemsetAsync(dvals.p, 0, n * sizeof(double), s), "memset");
     if (!diag.empty) {
+      std::vector<CopySeg> segs;
       for (int64_t b = 0; b < U.nblocks(); ++b) {
         const int64_t j = diag.L.find(U.tag[b]);
         if (j < 0) continue;
         if (diag.L.bsize[j] != U.bsize[b]) throw Error(MPSXX_ERR_SHAPE, "Davidson: diag block shape mismatch");
-        hip_check(hipMemcpyAsync(dvals.as<double>() + U.off[b], diag.ptr() + diag.L.off[j], U.bsize[b] * sizeof(double),
-                                 hipMemcpyDeviceToDevice, s),
-                  "diag align");
+        if (U.bsize[b] > 0) segs.push_back({U.off[b], diag.L.off[j], (int32_t)U.bsize[b], 1});
         std::memset(mask.data() + U.off[b], 1, U.bsize[b]);
       }
+      copy_segments(segs, diag.ptr(), dvals.as<double>(), s);
     }
     hip_check(hipMemcpyAsync(dmask.p, mask.data(), n, hipMemcpyHostToDevice, s), "mask");
     hip_check(hipStreamSynchronize(s), "mask sync");

@@ -116,12 +116,22 @@ DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
   return *this;
 }
 
-// Caching allocator behind DevBuf.  hipMalloc of large buffers costs ~1 ms (page mapping) and hipFree synchronizes the
-// device, and a DMRG sweep allocates thousands of them (outputs, scratch, task lists).  Released buffers are kept in a
-// size-class pool (up to MPSXX_DEVICE_CACHE_GB, default 32 GiB) and handed out again.  A buffer released since the last
+// Caching allocator behind DevBuf.  hipMalloc of large buffers costs ~1 ms per GB (page mapping) and hipFree
+// synchronizes the device, and a DMRG sweep allocates thousands of them (outputs, scratch, task lists).  Released
+// buffers are kept in a size-class pool (up to MPSXX_DEVICE_CACHE_GB, default 20 % of the device's memory) and handed
+// out again.  A buffer released since the last
 // device-wide synchronization may still be read by queued work on any stream, so handing one out is preceded by
 // hipDeviceSynchronize(), after which every pooled buffer is safe; allocations that find no pooled buffer call
 // hipMalloc and, if the device is out of memory, return the pool's buffers to the device and retry.
+size_t device_memory_fraction(double frac, double fallback_gb) {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || total_b == 0) {
+    (void)hipGetLastError();
+    return (size_t)(fallback_gb * 1073741824.0);
+  }
+  return (size_t)(frac * (double)total_b);
+}
+
 namespace {
 
 struct DevicePool {
@@ -141,9 +151,8 @@ struct DevicePool {
   }
   static size_t limit() {
     static const size_t b = [] {
-      double gb = 32.0;
-      if (const char* e = std::getenv("MPSXX_DEVICE_CACHE_GB")) gb = std::max(0.0, std::atof(e));
-      return (size_t)(gb * 1073741824.0);
+      if (const char* e = std::getenv("MPSXX_DEVICE_CACHE_GB")) return (size_t)(std::max(0.0, std::atof(e)) * 1073741824.0);
+      return device_memory_fraction(0.2, 32.0);
     }();
     return b;
   }
@@ -252,6 +261,17 @@ void Tensor::set_layout(const Layout& l, hipStream_t s) {
   empty = false;
 }
 
+void copy_segments(const std::vector<CopySeg>& segs, const double* x, double* y, hipStream_t s) {
+  if (segs.empty()) return;
+  std::vector<Chunk> chunks;
+  for (size_t i = 0; i < segs.size(); ++i)
+    for (int c = 0; c < segs[i].len; c += kAxpyChunk) chunks.push_back({(int32_t)i, c});
+  // the uploaded lists go back to the device pool when this returns; the pool reuses them only after a device-wide
+  // synchronization, so the launch below still reads them
+  DevBuf ds = upload_vec(segs), dc = upload_vec(chunks);
+  launch_copy<double, double>(dc.as<Chunk>(), (int)chunks.size(), ds.as<CopySeg>(), x, y, 1.0, 0.0, s);
+}
+
 void Tensor::relayout(const Layout& l, hipStream_t s) {
   if (empty) {
     set_layout(l, s);
@@ -260,14 +280,13 @@ void Tensor::relayout(const Layout& l, hipStream_t s) {
   if (l.hash == L.hash) return;
   DevBuf nb(std::max<int64_t>(l.nelem, 1) * sizeof(double));
   hip_check(hipMemsetAsync(nb.p, 0, std::max<int64_t>(l.nelem, 1) * sizeof(double), s), "memset");
+  std::vector<CopySeg> segs;
   for (int64_t b = 0; b < L.nblocks(); ++b) {
     const int64_t j = l.find(L.tag[b]);
-    if (j < 0) continue;
-    hip_check(hipMemcpyAsync(nb.as<double>() + l.off[j], ptr() + L.off[b], L.bsize[b] * sizeof(double),
-                             hipMemcpyDeviceToDevice, s),
-              "relayout copy");
+    if (j < 0 || L.bsize[b] == 0) continue;
+    segs.push_back({l.off[j], L.off[b], (int32_t)L.bsize[b], 1});
   }
-  hip_check(hipStreamSynchronize(s), "relayout sync");
+  copy_segments(segs, ptr(), nb.as<double>(), s);
   data = std::move(nb);
   L = l;
 }

@@ -23,6 +23,8 @@ struct DeviceMemoryStats {
   double malloc_ms = 0.0, free_ms = 0.0;
 };
 DeviceMemoryStats device_memory_stats();
+// frac of the current device's memory (fallback_gb GiB when it cannot be queried): default cache bounds.
+size_t device_memory_fraction(double frac, double fallback_gb);
 
 // Owning device buffer (from the library's caching pool; see engine.cpp).
 struct DevBuf {
@@ -44,6 +46,10 @@ struct DevBuf {
   void release();
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
+
+// Block placement in one launch: y[dst + r] = x[src + r] for every segment (CopySeg with stride 1); the segment list
+// is uploaded with the call (replaces one hipMemcpyAsync per block).
+void copy_segments(const std::vector<CopySeg>& segs, const double* x, double* y, hipStream_t s);
 
 // Host vector -> new device buffer (synchronous copy).
 template <typename T>

@@ -41,8 +41,8 @@ struct CachedPlan {
 };
 
 // Plan cache keyed by (op, precision, structure hashes, extra ints); least-recently-used eviction when it holds more
-// than kMax plans or more device memory than the budget (MPSXX_PLAN_CACHE_GB, default 64 GiB: plans of large QC
-// problems hold GB-sized intermediates, one per site).  The kKeep most recently used plans are never evicted, so the
+// than kMax plans or more device memory than the budget (MPSXX_PLAN_CACHE_GB, default 40 % of the device's memory:
+// plans of large QC problems hold GB-sized intermediates, one per site, and a sweep cycles through all of them).  The kKeep most recently used plans are never evicted, so the
 // plans an operation holds at once (a Davidson solve over its operator groups) stay valid.
 class PlanCache {
  public:
@@ -92,9 +92,8 @@ class PlanCache {
   std::chrono::steady_clock::time_point miss_t0_ = std::chrono::steady_clock::now();
   static size_t budget() {
     static const size_t b = [] {
-      double gb = 64.0;
-      if (const char* e = std::getenv("MPSXX_PLAN_CACHE_GB")) gb = std::max(1.0, std::atof(e));
-      return (size_t)(gb * 1073741824.0);
+      if (const char* e = std::getenv("MPSXX_PLAN_CACHE_GB")) return (size_t)(std::max(1.0, std::atof(e)) * 1073741824.0);
+      return device_memory_fraction(0.4, 64.0);
     }();
     return b;
   }
