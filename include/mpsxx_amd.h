@@ -230,9 +230,9 @@ const char* mpsxx_dmrg_last_error(void);
 int mpsxx_plan_cache_stats(int64_t* hits, int64_t* builds, int64_t* evictions, double* build_ms, int64_t* plans,
                            int64_t* device_bytes);
 /* Device allocations of the library since it was loaded: hipMalloc and hipFree calls and the host milliseconds they
- * took, buffers handed out again by the library's caching pool (MPSXX_DEVICE_CACHE_GB, default 20 % of device memory) and the device
- * synchronizations that took, the bytes in use and the bytes the pool holds.  Any pointer may be NULL.  No reference
- * counterpart (diagnostic). */
+ * took, buffers handed out again by the library's caching pool (MPSXX_DEVICE_CACHE_GB, default 20 % of device
+ * memory) and the device synchronizations that took, the bytes in use and the bytes the pool holds.  Any pointer may
+ * be NULL.  No reference counterpart (diagnostic). */
 int mpsxx_device_memory_stats(int64_t* mallocs, int64_t* frees, double* malloc_ms, double* free_ms, int64_t* reuses,
                               int64_t* pool_syncs, int64_t* bytes_live, int64_t* cached_bytes);
 /* Host threads of the contraction planner (threads <= 0: MPSXX_HOST_THREADS or the hardware concurrency, at most 16)

@@ -139,7 +139,8 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   {
     std::vector<CopySeg> segs;
     for (int64_t b = 0; b < wfnc.L.nblocks(); ++b)
-      if (wfnc.L.bsize[b] > 0) segs.push_back({U.off[U.find(wfnc.L.tag[b])], wfnc.L.off[b], (int32_t)wfnc.L.bsize[b], 1});
+      if (wfnc.L.bsize[b] > 0)
+        segs.push_back({U.off[U.find(wfnc.L.tag[b])], wfnc.L.off[b], (int32_t)wfnc.L.bsize[b], 1});
     copy_segments(segs, wfnc.ptr(), T(0), s);
   }
   // lower-root vectors aligned to U (H_k, S_k for every k)

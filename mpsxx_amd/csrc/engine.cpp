@@ -151,7 +151,8 @@ struct DevicePool {
   }
   static size_t limit() {
     static const size_t b = [] {
-      if (const char* e = std::getenv("MPSXX_DEVICE_CACHE_GB")) return (size_t)(std::max(0.0, std::atof(e)) * 1073741824.0);
+      if (const char* e = std::getenv("MPSXX_DEVICE_CACHE_GB"))
+        return (size_t)(std::max(0.0, std::atof(e)) * 1073741824.0);
       return device_memory_fraction(0.2, 32.0);
     }();
     return b;

@@ -92,7 +92,8 @@ class PlanCache {
   std::chrono::steady_clock::time_point miss_t0_ = std::chrono::steady_clock::now();
   static size_t budget() {
     static const size_t b = [] {
-      if (const char* e = std::getenv("MPSXX_PLAN_CACHE_GB")) return (size_t)(std::max(1.0, std::atof(e)) * 1073741824.0);
+      if (const char* e = std::getenv("MPSXX_PLAN_CACHE_GB"))
+        return (size_t)(std::max(1.0, std::atof(e)) * 1073741824.0);
       return device_memory_fraction(0.4, 64.0);
     }();
     return b;
