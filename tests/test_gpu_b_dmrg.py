@@ -73,6 +73,30 @@ def test_canonicalize_large_blocks_matches_oracle(forward, M, solver, monkeypatc
         assert np.max(np.abs(blk - (np.eye(blk.shape[0]) if i == j else 0.0))) < 1e-12
 
 
+@pytest.mark.parametrize("solver", ["jacobi", "rocsolver"])
+def test_canonicalize_rank_deficient_keeps_exact_rank(solver, monkeypatch):
+    """psi = A(l,n1,m) B(m,n2,r) through a middle bond of 2 states per sector has merged blocks of rank <= 2 but
+    dimensions up to ~30: with M = 0 (keep every value >= 1e-16) the device keeps exactly the numerical rank (values
+    below max(m,n) eps |A|_F are reported as zero, not kept as noise directions), discards no weight, and U s Vt
+    reconstructs psi."""
+    print("canonicalize rank-deficient", solver, flush=True)
+    monkeypatch.setenv("MPSXX_SVD", solver)
+    rng = np.random.default_rng(21)
+    L, i, qt = 8, 3, (8, 0)
+    lq, mq, rq = mps.left_bond_qns(L, i, qt), mps.left_bond_qns(L, i + 1, qt), mps.right_bond_qns(L, i + 2, qt)
+    A = mps.fill_allowed(mps.site_tensor(lq, [8] * len(lq), mq, [2] * len(mq)), rng)
+    B = mps.fill_allowed(mps.site_tensor(mq, [2] * len(mq), rq, [8] * len(rq), total=qt), rng)
+    psi = oracle.contract(1.0, A, [2], B, [0])
+    for forward in (True, False):
+        U, V, dw = _canon(forward, psi, 0)
+        kept = U.dshape[-1]
+        assert max(kept) <= 2, kept
+        assert abs(dw) < 1e-20 * max(1.0, psi.norm() ** 2)
+        rec = oracle.contract(1.0, U, [2], V, [0])
+        scale = max(float(np.max(np.abs(b))) for b in psi.blocks.values())
+        assert rec.max_abs_diff(psi) < 1e-11 * scale
+
+
 def test_merged_operator_matches_oracle():
     rng = np.random.default_rng(6)
     L, i = 8, 3
