@@ -239,14 +239,21 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   double eval = 0.0;
   std::vector<double> G(max_ritz * max_ritz), R, coef_h(max_ritz * max_ritz, 0.0);
   // one Davidson solve from the current trial_0 with the sigma arithmetic of `p`
-  auto solve = [&](int p, double tol_p) {
+  auto use_plans = [&](int p) {
     plans.clear();
     for (const DavGroup& g : groups) plans.push_back(&group_plan(g, U, p));
     convert_mask = ~0u;
+  };
+  // have_sigma0: T(0) is normalized and Sg(0) = H T(0) was computed at this precision already
+  auto solve = [&](int p, double tol_p, bool have_sigma0) {
+    use_plans(p);
     int niter = 0, iconv = 0;
     while (iconv < 1 && niter < 4) {
-    dnormalize(T(0));
-    multH(T(0), Sg(0));
+    if (!have_sigma0) {
+      dnormalize(T(0));
+      multH(T(0), Sg(0));
+    }
+    have_sigma0 = false;
     for (int m = 1; m <= max_ritz; ++m) {
       // heff(i,j) = <trial_i|sigma_j>  (diagonalize.hpp:339-365; the overlap matrix is computed but unused there)
       launch_gram(T(0), n, m, Sg(0), n, m, n, gram.as<double>(), part.as<double>(), npart, s);
@@ -299,12 +306,30 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   };
   if (prec == MPSXX_FP32 || prec == MPSXX_BF16X3) {
     // mixed precision: FP32 (or split-BF16) sigma vectors with FP32 accumulation inside each block GEMM until their
-    // noise floor, then FP64 refinement from that vector; Gram-Schmidt, the subspace problem and residuals stay FP64
-    solve(prec, std::max(tol, prec == MPSXX_FP32 ? kMixedTolerance : kMixedToleranceX3));
-    st.nsigma_fp32 = st.nsigma;
-    solve(MPSXX_FP64, tol);
+    // noise floor, then FP64 refinement from that vector; Gram-Schmidt, the subspace problem and residuals stay FP64.
+    // The first application is FP64: a start vector already within the low-precision noise floor (converged sweeps)
+    // is refined in FP64 directly, without a low-precision phase that could not improve it.
+    const double tol_lo = std::max(tol, prec == MPSXX_FP32 ? kMixedTolerance : kMixedToleranceX3);
+    use_plans(MPSXX_FP64);
+    dnormalize(T(0));
+    multH(T(0), Sg(0));
+    double r0 = 0.0;
+    {
+      const double e0 = dot(T(0), Sg(0));
+      launch_residual(Sg(0), T(0), e0, errv, n, gram.as<double>() + kNrm, part.as<double>(), npart, s);
+      hip_check(hipMemcpyAsync(&r0, gram.as<double>() + kNrm, sizeof(double), hipMemcpyDeviceToHost, s), "r0");
+      hip_check(hipStreamSynchronize(s), "r0 sync");
+    }
+    if (r0 < tol_lo) {
+      solve(MPSXX_FP64, tol, true);
+    } else {
+      const int64_t n64 = st.nsigma;
+      solve(prec, tol_lo, false);
+      st.nsigma_fp32 = st.nsigma - n64;
+      solve(MPSXX_FP64, tol, false);
+    }
   } else {
-    solve(MPSXX_FP64, tol);
+    solve(MPSXX_FP64, tol, false);
   }
   // wfnc <- trial_0 orthogonalized against the lower roots' S_k and normalized (Functor::orthonormalize)
   for (int k = 0; k < nlow; ++k) axpy(-dot(Sk(k), T(0)), Sk(k), T(0));

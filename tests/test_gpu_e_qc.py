@@ -75,12 +75,15 @@ def test_device_dmrg_merged_form_matches_fourindex_and_fci(prec):
     init = initial_mps(K, (K, 0), M, 0)
     dm = DeviceDMRG(m.devices(), init, M, form=SWEEP_MERGED)
     e = None
+    nlow = 0
     for _ in range(4):
         e, st = dm.sweep(M, 1e-10, precision=prec)
+        nlow += st["nsigma_fp32"]
     tol = 1e-10 if prec == mx.FP64 else 1e-8
     assert abs(e - e_fci) < tol, (e, e_fci)
     if prec != mx.FP64:
-        assert st["nsigma_fp32"] > 0
+        # low-precision sigma vectors in the early sweeps; a converged site starts in FP64 and stays there
+        assert nlow > 0
 
 
 def test_device_dmrg_merged_form_hubbard_matches_fourindex():
