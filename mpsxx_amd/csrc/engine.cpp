@@ -862,32 +862,28 @@ void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>&
     cost[ti] = ksum;
   }
   std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
-  std::vector<size_t> bigtasks;
-  int64_t nbig = 0, ubig = 0;
+  std::vector<size_t> bigtasks, splittasks;
+  int64_t nbig = 0, ubig = 0, usplit = 0;
   for (size_t ti : order) {
     const GemmTask& t = tasks[ti];
+    const int64_t units = units_of(t.pair_begin, t.pair_begin + t.pair_count);
     if (t.m <= pol.small_limit && t.n <= pol.small_limit) {
-      const int64_t units = units_of(t.pair_begin, t.pair_begin + t.pair_count);
-      std::vector<GemmTile>& dst = (t.pair_count >= 2 && units >= pol.split_units) ? out.split : out.small;
+      if (t.pair_count >= 2 && units >= pol.split_units) {
+        splittasks.push_back(ti);
+        usplit += (int64_t)((t.m + kSmallTile - 1) / kSmallTile) * ((t.n + kSmallTile - 1) / kSmallTile) * units;
+        continue;
+      }
       for (int r = 0; r < t.m; r += kSmallTile)
-        for (int c = 0; c < t.n; c += kSmallTile) dst.push_back({(int32_t)ti, r, c, 0});
+        for (int c = 0; c < t.n; c += kSmallTile) out.small.push_back({(int32_t)ti, r, c, 0});
       continue;
     }
     bigtasks.push_back(ti);
     const int64_t nt = (int64_t)((t.m + kGemmTile - 1) / kGemmTile) * ((t.n + kGemmTile - 1) / kGemmTile);
     nbig += nt;
-    ubig += nt * units_of(t.pair_begin, t.pair_begin + t.pair_count);
+    ubig += nt * units;
   }
-  // split the pair lists of long reductions when the big tiles alone cannot fill the device
-  const bool split = pol.split_big > 0 && nbig > 0 && nbig < pol.split_big;
-  const int64_t target = split ? std::max<int64_t>(8, (ubig + pol.split_big - 1) / pol.split_big) : 0;
-  for (size_t ti : bigtasks) {
-    const GemmTask& t = tasks[ti];
-    const int lo = t.pair_begin, hi = t.pair_begin + t.pair_count;
-    const int64_t units = units_of(lo, hi);
-    int nseg = 1;
-    if (split) nseg = (int)std::min<int64_t>({64, (int64_t)t.pair_count, std::max<int64_t>(1, (units + target - 1) / target)});
-    // segment boundaries with about units/nseg units each
+  // boundaries of nseg pair ranges of [lo, hi) with about units/nseg units each
+  auto cuts = [&](int lo, int hi, int64_t units, int nseg) {
     std::vector<int> cut{lo};
     if (nseg > 1) {
       int64_t acc = 0;
@@ -901,6 +897,19 @@ void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>&
       }
     }
     cut.push_back(hi);
+    return cut;
+  };
+  // split the pair lists of long reductions when the big tiles alone cannot fill the device
+  const bool split = pol.split_big > 0 && nbig > 0 && nbig < pol.split_big;
+  const int64_t target = split ? std::max<int64_t>(8, (ubig + pol.split_big - 1) / pol.split_big) : 0;
+  for (size_t ti : bigtasks) {
+    const GemmTask& t = tasks[ti];
+    const int lo = t.pair_begin, hi = t.pair_begin + t.pair_count;
+    const int64_t units = units_of(lo, hi);
+    int nseg = 1;
+    if (split)
+      nseg = (int)std::min<int64_t>({64, (int64_t)t.pair_count, std::max<int64_t>(1, (units + target - 1) / target)});
+    const std::vector<int> cut = cuts(lo, hi, units, nseg);
     const int nparts = (int)cut.size() - 1;
     for (int r = 0; r < t.m; r += kGemmTile)
       for (int c = 0; c < t.n; c += kGemmTile) {
@@ -911,6 +920,30 @@ void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>&
         out.red.push_back({(int32_t)ti, r, c, out.nslots, nparts, 0});
         for (int q = 0; q < nparts; ++q) out.big.push_back({(int32_t)ti, r, c, cut[q], cut[q + 1], out.nslots + q});
         out.nslots += nparts;
+      }
+  }
+  // the split 32x32 tiles: a tile whose reduction is much longer than the list's share per work-group
+  // (split_small work-groups in all) is cut into pair ranges of >= kSplitSegUnits units, one work-group each
+  const int64_t starget =
+      pol.split_small > 0 ? std::max<int64_t>(kSplitSegUnits, (usplit + pol.split_small - 1) / pol.split_small) : 0;
+  for (size_t ti : splittasks) {
+    const GemmTask& t = tasks[ti];
+    const int lo = t.pair_begin, hi = t.pair_begin + t.pair_count;
+    const int64_t units = units_of(lo, hi);
+    int nseg = 1;
+    if (starget > 0) nseg = (int)std::min<int64_t>({64, (int64_t)t.pair_count, std::max<int64_t>(1, units / starget)});
+    const std::vector<int> cut = cuts(lo, hi, units, nseg);
+    const int nparts = (int)cut.size() - 1;
+    for (int r = 0; r < t.m; r += kSmallTile)
+      for (int c = 0; c < t.n; c += kSmallTile) {
+        if (nparts == 1) {
+          out.split.push_back({(int32_t)ti, r, c, lo, hi, -1});
+          continue;
+        }
+        out.split_red.push_back({(int32_t)ti, r, c, out.split_slots, nparts, 0});
+        for (int q = 0; q < nparts; ++q)
+          out.split.push_back({(int32_t)ti, r, c, cut[q], cut[q + 1], out.split_slots + q});
+        out.split_slots += nparts;
       }
   }
 }
@@ -929,6 +962,9 @@ void set_gemm_policy(Step& st, const GemmPolicy& pol) {
   st.work_small = upload_vec(g.small);
   st.nwork_split = (int)g.split.size();
   st.work_split = upload_vec(g.split);
+  st.nwork_split_red = (int)g.split_red.size();
+  st.work_split_red = upload_vec(g.split_red);
+  st.split_partial.alloc((size_t)g.split_slots * kSmallTile * kSmallTile * sizeof(double));
 }
 
 bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, int y, int w1, int w2, int z,
@@ -1025,7 +1061,8 @@ void launch_step_t(const Step& st, const Operands& ops, unsigned parts, bool x3,
       launch_gemm_small<T, TC>(st.work_small.as<GemmTile>(), st.nwork_small, st.tasks.as<GemmTask>(),
                                st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
     if (parts & 4u)
-      launch_gemm_split<T, TC>(st.work_split.as<GemmTile>(), st.nwork_split, st.tasks.as<GemmTask>(),
+      launch_gemm_split<T, TC>(st.work_split.as<BigTile>(), st.nwork_split, st.work_split_red.as<RedTile>(),
+                               st.nwork_split_red, st.split_partial.p, st.tasks.as<GemmTask>(),
                                st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
   } else if (parts & 1u) {
     launch_axpy<T, TC>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(), st.pairs.as<AxpyPair>(), ops,
@@ -1083,10 +1120,14 @@ std::vector<KernelPart> step_parts(const Step& st) {
   }
   if (!g.small.empty()) out.push_back({KernelPart::GEMM_SMALL, 2u, (int)g.small.size(), fl, by});
   fl = by = 0;
-  for (const GemmTile& tl : g.split) {
+  for (const BigTile& tl : g.split) {
     const GemmTask& t = st.host.gtasks[tl.task];
-    tile_cost(tl.task, tl.row0, tl.col0, t.pair_begin, t.pair_begin + t.pair_count, kSmallTile, fl, by);
-    by += 8 * (int64_t)std::min(kSmallTile, t.m - tl.row0) * std::min(kSmallTile, t.n - tl.col0);
+    tile_cost(tl.task, tl.row0, tl.col0, tl.pair_lo, tl.pair_hi, kSmallTile, fl, by);
+    if (tl.slot < 0) by += 8 * (int64_t)std::min(kSmallTile, t.m - tl.row0) * std::min(kSmallTile, t.n - tl.col0);
+  }
+  for (const RedTile& rt : g.split_red) {
+    const GemmTask& t = st.host.gtasks[rt.task];
+    by += 8 * (int64_t)std::min(kSmallTile, t.m - rt.row0) * std::min(kSmallTile, t.n - rt.col0);
   }
   if (!g.split.empty()) out.push_back({KernelPart::GEMM_SPLIT, 4u, (int)g.split.size(), fl, by});
   return out;

@@ -116,14 +116,18 @@ struct HostLists {
 // How the blocks of a GEMM step are mapped to kernels: blocks with both sides <= small_limit run as 32x32 tiles, split
 // over the 4 waves of a work-group when they reduce over >= 2 pairs and >= split_units (pair, 16-wide k panel) units,
 // one wave per tile otherwise; larger blocks run as 64x64 LDS tiles, whose pair lists are split over several
-// work-groups (partial tiles + an ordered sum) while the step has fewer than split_big such tiles (0: never).
-// max_groups > 0 caps the grid of every launch.
+// work-groups (partial tiles + an ordered sum) while the step has fewer than split_big such tiles (0: never).  The
+// split 32x32 tiles are cut the same way into pair ranges of at least kSplitSegUnits units, each on its own
+// work-group, until the list holds about split_small work-groups (0: never).  max_groups > 0 caps the grid of every
+// launch.
 struct GemmPolicy {
   int small_limit = 64;
   int split_units = 2;
   int max_groups = 0;
   int split_big = 2048;
+  int split_small = 1024;
 };
+constexpr int kSplitSegUnits = 32;
 GemmPolicy default_policy();  // the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT / MPSXX_SPLIT_UNITS
 
 // Tile lists of one GEMM step (heaviest blocks first in each list).
@@ -131,7 +135,10 @@ struct GemmTiling {
   std::vector<BigTile> big;
   std::vector<RedTile> red;
   int nslots = 0;  // partial 64x64 tiles of the split big reductions
-  std::vector<GemmTile> small, split;
+  std::vector<GemmTile> small;
+  std::vector<BigTile> split;      // 32x32 tiles over a pair range; slot >= 0: partial 32x32 tile
+  std::vector<RedTile> split_red;  // 32x32 tiles whose partial sums are added in slot order
+  int split_slots = 0;
 };
 void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>& pairs, const GemmPolicy& pol,
                 GemmTiling& out);
@@ -149,8 +156,10 @@ struct Step {
   DevBuf work_red, partial;
   int nwork_small = 0;         // GEMM: 32x32 tiles of the small blocks (gemm_small_kernel)
   DevBuf work_small;
-  int nwork_split = 0;         // GEMM: 32x32 tiles with long reductions (gemm_split_kernel)
+  int nwork_split = 0;         // GEMM: 32x32 tiles with long reductions (gemm_split_kernel, BigTile)
   DevBuf work_split;
+  int nwork_split_red = 0;     // GEMM: split 32x32 tiles cut over work-groups (RedTile) and their partial buffer
+  DevBuf work_split_red, split_partial;
   int max_groups = 0;          // grid cap of the step's launches (0: one work-group per work item)
   GemmPolicy policy;           // GEMM: how the blocks were mapped to the tile lists
   int64_t flops = 0;
@@ -188,7 +197,7 @@ struct Plan {
     for (const DevBuf& b : slot_buf) n += b.bytes;
     for (const Step& st : steps)
       n += st.work.bytes + st.tasks.bytes + st.pairs.bytes + st.work_red.bytes + st.partial.bytes +
-           st.work_small.bytes + st.work_split.bytes;
+           st.work_small.bytes + st.work_split.bytes + st.work_split_red.bytes + st.split_partial.bytes;
     return n;
   }
 };

@@ -218,29 +218,45 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const BigTile* __rest
   }
 }
 
-// Sum of the partial tiles of split reductions, in slot order (deterministic), into C.
+// Sum of the partial tiles of split reductions, in slot order (deterministic), into C: one work-group per 16-row
+// strip of a tile (kRedStrips per tile), four consecutive elements per thread read as one vector, the slot loop
+// unrolled so that several loads are in flight.
+constexpr int kRedStrips = BM / 16;
 template <typename T, typename TC, bool Tab>
 __global__ __launch_bounds__(256) void gemm_reduce_kernel(const RedTile* __restrict__ red, int nred,
                                                           const GemmTask* __restrict__ tasks,
                                                           const T* __restrict__ partial, Operands ops, double alpha,
                                                           int accumulate) {
-  for (int w = blockIdx.x; w < nred; w += gridDim.x) {
-    const RedTile rt = red[w];
+  typedef typename Mfma<T>::acc_t vec_t;
+  for (int w = blockIdx.x; w < nred * kRedStrips; w += gridDim.x) {
+    const RedTile rt = red[w / kRedStrips];
     const GemmTask tk = tasks[rt.task];
-    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
-    for (int e = threadIdx.x; e < BM * BN; e += 256) {
-      const int row = rt.row0 + (e >> 6), col = rt.col0 + (e & 63);
-      if (row >= tk.m || col >= tk.n) continue;
-      double sum = 0.0;
-      for (int q = 0; q < rt.nslots; ++q) sum += (double)partial[(int64_t)(rt.slot0 + q) * (BM * BN) + e];
-      const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
-      const double v = alpha * sum;
-      C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
+    const int lr = (w % kRedStrips) * 16 + (threadIdx.x >> 4), lc = (threadIdx.x & 15) * 4;
+    const int row = rt.row0 + lr, col = rt.col0 + lc;
+    if (row >= tk.m || col >= tk.n) continue;
+    const vec_t* __restrict__ P =
+        reinterpret_cast<const vec_t*>(partial + (int64_t)rt.slot0 * (BM * BN) + lr * BN + lc);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll 4
+    for (int q = 0; q < rt.nslots; ++q) {
+      const vec_t v = P[(int64_t)q * (BM * BN / 4)];
+      s0 += (double)v[0];
+      s1 += (double)v[1];
+      s2 += (double)v[2];
+      s3 += (double)v[3];
     }
+    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
+    const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+    const double sum[4] = {s0, s1, s2, s3};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (col + j < tk.n) {
+        const double v = alpha * sum[j];
+        C[idx + j] = accumulate ? (TC)((double)C[idx + j] + v) : (TC)v;
+      }
   }
 }
 
-// work-groups of a task-list launch: one per work item, or at most max_groups (> 0) looping over the list
 static inline int groups(int items, int max_groups) { return max_groups > 0 && items > max_groups ? max_groups : items; }
 
 template <typename T, typename TC>
@@ -256,11 +272,11 @@ void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred,
                        ntiles, tasks, pairs, ops, alpha, accumulate, part);
   if (nred <= 0) return;
   if (ops.table)
-    hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, true>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red, nred,
-                       tasks, part, ops, alpha, accumulate);
+    hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, true>), dim3(groups(nred * kRedStrips, max_groups)), dim3(256), 0,
+                       s, red, nred, tasks, part, ops, alpha, accumulate);
   else
-    hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, false>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red, nred,
-                       tasks, part, ops, alpha, accumulate);
+    hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, false>), dim3(groups(nred * kRedStrips, max_groups)), dim3(256), 0,
+                       s, red, nred, tasks, part, ops, alpha, accumulate);
 }
 
 }  // namespace mpx

@@ -356,15 +356,17 @@ constexpr int kSplitPairs = 128;  // pair descriptors of one task staged in LDS 
 constexpr int kSplitWaves = 4;    // waves sharing the reduction of one 32x32 tile
 
 // K-split variant for 32x32 tiles with a long reduction (many pairs and/or large k): the four waves of a work-group
-// take every fourth (pair, 16-wide k panel) unit of the same tile (each wave loading its next unit while the MFMAs of
-// the current one run), then the partial tiles are summed in LDS in a fixed order (deterministic) and stored once.
-// The task's pair descriptors are first staged in LDS by the whole work-group (rounds of kSplitPairs), so the unit
-// loop waits only on operand loads, not on a descriptor load before each operand load.
+// take every fourth (pair, 16-wide k panel) unit of the tile's pair range (each wave loading its next unit while the
+// MFMAs of the current one run), then the partial tiles are summed in LDS in a fixed order (deterministic) and stored
+// once: into C, or, for a tile whose reduction is cut over several work-groups (slot >= 0), unscaled into the 32x32
+// row-major partial tile `slot`, which gemm_reduce32_kernel adds in slot order.  The pair descriptors are first
+// staged in LDS by the whole work-group (rounds of kSplitPairs), so the unit loop waits only on operand loads, not on
+// a descriptor load before each operand load.
 template <typename T, typename TC, bool Tab, int NW>
-__global__ __launch_bounds__(64 * NW) void gemm_split_kernel(const GemmTile* __restrict__ tiles, int ntiles,
+__global__ __launch_bounds__(64 * NW) void gemm_split_kernel(const BigTile* __restrict__ tiles, int ntiles,
                                                          const GemmTask* __restrict__ tasks,
                                                          const GemmPair* __restrict__ pairs, Operands ops,
-                                                         double alpha, int accumulate) {
+                                                         double alpha, int accumulate, T* __restrict__ partial) {
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T red[NW - 1][4][4][64];  // partial tiles of waves 1..NW-1: [wave-1][mi*2+ni][r][lane]
   __shared__ GemmPair sp[kSplitPairs];
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(64 * NW) void gemm_split_kernel(const GemmTile* __r
   const int lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    const GemmTile tl = tiles[w];
+    const BigTile tl = tiles[w];
     const GemmTask tk = tasks[tl.task];
     const T* __restrict__ A = opnd<const T, Tab>(ops.a, tk.inst);
     const T* __restrict__ B = opnd<const T, Tab>(ops.b, tk.inst);
@@ -391,9 +393,9 @@ __global__ __launch_bounds__(64 * NW) void gemm_split_kernel(const GemmTile* __r
     tlanes.vc0 = tlanes.c0 < tk.n;
     tlanes.vc1 = tlanes.c1 < tk.n;
     tlanes.lk = lk;
-    const int pend = tk.pair_begin + tk.pair_count;
-    int ubase = 0;  // units of the task before this round, modulo NW
-    for (int c0 = tk.pair_begin; c0 < pend; c0 += kSplitPairs) {  // uniform across the work-group
+    const int pend = tl.pair_hi;
+    int ubase = 0;  // units of the range before this round, modulo NW
+    for (int c0 = tl.pair_lo; c0 < pend; c0 += kSplitPairs) {  // uniform across the work-group
       const int nc = min(kSplitPairs, pend - c0);
       __syncthreads();  // the previous round's descriptors are no longer read
       for (int i = threadIdx.x; i < nc; i += 64 * NW) sp[i] = pairs[c0 + i];
@@ -443,9 +445,59 @@ __global__ __launch_bounds__(64 * NW) void gemm_split_kernel(const GemmTile* __r
             for (int q = 0; q < NW - 1; ++q) v = v + red[q][mi * 2 + ni][r][lane];
             acc[mi][ni][r] = v;
           }
-      store_tile<T, TC>(opnd<TC, Tab>(ops.c, tk.inst), tk, tl, acc, lr, lk, alpha, accumulate);
+      if (tl.slot >= 0) {
+        T* __restrict__ P = partial + (int64_t)tl.slot * (kSmallTile * kSmallTile);
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              P[(mi * 16 + Mfma<T>::row(lk, r)) * kSmallTile + ni * 16 + lr] = acc[mi][ni][r];
+      } else {
+        const GemmTile ct{tl.task, tl.row0, tl.col0, 0};
+        store_tile<T, TC>(opnd<TC, Tab>(ops.c, tk.inst), tk, ct, acc, lr, lk, alpha, accumulate);
+      }
     }
     __syncthreads();  // red is rewritten by the next tile
+  }
+}
+
+// Sum of the partial 32x32 tiles of a cut reduction in slot order, scaled and stored (or accumulated) into C: four
+// consecutive elements per thread read as one vector, the slot loop unrolled so that several loads are in flight.
+template <typename T, typename TC, bool Tab>
+__global__ __launch_bounds__(256) void gemm_reduce32_kernel(const RedTile* __restrict__ red, int nred,
+                                                            const GemmTask* __restrict__ tasks,
+                                                            const T* __restrict__ partial, Operands ops, double alpha,
+                                                            int accumulate) {
+  typedef typename Mfma<T>::acc_t vec_t;
+  constexpr int kTile = kSmallTile * kSmallTile;
+  for (int w = blockIdx.x; w < nred; w += gridDim.x) {
+    const RedTile rt = red[w];
+    const GemmTask tk = tasks[rt.task];
+    const int lr = threadIdx.x >> 3, lc = (threadIdx.x & 7) * 4;
+    const int row = rt.row0 + lr, col = rt.col0 + lc;
+    if (row >= tk.m || col >= tk.n) continue;
+    const vec_t* __restrict__ P =
+        reinterpret_cast<const vec_t*>(partial + (int64_t)rt.slot0 * kTile + lr * kSmallTile + lc);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll 4
+    for (int q = 0; q < rt.nslots; ++q) {
+      const vec_t v = P[(int64_t)q * (kTile / 4)];
+      s0 += (double)v[0];
+      s1 += (double)v[1];
+      s2 += (double)v[2];
+      s3 += (double)v[3];
+    }
+    TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
+    const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
+    const double sum[4] = {s0, s1, s2, s3};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (col + j < tk.n) {
+        const double v = alpha * sum[j];
+        C[idx + j] = accumulate ? (TC)((double)C[idx + j] + v) : (TC)v;
+      }
   }
 }
 
@@ -764,11 +816,11 @@ void launch_gemm_x3(const BigTile* tiles, int ntiles, const RedTile* red, int nr
                        tasks, pairs, ops, alpha, accumulate, part);
   if (nred <= 0) return;
   if (ops.table)
-    hipLaunchKernelGGL((gemm_reduce_kernel<float, TC, true>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red,
-                       nred, tasks, part, ops, alpha, accumulate);
+    hipLaunchKernelGGL((gemm_reduce_kernel<float, TC, true>), dim3(groups(nred * kRedStrips, max_groups)), dim3(256),
+                       0, s, red, nred, tasks, part, ops, alpha, accumulate);
   else
-    hipLaunchKernelGGL((gemm_reduce_kernel<float, TC, false>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red,
-                       nred, tasks, part, ops, alpha, accumulate);
+    hipLaunchKernelGGL((gemm_reduce_kernel<float, TC, false>), dim3(groups(nred * kRedStrips, max_groups)), dim3(256),
+                       0, s, red, nred, tasks, part, ops, alpha, accumulate);
 }
 template void launch_gemm_x3<float>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*, const GemmPair*,
                                     Operands, double, int, hipStream_t, int);
@@ -1558,17 +1610,24 @@ void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks,
 }
 
 template <typename T, typename TC>
-void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
-                       double alpha, int accumulate, hipStream_t s, int max_groups) {
+void launch_gemm_split(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
+                       const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
+                       hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
-  if (ops.table)
+  T* part = static_cast<T*>(partial);
+  if (ops.table) {
     hipLaunchKernelGGL((gemm_split_kernel<T, TC, true, kSplitWaves>), dim3(groups(ntiles, max_groups)),
-                       dim3(64 * kSplitWaves), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate);
-  else
+                       dim3(64 * kSplitWaves), 0, s, tiles, ntiles, tasks, pairs, ops, alpha, accumulate, part);
+    if (nred > 0)
+      hipLaunchKernelGGL((gemm_reduce32_kernel<T, TC, true>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red,
+                         nred, tasks, part, ops, alpha, accumulate);
+  } else {
     hipLaunchKernelGGL((gemm_split_kernel<T, TC, false, kSplitWaves>), dim3(groups(ntiles, max_groups)),
-                       dim3(64 * kSplitWaves), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate);
+                       dim3(64 * kSplitWaves), 0, s, tiles, ntiles, tasks, pairs, ops, alpha, accumulate, part);
+    if (nred > 0)
+      hipLaunchKernelGGL((gemm_reduce32_kernel<T, TC, false>), dim3(groups(nred, max_groups)), dim3(256), 0, s, red,
+                         nred, tasks, part, ops, alpha, accumulate);
+  }
 }
 
 #define MPX_INST_BIG(T, TC)                                                                                   \
@@ -1583,10 +1642,14 @@ MPX_INST_BIG(float, double)
 MPX_INST_GEMM(launch_gemm_small, double, double)
 MPX_INST_GEMM(launch_gemm_small, float, float)
 MPX_INST_GEMM(launch_gemm_small, float, double)
-MPX_INST_GEMM(launch_gemm_split, double, double)
-MPX_INST_GEMM(launch_gemm_split, float, float)
-MPX_INST_GEMM(launch_gemm_split, float, double)
 #undef MPX_INST_GEMM
+#define MPX_INST_SPLIT(T, TC)                                                                                     \
+  template void launch_gemm_split<T, TC>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*,       \
+                                         const GemmPair*, Operands, double, int, hipStream_t, int);
+MPX_INST_SPLIT(double, double)
+MPX_INST_SPLIT(float, float)
+MPX_INST_SPLIT(float, double)
+#undef MPX_INST_SPLIT
 template void launch_axpy<double, double>(const Chunk*, int, const AxpyTask*, const AxpyPair*, Operands, double, int,
                                           hipStream_t, int);
 template void launch_axpy<float, float>(const Chunk*, int, const AxpyTask*, const AxpyPair*, Operands, double, int,

@@ -111,10 +111,13 @@ template <typename T, typename TC>
 void launch_gemm_small(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
                        double alpha, int accumulate, hipStream_t s, int max_groups = 0);
 
-// 32x32 tiles with the (pair, k-panel) reduction split over the 4 waves of a work-group, LDS sum at the end.
+// 32x32 tiles with the (pair, k-panel) reduction split over the 4 waves of a work-group, LDS sum at the end.  The
+// tiles cover pair ranges; those with slot >= 0 write partial 32x32 tiles (T, row-major) that are then summed in slot
+// order over `red` into C.
 template <typename T, typename TC>
-void launch_gemm_split(const GemmTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
-                       double alpha, int accumulate, hipStream_t s, int max_groups = 0);
+void launch_gemm_split(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
+                       const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
+                       hipStream_t s, int max_groups = 0);
 
 // ops.a = V (vector side), ops.b = S (coefficients), ops.b2 = second coefficients of fused pairs, ops.c = C
 template <typename T, typename TC>
