@@ -224,11 +224,12 @@ const char* mpsxx_dmrg_last_error(void);
 
 /* ---- planner introspection (host only, no device work) ---------------------------------------------------------- */
 /* Counters of the plan cache since the library was loaded: lookups served from the cache, plans built and the host
- * milliseconds spent building them (device allocation included), plans evicted (MPSXX_PLAN_CACHE_GB bounds the cache's
- * device memory), and the plans and device bytes it holds now.  Any pointer may be NULL.  No reference counterpart
- * (the reference recomputes every contraction's structure on each call). */
+ * milliseconds spent building them (device allocation included), plans evicted, the plans and device bytes it holds
+ * now, and how often a plan released its intermediate buffers to keep the cache's device memory within
+ * MPSXX_PLAN_CACHE_GB (keeping its task lists) and allocated them again when next used.  Any pointer may be NULL.  No
+ * reference counterpart (the reference recomputes every contraction's structure on each call). */
 int mpsxx_plan_cache_stats(int64_t* hits, int64_t* builds, int64_t* evictions, double* build_ms, int64_t* plans,
-                           int64_t* device_bytes);
+                           int64_t* device_bytes, int64_t* releases, int64_t* restores);
 /* Device allocations of the library since it was loaded: hipMalloc and hipFree calls and the host milliseconds they
  * took, buffers handed out again by the library's caching pool (MPSXX_DEVICE_CACHE_GB, default 20 % of device
  * memory) and the device synchronizations that took, the bytes in use and the bytes the pool holds.  Any pointer may

@@ -45,13 +45,15 @@ VP = ctypes.c_void_p
 
 
 def plan_cache_stats():
-    """Plan cache counters (mpsxx_plan_cache_stats): hits, builds, evictions, build_ms, plans, device_bytes."""
-    i = [ctypes.c_int64() for _ in range(5)]
+    """Plan cache counters (mpsxx_plan_cache_stats): hits, builds, evictions, build_ms, plans, device_bytes,
+    releases, restores."""
+    i = [ctypes.c_int64() for _ in range(7)]
     ms = ctypes.c_double()
     _check(lib().mpsxx_plan_cache_stats(ctypes.byref(i[0]), ctypes.byref(i[1]), ctypes.byref(i[2]), ctypes.byref(ms),
-                                        ctypes.byref(i[3]), ctypes.byref(i[4])))
+                                        ctypes.byref(i[3]), ctypes.byref(i[4]), ctypes.byref(i[5]),
+                                        ctypes.byref(i[6])))
     return {"hits": i[0].value, "builds": i[1].value, "evictions": i[2].value, "build_ms": ms.value,
-            "plans": i[3].value, "device_bytes": i[4].value}
+            "plans": i[3].value, "device_bytes": i[4].value, "releases": i[5].value, "restores": i[6].value}
 
 
 def device_memory_stats():

@@ -477,7 +477,7 @@ int mpsxx_qc_mpo_create(int32_t norb, const double* oneint, const double* twoint
 }
 
 int mpsxx_plan_cache_stats(int64_t* hits, int64_t* builds, int64_t* evictions, double* build_ms, int64_t* plans,
-                           int64_t* device_bytes) {
+                           int64_t* device_bytes, int64_t* releases, int64_t* restores) {
   return guard_host([&] {
     const mpx::PlanCacheStats st = mpx::plan_cache_stats();
     if (hits) *hits = st.hits;
@@ -486,6 +486,8 @@ int mpsxx_plan_cache_stats(int64_t* hits, int64_t* builds, int64_t* evictions, d
     if (build_ms) *build_ms = st.build_ms;
     if (plans) *plans = st.plans;
     if (device_bytes) *device_bytes = st.device_bytes;
+    if (releases) *releases = st.releases;
+    if (restores) *restores = st.restores;
   });
 }
 

@@ -1153,8 +1153,27 @@ void run_plan_timed(Plan& P, const std::vector<const double*>& inputs, double* o
   run_plan_ex(P, inputs, out, s, events, ~0u);
 }
 
+void Plan::strip() {
+  for (DevBuf& b : slot_buf) b.release();
+  stripped = true;
+}
+
+bool Plan::restore() {
+  if (!stripped) return false;
+  bool any = false;
+  for (size_t i = 0; i < slot_layout.size(); ++i) {
+    const bool inter = slot_input[i] == -1, f32_copy = slot_input[i] >= 0 && slot_prec[i] == 1;
+    if (!(inter || f32_copy) || slot_buf[i].p) continue;
+    slot_buf[i].alloc(std::max<int64_t>(slot_layout[i].nelem, 1) * (slot_prec[i] ? sizeof(float) : sizeof(double)));
+    any = true;
+  }
+  stripped = false;
+  return any;
+}
+
 std::vector<void*> slot_pointers(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
                                  unsigned convert_mask) {
+  if (P.restore()) convert_mask = ~0u;  // fresh FP32 operand copies hold nothing yet
   const int ns = (int)P.slot_layout.size();
   std::vector<void*> ptr(ns, nullptr);
   for (int i = 0; i < ns; ++i) {

@@ -191,6 +191,11 @@ struct Plan {
   int64_t flops = 0;                   // algorithmic flops (reference order)
   int64_t bytes = 0;                   // algorithmic bytes (inputs + output, FP64)
   int precision = 0;
+  bool stripped = false;               // slot buffers released by strip(); run_plan re-allocates them first
+  // Release the slot buffers (intermediates, FP32 operand copies), keeping the task lists; restore() allocates them
+  // again (FP32 operand copies must then be converted again).  Returns whether anything was allocated.
+  void strip();
+  bool restore();
   // device memory held by the plan (intermediates, FP32 operand copies, task lists)
   size_t device_bytes() const {
     size_t n = 0;

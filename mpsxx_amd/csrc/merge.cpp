@@ -5,6 +5,10 @@
 // appearance, and the packed sectors of one merged sector are stacked in packed order.  QSTmerge
 // (QSTmerge.h:31-140) creates a merged block when at least one of its sub-blocks is stored and zero-fills the rest.
 #include <algorithm>
+#include <array>
+#include <list>
+#include <memory>
+#include <mutex>
 #include <cstdlib>
 #include <map>
 #include <unordered_map>
@@ -316,23 +320,94 @@ bool merged_operator_plan(bool fwd, const Layout& W, const Layout& E, MergedOpPl
 
 namespace {
 
+// Device copy of a scalar-MPO merged-operator plan.  Its structure depends only on the direction and the MPO and
+// environment layouts, which repeat from one sweep to the next once the bond dimensions settle, so the plans are kept
+// in an LRU cache of bounded device size (1% of device memory, at most kMergedKeep plans).
+struct MergedOpDevice {
+  Layout Y;
+  int nw = 0, nchunks = 0;
+  DevBuf tasks, chunks, slice_blk, env_off, col_begin, ent_w, ent_off;
+  size_t bytes() const {
+    return tasks.bytes + chunks.bytes + slice_blk.bytes + env_off.bytes + col_begin.bytes + ent_w.bytes + ent_off.bytes;
+  }
+};
+
+constexpr size_t kMergedKeep = 256;
+
+struct MergedOpCache {
+  typedef std::array<uint64_t, 3> Key;
+  typedef std::list<std::pair<Key, std::shared_ptr<MergedOpDevice>>> List;
+  std::mutex mu;
+  List lru;  // most recently used first
+  std::map<Key, List::iterator> idx;
+  size_t bytes = 0;
+  std::shared_ptr<MergedOpDevice> find(const Key& k) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = idx.find(k);
+    if (it == idx.end()) return nullptr;
+    lru.splice(lru.begin(), lru, it->second);
+    return it->second->second;
+  }
+  void insert(const Key& k, std::shared_ptr<MergedOpDevice> v) {
+    static const size_t budget = device_memory_fraction(0.01, 2.0);
+    std::lock_guard<std::mutex> lk(mu);
+    if (idx.count(k)) return;
+    bytes += v->bytes();
+    lru.emplace_front(k, std::move(v));
+    idx[k] = lru.begin();
+    while (lru.size() > 1 && (bytes > budget || lru.size() > kMergedKeep)) {
+      bytes -= lru.back().second->bytes();
+      idx.erase(lru.back().first);
+      lru.pop_back();
+    }
+  }
+  void clear() {
+    std::lock_guard<std::mutex> lk(mu);
+    idx.clear();
+    lru.clear();
+    bytes = 0;
+  }
+};
+
+MergedOpCache& merged_cache() {
+  static MergedOpCache c;
+  return c;
+}
+
 bool merged_operator_scalar(bool fwd, Tensor& xmpo, Tensor& xopr, Tensor& mopr, hipStream_t s) {
-  MergedOpPlan P;
-  if (!merged_operator_plan(fwd, xmpo.L, xopr.L, P)) return false;
-  mopr.set_layout(P.Y, s);  // zero: sub-blocks no pair reaches stay zero, as QSTmerge leaves them
-  if (!P.chunks.empty()) {
-    DevBuf dt = upload_vec(P.tasks), dc = upload_vec(P.chunks), dsb = upload_vec(P.slice_blk),
-           deo = upload_vec(P.env_off), dcb = upload_vec(P.col_begin), dew = upload_vec(P.ent_w),
-           deff = upload_vec(P.ent_off);
-    launch_merged_operator(dc.as<Chunk>(), (int)P.chunks.size(), dt.as<MergedOpTask>(), dsb.as<int32_t>(), P.nw,
-                           deo.as<int64_t>(), dcb.as<int32_t>(), dew.as<int32_t>(), deff.as<int64_t>(), xopr.ptr(),
-                           xmpo.ptr(), mopr.ptr(), s);
+  const MergedOpCache::Key key{fwd ? 1u : 0u, xmpo.L.hash, xopr.L.hash};
+  std::shared_ptr<MergedOpDevice> D = merged_cache().find(key);
+  if (!D) {
+    MergedOpPlan P;
+    if (!merged_operator_plan(fwd, xmpo.L, xopr.L, P)) return false;
+    D = std::make_shared<MergedOpDevice>();
+    D->Y = std::move(P.Y);
+    D->nw = P.nw;
+    D->nchunks = (int)P.chunks.size();
+    if (D->nchunks > 0) {
+      D->tasks = upload_vec(P.tasks);
+      D->chunks = upload_vec(P.chunks);
+      D->slice_blk = upload_vec(P.slice_blk);
+      D->env_off = upload_vec(P.env_off);
+      D->col_begin = upload_vec(P.col_begin);
+      D->ent_w = upload_vec(P.ent_w);
+      D->ent_off = upload_vec(P.ent_off);
+    }
+    merged_cache().insert(key, D);
+  }
+  mopr.set_layout(D->Y, s);  // zero: sub-blocks no pair reaches stay zero, as QSTmerge leaves them
+  if (D->nchunks > 0) {
+    launch_merged_operator(D->chunks.as<Chunk>(), D->nchunks, D->tasks.as<MergedOpTask>(), D->slice_blk.as<int32_t>(),
+                           D->nw, D->env_off.as<int64_t>(), D->col_begin.as<int32_t>(), D->ent_w.as<int32_t>(),
+                           D->ent_off.as<int64_t>(), xopr.ptr(), xmpo.ptr(), mopr.ptr(), s);
     hip_check(hipStreamSynchronize(s), "merged operator");
   }
   return true;
 }
 
 }  // namespace
+
+void clear_merged_operator_cache() { merged_cache().clear(); }
 
 // Merged block operator (reference src/optimize.hpp:273-368), one operator group:
 //   forward : L~(Lb, w', Lk) = merge_{(lb,nb)} merge_{(lk,nk)} sum_w L(lb,w,lk) W(w,nb,nk,w')

@@ -40,10 +40,14 @@ struct CachedPlan {
   size_t charged = 0;  // device bytes counted against the cache budget at insertion
 };
 
-// Plan cache keyed by (op, precision, structure hashes, extra ints); least-recently-used eviction when it holds more
-// than kMax plans or more device memory than the budget (MPSXX_PLAN_CACHE_GB, default 40 % of the device's memory:
-// plans of large QC problems hold GB-sized intermediates, one per site, and a sweep cycles through all of them).  The kKeep most recently used plans are never evicted, so the
-// plans an operation holds at once (a Davidson solve over its operator groups) stay valid.
+// Plan cache keyed by (op, precision, structure hashes, extra ints).  Plans of large QC problems hold GB-sized
+// intermediates, one set per site, and a sweep cycles through all of them, so a plain LRU over whole plans would
+// evict exactly the plans the sweep needs next.  Instead, while the device memory held by the plans exceeds the
+// budget (MPSXX_PLAN_CACHE_GB, default 40 % of the device's memory), the least recently used plans release their slot
+// buffers and keep their task lists (the costly part to rebuild); a released plan allocates its buffers again from
+// the device pool on its next use.  Whole plans are evicted, least recently used first, only beyond kMax plans or
+// when the task lists alone exceed the budget.  The kKeep most recently used plans are never released or evicted,
+// so the plans an operation holds at once (a Davidson solve over its operator groups) stay valid.
 class PlanCache {
  public:
   CachedPlan* find(const std::vector<uint64_t>& key) {
@@ -54,20 +58,22 @@ class PlanCache {
     }
     ++stats_.hits;
     lru_.splice(lru_.begin(), lru_, it->second.second);
-    return it->second.first.get();
+    CachedPlan* cp = it->second.first.get();
+    if (cp->plan.stripped) {
+      cp->plan.restore();
+      ++stats_.restores;
+      bytes_ -= cp->charged;
+      cp->charged = cp->plan.device_bytes();
+      bytes_ += cp->charged;
+      make_room(0);
+    }
+    return cp;
   }
   CachedPlan* insert(const std::vector<uint64_t>& key, std::unique_ptr<CachedPlan> p) {
     ++stats_.builds;
     stats_.build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - miss_t0_).count();
     const size_t add = p->plan.device_bytes();
-    while (map_.size() > kKeep && (map_.size() >= kMax || bytes_ + add > budget())) {
-      auto victim = lru_.back();
-      lru_.pop_back();
-      auto it = map_.find(victim);
-      bytes_ -= it->second.first->charged;
-      map_.erase(it);
-      ++stats_.evictions;
-    }
+    make_room(add);
     lru_.push_front(key);
     CachedPlan* raw = p.get();
     raw->charged = add;
@@ -88,6 +94,30 @@ class PlanCache {
   }
 
  private:
+  // make `add` more bytes fit: release buffers of the least recently used plans, then evict whole plans
+  void make_room(size_t add) {
+    auto evict_back = [&] {
+      auto victim = lru_.back();
+      lru_.pop_back();
+      auto it = map_.find(victim);
+      bytes_ -= it->second.first->charged;
+      map_.erase(it);
+      ++stats_.evictions;
+    };
+    while (map_.size() > kKeep && map_.size() >= kMax) evict_back();
+    size_t from_back = 0;
+    for (auto it = lru_.rbegin(); it != lru_.rend() && bytes_ + add > budget(); ++it, ++from_back) {
+      if (map_.size() - from_back <= kKeep) break;
+      CachedPlan* cp = map_.find(*it)->second.first.get();
+      if (cp->plan.stripped) continue;
+      cp->plan.strip();
+      ++stats_.releases;
+      bytes_ -= cp->charged;
+      cp->charged = cp->plan.device_bytes();
+      bytes_ += cp->charged;
+    }
+    while (map_.size() > kKeep && bytes_ + add > budget()) evict_back();
+  }
   PlanCacheStats stats_;
   std::chrono::steady_clock::time_point miss_t0_ = std::chrono::steady_clock::now();
   static size_t budget() {
@@ -98,7 +128,7 @@ class PlanCache {
     }();
     return b;
   }
-  static constexpr size_t kMax = 256;
+  static constexpr size_t kMax = 1024;
   static constexpr size_t kKeep = kPlanCacheKeep;
   size_t bytes_ = 0;
   std::map<std::vector<uint64_t>, std::pair<std::unique_ptr<CachedPlan>, std::list<std::vector<uint64_t>>::iterator>>
@@ -281,6 +311,7 @@ const Layout* existing(const Tensor& t) { return t.empty ? nullptr : &t.L; }
 
 void clear_plan_cache() {
   cache().clear();
+  clear_merged_operator_cache();
   last_run() = LastRun();
 }
 

@@ -61,6 +61,7 @@ void op_renormalize_merged(bool fwd, Tensor& opr0, Tensor& bra, Tensor& ket, Ten
 // spent building them, device allocation included), plans evicted, and what the cache holds now.
 struct PlanCacheStats {
   int64_t hits = 0, builds = 0, evictions = 0, plans = 0, device_bytes = 0;
+  int64_t releases = 0, restores = 0;  // plans whose slot buffers were released under the budget / allocated again
   double build_ms = 0.0;
 };
 PlanCacheStats plan_cache_stats();
@@ -81,6 +82,7 @@ Plan build_plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layo
                               const Layout& wfn, const Layout* out, int prec);
 
 void clear_plan_cache();
+void clear_merged_operator_cache();  // device copies of the scalar-MPO merged-operator plans
 
 // Quantum-number merging (include/btas/QSPARSE/QSTmerge.h) and merged operators (src/optimize.hpp:273-368)
 struct MergeInfo {

@@ -7,6 +7,8 @@
 // truncation to M, renormalize.  The sweep energy is the minimum of the site energies (make_sweep.cpp:115).
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <vector>
 
@@ -141,22 +143,37 @@ Dmrg* dmrg_create(int L, const std::vector<Tensor*>& mpo, const std::vector<Tens
 // The merged-operator build is counted in renorm_s.
 void optimize_merged(Dmrg& D, int i, bool forward, Tensor& psi, int M, double tol, int prec, SweepStats& st,
                      double& emin, hipStream_t s) {
+  // MPSXX_SWEEP_TRACE: every phase is synchronized and timed on its own, one line per step on stderr
+  static const bool trace = std::getenv("MPSXX_SWEEP_TRACE") != nullptr;
+  double ph[8] = {0};
+  auto lap = [&](clk::time_point& t, int k) {
+    if (!trace) return;
+    hip_check(hipStreamSynchronize(s), "trace");
+    ph[k] = 1e3 * since(t);
+    t = clk::now();
+  };
   auto t = clk::now();
+  auto tp = clk::now();
   Tensor Lm, Rm, wm;
   op_merged_operator(true, *D.mpo[i], *D.lenv[i], Lm, s);
+  lap(tp, 0);
   op_merged_operator(false, *D.mpo[i + 1], *D.renv[i + 2], Rm, s);
+  lap(tp, 1);
   op_qst_merge(psi, 2, 2, wm, s);
   st.renorm_s += since(t);
   t = clk::now();
+  tp = clk::now();
   Tensor diag = shape_only(wm.L);
   op_diag_merged(Lm, Rm, diag, s);
   hip_check(hipStreamSynchronize(s), "diag");
   st.diag_s += since(t);
+  ph[2] = 1e3 * since(tp);
   t = clk::now();
   DavidsonStats ds;
   std::vector<DavGroup> g{{nullptr, nullptr, &Lm, &Rm}};
   const double e = davidson_twosite(g, diag, wm, 0.1 * tol, prec, ds, s);
   st.davidson_s += since(t);
+  ph[3] = 1e3 * since(t);
   st.nsigma += ds.nsigma;
   st.nsigma_fp32 += ds.nsigma_fp32;
   st.sigma_ms += ds.sigma_ms;
@@ -165,21 +182,28 @@ void optimize_merged(Dmrg& D, int i, bool forward, Tensor& psi, int M, double to
   Tensor U, V;
   st.dweight += op_canonicalize_merged(forward, wm, psi.L, U, V, M, s);
   st.svd_s += since(t);
+  ph[4] = 1e3 * since(t);
   st.max_bond = std::max(st.max_bond, bond_dim(U.L, 2));
   t = clk::now();
-  if (forward) {
-    Tensor Um;
-    op_qst_merge(U, 2, 1, Um, s);  // U~((l,n1), s)
-    D.lenv[i + 1] = std::make_unique<Tensor>();
-    op_renormalize_merged(true, Lm, Um, Um, *D.lenv[i + 1], MPSXX_FP64, s);
-  } else {
-    Tensor Vm;
-    op_qst_merge(V, 1, 2, Vm, s);  // V~(s, (n2,r))
-    D.renv[i + 1] = std::make_unique<Tensor>();
-    op_renormalize_merged(false, Rm, Vm, Vm, *D.renv[i + 1], MPSXX_FP64, s);
-  }
+  tp = clk::now();
+  Tensor Xm;
+  if (forward) op_qst_merge(U, 2, 1, Xm, s);  // U~((l,n1), s)
+  else op_qst_merge(V, 1, 2, Xm, s);          // V~(s, (n2,r))
+  lap(tp, 5);
+  std::unique_ptr<Tensor>& env = forward ? D.lenv[i + 1] : D.renv[i + 1];
+  env = std::make_unique<Tensor>();
+  op_renormalize_merged(forward, forward ? Lm : Rm, Xm, Xm, *env, MPSXX_FP64, s);
+  const int64_t rflops = last_stats().flops;
   hip_check(hipStreamSynchronize(s), "renormalize");
   st.renorm_s += since(t);
+  if (trace) {
+    ph[6] = 1e3 * since(tp);
+    std::fprintf(stderr,
+                 "[sweep] site %d %s merge %.2f %.2f diag %.2f dav %.2f (%d sigma) svd %.2f qst %.2f renorm %.2f ms "
+                 "(%.2f GFLOP, %.1f TFLOP/s) bond %d\n",
+                 i, forward ? "fwd" : "bwd", ph[0], ph[1], ph[2], ph[3], ds.nsigma, ph[4], ph[5], ph[6],
+                 rflops * 1e-9, ph[6] > 0 ? rflops * 1e-9 / ph[6] : 0.0, bond_dim(U.L, 2));
+  }
   *D.mps[i] = std::move(U);
   *D.mps[i + 1] = std::move(V);
 }

@@ -40,13 +40,14 @@ def test_diagnostic_counters_are_host_only_and_consistent():
     """The plan-cache and device-allocation counters (no reference counterpart) are readable without a device: counts
     are non-negative, times finite, and NULL outputs are accepted."""
     pc = mpsxx_amd.plan_cache_stats()
-    assert set(pc) == {"hits", "builds", "evictions", "build_ms", "plans", "device_bytes"}
+    assert set(pc) == {"hits", "builds", "evictions", "build_ms", "plans", "device_bytes", "releases", "restores"}
     assert all(v >= 0 for v in pc.values())
     assert pc["plans"] <= pc["builds"]
+    assert pc["restores"] <= pc["releases"] <= pc["builds"] + pc["restores"]
     dm = mpsxx_amd.device_memory_stats()
     assert set(dm) == {"mallocs", "frees", "malloc_ms", "free_ms", "reuses", "pool_syncs", "bytes_live",
                        "cached_bytes"}
     assert all(v >= 0 for v in dm.values())
     lib = mpsxx_amd.lib()
-    assert lib.mpsxx_plan_cache_stats(None, None, None, None, None, None) == 0
+    assert lib.mpsxx_plan_cache_stats(None, None, None, None, None, None, None, None) == 0
     assert lib.mpsxx_device_memory_stats(None, None, None, None, None, None, None, None) == 0
