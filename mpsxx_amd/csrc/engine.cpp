@@ -741,8 +741,11 @@ Layout tie_layout(const Layout& X, const std::vector<int>& tie, std::vector<Copy
       }
       os.push_back(s);
     }
-    // innermost dimension forms the segment, outer dims enumerate segments
-    const int inner = Y.rank - 1;
+    // the innermost dimension of extent > 1 forms the segment (trailing unit extents leave the element order
+    // unchanged: a diagonal of an (R, 1, R) block is one segment of R elements, not R of one), outer dims enumerate
+    // segments
+    int inner = Y.rank - 1;
+    while (inner > 0 && od[inner] == 1) --inner;
     int64_t outer = 1;
     for (int n = 0; n < inner; ++n) outer *= od[n];
     if (od[inner] == 0) continue;

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <list>
@@ -40,6 +41,24 @@ struct CachedPlan {
   size_t charged = 0;  // device bytes counted against the cache budget at insertion
 };
 
+// MPSXX_PLAN_TRACE: host time of each plan build, split into the symbolic contractions, the device steps (tiling
+// and task-list uploads) and the intermediate allocations, printed on stderr
+struct BuildTrace {
+  bool on = std::getenv("MPSXX_PLAN_TRACE") != nullptr;
+  double sym_ms = 0.0, step_ms = 0.0, alloc_ms = 0.0;
+  int64_t pairs = 0;
+};
+BuildTrace& btrace() {
+  static thread_local BuildTrace t;
+  return t;
+}
+struct TraceScope {
+  double& acc;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit TraceScope(double& a) : acc(a) {}
+  ~TraceScope() { acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
 // Plan cache keyed by (op, precision, structure hashes, extra ints).  Plans of large QC problems hold GB-sized
 // intermediates, one set per site, and a sweep cycles through all of them, so a plain LRU over whole plans would
 // evict exactly the plans the sweep needs next.  Instead, while the device memory held by the plans exceeds the
@@ -54,6 +73,9 @@ class PlanCache {
     auto it = map_.find(key);
     if (it == map_.end()) {
       miss_t0_ = std::chrono::steady_clock::now();  // a miss is followed by the build and insert()
+      BuildTrace& bt = btrace();
+      bt.sym_ms = bt.step_ms = bt.alloc_ms = 0.0;
+      bt.pairs = 0;
       return nullptr;
     }
     ++stats_.hits;
@@ -71,7 +93,12 @@ class PlanCache {
   }
   CachedPlan* insert(const std::vector<uint64_t>& key, std::unique_ptr<CachedPlan> p) {
     ++stats_.builds;
-    stats_.build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - miss_t0_).count();
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - miss_t0_).count();
+    stats_.build_ms += ms;
+    const BuildTrace& bt = btrace();
+    if (bt.on)
+      std::fprintf(stderr, "[plan] op %d build %.2f ms: symbolic %.2f steps %.2f alloc %.2f, %lld pairs, %zu steps\n",
+                   (int)key[0], ms, bt.sym_ms, bt.step_ms, bt.alloc_ms, (long long)bt.pairs, p->plan.steps.size());
     const size_t add = p->plan.device_bytes();
     make_room(add);
     lru_.push_front(key);
@@ -141,6 +168,7 @@ PlanCache& cache() {
   return c;
 }
 
+
 }  // namespace
 
 PlanCacheStats plan_cache_stats() { return cache().stats(); }
@@ -168,8 +196,14 @@ class PlanBuilder {
   // intermediate = Contract(A[ca], B[cb])
   int contract(int a, bool ca_conj, const std::vector<int>& ca, int b, bool cb_conj, const std::vector<int>& cb) {
     View va{&P_.slot_layout[a], ca_conj}, vb{&P_.slot_layout[b], cb_conj};
-    ContractSym sym = contract_symbolic(va, ca, vb, cb, nullptr);
+    ContractSym sym;
+    {
+      TraceScope ts(btrace().sym_ms);
+      sym = contract_symbolic(va, ca, vb, cb, nullptr);
+    }
+    btrace().pairs += (int64_t)sym.pairs.size();
     const int id = add_inter(sym.C, P_.precision != MPSXX_FP64 ? 1 : 0);
+    TraceScope ts(btrace().step_ms);
     P_.steps.push_back(make_contract_step(sym, a, b, id, 0, 1.0));
     P_.flops += sym.flops;
     return id;
@@ -179,8 +213,14 @@ class PlanBuilder {
   void contract_out(int a, bool ca_conj, const std::vector<int>& ca, int b, bool cb_conj, const std::vector<int>& cb,
                     const Layout* out_existing, double alpha = 1.0) {
     View va{&P_.slot_layout[a], ca_conj}, vb{&P_.slot_layout[b], cb_conj};
-    ContractSym sym = contract_symbolic(va, ca, vb, cb, out_existing);
+    ContractSym sym;
+    {
+      TraceScope ts(btrace().sym_ms);
+      sym = contract_symbolic(va, ca, vb, cb, out_existing);
+    }
+    btrace().pairs += (int64_t)sym.pairs.size();
     const int id = add_output(sym.C);
+    TraceScope ts(btrace().step_ms);
     P_.steps.push_back(make_contract_step(sym, a, b, id, 1, alpha));
     P_.flops += sym.flops;
     P_.bytes += sym.C.nelem * 8;
@@ -193,9 +233,15 @@ class PlanBuilder {
                 int w2, const std::vector<int>& cb2) {
     static const bool nofuse = std::getenv("MPSXX_NO_FUSE") != nullptr;
     View vy{&P_.slot_layout[y], false}, vw1{&P_.slot_layout[w1], false};
-    ContractSym s1 = contract_symbolic(vy, ca1, vw1, cb1, nullptr);
-    View vx{&s1.C, false}, vw2{&P_.slot_layout[w2], false};
-    ContractSym s2 = contract_symbolic(vx, ca2, vw2, cb2, nullptr);
+    ContractSym s1, s2;
+    {
+      TraceScope ts(btrace().sym_ms);
+      s1 = contract_symbolic(vy, ca1, vw1, cb1, nullptr);
+      View vx{&s1.C, false}, vw2{&P_.slot_layout[w2], false};
+      s2 = contract_symbolic(vx, ca2, vw2, cb2, nullptr);
+    }
+    btrace().pairs += (int64_t)(s1.pairs.size() + s2.pairs.size());
+    TraceScope ts(btrace().step_ms);
     Step fused;
     const int prec = P_.precision != MPSXX_FP64 ? 1 : 0;
     if (!nofuse && make_fused_axpy_step(s1, s2, y, w1, w2, (int)P_.slot_layout.size(), 0, fused)) {
@@ -252,6 +298,7 @@ class PlanBuilder {
     P_.slot_layout.push_back(L);
     P_.slot_input.push_back(-1);
     P_.slot_prec.push_back(prec);
+    TraceScope ts(btrace().alloc_ms);
     P_.slot_buf.emplace_back(std::max<int64_t>(L.nelem, 1) * (prec ? sizeof(float) : sizeof(double)));
     return id;
   }
