@@ -841,6 +841,8 @@ GemmPolicy default_policy() {
     GemmPolicy q;
     if (const char* e = std::getenv("MPSXX_SMALL_GEMM_LIMIT")) q.small_limit = std::atoi(e);
     if (const char* e = std::getenv("MPSXX_SPLIT_UNITS")) q.split_units = std::atoi(e);
+    if (const char* e = std::getenv("MPSXX_SPLIT_SMALL")) q.split_small = std::atoi(e);
+    if (const char* e = std::getenv("MPSXX_SPLIT_SEG")) q.split_seg = std::max(1, std::atoi(e));
     return q;
   }();
   return p;
@@ -926,9 +928,9 @@ void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>&
       }
   }
   // the split 32x32 tiles: a tile whose reduction is much longer than the list's share per work-group
-  // (split_small work-groups in all) is cut into pair ranges of >= kSplitSegUnits units, one work-group each
+  // (split_small work-groups in all) is cut into pair ranges of >= split_seg units, one work-group each
   const int64_t starget =
-      pol.split_small > 0 ? std::max<int64_t>(kSplitSegUnits, (usplit + pol.split_small - 1) / pol.split_small) : 0;
+      pol.split_small > 0 ? std::max<int64_t>(pol.split_seg, (usplit + pol.split_small - 1) / pol.split_small) : 0;
   for (size_t ti : splittasks) {
     const GemmTask& t = tasks[ti];
     const int lo = t.pair_begin, hi = t.pair_begin + t.pair_count;

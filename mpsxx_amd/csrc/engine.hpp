@@ -117,18 +117,18 @@ struct HostLists {
 // over the 4 waves of a work-group when they reduce over >= 2 pairs and >= split_units (pair, 16-wide k panel) units,
 // one wave per tile otherwise; larger blocks run as 64x64 LDS tiles, whose pair lists are split over several
 // work-groups (partial tiles + an ordered sum) while the step has fewer than split_big such tiles (0: never).  The
-// split 32x32 tiles are cut the same way into pair ranges of at least kSplitSegUnits units, each on its own
-// work-group, until the list holds about split_small work-groups (0: never).  max_groups > 0 caps the grid of every
-// launch.
+// split 32x32 tiles are cut the same way into pair ranges of at least split_seg units, each on its own work-group,
+// until the list holds about split_small work-groups (0: never).  max_groups > 0 caps the grid of every launch.
 struct GemmPolicy {
   int small_limit = 64;
   int split_units = 2;
   int max_groups = 0;
   int split_big = 2048;
   int split_small = 1024;
+  int split_seg = 32;
 };
-constexpr int kSplitSegUnits = 32;
-GemmPolicy default_policy();  // the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT / MPSXX_SPLIT_UNITS
+// the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT, MPSXX_SPLIT_UNITS, MPSXX_SPLIT_SMALL, MPSXX_SPLIT_SEG
+GemmPolicy default_policy();
 
 // Tile lists of one GEMM step (heaviest blocks first in each list).
 struct GemmTiling {

@@ -141,6 +141,44 @@ def test_merged_sigma_qc_matches_oracle(norb, M):
     assert got32.diff_norm(ref) <= 2e-5 * ref.norm()
 
 
+_SEGMENTED_CHILD = r"""
+import json, sys
+sys.path[:0] = [sys.argv[1], sys.argv[1] + "/tests"]
+import numpy as np
+import mpsxx_amd as mx
+import oracle
+from mpsxx_amd import workloads
+Lm, Rm, psim, info = workloads.qc_merged_problem(8, 48, seed=3)
+ref = oracle.sigma_merged(Lm.to_qst(), Rm.to_qst(), psim.to_qst())
+scale = max(float(np.max(np.abs(b))) for b in ref.blocks.values() if b.size)
+out = {}
+for name, prec in (("fp64", mx.FP64), ("fp32", mx.FP32)):
+    got = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=prec).to_qst()
+    out[name] = [got.block_set() == ref.block_set(), got.max_abs_diff(ref) / scale, got.diff_norm(ref) / ref.norm()]
+print(json.dumps(out), flush=True)
+"""
+
+
+def test_merged_sigma_qc_with_every_split_reduction_cut_matches_oracle():
+    """The 32x32 tiles with long reductions cut over work-groups (partial tiles summed in slot order by
+    gemm_reduce32_kernel): a child process with a policy that cuts every such tile into single-unit pair ranges
+    (MPSXX_SPLIT_SEG=1, MPSXX_SPLIT_SMALL=100000) computes the merged QC sigma; same bars as the default policy."""
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, MPSXX_SPLIT_SEG="1", MPSXX_SPLIT_SMALL="100000")
+    print("qc merged sigma, every split reduction cut", flush=True)
+    r = subprocess.run([sys.executable, "-c", _SEGMENTED_CHILD, root], env=env, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    print(res, flush=True)
+    assert res["fp64"][0] and res["fp32"][0]
+    assert res["fp64"][1] <= 1e-12
+    assert res["fp32"][2] <= 2e-5
+
+
 def test_merged_sigma_qc_split_bf16_close_to_fp64():
     """MPSXX_BF16X3: the large blocks multiply as split-BF16 pairs (three bf16 MFMAs per product, ~2^-16 relative per
     product); bar 1e-4 relative Frobenius norm against the FP64 device result (itself pinned to the oracle above)."""
