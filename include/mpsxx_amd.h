@@ -231,7 +231,7 @@ const char* mpsxx_dmrg_last_error(void);
 int mpsxx_plan_cache_stats(int64_t* hits, int64_t* builds, int64_t* evictions, double* build_ms, int64_t* plans,
                            int64_t* device_bytes, int64_t* releases, int64_t* restores);
 /* Device allocations of the library since it was loaded: hipMalloc and hipFree calls and the host milliseconds they
- * took, buffers handed out again by the library's caching pool (MPSXX_DEVICE_CACHE_GB, default 20 % of device
+ * took, buffers handed out again by the library's caching pool (MPSXX_DEVICE_CACHE_GB, default 31 % of device
  * memory) and the device synchronizations that took, the bytes in use and the bytes the pool holds.  Any pointer may
  * be NULL.  No reference counterpart (diagnostic). */
 int mpsxx_device_memory_stats(int64_t* mallocs, int64_t* frees, double* malloc_ms, double* free_ms, int64_t* reuses,

@@ -118,7 +118,7 @@ DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
 
 // Caching allocator behind DevBuf.  hipMalloc of large buffers costs ~1 ms per GB (page mapping) and hipFree
 // synchronizes the device, and a DMRG sweep allocates thousands of them (outputs, scratch, task lists).  Released
-// buffers are kept in a size-class pool (up to MPSXX_DEVICE_CACHE_GB, default 20 % of the device's memory) and handed
+// buffers are kept in a size-class pool (up to MPSXX_DEVICE_CACHE_GB, default 31 % of the device's memory) and handed
 // out again.  A buffer released since the last
 // device-wide synchronization may still be read by queued work on any stream, so handing one out is preceded by
 // hipDeviceSynchronize(), after which every pooled buffer is safe; allocations that find no pooled buffer call
@@ -153,7 +153,7 @@ struct DevicePool {
     static const size_t b = [] {
       if (const char* e = std::getenv("MPSXX_DEVICE_CACHE_GB"))
         return (size_t)(std::max(0.0, std::atof(e)) * 1073741824.0);
-      return device_memory_fraction(0.2, 32.0);
+      return device_memory_fraction(0.31, 32.0);
     }();
     return b;
   }

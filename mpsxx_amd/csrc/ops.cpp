@@ -62,7 +62,7 @@ struct TraceScope {
 // Plan cache keyed by (op, precision, structure hashes, extra ints).  Plans of large QC problems hold GB-sized
 // intermediates, one set per site, and a sweep cycles through all of them, so a plain LRU over whole plans would
 // evict exactly the plans the sweep needs next.  Instead, while the device memory held by the plans exceeds the
-// budget (MPSXX_PLAN_CACHE_GB, default 40 % of the device's memory), the least recently used plans release their slot
+// budget (MPSXX_PLAN_CACHE_GB, default 31 % of the device's memory), the least recently used plans release their slot
 // buffers and keep their task lists (the costly part to rebuild); a released plan allocates its buffers again from
 // the device pool on its next use.  Whole plans are evicted, least recently used first, only beyond kMax plans or
 // when the task lists alone exceed the budget.  The kKeep most recently used plans are never released or evicted,
@@ -151,7 +151,7 @@ class PlanCache {
     static const size_t b = [] {
       if (const char* e = std::getenv("MPSXX_PLAN_CACHE_GB"))
         return (size_t)(std::max(1.0, std::atof(e)) * 1073741824.0);
-      return device_memory_fraction(0.4, 64.0);
+      return device_memory_fraction(0.31, 64.0);
     }();
     return b;
   }
