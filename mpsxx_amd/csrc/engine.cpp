@@ -465,30 +465,35 @@ ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const 
   std::vector<SideA> A(nA);
   std::vector<SideB> Bs(nB);
   {
-    int64_t dims[kMaxRank], st[kMaxRank];
-    for (int64_t i = 0; i < nA; ++i) {
-      const int32_t* ai = a.index(i);
-      SideA& x = A[i];
-      x.key = tag_over(a, ca, ai);
-      x.free = tag_over(a, ua, ai);
-      block_strides(a, ai, dims, st);
-      if (!group_view(ua, dims, st, x.m, x.sm) || !group_view(ca, dims, st, x.k, x.sk))
-        x.m = -1;  // only an error if the block takes part in a pair
-      QN q;
-      for (int u : ua) q = q + Av.qs(u, ai[u]);
-      x.q = q;
-    }
-    for (int64_t j = 0; j < nB; ++j) {
-      const int32_t* bj = b.index(j);
-      SideB& y = Bs[j];
-      y.key = tag_over(b, cb, bj);
-      y.free = tag_over(b, ub, bj);
-      block_strides(b, bj, dims, st);
-      if (!group_view(cb, dims, st, y.k, y.sk) || !group_view(ub, dims, st, y.n, y.sn)) y.n = -1;
-      QN q;
-      for (int u : ub) q = q + Bv.qs(u, bj[u]);
-      y.q = q;
-    }
+    // independent per block: contiguous block ranges on several host threads for large operands
+    const int64_t nblk = nA + nB;
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)host_threads(), nblk / 16384));
+    parallel_for(nth, [&](int t) {
+      int64_t dims[kMaxRank], st[kMaxRank];
+      for (int64_t i = nA * t / nth; i < nA * (t + 1) / nth; ++i) {
+        const int32_t* ai = a.index(i);
+        SideA& x = A[i];
+        x.key = tag_over(a, ca, ai);
+        x.free = tag_over(a, ua, ai);
+        block_strides(a, ai, dims, st);
+        if (!group_view(ua, dims, st, x.m, x.sm) || !group_view(ca, dims, st, x.k, x.sk))
+          x.m = -1;  // only an error if the block takes part in a pair
+        QN q;
+        for (int u : ua) q = q + Av.qs(u, ai[u]);
+        x.q = q;
+      }
+      for (int64_t j = nB * t / nth; j < nB * (t + 1) / nth; ++j) {
+        const int32_t* bj = b.index(j);
+        SideB& y = Bs[j];
+        y.key = tag_over(b, cb, bj);
+        y.free = tag_over(b, ub, bj);
+        block_strides(b, bj, dims, st);
+        if (!group_view(cb, dims, st, y.k, y.sk) || !group_view(ub, dims, st, y.n, y.sn)) y.n = -1;
+        QN q;
+        for (int u : ub) q = q + Bv.qs(u, bj[u]);
+        y.q = q;
+      }
+    });
   }
   int64_t nfreeB = 1;
   for (int u : ub) nfreeB *= b.nsec(u);
