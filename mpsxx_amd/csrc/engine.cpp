@@ -389,7 +389,7 @@ void gemm_indices(int transa, int transb, int NA, int NB, int NC, std::vector<in
 }
 
 ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const View& Bv, const std::vector<int>& cb,
-                              const Layout* existing) {
+                              const Layout* existing, const Layout* mask) {
   const Layout& a = *Av.L;
   const Layout& b = *Bv.L;
   const int K = (int)ca.size();
@@ -439,6 +439,17 @@ ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const 
     }
     C.set_shape((int)(ua.size() + ub.size()), qsym, Av.q() + Bv.q(), qs, ds);
   }
+  bool use_mask = mask && mask->qsym && mask->rank == C.rank;
+  for (int i = 0; use_mask && i < C.rank; ++i) use_mask = mask->nsec(i) == C.nsec(i);
+  // whether the output block of tag t (row-major over the output's sector counts) is allowed by mask's quantum numbers
+  auto mask_allows = [&](int64_t t) {
+    int32_t idx[kMaxRank];
+    for (int i = C.rank - 1; i >= 0; --i) {
+      idx[i] = (int32_t)(t % C.nsec(i));
+      t /= C.nsec(i);
+    }
+    return mask->allowed(idx);
+  };
   if (existing && existing->rank > 0) {
     if (existing->rank != C.rank) throw Error(MPSXX_ERR_SHAPE, "output rank mismatch");
     if (qsym && existing->qsym) {
@@ -589,7 +600,10 @@ ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const 
         const int64_t i = aord[r];
         const std::vector<int64_t>* pl = partners(i);
         if (!pl) continue;
-        for (int64_t j : *pl) loc.push_back({Bs[j].free, A[i].key, (int32_t)i, (int32_t)j});
+        for (int64_t j : *pl) {
+          if (use_mask && !mask_allows(af * nfreeB + Bs[j].free)) continue;
+          loc.push_back({Bs[j].free, A[i].key, (int32_t)i, (int32_t)j});
+        }
       }
       if (counting) {
         seen.clear();

@@ -87,9 +87,12 @@ struct ContractSym {
 };
 
 // Symbolic BTAS Contract(a[ca], b[cb]) (include/btas/QSPARSE/QSTCONTRACT.h:16-57 + STBLAS.h:298-359).  If `existing`
-// is non-null and non-empty, its q/qshape are checked and its blocks kept (output = existing U new).
+// is non-null and non-empty, its q/qshape are checked and its blocks kept (output = existing U new).  If `mask` is
+// non-null, quantum-number symmetric and has the output's sector counts, only output blocks that `mask`'s quantum
+// numbers allow receive pairs (for an output that is afterwards up-cast onto `mask`, which keeps exactly those);
+// otherwise it is ignored.
 ContractSym contract_symbolic(const View& A, const std::vector<int>& ca, const View& B, const std::vector<int>& cb,
-                              const Layout* existing);
+                              const Layout* existing, const Layout* mask = nullptr);
 
 // Planner threading: threads > 0 fixes the host thread count (0: MPSXX_HOST_THREADS or the hardware concurrency, at
 // most 16); a contraction uses one thread per min_pairs candidate pairs at most.  Plans do not depend on either.

@@ -193,13 +193,15 @@ class PlanBuilder {
     return id;
   }
 
-  // intermediate = Contract(A[ca], B[cb])
-  int contract(int a, bool ca_conj, const std::vector<int>& ca, int b, bool cb_conj, const std::vector<int>& cb) {
+  // intermediate = Contract(A[ca], B[cb]); with `mask`, only the output blocks that mask's quantum numbers allow (a
+  // layout with the output's sector counts onto which the intermediate is up-cast afterwards)
+  int contract(int a, bool ca_conj, const std::vector<int>& ca, int b, bool cb_conj, const std::vector<int>& cb,
+               const Layout* mask = nullptr) {
     View va{&P_.slot_layout[a], ca_conj}, vb{&P_.slot_layout[b], cb_conj};
     ContractSym sym;
     {
       TraceScope ts(btrace().sym_ms);
-      sym = contract_symbolic(va, ca, vb, cb, nullptr);
+      sym = contract_symbolic(va, ca, vb, cb, nullptr, mask);
     }
     btrace().pairs += (int64_t)sym.pairs.size();
     const int id = add_inter(sym.C, P_.precision != MPSXX_FP64 ? 1 : 0);
@@ -733,9 +735,10 @@ void op_diag_merged(Tensor& lopr, Tensor& ropr, Tensor& diag, hipStream_t s) {
   execute(make_key(OP_DIAGM, 0, {&lopr, &ropr}, &diag), [&] {
     PlanBuilder B(MPSXX_FP64);
     const int c = B.input(lopr.L, 0), d = B.input(ropr.L, 1);
-    // compute_diagonal_elements.hpp:72-79: Gemm(N,T,lopr_diag,ropr_diag)
+    // compute_diagonal_elements.hpp:72-79: Gemm(N,T,lopr_diag,ropr_diag); only the blocks the up-casting copy keeps
+    // (those the wavefunction's quantum numbers allow) are multiplied
     const int lo = B.tie(c, {0, 2}), ro = B.tie(d, {0, 2});
-    const int s1 = B.contract(lo, false, {1}, ro, false, {1});
+    const int s1 = B.contract(lo, false, {1}, ro, false, {1}, &target);
     B.upcast_out(s1, target);
     return B.finish();
   }, OUT_OVERWRITE, {&lopr, &ropr}, diag, s);
