@@ -271,6 +271,12 @@ int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tens
 int mpsxx_last_plan_profile(int reps, int32_t max_entries, double* ms, int64_t* flops, int64_t* bytes,
                             int32_t* kernel, int32_t* step, int32_t* nentries, int64_t* plan_flops, void* stream);
 
+/* Work items of the plan of the most recent hot-path call on this thread, summed over its steps: tiles[0] 64x64 tiles,
+ * [1] 64x64 tiles of split reductions (partial-tile sums), [2] 32x32 tiles of the one-wave kernel, [3] 32x32 tiles of
+ * the four-wave split kernel, [4] of those, tiles whose pair range was cut over several work-groups (partial-tile sums
+ * of gemm_reduce32_kernel), [5] coefficient (AXPY) chunks.  Diagnostic; no reference counterpart. */
+int mpsxx_last_plan_tiles(int64_t tiles[6]);
+
 /* ---- quantum-chemistry MPO (src/gen_qc_operators.cpp:18-202, src/boundary_opinfo.cpp:47-140) ------------------- */
 /* H = sum_{pq,s} h[p*K+q] a+_ps a_qs + 1/2 sum_{pqrs,st} v[((p*K+q)*K+r)*K+s] a+_ps a+_rt a_st a_qs + ecore, with v the
  * chemist-notation (pq|rs) (the reference's twoint(p,r,q,s), src/parsing_integral.cpp:139-147).  Bond operators are

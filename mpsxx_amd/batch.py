@@ -107,6 +107,15 @@ def dominant_kernel(entries):
     return name, agg[name], agg
 
 
+def last_plan_tiles():
+    """Work items of the plan of the most recent hot-path call (mpsxx_last_plan_tiles): 64x64 tiles, 64x64 tiles of
+    split reductions, one-wave 32x32 tiles, split 32x32 tiles, split 32x32 tiles cut over work-groups, AXPY chunks."""
+    t = (ctypes.c_int64 * 6)()
+    _check(lib().mpsxx_last_plan_tiles(t))
+    keys = ("big", "big_red", "small", "split", "split_cut", "axpy_chunks")
+    return dict(zip(keys, (int(x) for x in t)))
+
+
 def last_plan_profile(reps=20):
     """Per kernel launch of the plan of the most recent hot-path call (see mpsxx_last_plan_profile)."""
     pf = ctypes.c_int64()

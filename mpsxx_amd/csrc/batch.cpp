@@ -372,10 +372,23 @@ std::vector<GemmPolicy> BatchSigma::tune(int reps, hipStream_t s) {
       for (int lim : {0, 32, 64, 96, 128, 192})
         for (int su : {2, 4, 8, INT_MAX}) {
           if (lim == 0 && su != 2) continue;  // no small blocks: the split threshold is irrelevant
-          for (int cap : grid_caps) cand.push_back({lim, su, cap, pol_[k].split_big});
+          for (int cap : grid_caps) {
+            GemmPolicy c = pol_[k];  // keeps the 32x32 split settings
+            c.small_limit = lim;
+            c.split_units = su;
+            c.max_groups = cap;
+            cand.push_back(c);
+          }
         }
     } else {
-      for (int cap : grid_caps) cand.push_back({0, 0, cap, 0});
+      for (int cap : grid_caps) {
+        GemmPolicy c = pol_[k];
+        c.small_limit = 0;
+        c.split_units = 0;
+        c.max_groups = cap;
+        c.split_big = 0;
+        cand.push_back(c);
+      }
     }
     // the current policy first: a candidate must beat it by 1 % to replace it
     cand.insert(cand.begin(), pol_[k]);

@@ -439,13 +439,12 @@ int mpsxx_last_plan_profile(int reps, int32_t max_entries, double* ms, int64_t* 
                             int32_t* kernel, int32_t* step, int32_t* nentries, int64_t* plan_flops, void* stream) {
   return guard([&] {
     if (reps <= 0) throw mpx::Error(MPSXX_ERR_ARG, "profile: reps must be positive");
-    mpx::LastRun& r = mpx::last_run();
-    if (!r.plan) throw mpx::Error(MPSXX_ERR_ARG, "profile: no hot-path call to profile on this thread");
-    mpx::Plan& P = *r.plan;
+    std::vector<const double*> in;
+    mpx::Plan& P = mpx::last_plan(in);
     if (plan_flops) *plan_flops = P.flops;
     mpx::DevBuf out(std::max<int64_t>(P.out.nelem, 1) * sizeof(double));
-    mpx::run_plan(P, r.inputs, out.as<double>(), S(stream));
-    std::vector<mpx::PartTiming> t = mpx::profile_plan_parts(P, r.inputs, out.as<double>(), reps, S(stream));
+    mpx::run_plan(P, in, out.as<double>(), S(stream));
+    std::vector<mpx::PartTiming> t = mpx::profile_plan_parts(P, in, out.as<double>(), reps, S(stream));
     if (nentries) *nentries = (int32_t)t.size();
     if ((int)t.size() > max_entries) throw mpx::Error(MPSXX_ERR_ARG, "profile: max_entries too small");
     for (size_t i = 0; i < t.size(); ++i) {
@@ -454,6 +453,26 @@ int mpsxx_last_plan_profile(int reps, int32_t max_entries, double* ms, int64_t* 
       if (bytes) bytes[i] = t[i].part.bytes;
       if (kernel) kernel[i] = t[i].part.kind;
       if (step) step[i] = t[i].step;
+    }
+  });
+}
+
+int mpsxx_last_plan_tiles(int64_t tiles[6]) {
+  return guard([&] {
+    if (!tiles) throw mpx::Error(MPSXX_ERR_ARG, "plan tiles: null output");
+    std::vector<const double*> in;
+    const mpx::Plan& P = mpx::last_plan(in);
+    for (int k = 0; k < 6; ++k) tiles[k] = 0;
+    for (const mpx::Step& st : P.steps) {
+      if (st.kind == mpx::Step::GEMM) {
+        tiles[0] += st.nwork;
+        tiles[1] += st.nwork_red;
+        tiles[2] += st.nwork_small;
+        tiles[3] += st.nwork_split;
+        tiles[4] += st.nwork_split_red;
+      } else if (st.kind == mpx::Step::AXPY) {
+        tiles[5] += st.nwork;
+      }
     }
   });
 }
@@ -677,7 +696,11 @@ int mpsxx_batch_set_policy(mpsxx_batch b, int32_t nsteps, const int32_t* policy)
       const int32_t* q = policy + 4 * k;
       if (q[0] < 0 || q[1] < 1 || q[2] < 0 || q[3] < 0)
         throw mpx::Error(MPSXX_ERR_ARG, "batch set_policy: negative limit, threshold, grid cap or split target");
-      p[k] = {q[0], q[1], q[2], q[3]};
+      p[k] = mpx::default_policy();  // the 32x32 split settings are not part of the policy layout
+      p[k].small_limit = q[0];
+      p[k].split_units = q[1];
+      p[k].max_groups = q[2];
+      p[k].split_big = q[3];
     }
     x.set_policies(p);
   });

@@ -70,10 +70,18 @@ void sym_eig(int n, std::vector<double> a, std::vector<double>& w, std::vector<d
 
 namespace {
 
-Plan& group_plan(const DavGroup& g, const Layout& U, int prec) {
-  if (g.merged()) return plan_sigma_merged(g.lopr->L, g.ropr->L, U, &U, prec);
-  if (g.onesite()) return plan_sigma_onesite(g.lmpo->L, g.lopr->L, g.ropr->L, U, &U, prec);
-  return plan_sigma_twosite(g.lmpo->L, g.rmpo->L, g.lopr->L, g.ropr->L, U, &U, prec);
+// copy: which of the cached plans of this structure (see plan_sigma_twosite)
+Plan& group_plan(const DavGroup& g, const Layout& U, int prec, int copy = 0) {
+  if (g.merged()) return plan_sigma_merged(g.lopr->L, g.ropr->L, U, &U, prec, copy);
+  if (g.onesite()) return plan_sigma_onesite(g.lmpo->L, g.lopr->L, g.ropr->L, U, &U, prec, copy);
+  return plan_sigma_twosite(g.lmpo->L, g.rmpo->L, g.lopr->L, g.ropr->L, U, &U, prec, copy);
+}
+
+// Structure key of a group's operands: groups with equal keys would share one cached plan.
+std::vector<uint64_t> group_structure(const DavGroup& g) {
+  std::vector<uint64_t> k{(uint64_t)g.wfn_rank()};
+  for (const Tensor* t : {g.lmpo, g.rmpo, g.lopr, g.ropr}) k.push_back(t ? t->L.hash : 0ull);
+  return k;
 }
 
 }  // namespace
@@ -239,9 +247,17 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   double eval = 0.0;
   std::vector<double> G(max_ritz * max_ritz), R, coef_h(max_ritz * max_ritz, 0.0);
   // one Davidson solve from the current trial_0 with the sigma arithmetic of `p`
+  // Reduced-precision plans keep FP32 copies of their group's operators for the whole solve, so a group whose
+  // structure repeats an earlier group's takes a plan of its own; FP64 plans read the operators directly and are
+  // shared.
   auto use_plans = [&](int p) {
     plans.clear();
-    for (const DavGroup& g : groups) plans.push_back(&group_plan(g, U, p));
+    for (size_t g = 0; g < groups.size(); ++g) {
+      int copy = 0;
+      if (p != MPSXX_FP64)
+        for (size_t h = 0; h < g; ++h) copy += group_structure(groups[h]) == group_structure(groups[g]);
+      plans.push_back(&group_plan(groups[g], U, p, copy));
+    }
     convert_mask = ~0u;
   };
   // have_sigma0: T(0) is normalized and Sg(0) = H T(0) was computed at this precision already

@@ -14,6 +14,7 @@
 #include <numeric>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 
 #include "mpsxx_amd.h"
 
@@ -253,6 +254,38 @@ void DevBuf::release() {
   p = nullptr;
   bytes = 0;
   cls = 0;
+}
+
+namespace {
+struct TensorRegistry {
+  std::mutex mu;
+  std::unordered_set<const Tensor*> live;
+};
+TensorRegistry& tensor_registry() {
+  static TensorRegistry* r = new TensorRegistry;  // never destroyed: tensors may outlive static destruction order
+  return *r;
+}
+void register_tensor(const Tensor* t) {
+  TensorRegistry& r = tensor_registry();
+  std::lock_guard<std::mutex> g(r.mu);
+  r.live.insert(t);
+}
+}  // namespace
+
+Tensor::Tensor() { register_tensor(this); }
+Tensor::Tensor(Tensor&& o) noexcept : L(std::move(o.L)), data(std::move(o.data)), empty(o.empty) {
+  o.empty = true;
+  register_tensor(this);
+}
+Tensor::~Tensor() {
+  TensorRegistry& r = tensor_registry();
+  std::lock_guard<std::mutex> g(r.mu);
+  r.live.erase(this);
+}
+bool Tensor::alive(const Tensor* t) {
+  TensorRegistry& r = tensor_registry();
+  std::lock_guard<std::mutex> g(r.mu);
+  return r.live.count(t) != 0;
 }
 
 void Tensor::set_layout(const Layout& l, hipStream_t s) {
@@ -1179,6 +1212,7 @@ void run_plan_timed(Plan& P, const std::vector<const double*>& inputs, double* o
 
 void Plan::strip() {
   for (DevBuf& b : slot_buf) b.release();
+  conv_src.clear();
   stripped = true;
 }
 
@@ -1197,15 +1231,18 @@ bool Plan::restore() {
 
 std::vector<void*> slot_pointers(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
                                  unsigned convert_mask) {
-  if (P.restore()) convert_mask = ~0u;  // fresh FP32 operand copies hold nothing yet
+  P.restore();  // a restored plan's FP32 operand copies hold nothing (conv_src was cleared by strip)
   const int ns = (int)P.slot_layout.size();
+  if ((int)P.conv_src.size() != ns) P.conv_src.assign(ns, nullptr);
   std::vector<void*> ptr(ns, nullptr);
   for (int i = 0; i < ns; ++i) {
     if (P.slot_input[i] >= 0) {
       const double* src = inputs[P.slot_input[i]];
       if (P.slot_prec[i] == 1) {
-        if (convert_mask & (1u << P.slot_input[i]))
+        if ((convert_mask & (1u << P.slot_input[i])) || P.conv_src[i] != src) {
           launch_convert<double, float>(src, P.slot_buf[i].as<float>(), P.slot_layout[i].nelem, s);
+          P.conv_src[i] = src;
+        }
         ptr[i] = P.slot_buf[i].p;
       } else {
         ptr[i] = const_cast<double*>(src);

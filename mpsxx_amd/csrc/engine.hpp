@@ -59,11 +59,20 @@ DevBuf upload_vec(const std::vector<T>& v) {
   return b;
 }
 
-// Device-resident block-sparse FP64 array.
+// Device-resident block-sparse FP64 array.  Every Tensor object registers its address while it exists, so that
+// code holding a pointer recorded earlier can check that the object is still alive (Tensor::alive).
 struct Tensor {
   Layout L;
   DevBuf data;      // nelem doubles
   bool empty = true;  // size() == 0 in the reference sense (no structure yet)
+
+  Tensor();
+  ~Tensor();
+  Tensor(Tensor&& o) noexcept;
+  Tensor& operator=(Tensor&& o) noexcept = default;
+  Tensor(const Tensor&) = delete;
+  Tensor& operator=(const Tensor&) = delete;
+  static bool alive(const Tensor* t);
 
   double* ptr() const { return data.as<double>(); }
   void set_layout(const Layout& l, hipStream_t s);       // allocate + zero
@@ -195,6 +204,10 @@ struct Plan {
   int64_t bytes = 0;                   // algorithmic bytes (inputs + output, FP64)
   int precision = 0;
   bool stripped = false;               // slot buffers released by strip(); run_plan re-allocates them first
+  // per slot: the FP64 input pointer the slot's FP32 copy was last converted from (nullptr: the copy holds nothing).
+  // A copy is converted again when its bit is set in convert_mask, when it was never converted, or when the slot is
+  // bound to a different input than the one it was converted from.
+  std::vector<const double*> conv_src;
   // Release the slot buffers (intermediates, FP32 operand copies), keeping the task lists; restore() allocates them
   // again (FP32 operand copies must then be converted again).  Returns whether anything was allocated.
   void strip();

@@ -91,6 +91,12 @@ class PlanCache {
     }
     return cp;
   }
+  // lookup that builds nothing (nullptr if the plan is no longer cached)
+  CachedPlan* peek(const std::vector<uint64_t>& key) {
+    auto it = map_.find(key);
+    if (it == map_.end()) return nullptr;
+    return find(key);
+  }
   CachedPlan* insert(const std::vector<uint64_t>& key, std::unique_ptr<CachedPlan> p) {
     ++stats_.builds;
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - miss_t0_).count();
@@ -172,6 +178,41 @@ PlanCache& cache() {
 }  // namespace
 
 PlanCacheStats plan_cache_stats() { return cache().stats(); }
+
+namespace {
+thread_local std::vector<uint64_t> t_plan_key;  // key of the plan the last plan_sigma_* call returned
+}
+
+void record_last_run(const std::vector<uint64_t>& key, const std::vector<Tensor*>& ins) {
+  LastRun& r = last_run();
+  r.key = key;
+  r.tensors.clear();
+  r.inputs.clear();
+  r.hashes.clear();
+  for (const Tensor* t : ins) {
+    r.tensors.push_back(t);
+    r.inputs.push_back(t->ptr());
+    r.hashes.push_back(t->L.hash);
+  }
+}
+
+void record_last_run(const std::vector<uint64_t>& key, std::initializer_list<const Tensor*> ins) {
+  std::vector<Tensor*> v;
+  for (const Tensor* t : ins) v.push_back(const_cast<Tensor*>(t));
+  record_last_run(key, v);
+}
+
+Plan& last_plan(std::vector<const double*>& inputs) {
+  LastRun& r = last_run();
+  if (r.key.empty()) throw Error(MPSXX_ERR_ARG, "profile: no hot-path call to profile on this thread");
+  for (size_t i = 0; i < r.tensors.size(); ++i)
+    if (!Tensor::alive(r.tensors[i]) || r.tensors[i]->ptr() != r.inputs[i] || r.tensors[i]->L.hash != r.hashes[i])
+      throw Error(MPSXX_ERR_ARG, "profile: an input of the last hot-path call was freed or changed since");
+  CachedPlan* cp = cache().peek(r.key);
+  if (!cp) throw Error(MPSXX_ERR_ARG, "profile: the plan of the last hot-path call was evicted from the cache");
+  inputs = r.inputs;
+  return cp->plan;
+}
 
 namespace {
 
@@ -350,8 +391,7 @@ void execute(const std::vector<uint64_t>& key, BuildFn build, int out_mode, std:
   run_plan_ex(P, ptrs, out.ptr(), s, {}, convert_mask);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
-  last_run().plan = &P;
-  last_run().inputs = ptrs;
+  record_last_run(key, ins);
 }
 
 const Layout* existing(const Tensor& t) { return t.empty ? nullptr : &t.L; }
@@ -559,9 +599,10 @@ Plan build_plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layo
 }
 
 Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
-                         const Layout& wfn, const Layout* out, int prec) {
+                         const Layout& wfn, const Layout* out, int prec, int copy) {
   std::vector<uint64_t> key{OP_SIGMA2, (uint64_t)prec, lmpo.hash, rmpo.hash, lopr.hash, ropr.hash, wfn.hash,
-                            out ? out->hash : 0ull};
+                            out ? out->hash : 0ull, (uint64_t)copy};
+  t_plan_key = key;
   CachedPlan* cp = cache().find(key);
   if (cp) return cp->plan;
   auto np = std::make_unique<CachedPlan>();
@@ -578,13 +619,14 @@ void op_sigma_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Te
   run_plan_ex(P, {lmpo.ptr(), rmpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s, {}, convert_mask);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
-  last_run().plan = &P;
-  last_run().inputs = {lmpo.ptr(), rmpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()};
+  record_last_run(t_plan_key, {&lmpo, &rmpo, &lopr, &ropr, &wfn});
 }
 
 Plan& plan_sigma_onesite(const Layout& mpo, const Layout& lopr, const Layout& ropr, const Layout& wfn,
-                         const Layout* out, int prec) {
-  std::vector<uint64_t> key{OP_SIGMA1, (uint64_t)prec, mpo.hash, lopr.hash, ropr.hash, wfn.hash, out ? out->hash : 0ull};
+                         const Layout* out, int prec, int copy) {
+  std::vector<uint64_t> key{OP_SIGMA1, (uint64_t)prec, mpo.hash, lopr.hash, ropr.hash, wfn.hash,
+                            out ? out->hash : 0ull, (uint64_t)copy};
+  t_plan_key = key;
   CachedPlan* cp = cache().find(key);
   if (cp) return cp->plan;
   auto np = std::make_unique<CachedPlan>();
@@ -606,12 +648,14 @@ void op_sigma_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tens
   run_plan_ex(P, {mpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s, {}, ~0u);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
-  last_run().plan = &P;
-  last_run().inputs = {mpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()};
+  record_last_run(t_plan_key, {&mpo, &lopr, &ropr, &wfn});
 }
 
-Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wfn, const Layout* out, int prec) {
-  std::vector<uint64_t> key{OP_SIGMAM, (uint64_t)prec, lopr.hash, ropr.hash, wfn.hash, out ? out->hash : 0ull};
+Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wfn, const Layout* out, int prec,
+                        int copy) {
+  std::vector<uint64_t> key{OP_SIGMAM, (uint64_t)prec, lopr.hash, ropr.hash, wfn.hash, out ? out->hash : 0ull,
+                            (uint64_t)copy};
+  t_plan_key = key;
   CachedPlan* cp = cache().find(key);
   if (cp) return cp->plan;
   auto np = std::make_unique<CachedPlan>();
@@ -635,8 +679,7 @@ void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int p
   run_plan_ex(P, {lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s, {}, convert_mask);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
-  last_run().plan = &P;
-  last_run().inputs = {lopr.ptr(), ropr.ptr(), wfn.ptr()};
+  record_last_run(t_plan_key, {&lopr, &ropr, &wfn});
 }
 
 void op_sigma_overlap(Tensor& lsov, Tensor& rsov, Tensor& wfn, Tensor& sgv, hipStream_t s) {

@@ -17,11 +17,20 @@ OpStats& last_stats();
 
 // The plan and input pointers of the last hot-path call on this thread (for per-kernel profiling; the plan stays
 // valid while it is cached, the inputs while the caller keeps the tensors).
+// The most recent hot-path call on this thread: the cache key of its plan and its input tensors with the data
+// pointer and structure each had then.  last_plan() looks the plan up again and checks that every input still
+// exists unchanged (error otherwise), so a plan evicted since, or a tensor freed or reshaped since, is never used.
 struct LastRun {
-  Plan* plan = nullptr;
+  std::vector<uint64_t> key;
+  std::vector<const Tensor*> tensors;
   std::vector<const double*> inputs;
+  std::vector<uint64_t> hashes;
 };
 LastRun& last_run();
+void record_last_run(const std::vector<uint64_t>& key, std::initializer_list<const Tensor*> ins);
+void record_last_run(const std::vector<uint64_t>& key, const std::vector<Tensor*>& ins);
+// The plan of last_run() and its current input pointers; throws MPSXX_ERR_ARG if there is none or it went stale.
+Plan& last_plan(std::vector<const double*>& inputs);
 
 // Generic BTAS machinery
 void op_contract(double alpha, Tensor& a, bool conj_a, const std::vector<int>& ca, Tensor& b, bool conj_b,
@@ -68,14 +77,17 @@ PlanCacheStats plan_cache_stats();
 // The most recently used cached plans are never evicted: a caller may hold references to at most this many plans
 // from the accessors below at once (the Davidson driver holds one per operator group).
 constexpr size_t kPlanCacheKeep = 16;
-// Plan for a two-site sigma vector (exposed for the Davidson driver, which binds device pointers itself).
+// Plan for a two-site sigma vector (exposed for the Davidson driver, which binds device pointers itself).  `copy`
+// selects one of several cached plans of the same structures: reduced-precision plans hold FP32 copies of their
+// operands, so operator groups of equal structure need plans of their own to keep their copies resident.
 Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
-                         const Layout& wfn, const Layout* out, int prec);
+                         const Layout& wfn, const Layout* out, int prec, int copy = 0);
 // Cached plan of the one-site sigma vector (compute_sigma_vector.hpp:10-22).
 Plan& plan_sigma_onesite(const Layout& mpo, const Layout& lopr, const Layout& ropr, const Layout& wfn,
-                         const Layout* out, int prec);
+                         const Layout* out, int prec, int copy = 0);
 // Cached plan of the merged 2-index sigma vector (compute_sigma_vector.hpp:86-96).
-Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wfn, const Layout* out, int prec);
+Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wfn, const Layout* out, int prec,
+                        int copy = 0);
 // The same plan, uncached and owned by the caller (batched launches keep one per instance: its intermediates are
 // that instance's scratch).
 Plan build_plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,

@@ -152,9 +152,11 @@ Lm, Rm, psim, info = workloads.qc_merged_problem(8, 48, seed=3)
 ref = oracle.sigma_merged(Lm.to_qst(), Rm.to_qst(), psim.to_qst())
 scale = max(float(np.max(np.abs(b))) for b in ref.blocks.values() if b.size)
 out = {}
+from mpsxx_amd.batch import last_plan_tiles
 for name, prec in (("fp64", mx.FP64), ("fp32", mx.FP32)):
     got = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=prec).to_qst()
-    out[name] = [got.block_set() == ref.block_set(), got.max_abs_diff(ref) / scale, got.diff_norm(ref) / ref.norm()]
+    out[name] = [got.block_set() == ref.block_set(), got.max_abs_diff(ref) / scale, got.diff_norm(ref) / ref.norm(),
+                 last_plan_tiles()["split_cut"]]
 print(json.dumps(out), flush=True)
 """
 
@@ -175,6 +177,7 @@ def test_merged_sigma_qc_with_every_split_reduction_cut_matches_oracle():
     res = json.loads(r.stdout.strip().splitlines()[-1])
     print(res, flush=True)
     assert res["fp64"][0] and res["fp32"][0]
+    assert res["fp64"][3] > 0 and res["fp32"][3] > 0  # the cut path really ran
     assert res["fp64"][1] <= 1e-12
     assert res["fp32"][2] <= 2e-5
 
