@@ -81,3 +81,23 @@ def random_twosite_problem(L, i, rng, dims, U=4.0, qt=None):
     renv = random_env_right(rq, rd, mpos[i + 1], rng)
     psi = mps.fill_allowed(mps.twosite_wavefunction(lq, ld, rq, rd, qt), rng)
     return mpos[i], mpos[i + 1], lenv, renv, psi
+
+
+def isometric_chain(L, i, qt, rng):
+    """Random chain (every bond sector of dimension 1) whose left tensors are left isometries and right tensors right
+    isometries: each column (fixed right sector) of an A and each row (fixed left sector) of a B has unit norm.  Then
+    H_eff = P^T H P with P an isometry, so its spectrum lies within the chain Hamiltonian's."""
+    ch = mps.random_chain(L, i, qt, rng, dims=lambda q: 1)
+    for a in ch["A"]:
+        for r in range(a.nsec(2)):
+            keys = [k for k in a.blocks if k[2] == r]
+            nrm = np.sqrt(sum(float(np.sum(a.blocks[k] ** 2)) for k in keys))
+            for k in keys:
+                a.blocks[k] = a.blocks[k] / nrm
+    for b in ch["B"].values():
+        for l in range(b.nsec(0)):
+            keys = [k for k in b.blocks if k[0] == l]
+            nrm = np.sqrt(sum(float(np.sum(b.blocks[k] ** 2)) for k in keys))
+            for k in keys:
+                b.blocks[k] = b.blocks[k] / nrm
+    return ch

@@ -65,14 +65,19 @@ def test_twosite_ex_reconverts_other_operands():
 
 def test_davidson_fp32_two_groups_of_equal_structure():
     """H = H_a + H_b with H_a, H_b built from the same MPO and right environment and a random elementwise split of
-    the left environment (equal block structures, different contents).  The mixed-precision solve must give the FP64
-    single-group energy, and its FP32 phase must converge on the right operator: from a vector converged in FP32 to
-    ||r||^2 < 1e-9, the FP64 refinement at tolerance 1e-9 needs no more than a few sigma vectors beyond what the
-    single-group solve needs."""
+    the left environment (equal block structures, different contents), on an isometric chain (bounded spectrum).
+    The split must not change the FP64 energy, and the mixed-precision solve must run its FP32 phase on H itself:
+    at tolerance 1e-6 the FP32 phase converges to ||r||^2 < 1e-6 and the FP64 refinement from that vector then
+    needs at most two sigma vectors, so the two-group solve takes no more sigma vectors than the one-group solve
+    plus two (an FP32 phase on the wrong operator would leave a large FP64 residual)."""
+    from helpers import chain_envs, isometric_chain
+    from mpsxx_amd.mpo import hubbard_mpo
     print("davidson fp32 equal-structure groups", flush=True)
     rng = np.random.default_rng(113)
-    L, i = 8, 3
-    mpos, ch, lenv, renv = hubbard_problem(L, i, rng, dmax=4)
+    L, i = 10, 4
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    ch = isometric_chain(L, i, (L, 0), rng)
+    lenv, renv = chain_envs(mpos, ch, i, L)
     psi = ch["psi"]
     la = lenv.copy()
     lb = lenv.copy()
@@ -84,11 +89,12 @@ def test_davidson_fp32_two_groups_of_equal_structure():
     D = mx.DTensor
     one = [(D(mpos[i]), D(mpos[i + 1]), D(lenv), D(renv))]
     two = [(D(mpos[i]), D(mpos[i + 1]), D(la), D(renv)), (D(mpos[i]), D(mpos[i + 1]), D(lb), D(renv))]
-    e64, _ = mx.davidson_twosite(one, D(diag), D(psi), 1e-12)
-    e2_64, _ = mx.davidson_twosite(two, D(diag), D(psi), 1e-12)
+    e64, _ = mx.davidson_twosite(one, D(diag), D(psi), 1e-10)
+    e2_64, _ = mx.davidson_twosite(two, D(diag), D(psi), 1e-10)
+    print("fp64 one group", e64, "two groups", e2_64, flush=True)
     assert abs(e2_64 - e64) < 1e-10 * max(1.0, abs(e64))
-    e1_32, n1 = mx.davidson_twosite(one, D(diag), D(psi), 1e-9, precision=mx.FP32)
-    e2_32, n2 = mx.davidson_twosite(two, D(diag), D(psi), 1e-9, precision=mx.FP32)
+    e1_32, n1 = mx.davidson_twosite(one, D(diag), D(psi), 1e-6, precision=mx.FP32)
+    e2_32, n2 = mx.davidson_twosite(two, D(diag), D(psi), 1e-6, precision=mx.FP32)
     print("fp32 one group", e1_32, n1, "two groups", e2_32, n2, flush=True)
-    assert abs(e2_32 - e64) < 1e-8 * max(1.0, abs(e64))
-    assert n2 <= n1 + 4, (n1, n2)
+    assert abs(e2_32 - e64) < 1e-6 * max(1.0, abs(e64))
+    assert n2 <= n1 + 2, (n1, n2)
