@@ -94,6 +94,14 @@ int mpsxx_copy_upcast(mpsxx_tensor x, mpsxx_tensor y, void* stream);
 /* ---- BLAS-1 on block-sparse arrays (QSTBLAS.h:40-93, 242-252); FP64 accumulation ---------------------------------- */
 int mpsxx_dotc(mpsxx_tensor x, mpsxx_tensor y, double* result, void* stream);     /* synchronous               */
 int mpsxx_nrm2(mpsxx_tensor x, double* result, void* stream);                     /* synchronous               */
+/* mpsxx_dotc with no device allocation: no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The segment list,
+ * chunk list, partial sums and result live in a static 16 MiB scratch array that every device holds until the process
+ * ends (its address is resolved once per device, at mpsxx_init or on the first call); returns MPSXX_ERR_ARG when one
+ * call's lists need more (about 420,000 block segments, or 10^9 elements in 1,024-element chunks).  Same kernels and
+ * summation order as mpsxx_dotc, so the same result bit for bit.  Calls that overlap on one device race on the
+ * scratch array.  Synchronous. */
+int mpsxx_dotc_noalloc(mpsxx_tensor x, mpsxx_tensor y, double* result, void* stream);
 int mpsxx_axpy(double alpha, mpsxx_tensor x, mpsxx_tensor y, void* stream);       /* inserts missing y blocks  */
 int mpsxx_scal(double alpha, mpsxx_tensor x, void* stream);
 

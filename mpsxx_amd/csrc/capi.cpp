@@ -8,6 +8,7 @@
 
 #include "batch.hpp"
 #include "mpsxx_amd.h"
+#include "noalloc.hpp"
 #include "ops.hpp"
 #include "qcmpo.hpp"
 
@@ -19,6 +20,7 @@ bool g_init = false;
 void ensure_init() {
   if (!g_init) {
     mpx::hip_check(hipSetDevice(0), "hipSetDevice");
+    mpx::noalloc_init(0);
     g_init = true;
   }
 }
@@ -113,6 +115,7 @@ extern "C" {
 int mpsxx_init(int device) {
   try {
     mpx::hip_check(hipSetDevice(device), "hipSetDevice");
+    mpx::noalloc_init(device);  // the scratch array's address on this device
     g_init = true;
     return MPSXX_OK;
   } catch (const mpx::Error& e) {
@@ -254,6 +257,13 @@ int mpsxx_copy_upcast(mpsxx_tensor x, mpsxx_tensor y, void* stream) {
 
 int mpsxx_dotc(mpsxx_tensor x, mpsxx_tensor y, double* result, void* stream) {
   return guard([&] { *result = mpx::op_dotc(T(x), T(y), S(stream)); });
+}
+
+int mpsxx_dotc_noalloc(mpsxx_tensor x, mpsxx_tensor y, double* result, void* stream) {
+  return guard([&] {
+    if (!result) throw mpx::Error(MPSXX_ERR_ARG, "Dotc (no allocation): null result pointer");
+    *result = mpx::op_dotc_noalloc(T(x), T(y), S(stream));
+  });
 }
 
 int mpsxx_nrm2(mpsxx_tensor x, double* result, void* stream) {

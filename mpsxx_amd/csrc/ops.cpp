@@ -13,6 +13,7 @@
 #include <memory>
 
 #include "mpsxx_amd.h"
+#include "noalloc.hpp"
 
 namespace mpx {
 
@@ -484,8 +485,11 @@ struct SegList {
   std::vector<Chunk> chunks;
 };
 
-SegList match_segments(const Layout& x, const Layout& y) {
-  SegList r;
+// segments y[dst] <- x[src] of the blocks x and y share, and their kAxpyChunk-element chunks (r's lists are
+// overwritten, keeping their capacity)
+void match_segments_into(const Layout& x, const Layout& y, SegList& r) {
+  r.segs.clear();
+  r.chunks.clear();
   for (int64_t b = 0; b < x.nblocks(); ++b) {
     const int64_t j = y.find(x.tag[b]);
     if (j < 0 || x.bsize[b] == 0) continue;
@@ -493,6 +497,11 @@ SegList match_segments(const Layout& x, const Layout& y) {
   }
   for (size_t i = 0; i < r.segs.size(); ++i)
     for (int s = 0; s < r.segs[i].len; s += kAxpyChunk) r.chunks.push_back({(int32_t)i, s});
+}
+
+SegList match_segments(const Layout& x, const Layout& y) {
+  SegList r;
+  match_segments_into(x, y, r);
   return r;
 }
 
@@ -533,6 +542,37 @@ double op_dotc(Tensor& x, Tensor& y, hipStream_t s) {
   launch_dot(dch.as<Chunk>(), (int)m.chunks.size(), dsegs.as<CopySeg>(), x.ptr(), y.ptr(), part, d_out, s);
   double h = 0.0;
   hip_check(hipMemcpyAsync(&h, d_out, sizeof(double), hipMemcpyDeviceToHost, s), "dot result");
+  hip_check(hipStreamSynchronize(s), "dot sync");
+  return h;
+}
+
+double op_dotc_noalloc(Tensor& x, Tensor& y, hipStream_t s) {
+  if (x.empty || y.empty) return 0.0;
+  check_same_q(x, y, "Dotc");
+  static thread_local SegList m;  // host lists, reused from call to call
+  match_segments_into(y.L, x.L, m);  // dst = x offset, src = y offset, as op_dotc
+  // scratch layout: segments | chunks | partial sums | result, each part 256-byte aligned
+  auto up = [](size_t n) { return (n + 255) & ~size_t(255); };
+  const size_t nseg = m.segs.size(), nch = m.chunks.size();
+  const size_t off_ch = up(nseg * sizeof(CopySeg)), off_part = off_ch + up(nch * sizeof(Chunk));
+  const size_t off_res = off_part + up((nch + 1) * sizeof(double)), total = off_res + 256;
+  if (total > kNoAllocScratchBytes)
+    throw Error(MPSXX_ERR_ARG, "Dotc (no allocation): " + std::to_string(nseg) + " block segments in " +
+                                   std::to_string(nch) + " chunks need " + std::to_string(total) +
+                                   " bytes of scratch, more than the " + std::to_string(kNoAllocScratchBytes) +
+                                   " the library holds per device");
+  char* base = static_cast<char*>(noalloc_scratch());
+  if (nseg)
+    hip_check(hipMemcpyAsync(base, m.segs.data(), nseg * sizeof(CopySeg), hipMemcpyHostToDevice, s), "dot segments");
+  if (nch)
+    hip_check(hipMemcpyAsync(base + off_ch, m.chunks.data(), nch * sizeof(Chunk), hipMemcpyHostToDevice, s),
+              "dot chunks");
+  launch_dot(reinterpret_cast<const Chunk*>(base + off_ch), (int)nch, reinterpret_cast<const CopySeg*>(base),
+             x.ptr(), y.ptr(), reinterpret_cast<double*>(base + off_part), reinterpret_cast<double*>(base + off_res),
+             s);
+  hip_check(hipGetLastError(), "dot launch");
+  double h = 0.0;
+  hip_check(hipMemcpyAsync(&h, base + off_res, sizeof(double), hipMemcpyDeviceToHost, s), "dot result");
   hip_check(hipStreamSynchronize(s), "dot sync");
   return h;
 }

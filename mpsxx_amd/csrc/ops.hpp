@@ -42,6 +42,10 @@ void op_copy_upcast(Tensor& x, Tensor& y, hipStream_t s);
 
 // BLAS-1
 double op_dotc(Tensor& x, Tensor& y, hipStream_t s);
+// The same dot product with its segment list, chunk list, partial sums and result in the device's static scratch
+// array (noalloc.hip) instead of library-allocated buffers: no device allocation.  Calls that overlap on one device
+// race on that array.  Throws MPSXX_ERR_ARG when the lists do not fit it.
+double op_dotc_noalloc(Tensor& x, Tensor& y, hipStream_t s);
 double op_nrm2(Tensor& x, hipStream_t s);
 void op_axpy(double alpha, Tensor& x, Tensor& y, hipStream_t s);
 void op_scal(double alpha, Tensor& x, hipStream_t s);
