@@ -186,6 +186,22 @@ int mpsxx_davidson_twosite_ex(int ngroups, const mpsxx_tensor* lmpo, const mpsxx
  * Keeps every singular value >= the M-th largest (M <= 0: all >= 1e-16); returns the discarded weight. */
 int mpsxx_canonicalize_twosite(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmps, mpsxx_tensor rmps, int max_states,
                                double* discarded, void* stream);
+/* Truncation rules.  REFERENCE: exactly QSTLAPACK.h:209-222 (cutoff = the M-th largest singular value when
+ * 0 < M < count, else 1e-16; every value >= cutoff kept; singular values as computed).  STABLE (the default of
+ * mpsxx_canonicalize_twosite and of the sweeps): values within a relative 1e-12 of the M-th largest count as ties and
+ * are kept, the cutoff never drops below 1e-16, and values below max(m, n) eps |A|_F of their block are exact zeros.
+ * The two differ only on near-ties at the cutoff and on values at the rounding level, where the reference's own
+ * choice depends on the rounding of its SVD. */
+enum { MPSXX_TRUNCATION_REFERENCE = 0, MPSXX_TRUNCATION_STABLE = 1 };
+/* canonicalize(forward, wfnc, lmps, rmps, M, noise) (src/canonicalize.hpp:27-46): make_noise on wfnc in place, as the
+ * reference does, then the SVD with the given truncation rule. */
+int mpsxx_canonicalize_twosite_ex(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmps, mpsxx_tensor rmps,
+                                  int max_states, double noise, int rule, double* discarded, void* stream);
+/* make_noise (src/canonicalize.hpp:11-24): nothing for noise < 1e-8, else x += noise * p, where p holds every allowed
+ * block of x's net dense shape (sectors without a stored block excluded), each filled row-major with the draws of a
+ * fresh default-seeded std::mt19937 through std::uniform_real_distribution<double>(-1, 1), as the reference's
+ * generator constructor fills them; missing blocks are inserted.  Synchronous. */
+int mpsxx_make_noise(mpsxx_tensor x, double noise, void* stream);
 
 /* ---- device-resident two-site sweeps (dmrg/make_sweep.cpp:12-254, src/make_random_mpss.cpp:97-272) --------------- */
 typedef struct mpsxx_dmrg_s* mpsxx_dmrg;
@@ -215,6 +231,18 @@ int mpsxx_dmrg_site(mpsxx_dmrg d, int site, mpsxx_tensor out, void* stream);
  * the refinement dmrg/dmrg.cpp:76-114 runs after the two-site sweeps have converged. */
 enum { MPSXX_SWEEP_FOURINDEX = 0, MPSXX_SWEEP_MERGED = 1, MPSXX_SWEEP_ONESITE = 2 };
 int mpsxx_dmrg_set_form(mpsxx_dmrg d, int form);
+/* Noise and truncation rule of later sweeps: DMRGInput::noise (src/input.h:43, default 0), added by make_noise before
+ * every two-site or merged canonicalize (src/optimize.hpp:210,237,586,618 -> canonicalize.hpp:35,95); rule is
+ * MPSXX_TRUNCATION_STABLE (default) or MPSXX_TRUNCATION_REFERENCE.  One-site sweeps reject noise > 0. */
+int mpsxx_dmrg_set_truncation(mpsxx_dmrg d, double noise, int rule);
+/* The Davidson solves of the last sweep, in order (the reference logs them, diagonalize.hpp:317,363): per solve the
+ * site (two-site pair (site, site+1) or one-site centre), the direction, the eigenvalue and the number of micro
+ * iterations; evals holds every solve's lowest Ritz value after each subspace solve, concatenated, and rnorms the
+ * squared residual norm tested after it (the convergence test is rnorm < tolerance).  Call with null arrays for the
+ * counts. */
+int mpsxx_dmrg_sweep_trace(mpsxx_dmrg d, int32_t max_steps, int32_t* nsteps, int32_t* step_site,
+                           int32_t* step_forward, double* step_energy, int32_t* step_nevals, int32_t max_evals,
+                           int32_t* nevals, double* evals, double* rnorms);
 /* Excited states (dmrg/dmrg.cpp:36-72 with iroot > 0, dmrg/make_sweep.cpp, dmrg/optimize.hpp:127-230): register the
  * converged state `root` (energy E_k) as a lower root of `d`.  Both must have the centre at site 0 (as after
  * mpsxx_dmrg_create or a sweep).  Later sweeps of d project every lower root out of its Davidson solves and keep

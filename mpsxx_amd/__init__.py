@@ -303,6 +303,27 @@ def davidson_twosite(groups, diag, wfnc, tol, precision=FP64, lower=()):
     return e.value, ns.value
 
 
+TRUNCATION_REFERENCE = 0
+TRUNCATION_STABLE = 1
+
+
+def canonicalize_twosite(forward, wfnc, M, noise=0.0, rule=TRUNCATION_STABLE):
+    """canonicalize (src/canonicalize.hpp:27-46): make_noise on wfnc in place, then the blocked SVD truncated to M
+    states with the given rule.  Returns (lmps, rmps, discarded weight)."""
+    U, V = DTensor(), DTensor()
+    d = ctypes.c_double()
+    _check(lib().mpsxx_canonicalize_twosite_ex(ctypes.c_int(1 if forward else 0), _h(wfnc), _h(U), _h(V),
+                                               ctypes.c_int(M), ctypes.c_double(noise), ctypes.c_int(rule),
+                                               ctypes.byref(d), None))
+    return U, V, d.value
+
+
+def make_noise(x, noise):
+    """make_noise (src/canonicalize.hpp:11-24), in place."""
+    _check(lib().mpsxx_make_noise(_h(x), ctypes.c_double(noise), None))
+    return x
+
+
 def qst_merge(x, nrow, ncol, out=None):
     """QSTmerge of the first nrow and the last ncol indices (include/btas/QSPARSE/QSTmerge.h)."""
     out = _out(out)

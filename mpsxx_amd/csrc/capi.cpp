@@ -366,6 +366,20 @@ int mpsxx_canonicalize_twosite(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmps
   });
 }
 
+int mpsxx_canonicalize_twosite_ex(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmps, mpsxx_tensor rmps,
+                                  int max_states, double noise, int rule, double* discarded, void* stream) {
+  return guard([&] {
+    mpx::TruncationScope ts(rule);
+    mpx::op_make_noise(T(wfnc), noise, S(stream));
+    const double d = mpx::op_canonicalize_twosite(forward != 0, T(wfnc), T(lmps), T(rmps), max_states, S(stream));
+    if (discarded) *discarded = d;
+  });
+}
+
+int mpsxx_make_noise(mpsxx_tensor x, double noise, void* stream) {
+  return guard([&] { mpx::op_make_noise(T(x), noise, S(stream)); });
+}
+
 int mpsxx_compute_merged_overlap(int forward, mpsxx_tensor xmpo, mpsxx_tensor xsov, mpsxx_tensor msov,
                                  void* stream) {
   return guard([&] { mpx::op_merged_overlap(forward != 0, T(xmpo), T(xsov), T(msov), S(stream)); });

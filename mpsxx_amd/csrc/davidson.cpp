@@ -281,6 +281,7 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
       std::vector<double> rval;
       sym_eig(m, heff, rval, R);
       eval = rval[0];
+      st.evals.push_back(eval);
       // rotate: trial_j <- sum_i rvec(i,j) trial_i (and sigma), diagonalize.hpp:378-394.  The host staging array
       // outlives the asynchronous copy (the next write to it follows several stream synchronisations).
       for (int j = 0; j < m; ++j)
@@ -298,6 +299,7 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
       hip_check(hipMemcpyAsync(&rnorm, gram.as<double>() + kNrm, sizeof(double), hipMemcpyDeviceToHost, s), "rnorm");
       hip_check(hipStreamSynchronize(s), "rnorm sync");
       st.last_rnorm = rnorm;
+      st.rnorms.push_back(rnorm);
       if (rnorm < tol_p) {
         ++iconv;
         break;

@@ -1177,7 +1177,7 @@ __global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restric
 __global__ __launch_bounds__(256) void svd_finish_kernel(const SvdOut* __restrict__ outs,
                                                          const double* __restrict__ Xn, const double* __restrict__ Va,
                                                          double* S, int kmax, double* Ua, int64_t su_a, double* Vta,
-                                                         int64_t svt_a, int nmax) {
+                                                         int64_t svt_a, int nmax, int zero_rank) {
   extern __shared__ double sig[];  // kk values, then kk ranks
   const SvdOut o = outs[blockIdx.x];
   const int kk = o.kk, lx = o.u_from_x ? o.m : o.n, lv = o.u_from_x ? o.n : o.m;
@@ -1195,9 +1195,10 @@ __global__ __launch_bounds__(256) void svd_finish_kernel(const SvdOut* __restric
     if (lane == 0) sig[j] = sqrt(ss);
   }
   __syncthreads();
-  // numerical rank: values below max(m, n) eps |A|_F are at the rounding level of the decomposition (the
-  // backward-error bound of an orthogonal one); they are reported as exactly zero, so noise directions are never kept
-  if (w == 0) {
+  // numerical rank (zero_rank != 0): values below max(m, n) eps |A|_F are at the rounding level of the decomposition
+  // (the backward-error bound of an orthogonal one); they are reported as exactly zero, so noise directions are never
+  // kept
+  if (w == 0 && zero_rank) {
     double fro = 0.0;
     for (int j = lane; j < kk; j += 64) fro += sig[j] * sig[j];
     fro = wave_sum(fro);
@@ -1543,11 +1544,11 @@ void launch_svd_round(const SvdTask* tasks, int ntasks, const SvdBlk* blks, doub
 }
 
 void launch_svd_finish(const SvdOut* outs, int nb, const double* Xn, const double* Va, double* S, int kmax, double* Ua,
-                       int64_t su_a, double* Vta, int64_t svt_a, int nmax, int kk_max, hipStream_t s) {
+                       int64_t su_a, double* Vta, int64_t svt_a, int nmax, int kk_max, int zero_rank, hipStream_t s) {
   if (nb <= 0) return;
   const size_t lds = (size_t)kk_max * (sizeof(double) + sizeof(int));
   hipLaunchKernelGGL(svd_finish_kernel, dim3(nb), dim3(256), lds, s, outs, Xn, Va, S, kmax, Ua, su_a, Vta, svt_a,
-                     nmax);
+                     nmax, zero_rank);
 }
 
 void launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s) {

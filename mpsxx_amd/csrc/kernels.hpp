@@ -214,12 +214,13 @@ void launch_svd_qr_apply(const SvdQrBlk* blks, int nb, const double* X, const do
 // into S[blk * kmax + r].  U_A (m x kmax row-major, slot stride su_a) and Vt_A (kmax x nmax row-major, slot stride
 // svt_a) are filled as a LAPACK-style gesvd of A returns them: one of them from xn / sigma (U_A when u_from_x), the
 // other from va (kk columns of length n when u_from_x, else m).  Columns of zero singular value are left zero.
+// zero_rank != 0: values below max(m, n) eps |A|_F of their block are reported as exact zeros.
 struct SvdOut {
   int64_t xn_off, va_off;
   int32_t m, n, kk, u_from_x;
 };
 void launch_svd_finish(const SvdOut* outs, int nb, const double* Xn, const double* Va, double* S, int kmax, double* Ua,
-                       int64_t su_a, double* Vta, int64_t svt_a, int nmax, int kk_max, hipStream_t s);
+                       int64_t su_a, double* Vta, int64_t svt_a, int nmax, int kk_max, int zero_rank, hipStream_t s);
 
 // Davidson helpers on flat vectors of one layout (length n).
 // y = sum_i c[i] * X_i  with X_i = base + i*ld  (FP64; c lives in device memory)

@@ -126,6 +126,17 @@ struct MergedOpPlan {
 bool merged_operator_plan(bool fwd, const Layout& W, const Layout& E, MergedOpPlan& P);
 void op_merged_overlap(bool fwd, Tensor& xmpo, Tensor& xsov, Tensor& msov, hipStream_t s);
 
+// Truncation rule of the canonicalizations on this thread (MPSXX_TRUNCATION_STABLE by default); set_ returns the
+// previous rule.
+int truncation_rule();
+int set_truncation_rule(int rule);
+struct TruncationScope {
+  int old;
+  explicit TruncationScope(int rule) : old(set_truncation_rule(rule)) {}
+  ~TruncationScope() { set_truncation_rule(old); }
+};
+// make_noise (src/canonicalize.hpp:11-24): w += noise * U(-1, 1) draws, in place (nothing for noise < 1e-8).
+void op_make_noise(Tensor& w, double noise, hipStream_t s);
 // Two-site canonicalization with truncation to M states (src/canonicalize.hpp:27-46); returns the discarded weight.
 double op_canonicalize_twosite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor& rmps, int M, hipStream_t s);
 // One-site form (src/canonicalize.hpp:49-85, noise 0): forward (LeftArrow) lmps = U(l,n,s), rmps = s*Vt(s,r);
@@ -149,6 +160,8 @@ struct DavGroup {
   int wfn_rank() const { return merged() ? 2 : onesite() ? 3 : 4; }
 };
 struct DavidsonStats {
+  std::vector<double> evals;   // lowest Ritz value after each subspace solve (diagonalize.hpp:363's micro iterations)
+  std::vector<double> rnorms;  // ||r||^2 of the residual tested after each of them (diagonalize.hpp:402-408)
   int nsigma = 0;
   int nsigma_fp32 = 0;    // sigma vectors applied in FP32 (mixed precision)
   double sigma_ms = 0.0;  // device time spent in sigma vectors

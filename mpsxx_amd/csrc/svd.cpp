@@ -14,7 +14,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
+#include <climits>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -26,6 +28,19 @@
 
 namespace mpx {
 
+namespace {
+thread_local int t_trunc_rule = MPSXX_TRUNCATION_STABLE;
+}
+
+int truncation_rule() { return t_trunc_rule; }
+
+int set_truncation_rule(int rule) {
+  if (rule != MPSXX_TRUNCATION_REFERENCE && rule != MPSXX_TRUNCATION_STABLE)
+    throw Error(MPSXX_ERR_ARG, "canonicalize: unknown truncation rule");
+  const int old = t_trunc_rule;
+  t_trunc_rule = rule;
+  return old;
+}
 
 namespace {
 
@@ -53,6 +68,66 @@ void run_copy(const std::vector<CopySeg>& segs, const double* x, double* y, hipS
 }
 
 }  // namespace
+
+// make_noise (src/canonicalize.hpp:11-24).  The reference builds wfpt = QSTArray(q, qshape, check_net_dshape(),
+// std::bind(uniform_real_distribution(-1, 1), mt19937())) and adds noise * wfpt.  Its generator constructor passes the
+// generator by value to every block (STArray.h:243-262, TArray.h:779-782), so every block is filled, row-major, by a
+// fresh copy of the default-seeded engine: element e of any block is the e-th draw.  The draws come from the same
+// standard-library engine and distribution on the host, once, and one copy launch adds noise * draws[0, len) to every
+// block of the net-dense-shape pattern (every allowed index tuple whose sectors all hold a stored block), inserting
+// the missing ones as Axpy does (STBLAS.h:135-145).
+void op_make_noise(Tensor& w, double noise, hipStream_t s) {
+  if (noise < 1.0e-8 || w.empty) return;
+  const Layout& X = w.L;
+  const int N = X.rank;
+  std::vector<std::vector<char>> present(N);
+  for (int i = 0; i < N; ++i) present[i].assign(X.ds[i].size(), 0);
+  for (int64_t b = 0; b < X.nblocks(); ++b)
+    for (int i = 0; i < N; ++i) present[i][X.index(b)[i]] = 1;
+  Layout U = X;
+  int64_t combos = 1;
+  for (int i = 0; i < N; ++i) combos *= std::max(1, X.nsec(i));
+  if (combos > (int64_t)1 << 26) throw Error(MPSXX_ERR_ARG, "make_noise: too many sector combinations");
+  std::vector<int32_t> idx(N, 0);
+  for (int64_t t = 0; t < combos && N > 0; ++t) {
+    int64_t r = t;
+    bool ok = true;
+    for (int i = N - 1; i >= 0; --i) {
+      idx[i] = (int32_t)(r % X.nsec(i));
+      r /= X.nsec(i);
+      ok = ok && present[i][idx[i]] && X.ds[i][idx[i]] > 0;
+    }
+    if (ok && X.allowed(idx.data())) U.add_block(idx.data());
+  }
+  U.finalize();
+  w.relayout(U, s);
+  int64_t maxlen = 0;
+  for (int64_t b = 0; b < U.nblocks(); ++b) maxlen = std::max(maxlen, U.bsize[b]);
+  if (maxlen > INT32_MAX) throw Error(MPSXX_ERR_ARG, "make_noise: block too large");
+  // the draw sequence of the default-seeded engine, extended on demand and kept on the device
+  static std::vector<double> draws;
+  static DevBuf ddraws;
+  if ((int64_t)draws.size() < maxlen) {
+    std::mt19937 rgen;
+    std::uniform_real_distribution<double> dist(-1.0, 1.0);
+    draws.resize((size_t)maxlen);
+    for (double& v : draws) v = dist(rgen);
+    ddraws = upload_vec(draws);
+  }
+  std::vector<CopySeg> segs;
+  std::vector<Chunk> chunks;
+  for (int64_t b = 0; b < U.nblocks(); ++b) {
+    if (U.bsize[b] == 0) continue;
+    segs.push_back({U.off[b], 0, (int32_t)U.bsize[b], 1});
+    for (int c = 0; c < segs.back().len; c += kAxpyChunk) chunks.push_back({(int32_t)(segs.size() - 1), c});
+  }
+  if (chunks.empty()) return;
+  DevBuf ds = upload_vec(segs), dc = upload_vec(chunks);
+  launch_copy<double, double>(dc.as<Chunk>(), (int)chunks.size(), ds.as<CopySeg>(), ddraws.as<double>(), w.ptr(), noise,
+                              1.0, s);
+  hip_check(hipGetLastError(), "make_noise launch");
+  hip_check(hipStreamSynchronize(s), "make_noise");
+}
 
 namespace {
 
@@ -333,7 +408,7 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
   hip_check(hipMemsetAsync(Ua, 0, nb * sv * sizeof(double), s), "memset");
   hip_check(hipMemsetAsync(Vta, 0, nb * su * sizeof(double), s), "memset");
   launch_svd_finish(douts.as<SvdOut>(), nb, Xj, qr ? Z.as<double>() : V.as<double>(), Sv, kmax, Ua, sv, Vta, su, nmax,
-                    kk_max, s);
+                    kk_max, truncation_rule() == MPSXX_TRUNCATION_STABLE ? 1 : 0, s);
   hip_check(hipGetLastError(), "svd finish launch");
   hip_check(hipStreamSynchronize(s), "svd finish");
   phase();  // 4: Q application and output
@@ -441,7 +516,7 @@ double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo&
   } else {
     rocsolver_svd(blks, rects, src, src_elems, mmax, nmax, kmax, sa, su, sv, Sv, U, V, sv_h, s);
     // numerical rank as on the Jacobi path: values below max(m, n) eps |A|_F count as zero
-    for (int k = 0; k < nb; ++k) {
+    for (int k = 0; k < nb && truncation_rule() == MPSXX_TRUNCATION_STABLE; ++k) {
       const int kk = std::min(blks[k].m, blks[k].n);
       double fro = 0.0;
       for (int r = 0; r < kk; ++r) fro += sv_h[(size_t)k * kmax + r] * sv_h[(size_t)k * kmax + r];
@@ -455,12 +530,14 @@ double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo&
   for (int k = 0; k < nb; ++k)
     for (int r = 0; r < std::min(blks[k].m, blks[k].n); ++r) all.push_back(sv_h[(size_t)k * kmax + r]);
   std::sort(all.rbegin(), all.rend());
-  // tie rule: values within a relative 1e-12 of the M-th largest count as equal to it, so exact degeneracies (e.g.
-  // the +Sz/-Sz sector pairs) that rounding splits are kept together on every backend
-  // (never below the default 1e-16: when fewer than M values exceed it, zeros and rounding-level values are not kept
+  // MPSXX_TRUNCATION_REFERENCE: cutoff = the M-th largest value (0 < M < count), else 1e-16 (QSTLAPACK.h:209-213).
+  // MPSXX_TRUNCATION_STABLE: values within a relative 1e-12 of the M-th largest count as equal to it, so exact
+  // degeneracies (e.g. the +Sz/-Sz sector pairs) that rounding splits are kept together on every backend, and the
+  // cutoff never drops below 1e-16 (when fewer than M values exceed it, zeros and rounding-level values are not kept
   // just because the M-th largest is one of them)
   double cutoff = 1.0e-16;
-  if (M > 0 && M < (int)all.size()) cutoff = std::max(cutoff, all[M - 1] * (1.0 - 1e-12));
+  if (M > 0 && M < (int)all.size())
+    cutoff = truncation_rule() == MPSXX_TRUNCATION_REFERENCE ? all[M - 1] : std::max(cutoff, all[M - 1] * (1.0 - 1e-12));
   double dnorm = 0.0;
   std::vector<int> kept(nb, 0);
   for (int k = 0; k < nb; ++k) {

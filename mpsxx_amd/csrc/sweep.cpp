@@ -33,14 +33,25 @@ struct LowerRoot {
   std::vector<std::unique_ptr<Tensor>> lH, rH, lS, rS;
 };
 
+// One Davidson solve of a sweep: the site pair (site, site + 1) or one-site centre, its direction, the eigenvalue and
+// the lowest Ritz value after every subspace solve (the reference prints them as micro iterations).
+struct StepTrace {
+  int site = 0, forward = 1;
+  double energy = 0.0;
+  std::vector<double> evals, rnorms;
+};
+
 struct Dmrg {
   int L = 0;
+  std::vector<StepTrace> trace;  // the Davidson solves of the last sweep, in order
   std::vector<Tensor*> mpo;                 // borrowed
   std::vector<std::unique_ptr<Tensor>> mps; // site tensors (A left of the centre, wave at the centre, B right)
   std::vector<std::unique_ptr<Tensor>> lenv, renv;  // lenv[i]: left of site i; renv[i]: right of site i-1
   int center = 0;
   int form = MPSXX_SWEEP_FOURINDEX;  // two-site 4-index, merged (src/make_sweep.cpp:112) or one-site
   std::vector<LowerRoot> lower;
+  double noise = 0.0;                   // DMRGInput::noise (src/input.h:43), passed to every canonicalize
+  int trunc_rule = MPSXX_TRUNCATION_STABLE;
 };
 
 namespace {
@@ -172,6 +183,7 @@ void optimize_merged(Dmrg& D, int i, bool forward, Tensor& psi, int M, double to
   DavidsonStats ds;
   std::vector<DavGroup> g{{nullptr, nullptr, &Lm, &Rm}};
   const double e = davidson_twosite(g, diag, wm, 0.1 * tol, prec, ds, s);
+  D.trace.push_back({i, forward ? 1 : 0, e, ds.evals, ds.rnorms});
   st.davidson_s += since(t);
   ph[3] = 1e3 * since(t);
   st.nsigma += ds.nsigma;
@@ -180,6 +192,7 @@ void optimize_merged(Dmrg& D, int i, bool forward, Tensor& psi, int M, double to
   if (e < emin) emin = e;
   t = clk::now();
   Tensor U, V;
+  op_make_noise(wm, D.noise, s);  // canonicalize of the merged wavefunction (canonicalize.hpp:88-106)
   st.dweight += op_canonicalize_merged(forward, wm, psi.L, U, V, M, s);
   st.svd_s += since(t);
   ph[4] = 1e3 * since(t);
@@ -214,6 +227,11 @@ void optimize_merged(Dmrg& D, int i, bool forward, Tensor& psi, int M, double to
 // multiplied into the next site, which becomes the wave.  Forward over sites 0..L-2, backward over L-1..1; the centre
 // ends at site 0.
 double dmrg_sweep_onesite(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStream_t s) {
+  if (D.noise >= 1.0e-8)
+    throw Error(MPSXX_ERR_UNSUPPORTED, "dmrg: noise is not supported in one-site sweeps (canonicalize.hpp:57-66 "
+                                       "turns them into two-site decompositions)");
+  TruncationScope trunc(D.trunc_rule);
+  D.trace.clear();
   const int L = D.L;
   const auto t0 = clk::now();
   double emin = 1.0e8;
@@ -238,6 +256,7 @@ double dmrg_sweep_onesite(Dmrg& D, int M, double tol, int prec, SweepStats& st, 
       low.push_back({&Hk[k], &Sk[k], R.energy});
     }
     const double e = davidson_twosite(g, diag, psi, 0.1 * tol, prec, ds, s, low);
+    D.trace.push_back({i, forward ? 1 : 0, e, ds.evals, ds.rnorms});
     st.davidson_s += since(t);
     st.nsigma += ds.nsigma;
     st.nsigma_fp32 += ds.nsigma_fp32;
@@ -289,6 +308,8 @@ double dmrg_sweep_onesite(Dmrg& D, int M, double tol, int prec, SweepStats& st, 
 
 // One forward + one backward two-site sweep; returns the minimum site energy.
 double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStream_t s) {
+  TruncationScope trunc(D.trunc_rule);
+  D.trace.clear();
   const int L = D.L;
   if (D.form == MPSXX_SWEEP_MERGED && !D.lower.empty())
     throw Error(MPSXX_ERR_UNSUPPORTED, "dmrg: lower roots are not supported with the merged sweep form");
@@ -322,6 +343,7 @@ double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStrea
       low.push_back({&Hk[k], &Sk[k], R.energy});
     }
     const double e = davidson_twosite(g, diag, psi, 0.1 * tol, prec, ds, s, low);
+    D.trace.push_back({i, forward ? 1 : 0, e, ds.evals, ds.rnorms});
     st.davidson_s += since(t);
     st.nsigma += ds.nsigma;
     st.nsigma_fp32 += ds.nsigma_fp32;
@@ -329,6 +351,7 @@ double dmrg_sweep(Dmrg& D, int M, double tol, int prec, SweepStats& st, hipStrea
     if (e < emin) emin = e;
     t = clk::now();
     Tensor U, V;
+    op_make_noise(psi, D.noise, s);  // canonicalize.hpp:35
     st.dweight += op_canonicalize_twosite(forward, psi, U, V, M, s);
     st.svd_s += since(t);
     st.max_bond = std::max(st.max_bond, bond_dim(U.L, 2));
@@ -576,6 +599,43 @@ int mpsxx_dmrg_add_lower_root(mpsxx_dmrg d, mpsxx_dmrg root, double energy, void
     if (!d || !d->d || !root || !root->d) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: null handle");
     if (d == root) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: a state cannot be its own lower root");
     mpx::dmrg_add_lower_root(*d->d, *root->d, energy, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int mpsxx_dmrg_sweep_trace(mpsxx_dmrg d, int32_t max_steps, int32_t* nsteps, int32_t* step_site,
+                           int32_t* step_forward, double* step_energy, int32_t* step_nevals, int32_t max_evals,
+                           int32_t* nevals, double* evals, double* rnorms) {
+  return dguard([&] {
+    if (!d || !d->d) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: null handle");
+    const auto& tr = d->d->trace;
+    int64_t ne = 0;
+    for (const auto& t : tr) ne += (int64_t)t.evals.size();
+    if (nsteps) *nsteps = (int32_t)tr.size();
+    if (nevals) *nevals = (int32_t)ne;
+    if (!step_site && !step_forward && !step_energy && !step_nevals && !evals && !rnorms) return;  // counts only
+    if ((int64_t)tr.size() > max_steps || ne > max_evals) throw mpx::Error(MPSXX_ERR_ARG, "dmrg trace: arrays too small");
+    int64_t k = 0;
+    for (size_t i = 0; i < tr.size(); ++i) {
+      if (step_site) step_site[i] = tr[i].site;
+      if (step_forward) step_forward[i] = tr[i].forward;
+      if (step_energy) step_energy[i] = tr[i].energy;
+      if (step_nevals) step_nevals[i] = (int32_t)tr[i].evals.size();
+      for (size_t j = 0; j < tr[i].evals.size(); ++j, ++k) {
+        if (evals) evals[k] = tr[i].evals[j];
+        if (rnorms) rnorms[k] = j < tr[i].rnorms.size() ? tr[i].rnorms[j] : 0.0;
+      }
+    }
+  });
+}
+
+int mpsxx_dmrg_set_truncation(mpsxx_dmrg d, double noise, int rule) {
+  return dguard([&] {
+    if (!d || !d->d) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: null handle");
+    if (rule != MPSXX_TRUNCATION_REFERENCE && rule != MPSXX_TRUNCATION_STABLE)
+      throw mpx::Error(MPSXX_ERR_ARG, "dmrg: unknown truncation rule");
+    if (!(noise >= 0.0)) throw mpx::Error(MPSXX_ERR_ARG, "dmrg: negative noise");
+    d->d->noise = noise;
+    d->d->trunc_rule = rule;
   });
 }
 

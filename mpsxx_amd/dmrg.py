@@ -60,6 +60,46 @@ def initial_mps(L, qt, M, seed=0):
     return mps
 
 
+def _mt19937_uniform_pm1(rs):
+    """One std::uniform_real_distribution<double>(-1, 1) draw of libstdc++ from the std::mt19937 behind the legacy
+    numpy RandomState ``rs`` (seeded with an integer, it initialises the Mersenne Twister exactly as std::mt19937's
+    seed constructor does): generate_canonical<double, 53> combines two 32-bit outputs, (x0 + x1 2^32) / 2^64 (a
+    value that rounds to 1 becomes the largest double below 1), and the distribution returns a + (b - a) u."""
+    x0, x1 = rs.randint(0, 2 ** 32, size=2, dtype=np.uint64)
+    u = (float(x0) + float(x1) * 4294967296.0) / 18446744073709551616.0
+    if u >= 1.0:
+        u = float(np.nextafter(1.0, 0.0))
+    return u * 2.0 - 1.0
+
+
+def reference_initial_mps(L, qt, M):
+    """The reference's initial MPS (src/make_random_mpss.cpp:97-171, identical in dmrg/make_random_mpss.cpp): a
+    default-seeded std::mt19937 (seed 5489) and uniform_real_distribution(-1, 1) give ONE value per site, in site
+    order, and every element of every allowed block of that site is set to it (QSTArray::resize's constant overload
+    wins, :153,171), then the site tensor is normalized.  Bond sectors as generate_quantum_states (capped at M), every
+    sector of dimension 1, total QN 0 on every site except the last, which carries (N, 2Sz)."""
+    rs = np.random.RandomState(5489)
+    qb = generate_quantum_states(L, qt, M)
+    mps = []
+    ql = [(0, 0)]
+    import itertools
+    for i in range(L):
+        qr = qb[i] if i < L - 1 else [(0, 0)]
+        total = (0, 0) if i < L - 1 else tuple(qt)
+        a = QST(total, [ql, SITE_Q, [(-p, -s) for (p, s) in qr]], [[1] * len(ql), [1] * 4, [1] * len(qr)])
+        v = _mt19937_uniform_pm1(rs)
+        for idx in itertools.product(range(len(ql)), range(4), range(len(qr))):
+            if a.allowed(idx):
+                a.blocks[idx] = np.full((1, 1, 1), v)
+        nrm = a.norm()
+        if nrm > 0:
+            for k in a.blocks:
+                a.blocks[k] /= nrm
+        mps.append(a)
+        ql = qr
+    return mps
+
+
 class SweepStats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("sigma_ms", ctypes.c_double), ("davidson_s", ctypes.c_double),
                 ("svd_s", ctypes.c_double), ("renorm_s", ctypes.c_double), ("diag_s", ctypes.c_double),
@@ -107,6 +147,41 @@ class DeviceDMRG:
         rc = lib().mpsxx_dmrg_set_form(self.h, ctypes.c_int(form))
         if rc != 0:
             raise RuntimeError("mpsxx_dmrg_set_form: " + lib().mpsxx_dmrg_last_error().decode())
+
+    def sweep_trace(self):
+        """Davidson solves of the last sweep: list of (site, forward, energy, [Ritz value per micro iteration],
+        [||r||^2 tested after each])."""
+        ns, ne = ctypes.c_int32(), ctypes.c_int32()
+        rc = lib().mpsxx_dmrg_sweep_trace(self.h, 0, ctypes.byref(ns), None, None, None, None, 0, ctypes.byref(ne),
+                                          None, None)
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_sweep_trace: " + lib().mpsxx_dmrg_last_error().decode())
+        n, m = ns.value, ne.value
+        site = np.zeros(max(n, 1), dtype=np.int32)
+        fwd = np.zeros(max(n, 1), dtype=np.int32)
+        en = np.zeros(max(n, 1))
+        cnt = np.zeros(max(n, 1), dtype=np.int32)
+        ev = np.zeros(max(m, 1))
+        rn = np.zeros(max(m, 1))
+        vp = ctypes.c_void_p
+        rc = lib().mpsxx_dmrg_sweep_trace(self.h, ctypes.c_int32(n), ctypes.byref(ns), site.ctypes.data_as(vp),
+                                          fwd.ctypes.data_as(vp), en.ctypes.data_as(vp), cnt.ctypes.data_as(vp),
+                                          ctypes.c_int32(m), ctypes.byref(ne), ev.ctypes.data_as(vp),
+                                          rn.ctypes.data_as(vp))
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_sweep_trace: " + lib().mpsxx_dmrg_last_error().decode())
+        out, k = [], 0
+        for i in range(n):
+            out.append((int(site[i]), bool(fwd[i]), float(en[i]), [float(x) for x in ev[k:k + cnt[i]]],
+                        [float(x) for x in rn[k:k + cnt[i]]]))
+            k += cnt[i]
+        return out
+
+    def set_truncation(self, noise=0.0, rule=1):
+        """Noise (DMRGInput::noise) and truncation rule (1 stable, 0 the reference's) of later sweeps."""
+        rc = lib().mpsxx_dmrg_set_truncation(self.h, ctypes.c_double(noise), ctypes.c_int(rule))
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_set_truncation: " + lib().mpsxx_dmrg_last_error().decode())
 
     def add_lower_root(self, root, energy):
         """Project the converged state of DeviceDMRG `root` (energy E_k) out of later sweeps (excited states)."""

@@ -224,3 +224,60 @@ def dmrg(mpos, init_mps, Ms, tol):
     _run(lib().orc_dmrg, ctypes.c_int(L), am, ai, ctypes.c_int(len(Ms)), Ms.ctypes.data_as(VP), ctypes.c_double(tol),
          e.ctypes.data_as(VP), ctypes.byref(ns))
     return e, ns.value
+
+
+def sigma_conj(mpo, lopr, ropr, wfn, sgv=None):
+    """compute_sigma_vector_conj (src/compute_sigma_vector.hpp:38-51), result in sgv's own quantum numbers."""
+    out = OTensor(sgv) if sgv is not None else OTensor()
+    ins = [_o(x) for x in (mpo, lopr, ropr, wfn)]
+    _run(lib().orc_sigma_conj, *[x.h for x in ins], out.h)
+    return out.to_qst()
+
+
+def make_noise(wfn, noise):
+    """make_noise (src/canonicalize.hpp:11-24) on a copy of wfn."""
+    W = OTensor(wfn)
+    _run(lib().orc_make_noise, W.h, ctypes.c_double(noise))
+    return W.to_qst()
+
+
+def canonicalize_twosite_ex(forward, wfn, M, noise=0.0, rule=0):
+    """canonicalize (src/canonicalize.hpp:27-46) with noise; rule 0 = the reference's truncation
+    (QSTLAPACK.h:209-213), 1 = the stable variant (tie band, 1e-16 floor, numerical rank)."""
+    W = _o(wfn)
+    L, R = OTensor(), OTensor()
+    d = ctypes.c_double()
+    _run(lib().orc_canonicalize_twosite_ex, ctypes.c_int(1 if forward else 0), W.h, L.h, R.h, ctypes.c_int(M),
+         ctypes.c_double(noise), ctypes.c_int(rule), ctypes.byref(d))
+    return L.to_qst(), R.to_qst(), d.value
+
+
+def truncation_cutoff(values, M, rule=0):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    c = ctypes.c_double()
+    _run(lib().orc_truncation_cutoff, ctypes.c_int(v.size), v.ctypes.data_as(VP), ctypes.c_int(M), ctypes.c_int(rule),
+         ctypes.byref(c))
+    return c.value
+
+
+def merged_operator(forward, mpo, opr):
+    """compute_merged_operator for one group (src/optimize.hpp:296-368)."""
+    out = OTensor()
+    M, O = _o(mpo), _o(opr)
+    _run(lib().orc_merged_operator, ctypes.c_int(1 if forward else 0), M.h, O.h, out.h)
+    return out.to_qst()
+
+
+def qst_merge(x, nrow, ncol):
+    """QSTmerge of the first nrow and the last ncol indices (include/btas/QSPARSE/QSTmerge.h)."""
+    out = OTensor()
+    X = _o(x)
+    _run(lib().orc_qst_merge, X.h, ctypes.c_int(nrow), ctypes.c_int(ncol), out.h)
+    return out.to_qst()
+
+
+def mt19937_uniform(n):
+    """First n draws of std::uniform_real_distribution<double>(-1, 1) on a default-seeded std::mt19937."""
+    out = np.zeros(n)
+    _run(lib().orc_mt19937_uniform, ctypes.c_int(n), out.ctypes.data_as(VP))
+    return out

@@ -22,10 +22,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <numeric>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -645,18 +648,30 @@ QST expand_cols(const QST& a, const MergeInfo& Mc, const std::vector<std::vector
   return b;
 }
 
-// Tie rule at the truncation cutoff: values within a relative 1e-12 of the M-th largest count as equal to it (so
-// exact degeneracies split by rounding are kept together on every backend).  The cutoff never drops below the
-// reference's default 1e-16 (QSTLAPACK.h:210): when fewer than M values exceed it, values at the rounding level are
-// not kept just because the M-th largest is one of them.
-double truncation_cutoff(std::vector<double> all, int M) {
+// Truncation cutoff over all singular values.
+//   rule 0, the reference's (QSTLAPACK.h:209-213): cutoff = 1e-16, or the M-th largest value when 0 < M < count;
+//     every value >= cutoff is kept.
+//   rule 1, the stable variant: values within a relative 1e-12 of the M-th largest count as equal to it (exact
+//     degeneracies split by rounding stay together on every backend), and the cutoff never drops below 1e-16 (values
+//     at the rounding level are not kept just because the M-th largest is one of them).
+double truncation_cutoff(std::vector<double> all, int M, int rule) {
   std::sort(all.rbegin(), all.rend());
   double cutoff = 1.0e-16;
-  if (M > 0 && M < (int)all.size()) cutoff = std::max(cutoff, all[M - 1] * (1.0 - 1e-12));
+  if (M > 0 && M < (int)all.size()) cutoff = rule == 0 ? all[M - 1] : std::max(cutoff, all[M - 1] * (1.0 - 1e-12));
   return cutoff;
 }
 
-double canonicalize_twosite(bool forward, const QST& wfnc, QST& lmps, QST& rmps, int M) {
+// Rule 1 also reports singular values below max(m, n) eps |A|_F of their block (the backward-error level of an
+// orthogonal decomposition) as exact zeros; rule 0 keeps them as computed.
+void numerical_rank(int m, int n, std::vector<double>& s) {
+  double fro = 0.0;
+  for (double v : s) fro += v * v;
+  const double thr = std::max(m, n) * 2.220446049250313e-16 * std::sqrt(fro);
+  for (double& v : s)
+    if (v < thr) v = 0.0;
+}
+
+double canonicalize_twosite(bool forward, const QST& wfnc, QST& lmps, QST& rmps, int M, int rule = 1) {
   MergeInfo Mr = merge_info({wfnc.qs[0], wfnc.qs[1]}, {wfnc.ds[0], wfnc.ds[1]});
   MergeInfo Mc = merge_info({wfnc.qs[2], wfnc.qs[3]}, {wfnc.ds[2], wfnc.ds[3]});
   QST a = merge_cols(merge_rows(Mr, 2, wfnc), Mc, 2);
@@ -670,11 +685,12 @@ double canonicalize_twosite(bool forward, const QST& wfnc, QST& lmps, QST& rmps,
     d.m = a.ds[0][ai[0]];
     d.n = a.ds[1][ai[1]];
     jacobi_svd(d.m, d.n, kv.second->data(), d.U, d.s, d.Vt);
+    if (rule == 1) numerical_rank(d.m, d.n, d.s);
     dec.push_back(std::move(d));
   }
   std::vector<double> all;
   for (auto& d : dec) all.insert(all.end(), d.s.begin(), d.s.end());
-  const double cutoff = truncation_cutoff(all, M);
+  const double cutoff = truncation_cutoff(all, M, rule);
   double dnorm = 0.0;
   // kept sectors in sector order (rows for LeftArrow, columns for RightArrow)
   const int nsec = forward ? a.nsec(0) : a.nsec(1);
@@ -742,6 +758,51 @@ void sigma_overlap(const QST& lsov, const QST& rsov, const QST& wfn, QST& sgv) {
   QST s1;  // compute_sigma_vector.hpp:32-33, 80-81, 106-107
   gemm(0, 0, 1.0, lsov, wfn, 1.0, s1, wfn.rank);
   gemm(0, 1, 1.0, s1, rsov, 1.0, sgv, wfn.rank);
+}
+
+// compute_sigma_vector_conj (compute_sigma_vector.hpp:38-51): < phi | = < psi | H^*.  Gemm(Trans, ConjTrans, lopr,
+// wfn0) contracts lopr's first index with the last index of the conjugated wfn0 (QSTBLAS.h:213-225,
+// btas_contract_qshape.h:60-108), then Contract(scr1, {0,2}, mpo0, {0,1}) and Contract(scr2, {1,3}, ropr, {0,1})
+// accumulate into sgv0.conjugate().  The reference's target is a temporary; here it is sgv0 itself: the result is
+// computed with the conjugated quantum numbers and they are flipped back.
+void sigma_conj(const QST& mpo, const QST& lopr, const QST& ropr, const QST& wfn, QST& sgv) {
+  QST s1, s2;
+  contract(1.0, lopr, {0}, conjugate(wfn), {2}, 1.0, s1);
+  contract(1.0, s1, {0, 2}, mpo, {0, 1}, 1.0, s2);
+  QST out = sgv.empty() ? QST() : conjugate(sgv);
+  contract(1.0, s2, {1, 3}, ropr, {0, 1}, 1.0, out);
+  sgv = conjugate(out);
+}
+
+// make_noise (canonicalize.hpp:11-24): nothing below 1e-8; otherwise wfnc += noise * wfpt, where wfpt holds every
+// allowed block of wfnc's net dense shape (STArray.h:386-401: a sector's dimension is taken from the stored blocks,
+// sectors without a stored block have dimension 0 and get no block) and is filled by QSTArray's generator
+// constructor with std::bind(uniform_real_distribution(-1, 1), mt19937()).  That constructor hands the generator to
+// every block by value (STArray.h:243-262, TArray.h:779-782), so each block is filled, in row-major order, by a fresh
+// copy of the default-seeded engine: block element e holds the e-th draw of the sequence.
+void make_noise(QST& w, double noise) {
+  if (noise < 1.0e-8) return;
+  std::vector<std::vector<int>> net(w.rank);
+  for (int i = 0; i < w.rank; ++i) net[i].assign(w.ds[i].size(), 0);
+  for (const auto& kv : w.blocks) {
+    std::vector<int> idx = w.index(kv.first);
+    for (int i = 0; i < w.rank; ++i) net[i][idx[i]] = w.ds[i][idx[i]];
+  }
+  QST p;
+  p.set_shape(w.rank, w.qsym, w.q, w.qs, net);
+  long nblk = 1;
+  for (int i = 0; i < w.rank; ++i) nblk *= p.nsec(i);
+  for (long t = 0; t < nblk; ++t) {
+    std::vector<int> idx = p.index(t);
+    if (!p.allowed(idx) || p.bsize(idx) == 0) continue;
+    auto b = std::make_shared<Block>(p.bsize(idx));
+    std::mt19937 rgen;  // default seed, a fresh copy per block
+    std::uniform_real_distribution<double> dist(-1.0, 1.0);
+    for (double& v : *b) v = dist(rgen);
+    p.blocks[t] = b;
+  }
+  p.ds = w.ds;  // same sector dims as w wherever a block exists
+  axpy(noise, p, w);
 }
 
 void diag_twosite(const QST& lmpo, const QST& rmpo, const QST& lopr, const QST& ropr, QST& diag) {
@@ -920,6 +981,7 @@ double davidson(Functor2& multH, const QST& diag, QST& wfnc, double tolerance) {
       QST errv = deep_copy(sigma[0]);
       axpy(-eval, trial[0], errv);
       const double rnorm = dotc(errv, errv);
+      if (std::getenv("ORACLE_DAVIDSON_TRACE")) fprintf(stderr, "oracle davidson m %d eval %.16f rnorm %.6e\n", m, eval, rnorm);
       if (rnorm < tolerance) { ++iconv; break; }
       if (m < max_ritz) {
         precondition(eval, diag, errv);
@@ -1180,6 +1242,31 @@ int orc_davidson_twosite(int ngroups, orc_tensor** lmpo, orc_tensor** rmpo, orc_
 
 int orc_canonicalize_twosite(int fwd, orc_tensor* wfnc, orc_tensor* lmps, orc_tensor* rmps, int M, double* dnorm) {
   ORC_GUARD(*dnorm = orc::canonicalize_twosite(fwd != 0, wfnc->a, lmps->a, rmps->a, M))
+}
+// canonicalize.hpp:27-46 with noise (make_noise on a copy of wfnc) and a truncation rule (0 reference, 1 stable)
+int orc_canonicalize_twosite_ex(int fwd, orc_tensor* wfnc, orc_tensor* lmps, orc_tensor* rmps, int M, double noise,
+                                int rule, double* dnorm) {
+  ORC_GUARD({
+    orc::QST w = orc::deep_copy(wfnc->a);
+    orc::make_noise(w, noise);
+    *dnorm = orc::canonicalize_twosite(fwd != 0, w, lmps->a, rmps->a, M, rule);
+  })
+}
+int orc_truncation_cutoff(int n, const double* values, int M, int rule, double* cutoff) {
+  ORC_GUARD(*cutoff = orc::truncation_cutoff(std::vector<double>(values, values + n), M, rule))
+}
+int orc_make_noise(orc_tensor* w, double noise) { ORC_GUARD(orc::make_noise(w->a, noise)) }
+// the first n values of uniform_real_distribution<double>(-1, 1) on one default-seeded mt19937, in draw order (the
+// per-site values of make_random_mpss.cpp:108-109,153,171)
+int orc_mt19937_uniform(int n, double* out) {
+  ORC_GUARD({
+    std::mt19937 rgen;
+    std::uniform_real_distribution<double> dist(-1.0, 1.0);
+    for (int i = 0; i < n; ++i) out[i] = dist(rgen);
+  })
+}
+int orc_sigma_conj(orc_tensor* mpo, orc_tensor* lopr, orc_tensor* ropr, orc_tensor* wfn, orc_tensor* sgv) {
+  ORC_GUARD(orc::sigma_conj(mpo->a, lopr->a, ropr->a, wfn->a, sgv->a))
 }
 
 // Full two-site DMRG on the CPU: nsweeps sweeps at bond dimension Ms[k]; energies[k] = sweep energy.

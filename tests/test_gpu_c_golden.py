@@ -27,3 +27,101 @@ def test_hubbard12_ground_energy_matches_reference_log():
             break
         e_prev = e
     assert abs(e - E_HUB12) < 2e-7, e
+
+
+# dmrg/tests/hub12_dmrg.out:75-149, 160-220: the first Davidson solves of the reference's run (sweep 0, forward,
+# sites 0, 1, 2), every micro iteration (the lowest Ritz value after each subspace solve), from the reference's own
+# initial MPS (make_random_mpss.cpp) and the HUBBARD12 integrals
+LOG_SITE0 = [3.0000000000000004, 1.8617705912932749, 1.0290576171582488, 0.5697198777086202, 0.5326562857212069,
+             0.5254568016562132, 0.5246767625435482, 0.5245658651376863, 0.5245546183114905, 0.5245522342425422,
+             0.5245522342425409, 0.5245511837705461, 0.5245505329037680, 0.5245502001885987, 0.5245500986518984,
+             0.5245500884652824]
+LOG_SITE1 = [0.5245500884652829, -0.1589334817562530, -0.4244113446384140, -0.6755912338803886, -0.8683327863842815,
+             -0.9610349934147215, -0.9835999284666719, -0.9913562388391549, -0.9935683192429254, -0.9939961772298870,
+             -0.9939961772298852, -0.9940596581682223, -0.9940819698502181, -0.9940891445708544, -0.9940912232980876,
+             -0.9940920175912688, -0.9940922765753228, -0.9940923456231269, -0.9940923570293904]
+# per-site Davidson energies of that forward half-sweep, sites 0..10 (hub12_dmrg.out "Energy =" lines of sweep 0), and
+# the sweep energy (the minimum over both half-sweeps, dmrg/make_sweep.cpp:115,218,253)
+LOG_SWEEP0_SITES = [0.5245500884652824, -0.9940923570293904, -2.2969457542437093, -3.7441463846935483,
+                    -5.0734897752010477, -6.4225641276743382, -7.7943700500667488, -9.1752889780649856,
+                    -10.3443856797587728, -11.6689393447820571, -11.6689393447820411]
+LOG_SWEEP0_ENERGY = -11.8406352042230001
+
+
+def test_hubbard12_first_davidson_trajectories_match_reference_log():
+    """With the reference's initial MPS (one default-seeded mt19937 draw per site, reference_initial_mps), the
+    HUBBARD12 model read from its FCIDUMP through the QC MPO, M = 200 and the reference truncation rule, the first
+    sweep's Davidson solves at sites 0 and 1 reproduce every logged micro-iteration value to 1e-9, and every site of
+    the first forward half-sweep its logged energy.  The initial state is the uniform superposition of the
+    configurations the bond sectors allow, so the trajectory also pins the fermionic sign convention of the MPO
+    against the reference's.
+
+    The log's convergence decisions correspond to a Davidson tolerance ||r||^2 < 1e-8, i.e. tole itself: at site 0 it
+    stops after ||r||^2 = 1.9e-9 and at site 1 after 9.3e-9 but continues after 3.3e-8 (values from this device
+    path and, identically, from the oracle), whereas dmrg/make_sweep.cpp:113 now passes 0.1 tole.  So the sweep runs
+    with tolerance 1e-7 (Davidson 1e-8)."""
+    import os
+
+    import numpy as np
+
+    from mpsxx_amd import qc
+    from mpsxx_amd.dmrg import reference_initial_mps
+    print("hub12 reference trajectory", flush=True)
+    here = os.path.dirname(os.path.abspath(__file__))
+    norb, nelec, ms2, h, v, ecore = qc.read_fcidump(os.path.join(here, "golden", "hubbard12.FCIDUMP"))
+    mpos = qc.QcMpo(h, v, ecore).devices()
+    M = 200
+    dm = DeviceDMRG(mpos, reference_initial_mps(norb, (nelec, ms2), M), M)
+    dm.set_truncation(0.0, 0)
+    e_sweep, _ = dm.sweep(M, 1e-7)
+    tr = dm.sweep_trace()
+    assert [t[0] for t in tr[:11]] == list(range(11)) and all(t[1] for t in tr[:11])
+    for t, ref in ((tr[0], LOG_SITE0), (tr[1], LOG_SITE1)):
+        print("got", t[3], "rnorm", t[4], flush=True)
+        assert len(t[3]) == len(ref), (t[3], ref)
+        assert np.max(np.abs(np.array(t[3]) - np.array(ref))) < 1e-9
+    site_e = np.array([t[2] for t in tr[:11]])
+    dev = np.abs(site_e - np.array(LOG_SWEEP0_SITES))
+    print("site energies", site_e.tolist(), "max deviation", dev.max(), "sweep", e_sweep, flush=True)
+    assert dev.max() < 1e-10
+    assert abs(e_sweep - LOG_SWEEP0_ENERGY) < 1e-10
+
+
+# hub12_dmrg.out:1164,1749,2319 (two-site sweeps of root 0), 2839,3359 (one-site sweeps), 3415-...: root 1's two-site
+# sweeps with root 0 projected out (dmrg/dmrg.cpp:36-114: two-site sweeps until |dE| < tole, then one-site sweeps)
+LOG_ROOT0_TWOSITE = [-11.8406352042230267, -11.8406372803113875, -11.8406372807034241]
+LOG_ROOT0_ONESITE = [-11.8406372775523412, -11.8406372775523305]
+LOG_ROOT1_TWOSITE = [-11.4515229623285073, -11.4516085030721673, -11.4516085043337537]
+
+
+def test_hubbard12_sweep_energies_match_reference_log():
+    """The reference's whole root-0 run (three two-site sweeps, then two one-site sweeps, dmrg/dmrg.cpp:44-114) and
+    root 1's two-site sweeps against root 0, from the reference's initial MPS (make_random_mpss gives every root the
+    same one), with the Davidson tolerance of the log (see the trajectory test): every logged sweep energy to 1e-11
+    (measured: 4.6e-13 at most)."""
+    import os
+
+    import numpy as np
+
+    from mpsxx_amd import qc
+    from mpsxx_amd.dmrg import SWEEP_ONESITE, reference_initial_mps
+    print("hub12 reference sweeps", flush=True)
+    here = os.path.dirname(os.path.abspath(__file__))
+    norb, nelec, ms2, h, v, ecore = qc.read_fcidump(os.path.join(here, "golden", "hubbard12.FCIDUMP"))
+    mpos = qc.QcMpo(h, v, ecore).devices()
+    M = 200
+    init = reference_initial_mps(norb, (nelec, ms2), M)
+    r0 = DeviceDMRG(mpos, init, M)
+    r0.set_truncation(0.0, 0)
+    got = [r0.sweep(M, 1e-7)[0] for _ in LOG_ROOT0_TWOSITE]
+    r0.set_form(SWEEP_ONESITE)
+    got1 = [r0.sweep(M, 1e-7)[0] for _ in LOG_ROOT0_ONESITE]
+    r1 = DeviceDMRG(mpos, init, M)
+    r1.set_truncation(0.0, 0)
+    r1.add_lower_root(r0, got1[-1])
+    got2 = [r1.sweep(M, 1e-7)[0] for _ in LOG_ROOT1_TWOSITE]
+    for g, ref in ((got, LOG_ROOT0_TWOSITE), (got1, LOG_ROOT0_ONESITE), (got2, LOG_ROOT1_TWOSITE)):
+        print("sweeps", g, "log", ref, "dev", np.abs(np.array(g) - np.array(ref)).tolist(), flush=True)
+    assert np.max(np.abs(np.array(got) - LOG_ROOT0_TWOSITE)) < 1e-11
+    assert np.max(np.abs(np.array(got1) - LOG_ROOT0_ONESITE)) < 1e-11
+    assert np.max(np.abs(np.array(got2) - LOG_ROOT1_TWOSITE)) < 1e-11

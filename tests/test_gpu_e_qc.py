@@ -222,3 +222,40 @@ def test_hubbard12_excited_states_match_reference_log():
         done.append(dm)
         es.append(e)
     assert np.max(np.abs(np.array(es) - np.array(ref))) < 2e-6, (es, ref)
+
+
+# Molpro FCI of the reference's C12H14 pi-system FCIDUMP (mpsxx/qcdmrg/benchmark/FCIDUMP; the same integrals as
+# dmrg/tests/FCIDUMP12 up to header formatting), total energy with E_core: mpsxx/qcdmrg/benchmark/molpro/
+# c12h14.fci.out:979.  Orbital order of the reference's run: mpsxx/qcdmrg/benchmark/reorder.dat (1-based).
+E_FCI_C12H14 = -457.052106408795
+C12H14_ORDER = [6, 9, 5, 11, 12, 4, 3, 7, 8, 2, 10, 1]
+
+
+def test_c12h14_merged_sweeps_reach_molpro_fci():
+    """The reference's own molecule: the C12H14 FCIDUMP read as the reference reads it (twoint(p,r,q,s) = (pq|rs),
+    src/parsing_integral.cpp:139-147) pins the two-electron convention of the FCIDUMP -> QC MPO path, which the
+    Hubbard FCIDUMP (only (ii|ii) and hopping terms) cannot.  Merged two-site FP64 sweeps in the reference's orbital
+    order, M 250 -> 500 -> 1000 -> 2000 (the centre bond of 12 orbitals holds at most 4^6 = 4096 states), reach the
+    Molpro FCI energy to 1e-6; FP32 sigma vectors with FP64 refinement land within 1e-8 of the FP64 sweeps."""
+    from mpsxx_amd.dmrg import SWEEP_MERGED
+    print("c12h14 fci", flush=True)
+    norb, nelec, ms2, h, v, ecore = qc.read_fcidump(os.path.join(HERE, "golden", "c12h14.FCIDUMP"))
+    assert (norb, nelec, ms2) == (12, 12, 0) and abs(ecore - -0.4422645349012726e3) < 1e-12
+    o = [k - 1 for k in C12H14_ORDER]
+    h = h[np.ix_(o, o)]
+    v = v[np.ix_(o, o, o, o)]
+    mpos = qc.QcMpo(h, v, ecore).devices()
+    sched = [250, 500, 1000, 2000, 2000, 2000, 2000]
+    res = {}
+    for prec in (mx.FP64, mx.FP32):
+        dm = DeviceDMRG(mpos, initial_mps(norb, (nelec, ms2), sched[0], 0), sched[0], form=SWEEP_MERGED)
+        e_prev = None
+        for k, M in enumerate(sched):
+            e, st = dm.sweep(M, 1e-9, prec)
+            print("prec", prec, "sweep", k, "M", M, e, st["seconds"], flush=True)
+            if k >= 4 and e_prev is not None and abs(e - e_prev) < 1e-10:
+                break
+            e_prev = e
+        res[prec] = e
+    assert abs(res[mx.FP64] - E_FCI_C12H14) < 1e-6, res
+    assert abs(res[mx.FP32] - res[mx.FP64]) < 1e-8, res

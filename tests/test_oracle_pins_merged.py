@@ -1,0 +1,466 @@
+"""Pins of the oracle functions the projected-Hamiltonian checks in test_oracle_pins.py do not reach (CPU).
+
+* QSTmerge against an independent numpy restatement of QSTmergeInfo (include/btas/QSPARSE/QSTmergeInfo.h:44-95:
+  packed index = row-major product of the merged indices, merged sectors numbered by first appearance, sub-blocks of
+  one merged sector stacked in ascending packed index);
+* the merged operators L~ = merge(L W), R~ = merge(W R) against a dense einsum through the same merge map;
+* merged sigma, merged diagonal and merged renormalization against the 4-index forms after merging (SURVEY's
+  "4-index sigma == merged sigma after merging"), which test_oracle_pins.py ties to exact diagonalization, and the
+  merged renormalization chain against <Psi|H|Psi>;
+* the one-site diagonal against the diagonal of the one-site P^T H P;
+* the conjugate sigma form against H_eff^T psi = H_eff psi (exact diagonalization);
+* make_noise against a restatement of std::mt19937 (checked against the C++ standard's known answer) and of
+  libstdc++'s uniform_real_distribution;
+* the truncation cutoff, both rules, on spectra whose answers are fixed by hand, and on a wavefunction with a
+  prescribed spectrum;
+* the Jacobi eigen-solver of the oracle's Davidson against numpy.linalg.eigh.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import ed
+import oracle
+from helpers import chain_envs, projectors
+from mpsxx_amd import mps
+from mpsxx_amd.mpo import hubbard_mpo
+from mpsxx_amd.qst import QST
+
+
+# -- independent restatement of QSTmergeInfo / QSTmerge ---------------------------------------------------------------
+def merge_map(qshapes, dshapes):
+    """Merged sectors of indices (qshapes, dshapes): returns (qm, dm, where) with where[packed] = (sector, offset)."""
+    packed_q, packed_d = [(0, 0)], [1]
+    for qs, ds in zip(qshapes, dshapes):
+        packed_q = [(a[0] + b[0], a[1] + b[1]) for a in packed_q for b in qs]
+        packed_d = [x * y for x in packed_d for y in ds]
+    order = {}
+    for q in packed_q:
+        if q not in order:
+            order[q] = len(order)
+    qm = [None] * len(order)
+    for q, k in order.items():
+        qm[k] = q
+    dm = [0] * len(order)
+    where = []
+    for q, d in zip(packed_q, packed_d):
+        k = order[q]
+        where.append((k, dm[k]))
+        dm[k] += d
+    return qm, dm, where
+
+
+def np_merge(x, nrow, ncol):
+    """QSTmerge of the first nrow and the last ncol indices of x (nrow or ncol = 1 keeps that side)."""
+    N = x.rank
+    mid = list(range(nrow, N - ncol))
+    rq, rd, rw = merge_map(x.qshape[:nrow], x.dshape[:nrow])
+    cq, cd, cw = merge_map(x.qshape[N - ncol:], x.dshape[N - ncol:])
+    y = QST(x.q, [rq] + [x.qshape[i] for i in mid] + [cq], [rd] + [x.dshape[i] for i in mid] + [cd], x.qsym)
+
+    def packed(idx, lo, hi):
+        p = 0
+        for i in range(lo, hi):
+            p = p * x.nsec(i) + idx[i]
+        return p
+
+    for idx, blk in x.blocks.items():
+        R, ro = rw[packed(idx, 0, nrow)]
+        C, co = cw[packed(idx, N - ncol, N)]
+        yi = (R,) + tuple(idx[i] for i in mid) + (C,)
+        if yi not in y.blocks:
+            y.blocks[yi] = np.zeros(y.block_shape(yi))
+        sh = blk.shape
+        nr = int(np.prod(sh[:nrow]))
+        nc = int(np.prod(sh[N - ncol:]))
+        b = blk.reshape((nr,) + tuple(sh[i] for i in mid) + (nc,))
+        y.blocks[yi][(slice(ro, ro + nr),) + (slice(None),) * len(mid) + (slice(co, co + nc),)] = b
+    return y
+
+
+def dense_perm(qshapes, dshapes):
+    """Dense position of each element of the merged index: perm[merged dense index] = packed dense index."""
+    qm, dm, where = merge_map(qshapes, dshapes)
+    moff = np.concatenate([[0], np.cumsum(dm)]).astype(int)
+    offs = [np.concatenate([[0], np.cumsum(d)]).astype(int) for d in dshapes]
+    perm = np.zeros(sum(dm), dtype=int)
+    sizes = [sum(d) for d in dshapes]
+    for p, combo in enumerate(itertools.product(*[range(len(d)) for d in dshapes])):
+        k, o = where[p]
+        ranges = [range(offs[i][c], offs[i][c] + dshapes[i][c]) for i, c in enumerate(combo)]
+        for e, el in enumerate(itertools.product(*ranges)):
+            perm[moff[k] + o + e] = np.ravel_multi_index(el, sizes)
+    return perm
+
+
+def assert_close(a, b, tol=1e-12):
+    scale = max(1.0, max((float(np.max(np.abs(v))) for v in b.blocks.values() if v.size), default=1.0))
+    assert a.q == b.q and a.qshape == b.qshape and a.dshape == b.dshape
+    assert a.max_abs_diff(b) <= tol * scale
+
+
+def _problem(L, i, seed, dmax=2, U=4.0):
+    rng = np.random.default_rng(seed)
+    mpos = hubbard_mpo(L, 1.0, U)
+    ch = mps.random_chain(L, i, (L, L % 2), rng, dmax=dmax)
+    lenv, renv = chain_envs(mpos, ch, i, L)
+    return mpos, ch, lenv, renv
+
+
+# -- QSTmerge ------------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("seed", [0, 1])
+def test_qst_merge_matches_restatement(seed):
+    mpos, ch, lenv, renv = _problem(6, 2, seed, dmax=3)
+    psi = ch["psi"]
+    for nr, nc in ((2, 2), (2, 1), (1, 2)):
+        got = oracle.qst_merge(psi, nr, nc)
+        ref = np_merge(psi, nr, nc)
+        assert got.block_set() == ref.block_set()
+        assert_close(got, ref, 0.0)
+    # a hand-checked case: two indices with sectors (0,0),(1,1) of dims 2, 1 merge to (0,0) d4, (1,1) d4, (2,2) d1
+    qm, dm, where = merge_map([[(0, 0), (1, 1)], [(0, 0), (1, 1)]], [[2, 1], [2, 1]])
+    assert qm == [(0, 0), (1, 1), (2, 2)] and dm == [4, 4, 1]
+    assert where == [(0, 0), (1, 0), (1, 2), (2, 0)]
+
+
+# -- merged operators ----------------------------------------------------------------------------------------------
+def test_merged_operators_match_dense_einsum():
+    L, i = 6, 2
+    mpos, ch, lenv, renv = _problem(L, i, 3, dmax=2)
+    W1, W2 = mpos[i], mpos[i + 1]
+    Lt = oracle.merged_operator(True, W1, lenv)
+    Rt = oracle.merged_operator(False, W2, renv)
+    # L~(Lb, w', Lk) = sum_w L(lb, w, lk) W(w, nb, nk, w') with Lb = (lb, nb), Lk = (lk, nk) merged
+    x = np.einsum("awb,wnmv->anvbm", lenv.dense(), W1.dense())
+    pb = dense_perm([lenv.qshape[0], W1.qshape[1]], [lenv.dshape[0], W1.dshape[1]])
+    pk = dense_perm([lenv.qshape[2], W1.qshape[2]], [lenv.dshape[2], W1.dshape[2]])
+    sa, sn, sv, sb, sm = x.shape
+    ref = x.reshape(sa * sn, sv, sb * sm)[pb][:, :, pk]
+    assert np.max(np.abs(Lt.dense() - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    # R~(Rb, w, Rk) = sum_w' W(w, nb, nk, w') R(rb, w', rk) with Rb = (nb, rb), Rk = (nk, rk) merged
+    y = np.einsum("wnmv,avb->nawmb", W2.dense(), renv.dense())
+    pb = dense_perm([W2.qshape[1], renv.qshape[0]], [W2.dshape[1], renv.dshape[0]])
+    pk = dense_perm([W2.qshape[2], renv.qshape[2]], [W2.dshape[2], renv.dshape[2]])
+    sn, sa, sw, sm, sb = y.shape
+    ref = y.reshape(sn * sa, sw, sm * sb)[pb][:, :, pk]
+    assert np.max(np.abs(Rt.dense() - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+# -- merged sigma, diagonal, renormalization -----------------------------------------------------------------------
+@pytest.mark.parametrize("L,i,seed", [(6, 2, 4), (6, 0, 5), (6, 4, 6)])
+def test_merged_sigma_and_diagonal_equal_merged_fourindex(L, i, seed):
+    mpos, ch, lenv, renv = _problem(L, i, seed, dmax=3)
+    psi = ch["psi"]
+    Lt = oracle.merged_operator(True, mpos[i], lenv)
+    Rt = oracle.merged_operator(False, mpos[i + 1], renv)
+    psim = np_merge(psi, 2, 2)
+    s4 = oracle.sigma_twosite(mpos[i], mpos[i + 1], lenv, renv, psi)
+    sm = oracle.sigma_merged(Lt, Rt, psim)
+    ref = np_merge(s4, 2, 2)
+    assert sm.block_set() == ref.block_set()
+    assert_close(sm, ref)
+    d4 = oracle.diag_twosite(mpos[i], mpos[i + 1], lenv, renv, psi.empty_like())
+    dm = oracle.diag_merged(Lt, Rt, psim.empty_like())
+    dref = np_merge(d4, 2, 2)
+    assert dm.block_set() == dref.block_set()
+    assert_close(dm, dref)
+
+
+def test_merged_diagonal_is_the_diagonal_of_the_merged_operator():
+    """diag_merged(L~, R~) equals the diagonal of the dense merged H_eff assembled column by column."""
+    mpos, ch, lenv, renv = _problem(5, 1, 7, dmax=2)
+    i = 1
+    psi = ch["psi"]
+    Lt = oracle.merged_operator(True, mpos[i], lenv)
+    Rt = oracle.merged_operator(False, mpos[i + 1], renv)
+    psim = np_merge(psi, 2, 2)
+    keys = psim.sorted_keys()
+    d = oracle.diag_merged(Lt, Rt, psim.empty_like())
+    for k in keys:
+        blk = psim.blocks[k]
+        for e in range(blk.size):
+            unit = psim.empty_like()
+            for kk in keys:
+                unit.blocks[kk] = np.zeros(psim.blocks[kk].shape)
+            unit.blocks[k].flat[e] = 1.0
+            col = oracle.sigma_merged(Lt, Rt, unit)
+            hee = float(col.blocks[k].flat[e]) if k in col.blocks else 0.0
+            dee = float(d.blocks[k].flat[e]) if k in d.blocks else 0.0  # an absent block: no contribution
+            assert abs(dee - hee) < 1e-11 * max(1.0, abs(hee))
+
+
+def test_merged_renormalization_chain_energy():
+    """L_{k+1} = U~^+ L~_k U~ (renormalize merged) equals the 4-index renormalization at every site, and the full
+    forward chain gives <Psi|H|Psi> by exact diagonalization; the backward chain (merged B sites) the same."""
+    L = 5
+    rng = np.random.default_rng(9)
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    qt = (L, 1)
+    lq = [mps.left_bond_qns(L, k, qt) for k in range(L + 1)]
+    lq[L] = [qt]
+    ld = [[1]] + [[int(rng.integers(1, 3)) for _ in lq[k]] for k in range(1, L)] + [[1]]
+    A = [mps.fill_allowed(mps.site_tensor(lq[k], ld[k], lq[k + 1], ld[k + 1]), rng) for k in range(L)]
+    env = mps.boundary_env()
+    for k in range(L):
+        Lt = oracle.merged_operator(True, mpos[k], env)
+        Am = np_merge(A[k], 2, 1)
+        nxt = oracle.renormalize_merged(True, Lt, Am, Am)
+        ref = oracle.renormalize(True, mpos[k], env, A[k], A[k])
+        assert_close(nxt, ref)
+        env = nxt
+    e_left = float(env.dense().reshape(-1)[0])
+    Psi = np.ones((1, 1))
+    for k in range(L):
+        x = np.einsum("xa,anb->xnb", Psi, A[k].dense())
+        Psi = x.reshape(-1, x.shape[2])
+    Psi = Psi[:, 0]
+    h, v = ed.hubbard_integrals(L, 1.0, 4.0)
+    H = ed.hamiltonian(h, v, L)
+    assert abs(e_left - Psi @ (H @ Psi)) < 1e-9 * max(1.0, abs(e_left))
+    # backward: right-canonical-shaped tensors B_k(l, n, r), merged over (n, r)
+    ch = mps.random_chain(L, 0, qt, rng, dmax=2)
+    renv = mps.boundary_env()
+    for k in range(L - 1, 1, -1):
+        Rt = oracle.merged_operator(False, mpos[k], renv)
+        Bm = np_merge(ch["B"][k], 1, 2)
+        nxt = oracle.renormalize_merged(False, Rt, Bm, Bm)
+        ref = oracle.renormalize(False, mpos[k], renv, ch["B"][k], ch["B"][k])
+        assert_close(nxt, ref)
+        renv = nxt
+
+
+# -- one-site diagonal ---------------------------------------------------------------------------------------------
+def _onesite_problem(seed):
+    L, i = 5, 2
+    rng = np.random.default_rng(seed)
+    mpos = hubbard_mpo(L, 1.0, 2.0)
+    qt = (L, 1)
+    ch = mps.random_chain(L, i, qt, rng, dmax=2)
+    rq_i1 = mps.right_bond_qns(L, i + 1, qt)
+    rd_i1 = [int(rng.integers(1, 3)) for _ in rq_i1]
+    B = dict(ch["B"])
+    B[i + 1] = mps.fill_allowed(mps.site_tensor(rq_i1, rd_i1, ch["rq"][i + 2], ch["rd"][i + 2]), rng)
+    wf = mps.fill_allowed(mps.site_tensor(ch["lq"][i], ch["ld"][i], rq_i1, rd_i1, total=qt), rng)
+    lenv = mps.boundary_env()
+    for k in range(i):
+        lenv = oracle.renormalize(True, mpos[k], lenv, ch["A"][k], ch["A"][k])
+    renv = mps.boundary_env()
+    for k in range(L - 1, i, -1):
+        renv = oracle.renormalize(False, mpos[k], renv, B[k], B[k])
+    PL = np.ones((1, 1))
+    for k in range(i):
+        x = np.einsum("xa,anb->xnb", PL, ch["A"][k].dense())
+        PL = x.reshape(-1, x.shape[2])
+    PR = np.ones((1, 1))
+    for k in range(L - 1, i, -1):
+        x = np.einsum("anb,by->any", B[k].dense(), PR)
+        PR = x.reshape(x.shape[0], -1)
+    h, v = ed.hubbard_integrals(L, 1.0, 2.0)
+    H = ed.hamiltonian(h, v, L)
+    return mpos[i], lenv, renv, wf, PL, PR, H
+
+
+def _allowed_mask(a):
+    mask = np.zeros(tuple(sum(d) for d in a.dshape), dtype=bool)
+    offs = [a.offsets(k) for k in range(a.rank)]
+    for idx in itertools.product(*[range(a.nsec(k)) for k in range(a.rank)]):
+        if a.allowed(idx):
+            mask[tuple(slice(offs[k][j], offs[k][j] + a.dshape[k][j]) for k, j in enumerate(idx))] = True
+    return mask
+
+
+def test_diagonal_onesite_equals_projected_diagonal():
+    W, lenv, renv, wf, PL, PR, H = _onesite_problem(10)
+    d = oracle.diag_onesite(W, lenv, renv, wf.empty_like())
+    Hd = H.toarray()
+    X, Y = PL.shape[0], PR.shape[1]
+    Hr = Hd.reshape(X, 4, Y, X, 4, Y)
+    ref = np.einsum("xnyXnY,xa,Xa,by,bY->anb", Hr, PL, PL, PR, PR, optimize=True)
+    mask = _allowed_mask(wf)
+    assert np.max(np.abs((d.dense() - ref)[mask])) < 1e-10 * max(1.0, np.max(np.abs(ref)))
+
+
+# -- conjugate sigma -----------------------------------------------------------------------------------------------
+def test_sigma_conj_is_the_transposed_effective_hamiltonian():
+    """compute_sigma_vector_conj contracts lopr's bra index with wfn0's last index, mpo0's (w, bra) with scr1's
+    (w, wfn0's first index) and ropr's (bra, w') with scr1's (wfn0's second index, w') (compute_sigma_vector.hpp:
+    45-50): with wfn0 = psi laid out as (n, r, l) the result is H_eff^T psi, which for the symmetric projected
+    Hamiltonian is P^T H P psi."""
+    W, lenv, renv, wf, PL, PR, H = _onesite_problem(11)
+    wt = QST(wf.q, [wf.qshape[1], wf.qshape[2], wf.qshape[0]], [wf.dshape[1], wf.dshape[2], wf.dshape[0]])
+    for (a, c, d), blk in wf.blocks.items():
+        wt.blocks[(c, d, a)] = blk.transpose(1, 2, 0).copy()
+    got = oracle.sigma_conj(W, lenv, renv, wt)
+    assert got.q == wf.q and got.qshape == wf.qshape
+    Psi = np.einsum("xa,anb,by->xny", PL, wf.dense(), PR).reshape(-1)
+    HPsi = (H @ Psi).reshape(PL.shape[0], 4, PR.shape[1])
+    ref = np.einsum("xa,xny,by->anb", PL, HPsi, PR)
+    assert np.max(np.abs(got.dense() - ref)) < 1e-10 * max(1.0, np.max(np.abs(ref)))
+    # accumulates into a non-empty output
+    twice = oracle.sigma_conj(W, lenv, renv, wt, got)
+    assert np.max(np.abs(twice.dense() - 2 * ref)) < 2e-10 * max(1.0, np.max(np.abs(ref)))
+
+
+# -- make_noise ----------------------------------------------------------------------------------------------------
+class MT19937:
+    """The 32-bit Mersenne Twister (Matsumoto & Nishimura 1998) with the C++ standard's parameters."""
+
+    def __init__(self, seed=5489):
+        self.mt = [0] * 624
+        self.mt[0] = seed & 0xFFFFFFFF
+        for i in range(1, 624):
+            self.mt[i] = (1812433253 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 30)) + i) & 0xFFFFFFFF
+        self.i = 624
+
+    def __call__(self):
+        if self.i >= 624:
+            for k in range(624):
+                y = (self.mt[k] & 0x80000000) | (self.mt[(k + 1) % 624] & 0x7FFFFFFF)
+                self.mt[k] = self.mt[(k + 397) % 624] ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+            self.i = 0
+        y = self.mt[self.i]
+        self.i += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        return y
+
+
+def uniform_pm1(gen):
+    """std::uniform_real_distribution<double>(-1, 1) of libstdc++: generate_canonical<double, 53> takes two 32-bit
+    draws, (x0 + x1 2^32) / 2^64 (values that round to 1 become the largest double below 1), then a + (b - a) u."""
+    import math
+    u = (gen() + gen() * 4294967296.0) / 18446744073709551616.0
+    if u >= 1.0:
+        u = math.nextafter(1.0, 0.0)
+    return u * 2.0 - 1.0
+
+
+def test_mt19937_known_answer():
+    g = MT19937()
+    for _ in range(9999):
+        g()
+    assert g() == 4123659995  # the C++ standard's check value for mt19937 ([rand.predef])
+
+
+def test_make_noise_matches_restatement():
+    mpos, ch, lenv, renv = _problem(6, 2, 12, dmax=3)
+    psi = ch["psi"].copy()
+    keys = psi.sorted_keys()
+    for k in keys[::4]:  # blocks that are allowed but absent get noise too
+        del psi.blocks[k]
+    assert oracle.make_noise(psi, 5e-9).max_abs_diff(psi) == 0.0  # below 1e-8: unchanged
+    noise = 1e-3
+    got = oracle.make_noise(psi, noise)
+    g = MT19937()
+    nmax = max(int(np.prod(psi.block_shape(k))) for k in keys)
+    u = np.array([uniform_pm1(g) for _ in range(nmax)])
+    present = [set(k[i] for k in psi.blocks) for i in range(psi.rank)]
+    expect = {}
+    for idx in itertools.product(*[range(psi.nsec(i)) for i in range(psi.rank)]):
+        if not psi.allowed(idx) or any(idx[i] not in present[i] for i in range(psi.rank)):
+            continue
+        n = int(np.prod(psi.block_shape(idx)))
+        base = psi.blocks.get(idx, np.zeros(psi.block_shape(idx)))
+        expect[idx] = base + noise * u[:n].reshape(psi.block_shape(idx))
+    assert got.block_set() == set(expect)
+    for k, v in expect.items():
+        assert np.max(np.abs(got.blocks[k] - v)) <= 1e-15
+
+
+# -- truncation ----------------------------------------------------------------------------------------------------
+def test_truncation_cutoff_rules_on_hand_checked_spectra():
+    cut = oracle.truncation_cutoff
+    # exact tie at the M-th value: both rules cut at it, so both tied values are kept
+    assert cut([3.0, 2.0, 2.0, 1.0], 2, 0) == 2.0
+    assert cut([3.0, 2.0, 2.0, 1.0], 2, 1) == 2.0 * (1.0 - 1e-12)
+    # a near tie (relative 1e-14) below the M-th value: the reference keeps M, the stable rule both
+    s = [1.0, 0.5, 0.5 * (1.0 - 1e-14), 0.1]
+    c0, c1 = cut(s, 2, 0), cut(s, 2, 1)
+    assert sum(v >= c0 for v in s) == 2 and sum(v >= c1 for v in s) == 3
+    # M at or above the count: the reference's default 1e-16
+    assert cut([1.0, 1e-17], 2, 0) == 1e-16 and cut([1.0, 1e-17], 5, 1) == 1e-16
+    # an M-th value at the rounding level: the reference cuts there, the stable rule at 1e-16
+    assert cut([1.0, 1e-17, 1e-18], 2, 0) == 1e-17 and cut([1.0, 1e-17, 1e-18], 2, 1) == 1e-16
+    # M <= 0: 1e-16
+    assert cut([1.0, 0.5], 0, 0) == 1e-16
+
+
+def prescribed_wavefunction(rng, spectra):
+    """A two-site wavefunction whose merged blocks (row sector, column sector), taken in ascending order, have the
+    given singular values (a block with fewer values than its size gets zeros), through the merge restatement."""
+    L, i = 6, 2
+    ch = mps.random_chain(L, i, (L, 0), rng, dmax=2)
+    psi = ch["psi"]
+    pm = np_merge(psi, 2, 2)
+    keys = [k for k in pm.sorted_keys() if min(pm.blocks[k].shape) >= max(len(s) for s in spectra)]
+    keys += [k for k in pm.sorted_keys() if k not in keys]
+    for k, sv in zip(keys, spectra):
+        m, n = pm.blocks[k].shape
+        kk = min(m, n)
+        s = np.zeros(kk)
+        s[:min(kk, len(sv))] = sv[:kk]
+        U, _ = np.linalg.qr(rng.standard_normal((m, kk)))
+        V, _ = np.linalg.qr(rng.standard_normal((n, kk)))
+        pm.blocks[k] = (U * s) @ V.T
+    for k in keys[len(spectra):]:
+        pm.blocks[k] = np.zeros(pm.blocks[k].shape)
+    # expand back: every psi block is a sub-rectangle of its merged block
+    rq, rd, rw = merge_map(psi.qshape[:2], psi.dshape[:2])
+    cq, cd, cw = merge_map(psi.qshape[2:], psi.dshape[2:])
+    out = psi.empty_like()
+    for idx in psi.blocks:
+        R, ro = rw[idx[0] * psi.nsec(1) + idx[1]]
+        C, co = cw[idx[2] * psi.nsec(3) + idx[3]]
+        sh = psi.block_shape(idx)
+        out.blocks[idx] = pm.blocks[(R, C)][ro:ro + sh[0] * sh[1], co:co + sh[2] * sh[3]].reshape(sh).copy()
+    return out, pm
+
+
+def _kept(q):
+    return sum(q.dshape[2]) if q is not None else 0
+
+
+def test_canonicalize_rules_on_a_prescribed_spectrum():
+    """Merged blocks with singular values {1.0, 0.5} and {0.5 (1 - 1e-13), 0.2}: at M = 2 the reference rule keeps
+    the two values >= 0.5, the stable rule also the near-tied third; both return isometries and the discarded weight
+    of the values they drop."""
+    rng = np.random.default_rng(13)
+    s_near = 0.5 * (1.0 - 1e-13)
+    psi, pm = prescribed_wavefunction(rng, [[1.0, 0.5], [s_near, 0.2]])
+    for rule, kept, dropped in ((0, 2, s_near ** 2 + 0.04), (1, 3, 0.04)):
+        U, V, d = oracle.canonicalize_twosite_ex(True, psi, 2, rule=rule)
+        assert _kept(U) == kept, (rule, U.dshape)
+        assert abs(d - dropped) < 1e-12
+
+
+def test_jacobi_eig_matches_numpy():
+    rng = np.random.default_rng(14)
+    for n in (1, 2, 5, 10):
+        a = rng.standard_normal((n, n))
+        a = a + a.T
+        w, v = oracle.jacobi_eig(a)
+        wr = np.linalg.eigvalsh(a)
+        assert np.max(np.abs(w - wr)) < 1e-12 * max(1.0, np.max(np.abs(wr)))
+        assert np.max(np.abs(a @ v - v * w)) < 1e-11 * max(1.0, np.max(np.abs(wr)))
+        assert np.max(np.abs(v.T @ v - np.eye(n))) < 1e-12
+
+
+def test_reference_initial_mps_draws_match_libstdcxx():
+    """reference_initial_mps (make_random_mpss.cpp:97-171) takes one draw per site from a default-seeded
+    std::mt19937 through uniform_real_distribution(-1, 1): its draw sequence equals the C++ library's bit for bit
+    and equals the restatement above; every allowed element of site i equals sign(draw_i) / sqrt(#elements)."""
+    from mpsxx_amd import dmrg
+    ref = oracle.mt19937_uniform(40)
+    rs = np.random.RandomState(5489)
+    got = np.array([dmrg._mt19937_uniform_pm1(rs) for _ in range(40)])
+    assert np.array_equal(got, ref)
+    g = MT19937()
+    assert np.array_equal(np.array([uniform_pm1(g) for _ in range(40)]), ref)
+    L = 12
+    mps0 = dmrg.reference_initial_mps(L, (L, 0), 200)
+    for i, a in enumerate(mps0):
+        vals = np.concatenate([b.ravel() for b in a.blocks.values()])
+        assert np.allclose(vals, np.sign(ref[i]) / np.sqrt(vals.size), rtol=0, atol=1e-15)
