@@ -197,6 +197,16 @@ enum { MPSXX_TRUNCATION_REFERENCE = 0, MPSXX_TRUNCATION_STABLE = 1 };
  * reference does, then the SVD with the given truncation rule. */
 int mpsxx_canonicalize_twosite_ex(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmps, mpsxx_tensor rmps,
                                   int max_states, double noise, int rule, double* discarded, void* stream);
+/* one-site canonicalize(forward, wfnc, lmps, rmps, M, noise) (src/canonicalize.hpp:49-85).  Forward: lmps = U,
+ * rmps (the right neighbour, in) <- s Vt rmps; backward: rmps = Vt, lmps (the left neighbour, in) <- lmps U s.  With
+ * noise >= 1e-8 the two-site wavefunction wfnc rmps (lmps wfnc) goes through the two-site form with noise, as the
+ * reference does (canonicalize.hpp:57-66). */
+int mpsxx_canonicalize_onesite_ex(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmps, mpsxx_tensor rmps,
+                                  int max_states, double noise, int rule, double* discarded, void* stream);
+/* merged canonicalize(forward, wfnc, lmat, rmat, M, noise) on a 2-index wavefunction (src/canonicalize.hpp:88-106):
+ * make_noise on wfnc in place, then lmat = U, rmat = s Vt (forward) or lmat = U s, rmat = Vt (backward). */
+int mpsxx_canonicalize_merged_ex(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmat, mpsxx_tensor rmat,
+                                 int max_states, double noise, int rule, double* discarded, void* stream);
 /* make_noise (src/canonicalize.hpp:11-24): nothing for noise < 1e-8, else x += noise * p, where p holds every allowed
  * block of x's net dense shape (sectors without a stored block excluded), each filled row-major with the draws of a
  * fresh default-seeded std::mt19937 through std::uniform_real_distribution<double>(-1, 1), as the reference's

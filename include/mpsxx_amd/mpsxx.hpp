@@ -306,4 +306,74 @@ void renormalize(bool forward, const Arr<double, 3, Q>& opr0, const Arr<double, 
   download(f, opr1);
 }
 
+//! Compute merged block operators (src/optimize.hpp:273-368): for every root k and operator group g,
+//! mHopr[k][g] = merge(xHopr[k][g] . xmpo[g]) (forward: rows (l, n), columns (l', n'); backward: rows (n, r), columns
+//! (n', r')), and mSopr[k] = merge(xSopr[k] (x) I_site).  The outputs are resized as the reference resizes them.
+template <template <typename, size_t, class> class Arr, class Q>
+void compute_merged_operator(bool forward, const std::vector<Arr<double, 4, Q>>& xmpo,
+                             const std::vector<std::vector<Arr<double, 3, Q>>>& xHopr,
+                             std::vector<std::vector<Arr<double, 3, Q>>>& mHopr,
+                             const std::vector<Arr<double, 2, Q>>& xSopr, std::vector<Arr<double, 2, Q>>& mSopr) {
+  using namespace mpsxx_amd;
+  if (xmpo.empty() || xHopr.empty()) throw std::runtime_error("compute_merged_operator: no groups or roots");
+  const size_t nroot = xHopr.size();
+  if (xSopr.size() != nroot) throw std::runtime_error("compute_merged_operator: one overlap per root");
+  mHopr.resize(nroot);
+  mSopr.resize(nroot);
+  for (size_t k = 0; k < nroot; ++k) {
+    if (xHopr[k].size() != xmpo.size()) throw std::runtime_error("compute_merged_operator: one operator per group");
+    mHopr[k].resize(xmpo.size());
+    for (size_t g = 0; g < xmpo.size(); ++g) {
+      Handle w = upload(xmpo[g]), o = upload(xHopr[k][g]), out;
+      check(mpsxx_compute_merged_operator(forward ? 1 : 0, w.get(), o.get(), out.get(), nullptr));
+      download(out, mHopr[k][g]);
+    }
+    Handle w = upload(xmpo[0]), so = upload(xSopr[k]), out;
+    check(mpsxx_compute_merged_overlap(forward ? 1 : 0, w.get(), so.get(), out.get(), nullptr));
+    download(out, mSopr[k]);
+  }
+}
+
+//! Canonicalize two-site MPS (src/canonicalize.hpp:27-46): make_noise on wfnc (in place), blocked SVD truncated by
+//! the reference's rule (QSTLAPACK.h:209-222), lmps = U and rmps = s Vt (forward) or lmps = U s and rmps = Vt.
+template <template <typename, size_t, class> class Arr, class Q>
+void canonicalize(bool forward, Arr<double, 4, Q>& wfnc, Arr<double, 3, Q>& lmps, Arr<double, 3, Q>& rmps,
+                  const int& M = 0, const double& noise = 0.0) {
+  using namespace mpsxx_amd;
+  Handle w = upload(wfnc), l, r;
+  check(mpsxx_canonicalize_twosite_ex(forward ? 1 : 0, w.get(), l.get(), r.get(), M, noise,
+                                      MPSXX_TRUNCATION_REFERENCE, nullptr, nullptr));
+  download(l, lmps);
+  download(r, rmps);
+  if (noise >= 1.0e-8) download(w, wfnc);
+}
+
+//! Canonicalize one-site MPS (src/canonicalize.hpp:49-85): forward lmps = U and rmps <- s Vt rmps; backward
+//! rmps = Vt and lmps <- lmps U s; with noise the two-site wavefunction with the neighbour goes through the two-site
+//! form (:57-66).
+template <template <typename, size_t, class> class Arr, class Q>
+void canonicalize(bool forward, Arr<double, 3, Q>& wfnc, Arr<double, 3, Q>& lmps, Arr<double, 3, Q>& rmps,
+                  const int& M = 0, const double& noise = 0.0) {
+  using namespace mpsxx_amd;
+  Handle w = upload(wfnc), l = upload(lmps), r = upload(rmps);
+  check(mpsxx_canonicalize_onesite_ex(forward ? 1 : 0, w.get(), l.get(), r.get(), M, noise,
+                                      MPSXX_TRUNCATION_REFERENCE, nullptr, nullptr));
+  download(l, lmps);
+  download(r, rmps);
+}
+
+//! Canonicalize merged MPS (src/canonicalize.hpp:88-106): make_noise on wfnc (in place), lmat = U and rmat = s Vt
+//! (forward) or lmat = U s and rmat = Vt.
+template <template <typename, size_t, class> class Arr, class Q>
+void canonicalize(bool forward, Arr<double, 2, Q>& wfnc, Arr<double, 2, Q>& lmat, Arr<double, 2, Q>& rmat,
+                  const int& M = 0, const double& noise = 0.0) {
+  using namespace mpsxx_amd;
+  Handle w = upload(wfnc), l, r;
+  check(mpsxx_canonicalize_merged_ex(forward ? 1 : 0, w.get(), l.get(), r.get(), M, noise,
+                                     MPSXX_TRUNCATION_REFERENCE, nullptr, nullptr));
+  download(l, lmat);
+  download(r, rmat);
+  if (noise >= 1.0e-8) download(w, wfnc);
+}
+
 }  // namespace mpsxx

@@ -376,6 +376,56 @@ int mpsxx_canonicalize_twosite_ex(int forward, mpsxx_tensor wfnc, mpsxx_tensor l
   });
 }
 
+int mpsxx_canonicalize_onesite_ex(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmps, mpsxx_tensor rmps,
+                                  int max_states, double noise, int rule, double* discarded, void* stream) {
+  return guard([&] {
+    mpx::TruncationScope ts(rule);
+    mpx::Tensor& w = T(wfnc);
+    mpx::Tensor& l = T(lmps);
+    mpx::Tensor& r = T(rmps);
+    hipStream_t st = S(stream);
+    double d = 0.0;
+    if (noise >= 1.0e-8) {
+      // canonicalize.hpp:57-66: the two-site wavefunction with the neighbour, then the two-site form with noise
+      mpx::Tensor two;
+      if (forward) mpx::op_contract(1.0, w, false, {2}, r, false, {0}, 0.0, two, st);
+      else mpx::op_contract(1.0, l, false, {2}, w, false, {0}, 0.0, two, st);
+      mpx::op_make_noise(two, noise, st);
+      mpx::Tensor U, V;
+      d = mpx::op_canonicalize_twosite(forward != 0, two, U, V, max_states, st);
+      l = std::move(U);
+      r = std::move(V);
+    } else {
+      // canonicalize.hpp:68-84: forward lmps = U, rmps <- (s Vt) rmps; backward rmps = Vt, lmps <- lmps (U s)
+      mpx::Tensor U, V, next;
+      d = mpx::op_canonicalize_onesite(forward != 0, w, U, V, max_states, st);
+      if (forward) {
+        mpx::op_contract(1.0, V, false, {1}, r, false, {0}, 0.0, next, st);
+        l = std::move(U);
+        r = std::move(next);
+      } else {
+        mpx::op_contract(1.0, l, false, {2}, U, false, {0}, 0.0, next, st);
+        l = std::move(next);
+        r = std::move(V);
+      }
+    }
+    mpx::hip_check(hipStreamSynchronize(st), "canonicalize");
+    if (discarded) *discarded = d;
+  });
+}
+
+int mpsxx_canonicalize_merged_ex(int forward, mpsxx_tensor wfnc, mpsxx_tensor lmat, mpsxx_tensor rmat,
+                                 int max_states, double noise, int rule, double* discarded, void* stream) {
+  return guard([&] {
+    mpx::TruncationScope ts(rule);
+    mpx::Tensor& w = T(wfnc);
+    if (w.empty || w.L.rank != 2) throw mpx::Error(MPSXX_ERR_ARG, "canonicalize: expects a 2-index wavefunction");
+    mpx::op_make_noise(w, noise, S(stream));
+    const double d = mpx::op_canonicalize_onesite(forward != 0, w, T(lmat), T(rmat), max_states, S(stream));
+    if (discarded) *discarded = d;
+  });
+}
+
 int mpsxx_make_noise(mpsxx_tensor x, double noise, void* stream) {
   return guard([&] { mpx::op_make_noise(T(x), noise, S(stream)); });
 }

@@ -194,16 +194,18 @@ double op_canonicalize_merged(bool forward, Tensor& wm, const Layout& shape4, Te
 }
 
 double op_canonicalize_onesite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor& rmps, int M, hipStream_t s) {
-  if (wfnc.empty || wfnc.L.rank != 3) throw Error(MPSXX_ERR_ARG, "canonicalize: expects a 3-index wavefunction");
+  if (wfnc.empty || (wfnc.L.rank != 3 && wfnc.L.rank != 2))
+    throw Error(MPSXX_ERR_ARG, "canonicalize: expects a 3-index (or 2-index merged) wavefunction");
   const Layout& X = wfnc.L;
-  // LeftArrow: rows (l, n), columns r; RightArrow: rows l, columns (n, r)  (canonicalize.hpp:71-84)
-  const int nr = forward ? 2 : 1;
+  // LeftArrow: rows (l, n), columns r; RightArrow: rows l, columns (n, r)  (canonicalize.hpp:71-84); a 2-index
+  // (merged) wavefunction is the matrix itself (canonicalize.hpp:88-106)
+  const int nr = X.rank == 2 ? 1 : forward ? 2 : 1;
   auto side = [&](int lo, int hi) {
     std::vector<std::vector<QN>> qs(X.qs.begin() + lo, X.qs.begin() + hi);
     std::vector<std::vector<int>> ds(X.ds.begin() + lo, X.ds.begin() + hi);
     return merge_info(qs, ds);
   };
-  MergeInfo MR = side(0, nr), MC = side(nr, 3);
+  MergeInfo MR = side(0, nr), MC = side(nr, X.rank);
   auto packed = [&](const int32_t* xi, int lo, int hi, const MergeInfo& Mi) {
     return hi - lo == 1 ? xi[lo] : xi[lo] * Mi.radix[1] + xi[lo + 1];
   };
@@ -212,7 +214,7 @@ double op_canonicalize_onesite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor&
     std::vector<std::pair<int, int>> seen;
     for (int64_t b = 0; b < X.nblocks(); ++b) {
       const int32_t* xi = X.index(b);
-      seen.push_back({MR.p2m[packed(xi, 0, nr, MR)], MC.p2m[packed(xi, nr, 3, MC)]});
+      seen.push_back({MR.p2m[packed(xi, 0, nr, MR)], MC.p2m[packed(xi, nr, X.rank, MC)]});
     }
     std::sort(seen.begin(), seen.end());
     seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
@@ -221,7 +223,7 @@ double op_canonicalize_onesite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor&
   std::vector<Rect> rects;
   for (int64_t b = 0; b < X.nblocks(); ++b) {
     const int32_t* xi = X.index(b);
-    const int pr = packed(xi, 0, nr, MR), pc = packed(xi, nr, 3, MC);
+    const int pr = packed(xi, 0, nr, MR), pc = packed(xi, nr, X.rank, MC);
     const Blk key{MR.p2m[pr], MC.p2m[pc], 0, 0};
     const int k = (int)(std::lower_bound(blks.begin(), blks.end(), key, [](const Blk& x, const Blk& y) {
                           return x.i != y.i ? x.i < y.i : x.j < y.j;

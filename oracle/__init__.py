@@ -281,3 +281,11 @@ def mt19937_uniform(n):
     out = np.zeros(n)
     _run(lib().orc_mt19937_uniform, ctypes.c_int(n), out.ctypes.data_as(VP))
     return out
+
+
+def merged_overlap(forward, mpo, sov):
+    """Merged overlap S~ = merge(S (x) I_site) of compute_merged_operator (src/optimize.hpp:326-328, 363-365)."""
+    out = OTensor()
+    M, S = _o(mpo), _o(sov)
+    _run(lib().orc_merged_overlap, ctypes.c_int(1 if forward else 0), M.h, S.h, out.h)
+    return out.to_qst()

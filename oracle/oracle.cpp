@@ -507,7 +507,27 @@ QST contract_symbols(const QST& a, const std::vector<int>& sa, const QST& b, con
   for (int s : sb)
     if (std::find(sa.begin(), sa.end(), s) == sa.end()) axb.push_back(s);
   QST c;
-  contract(1.0, a, ca, b, cb, 1.0, c);
+  if (ca.empty()) {
+    // no common symbol: the direct product c(a..., b...) = a(a...) b(b...) (QSTCONTRACT.h with K = 0)
+    std::vector<std::vector<QN>> qs(a.qs);
+    std::vector<std::vector<int>> ds(a.ds);
+    qs.insert(qs.end(), b.qs.begin(), b.qs.end());
+    ds.insert(ds.end(), b.ds.begin(), b.ds.end());
+    c.set_shape(a.rank + b.rank, a.qsym && b.qsym, a.q + b.q, qs, ds);
+    long nbt = 1;
+    for (int i = 0; i < b.rank; ++i) nbt *= b.nsec(i);
+    for (const auto& x : a.blocks)
+      for (const auto& y : b.blocks) {
+        const long t = x.first * nbt + y.first;
+        if (!c.allowed(c.index(t))) continue;
+        auto blk = std::make_shared<Block>(x.second->size() * y.second->size());
+        for (size_t i = 0; i < x.second->size(); ++i)
+          for (size_t j = 0; j < y.second->size(); ++j) (*blk)[i * y.second->size() + j] = (*x.second)[i] * (*y.second)[j];
+        c.blocks[t] = blk;
+      }
+  } else {
+    contract(1.0, a, ca, b, cb, 1.0, c);
+  }
   if (axb == sc) return c;
   std::vector<int> perm;
   for (int s : sc) perm.push_back((int)(std::find(axb.begin(), axb.end(), s) - axb.begin()));
@@ -528,6 +548,30 @@ QST merged_operator(bool fwd, const QST& mpo, const QST& opr) {
   QST r = contract_symbols(mpo, {0, 1, 2, 3}, opr, {4, 3, 5}, {1, 4, 0, 2, 5});
   QST s = merge_rows(Mb, 2, r);
   return merge_cols(s, Mk, 2);
+}
+
+// merged overlap of compute_merged_operator (src/optimize.hpp:296-299,326-328 forward; :334-336,363-365 backward):
+// I = QSTArray(zero, (mpo.qshape(1), mpo.qshape(2)), (mpo.dshape(1), mpo.dshape(2)), 1.0) on its allowed blocks;
+// forward Contract(S, (0,1), I, (2,3), -> (0,2,1,3)), backward Contract(I, (0,1), S, (2,3), -> (0,2,1,3)), then
+// QSTmerge of the first two and the last two indices with the operators' merge infos.
+QST merged_overlap(bool fwd, const QST& mpo, const QST& sov) {
+  QST I;
+  I.set_shape(2, true, QN(), {mpo.qs[1], mpo.qs[2]}, {mpo.ds[1], mpo.ds[2]});
+  for (long t = 0; t < I.nsec(0) * I.nsec(1); ++t) {
+    std::vector<int> idx = I.index(t);
+    if (!I.allowed(idx) || I.bsize(idx) == 0) continue;
+    I.blocks[t] = std::make_shared<Block>(I.bsize(idx), 1.0);
+  }
+  if (fwd) {
+    MergeInfo Mb = merge_info({sov.qs[0], mpo.qs[1]}, {sov.ds[0], mpo.ds[1]});
+    MergeInfo Mk = merge_info({sov.qs[1], mpo.qs[2]}, {sov.ds[1], mpo.ds[2]});
+    QST x = contract_symbols(sov, {0, 1}, I, {2, 3}, {0, 2, 1, 3});
+    return merge_cols(merge_rows(Mb, 2, x), Mk, 2);
+  }
+  MergeInfo Mb = merge_info({mpo.qs[1], sov.qs[0]}, {mpo.ds[1], sov.ds[0]});
+  MergeInfo Mk = merge_info({mpo.qs[2], sov.qs[1]}, {mpo.ds[2], sov.ds[1]});
+  QST x = contract_symbols(I, {0, 1}, sov, {2, 3}, {0, 2, 1, 3});
+  return merge_cols(merge_rows(Mb, 2, x), Mk, 2);
 }
 
 // ---- QN-blocked SVD with truncation (QSTLAPACK.h:122-288) and two-site canonicalize (canonicalize.hpp:27-46) ------
@@ -1209,6 +1253,9 @@ int orc_tie(orc_tensor* x, const int32_t* t, int32_t K, orc_tensor* y) {
   ORC_GUARD(y->a = orc::tie(x->a, std::vector<int>(t, t + K)))
 }
 int orc_copy_upcast(orc_tensor* x, orc_tensor* y) { ORC_GUARD(orc::copy_upcast(x->a, y->a)) }
+int orc_merged_overlap(int fwd, orc_tensor* mpo, orc_tensor* sov, orc_tensor* out) {
+  ORC_GUARD(out->a = orc::merged_overlap(fwd != 0, mpo->a, sov->a))
+}
 int orc_merged_operator(int fwd, orc_tensor* mpo, orc_tensor* opr, orc_tensor* out) {
   ORC_GUARD(out->a = orc::merged_operator(fwd != 0, mpo->a, opr->a))
 }

@@ -84,6 +84,8 @@ void write_array(std::ostream& out, const mock::QSTArray<double, N, Q>& a) {
   mpsxx::renormalize(true, s2, a3, b3, t2);                         // overlap
   mpsxx::renormalize(false, o3, s2, t2, a3);                        // merged operator
   mpsxx::renormalize(true, s2, t2, u2, v2);                         // merged overlap
+  mpsxx::canonicalize(true, c3, a3, b3, 10, 0.0);                   // one-site
+  mpsxx::canonicalize(false, u2, s2, t2, 10, 1e-3);                 // merged
 }
 
 int main(int argc, char** argv) {
@@ -94,9 +96,12 @@ int main(int argc, char** argv) {
   std::ifstream in(argv[1]);
   mock::QSTArray<double, 4, Q> lmpo, rmpo, wfn, sgv;
   mock::QSTArray<double, 4, Q> diag, diag2;
-  mock::QSTArray<double, 3, Q> lopr, ropr, ket, lnext;
+  mock::QSTArray<double, 3, Q> lopr, ropr, ket, lnext, lcan, rcan;
+  mock::QSTArray<double, 2, Q> lsov, rsov;
+  std::vector<std::vector<mock::QSTArray<double, 3, Q>>> mlH, mrH;
+  std::vector<mock::QSTArray<double, 2, Q>> mlS, mrS;
   if (!read_array(in, lmpo) || !read_array(in, rmpo) || !read_array(in, lopr) || !read_array(in, ropr) ||
-      !read_array(in, wfn) || !read_array(in, ket)) {
+      !read_array(in, wfn) || !read_array(in, ket) || !read_array(in, lsov) || !read_array(in, rsov)) {
     std::cerr << "bad input\n";
     return 3;
   }
@@ -112,6 +117,13 @@ int main(int argc, char** argv) {
     diag2.resize(wfn.q(), wfn.qshape());  // as QSTArray(q, qshape): no dense shapes
 
     mpsxx::compute_diagonal_elements(lg, rg, log, rog, diag2);
+    // merged operators of one root and two groups (optimize.hpp:273-368), both directions
+    std::vector<std::vector<mock::QSTArray<double, 3, Q>>> xl{{lopr, lopr}}, xr{{ropr, ropr}};
+    std::vector<mock::QSTArray<double, 2, Q>> sl{lsov}, sr{rsov};
+    mpsxx::compute_merged_operator(true, lg, xl, mlH, sl, mlS);
+    mpsxx::compute_merged_operator(false, rg, xr, mrH, sr, mrS);
+    // two-site canonicalize with noise, as optimize.hpp:210 calls it (wfnc carries the noise afterwards)
+    mpsxx::canonicalize(true, wfn, lcan, rcan, 12, 1e-3);
   } catch (const std::exception& e) {
     std::cerr << e.what() << "\n";
     return 4;
@@ -121,5 +133,13 @@ int main(int argc, char** argv) {
   write_array(out, diag);
   write_array(out, lnext);
   write_array(out, diag2);
+  write_array(out, mlH[0][0]);
+  write_array(out, mlH[0][1]);
+  write_array(out, mlS[0]);
+  write_array(out, mrH[0][1]);
+  write_array(out, mrS[0]);
+  write_array(out, lcan);
+  write_array(out, rcan);
+  write_array(out, wfn);
   return 0;
 }

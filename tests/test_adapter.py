@@ -3,8 +3,9 @@ container).  tests/adapter/mock_qstarray.hpp stands in for btas::QSTArray (the A
 tests/adapter/sigma_adapter.cpp calls the templates as src/optimize.hpp:60-75 does.
 
 * CPU: the header compiles against the container API and links against the library's exported symbols;
-* GPU: sigma (twice, accumulating), diagonal elements into a structure-only container, the two-group diagonal sum and
-  a forward renormalization through the header agree with the oracle (identical block sets, 1e-12 of the output
+* GPU: sigma (twice, accumulating), diagonal elements into a structure-only container, the two-group diagonal sum,
+  a forward renormalization, compute_merged_operator over two groups in both directions (operators and overlaps) and
+  canonicalize with noise through the header agree with the oracle (identical block sets, 1e-12 of the output
   scale)."""
 import os
 import subprocess
@@ -82,14 +83,29 @@ def test_header_calls_match_oracle(tmp_path):
     psi = ch["psi"]
     nq = mps.left_bond_qns(L, i + 1, (L, L % 2))
     ket = mps.fill_allowed(mps.site_tensor(ch["lq"][i], ch["ld"][i], nq, [2] * len(nq)), rng)
+    from helpers import chain_overlaps
+    lsov, rsov = chain_overlaps(ch, i, L)
     inp, out = tmp_path / "in.txt", tmp_path / "out.txt"
     with open(inp, "w") as f:
-        for a in (mpos[i], mpos[i + 1], lenv, renv, psi, ket):
+        for a in (mpos[i], mpos[i + 1], lenv, renv, psi, ket, lsov, rsov):
             _write(f, a)
     r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     tok = iter(open(out).read().split())
     sgv, diag, lnext, diag2 = _read(tok), _read(tok), _read(tok), _read(tok)
+    mlh0, mlh1, mls, mrh1, mrs, lcan, rcan, wnoisy = [_read(tok) for _ in range(8)]
+    # compute_merged_operator: both groups and the overlaps (optimize.hpp:273-368)
+    _same(mlh0, oracle.merged_operator(True, mpos[i], lenv))
+    _same(mlh1, oracle.merged_operator(True, mpos[i], lenv))
+    _same(mls, oracle.merged_overlap(True, mpos[i], lsov))
+    _same(mrh1, oracle.merged_operator(False, mpos[i + 1], renv))
+    _same(mrs, oracle.merged_overlap(False, mpos[i + 1], rsov))
+    # canonicalize with noise and the reference truncation rule; wfnc keeps the noise (canonicalize.hpp:35)
+    _same(wnoisy, oracle.make_noise(psi, 1e-3))
+    Ur, Vr, _ = oracle.canonicalize_twosite_ex(True, psi, 12, 1e-3, 0)
+    assert lcan.dshape == Ur.dshape and rcan.dshape == Vr.dshape and lcan.block_set() == Ur.block_set()
+    rec = oracle.contract(1.0, lcan, [2], rcan, [0])
+    _same(rec, oracle.contract(1.0, Ur, [2], Vr, [0]))
 
     ref = oracle.sigma_twosite(mpos[i], mpos[i + 1], lenv, renv, psi)
     ref2 = oracle.sigma_twosite(mpos[i], mpos[i + 1], lenv, renv, psi, sgv=ref.copy())

@@ -3,6 +3,8 @@ rule, against the oracle (needs an MI355X).
 
 Bars: block sets exact; sigma within 1e-12 of the output scale; noise within 1e-15; canonicalize: same kept sectors
 and dimensions, discarded weight within 1e-12, reconstructions within 1e-11 of the input scale."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -108,3 +110,69 @@ def test_sweeps_with_noise_reach_the_ground_state():
     h, v = ed.hubbard_integrals(L, 1.0, 4.0)
     e0 = ed.ground_energy(h, v, L, L, 0)
     assert abs(e - e0) <= 1e-9, (e, e0)
+
+
+def _svd_discarded(blocks, M):
+    """Discarded weight of the reference rule (QSTLAPACK.h:209-222) over the singular values of the given blocks."""
+    s = np.sort(np.concatenate([np.linalg.svd(b, compute_uv=False) for b in blocks]))[::-1]
+    cut = s[M - 1] if 0 < M < s.size else 1e-16
+    return float(np.sum(s[s < cut] ** 2))
+
+
+@pytest.mark.parametrize("forward", [True, False])
+def test_canonicalize_onesite_reference_semantics(forward):
+    """One-site canonicalize (canonicalize.hpp:49-85): forward lmps = U, rmps <- s Vt rmps; backward rmps = Vt,
+    lmps <- lmps U s.  With M = 0 the product lmps rmps equals wfnc rmps (lmps wfnc) and the new canonical factor is
+    an isometry; with M = 6 the discarded weight is the reference rule's over numpy's singular values of the merged
+    (l, n) x r (l x (n, r)) blocks."""
+    from test_oracle_pins_merged import np_merge
+    print("canonicalize onesite", forward, flush=True)
+    rng = np.random.default_rng(31)
+    L, i, qt = 8, 3, (8, 0)
+    ch = mps.random_chain(L, i, qt, rng, dmax=4)
+    rq = mps.right_bond_qns(L, i + 1, qt)
+    rd = [int(rng.integers(1, 5)) for _ in rq]
+    wf = mps.fill_allowed(mps.site_tensor(ch["lq"][i], ch["ld"][i], rq, rd, total=qt), rng)
+    nb = mps.fill_allowed(mps.site_tensor(rq, rd, ch["rq"][i + 2], ch["rd"][i + 2]), rng)  # right neighbour
+    pb = ch["A"][i - 1]  # left neighbour
+    D = mx.DTensor
+    for M in (0, 6):
+        w, l, r = D(wf), D(pb), D(nb)
+        d = ctypes.c_double()
+        rc = mx.lib().mpsxx_canonicalize_onesite_ex(int(forward), w.h, l.h, r.h, M, ctypes.c_double(0.0), 0,
+                                                    ctypes.byref(d), None)
+        assert rc == 0, mx.lib().mpsxx_last_error()
+        lq, rq_ = l.to_qst(), r.to_qst()
+        if M == 0:
+            rec = oracle.contract(1.0, lq, [2], rq_, [0])
+            ref = oracle.contract(1.0, wf, [2], nb, [0]) if forward else oracle.contract(1.0, pb, [2], wf, [0])
+            assert rec.max_abs_diff(ref) < 1e-11 * max(float(np.max(np.abs(b))) for b in ref.blocks.values())
+            iso = oracle.contract(1.0, lq, [0, 1], lq, [0, 1], conj_a=True) if forward else \
+                oracle.contract(1.0, rq_, [1, 2], rq_, [1, 2], conj_b=True)
+            for (a, b), blk in iso.blocks.items():
+                assert np.max(np.abs(blk - (np.eye(blk.shape[0]) if a == b else 0.0))) < 1e-12
+        mat = np_merge(wf, 2, 1) if forward else np_merge(wf, 1, 2)
+        assert abs(d.value - _svd_discarded(list(mat.blocks.values()), M)) < 1e-12 * wf.norm() ** 2
+
+
+@pytest.mark.parametrize("forward", [True, False])
+def test_canonicalize_merged_equals_twosite_on_the_expanded_wavefunction(forward):
+    """Merged canonicalize (canonicalize.hpp:88-106) of merge(psi) keeps the spectrum of the two-site form of psi:
+    same discarded weight, and lmat rmat equals merge(U V) of the oracle's two-site canonicalize (reference rule)."""
+    from test_oracle_pins_merged import np_merge
+    print("canonicalize merged", forward, flush=True)
+    rng = np.random.default_rng(32)
+    ch = mps.random_chain(8, 3, (8, 0), rng, dmax=5)
+    psi = ch["psi"]
+    M = 10
+    D = mx.DTensor
+    w, l, r = D(np_merge(psi, 2, 2)), D(), D()
+    d = ctypes.c_double()
+    rc = mx.lib().mpsxx_canonicalize_merged_ex(int(forward), w.h, l.h, r.h, M, ctypes.c_double(0.0), 0,
+                                               ctypes.byref(d), None)
+    assert rc == 0, mx.lib().mpsxx_last_error()
+    Ur, Vr, dwr = oracle.canonicalize_twosite_ex(forward, psi, M, 0.0, 0)
+    assert abs(d.value - dwr) < 1e-12 * max(1.0, psi.norm() ** 2)
+    rec = oracle.contract(1.0, l.to_qst(), [1], r.to_qst(), [0])
+    ref = np_merge(oracle.contract(1.0, Ur, [2], Vr, [0]), 2, 2)
+    assert rec.max_abs_diff(ref) < 1e-11 * max(float(np.max(np.abs(b))) for b in psi.blocks.values())

@@ -464,3 +464,26 @@ def test_reference_initial_mps_draws_match_libstdcxx():
     for i, a in enumerate(mps0):
         vals = np.concatenate([b.ravel() for b in a.blocks.values()])
         assert np.allclose(vals, np.sign(ref[i]) / np.sqrt(vals.size), rtol=0, atol=1e-15)
+
+
+def test_merged_overlap_matches_dense_kronecker():
+    """S~ = merge(S (x) I): forward rows (lb, nb), columns (lk, nk); backward rows (nb, rb), columns (nk, rk)."""
+    from helpers import chain_overlaps
+    L, i = 6, 2
+    rng = np.random.default_rng(15)
+    ch = mps.random_chain(L, i, (L, 0), rng, dmax=3)
+    lsov, rsov = chain_overlaps(ch, i, L)
+    W = hubbard_mpo(L, 1.0, 4.0)[i]
+    eye = np.eye(4)
+    got = oracle.merged_overlap(True, W, lsov)
+    x = np.einsum("ab,nm->anbm", lsov.dense(), eye)
+    pb = dense_perm([lsov.qshape[0], W.qshape[1]], [lsov.dshape[0], W.dshape[1]])
+    pk = dense_perm([lsov.qshape[1], W.qshape[2]], [lsov.dshape[1], W.dshape[2]])
+    ref = x.reshape(x.shape[0] * 4, x.shape[2] * 4)[pb][:, pk]
+    assert np.max(np.abs(got.dense() - ref)) < 1e-14
+    got = oracle.merged_overlap(False, W, rsov)
+    y = np.einsum("nm,ab->namb", eye, rsov.dense())
+    pb = dense_perm([W.qshape[1], rsov.qshape[0]], [W.dshape[1], rsov.dshape[0]])
+    pk = dense_perm([W.qshape[2], rsov.qshape[1]], [W.dshape[2], rsov.dshape[1]])
+    ref = y.reshape(4 * y.shape[1], 4 * y.shape[3])[pb][:, pk]
+    assert np.max(np.abs(got.dense() - ref)) < 1e-14
