@@ -276,6 +276,13 @@ const char* mpsxx_dmrg_last_error(void);
  * reference counterpart (the reference recomputes every contraction's structure on each call). */
 int mpsxx_plan_cache_stats(int64_t* hits, int64_t* builds, int64_t* evictions, double* build_ms, int64_t* plans,
                            int64_t* device_bytes, int64_t* releases, int64_t* restores);
+/* Cached plans are keyed by 64-bit structure hashes of their operands and taken on a hit only when the operands'
+ * block structures equal the ones the plan was built for; `collisions` = lookups whose key held a plan of other
+ * structures (the plan is then looked up and built under an extended key).  mpsxx_set_structure_hash_bits keeps only
+ * the low `bits` bits (1..64) of structure hashes computed from then on, so that tests can make distinct structures
+ * collide; 64 (the default) restores full hashes.  Diagnostic; no reference counterpart. */
+int mpsxx_plan_cache_collisions(int64_t* collisions);
+int mpsxx_set_structure_hash_bits(int32_t bits);
 /* Device allocations of the library since it was loaded: hipMalloc and hipFree calls and the host milliseconds they
  * took, buffers handed out again by the library's caching pool (MPSXX_DEVICE_CACHE_GB, default 31 % of device
  * memory) and the device synchronizations that took, the bytes in use and the bytes the pool holds.  Any pointer may
@@ -320,8 +327,9 @@ int mpsxx_last_plan_profile(int reps, int32_t max_entries, double* ms, int64_t* 
 /* Work items of the plan of the most recent hot-path call on this thread, summed over its steps: tiles[0] 64x64 tiles,
  * [1] 64x64 tiles of split reductions (partial-tile sums), [2] 32x32 tiles of the one-wave kernel, [3] 32x32 tiles of
  * the four-wave split kernel, [4] of those, tiles whose pair range was cut over several work-groups (partial-tile sums
- * of gemm_reduce32_kernel), [5] coefficient (AXPY) chunks.  Diagnostic; no reference counterpart. */
-int mpsxx_last_plan_tiles(int64_t tiles[6]);
+ * of gemm_reduce32_kernel), [5] coefficient (AXPY) chunks, [6] 128x128 tiles of the largest blocks, [7] 128x128
+ * tiles of split reductions.  Diagnostic; no reference counterpart. */
+int mpsxx_last_plan_tiles(int64_t tiles[8]);
 
 /* ---- quantum-chemistry MPO (src/gen_qc_operators.cpp:18-202, src/boundary_opinfo.cpp:47-140) ------------------- */
 /* H = sum_{pq,s} h[p*K+q] a+_ps a_qs + 1/2 sum_{pqrs,st} v[((p*K+q)*K+r)*K+s] a+_ps a+_rt a_st a_qs + ecore, with v the

@@ -52,8 +52,17 @@ def plan_cache_stats():
     _check(lib().mpsxx_plan_cache_stats(ctypes.byref(i[0]), ctypes.byref(i[1]), ctypes.byref(i[2]), ctypes.byref(ms),
                                         ctypes.byref(i[3]), ctypes.byref(i[4]), ctypes.byref(i[5]),
                                         ctypes.byref(i[6])))
+    c = ctypes.c_int64()
+    _check(lib().mpsxx_plan_cache_collisions(ctypes.byref(c)))
     return {"hits": i[0].value, "builds": i[1].value, "evictions": i[2].value, "build_ms": ms.value,
-            "plans": i[3].value, "device_bytes": i[4].value, "releases": i[5].value, "restores": i[6].value}
+            "plans": i[3].value, "device_bytes": i[4].value, "releases": i[5].value, "restores": i[6].value,
+            "collisions": c.value}
+
+
+def set_structure_hash_bits(bits):
+    """Keep only the low `bits` bits of structure hashes computed from now on (mpsxx_set_structure_hash_bits; 64
+    restores full hashes): distinct block structures then share cache keys, which the plan caches must detect."""
+    _check(lib().mpsxx_set_structure_hash_bits(ctypes.c_int32(bits)))
 
 
 def device_memory_stats():

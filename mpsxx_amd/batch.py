@@ -71,7 +71,8 @@ class SigmaBatch:
             pass
 
 
-KERNELS = ["gemm_grouped_kernel", "gemm_small_kernel", "gemm_split_kernel", "axpy_grouped_kernel", "gemm_x3_kernel"]
+KERNELS = ["gemm_grouped_kernel", "gemm_small_kernel", "gemm_split_kernel", "axpy_grouped_kernel", "gemm_x3_kernel",
+           "gemm128_kernel"]
 
 
 def _profile_entries(call, cap=256):
@@ -109,10 +110,11 @@ def dominant_kernel(entries):
 
 def last_plan_tiles():
     """Work items of the plan of the most recent hot-path call (mpsxx_last_plan_tiles): 64x64 tiles, 64x64 tiles of
-    split reductions, one-wave 32x32 tiles, split 32x32 tiles, split 32x32 tiles cut over work-groups, AXPY chunks."""
-    t = (ctypes.c_int64 * 6)()
+    split reductions, one-wave 32x32 tiles, split 32x32 tiles, split 32x32 tiles cut over work-groups, AXPY chunks,
+    128x128 tiles, 128x128 tiles of split reductions."""
+    t = (ctypes.c_int64 * 8)()
     _check(lib().mpsxx_last_plan_tiles(t))
-    keys = ("big", "big_red", "small", "split", "split_cut", "axpy_chunks")
+    keys = ("big", "big_red", "small", "split", "split_cut", "axpy_chunks", "big128", "big128_red")
     return dict(zip(keys, (int(x) for x in t)))
 
 

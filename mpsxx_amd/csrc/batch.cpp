@@ -35,7 +35,10 @@ BatchSigma::BatchSigma(const std::vector<Operand6>& inst, int prec, hipStream_t 
     if (!o.sgv) throw Error(MPSXX_ERR_ARG, "batch compute_sigma_vector: missing output");
     plans_.push_back(build_plan_sigma_twosite(o.lmpo->L, o.rmpo->L, o.lopr->L, o.ropr->L, o.wfn->L,
                                               o.sgv->empty ? nullptr : &o.sgv->L, prec));
-    for (Tensor* t : {o.lmpo, o.rmpo, o.lopr, o.ropr, o.wfn}) in_hash_.push_back(t->L.hash);
+    for (Tensor* t : {o.lmpo, o.rmpo, o.lopr, o.ropr, o.wfn}) {
+      in_hash_.push_back(t->L.hash);
+      in_layout_.push_back(t->L);
+    }
     flops_ += plans_.back().flops;
     bytes_ += plans_.back().bytes;
   }
@@ -124,7 +127,8 @@ BatchSigma::BatchSigma(const std::vector<Operand6>& inst, int prec, hipStream_t 
   // tile policies: remembered ones for this structure, else the defaults
   auto it = tuned_policies().find(structure_key());
   if (it != tuned_policies().end() && it->second.size() == launches_.size()) pol_ = it->second;
-  else pol_.assign(launches_.size(), default_policy());
+  else pol_.assign(launches_.size(), default_policy(prec_));
+  for (GemmPolicy& p : pol_) p = policy_for(p, prec_);
   for (int k = 0; k < (int)launches_.size(); ++k) build_step_lists(k);
   // FP32 input copies
   conv_chunks_.resize(5);
@@ -196,6 +200,7 @@ void BatchSigma::set_policies(const std::vector<GemmPolicy>& pol) {
   if (pol.size() != launches_.size()) throw Error(MPSXX_ERR_ARG, "batch: one policy per launch expected");
   hip_check(hipDeviceSynchronize(), "batch sync");  // launches in flight may still read the old tile lists
   pol_ = pol;
+  for (GemmPolicy& p : pol_) p = policy_for(p, prec_);
   for (int k = 0; k < (int)launches_.size(); ++k) build_step_lists(k);
   drop_graphs();
 }
@@ -205,13 +210,13 @@ void BatchSigma::check_structures(hipStream_t s) {
     const Operand6& o = inst_[i];
     const Tensor* in[5] = {o.lmpo, o.rmpo, o.lopr, o.ropr, o.wfn};
     for (int j = 0; j < 5; ++j)
-      if (in[j]->empty || in[j]->L.hash != in_hash_[5 * i + j])
+      if (in[j]->empty || !in[j]->L.same_structure(in_layout_[5 * i + j]))
         throw Error(MPSXX_ERR_SHAPE, "batch: an input's block structure changed since the batch was built");
     const Layout& out = plans_[i].out;
     Tensor& y = *o.sgv;
     if (y.empty) {
       y.set_layout(out, s);
-    } else if (y.L.hash != out.hash) {
+    } else if (!y.L.same_structure(out)) {
       // the output may only have lost blocks (e.g. a caller-side clear and refill); every block must still exist
       if (!same_shape(y.L, out)) throw Error(MPSXX_ERR_QN, "batch compute_sigma_vector: output quantum shape mismatch");
       for (int64_t b = 0; b < y.L.nblocks(); ++b)

@@ -43,6 +43,7 @@ class BatchSigma {
 
   int size() const { return n_; }
   int nsteps() const { return (int)launches_.size(); }
+  int precision() const { return prec_; }
   int ngroups() const { return (int)groups_.size(); }
   int64_t flops() const { return flops_; }
   int64_t bytes() const { return bytes_; }
@@ -66,14 +67,15 @@ class BatchSigma {
   void check_structures(hipStream_t s);
   void upload_tables(hipStream_t s, const std::vector<void*>* out_override);
   void enqueue(unsigned convert_mask, hipStream_t s, const std::vector<hipEvent_t>* events);
-  void launch_one(int k, hipStream_t s, unsigned parts = 7u);
+  void launch_one(int k, hipStream_t s, unsigned parts = kAllParts);
   void build_step_lists(int k);
   void drop_graphs();
   std::vector<uint64_t> structure_key() const;
 
   int n_ = 0, prec_ = 0;
   std::vector<Operand6> inst_;
-  std::vector<uint64_t> in_hash_;  // 5 input structure hashes per instance, fixed at creation
+  std::vector<uint64_t> in_hash_;  // 5 input structure hashes per instance, fixed at creation (tuned-policy key)
+  std::vector<Layout> in_layout_;  // the 5 input layouts per instance the plans were built for
   std::vector<Plan> plans_;
   std::vector<Group> groups_;
   std::vector<Launch> launches_;   // group-major, each group's steps in plan order

@@ -531,12 +531,12 @@ int mpsxx_last_plan_profile(int reps, int32_t max_entries, double* ms, int64_t* 
   });
 }
 
-int mpsxx_last_plan_tiles(int64_t tiles[6]) {
+int mpsxx_last_plan_tiles(int64_t tiles[8]) {
   return guard([&] {
     if (!tiles) throw mpx::Error(MPSXX_ERR_ARG, "plan tiles: null output");
     std::vector<const double*> in;
     const mpx::Plan& P = mpx::last_plan(in);
-    for (int k = 0; k < 6; ++k) tiles[k] = 0;
+    for (int k = 0; k < 8; ++k) tiles[k] = 0;
     for (const mpx::Step& st : P.steps) {
       if (st.kind == mpx::Step::GEMM) {
         tiles[0] += st.nwork;
@@ -544,6 +544,8 @@ int mpsxx_last_plan_tiles(int64_t tiles[6]) {
         tiles[2] += st.nwork_small;
         tiles[3] += st.nwork_split;
         tiles[4] += st.nwork_split_red;
+        tiles[6] += st.nwork2;
+        tiles[7] += st.nwork_red2;
       } else if (st.kind == mpx::Step::AXPY) {
         tiles[5] += st.nwork;
       }
@@ -581,6 +583,20 @@ int mpsxx_plan_cache_stats(int64_t* hits, int64_t* builds, int64_t* evictions, d
     if (device_bytes) *device_bytes = st.device_bytes;
     if (releases) *releases = st.releases;
     if (restores) *restores = st.restores;
+  });
+}
+
+int mpsxx_plan_cache_collisions(int64_t* collisions) {
+  return guard_host([&] {
+    if (!collisions) throw mpx::Error(MPSXX_ERR_ARG, "plan cache collisions: null output");
+    *collisions = mpx::plan_cache_stats().collisions;
+  });
+}
+
+int mpsxx_set_structure_hash_bits(int32_t bits) {
+  return guard_host([&] {
+    if (bits < 1 || bits > 64) throw mpx::Error(MPSXX_ERR_ARG, "structure hash bits: expected 1..64");
+    mpx::set_structure_hash_bits(bits);
   });
 }
 
@@ -770,7 +786,7 @@ int mpsxx_batch_set_policy(mpsxx_batch b, int32_t nsteps, const int32_t* policy)
       const int32_t* q = policy + 4 * k;
       if (q[0] < 0 || q[1] < 1 || q[2] < 0 || q[3] < 0)
         throw mpx::Error(MPSXX_ERR_ARG, "batch set_policy: negative limit, threshold, grid cap or split target");
-      p[k] = mpx::default_policy();  // the 32x32 split settings are not part of the policy layout
+      p[k] = mpx::default_policy(x.precision());  // the 32x32 split and 128x128 settings are not part of the layout
       p[k].small_limit = q[0];
       p[k].split_units = q[1];
       p[k].max_groups = q[2];

@@ -97,7 +97,7 @@ Layout davidson_layout(const std::vector<DavGroup>& groups, const Layout& psi) {
         if (next.find(P.out.tag[b]) < 0) next.add_block(P.out.index(b));
     }
     next.finalize();
-    if (next.hash == U.hash) return U;
+    if (next.same_structure(U)) return U;
     U = next;
   }
   return U;

@@ -96,7 +96,15 @@ void Layout::finalize() {
     }
   }
   for (int64_t t : tag) h = fnv(h, (uint64_t)t);
-  hash = h;
+  hash = h & structure_hash_mask();
+}
+
+namespace {
+std::atomic<uint64_t> g_hash_mask{~0ull};
+}
+uint64_t structure_hash_mask() { return g_hash_mask.load(std::memory_order_relaxed); }
+void set_structure_hash_bits(int bits) {
+  g_hash_mask.store(bits >= 64 ? ~0ull : ((1ull << std::max(bits, 0)) - 1ull), std::memory_order_relaxed);
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -311,7 +319,7 @@ void Tensor::relayout(const Layout& l, hipStream_t s) {
     set_layout(l, s);
     return;
   }
-  if (l.hash == L.hash) return;
+  if (l.same_structure(L)) return;
   DevBuf nb(std::max<int64_t>(l.nelem, 1) * sizeof(double));
   hip_check(hipMemsetAsync(nb.p, 0, std::max<int64_t>(l.nelem, 1) * sizeof(double), s), "memset");
   std::vector<CopySeg> segs;
@@ -818,7 +826,8 @@ Layout tie_layout(const Layout& X, const std::vector<int>& tie, std::vector<Copy
 // ---------------------------------------------------------------------------------------------------------------------
 // Steps
 
-Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumulate, double alpha) {
+Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumulate, double alpha,
+                        const GemmPolicy& pol) {
   Step st;
   st.a = a;
   st.b = b;
@@ -882,7 +891,7 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
   st.pairs = upload_vec(sym.pairs);
   st.host.gtasks = sym.tasks;
   st.host.gpairs = sym.pairs;
-  set_gemm_policy(st, default_policy());
+  set_gemm_policy(st, pol);
   return st;
 }
 
@@ -895,9 +904,28 @@ GemmPolicy default_policy() {
     if (const char* e = std::getenv("MPSXX_SPLIT_UNITS")) q.split_units = std::atoi(e);
     if (const char* e = std::getenv("MPSXX_SPLIT_SMALL")) q.split_small = std::atoi(e);
     if (const char* e = std::getenv("MPSXX_SPLIT_SEG")) q.split_seg = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("MPSXX_BIG_TILE_MIN")) q.big_tile_min = std::max(0, std::atoi(e));
     return q;
   }();
   return p;
+}
+
+GemmPolicy policy_for(GemmPolicy p, int precision) {
+  if (precision == MPSXX_BF16X3) p.big_tile_min = 0;
+  return p;
+}
+
+GemmPolicy default_policy(int precision) {
+  // measured on the norb=24 M=1024 merged sigma: 128x128 tiles take FP32 from 22.0 to 19.5 ms; the FP64 variant holds
+  // one work-group per CU (its 128 accumulator registers per lane) and with the 64x64 tiles of the remaining blocks
+  // took 35.8 ms against 33.0 ms for 64x64 tiles throughout
+  static const int f64_min = [] {
+    const char* e = std::getenv("MPSXX_BIG_TILE_MIN_F64");
+    return e ? std::max(0, std::atoi(e)) : 0;
+  }();
+  GemmPolicy p = default_policy();
+  if (precision == MPSXX_FP64) p.big_tile_min = f64_min;
+  return policy_for(p, precision);
 }
 
 void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>& pairs, const GemmPolicy& pol,
@@ -919,6 +947,18 @@ void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>&
     cost[ti] = ksum;
   }
   std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  // tile edge of a large block: 128 when both sides reach big_tile_min and every in-block offset of its pairs fits
+  // the 32-bit offsets of gemm128_kernel, else 64
+  auto edge_of = [&](const GemmTask& t) {
+    if (pol.big_tile_min <= 0 || t.m < pol.big_tile_min || t.n < pol.big_tile_min) return kGemmTile;
+    for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) {
+      const GemmPair& g = pairs[p];
+      const int64_t pad_m = t.m + kGemmTile2, pad_n = t.n + kGemmTile2, pad_k = g.k + 16;
+      if (pad_m * g.sa_m + pad_k * g.sa_k >= INT32_MAX || pad_k * g.sb_k + pad_n * g.sb_n >= INT32_MAX)
+        return kGemmTile;
+    }
+    return kGemmTile2;
+  };
   std::vector<size_t> bigtasks, splittasks;
   int64_t nbig = 0, ubig = 0, usplit = 0;
   for (size_t ti : order) {
@@ -935,7 +975,9 @@ void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>&
       continue;
     }
     bigtasks.push_back(ti);
-    const int64_t nt = (int64_t)((t.m + kGemmTile - 1) / kGemmTile) * ((t.n + kGemmTile - 1) / kGemmTile);
+    // tile count in 64x64 equivalents (a 128x128 tile carries four)
+    const int E = edge_of(t);
+    const int64_t nt = (int64_t)((t.m + E - 1) / E) * ((t.n + E - 1) / E) * (E / kGemmTile) * (E / kGemmTile);
     nbig += nt;
     ubig += nt * units;
   }
@@ -968,16 +1010,29 @@ void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>&
       nseg = (int)std::min<int64_t>({64, (int64_t)t.pair_count, std::max<int64_t>(1, (units + target - 1) / target)});
     const std::vector<int> cut = cuts(lo, hi, units, nseg);
     const int nparts = (int)cut.size() - 1;
-    for (int r = 0; r < t.m; r += kGemmTile)
-      for (int c = 0; c < t.n; c += kGemmTile) {
-        if (nparts == 1) {
-          out.big.push_back({(int32_t)ti, r, c, lo, hi, -1});
-          continue;
-        }
-        out.red.push_back({(int32_t)ti, r, c, out.nslots, nparts, 0});
-        for (int q = 0; q < nparts; ++q) out.big.push_back({(int32_t)ti, r, c, cut[q], cut[q + 1], out.nslots + q});
-        out.nslots += nparts;
+    const int E = edge_of(t);
+    std::vector<BigTile>& big = E == kGemmTile ? out.big : out.big2;
+    std::vector<RedTile>& red = E == kGemmTile ? out.red : out.red2;
+    int& nslots = E == kGemmTile ? out.nslots : out.nslots2;
+    if (nparts == 1) {
+      // the column tiles of a row band follow each other: they read the same A panels
+      for (int r = 0; r < t.m; r += E)
+        for (int c = 0; c < t.n; c += E) big.push_back({(int32_t)ti, r, c, lo, hi, -1});
+      continue;
+    }
+    // split reduction: slots of tile (r, c) are contiguous (summed in order by the reduce kernel); the tiles are
+    // listed row band by row band and pair range by pair range, so that consecutive tiles share their A panels
+    const int ncol = (t.n + E - 1) / E;
+    const int slot_base = nslots;
+    for (int r = 0; r < t.m; r += E)
+      for (int c = 0; c < t.n; c += E) {
+        red.push_back({(int32_t)ti, r, c, nslots, nparts, 0});
+        nslots += nparts;
       }
+    for (int r = 0, rb = 0; r < t.m; r += E, ++rb)
+      for (int q = 0; q < nparts; ++q)
+        for (int c = 0, cb = 0; c < t.n; c += E, ++cb)
+          big.push_back({(int32_t)ti, r, c, cut[q], cut[q + 1], slot_base + (rb * ncol + cb) * nparts + q});
   }
   // the split 32x32 tiles: a tile whose reduction is much longer than the list's share per work-group
   // (split_small work-groups in all) is cut into pair ranges of >= split_seg units, one work-group each
@@ -1021,6 +1076,11 @@ void set_gemm_policy(Step& st, const GemmPolicy& pol) {
   st.work_split = upload_vec(g.split);
   st.nwork_split_red = (int)g.split_red.size();
   st.work_split_red = upload_vec(g.split_red);
+  st.nwork2 = (int)g.big2.size();
+  st.work2 = upload_vec(g.big2);
+  st.nwork_red2 = (int)g.red2.size();
+  st.work_red2 = upload_vec(g.red2);
+  st.partial2.alloc((size_t)g.nslots2 * kGemmTile2 * kGemmTile2 * sizeof(double));
   st.split_partial.alloc((size_t)g.split_slots * kSmallTile * kSmallTile * sizeof(double));
 }
 
@@ -1121,6 +1181,10 @@ void launch_step_t(const Step& st, const Operands& ops, unsigned parts, bool x3,
       launch_gemm_split<T, TC>(st.work_split.as<BigTile>(), st.nwork_split, st.work_split_red.as<RedTile>(),
                                st.nwork_split_red, st.split_partial.p, st.tasks.as<GemmTask>(),
                                st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
+    if (parts & 8u)
+      launch_gemm128<T, TC>(st.work2.as<BigTile>(), st.nwork2, st.work_red2.as<RedTile>(), st.nwork_red2,
+                            st.partial2.p, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha,
+                            st.accumulate, s, st.max_groups);
   } else if (parts & 1u) {
     launch_axpy<T, TC>(st.work.as<Chunk>(), st.nwork, st.tasks.as<AxpyTask>(), st.pairs.as<AxpyPair>(), ops,
                        st.alpha, st.accumulate, s, st.max_groups);
@@ -1187,6 +1251,19 @@ std::vector<KernelPart> step_parts(const Step& st) {
     by += 8 * (int64_t)std::min(kSmallTile, t.m - rt.row0) * std::min(kSmallTile, t.n - rt.col0);
   }
   if (!g.split.empty()) out.push_back({KernelPart::GEMM_SPLIT, 4u, (int)g.split.size(), fl, by});
+  fl = by = 0;
+  for (const BigTile& tl : g.big2) {
+    tile_cost(tl.task, tl.row0, tl.col0, tl.pair_lo, tl.pair_hi, kGemmTile2, fl, by);
+    if (tl.slot < 0) {
+      const GemmTask& t = st.host.gtasks[tl.task];
+      by += 8 * (int64_t)std::min(kGemmTile2, t.m - tl.row0) * std::min(kGemmTile2, t.n - tl.col0);
+    }
+  }
+  for (const RedTile& rt : g.red2) {
+    const GemmTask& t = st.host.gtasks[rt.task];
+    by += 8 * (int64_t)std::min(kGemmTile2, t.m - rt.row0) * std::min(kGemmTile2, t.n - rt.col0);
+  }
+  if (!g.big2.empty()) out.push_back({KernelPart::GEMM128, 8u, (int)g.big2.size(), fl, by});
   return out;
 }
 
@@ -1197,6 +1274,7 @@ const char* part_kernel_name(int kind) {
     case KernelPart::GEMM_SMALL: return "gemm_small_kernel";
     case KernelPart::GEMM_SPLIT: return "gemm_split_kernel";
     case KernelPart::AXPY: return "axpy_grouped_kernel";
+    case KernelPart::GEMM128: return "gemm128_kernel";
     default: return "copy_kernel";
   }
 }
@@ -1274,7 +1352,7 @@ void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out,
     switch (st.kind) {
       case Step::GEMM:
       case Step::AXPY:
-        launch_step(st, step_operands(st, ptr), f32, cf32, s, 7u, P.precision == MPSXX_BF16X3);
+        launch_step(st, step_operands(st, ptr), f32, cf32, s, kAllParts, P.precision == MPSXX_BF16X3);
         break;
       case Step::COPY:
         if (!f32 && !cf32)

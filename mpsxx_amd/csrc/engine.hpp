@@ -138,9 +138,16 @@ struct GemmPolicy {
   int split_big = 2048;
   int split_small = 1024;
   int split_seg = 32;
+  int big_tile_min = 96;  // blocks with both sides >= this run as 128x128 tiles (gemm128_kernel); 0: never
 };
-// the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT, MPSXX_SPLIT_UNITS, MPSXX_SPLIT_SMALL, MPSXX_SPLIT_SEG
+// the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT, MPSXX_SPLIT_UNITS, MPSXX_SPLIT_SMALL, MPSXX_SPLIT_SEG,
+// MPSXX_BIG_TILE_MIN
 GemmPolicy default_policy();
+// the defaults for the block GEMMs of one precision (MPSXX_FP64 / MPSXX_FP32 / MPSXX_BF16X3): 128x128 tiles for FP32
+// only (MPSXX_BIG_TILE_MIN_F64 enables them for FP64; the split-BF16 kernel has 64x64 tiles only)
+GemmPolicy default_policy(int precision);
+// `p` with the tile settings `precision` has kernels for
+GemmPolicy policy_for(GemmPolicy p, int precision);
 
 // Tile lists of one GEMM step (heaviest blocks first in each list).
 struct GemmTiling {
@@ -151,6 +158,9 @@ struct GemmTiling {
   std::vector<BigTile> split;      // 32x32 tiles over a pair range; slot >= 0: partial 32x32 tile
   std::vector<RedTile> split_red;  // 32x32 tiles whose partial sums are added in slot order
   int split_slots = 0;
+  std::vector<BigTile> big2;       // 128x128 tiles of the largest blocks (split reductions as for the 64x64 tiles)
+  std::vector<RedTile> red2;
+  int nslots2 = 0;
 };
 void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>& pairs, const GemmPolicy& pol,
                 GemmTiling& out);
@@ -172,6 +182,10 @@ struct Step {
   DevBuf work_split;
   int nwork_split_red = 0;     // GEMM: split 32x32 tiles cut over work-groups (RedTile) and their partial buffer
   DevBuf work_split_red, split_partial;
+  int nwork2 = 0;              // GEMM: 128x128 tiles of the largest blocks (gemm128_kernel), their split reductions
+  DevBuf work2;
+  int nwork_red2 = 0;
+  DevBuf work_red2, partial2;
   int max_groups = 0;          // grid cap of the step's launches (0: one work-group per work item)
   GemmPolicy policy;           // GEMM: how the blocks were mapped to the tile lists
   int64_t flops = 0;
@@ -181,7 +195,7 @@ struct Step {
 // One kernel launch of a step, with its share of the step's algorithmic cost: flops 2*rows*cols*k summed over the
 // tiles of the launch (GEMM) or 2*len per pair (AXPY); bytes = 8 * (operand elements read once per tile + outputs).
 struct KernelPart {
-  enum Kind { GEMM64 = 0, GEMM_SMALL = 1, GEMM_SPLIT = 2, AXPY = 3, GEMM_X3 = 4 };
+  enum Kind { GEMM64 = 0, GEMM_SMALL = 1, GEMM_SPLIT = 2, AXPY = 3, GEMM_X3 = 4, GEMM128 = 5 };
   int kind;
   unsigned mask;  // `parts` bit of launch_step
   int items;
@@ -218,13 +232,15 @@ struct Plan {
     for (const DevBuf& b : slot_buf) n += b.bytes;
     for (const Step& st : steps)
       n += st.work.bytes + st.tasks.bytes + st.pairs.bytes + st.work_red.bytes + st.partial.bytes +
-           st.work_small.bytes + st.work_split.bytes + st.work_split_red.bytes + st.split_partial.bytes;
+           st.work_small.bytes + st.work_split.bytes + st.work_split_red.bytes + st.split_partial.bytes +
+           st.work2.bytes + st.work_red2.bytes + st.partial2.bytes;
     return n;
   }
 };
 
 // Builders of device steps from symbolic results.
-Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumulate, double alpha);
+Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumulate, double alpha,
+                        const GemmPolicy& pol = default_policy());
 Step make_copy_step(const std::vector<CopySeg>& segs, int x, int y, double alpha, double beta);
 // Two consecutive coefficient contractions X = Contract(Y, W1), Z = Contract(X, W2) (both with a scalar B side)
 // executed as one pass Z[r] += sum W1*W2*Y[..] without materialising X; returns false (no step) when either
@@ -233,9 +249,10 @@ bool make_fused_axpy_step(const ContractSym& first, const ContractSym& second, i
                           int accumulate, Step& out);
 
 // Launch the kernels of one GEMM or AXPY step (FP64, FP32, or FP32 operands into an FP64 output); `parts` selects
-// the tile lists (bit 0: 64x64 tiles or the AXPY chunks, bit 1: small tiles, bit 2: split tiles).
-// x3: FP32 operands of the 64x64 tiles multiplied as split-BF16 pairs (precision MPSXX_BF16X3).
-void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s, unsigned parts = 7u,
+// the tile lists (bit 0: 64x64 tiles or the AXPY chunks, bit 1: small tiles, bit 2: split tiles, bit 3: 128x128
+// tiles).  x3: FP32 operands of the 64x64 tiles multiplied as split-BF16 pairs (precision MPSXX_BF16X3).
+constexpr unsigned kAllParts = 15u;
+void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipStream_t s, unsigned parts = kAllParts,
                  bool x3 = false);
 
 // Execute a plan: inputs[i] are the FP64 device pointers of the bound inputs, out the FP64 output pointer.

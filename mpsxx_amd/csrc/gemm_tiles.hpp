@@ -38,6 +38,21 @@ template <> struct Mfma<float> {
 
 constexpr int BM = 64, BN = 64, BK = 16, LDP = 64 + 1;
 
+// Work-group -> work item of a launch with one work-group per item.  Work-groups are dispatched round-robin over the
+// 8 XCDs (work-group b runs on XCD b % 8), each XCD with an L2 of its own.  Items are taken in chunks of `chunk`
+// consecutive items, which share operand panels: the i-th work-group of XCD x takes item i % chunk of chunk
+// 8 (i / chunk) + x, so one chunk runs on one XCD and shares its L2, while the chunks, and with them the heavy items at
+// the head of the list, are dealt over the XCDs in turn.  A bijection: the tail that does not fill a round of 8 chunks
+// keeps the identity, as do persistent launches (fewer work-groups than items) and chunk <= 1.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_item(int b, int nwg, int nitems, int chunk) {
+  if (nwg != nitems || chunk <= 1) return b;
+  const int full = nitems / (kXcds * chunk) * (kXcds * chunk);
+  if (b >= full) return b;
+  const int x = b % kXcds, i = b / kXcds;
+  return ((i / chunk) * kXcds + x) * chunk + i % chunk;
+}
+
 // operand base of the task's instance: Tab = the launch passes per-instance pointer tables (batched launches, which
 // are distinct kernel symbols in profiles)
 template <typename T, bool Tab>
@@ -114,7 +129,8 @@ template <typename T, typename TC, bool Tab>
 __global__ __launch_bounds__(256) void gemm_grouped_kernel(const BigTile* __restrict__ tiles, int ntiles,
                                                            const GemmTask* __restrict__ tasks,
                                                            const GemmPair* __restrict__ pairs, Operands ops,
-                                                           double alpha, int accumulate, T* __restrict__ partial) {
+                                                           double alpha, int accumulate, T* __restrict__ partial,
+                                                           int xchunk) {
   typedef typename Mfma<T>::acc_t acc_t;
   __shared__ T As[2][BK][LDP];
   __shared__ T Bs[2][BK][LDP];
@@ -125,10 +141,13 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const BigTile* __rest
   const int lr = lane & 15, lk = lane >> 4;
 
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    const BigTile tl = tiles[w];
+    const BigTile tl = tiles[xcd_item(w, gridDim.x, ntiles, xchunk)];
     const GemmTask tk = tasks[tl.task];
     const T* __restrict__ A = opnd<const T, Tab>(ops.a, tk.inst);
     const T* __restrict__ B = opnd<const T, Tab>(ops.b, tk.inst);
+    // 16x16 sub-tiles of this wave that lie wholly in the padding of an edge tile skip their MFMAs (uniform per wave)
+    const bool live_r0 = tl.row0 + wr * 32 < tk.m, live_r1 = tl.row0 + wr * 32 + 16 < tk.m;
+    const bool live_c0 = tl.col0 + wc * 32 < tk.n, live_c1 = tl.col0 + wc * 32 + 16 < tk.n;
 
     acc_t acc[2][2];
 #pragma unroll
@@ -169,10 +188,10 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const BigTile* __rest
         const T a1 = As[buf][kk + lk][wr * 32 + 16 + lr];
         const T b0 = Bs[buf][kk + lk][wc * 32 + lr];
         const T b1 = Bs[buf][kk + lk][wc * 32 + 16 + lr];
-        acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
-        acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
-        acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
-        acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+        if (live_r0 && live_c0) acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+        if (live_r0 && live_c1) acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+        if (live_r1 && live_c0) acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+        if (live_r1 && live_c1) acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
       }
       if (more) {
         store_a(As[buf ^ 1], ra, npr.sa_k == 1, tid);
@@ -264,12 +283,13 @@ void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred,
                  const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
   T* part = static_cast<T*>(partial);
+  const int xc = xcd_chunk();
   if (ops.table)
     hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate, part);
+                       ntiles, tasks, pairs, ops, alpha, accumulate, part, xc);
   else
     hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate, part);
+                       ntiles, tasks, pairs, ops, alpha, accumulate, part, xc);
   if (nred <= 0) return;
   if (ops.table)
     hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, true>), dim3(groups(nred * kRedStrips, max_groups)), dim3(256), 0,

@@ -2,8 +2,11 @@
 // instantiation in kernels_f64.hip).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "kernels.hpp"
 #include "gemm_tiles.hpp"
+#include "gemm128.hpp"
 
 namespace mpx {
 
@@ -97,7 +100,7 @@ template <typename TC, bool Tab>
 __global__ __launch_bounds__(256) void gemm_x3_kernel(const BigTile* __restrict__ tiles, int ntiles,
                                                       const GemmTask* __restrict__ tasks,
                                                       const GemmPair* __restrict__ pairs, Operands ops, double alpha,
-                                                      int accumulate, float* __restrict__ partial) {
+                                                      int accumulate, float* __restrict__ partial, int xchunk) {
   __shared__ __attribute__((aligned(16))) __bf16 Ah[2][BM][XLD];
   __shared__ __attribute__((aligned(16))) __bf16 Al[2][BM][XLD];
   __shared__ __attribute__((aligned(16))) __bf16 Bh[2][BN][XLD];
@@ -109,10 +112,12 @@ __global__ __launch_bounds__(256) void gemm_x3_kernel(const BigTile* __restrict_
   const int lr = lane & 15, lk = lane >> 4;
 
   for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    const BigTile tl = tiles[w];
+    const BigTile tl = tiles[xcd_item(w, gridDim.x, ntiles, xchunk)];
     const GemmTask tk = tasks[tl.task];
     const float* __restrict__ A = opnd<const float, Tab>(ops.a, tk.inst);
     const float* __restrict__ B = opnd<const float, Tab>(ops.b, tk.inst);
+    const bool live_r[2] = {tl.row0 + wr * 32 < tk.m, tl.row0 + wr * 32 + 16 < tk.m};
+    const bool live_c[2] = {tl.col0 + wc * 32 < tk.n, tl.col0 + wc * 32 + 16 < tk.n};
     flt4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -156,6 +161,7 @@ __global__ __launch_bounds__(256) void gemm_x3_kernel(const BigTile* __restrict_
       for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni) {
+          if (!(live_r[mi] && live_c[ni])) continue;  // wholly in the padding of an edge tile
           acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
           acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
           acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
@@ -802,18 +808,27 @@ static inline int grid_for(int64_t n) {
 }
 
 
+int xcd_chunk() {
+  static const int c = [] {
+    const char* e = std::getenv("MPSXX_XCD_CHUNK");
+    return e ? std::atoi(e) : 16;
+  }();
+  return c;
+}
+
 template <typename TC>
 void launch_gemm_x3(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
                     const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
                     hipStream_t s, int max_groups) {
   if (ntiles <= 0) return;
   float* part = static_cast<float*>(partial);
+  const int xc = xcd_chunk();
   if (ops.table)
     hipLaunchKernelGGL((gemm_x3_kernel<TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles, ntiles,
-                       tasks, pairs, ops, alpha, accumulate, part);
+                       tasks, pairs, ops, alpha, accumulate, part, xc);
   else
     hipLaunchKernelGGL((gemm_x3_kernel<TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles, ntiles,
-                       tasks, pairs, ops, alpha, accumulate, part);
+                       tasks, pairs, ops, alpha, accumulate, part, xc);
   if (nred <= 0) return;
   if (ops.table)
     hipLaunchKernelGGL((gemm_reduce_kernel<float, TC, true>), dim3(groups(nred * kRedStrips, max_groups)), dim3(256),
@@ -1637,6 +1652,10 @@ void launch_gemm_split(const BigTile* tiles, int ntiles, const RedTile* red, int
 MPX_INST_BIG(float, float)
 MPX_INST_BIG(float, double)
 #undef MPX_INST_BIG
+template void launch_gemm128<float, float>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*,
+                                           const GemmPair*, Operands, double, int, hipStream_t, int);
+template void launch_gemm128<float, double>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*,
+                                            const GemmPair*, Operands, double, int, hipStream_t, int);
 #define MPX_INST_GEMM(FN, T, TC) \
   template void FN<T, TC>(const GemmTile*, int, const GemmTask*, const GemmPair*, Operands, double, int, hipStream_t, \
                            int);

@@ -89,6 +89,9 @@ struct CopySeg {
 };
 
 constexpr int kGemmTile = 64;     // C tile edge of the grouped GEMM
+constexpr int kGemmTile2 = 128;   // C tile edge of the large-block grouped GEMM (gemm128.hpp)
+// Tiles per XCD chunk of the 64x64 launches (MPSXX_XCD_CHUNK, default 16; <= 1: plain round-robin), see xcd_item.
+int xcd_chunk();
 constexpr int kAxpyChunk = 1024;  // elements per work-group in the AXPY / copy kernels
 
 // Launchers.  T = storage/arithmetic type of A and B (double or float), TC = type of C.  max_groups > 0 caps the grid
@@ -97,6 +100,10 @@ constexpr int kAxpyChunk = 1024;  // elements per work-group in the AXPY / copy 
 // (type T, 64*64 per slot) that a second launch sums in slot order.
 template <typename T, typename TC>
 void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
+                 const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups = 0);
+// 128x128 tiles of blocks with both sides >= kBig2Min (gemm128.hpp); partial slots of 128x128
+template <typename T, typename TC>
+void launch_gemm128(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
                  const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups = 0);
 
 // Same tiles with FP32 operands multiplied as split-BF16 pairs (three bf16 MFMAs per product, FP32 accumulation).

@@ -87,6 +87,12 @@ struct Layout {
     return true;
   }
   bool same_dshape(const Layout& o) const { return ds == o.ds; }
+  // Equal block structure (rank, symmetry, total and sector quantum numbers, sector dimensions, stored blocks), so
+  // equal offsets: the structure hash is compared first, the fields decide.  Cost O(sectors + blocks).
+  bool same_structure(const Layout& o) const {
+    if (hash != o.hash || rank != o.rank || qsym != o.qsym || !(q == o.q) || nelem != o.nelem) return false;
+    return ds == o.ds && qs == o.qs && tag == o.tag;
+  }
 
   // Sort blocks by tag, drop duplicates, assign canonical offsets and hash the structure.
   void finalize();
@@ -95,6 +101,11 @@ struct Layout {
   // Insert a block (before finalize); duplicates are dropped by finalize.
   void add_block(const int32_t* idx) { bidx.insert(bidx.end(), idx, idx + rank); tag.push_back(compute_tag(idx)); }
 };
+
+// Bits of the structure hash kept (all 64 by default).  Fewer bits make distinct structures share hashes, which tests
+// use to exercise the structural checks of the plan caches; hashes of layouts finalized earlier are unchanged.
+uint64_t structure_hash_mask();
+void set_structure_hash_bits(int bits);
 
 // Structure hash combine (FNV-1a 64).
 inline uint64_t fnv(uint64_t h, uint64_t v) {

@@ -324,6 +324,7 @@ namespace {
 // environment layouts, which repeat from one sweep to the next once the bond dimensions settle, so the plans are kept
 // in an LRU cache of bounded device size (1% of device memory, at most kMergedKeep plans).
 struct MergedOpDevice {
+  Layout mpo, opr;  // the operand layouts the plan was built for (a hit needs these structures, not only the hashes)
   Layout Y;
   int nw = 0, nchunks = 0;
   DevBuf tasks, chunks, slice_blk, env_off, col_begin, ent_w, ent_off;
@@ -341,17 +342,25 @@ struct MergedOpCache {
   List lru;  // most recently used first
   std::map<Key, List::iterator> idx;
   size_t bytes = 0;
-  std::shared_ptr<MergedOpDevice> find(const Key& k) {
+  std::shared_ptr<MergedOpDevice> find(const Key& k, const Layout& mpo, const Layout& opr) {
     std::lock_guard<std::mutex> lk(mu);
     auto it = idx.find(k);
     if (it == idx.end()) return nullptr;
+    const MergedOpDevice& d = *it->second->second;
+    if (!d.mpo.same_structure(mpo) || !d.opr.same_structure(opr)) return nullptr;  // hash collision: rebuild
     lru.splice(lru.begin(), lru, it->second);
     return it->second->second;
   }
+  // insert, replacing a plan of other structures under the same key (callers hold their plans by shared_ptr)
   void insert(const Key& k, std::shared_ptr<MergedOpDevice> v) {
     static const size_t budget = device_memory_fraction(0.01, 2.0);
     std::lock_guard<std::mutex> lk(mu);
-    if (idx.count(k)) return;
+    auto old = idx.find(k);
+    if (old != idx.end()) {
+      bytes -= old->second->second->bytes();
+      lru.erase(old->second);
+      idx.erase(old);
+    }
     bytes += v->bytes();
     lru.emplace_front(k, std::move(v));
     idx[k] = lru.begin();
@@ -376,11 +385,13 @@ MergedOpCache& merged_cache() {
 
 bool merged_operator_scalar(bool fwd, Tensor& xmpo, Tensor& xopr, Tensor& mopr, hipStream_t s) {
   const MergedOpCache::Key key{fwd ? 1u : 0u, xmpo.L.hash, xopr.L.hash};
-  std::shared_ptr<MergedOpDevice> D = merged_cache().find(key);
+  std::shared_ptr<MergedOpDevice> D = merged_cache().find(key, xmpo.L, xopr.L);
   if (!D) {
     MergedOpPlan P;
     if (!merged_operator_plan(fwd, xmpo.L, xopr.L, P)) return false;
     D = std::make_shared<MergedOpDevice>();
+    D->mpo = xmpo.L;
+    D->opr = xopr.L;
     D->Y = std::move(P.Y);
     D->nw = P.nw;
     D->nchunks = (int)P.chunks.size();

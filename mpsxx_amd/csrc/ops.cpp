@@ -40,6 +40,19 @@ struct CachedPlan {
   Plan plan;
   int out_mode = OUT_ACCUMULATE;
   size_t charged = 0;  // device bytes counted against the cache budget at insertion
+  uint64_t id = 0;     // identity of this build within the cache
+  // the operand layouts the plan was built for, in key order (absent operands empty, present[k] = 0): a hit is
+  // taken only when the caller's layouts have this structure, not on equal structure hashes alone
+  std::vector<Layout> shape;
+  std::vector<char> present;
+  bool matches(const std::vector<const Layout*>& ls) const {
+    if (ls.size() != shape.size()) return false;
+    for (size_t k = 0; k < ls.size(); ++k) {
+      if ((ls[k] != nullptr) != (present[k] != 0)) return false;
+      if (ls[k] && !ls[k]->same_structure(shape[k])) return false;
+    }
+    return true;
+  }
 };
 
 // MPSXX_PLAN_TRACE: host time of each plan build, split into the symbolic contractions, the device steps (tiling
@@ -70,8 +83,17 @@ struct TraceScope {
 // so the plans an operation holds at once (a Davidson solve over its operator groups) stay valid.
 class PlanCache {
  public:
-  CachedPlan* find(const std::vector<uint64_t>& key) {
+  // The plan cached under `key` if it was built for operands of the structures `ls` (key order, nullptr for an
+  // absent operand).  Where the key holds a plan of other structures (a structure-hash collision), the lookup moves
+  // on to the key extended by one more word, and `key` returns the key the plan is (to be) cached under; plans
+  // other callers hold stay valid.
+  CachedPlan* find(std::vector<uint64_t>& key, const std::vector<const Layout*>& ls) {
     auto it = map_.find(key);
+    while (it != map_.end() && !it->second.first->matches(ls)) {
+      ++stats_.collisions;
+      key.push_back(0x9e3779b97f4a7c15ull);
+      it = map_.find(key);
+    }
     if (it == map_.end()) {
       miss_t0_ = std::chrono::steady_clock::now();  // a miss is followed by the build and insert()
       BuildTrace& bt = btrace();
@@ -80,26 +102,30 @@ class PlanCache {
       return nullptr;
     }
     ++stats_.hits;
-    lru_.splice(lru_.begin(), lru_, it->second.second);
-    CachedPlan* cp = it->second.first.get();
-    if (cp->plan.stripped) {
-      cp->plan.restore();
-      ++stats_.restores;
-      bytes_ -= cp->charged;
-      cp->charged = cp->plan.device_bytes();
-      bytes_ += cp->charged;
-      make_room(0);
-    }
-    return cp;
+    return touch(it);
   }
-  // lookup that builds nothing (nullptr if the plan is no longer cached)
-  CachedPlan* peek(const std::vector<uint64_t>& key) {
+  // identity of the plan cached under `key` (0: none)
+  uint64_t id_of(const std::vector<uint64_t>& key) const {
     auto it = map_.find(key);
-    if (it == map_.end()) return nullptr;
-    return find(key);
+    return it == map_.end() ? 0 : it->second.first->id;
   }
-  CachedPlan* insert(const std::vector<uint64_t>& key, std::unique_ptr<CachedPlan> p) {
+  // lookup that builds nothing: the plan with identity `id` if it is still cached under `key`, else nullptr
+  CachedPlan* peek(const std::vector<uint64_t>& key, uint64_t id) {
+    auto it = map_.find(key);
+    if (it == map_.end() || it->second.first->id != id) return nullptr;
+    return touch(it);
+  }
+  CachedPlan* insert(const std::vector<uint64_t>& key, std::unique_ptr<CachedPlan> p,
+                     const std::vector<const Layout*>& ls) {
     ++stats_.builds;
+    p->id = ++next_id_;
+    p->shape.assign(ls.size(), Layout());
+    p->present.assign(ls.size(), 0);
+    for (size_t k = 0; k < ls.size(); ++k)
+      if (ls[k]) {
+        p->shape[k] = *ls[k];
+        p->present[k] = 1;
+      }
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - miss_t0_).count();
     stats_.build_ms += ms;
     const BuildTrace& bt = btrace();
@@ -152,7 +178,24 @@ class PlanCache {
     }
     while (map_.size() > kKeep && bytes_ + add > budget()) evict_back();
   }
+  using Map = std::map<std::vector<uint64_t>,
+                       std::pair<std::unique_ptr<CachedPlan>, std::list<std::vector<uint64_t>>::iterator>>;
+  // a hit: most recently used, slot buffers allocated again if they were released
+  CachedPlan* touch(Map::iterator it) {
+    lru_.splice(lru_.begin(), lru_, it->second.second);
+    CachedPlan* cp = it->second.first.get();
+    if (cp->plan.stripped) {
+      cp->plan.restore();
+      ++stats_.restores;
+      bytes_ -= cp->charged;
+      cp->charged = cp->plan.device_bytes();
+      bytes_ += cp->charged;
+      make_room(0);
+    }
+    return cp;
+  }
   PlanCacheStats stats_;
+  uint64_t next_id_ = 0;
   std::chrono::steady_clock::time_point miss_t0_ = std::chrono::steady_clock::now();
   static size_t budget() {
     static const size_t b = [] {
@@ -165,8 +208,7 @@ class PlanCache {
   static constexpr size_t kMax = 1024;
   static constexpr size_t kKeep = kPlanCacheKeep;
   size_t bytes_ = 0;
-  std::map<std::vector<uint64_t>, std::pair<std::unique_ptr<CachedPlan>, std::list<std::vector<uint64_t>>::iterator>>
-      map_;
+  Map map_;
   std::list<std::vector<uint64_t>> lru_;
 };
 
@@ -187,6 +229,7 @@ thread_local std::vector<uint64_t> t_plan_key;  // key of the plan the last plan
 void record_last_run(const std::vector<uint64_t>& key, const std::vector<Tensor*>& ins) {
   LastRun& r = last_run();
   r.key = key;
+  r.plan_id = cache().id_of(key);
   r.tensors.clear();
   r.inputs.clear();
   r.hashes.clear();
@@ -209,8 +252,11 @@ Plan& last_plan(std::vector<const double*>& inputs) {
   for (size_t i = 0; i < r.tensors.size(); ++i)
     if (!Tensor::alive(r.tensors[i]) || r.tensors[i]->ptr() != r.inputs[i] || r.tensors[i]->L.hash != r.hashes[i])
       throw Error(MPSXX_ERR_ARG, "profile: an input of the last hot-path call was freed or changed since");
-  CachedPlan* cp = cache().peek(r.key);
+  CachedPlan* cp = cache().peek(r.key, r.plan_id);
   if (!cp) throw Error(MPSXX_ERR_ARG, "profile: the plan of the last hot-path call was evicted from the cache");
+  for (size_t i = 0; i < r.tensors.size(); ++i)
+    if (i >= cp->shape.size() || !cp->present[i] || !r.tensors[i]->L.same_structure(cp->shape[i]))
+      throw Error(MPSXX_ERR_ARG, "profile: an input of the last hot-path call changed its block structure since");
   inputs = r.inputs;
   return cp->plan;
 }
@@ -248,7 +294,7 @@ class PlanBuilder {
     btrace().pairs += (int64_t)sym.pairs.size();
     const int id = add_inter(sym.C, P_.precision != MPSXX_FP64 ? 1 : 0);
     TraceScope ts(btrace().step_ms);
-    P_.steps.push_back(make_contract_step(sym, a, b, id, 0, 1.0));
+    P_.steps.push_back(make_contract_step(sym, a, b, id, 0, 1.0, policy()));
     P_.flops += sym.flops;
     return id;
   }
@@ -265,7 +311,7 @@ class PlanBuilder {
     btrace().pairs += (int64_t)sym.pairs.size();
     const int id = add_output(sym.C);
     TraceScope ts(btrace().step_ms);
-    P_.steps.push_back(make_contract_step(sym, a, b, id, 1, alpha));
+    P_.steps.push_back(make_contract_step(sym, a, b, id, 1, alpha, policy()));
     P_.flops += sym.flops;
     P_.bytes += sym.C.nelem * 8;
   }
@@ -295,9 +341,9 @@ class PlanBuilder {
       return id;
     }
     const int x = add_inter(s1.C, prec);
-    P_.steps.push_back(make_contract_step(s1, y, w1, x, 0, 1.0));
+    P_.steps.push_back(make_contract_step(s1, y, w1, x, 0, 1.0, policy()));
     const int z = add_inter(s2.C, prec);
-    P_.steps.push_back(make_contract_step(s2, x, w2, z, 0, 1.0));
+    P_.steps.push_back(make_contract_step(s2, x, w2, z, 0, 1.0, policy()));
     P_.flops += s1.flops + s2.flops;
     return z;
   }
@@ -337,6 +383,8 @@ class PlanBuilder {
   Plan finish() { return std::move(P_); }
 
  private:
+  // tile policy of the plan's GEMM steps
+  GemmPolicy policy() const { return default_policy(P_.precision); }
   int add_inter(const Layout& L, int prec) {
     const int id = (int)P_.slot_layout.size();
     P_.slot_layout.push_back(L);
@@ -373,20 +421,29 @@ std::vector<uint64_t> make_key(int op, int prec, std::initializer_list<const Ten
   return k;
 }
 
+// the operand layouts a plan cached for (ins..., out) is checked against on a hit; an empty output is absent
+std::vector<const Layout*> key_layouts(const std::vector<Tensor*>& ins, const Tensor* out) {
+  std::vector<const Layout*> ls;
+  for (const Tensor* t : ins) ls.push_back(&t->L);
+  ls.push_back(out && !out->empty ? &out->L : nullptr);
+  return ls;
+}
+
 // Look up or build a plan, shape the output, run.
 template <typename BuildFn>
-void execute(const std::vector<uint64_t>& key, BuildFn build, int out_mode, std::vector<Tensor*> ins, Tensor& out,
+void execute(std::vector<uint64_t> key, BuildFn build, int out_mode, std::vector<Tensor*> ins, Tensor& out,
              hipStream_t s, unsigned convert_mask = ~0u) {
-  CachedPlan* cp = cache().find(key);
+  const std::vector<const Layout*> ls = key_layouts(ins, &out);
+  CachedPlan* cp = cache().find(key, ls);
   if (!cp) {
     auto np = std::make_unique<CachedPlan>();
     np->plan = build();
     np->out_mode = out_mode;
-    cp = cache().insert(key, std::move(np));
+    cp = cache().insert(key, std::move(np), ls);
   }
   Plan& P = cp->plan;
   if (out_mode == OUT_OVERWRITE || out.empty) out.set_layout(P.out, s);
-  else if (out.L.hash != P.out.hash) out.relayout(P.out, s);
+  else if (!out.L.same_structure(P.out)) out.relayout(P.out, s);
   std::vector<const double*> ptrs;
   for (Tensor* t : ins) ptrs.push_back(t->ptr());
   run_plan_ex(P, ptrs, out.ptr(), s, {}, convert_mask);
@@ -642,12 +699,13 @@ Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& l
                          const Layout& wfn, const Layout* out, int prec, int copy) {
   std::vector<uint64_t> key{OP_SIGMA2, (uint64_t)prec, lmpo.hash, rmpo.hash, lopr.hash, ropr.hash, wfn.hash,
                             out ? out->hash : 0ull, (uint64_t)copy};
+  const std::vector<const Layout*> ls{&lmpo, &rmpo, &lopr, &ropr, &wfn, out};
+  CachedPlan* cp = cache().find(key, ls);
   t_plan_key = key;
-  CachedPlan* cp = cache().find(key);
   if (cp) return cp->plan;
   auto np = std::make_unique<CachedPlan>();
   np->plan = build_plan_sigma_twosite(lmpo, rmpo, lopr, ropr, wfn, out, prec);
-  return cache().insert(key, std::move(np))->plan;
+  return cache().insert(key, std::move(np), ls)->plan;
 }
 
 void op_sigma_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec,
@@ -655,7 +713,7 @@ void op_sigma_twosite(Tensor& lmpo, Tensor& rmpo, Tensor& lopr, Tensor& ropr, Te
   for (Tensor* t : {&lmpo, &rmpo, &lopr, &ropr, &wfn}) require(*t, "compute_sigma_vector");
   Plan& P = plan_sigma_twosite(lmpo.L, rmpo.L, lopr.L, ropr.L, wfn.L, existing(sgv), prec);
   if (sgv.empty) sgv.set_layout(P.out, s);
-  else if (sgv.L.hash != P.out.hash) sgv.relayout(P.out, s);
+  else if (!sgv.L.same_structure(P.out)) sgv.relayout(P.out, s);
   run_plan_ex(P, {lmpo.ptr(), rmpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s, {}, convert_mask);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
@@ -666,8 +724,9 @@ Plan& plan_sigma_onesite(const Layout& mpo, const Layout& lopr, const Layout& ro
                          const Layout* out, int prec, int copy) {
   std::vector<uint64_t> key{OP_SIGMA1, (uint64_t)prec, mpo.hash, lopr.hash, ropr.hash, wfn.hash,
                             out ? out->hash : 0ull, (uint64_t)copy};
+  const std::vector<const Layout*> ls{&mpo, &lopr, &ropr, &wfn, out};
+  CachedPlan* cp = cache().find(key, ls);
   t_plan_key = key;
-  CachedPlan* cp = cache().find(key);
   if (cp) return cp->plan;
   auto np = std::make_unique<CachedPlan>();
   PlanBuilder B(prec);
@@ -677,14 +736,14 @@ Plan& plan_sigma_onesite(const Layout& mpo, const Layout& lopr, const Layout& ro
   const int s2 = B.contract(s1, false, {1, 2}, sm, false, {0, 2});
   B.contract_out(s2, false, {3, 1}, sr, false, {1, 2}, out);
   np->plan = B.finish();
-  return cache().insert(key, std::move(np))->plan;
+  return cache().insert(key, std::move(np), ls)->plan;
 }
 
 void op_sigma_onesite(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s) {
   for (Tensor* t : {&mpo, &lopr, &ropr, &wfn}) require(*t, "compute_sigma_vector");
   Plan& P = plan_sigma_onesite(mpo.L, lopr.L, ropr.L, wfn.L, existing(sgv), prec);
   if (sgv.empty) sgv.set_layout(P.out, s);
-  else if (sgv.L.hash != P.out.hash) sgv.relayout(P.out, s);
+  else if (!sgv.L.same_structure(P.out)) sgv.relayout(P.out, s);
   run_plan_ex(P, {mpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s, {}, ~0u);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
@@ -695,8 +754,9 @@ Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wf
                         int copy) {
   std::vector<uint64_t> key{OP_SIGMAM, (uint64_t)prec, lopr.hash, ropr.hash, wfn.hash, out ? out->hash : 0ull,
                             (uint64_t)copy};
+  const std::vector<const Layout*> ls{&lopr, &ropr, &wfn, out};
+  CachedPlan* cp = cache().find(key, ls);
   t_plan_key = key;
-  CachedPlan* cp = cache().find(key);
   if (cp) return cp->plan;
   auto np = std::make_unique<CachedPlan>();
   PlanBuilder B(prec);
@@ -705,7 +765,7 @@ Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wf
   const int s1 = B.contract(sl, false, {2}, sw, false, {0});
   B.contract_out(s1, false, {1, 2}, sr, false, {1, 2}, out);
   np->plan = B.finish();
-  return cache().insert(key, std::move(np))->plan;
+  return cache().insert(key, std::move(np), ls)->plan;
 }
 
 void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int prec, hipStream_t s,
@@ -715,7 +775,7 @@ void op_sigma_merged(Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor& sgv, int p
     throw Error(MPSXX_ERR_ARG, "merged sigma: expects 3-index operators and a 2-index wavefunction");
   Plan& P = plan_sigma_merged(lopr.L, ropr.L, wfn.L, existing(sgv), prec);
   if (sgv.empty) sgv.set_layout(P.out, s);
-  else if (sgv.L.hash != P.out.hash) sgv.relayout(P.out, s);
+  else if (!sgv.L.same_structure(P.out)) sgv.relayout(P.out, s);
   run_plan_ex(P, {lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s, {}, convert_mask);
   last_stats().flops = P.flops;
   last_stats().bytes = P.bytes;
@@ -752,7 +812,8 @@ void op_sigma_conj(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor&
     ex = &exneg;
   }
   auto key = make_key(OP_SIGMA_CONJ, 0, {&mpo, &lopr, &ropr, &wfn}, &sgv);
-  CachedPlan* cp = cache().find(key);
+  const std::vector<const Layout*> ls = key_layouts({&mpo, &lopr, &ropr, &wfn}, &sgv);
+  CachedPlan* cp = cache().find(key, ls);
   if (!cp) {
     PlanBuilder B(MPSXX_FP64);
     const int sm = B.input(mpo.L, 0), sl = B.input(lopr.L, 1), sr = B.input(ropr.L, 2), sw = B.input(wfn.L, 3);
@@ -762,7 +823,7 @@ void op_sigma_conj(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor&
     B.contract_out(s2, false, {1, 3}, sr, false, {0, 1}, ex);
     auto np = std::make_unique<CachedPlan>();
     np->plan = B.finish();
-    cp = cache().insert(key, std::move(np));
+    cp = cache().insert(key, std::move(np), ls);
   }
   Plan& P = cp->plan;
   Layout outL = P.out;
@@ -771,7 +832,7 @@ void op_sigma_conj(Tensor& mpo, Tensor& lopr, Tensor& ropr, Tensor& wfn, Tensor&
     for (QN& x : v) x = -x;
   outL.finalize();
   if (sgv.empty) sgv.set_layout(outL, s);
-  else if (sgv.L.hash != outL.hash) sgv.relayout(outL, s);
+  else if (!sgv.L.same_structure(outL)) sgv.relayout(outL, s);
   run_plan(P, {mpo.ptr(), lopr.ptr(), ropr.ptr(), wfn.ptr()}, sgv.ptr(), s);
 }
 

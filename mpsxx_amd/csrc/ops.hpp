@@ -22,6 +22,7 @@ OpStats& last_stats();
 // exists unchanged (error otherwise), so a plan evicted since, or a tensor freed or reshaped since, is never used.
 struct LastRun {
   std::vector<uint64_t> key;
+  uint64_t plan_id = 0;  // identity of the cached plan the call ran
   std::vector<const Tensor*> tensors;
   std::vector<const double*> inputs;
   std::vector<uint64_t> hashes;
@@ -75,6 +76,7 @@ void op_renormalize_merged(bool fwd, Tensor& opr0, Tensor& bra, Tensor& ket, Ten
 struct PlanCacheStats {
   int64_t hits = 0, builds = 0, evictions = 0, plans = 0, device_bytes = 0;
   int64_t releases = 0, restores = 0;  // plans whose slot buffers were released under the budget / allocated again
+  int64_t collisions = 0;              // lookups whose key held a plan of other operand structures
   double build_ms = 0.0;
 };
 PlanCacheStats plan_cache_stats();

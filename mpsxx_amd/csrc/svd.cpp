@@ -426,6 +426,34 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
   return true;
 }
 
+// The argument rules of rocsolver_dgesvd_strided_batched / rocsolver_dgesvdj_strided_batched with singular vectors
+// (rocSOLVER's documented requirements for an m x n column-major A, k = min(m, n): lda >= m, strideA >= lda n,
+// strideS >= k, ldu >= m, strideU >= ldu k, ldv >= k, strideV >= ldv n, strideE >= k - 1), and that each device
+// buffer holds the `batch` strided problems the call reads or writes, checked on the host before every call.
+void check_gesvd_args(int m, int n, int lda, int64_t sA, int64_t sS, int ldu, int64_t sU, int ldv, int64_t sV,
+                      int64_t sE, int batch, const DevBuf& A, const DevBuf& S, const DevBuf& U, const DevBuf& V,
+                      const DevBuf* E) {
+  const int64_t k = std::min(m, n);
+  auto fail = [&](const char* what) {
+    throw Error(MPSXX_ERR_HIP, std::string("svd: rocSOLVER argument check failed (") + what + ") for m " +
+                                   std::to_string(m) + " n " + std::to_string(n) + " batch " + std::to_string(batch));
+  };
+  if (m < 1 || n < 1 || batch < 1) fail("empty problem");
+  if (lda < m || sA < (int64_t)lda * n) fail("lda / strideA");
+  if (sS < k) fail("strideS");
+  if (ldu < m || sU < (int64_t)ldu * k) fail("ldu / strideU");
+  if (ldv < k || sV < (int64_t)ldv * n) fail("ldv / strideV");
+  if (E && sE < k - 1) fail("strideE");
+  auto holds = [&](const DevBuf& b, int64_t stride, int64_t extent) {
+    return (int64_t)b.bytes >= ((int64_t)(batch - 1) * stride + extent) * (int64_t)sizeof(double);
+  };
+  if (!holds(A, sA, (int64_t)lda * n)) fail("A buffer");
+  if (!holds(S, sS, k)) fail("S buffer");
+  if (!holds(U, sU, (int64_t)ldu * k)) fail("U buffer");
+  if (!holds(V, sV, (int64_t)ldv * n)) fail("V buffer");
+  if (E && !holds(*E, sE, std::max<int64_t>(k - 1, 1))) fail("E buffer");
+}
+
 // The rocSOLVER path: every block zero-padded to (mmax x nmax), one strided-batched gesvd (QR iteration) over all.
 void rocsolver_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, const double* src, int64_t src_elems,
                    int mmax, int nmax, int kmax, int64_t sa, int64_t su, int64_t sv, DevBuf& Sv, DevBuf& U, DevBuf& V,
@@ -447,6 +475,7 @@ void rocsolver_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects,
   // gesvd destroys A; keep the input for the (rare) blocks whose bidiagonal QR iteration does not converge
   DevBuf A0(nb * sa * sizeof(double));
   hip_check(hipMemcpyAsync(A0.p, A.p, nb * sa * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
+  check_gesvd_args(nmax, mmax, nmax, sa, kmax, nmax, su, kmax, sv, std::max(kmax, 2), nb, A, Sv, U, V, &E);
   rocblas_status st = rocsolver_dgesvd_strided_batched(
       h, rocblas_svect_singular, rocblas_svect_singular, nmax, mmax, A.as<double>(), nmax, sa, Sv.as<double>(), kmax,
       U.as<double>(), nmax, su, V.as<double>(), kmax, sv, E.as<double>(), std::max(kmax, 2), rocblas_outofplace,
@@ -461,6 +490,8 @@ void rocsolver_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects,
   for (int k = 0; k < nb; ++k) {
     if (info_h[k] == 0) continue;
     DevBuf res(sizeof(double)), nsw(sizeof(rocblas_int)), inf(sizeof(rocblas_int));
+    // block k alone, in its batch slot: the slot lies inside the buffers when slots 0..k do
+    check_gesvd_args(nmax, mmax, nmax, sa, kmax, nmax, su, kmax, sv, 0, k + 1, A0, Sv, U, V, nullptr);
     st = rocsolver_dgesvdj_strided_batched(h, rocblas_svect_singular, rocblas_svect_singular, nmax, mmax,
                                            A0.as<double>() + k * sa, nmax, sa, 0.0, res.as<double>(), 100,
                                            nsw.as<rocblas_int>(), Sv.as<double>() + (int64_t)k * kmax, kmax,

@@ -40,7 +40,8 @@ def test_diagnostic_counters_are_host_only_and_consistent():
     """The plan-cache and device-allocation counters (no reference counterpart) are readable without a device: counts
     are non-negative, times finite, and NULL outputs are accepted."""
     pc = mpsxx_amd.plan_cache_stats()
-    assert set(pc) == {"hits", "builds", "evictions", "build_ms", "plans", "device_bytes", "releases", "restores"}
+    assert set(pc) == {"hits", "builds", "evictions", "build_ms", "plans", "device_bytes", "releases", "restores",
+                       "collisions"}
     assert all(v >= 0 for v in pc.values())
     assert pc["plans"] <= pc["builds"]
     assert pc["restores"] <= pc["releases"] <= pc["builds"] + pc["restores"]
