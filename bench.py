@@ -1,26 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark: two-site sigma vector H_eff|psi> on one MI355X (block GEMMs on MFMA).
+"""Benchmark: H_eff|psi> TFLOP/s and DMRG sweep time on one MI355X, FP32-MFMA mixed precision beside FP64.
 
-Headline workload: open Hubbard chain L=32, t=1, U=4, half filling, 2Sz=0, bond dimension M=512, two-site centre at
-the middle bond, compact nearest-neighbour MPO.  Sector dimensions follow a converged chain's spectrum shape and every
-quantum-number-allowed block holds random N(0,1)/sqrt(m) entries (synthetic data, see mpsxx_amd/workloads.py).  One
-step = one sigma-vector application with inputs resident in HBM.
+Headline workload: the merged-form two-site sigma vector (src/compute_sigma_vector.hpp:86-96) at the middle bond
+(sites 11, 12) of a 24-orbital quantum-chemistry chain, N = 24, 2Sz = 0, bond dimension M = 1024.  Integrals are
+synthetic from seed 1 (h_pq symmetric, diagonal -2..-0.5 linear, off-diagonal N(0, 0.05); v_pqrs = sum_k
+L^k_pq L^k_rs over 48 symmetric L^k with N(0, 0.1) entries), the MPO is the QC generator's (D_w = 1226 at the middle
+bond).  Sector dimensions follow a converged chain's spectrum shape and every allowed block holds random entries
+(synthetic data, mpsxx_amd/workloads.py); the merged operators L~, R~ are built on the device from the environments
+(compute_merged_operator) and stay resident, as inside Davidson.  One step = one sigma application with its inputs
+resident in HBM; `value` = algorithmic TFLOP/s of the FP32-MFMA sigma (flops = 2 m n k over the block pairs of the
+reference contraction order, counted by the planner).
 
-Prints one JSON line.  `value` = algorithmic TFLOP/s of the sigma vector (reference contraction order, flops counted
-by the planner); `roofline` = the kernel with the most device time per sigma (all its launches: algorithmic flops /
-device time, HIP events on the launch stream) against its bound; `cpu_baseline` = the oracle (FP64 CPU restatement of
-the reference algorithm) on the same inputs on one host core.  Beside it: the same sigma with FP32 block GEMMs, and a
-batch of 64 independent chains (L in 16..64, U in {2,4,8}, M in {128,256,512}) applied sequentially in FP64 and as one
-grouped launch per step (FP64 and FP32, hipGraph replay, autotuned tiles), and the merged-form sigma at the middle of
-a 24-orbital quantum-chemistry chain at M=1024 (synthetic integrals from seed 1, QC MPO with D_w=1226) in FP64, FP32 and
-split-BF16x3 with the operators resident; and the sweep half of the metric: device-resident two-site sweeps of the
-L=32 chain over M 128x3, 256x3, 512x2 in FP64 and in mixed precision (FP32 sigma vectors + FP64 refinement), with
-the energy difference between them; and merged-form sweeps of the 24-orbital chain over M 256, 512, 1024, 1024 in
-FP64 and mixed precision.
+Beside it, on the same inputs: the FP64 and split-BF16x3 sigma.  The sigma on a real state: the merged operators at
+the middle bond of the state the FP64 sweeps below converge to.  The sweep half of the metric: merged two-site
+sweeps of the same chain over M 256, 512, then 1024 until the energy changes by less than 1e-10, in FP64 and in
+mixed precision (FP32 or split-BF16x3 sigma vectors, FP64 Davidson, refinement and SVD), with sweep times and
+|E - E_fp64|.  The CPU baseline: the oracle (FP64 restatement of the reference's block contraction) with its block
+dgemm in numpy's bundled OpenBLAS, on one core and on all cores given to the process, over a bounded sample of the
+headline sigma (the terms of every k-th sector of the MPO bond index; sigma is a sum over that index), compared
+with the device's FP64 result on the same sample.  Secondary legs: the two-site sigma of the Hubbard chain L = 32,
+M = 512; 64 independent Hubbard chains as one grouped launch; two-site sweeps of the Hubbard chain L = 32.
 """
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -33,17 +37,21 @@ sys.path.insert(0, ROOT)
 import mpsxx_amd as mx  # noqa: E402
 from mpsxx_amd import workloads  # noqa: E402
 from mpsxx_amd.batch import (ALL_INPUTS, WFN_ONLY, SigmaBatch, dominant_kernel, last_plan_profile,  # noqa: E402
-                             sigma_twosite_profile)
+                             last_plan_tiles, sigma_twosite_profile)
+from mpsxx_amd.qst import QST  # noqa: E402
 
+METRIC = "H·ψ TFLOP/s + sweep time, FP32-MFMA vs FP64 GPU at |ΔE|≤1e-8 Eh"
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
-FP32_PEAK_TFLOPS = 157.3
+PEAK_TFLOPS = {mx.FP64: 78.6, mx.FP32: 157.3, mx.BF16X3: 2500.0}
+PNAME = {mx.FP64: "fp64", mx.FP32: "fp32", mx.BF16X3: "bf16x3"}
+HEADLINE_TRAFFIC = "traffic_sigma_merged_qc_norb24_M1024.md"
+HUBBARD_TRAFFIC = "traffic_sigma_twosite_hubbard_L32_M512.md"
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on the headline workload from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_sigma_twosite_hubbard_L32_M512.md")
+def measured_traffic(fname, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, scaled
+    by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -53,44 +61,28 @@ def measured_traffic(kernel):
     return None
 
 
-def sigma_stats(dl, prec):
-    flops = ctypes.c_int64()
-    nbytes = ctypes.c_int64()
-    nsteps = ctypes.c_int32()
-    mx._check(mx.lib().mpsxx_sigma_twosite_stats(*[d.h for d in dl], ctypes.c_int(prec), ctypes.byref(flops),
-                                                 ctypes.byref(nbytes), ctypes.byref(nsteps)))
-    return flops.value, nbytes.value
-
-
-def time_sigma(dl, prec, steps, warmup):
-    """Wall time per sigma application (inputs in HBM, output accumulating as in Davidson's multH)."""
-    out = mx.DTensor()
-    mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
-    for _ in range(max(warmup, 1)):
-        mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
-    mx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
-    mx.synchronize()
-    return (time.perf_counter() - t0) / steps
-
-
 def roofline_of(entries, prec):
+    """Roofline of the kernel with the most device time over all its launches of one application: algorithmic
+    flops (bytes for the AXPY kernel) / device time from HIP events around each launch on its stream.  For split
+    BF16x3 the flops are the bf16 MFMA products issued (three per FP32 product) against the bf16 peak."""
     name, agg, allk = dominant_kernel(entries)
     per_kernel = {k: dict(ms=v["ms"], launches=v["launches"], flops=v["flops"], bytes=v["bytes"],
-                          avg_launch_us=1e3 * v["ms"] / v["launches"]) for k, v in allk.items()}
+                          avg_launch_us=1e3 * v["ms"] / v["launches"],
+                          tflops=v["flops"] / max(v["ms"], 1e-12) / 1e9) for k, v in allk.items()}
     sec = agg["ms"] * 1e-3
     if name == "axpy_grouped_kernel":
         achieved = agg["bytes"] / sec / 1e9
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS)
     else:
-        peak = FP64_PEAK_TFLOPS if prec == mx.FP64 else FP32_PEAK_TFLOPS
-        achieved = agg["flops"] / sec / 1e12
-        roof = dict(bound="mfma", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak)
+        mult = 3 if prec == mx.BF16X3 else 1
+        achieved = mult * agg["flops"] / sec / 1e12
+        roof = dict(bound="mfma", achieved=achieved, peak=PEAK_TFLOPS[prec], unit="TFLOP/s",
+                    frac=achieved / PEAK_TFLOPS[prec])
+        if prec == mx.BF16X3:
+            roof["note"] = "bf16 MFMA products issued (3 per FP32 product) against the dense bf16 peak"
     roof.update(traffic=None, kernel=name, launches_per_sigma=agg["launches"],
                 avg_launch_us=1e3 * agg["ms"] / agg["launches"], algorithmic_flops=agg["flops"],
-                algorithmic_bytes=agg["bytes"])
+                algorithmic_bytes=agg["bytes"], device_ms_per_sigma=sum(v["ms"] for v in allk.values()))
     return roof, per_kernel
 
 
@@ -103,11 +95,236 @@ def qdot(a, b):
     return float(sum(np.vdot(a.blocks[k], b.blocks[k]) for k in a.blocks if k in b.blocks))
 
 
+# -- merged sigma ------------------------------------------------------------------------------------------------
+def merged_leg(Lm, Rm, psim, prec, steps, warmup, ref=None):
+    """Time `steps` merged sigma applications with the operators resident (their FP32 / split copies made by the
+    first call and kept, only the wavefunction refreshed per call, as inside Davidson's multH); profile each kernel
+    launch of the plan with HIP events; one fresh application for the accuracy against `ref`."""
+    fresh = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=prec)  # converts every input once
+    mx.synchronize()
+    one = fresh.to_qst()
+    del fresh
+    out = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=prec)
+    mask = 1 << 2
+    for _ in range(max(warmup, 1)):
+        mx.compute_sigma_vector_merged_ex(Lm, Rm, psim, out, prec, mask)
+    mx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mx.compute_sigma_vector_merged_ex(Lm, Rm, psim, out, prec, mask)
+    mx.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    ent, pf = last_plan_profile(3)
+    tiles = last_plan_tiles()
+    roof, per_kernel = roofline_of(ent, prec)
+    r = dict(ms=sec * 1e3, tflops=pf / sec / 1e12, flops_per_sigma=pf, roofline=roof, kernels=per_kernel,
+             tiles=tiles)
+    if ref is not None:
+        r["rel_diff_vs_fp64"] = rel_diff(one, ref)
+    del out
+    return r, one, sec, pf
+
+
+def sample_sectors(Lh, Rh, stride):
+    """L~ and R~ restricted to every `stride`-th sector of their shared MPO bond index (index 1): sigma =
+    sum over that index of independent terms, so the restriction is the sum over the sample's sectors."""
+    keep = set(range(0, len(Lh.qshape[1]), stride))
+    out = []
+    for X in (Lh, Rh):
+        Y = QST(X.q, X.qshape, X.dshape, X.qsym)
+        Y.blocks = {k: v for k, v in X.blocks.items() if k[1] in keep}
+        out.append(Y)
+    return out[0], out[1], len(keep)
+
+
+def cpu_cores():
+    """Cores the process may use: its affinity mask, capped by OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(Lm, Rm, psim, total_flops, cpu_seconds):
+    """The oracle's merged sigma (FP64, block dgemm in OpenBLAS) on a sample of the headline sigma sized for about
+    `cpu_seconds` on one core (calibrated on a 1/64 sample), timed on one core and on all cores, and the device FP64
+    sigma of the same sample against it."""
+    import oracle
+    Lh, Rh, ph = Lm.to_qst(), Rm.to_qst(), psim.to_qst()
+    nw = len(Lh.qshape[1])
+
+    def device_sample(Ls, Rs):
+        dL, dR = mx.DTensor(Ls), mx.DTensor(Rs)
+        y = mx.compute_sigma_vector_merged(dL, dR, psim, precision=mx.FP64)
+        mx.synchronize()
+        _, pf = last_plan_profile(1)
+        return y.to_qst(), pf
+
+    def timed(Ls, Rs, threads):
+        oracle.set_threads(threads)
+        t0 = time.perf_counter()
+        y = oracle.sigma_merged(Ls, Rs, ph)
+        return y, time.perf_counter() - t0
+
+    blas = oracle.use_blas(True, 1)
+    try:
+        Ls, Rs, _ = sample_sectors(Lh, Rh, 64)
+        _, pf_probe = device_sample(Ls, Rs)
+        _, t_probe = timed(Ls, Rs, 1)
+        stride = max(1, int(math.ceil(total_flops / max(pf_probe / t_probe * cpu_seconds, 1.0))))
+        stride = min(stride, nw)
+        Ls, Rs, nkeep = sample_sectors(Lh, Rh, stride)
+        dev, pf = device_sample(Ls, Rs)
+        y1, t1 = timed(Ls, Rs, 1)
+        nc = cpu_cores()
+        yn, tn = timed(Ls, Rs, nc)
+    finally:
+        oracle.use_blas(False)
+        oracle.set_threads(1)
+    scale = max(float(np.max(np.abs(b))) for b in y1.blocks.values() if b.size)
+    sample = ("the merged sigma terms of every %d-th of the %d sectors of the MPO bond index (%d sectors, %.3g of "
+              "the %.3g flops of one sigma)" % (stride, nw, nkeep, pf, total_flops))
+    machine = dict(nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)), cpu_model=cpu_model(),
+                   dgemm="numpy's bundled " + blas.strip(), threading="OpenMP over output blocks, 1 BLAS thread each")
+    one = dict(value=pf / t1 / 1e12, unit="TFLOP/s", cores=1, kind="port", sample=sample, seconds=t1,
+               sigma_seconds_extrapolated=total_flops * t1 / pf)
+    allc = dict(value=pf / tn / 1e12, unit="TFLOP/s", cores=nc, kind="port", sample=sample, seconds=tn,
+                sigma_seconds_extrapolated=total_flops * tn / pf, single_core=one, **machine)
+    parity = dict(same_blocks=dev.block_set() == y1.block_set(), max_abs_diff=dev.max_abs_diff(y1) / scale,
+                  all_cores_max_abs_diff=yn.max_abs_diff(y1) / scale, sample_flops=pf)
+    return allc, parity
+
+
+# -- sweeps --------------------------------------------------------------------------------------------------------
+def qc_sweep_leg(norb, schedule, precisions, max_final=6, keep_fp64_state=False):
+    """Merged two-site sweeps (src/make_sweep.cpp:112) of the QC chain with synthetic integrals from seed 1 over the
+    M schedule, the last M repeated until the sweep energy changes by less than 1e-10 (src/dmrg.cpp:44-78, at most
+    max_final sweeps at that M): per precision the total and per-sweep seconds, the converged energy and
+    |E - E_fp64|.  With keep_fp64_state, also the MPO and site tensors of the converged FP64 state."""
+    from mpsxx_amd import qc
+    from mpsxx_amd.dmrg import SWEEP_MERGED, DeviceDMRG, initial_mps
+    h, v = qc.synthetic_integrals(norb, seed=1)
+    m = qc.QcMpo(h, v)
+    mpos = m.devices()
+    out = {"norb": norb, "schedule": schedule, "tol": 1e-10, "D_w_max": int(max(m.bond_dims()))}
+    e64, state = None, None
+    for prec in precisions:
+        name = {mx.FP64: "fp64", mx.FP32: "fp32_mixed", mx.BF16X3: "bf16x3_mixed"}[prec]
+        print("qc sweeps: %s" % name, file=sys.stderr, flush=True)
+        dm = DeviceDMRG(mpos, initial_mps(norb, (norb, 0), schedule[0], 0), schedule[0], form=SWEEP_MERGED)
+        secs, e, st = [], None, None
+        t0 = time.perf_counter()
+        for M in schedule[:-1]:
+            e, st = dm.sweep(M, 1e-10, precision=prec)
+            secs.append(round(st["seconds"], 3))
+        for k in range(max_final):
+            e_prev = e
+            e, st = dm.sweep(schedule[-1], 1e-10, precision=prec)
+            secs.append(round(st["seconds"], 3))
+            if k > 0 and abs(e - e_prev) < 1e-10:
+                break
+        r = {"seconds_total": time.perf_counter() - t0, "seconds_per_sweep": secs, "sweeps": len(secs),
+             "energy": e, "last_sweep_svd_s": st["svd_s"], "last_sweep_davidson_s": st["davidson_s"],
+             "last_sweep_renorm_s": st["renorm_s"], "max_bond": st["max_bond"]}
+        if prec == mx.FP64:
+            e64 = e
+            if keep_fp64_state:
+                state = (mpos, [dm.site_device(k) for k in range(norb)])
+        elif e64 is not None:
+            r["abs_dE_vs_fp64"] = abs(e - e64)
+        out[name] = r
+        del dm
+    if "fp64" in out:
+        for name in ("fp32_mixed", "bf16x3_mixed"):
+            if name in out:
+                out[name]["speedup_vs_fp64"] = out["fp64"]["seconds_total"] / out[name]["seconds_total"]
+    return out, state
+
+
+def hubbard_sweep_leg(L, schedule, precisions):
+    """Device-resident two-site sweeps of the open Hubbard chain (t=1, U=4, half filling, 2Sz=0) from the seeded
+    random initial MPS over the M schedule; per precision: total and per-sweep seconds, final energy, and for the
+    mixed-precision runs |E - E_fp64|."""
+    from mpsxx_amd.dmrg import DeviceDMRG, initial_mps
+    from mpsxx_amd.mpo import hubbard_mpo
+    mpos = hubbard_mpo(L, 1.0, 4.0)
+    init = initial_mps(L, (L, 0), schedule[0], 0)
+    out = {"L": L, "schedule": schedule, "tol": 1e-10}
+    e64 = None
+    for prec in precisions:
+        name = {mx.FP64: "fp64", mx.FP32: "fp32_mixed", mx.BF16X3: "bf16x3_mixed"}[prec]
+        print("hubbard sweeps: %s" % name, file=sys.stderr, flush=True)
+        dm = DeviceDMRG(mpos, init, schedule[0])
+        secs, e, st = [], None, None
+        t0 = time.perf_counter()
+        for M in schedule:
+            e, st = dm.sweep(M, 1e-10, precision=prec)
+            secs.append(round(st["seconds"], 3))
+        r = {"seconds_total": time.perf_counter() - t0, "seconds_per_sweep": secs, "energy": e,
+             "last_sweep_svd_s": st["svd_s"], "last_sweep_davidson_s": st["davidson_s"],
+             "last_sweep_sigma_ms": st["sigma_ms"]}
+        if prec == mx.FP64:
+            e64 = e
+        elif e64 is not None:
+            r["abs_dE_vs_fp64"] = abs(e - e64)
+        out[name] = r
+    return out
+
+
+# -- secondary sigma legs ------------------------------------------------------------------------------------------
+def hubbard_sigma_leg(L, M, U, steps, warmup):
+    """Two-site sigma (src/compute_sigma_vector.hpp:54-70) of the open Hubbard chain L, M at the middle bond (compact
+    D_w = 6 MPO, synthetic fill), FP64 and FP32, with its per-kernel roofline."""
+    lm, rm, le, re_, psi = workloads.hubbard_twosite(L, M, U)
+    dl = [mx.DTensor(x) for x in (lm, rm, le, re_, psi)]
+    res = dict(L=L, M=M, U=U, psi_blocks=len(psi.blocks), psi_elems=psi.nelem())
+    ref = None
+    for prec in (mx.FP64, mx.FP32):
+        out = mx.DTensor()
+        for _ in range(max(warmup, 1) + 1):
+            mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
+        mx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            mx.compute_sigma_vector_twosite(*dl, sgv0=out, precision=prec)
+        mx.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        flops = ctypes.c_int64()
+        nbytes = ctypes.c_int64()
+        nsteps = ctypes.c_int32()
+        mx._check(mx.lib().mpsxx_sigma_twosite_stats(*[d.h for d in dl], ctypes.c_int(prec), ctypes.byref(flops),
+                                                     ctypes.byref(nbytes), ctypes.byref(nsteps)))
+        roof, per_kernel = roofline_of(sigma_twosite_profile(*dl, precision=prec, reps=50), prec)
+        if prec == mx.FP64:
+            roof["traffic"] = measured_traffic(HUBBARD_TRAFFIC, roof["kernel"])
+        one = mx.compute_sigma_vector_twosite(*dl, precision=prec).to_qst()
+        r = dict(ms=sec * 1e3, tflops=flops.value / sec / 1e12, flops_per_sigma=flops.value,
+                 bytes_per_sigma=nbytes.value, roofline=roof, kernels=per_kernel)
+        if ref is None:
+            ref = one
+        else:
+            r["rel_diff_vs_fp64"] = rel_diff(one, ref)
+        res[PNAME[prec]] = r
+    return res
+
+
 def batched_leg(n, reps):
+    """n independent Hubbard chains (L in 16..64, U in {2,4,8}, M in {128,256,512}) applied sequentially in FP64
+    and as one grouped launch per step (FP64 and FP32, hipGraph replay, autotuned tiles)."""
     specs = workloads.batch_specs(n, 4)
     probs = [workloads.hubbard_twosite(L=L, M=M, U=U, seed=k) for k, (L, U, M) in enumerate(specs)]
     dev = [tuple(mx.DTensor(x) for x in p) for p in probs]
-    # plain FP64 path: one compute_sigma_vector call per instance
     outs = [mx.DTensor() for _ in dev]
     for d, o in zip(dev, outs):
         mx.compute_sigma_vector_twosite(*d, sgv0=o)
@@ -132,27 +349,12 @@ def batched_leg(n, reps):
             res[name + "_graph_wfn_ms"] = b.time(reps, WFN_ONLY, True)
         prof = b.profile(5)
         res[name + "_roofline"], _ = roofline_of(prof, prec)
-        # accuracy of one application against the plain FP64 path
         for t in inst:
             t[5].clear()
         b.run(ALL_INPUTS, True)
         mx.synchronize()
         outs_p = [t[5].to_qst() for t in inst]
         res[name + "_max_rel_err"] = max(rel_diff(o, r) for o, r in zip(outs_p, ref))
-        if prec == mx.FP32:
-            # per-instance energy functional <psi|H psi>/<psi|psi> from this launch vs the plain FP64 path
-            # (the environments are random, so E can be near zero: the error is also given on the operator scale
-            # ||H psi|| / ||psi||, which bounds |E|)
-            errs, errs_op = [], []
-            for pr, o, r in zip(probs, outs_p, ref):
-                psi_q = pr[4]
-                nn = qdot(psi_q, psi_q)
-                e32, e64 = qdot(psi_q, o) / nn, qdot(psi_q, r) / nn
-                errs.append(abs(e32 - e64) / max(abs(e64), 1e-300))
-                errs_op.append(abs(e32 - e64) / max(np.sqrt(qdot(r, r) / nn), 1e-300))
-            res["fp32_energy_rel_err_max"] = max(errs)
-            res["fp32_energy_rel_err_median"] = float(np.median(errs))
-            res["fp32_energy_err_operator_scale_max"] = max(errs_op)
         b.free()
     fl = res["flops_per_batch"]
     for k in list(res):
@@ -164,192 +366,102 @@ def batched_leg(n, reps):
     return res
 
 
-def qc_leg(norb, M, reps):
-    """Merged-form sigma (src/compute_sigma_vector.hpp:86-96) at the middle of a QC chain, FP64 / FP32 / split-BF16."""
-    Lm, Rm, psim, info = workloads.qc_merged_problem(norb, M, 1)
-    res = dict(info)
-    ref = None
-    for prec, name in ((mx.FP64, "fp64"), (mx.FP32, "fp32"), (mx.BF16X3, "bf16x3")):
-        out = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=prec)  # converts every input once
-        mx.synchronize()
-        one = out.to_qst()
-        if ref is None:
-            ref = one
-        ent, pf = last_plan_profile(3)
-        roof, _ = roofline_of(ent, prec if prec != mx.BF16X3 else mx.FP32)
-        # operators resident (FP32 copies kept): only the wavefunction is refreshed per call, as inside Davidson
-        mask = 1 << 2
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            mx.compute_sigma_vector_merged_ex(Lm, Rm, psim, out, prec, mask)
-        mx.synchronize()
-        sec = (time.perf_counter() - t0) / reps
-        res[name] = dict(ms=sec * 1e3, tflops=pf / sec / 1e12, rel_diff_vs_fp64=rel_diff(one, ref), roofline=roof)
-        res["flops_per_sigma"] = pf
-        del out
-    res["speedup_fp32_vs_fp64"] = res["fp64"]["ms"] / res["fp32"]["ms"]
-    res["speedup_bf16x3_vs_fp64"] = res["fp64"]["ms"] / res["bf16x3"]["ms"]
-    return res
-
-
-def sweep_leg(L, schedule, precisions):
-    """Device-resident two-site sweeps of the open Hubbard chain (t=1, U=4, half filling, 2Sz=0) from the seeded
-    random initial MPS over the M schedule; per precision: total and per-sweep seconds, final energy, and for the
-    mixed-precision runs |E - E_fp64|."""
-    from mpsxx_amd.dmrg import DeviceDMRG, initial_mps
-    from mpsxx_amd.mpo import hubbard_mpo
-    mpos = hubbard_mpo(L, 1.0, 4.0)
-    init = initial_mps(L, (L, 0), schedule[0], 0)
-    out = {"L": L, "schedule": schedule, "tol": 1e-10}
-    e64 = None
-    for prec in precisions:
-        name = {mx.FP64: "fp64", mx.FP32: "fp32_mixed", mx.BF16X3: "bf16x3_mixed"}[prec]
-        print("sweep leg: %s" % name, file=sys.stderr, flush=True)
-        dm = DeviceDMRG(mpos, init, schedule[0])
-        secs, e, st = [], None, None
-        t0 = time.perf_counter()
-        for M in schedule:
-            e, st = dm.sweep(M, 1e-10, precision=prec)
-            secs.append(round(st["seconds"], 3))
-        total = time.perf_counter() - t0
-        r = {"seconds_total": total, "seconds_per_sweep": secs, "energy": e, "last_sweep_svd_s": st["svd_s"],
-             "last_sweep_davidson_s": st["davidson_s"], "last_sweep_sigma_ms": st["sigma_ms"]}
-        if prec == mx.FP64:
-            e64 = e
-        elif e64 is not None:
-            r["abs_dE_vs_fp64"] = abs(e - e64)
-        out[name] = r
-    return out
-
-
-def qc_sweep_leg(norb, schedule, precisions, max_final=6):
-    """Merged two-site sweeps (src/make_sweep.cpp:112) of the QC chain with synthetic integrals from seed 1 over the
-    M schedule, the last M repeated until the sweep energy changes by less than the tolerance (src/dmrg.cpp:44-78,
-    at most max_final sweeps at that M): per precision the total and per-sweep seconds, the converged energy and
-    |E - E_fp64|."""
-    from mpsxx_amd import qc
-    from mpsxx_amd.dmrg import SWEEP_MERGED, DeviceDMRG, initial_mps
-    h, v = qc.synthetic_integrals(norb, seed=1)
-    m = qc.QcMpo(h, v)
-    mpos = m.devices()
-    out = {"norb": norb, "schedule": schedule, "tol": 1e-10, "D_w_max": max(m.bond_dims())}
-    e64 = None
-    for prec in precisions:
-        name = {mx.FP64: "fp64", mx.FP32: "fp32_mixed", mx.BF16X3: "bf16x3_mixed"}[prec]
-        print("qc sweep leg: %s" % name, file=sys.stderr, flush=True)
-        dm = DeviceDMRG(mpos, initial_mps(norb, (norb, 0), schedule[0], 0), schedule[0], form=SWEEP_MERGED)
-        secs, e = [], None
-        t0 = time.perf_counter()
-        for M in schedule[:-1]:
-            e, st = dm.sweep(M, 1e-10, precision=prec)
-            secs.append(round(st["seconds"], 3))
-        for k in range(max_final):
-            e_prev = e
-            e, st = dm.sweep(schedule[-1], 1e-10, precision=prec)
-            secs.append(round(st["seconds"], 3))
-            if k > 0 and abs(e - e_prev) < 1e-10:
-                break
-        r = {"seconds_total": time.perf_counter() - t0, "seconds_per_sweep": secs, "energy": e,
-             "last_sweep_svd_s": st["svd_s"], "last_sweep_davidson_s": st["davidson_s"],
-             "last_sweep_renorm_s": st["renorm_s"]}
-        if prec == mx.FP64:
-            e64 = e
-        elif e64 is not None:
-            r["abs_dE_vs_fp64"] = abs(e - e64)
-        out[name] = r
-        del dm
-    return out
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--L", type=int, default=32)
-    ap.add_argument("--M", type=int, default=512)
-    ap.add_argument("--U", type=float, default=4.0)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--batch", type=int, default=64, help="instances of the batched leg (0: skip it)")
-    ap.add_argument("--qc-norb", type=int, default=24, help="orbitals of the quantum-chemistry leg (0: skip it)")
+    ap.add_argument("--steps", type=int, default=30, help="timed sigma applications of the headline workload")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", choices=["fp32", "bf16x3", "fp64"], default="fp32",
+                    help="block-GEMM precision of the headline sigma")
+    ap.add_argument("--qc-norb", type=int, default=24)
     ap.add_argument("--qc-M", type=int, default=1024)
-    ap.add_argument("--sweeps", type=int, default=1, help="run the sweep leg (L=32, M 128x3, 256x3, 512x2; 0: skip)")
-    ap.add_argument("--qc-sweeps", type=int, default=1,
-                    help="run the QC merged-sweep leg (norb --qc-norb, M 256, 512, then 1024 until converged; 0: skip)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target one-core seconds of the CPU sample")
+    ap.add_argument("--qc-sweeps", type=int, default=1, help="QC merged sweeps, M 256, 512, 1024.. (0: skip)")
+    ap.add_argument("--snapshot", type=int, default=1, help="sigma on the converged FP64 sweep state (0: skip)")
+    ap.add_argument("--hubbard", type=int, default=1, help="Hubbard L=32 M=512 two-site sigma leg (0: skip)")
+    ap.add_argument("--batch", type=int, default=64, help="instances of the batched leg (0: skip)")
+    ap.add_argument("--sweeps", type=int, default=1, help="Hubbard L=32 sweeps, M 128x3, 256x3, 512x2 (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py measures one MI355X: the sigma-vector path runs on a single GPU in this build "
-              "(multi-GPU sharding over operator groups is design-only); run with --gpus 1", file=sys.stderr)
+              "(sharding over operator groups is design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    prec = mx.FP64 if args.precision == "fp64" else mx.FP32
-    lm, rm, le, re_, psi = workloads.hubbard_twosite(args.L, args.M, args.U)
-    dl = [mx.DTensor(x) for x in (lm, rm, le, re_, psi)]
-    print("headline sigma: L=%d M=%d %s" % (args.L, args.M, args.precision), file=sys.stderr, flush=True)
-    flops, nbytes = sigma_stats(dl, prec)
-    sec = time_sigma(dl, prec, args.steps, args.warmup)
-    tflops = flops / sec / 1e12
-    entries = sigma_twosite_profile(*dl, precision=prec, reps=50)
-    roof, per_kernel = roofline_of(entries, prec)
-    if prec == mx.FP64 and args.L == 32 and args.M == 512 and args.U == 4.0:
-        # HBM bytes per launch from the rocprofv3 --pmc runs of this workload (profiles/), when committed
-        roof["traffic"] = measured_traffic(roof["kernel"])
+    prec = {"fp32": mx.FP32, "bf16x3": mx.BF16X3, "fp64": mx.FP64}[args.precision]
+    print("headline: merged sigma norb=%d M=%d" % (args.qc_norb, args.qc_M), file=sys.stderr, flush=True)
+    Lm, Rm, psim, info = workloads.qc_merged_problem(args.qc_norb, args.qc_M, 1)
+    beside = {}
+    r64, ref, _, _ = merged_leg(Lm, Rm, psim, mx.FP64, args.steps, args.warmup)
+    beside["fp64"] = r64
+    head = None
+    for p in (mx.FP32, mx.BF16X3):
+        r, _, sec, pf = merged_leg(Lm, Rm, psim, p, args.steps, args.warmup, ref)
+        r["speedup_vs_fp64"] = r64["ms"] / r["ms"]
+        beside[PNAME[p]] = r
+        if p == prec:
+            head = (r, sec, pf)
+    if prec == mx.FP64:
+        head = (r64, r64["ms"] * 1e-3, r64["flops_per_sigma"])
+    hr, sec, flops = head
+    roof = dict(hr["roofline"])
+    if args.qc_norb == 24 and args.qc_M == 1024:
+        roof["traffic"] = measured_traffic(HEADLINE_TRAFFIC, "%s:%s" % (PNAME[prec], roof["kernel"]))
         if roof["traffic"] is not None:
-            roof["traffic_source"] = "profiles/traffic_sigma_twosite_hubbard_L32_M512.md (rocprofv3 --pmc FETCH_SIZE, " \
-                                     "WRITE_SIZE; copy-kernel calibration 2.0 and 1.0)"
+            roof["traffic_source"] = "profiles/%s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, copy-kernel " \
+                                     "calibration)" % HEADLINE_TRAFFIC
 
-    # the other precision on the same inputs
-    other = mx.FP32 if prec == mx.FP64 else mx.FP64
-    sec_o = time_sigma(dl, other, args.steps, args.warmup)
-    one = mx.compute_sigma_vector_twosite(*dl, precision=prec).to_qst()
-    one_o = mx.compute_sigma_vector_twosite(*dl, precision=other).to_qst()
-    other_leg = dict(precision="fp32" if other == mx.FP32 else "fp64", ms_per_sigma=sec_o * 1e3,
-                     tflops=flops / sec_o / 1e12, rel_diff_vs_headline=rel_diff(one_o, one))
+    cpu, parity = None, None
+    if not args.no_cpu:
+        print("cpu baseline", file=sys.stderr, flush=True)
+        cpu, parity = cpu_leg(Lm, Rm, psim, flops, args.cpu_seconds)
+    del Lm, Rm, psim
 
+    qc_sweeps, snapshot = None, None
+    if args.qc_sweeps > 0:
+        precs = [mx.FP64, mx.FP32, mx.BF16X3]
+        qc_sweeps, state = qc_sweep_leg(args.qc_norb, [256, 512, args.qc_M], precs,
+                                        keep_fp64_state=args.snapshot > 0)
+        if state is not None:
+            print("snapshot sigma", file=sys.stderr, flush=True)
+            mpos, sites = state
+            Ls, Rs, ps, sinfo = workloads.merged_problem_from_state(mpos, sites, args.qc_norb // 2 - 1)
+            del sites
+            snapshot = dict(sinfo, state="converged FP64 merged-sweep state at M=%d (site tensors as the last sweep "
+                                         "left them)" % args.qc_M)
+            s64, sref, _, _ = merged_leg(Ls, Rs, ps, mx.FP64, 10, 2)
+            snapshot["fp64"] = s64
+            for p in (mx.FP32, mx.BF16X3):
+                r, _, _, _ = merged_leg(Ls, Rs, ps, p, 10, 2, sref)
+                r["speedup_vs_fp64"] = s64["ms"] / r["ms"]
+                snapshot[PNAME[p]] = r
+            del Ls, Rs, ps
+
+    hub = None
+    if args.hubbard > 0:
+        print("hubbard sigma", file=sys.stderr, flush=True)
+        hub = hubbard_sigma_leg(32, 512, 4.0, 200, 20)
     batched = None
     if args.batch > 0:
         print("batched leg: %d instances" % args.batch, file=sys.stderr, flush=True)
         batched = batched_leg(args.batch, 20)
-
-    qcres = None
-    if args.qc_norb > 0:
-        print("quantum-chemistry leg: norb=%d M=%d" % (args.qc_norb, args.qc_M), file=sys.stderr, flush=True)
-        qcres = qc_leg(args.qc_norb, args.qc_M, 5)
-
-    sweeps = None
+    hsweeps = None
     if args.sweeps > 0:
-        sweeps = sweep_leg(32, [128] * 3 + [256] * 3 + [512] * 2, [mx.FP64, mx.FP32])
+        hsweeps = hubbard_sweep_leg(32, [128] * 3 + [256] * 3 + [512] * 2, [mx.FP64, mx.FP32])
 
-    qc_sweeps = None
-    if args.qc_sweeps > 0 and args.qc_norb > 0:
-        qc_sweeps = qc_sweep_leg(args.qc_norb, [256, 512, 1024], [mx.FP64, mx.FP32])
-
-    cpu = None
-    parity = None
-    if not args.no_cpu:
-        print("cpu baseline", file=sys.stderr, flush=True)
-        import oracle
-        oracle.set_threads(1)
-        reps = 0
-        t0 = time.perf_counter()
-        while True:
-            ref = oracle.sigma_twosite(lm, rm, le, re_, psi)
-            reps += 1
-            if time.perf_counter() - t0 > args.cpu_seconds or reps >= 200:
-                break
-        tc = (time.perf_counter() - t0) / reps
-        cpu = dict(value=flops / tc / 1e12, unit="TFLOP/s", cores=1, kind="port",
-                   sample="%d sigma applications of the headline workload (%.3f s each)" % (reps, tc),
-                   sigma_per_s=1.0 / tc)
-        scale = max(float(np.max(np.abs(b))) for b in ref.blocks.values())
-        parity = dict(same_blocks=one.block_set() == ref.block_set(), max_abs_diff=one.max_abs_diff(ref) / scale)
+    sweep_time = None
+    if qc_sweeps is not None:
+        mixed = "fp32_mixed" if prec != mx.BF16X3 else "bf16x3_mixed"
+        sweep_time = dict(workload="qc merged sweeps norb=%d, M 256, 512, %d until |dE| < 1e-10" % (
+            args.qc_norb, args.qc_M), fp64_s=qc_sweeps["fp64"]["seconds_total"],
+            mixed=mixed, mixed_s=qc_sweeps[mixed]["seconds_total"],
+            speedup=qc_sweeps[mixed]["speedup_vs_fp64"], abs_dE=qc_sweeps[mixed]["abs_dE_vs_fp64"],
+            abs_dE_ok=qc_sweeps[mixed]["abs_dE_vs_fp64"] <= 1e-8)
 
     res = {
-        "metric": "H_psi TFLOP/s (two-site sigma vector, block GEMMs on MFMA)",
-        "value": tflops,
+        "metric": METRIC,
+        "value": hr["tflops"],
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -358,23 +470,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == mx.FP64 else "f32",
+        "dtype": {mx.FP32: "f32", mx.BF16X3: "bf16x3", mx.FP64: "f64"}[prec],
         "data": "synthetic",
-        "config": {"workload": "hubbard_sigma_twosite", "L": args.L, "M": args.M, "U": args.U, "t": 1.0,
-                   "N": args.L, "twoSz": 0, "mpo": "compact nearest-neighbour (D_w=6)",
-                   "psi_blocks": len(psi.blocks), "psi_elems": psi.nelem(), "precision": args.precision},
+        "config": {"workload": "qc_sigma_merged", "norb": args.qc_norb, "N": args.qc_norb, "twoSz": 0,
+                   "M": args.qc_M, "integrals": "synthetic, seed 1", "mpo": "QC generator, D_w=%d" % info["D_w"],
+                   "site": info["site"], "psi_blocks": info["psi_blocks"], "psi_elems": info["psi_elems"],
+                   "env_elems": info["env_elems"], "precision": args.precision,
+                   "operators": "resident (FP32 / split copies kept), wavefunction refreshed per call"},
         "sigma_per_s": 1.0 / sec,
         "flops_per_sigma": flops,
-        "bytes_per_sigma": nbytes,
-        "kernels": per_kernel,
         "roofline": roof,
         "cpu_baseline": cpu,
         "parity": parity,
-        "other_precision": other_leg,
-        "batched": batched,
-        "qc_merged": qcres,
-        "sweeps": sweeps,
+        "sweep_time": sweep_time,
+        "headline_kernels": hr["kernels"],
+        "precisions": beside,
+        "snapshot": snapshot,
         "qc_sweeps": qc_sweeps,
+        "hubbard_sigma": hub,
+        "batched": batched,
+        "hubbard_sweeps": hsweeps,
     }
     print(json.dumps(res))
 

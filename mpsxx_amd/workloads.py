@@ -187,3 +187,27 @@ def qc_merged_problem(norb=24, M=1024, seed=1, width=1.5):
                 left_sectors=len(lq), right_sectors=len(rq), psi_blocks=len(psi.blocks), psi_elems=psi.nelem(),
                 env_elems=lenv.nelem() + renv.nelem())
     return Lm, Rm, psim, info
+
+
+def merged_problem_from_state(mpos, sites, i):
+    """Device tensors (Lm, Rm, psim, info) of the merged two-site sigma at sites (i, i+1) of an MPS held on the device
+    (`sites` = DTensors A(l, n, r), e.g. the site tensors of a converged sweep): the left environment renormalized
+    over sites 0..i-1 and the right one over sites L-1..i+2 (src/renormalize.hpp:10-32), the merged operators of
+    sites i and i+1 (src/optimize.hpp:273-368) and the two-site wavefunction A_i A_{i+1} merged to (l n, n r)."""
+    from . import DTensor, compute_merged_operator, contract, qst_merge, renormalize
+    L = len(sites)
+    lenv = DTensor(mps.boundary_env())
+    for k in range(i):
+        lenv = renormalize(True, mpos[k], lenv, sites[k], sites[k])
+    renv = DTensor(mps.boundary_env())
+    for k in range(L - 1, i + 1, -1):
+        renv = renormalize(False, mpos[k], renv, sites[k], sites[k])
+    psi = contract(1.0, sites[i], [2], sites[i + 1], [0])
+    Lm = compute_merged_operator(True, mpos[i], lenv)
+    Rm = compute_merged_operator(False, mpos[i + 1], renv)
+    psim = qst_merge(psi, 2, 2)
+    ps = psi.to_qst()
+    info = dict(site=i, psi_blocks=len(ps.blocks), psi_elems=ps.nelem(),
+                left_sectors=len(ps.qshape[0]), right_sectors=len(ps.qshape[3]),
+                bond_left=int(sum(ps.dshape[0])), bond_right=int(sum(ps.dshape[3])))
+    return Lm, Rm, psim, info

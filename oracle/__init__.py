@@ -40,6 +40,37 @@ def set_threads(n):
     lib().orc_set_threads(ctypes.c_int(int(n)))
 
 
+_blas = None
+
+
+def openblas():
+    """(library, version string) of numpy's bundled ILP64 OpenBLAS (scipy-openblas), loaded once."""
+    global _blas
+    if _blas is None:
+        import glob
+        libs = os.path.join(os.path.dirname(os.path.dirname(np.__file__)), "numpy.libs")
+        cand = sorted(glob.glob(os.path.join(libs, "libscipy_openblas64_*.so")))
+        if not cand:
+            raise RuntimeError("oracle: numpy's bundled OpenBLAS not found under " + libs)
+        b = ctypes.CDLL(cand[0])
+        b.scipy_openblas_get_config64_.restype = ctypes.c_char_p
+        _blas = (b, b.scipy_openblas_get_config64_().decode())
+    return _blas
+
+
+def use_blas(enable=True, threads=1):
+    """Route the oracle's block dgemm (C += A B per block pair) through numpy's bundled OpenBLAS cblas_dgemm
+    (64-bit integers) with `threads` BLAS threads, as the reference calls CBLAS dgemm per block pair; False restores
+    the oracle's own loops.  For CPU timings: the values agree with the loops to rounding (tests/test_oracle_blas.py)."""
+    if not enable:
+        lib().orc_set_dgemm(None)
+        return None
+    b, cfg = openblas()
+    b.scipy_openblas_set_num_threads64_(ctypes.c_int(int(threads)))
+    lib().orc_set_dgemm(ctypes.cast(b.scipy_cblas_dgemm64_, VP))
+    return cfg
+
+
 def _check(rc):
     if rc != 0:
         raise RuntimeError("oracle: " + lib().orc_last_error().decode())

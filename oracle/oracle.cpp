@@ -153,8 +153,20 @@ QST permute(const QST& a, const std::vector<int>& perm) {
   return c;
 }
 
+// Optional CBLAS dgemm with 64-bit integers (an ILP64 BLAS such as numpy's bundled OpenBLAS), set by
+// orc_set_dgemm for CPU timings: the reference's contractions call CBLAS dgemm per block pair (BTAS
+// DENSE/BLAS_STL_vector.h through STBLAS.h:298-359).  Unset (the default) the loops below run.
+typedef void (*cblas_dgemm64_fn)(int order, int transa, int transb, int64_t m, int64_t n, int64_t k, double alpha,
+                                 const double* a, int64_t lda, const double* b, int64_t ldb, double beta, double* c,
+                                 int64_t ldc);
+cblas_dgemm64_fn g_cblas_dgemm = nullptr;
+
 // C(m x n) += alpha * A(m x k) * B(k x n), row-major
 void dgemm(long m, long n, long k, double alpha, const double* A, const double* B, double* C) {
+  if (g_cblas_dgemm) {
+    g_cblas_dgemm(101 /* CblasRowMajor */, 111 /* CblasNoTrans */, 111, m, n, k, alpha, A, k, B, n, 1.0, C, n);
+    return;
+  }
   for (long i = 0; i < m; ++i) {
     double* c = C + i * n;
     for (long p = 0; p < k; ++p) {
@@ -1129,6 +1141,7 @@ extern "C" {
 const char* orc_last_error(void) { return g_err.c_str(); }
 
 void orc_set_threads(int n) { orc::g_threads = n > 0 ? n : 1; }
+void orc_set_dgemm(void* fn) { orc::g_cblas_dgemm = reinterpret_cast<orc::cblas_dgemm64_fn>(fn); }
 
 orc_tensor* orc_create(const mpsxx_qst_desc* d, const double* data) {
   auto* t = new orc_tensor;
