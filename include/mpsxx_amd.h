@@ -312,7 +312,7 @@ int mpsxx_sigma_twosite_stats(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor
 /* Times every kernel launch of the two-site sigma plan separately (HIP events on the launch stream around each launch,
  * mean over `reps`, scratch output) and returns per launch: device ms, its share of the algorithmic flops and bytes,
  * the kernel (0 gemm_grouped_kernel 64x64, 1 gemm_small_kernel, 2 gemm_split_kernel, 3 axpy_grouped_kernel,
- * 4 gemm_x3_kernel, 5 gemm128_kernel 128x128) and the plan step it belongs to. */
+ * 4 gemm_x3_kernel, 5 gemm128_kernel 128x128, 6 gemm_x3_128_kernel) and the plan step it belongs to. */
 int mpsxx_sigma_twosite_profile(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
                                 mpsxx_tensor wfn0, int precision, int reps, int32_t max_entries, double* ms,
                                 int64_t* flops, int64_t* bytes, int32_t* kernel, int32_t* step, int32_t* nentries,

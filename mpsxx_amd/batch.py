@@ -72,7 +72,7 @@ class SigmaBatch:
 
 
 KERNELS = ["gemm_grouped_kernel", "gemm_small_kernel", "gemm_split_kernel", "axpy_grouped_kernel", "gemm_x3_kernel",
-           "gemm128_kernel"]
+           "gemm128_kernel", "gemm_x3_128_kernel"]
 
 
 def _profile_entries(call, cap=256):

@@ -349,6 +349,7 @@ std::vector<PartTiming> BatchSigma::profile(int reps, hipStream_t s) {
     for (KernelPart kp : step_parts(launches_[k].st)) {
       const double ms = time_launches([&] { launch_one(k, s, kp.mask); }, reps, s);
       if (prec_ == MPSXX_BF16X3 && kp.kind == KernelPart::GEMM64) kp.kind = KernelPart::GEMM_X3;
+      if (prec_ == MPSXX_BF16X3 && kp.kind == KernelPart::GEMM128) kp.kind = KernelPart::GEMM_X3_128;
       res.push_back({k, kp, ms});
     }
   hip_check(hipStreamSynchronize(s), "profile sync");

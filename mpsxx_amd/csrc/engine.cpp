@@ -902,6 +902,7 @@ GemmPolicy default_policy() {
     GemmPolicy q;
     if (const char* e = std::getenv("MPSXX_SMALL_GEMM_LIMIT")) q.small_limit = std::atoi(e);
     if (const char* e = std::getenv("MPSXX_SPLIT_UNITS")) q.split_units = std::atoi(e);
+    if (const char* e = std::getenv("MPSXX_SPLIT_BIG")) q.split_big = std::max(0, std::atoi(e));
     if (const char* e = std::getenv("MPSXX_SPLIT_SMALL")) q.split_small = std::atoi(e);
     if (const char* e = std::getenv("MPSXX_SPLIT_SEG")) q.split_seg = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("MPSXX_BIG_TILE_MIN")) q.big_tile_min = std::max(0, std::atoi(e));
@@ -911,7 +912,7 @@ GemmPolicy default_policy() {
 }
 
 GemmPolicy policy_for(GemmPolicy p, int precision) {
-  if (precision == MPSXX_BF16X3) p.big_tile_min = 0;
+  (void)precision;  // every precision has 128x128 tiles
   return p;
 }
 
@@ -953,7 +954,7 @@ void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>&
     if (pol.big_tile_min <= 0 || t.m < pol.big_tile_min || t.n < pol.big_tile_min) return kGemmTile;
     for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) {
       const GemmPair& g = pairs[p];
-      const int64_t pad_m = t.m + kGemmTile2, pad_n = t.n + kGemmTile2, pad_k = g.k + 16;
+      const int64_t pad_m = t.m + kGemmTile2, pad_n = t.n + kGemmTile2, pad_k = g.k + 32;
       if (pad_m * g.sa_m + pad_k * g.sa_k >= INT32_MAX || pad_k * g.sb_k + pad_n * g.sb_n >= INT32_MAX)
         return kGemmTile;
     }
@@ -1181,7 +1182,11 @@ void launch_step_t(const Step& st, const Operands& ops, unsigned parts, bool x3,
       launch_gemm_split<T, TC>(st.work_split.as<BigTile>(), st.nwork_split, st.work_split_red.as<RedTile>(),
                                st.nwork_split_red, st.split_partial.p, st.tasks.as<GemmTask>(),
                                st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
-    if (parts & 8u)
+    if ((parts & 8u) && x3)
+      launch_gemm_x3_128<TC>(st.work2.as<BigTile>(), st.nwork2, st.work_red2.as<RedTile>(), st.nwork_red2,
+                             st.partial2.p, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha,
+                             st.accumulate, s, st.max_groups);
+    else if (parts & 8u)
       launch_gemm128<T, TC>(st.work2.as<BigTile>(), st.nwork2, st.work_red2.as<RedTile>(), st.nwork_red2,
                             st.partial2.p, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha,
                             st.accumulate, s, st.max_groups);
@@ -1275,6 +1280,7 @@ const char* part_kernel_name(int kind) {
     case KernelPart::GEMM_SPLIT: return "gemm_split_kernel";
     case KernelPart::AXPY: return "axpy_grouped_kernel";
     case KernelPart::GEMM128: return "gemm128_kernel";
+    case KernelPart::GEMM_X3_128: return "gemm_x3_128_kernel";
     default: return "copy_kernel";
   }
 }
@@ -1403,6 +1409,7 @@ std::vector<PartTiming> profile_plan_parts(Plan& P, const std::vector<const doub
     for (KernelPart kp : step_parts(st)) {
       const double ms = time_launches([&] { launch_step(st, ops, f32, cf32, s, kp.mask, x3); }, reps, s);
       if (x3 && kp.kind == KernelPart::GEMM64) kp.kind = KernelPart::GEMM_X3;
+      if (x3 && kp.kind == KernelPart::GEMM128) kp.kind = KernelPart::GEMM_X3_128;
       res.push_back({(int)si, kp, ms});
     }
   }

@@ -135,16 +135,19 @@ struct GemmPolicy {
   int small_limit = 64;
   int split_units = 2;
   int max_groups = 0;
-  int split_big = 2048;
+  // in 64x64-tile equivalents (a 128x128 tile counts four); measured on the norb=24 M=1024 merged sigma, whose
+  // second step holds a few hundred long-K tiles: 2048 -> 16384 takes FP64 from 32.9 to 29.1 ms, FP32 from 19.7 to
+  // 17.3 ms and split-BF16x3 from 14.5 to 13.7 ms, with the L=32 M=512 Hubbard sigma unchanged
+  int split_big = 16384;
   int split_small = 1024;
   int split_seg = 32;
   int big_tile_min = 96;  // blocks with both sides >= this run as 128x128 tiles (gemm128_kernel); 0: never
 };
-// the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT, MPSXX_SPLIT_UNITS, MPSXX_SPLIT_SMALL, MPSXX_SPLIT_SEG,
-// MPSXX_BIG_TILE_MIN
+// the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT, MPSXX_SPLIT_UNITS, MPSXX_SPLIT_BIG, MPSXX_SPLIT_SMALL,
+// MPSXX_SPLIT_SEG, MPSXX_BIG_TILE_MIN
 GemmPolicy default_policy();
 // the defaults for the block GEMMs of one precision (MPSXX_FP64 / MPSXX_FP32 / MPSXX_BF16X3): 128x128 tiles for FP32
-// only (MPSXX_BIG_TILE_MIN_F64 enables them for FP64; the split-BF16 kernel has 64x64 tiles only)
+// and split BF16 only (MPSXX_BIG_TILE_MIN_F64 enables them for FP64)
 GemmPolicy default_policy(int precision);
 // `p` with the tile settings `precision` has kernels for
 GemmPolicy policy_for(GemmPolicy p, int precision);
@@ -195,7 +198,7 @@ struct Step {
 // One kernel launch of a step, with its share of the step's algorithmic cost: flops 2*rows*cols*k summed over the
 // tiles of the launch (GEMM) or 2*len per pair (AXPY); bytes = 8 * (operand elements read once per tile + outputs).
 struct KernelPart {
-  enum Kind { GEMM64 = 0, GEMM_SMALL = 1, GEMM_SPLIT = 2, AXPY = 3, GEMM_X3 = 4, GEMM128 = 5 };
+  enum Kind { GEMM64 = 0, GEMM_SMALL = 1, GEMM_SPLIT = 2, AXPY = 3, GEMM_X3 = 4, GEMM128 = 5, GEMM_X3_128 = 6 };
   int kind;
   unsigned mask;  // `parts` bit of launch_step
   int items;

@@ -111,6 +111,11 @@ template <typename TC>
 void launch_gemm_x3(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
                     const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
                     hipStream_t s, int max_groups = 0);
+// The 128x128 tiles of launch_gemm128 as split-BF16 products (gemm_x3_128.hpp); FP32 partial slots of 128x128
+template <typename TC>
+void launch_gemm_x3_128(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
+                        const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
+                        hipStream_t s, int max_groups = 0);
 
 // 32x32 tiles, one wave each, operands loaded straight into the MFMA registers (blocks up to ~64 on a side).
 constexpr int kSmallTile = 32;
