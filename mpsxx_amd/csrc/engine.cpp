@@ -739,9 +739,14 @@ ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const 
       task.pair_begin = (int32_t)(pair_base[c] + lo);
       task.pair_count = (int32_t)(hi - lo);
       task.inst = 0;
-      task.pad = 0;
       task.m = (int32_t)A[ch.src[2 * t]].m;
       task.n = (int32_t)Bs[ch.src[2 * t + 1]].n;
+      task.layout = kTaskContigA | kTaskContigB;
+      for (int32_t q = task.pair_begin; q < task.pair_begin + task.pair_count; ++q) {
+        const GemmPair& g = R.pairs[q];
+        if (g.sa_k != 1) task.layout &= ~kTaskContigA;
+        if (g.sb_k != 1) task.layout &= ~kTaskContigB;
+      }
     }
   });
   return R;
@@ -948,15 +953,17 @@ void gemm_tiles(const std::vector<GemmTask>& tasks, const std::vector<GemmPair>&
     cost[ti] = ksum;
   }
   std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
-  // tile edge of a large block: 128 when both sides reach big_tile_min and every in-block offset of its pairs fits
-  // the 32-bit offsets of gemm128_kernel, else 64
+  // tile edge of a large block: 128 when both sides reach big_tile_min and every in-block offset of its pairs, in
+  // elements and in bytes, fits the 32-bit offsets of gemm128_kernel and the buffer loads of gemm_x3_128_kernel
+  // (below 2^28 elements), else 64
   auto edge_of = [&](const GemmTask& t) {
     if (pol.big_tile_min <= 0 || t.m < pol.big_tile_min || t.n < pol.big_tile_min) return kGemmTile;
+    constexpr int64_t kMaxOff = int64_t(1) << 28;
     for (int p = t.pair_begin; p < t.pair_begin + t.pair_count; ++p) {
       const GemmPair& g = pairs[p];
       const int64_t pad_m = t.m + kGemmTile2, pad_n = t.n + kGemmTile2, pad_k = g.k + 32;
-      if (pad_m * g.sa_m + pad_k * g.sa_k >= INT32_MAX || pad_k * g.sb_k + pad_n * g.sb_n >= INT32_MAX)
-        return kGemmTile;
+      if (g.sa_m < 0 || g.sa_k < 0 || g.sb_k < 0 || g.sb_n < 0) return kGemmTile;
+      if (pad_m * g.sa_m + pad_k * g.sa_k >= kMaxOff || pad_k * g.sb_k + pad_n * g.sb_n >= kMaxOff) return kGemmTile;
     }
     return kGemmTile2;
   };
@@ -1077,6 +1084,19 @@ void set_gemm_policy(Step& st, const GemmPolicy& pol) {
   st.work_split = upload_vec(g.split);
   st.nwork_split_red = (int)g.split_red.size();
   st.work_split_red = upload_vec(g.split_red);
+  // the 128x128 tiles grouped by their task's operand layouts (one split-BF16 kernel per layout), order kept within
+  {
+    const int32_t order[4] = {kTaskContigA | kTaskContigB, kTaskContigA, kTaskContigB, 0};
+    std::vector<BigTile> grouped;
+    grouped.reserve(g.big2.size());
+    for (int q = 0; q < 4; ++q) {
+      const size_t before = grouped.size();
+      for (const BigTile& tl : g.big2)
+        if ((st.host.gtasks[tl.task].layout & (kTaskContigA | kTaskContigB)) == order[q]) grouped.push_back(tl);
+      st.nwork2_layout[q] = (int)(grouped.size() - before);
+    }
+    g.big2.swap(grouped);
+  }
   st.nwork2 = (int)g.big2.size();
   st.work2 = upload_vec(g.big2);
   st.nwork_red2 = (int)g.red2.size();
@@ -1183,7 +1203,7 @@ void launch_step_t(const Step& st, const Operands& ops, unsigned parts, bool x3,
                                st.nwork_split_red, st.split_partial.p, st.tasks.as<GemmTask>(),
                                st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s, st.max_groups);
     if ((parts & 8u) && x3)
-      launch_gemm_x3_128<TC>(st.work2.as<BigTile>(), st.nwork2, st.work_red2.as<RedTile>(), st.nwork_red2,
+      launch_gemm_x3_128<TC>(st.work2.as<BigTile>(), st.nwork2_layout, st.work_red2.as<RedTile>(), st.nwork_red2,
                              st.partial2.p, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha,
                              st.accumulate, s, st.max_groups);
     else if (parts & 8u)

@@ -186,6 +186,7 @@ struct Step {
   int nwork_split_red = 0;     // GEMM: split 32x32 tiles cut over work-groups (RedTile) and their partial buffer
   DevBuf work_split_red, split_partial;
   int nwork2 = 0;              // GEMM: 128x128 tiles of the largest blocks (gemm128_kernel), their split reductions
+  int nwork2_layout[4] = {0, 0, 0, 0};  // of those, tiles of tasks with layout flags 3, 1, 2, 0, listed in that order
   DevBuf work2;
   int nwork_red2 = 0;
   DevBuf work_red2, partial2;

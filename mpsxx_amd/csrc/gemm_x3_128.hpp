@@ -4,19 +4,25 @@
 // Every FP32 operand element is split on its way into LDS as x = hi + lo, hi = bf16(x), lo = bf16(x - hi), and each
 // product is formed as lo*hi + hi*lo + hi*hi by three v_mfma_f32_16x16x32_bf16 with FP32 accumulation (the lo*lo term,
 // ~2^-16 relative, is dropped).  One 256-thread work-group per 128x128 C tile; 4 waves each own a 64x64 quarter as
-// 4x4 MFMA tiles, so per 32-wide k panel a wave reads 16 operand vectors (hi and lo of 4 A rows-blocks and 4 B
-// column blocks) for 48 MFMAs.
+// 4x4 MFMA tiles, so per 32-wide k panel a wave reads 16 operand vectors (hi and lo of 4 A row blocks and 4 B column
+// blocks) for 48 MFMAs.
 //
 // LDS holds each panel as [line][k] bf16 (line = row of A or column of B), 64 bytes per line in four 16-byte chunks:
 // MFMA lane l reads line l%16 of its sub-tile at k 8*(l/16)..+7 as one 16-byte vector.  Chunk c of line r is stored
-// at chunk c ^ ((r >> 2) & 3) (XOR swizzle), so that the 16 lanes of a read, 16 consecutive lines at one chunk, hit 16
-// distinct 16-byte bank groups without padding.  Two panel buffers (hi and lo of A and B, 64 KB) leave room for two
-// work-groups per CU; the next panel is loaded into registers while the current one is multiplied.
+// at chunk c ^ ((r >> 2) & 3) (XOR swizzle), so that the 16 lanes of a read, 16 consecutive lines at one chunk, spread
+// over the banks without padding.  Two panel buffers (hi and lo of A and B, 64 KB) leave room for two work-groups per
+// CU.
 //
-// Panel loads: each thread moves 16 elements of A and 16 of B.  For an operand whose unit stride runs along k, a
-// thread takes two runs of 8 consecutive k (lines t/4 and t/4 + 64, k 8*(t%4)..+7), read as two 4-wide vector loads
-// each; otherwise it takes 16 consecutive k of line t%128 (k half t/128), one element per load, 128 consecutive lines
-// per load instruction over the work-group.  Panels wholly inside the block skip the bounds checks.
+// Panels flow global -> registers -> LDS one panel ahead of the MFMAs: while the MFMAs run on panel i from one LDS
+// buffer, panel i+1 is loaded into registers and then stored into the other.  All loads are buffer loads against a
+// resource covering the operand block: an element outside the block gets an offset beyond the resource's extent and
+// reads as zero, so every panel (inside the block or on its edge, and the empty panel past the end of the tile) goes
+// through the same loads and no branch surrounds a load; the compiler can then count the loads in flight exactly.  The kernel is specialised per operand
+// layout (the host groups the tiles by their task's layout flags, all pairs of a task alike, one launch per group):
+//   unit stride along k: a thread takes two runs of 8 consecutive k (lines t/4 and t/4 + 64, k 8*(t%4)..+7) as four
+//     16-byte loads; k past the pair's end within a run is zeroed when the run is stored;
+//   otherwise: a thread takes 16 consecutive k of line t%128 (k half t/128), one element per load, 128 consecutive
+//     lines per load instruction over the work-group.
 #pragma once
 
 #include "gemm128.hpp"
@@ -24,69 +30,74 @@
 namespace mpx {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));  // 4-wide load at 4-byte alignment
+typedef float flt4v __attribute__((ext_vector_type(4)));
 
-constexpr int X2K = 32;  // k panel: one bf16 MFMA depth
+constexpr int X2K = 32;                      // k panel: one bf16 MFMA depth
+constexpr int32_t kOutOfBlock = 0x7fffffff;  // a byte offset beyond every block extent (the host keeps blocks < 2^30 B)
 
-struct X3View {
-  int32_t off0;  // offset of the thread's first element for k0 = 0
-  int32_t rstep; // offset of run 1 relative to run 0
-  int32_t s_k;   // offset step per unit of k
-  int32_t l0, lrun;  // line of run 0 relative to the tile's first line, line offset of run 1
-  int32_t kk0, krun; // k of run 0 in the panel, k offset of run 1
-  int32_t lim;       // lines of the block from the tile's first line on
-  bool contig;       // unit stride along k: vector loads
+// Operand element -> thread map of one layout.
+template <bool Contig>
+struct X3Map {
+  static __device__ __forceinline__ int l0(int tid) { return Contig ? tid >> 2 : tid & 127; }
+  static constexpr int lrun = Contig ? 64 : 0;
+  static __device__ __forceinline__ int kk0(int tid) { return Contig ? 8 * (tid & 3) : 16 * (tid >> 7); }
+  static constexpr int krun = Contig ? 0 : 8;
 };
 
-__device__ __forceinline__ void x3_view(X3View& v, bool contig, int tid, int line0, int nline, int64_t s_line,
-                                        int64_t s_k) {
-  v.contig = contig;
-  if (contig) {
-    v.l0 = tid >> 2;
-    v.lrun = 64;
-    v.kk0 = 8 * (tid & 3);
-    v.krun = 0;
-  } else {
-    v.l0 = tid & 127;
-    v.lrun = 0;
-    v.kk0 = 16 * (tid >> 7);
-    v.krun = 8;
-  }
-  v.off0 = (int32_t)((int64_t)(line0 + v.l0) * s_line + (int64_t)v.kk0 * s_k);
-  v.rstep = (int32_t)(v.lrun * s_line + v.krun * s_k);
-  v.s_k = (int32_t)s_k;
+// One operand of the cursor's pair: the buffer resource of its block and the thread's offsets.
+struct X3View {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int32_t boff0;  // byte offset of the thread's first element for k0 = 0
+  int32_t brun;   // byte offset of run 1 relative to run 0
+  int32_t bsk;    // bytes per unit of k
+  int32_t lim;    // lines of the block from the tile's first line on
+};
+
+// One register set: a panel's 16 elements per operand and, for a unit-stride-k operand, the k left per run.
+struct X3Set {
+  float a[16], b[16];
+  int kla[2], klb[2];
+};
+
+template <bool Contig>
+__device__ __forceinline__ void x3_view(X3View& v, int tid, const float* __restrict__ block, int line0, int nline,
+                                        int klen, int64_t s_line, int64_t s_k) {
+  typedef X3Map<Contig> M;
+  // elements spanned by the block (0 for an empty one: then every element reads as zero)
+  const int64_t extent = nline > 0 && klen > 0 ? (int64_t)(nline - 1) * s_line + (int64_t)(klen - 1) * s_k + 1 : 0;
+  v.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(block), (short)0, (int)(extent * 4), 0x00020000);
+  v.boff0 = (int32_t)(4 * ((int64_t)(line0 + M::l0(tid)) * s_line + (int64_t)M::kk0(tid) * s_k));
+  v.brun = (int32_t)(4 * (M::lrun * s_line + M::krun * s_k));
+  v.bsk = (int32_t)(4 * s_k);
   v.lim = nline - line0;
 }
 
-// the thread's 16 elements of the panel starting at k0 (zero outside the block)
-__device__ __forceinline__ void x3_panel_load(float (&r)[16], const float* __restrict__ P, const X3View& v, int k0,
-                                              int klen) {
-  const float* __restrict__ q = P + (v.off0 + k0 * v.s_k);
-  const bool full = v.lim >= B2M && k0 + X2K <= klen;  // uniform over the work-group
-  if (full && v.contig) {
+// The thread's elements of the panel at k0; valid = false (past the tile's last panel) reads zeros.  kl[run] = k of
+// the run inside the pair (unit-stride-k layout: elements j >= kl[run] are zeroed at the store).
+template <bool Contig>
+__device__ __forceinline__ void x3_load(float (&r)[16], int (&kl)[2], const X3View& v, int tid, int k0, int klen,
+                                        bool valid) {
+  typedef X3Map<Contig> M;
+  const int32_t base = v.boff0 + k0 * v.bsk;
 #pragma unroll
-    for (int run = 0; run < 2; ++run) {
-      const f32x4u x0 = *reinterpret_cast<const f32x4u*>(q + run * v.rstep);
-      const f32x4u x1 = *reinterpret_cast<const f32x4u*>(q + run * v.rstep + 4);
+  for (int run = 0; run < 2; ++run) {
+    const bool lok = valid && M::l0(tid) + run * M::lrun < v.lim;
+    const int kleft = klen - k0 - M::kk0(tid) - run * M::krun;
+    kl[run] = kleft;
+    const int32_t rb = base + run * v.brun;
+    if constexpr (Contig) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        r[8 * run + j] = x0[j];
-        r[8 * run + 4 + j] = x1[j];
+      for (int h = 0; h < 2; ++h) {
+        const int32_t off = (lok && 4 * h < kleft) ? rb + 16 * h : kOutOfBlock;
+        const flt4v x = __builtin_bit_cast(flt4v, __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, off, 0, 0));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r[8 * run + 4 * h + j] = x[j];
       }
-    }
-  } else if (full) {
-#pragma unroll
-    for (int run = 0; run < 2; ++run)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) r[8 * run + j] = q[run * v.rstep + j * v.s_k];
-  } else {
-#pragma unroll
-    for (int run = 0; run < 2; ++run) {
-      const bool lok = v.l0 + run * v.lrun < v.lim;
+    } else {
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const bool ok = lok && k0 + v.kk0 + run * v.krun + j < klen;
-        r[8 * run + j] = ok ? q[run * v.rstep + j * v.s_k] : 0.f;
+        const int32_t off = (lok && j < kleft) ? rb + j * v.bsk : kOutOfBlock;
+        r[8 * run + j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(v.rsrc, off, 0, 0));
       }
     }
   }
@@ -95,34 +106,110 @@ __device__ __forceinline__ void x3_panel_load(float (&r)[16], const float* __res
 __device__ __forceinline__ int x3_chunk(int line, int chunk) { return (chunk ^ ((line >> 2) & 3)) * 8; }
 
 // split the thread's elements and store them as two 16-byte vectors per run into the hi and lo panels
-__device__ __forceinline__ void x3_panel_store(__bf16 (*Hi)[X2K], __bf16 (*Lo)[X2K], const float (&r)[16],
-                                               const X3View& v) {
+template <bool Contig>
+__device__ __forceinline__ void x3_store(__bf16 (*Hi)[X2K], __bf16 (*Lo)[X2K], const float (&r)[16],
+                                         const int (&kl)[2], int tid) {
+  typedef X3Map<Contig> M;
 #pragma unroll
   for (int run = 0; run < 2; ++run) {
     bf16x8 h, l;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const float x = r[8 * run + j];
+      const float x = Contig ? (j < kl[run] ? r[8 * run + j] : 0.f) : r[8 * run + j];
       const __bf16 hb = (__bf16)x;
       h[j] = hb;
       l[j] = (__bf16)(x - (float)hb);
     }
-    const int line = v.l0 + run * v.lrun, c = x3_chunk(line, (v.kk0 + run * v.krun) >> 3);
+    const int line = M::l0(tid) + run * M::lrun, c = x3_chunk(line, (M::kk0(tid) + run * M::krun) >> 3);
     *reinterpret_cast<bf16x8*>(&Hi[line][c]) = h;
     *reinterpret_cast<bf16x8*>(&Lo[line][c]) = l;
   }
 }
 
-template <typename TC, bool Tab>
+struct X3Lds {
+  __bf16 Ah[2][B2M][X2K];
+  __bf16 Al[2][B2M][X2K];
+  __bf16 Bh[2][B2N][X2K];
+  __bf16 Bl[2][B2N][X2K];
+};
+
+// The panel pipeline of one tile into acc, for operand layouts CA, CB.
+template <bool CA, bool CB>
+__device__ __forceinline__ void x3_tile(flt4 (&acc)[4][4], X3Lds& S, const BigTile& tl, const GemmTask& tk,
+                                        const GemmPair* __restrict__ pairs, const float* __restrict__ A,
+                                        const float* __restrict__ B, int tid, int wr, int wc, int lr, int lk,
+                                        const bool (&live_r)[4], const bool (&live_c)[4]) {
+  // the load cursor (cp, ck0) walks the tile's (pair, k panel) sequence; cva, cvb view the cursor's pair
+  int cp = tl.pair_lo, ck0 = 0;
+  GemmPair cpr = pairs[cp];
+  X3View cva, cvb;
+  x3_view<CA>(cva, tid, A + cpr.a_off, tl.row0, tk.m, cpr.k, cpr.sa_m, cpr.sa_k);
+  x3_view<CB>(cvb, tid, B + cpr.b_off, tl.col0, tk.n, cpr.k, cpr.sb_n, cpr.sb_k);
+  bool cur = true;  // the cursor is on a panel of the tile (uniform)
+  auto load = [&](X3Set& s) {
+    x3_load<CA>(s.a, s.kla, cva, tid, ck0, cpr.k, cur);
+    x3_load<CB>(s.b, s.klb, cvb, tid, ck0, cpr.k, cur);
+    if (!cur) return;
+    ck0 += X2K;
+    if (ck0 < cpr.k) return;
+    ck0 = 0;
+    if (++cp >= tl.pair_hi) {
+      cur = false;
+      return;
+    }
+    cpr = pairs[cp];
+    x3_view<CA>(cva, tid, A + cpr.a_off, tl.row0, tk.m, cpr.k, cpr.sa_m, cpr.sa_k);
+    x3_view<CB>(cvb, tid, B + cpr.b_off, tl.col0, tk.n, cpr.k, cpr.sb_n, cpr.sb_k);
+  };
+  auto store = [&](const X3Set& s, int b) {
+    x3_store<CA>(S.Ah[b], S.Al[b], s.a, s.kla, tid);
+    x3_store<CB>(S.Bh[b], S.Bl[b], s.b, s.klb, tid);
+  };
+  auto mfma = [&](int b) {  // the B vectors of the wave's 4 column blocks stay in registers, A is read per row block
+    bf16x8 bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int rb_ = wc * 64 + i * 16 + lr;
+      bh[i] = *reinterpret_cast<const bf16x8*>(&S.Bh[b][rb_][x3_chunk(rb_, lk)]);
+      bl[i] = *reinterpret_cast<const bf16x8*>(&S.Bl[b][rb_][x3_chunk(rb_, lk)]);
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+      const int ra_ = wr * 64 + mi * 16 + lr;
+      const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&S.Ah[b][ra_][x3_chunk(ra_, lk)]);
+      const bf16x8 al = *reinterpret_cast<const bf16x8*>(&S.Al[b][ra_][x3_chunk(ra_, lk)]);
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni) {
+        if (!(live_r[mi] && live_c[ni])) continue;  // wholly in the padding of an edge tile
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ni], acc[mi][ni], 0, 0, 0);
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ni], acc[mi][ni], 0, 0, 0);
+        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ni], acc[mi][ni], 0, 0, 0);
+      }
+    }
+  };
+  X3Set x;
+  load(x);  // panel 0
+  store(x, 0);
+  __syncthreads();
+  int buf = 0;
+  while (true) {
+    const bool more = cur;  // a next panel: loaded into x while the MFMAs run on the current one (uniform)
+    load(x);                // zeros past the tile's last panel
+    mfma(buf);
+    if (!more) break;
+    store(x, buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+}
+
+template <typename TC, bool Tab, bool CA, bool CB>
 __global__ __launch_bounds__(256, 2) void gemm_x3_128_kernel(const BigTile* __restrict__ tiles, int ntiles,
                                                              const GemmTask* __restrict__ tasks,
                                                              const GemmPair* __restrict__ pairs, Operands ops,
                                                              double alpha, int accumulate,
                                                              float* __restrict__ partial, int xchunk) {
-  __shared__ __attribute__((aligned(16))) __bf16 Ah[2][B2M][X2K];
-  __shared__ __attribute__((aligned(16))) __bf16 Al[2][B2M][X2K];
-  __shared__ __attribute__((aligned(16))) __bf16 Bh[2][B2N][X2K];
-  __shared__ __attribute__((aligned(16))) __bf16 Bl[2][B2N][X2K];
+  __shared__ __attribute__((aligned(16))) X3Lds S;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -145,62 +232,7 @@ __global__ __launch_bounds__(256, 2) void gemm_x3_128_kernel(const BigTile* __re
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = flt4{0, 0, 0, 0};
-
-    int p = tl.pair_lo, k0 = 0;
-    GemmPair pr = pairs[p];
-    X3View va, vb;
-    x3_view(va, pr.sa_k == 1, tid, tl.row0, tk.m, pr.sa_m, pr.sa_k);
-    x3_view(vb, pr.sb_k == 1, tid, tl.col0, tk.n, pr.sb_n, pr.sb_k);
-    float ra[16], rb[16];
-    x3_panel_load(ra, A + pr.a_off, va, 0, pr.k);
-    x3_panel_load(rb, B + pr.b_off, vb, 0, pr.k);
-    x3_panel_store(Ah[0], Al[0], ra, va);
-    x3_panel_store(Bh[0], Bl[0], rb, vb);
-    __syncthreads();
-    int buf = 0;
-    // va, vb describe the panel held in ra, rb (the next one once it is loaded)
-    while (true) {
-      int np = p, nk0 = k0 + X2K;
-      if (nk0 >= pr.k) {
-        np = p + 1;
-        nk0 = 0;
-      }
-      const bool more = np < tl.pair_hi;  // uniform over the work-group
-      if (more) {
-        if (np != p) {
-          pr = pairs[np];
-          x3_view(va, pr.sa_k == 1, tid, tl.row0, tk.m, pr.sa_m, pr.sa_k);
-          x3_view(vb, pr.sb_k == 1, tid, tl.col0, tk.n, pr.sb_n, pr.sb_k);
-        }
-        x3_panel_load(ra, A + pr.a_off, va, nk0, pr.k);
-        x3_panel_load(rb, B + pr.b_off, vb, nk0, pr.k);
-      }
-      bf16x8 ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int ra_ = wr * 64 + i * 16 + lr, rb_ = wc * 64 + i * 16 + lr;
-        ah[i] = *reinterpret_cast<const bf16x8*>(&Ah[buf][ra_][x3_chunk(ra_, lk)]);
-        al[i] = *reinterpret_cast<const bf16x8*>(&Al[buf][ra_][x3_chunk(ra_, lk)]);
-        bh[i] = *reinterpret_cast<const bf16x8*>(&Bh[buf][rb_][x3_chunk(rb_, lk)]);
-        bl[i] = *reinterpret_cast<const bf16x8*>(&Bl[buf][rb_][x3_chunk(rb_, lk)]);
-      }
-#pragma unroll
-      for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 4; ++ni) {
-          if (!(live_r[mi] && live_c[ni])) continue;  // wholly in the padding of an edge tile
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
-        }
-      if (!more) break;
-      x3_panel_store(Ah[buf ^ 1], Al[buf ^ 1], ra, va);
-      x3_panel_store(Bh[buf ^ 1], Bl[buf ^ 1], rb, vb);
-      __syncthreads();
-      buf ^= 1;
-      p = np;
-      k0 = nk0;
-    }
+    x3_tile<CA, CB>(acc, S, tl, tk, pairs, A, B, tid, wr, wc, lr, lk, live_r, live_c);
 
     if (tl.slot < 0) {
       TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
@@ -235,19 +267,34 @@ __global__ __launch_bounds__(256, 2) void gemm_x3_128_kernel(const BigTile* __re
   }
 }
 
+template <typename TC, bool Tab, bool CA, bool CB>
+void launch_x3_128_group(const BigTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
+                         double alpha, int accumulate, float* part, int max_groups, hipStream_t s) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL((gemm_x3_128_kernel<TC, Tab, CA, CB>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
+                     ntiles, tasks, pairs, ops, alpha, accumulate, part, xcd_chunk());
+}
+
+template <typename TC, bool Tab>
+void launch_x3_128_groups(const BigTile* tiles, const int (&n)[4], const GemmTask* tasks, const GemmPair* pairs,
+                          Operands ops, double alpha, int accumulate, float* part, int max_groups, hipStream_t s) {
+  launch_x3_128_group<TC, Tab, true, true>(tiles, n[0], tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
+  tiles += n[0];
+  launch_x3_128_group<TC, Tab, true, false>(tiles, n[1], tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
+  tiles += n[1];
+  launch_x3_128_group<TC, Tab, false, true>(tiles, n[2], tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
+  tiles += n[2];
+  launch_x3_128_group<TC, Tab, false, false>(tiles, n[3], tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
+}
+
 template <typename TC>
-void launch_gemm_x3_128(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
+void launch_gemm_x3_128(const BigTile* tiles, const int (&ntiles)[4], const RedTile* red, int nred, void* partial,
                         const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
                         hipStream_t s, int max_groups) {
-  if (ntiles <= 0) return;
+  if (ntiles[0] + ntiles[1] + ntiles[2] + ntiles[3] <= 0) return;
   float* part = static_cast<float*>(partial);
-  const int xc = xcd_chunk();
-  if (ops.table)
-    hipLaunchKernelGGL((gemm_x3_128_kernel<TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate, part, xc);
-  else
-    hipLaunchKernelGGL((gemm_x3_128_kernel<TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate, part, xc);
+  if (ops.table) launch_x3_128_groups<TC, true>(tiles, ntiles, tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
+  else launch_x3_128_groups<TC, false>(tiles, ntiles, tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
   if (nred <= 0) return;
   if (ops.table)
     hipLaunchKernelGGL((gemm_reduce128_kernel<float, TC, true>), dim3(groups(nred * kRedStrips2, max_groups)),

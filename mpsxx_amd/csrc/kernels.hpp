@@ -27,9 +27,10 @@ struct GemmTask {
   int64_t c_off;
   int32_t m, n;
   int32_t pair_begin, pair_count;
-  int32_t inst;  // instance of a batched launch (index into the operand pointer tables); 0 otherwise
-  int32_t pad;
+  int32_t inst;    // instance of a batched launch (index into the operand pointer tables); 0 otherwise
+  int32_t layout;  // kTaskContigA: every pair's A has unit stride along k; kTaskContigB: every pair's B
 };
+constexpr int32_t kTaskContigA = 1, kTaskContigB = 2;
 
 // Operand bases of one launch: direct pointers (table == 0), or device tables of per-instance pointers indexed by the
 // task's `inst` (table == 1), so one grouped launch covers many independent problems.
@@ -111,9 +112,10 @@ template <typename TC>
 void launch_gemm_x3(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
                     const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
                     hipStream_t s, int max_groups = 0);
-// The 128x128 tiles of launch_gemm128 as split-BF16 products (gemm_x3_128.hpp); FP32 partial slots of 128x128
+// The 128x128 tiles of launch_gemm128 as split-BF16 products (gemm_x3_128.hpp); FP32 partial slots of 128x128.  The
+// tiles come grouped by their task's layout flags (GemmTask::layout) 3, 1, 2, 0, ntiles[q] of group q, one launch each.
 template <typename TC>
-void launch_gemm_x3_128(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
+void launch_gemm_x3_128(const BigTile* tiles, const int (&ntiles)[4], const RedTile* red, int nred, void* partial,
                         const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
                         hipStream_t s, int max_groups = 0);
 

@@ -3,9 +3,9 @@
 
 namespace mpx {
 
-template void launch_gemm_x3_128<float>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*,
+template void launch_gemm_x3_128<float>(const BigTile*, const int (&)[4], const RedTile*, int, void*, const GemmTask*,
                                         const GemmPair*, Operands, double, int, hipStream_t, int);
-template void launch_gemm_x3_128<double>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*,
+template void launch_gemm_x3_128<double>(const BigTile*, const int (&)[4], const RedTile*, int, void*, const GemmTask*,
                                          const GemmPair*, Operands, double, int, hipStream_t, int);
 
 }  // namespace mpx
