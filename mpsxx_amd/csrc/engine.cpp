@@ -1207,7 +1207,7 @@ void launch_step_t(const Step& st, const Operands& ops, unsigned parts, bool x3,
                              st.partial2.p, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha,
                              st.accumulate, s, st.max_groups);
     else if (parts & 8u)
-      launch_gemm128<T, TC>(st.work2.as<BigTile>(), st.nwork2, st.work_red2.as<RedTile>(), st.nwork_red2,
+      launch_gemm128<T, TC>(st.work2.as<BigTile>(), st.nwork2_layout, st.work_red2.as<RedTile>(), st.nwork_red2,
                             st.partial2.p, st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha,
                             st.accumulate, s, st.max_groups);
   } else if (parts & 1u) {

@@ -33,7 +33,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float flt4v __attribute__((ext_vector_type(4)));
 
 constexpr int X2K = 32;                      // k panel: one bf16 MFMA depth
-constexpr int32_t kOutOfBlock = 0x7fffffff;  // a byte offset beyond every block extent (the host keeps blocks < 2^30 B)
 
 // Operand element -> thread map of one layout.
 template <bool Contig>

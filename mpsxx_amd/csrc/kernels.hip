@@ -1652,9 +1652,9 @@ void launch_gemm_split(const BigTile* tiles, int ntiles, const RedTile* red, int
 MPX_INST_BIG(float, float)
 MPX_INST_BIG(float, double)
 #undef MPX_INST_BIG
-template void launch_gemm128<float, float>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*,
+template void launch_gemm128<float, float>(const BigTile*, const int (&)[4], const RedTile*, int, void*, const GemmTask*,
                                            const GemmPair*, Operands, double, int, hipStream_t, int);
-template void launch_gemm128<float, double>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*,
+template void launch_gemm128<float, double>(const BigTile*, const int (&)[4], const RedTile*, int, void*, const GemmTask*,
                                             const GemmPair*, Operands, double, int, hipStream_t, int);
 #define MPX_INST_GEMM(FN, T, TC) \
   template void FN<T, TC>(const GemmTile*, int, const GemmTask*, const GemmPair*, Operands, double, int, hipStream_t, \

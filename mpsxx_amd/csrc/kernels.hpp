@@ -102,10 +102,12 @@ constexpr int kAxpyChunk = 1024;  // elements per work-group in the AXPY / copy 
 template <typename T, typename TC>
 void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
                  const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups = 0);
-// 128x128 tiles of blocks with both sides >= kBig2Min (gemm128.hpp); partial slots of 128x128
+// 128x128 tiles of the largest blocks (gemm128.hpp), grouped by their task's layout flags 3, 1, 2, 0 (ntiles[q] of
+// group q, one launch each); partial slots of 128x128
 template <typename T, typename TC>
-void launch_gemm128(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
-                 const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups = 0);
+void launch_gemm128(const BigTile* tiles, const int (&ntiles)[4], const RedTile* red, int nred, void* partial,
+                    const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
+                    hipStream_t s, int max_groups = 0);
 
 // Same tiles with FP32 operands multiplied as split-BF16 pairs (three bf16 MFMAs per product, FP32 accumulation).
 template <typename TC>
