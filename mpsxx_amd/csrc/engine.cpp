@@ -1073,6 +1073,21 @@ void set_gemm_policy(Step& st, const GemmPolicy& pol) {
   gemm_tiles(st.host.gtasks, st.host.gpairs, pol, g);
   st.policy = pol;
   st.max_groups = pol.max_groups;
+  // the 64x64 and 128x128 tiles grouped by their task's operand layouts (one kernel per layout), order kept within
+  auto group_by_layout = [&](std::vector<BigTile>& tl_list, int (&counts)[4]) {
+    const int32_t order[4] = {kTaskContigA | kTaskContigB, kTaskContigA, kTaskContigB, 0};
+    std::vector<BigTile> grouped;
+    grouped.reserve(tl_list.size());
+    for (int q = 0; q < 4; ++q) {
+      const size_t before = grouped.size();
+      for (const BigTile& tl : tl_list)
+        if ((st.host.gtasks[tl.task].layout & (kTaskContigA | kTaskContigB)) == order[q]) grouped.push_back(tl);
+      counts[q] = (int)(grouped.size() - before);
+    }
+    tl_list.swap(grouped);
+  };
+  group_by_layout(g.big, st.nwork_layout);
+  group_by_layout(g.big2, st.nwork2_layout);
   st.nwork = (int)g.big.size();
   st.work = upload_vec(g.big);
   st.nwork_red = (int)g.red.size();
@@ -1084,19 +1099,6 @@ void set_gemm_policy(Step& st, const GemmPolicy& pol) {
   st.work_split = upload_vec(g.split);
   st.nwork_split_red = (int)g.split_red.size();
   st.work_split_red = upload_vec(g.split_red);
-  // the 128x128 tiles grouped by their task's operand layouts (one split-BF16 kernel per layout), order kept within
-  {
-    const int32_t order[4] = {kTaskContigA | kTaskContigB, kTaskContigA, kTaskContigB, 0};
-    std::vector<BigTile> grouped;
-    grouped.reserve(g.big2.size());
-    for (int q = 0; q < 4; ++q) {
-      const size_t before = grouped.size();
-      for (const BigTile& tl : g.big2)
-        if ((st.host.gtasks[tl.task].layout & (kTaskContigA | kTaskContigB)) == order[q]) grouped.push_back(tl);
-      st.nwork2_layout[q] = (int)(grouped.size() - before);
-    }
-    g.big2.swap(grouped);
-  }
   st.nwork2 = (int)g.big2.size();
   st.work2 = upload_vec(g.big2);
   st.nwork_red2 = (int)g.red2.size();
@@ -1192,7 +1194,7 @@ void launch_step_t(const Step& st, const Operands& ops, unsigned parts, bool x3,
                          st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s,
                          st.max_groups);
     else if (parts & 1u)
-      launch_gemm<T, TC>(st.work.as<BigTile>(), st.nwork, st.work_red.as<RedTile>(), st.nwork_red, st.partial.p,
+      launch_gemm<T, TC>(st.work.as<BigTile>(), st.nwork_layout, st.work_red.as<RedTile>(), st.nwork_red, st.partial.p,
                          st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s,
                          st.max_groups);
     if (parts & 2u)

@@ -187,6 +187,7 @@ struct Step {
   DevBuf work_split_red, split_partial;
   int nwork2 = 0;              // GEMM: 128x128 tiles of the largest blocks (gemm128_kernel), their split reductions
   int nwork2_layout[4] = {0, 0, 0, 0};  // of those, tiles of tasks with layout flags 3, 1, 2, 0, listed in that order
+  int nwork_layout[4] = {0, 0, 0, 0};   // the same for the 64x64 tiles (work, nwork)
   DevBuf work2;
   int nwork_red2 = 0;
   DevBuf work_red2, partial2;

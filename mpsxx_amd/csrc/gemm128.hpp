@@ -4,16 +4,9 @@
 // One 256-thread work-group per 128x128 C tile, 4 waves each owning a 64x64 quarter computed as 4x4
 // v_mfma_{f64,f32}_16x16x4 tiles: per 4-wide k step a wave reads 4 A and 4 B operands from LDS and issues 16 MFMAs,
 // four times the MFMAs per operand byte of the 64x64 tiles.  A and B panels (128x16, 16x128) are staged in LDS k-major
-// and double-buffered; the next panel is loaded into registers while the MFMAs run on the current one.  Each thread
-// moves 8 elements of each operand per panel through buffer loads against a resource covering the operand block: an
-// element outside the block gets an offset beyond the extent and reads as zero, so every panel (inside the block or
-// on its edge, and the empty panel past the end of the tile) goes through the same branch-free loads and the compiler
-// counts the loads in flight exactly.  The kernel is specialised per operand layout (the host groups the tiles by
-// their task's layout flags, one launch per group):
-//   unit stride along k: a thread takes 4 consecutive k of lines t/4 and t/4 + 64 (k 4*(t%4)..+3), one 16-byte load
-//     per line for FP32 and two for FP64; k past the pair's end is zeroed when the panel is stored;
-//   otherwise: a thread takes k t/128 + 2*i (i = 0..7) of line t%128, one element per load, 128 consecutive lines
-//     per load instruction over the work-group.
+// and double-buffered; the next panel is loaded into registers while the MFMAs run on the current one.  Panels move
+// through the buffer loads of gemm_tiles.hpp (8 elements of each operand per thread), one kernel per operand layout
+// (the host groups the tiles by their task's layout flags, one launch per group).
 // 16x16 sub-tiles that lie wholly in the padding of an edge tile skip their MFMAs.
 #pragma once
 
@@ -22,95 +15,6 @@
 namespace mpx {
 
 constexpr int B2M = 128, B2N = 128, LDP2 = 128 + 4;
-constexpr int32_t kOutOfBlock = 0x7fffffff;  // a byte offset beyond every block extent (the host keeps blocks < 2^30 B)
-
-// One operand of the current pair: the buffer resource of its block and the thread's offsets.
-struct G2View {
-  __amdgpu_buffer_rsrc_t rsrc;
-  int32_t boff0;  // byte offset of the thread's first element for k0 = 0
-  int32_t brun;   // byte offset of the thread's second line (unit-stride-k layout) or second k (otherwise)
-  int32_t bsk;    // bytes per unit of k
-  int32_t lim;    // lines of the block from the tile's first line on
-};
-
-template <bool Contig>
-struct G2Map {
-  static __device__ __forceinline__ int l0(int tid) { return Contig ? tid >> 2 : tid & 127; }
-  static __device__ __forceinline__ int kk0(int tid) { return Contig ? 4 * (tid & 3) : tid >> 7; }
-};
-
-template <typename T, bool Contig>
-__device__ __forceinline__ void g2_view(G2View& v, int tid, const T* __restrict__ block, int line0, int nline,
-                                        int klen, int64_t s_line, int64_t s_k) {
-  typedef G2Map<Contig> M;
-  // elements spanned by the block (0 for an empty one: then every element reads as zero)
-  const int64_t extent = nline > 0 && klen > 0 ? (int64_t)(nline - 1) * s_line + (int64_t)(klen - 1) * s_k + 1 : 0;
-  v.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(block), (short)0, (int)(extent * (int64_t)sizeof(T)),
-                                             0x00020000);
-  v.boff0 = (int32_t)(sizeof(T) * ((int64_t)(line0 + M::l0(tid)) * s_line + (int64_t)M::kk0(tid) * s_k));
-  v.brun = (int32_t)(sizeof(T) * (Contig ? 64 * s_line : 2 * s_k));
-  v.bsk = (int32_t)(sizeof(T) * s_k);
-  v.lim = nline - line0;
-}
-
-template <typename T>
-__device__ __forceinline__ T g2_load1(__amdgpu_buffer_rsrc_t r, int32_t off) {
-  if constexpr (sizeof(T) == 8) return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-  else return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
-
-// The thread's 8 elements of the panel at k0 (valid = false past the tile's last panel: zeros).  kl = k of the
-// thread's first element left in the pair (unit-stride-k layout: elements i >= kl of a line are zeroed at the store).
-template <typename T, bool Contig>
-__device__ __forceinline__ void g2_load(T (&r)[8], int& kl, const G2View& v, int tid, int k0, int klen, bool valid) {
-  typedef G2Map<Contig> M;
-  const int32_t base = v.boff0 + k0 * v.bsk;
-  kl = klen - k0 - M::kk0(tid);
-  if constexpr (Contig) {
-    typedef float f4 __attribute__((ext_vector_type(4)));
-    typedef double d2 __attribute__((ext_vector_type(2)));
-#pragma unroll
-    for (int run = 0; run < 2; ++run) {
-      const bool ok = valid && M::l0(tid) + 64 * run < v.lim && kl > 0;
-      const int32_t off = ok ? base + run * v.brun : kOutOfBlock;
-      if constexpr (sizeof(T) == 8) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const d2 x = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, ok ? off + 16 * h : off, 0, 0));
-          r[4 * run + 2 * h] = x[0];
-          r[4 * run + 2 * h + 1] = x[1];
-        }
-      } else {
-        const f4 x = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, off, 0, 0));
-#pragma unroll
-        for (int i = 0; i < 4; ++i) r[4 * run + i] = x[i];
-      }
-    }
-  } else {
-    const bool lok = valid && M::l0(tid) < v.lim;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int32_t off = (lok && 2 * i < kl) ? base + i * v.brun : kOutOfBlock;
-      r[i] = g2_load1<T>(v.rsrc, off);
-    }
-  }
-}
-
-template <typename T, bool Contig>
-__device__ __forceinline__ void g2_store(T (*S)[LDP2], const T (&r)[8], int kl, int tid) {
-  typedef G2Map<Contig> M;
-  const int line = M::l0(tid), k = M::kk0(tid);
-  if constexpr (Contig) {
-#pragma unroll
-    for (int run = 0; run < 2; ++run)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) S[k + i][line + 64 * run] = i < kl ? r[4 * run + i] : T(0);
-  } else {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) S[k + 2 * i][line] = r[i];
-  }
-}
-
 // FP64: one work-group per CU (each wave has the whole 512-entry register file for its 128 accumulator registers and
 // the panel prefetch; the 64-cycle FP64 MFMAs of one panel cover the prefetch latency); FP32: two
 template <typename T, typename TC, bool Tab, bool CA, bool CB>
@@ -150,15 +54,15 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void gemm128_kernel(co
     // the load cursor (cp, ck0) walks the tile's (pair, k panel) sequence; va, vb view the cursor's pair
     int cp = tl.pair_lo, ck0 = 0;
     GemmPair cpr = pairs[cp];
-    G2View va, vb;
-    g2_view<T, CA>(va, tid, A + cpr.a_off, tl.row0, tk.m, cpr.k, cpr.sa_m, cpr.sa_k);
-    g2_view<T, CB>(vb, tid, B + cpr.b_off, tl.col0, tk.n, cpr.k, cpr.sb_n, cpr.sb_k);
+    PanelView va, vb;
+    panel_view<T, CA, B2M>(va, tid, A + cpr.a_off, tl.row0, tk.m, cpr.k, cpr.sa_m, cpr.sa_k);
+    panel_view<T, CB, B2M>(vb, tid, B + cpr.b_off, tl.col0, tk.n, cpr.k, cpr.sb_n, cpr.sb_k);
     bool cur = true;  // the cursor is on a panel of the tile (uniform)
     T ra[8], rb[8];
     int kla, klb;
     auto load = [&]() {
-      g2_load<T, CA>(ra, kla, va, tid, ck0, cpr.k, cur);
-      g2_load<T, CB>(rb, klb, vb, tid, ck0, cpr.k, cur);
+      panel_load<T, CA, B2M>(ra, kla, va, tid, ck0, cpr.k, cur);
+      panel_load<T, CB, B2M>(rb, klb, vb, tid, ck0, cpr.k, cur);
       if (!cur) return;
       ck0 += BK;
       if (ck0 < cpr.k) return;
@@ -168,12 +72,12 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void gemm128_kernel(co
         return;
       }
       cpr = pairs[cp];
-      g2_view<T, CA>(va, tid, A + cpr.a_off, tl.row0, tk.m, cpr.k, cpr.sa_m, cpr.sa_k);
-      g2_view<T, CB>(vb, tid, B + cpr.b_off, tl.col0, tk.n, cpr.k, cpr.sb_n, cpr.sb_k);
+      panel_view<T, CA, B2M>(va, tid, A + cpr.a_off, tl.row0, tk.m, cpr.k, cpr.sa_m, cpr.sa_k);
+      panel_view<T, CB, B2M>(vb, tid, B + cpr.b_off, tl.col0, tk.n, cpr.k, cpr.sb_n, cpr.sb_k);
     };
     load();
-    g2_store<T, CA>(As[0], ra, kla, tid);
-    g2_store<T, CB>(Bs[0], rb, klb, tid);
+    panel_store<T, CA, B2M, LDP2>(As[0], ra, kla, tid);
+    panel_store<T, CB, B2M, LDP2>(Bs[0], rb, klb, tid);
     __syncthreads();
     int buf = 0;
     while (true) {
@@ -194,8 +98,8 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void gemm128_kernel(co
             if (live_r[mi] && live_c[ni]) acc[mi][ni] = Mfma<T>::op(a[mi], b[ni], acc[mi][ni]);
       }
       if (!more) break;
-      g2_store<T, CA>(As[buf ^ 1], ra, kla, tid);
-      g2_store<T, CB>(Bs[buf ^ 1], rb, klb, tid);
+      panel_store<T, CA, B2M, LDP2>(As[buf ^ 1], ra, kla, tid);
+      panel_store<T, CB, B2M, LDP2>(Bs[buf ^ 1], rb, klb, tid);
       __syncthreads();
       buf ^= 1;
     }

@@ -61,63 +61,107 @@ __device__ __forceinline__ T* opnd(const void* p, int inst) {
   else return reinterpret_cast<T*>(const_cast<void*>(p));
 }
 
-// Panel staging of the 64x64 kernel: each of the 256 threads moves 4 elements of the 64x16 A panel and 4 of the 16x64
-// B panel.  The element -> thread map follows the operand's unit-stride direction (k for A stored as used, rows for A
-// used transposed; columns for B as stored, k for B transposed) so that global loads are coalesced either way.
+// Panel staging of the 64x64 and 128x128 tiles through buffer loads.  A 16-wide k panel of LINES lines (rows of A or
+// columns of B) is moved by 256 threads, LINES/16 elements each, into LDS k-major ([k][line]).  Every load goes
+// against a buffer resource covering the operand block: an element outside the block gets an offset beyond the
+// extent and reads as zero, so every panel (inside the block or on its edge, and the empty panel past the end of a
+// tile) goes through the same branch-free loads and the compiler counts the loads in flight exactly (guarded loads in
+// several code paths made it wait for every outstanding load, the prefetch included, before each LDS store).  Element
+// -> thread maps, chosen per operand layout:
+//   unit stride along k: a thread takes 4 consecutive k (4*(t%4)..+3) of lines t/4 + 64*run, run < LINES/64, one
+//     16-byte load per line for FP32 and two for FP64; k past the pair's end is zeroed when the panel is stored;
+//   otherwise: a thread takes line t%LINES at k t/LINES + (256/LINES)*i, one element per load, LINES consecutive
+//     lines per load instruction over the work-group.
+constexpr int32_t kOutOfBlock = 0x7fffffff;  // a byte offset beyond every block extent (the host keeps blocks < 2^30 B)
+
+// One operand of the current pair: the buffer resource of its block and the thread's offsets.
+struct PanelView {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int32_t boff0;  // byte offset of the thread's first element for k0 = 0
+  int32_t bstep;  // byte offset between the thread's lines (unit-stride-k layout) or k values (otherwise)
+  int32_t bsk;    // bytes per unit of k
+  int32_t lim;    // lines of the block from the tile's first line on
+};
+
+template <bool Contig, int LINES>
+struct PanelMap {
+  static constexpr int E = LINES * BK / 256;      // elements per thread
+  static constexpr int KSTEP = 256 / LINES;        // k step between the thread's elements (general layout)
+  static __device__ __forceinline__ int l0(int tid) { return Contig ? tid >> 2 : tid % LINES; }
+  static __device__ __forceinline__ int kk0(int tid) { return Contig ? 4 * (tid & 3) : tid / LINES; }
+};
+
+template <typename T, bool Contig, int LINES>
+__device__ __forceinline__ void panel_view(PanelView& v, int tid, const T* __restrict__ block, int line0, int nline,
+                                           int klen, int64_t s_line, int64_t s_k) {
+  typedef PanelMap<Contig, LINES> M;
+  // elements spanned by the block (0 for an empty one: then every element reads as zero)
+  const int64_t extent = nline > 0 && klen > 0 ? (int64_t)(nline - 1) * s_line + (int64_t)(klen - 1) * s_k + 1 : 0;
+  v.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(block), (short)0, (int)(extent * (int64_t)sizeof(T)),
+                                             0x00020000);
+  v.boff0 = (int32_t)(sizeof(T) * ((int64_t)(line0 + M::l0(tid)) * s_line + (int64_t)M::kk0(tid) * s_k));
+  v.bstep = (int32_t)(sizeof(T) * (Contig ? 64 * s_line : M::KSTEP * s_k));
+  v.bsk = (int32_t)(sizeof(T) * s_k);
+  v.lim = nline - line0;
+}
+
 template <typename T>
-__device__ __forceinline__ void load_a(T (&r)[4], const T* __restrict__ Ap, const GemmPair& pr, int row0, int m, int k0,
-                                       int tid) {
-  if (pr.sa_k == 1) {
+__device__ __forceinline__ T buffer_load1(__amdgpu_buffer_rsrc_t r, int32_t off) {
+  if constexpr (sizeof(T) == 8) return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  else return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+// The thread's elements of the panel at k0 (valid = false past the tile's last panel: zeros).  kl = k of the
+// thread's first element left in the pair (unit-stride-k layout: elements i >= kl of a line are zeroed at the store).
+template <typename T, bool Contig, int LINES>
+__device__ __forceinline__ void panel_load(T (&r)[(PanelMap<Contig, LINES>::E)], int& kl, const PanelView& v, int tid,
+                                           int k0, int klen, bool valid) {
+  typedef PanelMap<Contig, LINES> M;
+  const int32_t base = v.boff0 + k0 * v.bsk;
+  kl = klen - k0 - M::kk0(tid);
+  if constexpr (Contig) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef double d2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j, kk = e & 15, i = e >> 4;
-      const int gi = row0 + i, gk = k0 + kk;
-      r[j] = (gi < m && gk < pr.k) ? Ap[(int64_t)gi * pr.sa_m + gk] : T(0);
+    for (int run = 0; run < LINES / 64; ++run) {
+      const bool ok = valid && M::l0(tid) + 64 * run < v.lim && kl > 0;
+      const int32_t off = ok ? base + run * v.bstep : kOutOfBlock;
+      if constexpr (sizeof(T) == 8) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const d2 x =
+              __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, ok ? off + 16 * h : off, 0, 0));
+          r[4 * run + 2 * h] = x[0];
+          r[4 * run + 2 * h + 1] = x[1];
+        }
+      } else {
+        const f4 x = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, off, 0, 0));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[4 * run + i] = x[i];
+      }
     }
   } else {
+    const bool lok = valid && M::l0(tid) < v.lim;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j, i = e & 63, kk = e >> 6;
-      const int gi = row0 + i, gk = k0 + kk;
-      r[j] = (gi < m && gk < pr.k) ? Ap[(int64_t)gi * pr.sa_m + (int64_t)gk * pr.sa_k] : T(0);
+    for (int i = 0; i < M::E; ++i) {
+      const int32_t off = (lok && M::KSTEP * i < kl) ? base + i * v.bstep : kOutOfBlock;
+      r[i] = buffer_load1<T>(v.rsrc, off);
     }
   }
 }
-template <typename T>
-__device__ __forceinline__ void store_a(T (*As)[LDP], const T (&r)[4], bool contig_k, int tid) {
+
+template <typename T, bool Contig, int LINES, int LD>
+__device__ __forceinline__ void panel_store(T (*S)[LD], const T (&r)[(PanelMap<Contig, LINES>::E)], int kl, int tid) {
+  typedef PanelMap<Contig, LINES> M;
+  const int line = M::l0(tid), k = M::kk0(tid);
+  if constexpr (Contig) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int e = tid + 256 * j;
-    if (contig_k) As[e & 15][e >> 4] = r[j];
-    else As[e >> 6][e & 63] = r[j];
-  }
-}
-template <typename T>
-__device__ __forceinline__ void load_b(T (&r)[4], const T* __restrict__ Bp, const GemmPair& pr, int col0, int n, int k0,
-                                       int tid) {
-  if (pr.sb_n == 1) {
+    for (int run = 0; run < LINES / 64; ++run)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j, jj = e & 63, kk = e >> 6;
-      const int gj = col0 + jj, gk = k0 + kk;
-      r[j] = (gj < n && gk < pr.k) ? Bp[(int64_t)gk * pr.sb_k + gj] : T(0);
-    }
+      for (int i = 0; i < 4; ++i) S[k + i][line + 64 * run] = i < kl ? r[4 * run + i] : T(0);
   } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j, kk = e & 15, jj = e >> 4;
-      const int gj = col0 + jj, gk = k0 + kk;
-      r[j] = (gj < n && gk < pr.k) ? Bp[(int64_t)gk * pr.sb_k + (int64_t)gj * pr.sb_n] : T(0);
-    }
-  }
-}
-template <typename T>
-__device__ __forceinline__ void store_b(T (*Bs)[LDP], const T (&r)[4], bool contig_n, int tid) {
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int e = tid + 256 * j;
-    if (contig_n) Bs[e >> 6][e & 63] = r[j];
-    else Bs[e & 15][e >> 4] = r[j];
+    for (int i = 0; i < M::E; ++i) S[k + M::KSTEP * i][line] = r[i];
   }
 }
 
@@ -125,13 +169,16 @@ __device__ __forceinline__ void store_b(T (*Bs)[LDP], const T (&r)[4], bool cont
 // reduction is split over several work-groups) is software-pipelined: the next panel is loaded into registers while
 // the MFMAs run on the current one from one of two LDS buffers, one barrier per panel.  A tile with slot >= 0 writes
 // its unscaled partial sum to partial[slot] (64x64, row-major); gemm_reduce_kernel adds the slots of a tile in order.
-template <typename T, typename TC, bool Tab>
+// One kernel per operand layout (CA, CB: unit stride along k for every pair of the task's A, B).
+template <typename T, typename TC, bool Tab, bool CA, bool CB>
 __global__ __launch_bounds__(256) void gemm_grouped_kernel(const BigTile* __restrict__ tiles, int ntiles,
                                                            const GemmTask* __restrict__ tasks,
                                                            const GemmPair* __restrict__ pairs, Operands ops,
                                                            double alpha, int accumulate, T* __restrict__ partial,
                                                            int xchunk) {
   typedef typename Mfma<T>::acc_t acc_t;
+  typedef PanelMap<CA, BM> MA;
+  typedef PanelMap<CB, BN> MB;
   __shared__ T As[2][BK][LDP];
   __shared__ T Bs[2][BK][LDP];
   const int tid = threadIdx.x;
@@ -155,33 +202,38 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const BigTile* __rest
 #pragma unroll
       for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    // the tile's k panels over its pair range, counted first so that the panel loop is a plain counted loop (the
-    // accumulators then stay in the MFMA registers instead of being copied around a mid-loop exit)
-    int npanel = 0;  // every pair takes at least one panel (k = 0 pairs contribute a zero panel)
-    for (int q = tl.pair_lo; q < tl.pair_hi; ++q) npanel += max(1, (pairs[q].k + BK - 1) / BK);
-    // prologue: first panel into buffer 0
-    int p = tl.pair_lo, k0 = 0;
-    GemmPair pr = pairs[p];
-    T ra[4], rb[4];
-    load_a(ra, A + pr.a_off, pr, tl.row0, tk.m, 0, tid);
-    load_b(rb, B + pr.b_off, pr, tl.col0, tk.n, 0, tid);
-    store_a(As[0], ra, pr.sa_k == 1, tid);
-    store_b(Bs[0], rb, pr.sb_n == 1, tid);
+    // the load cursor (cp, ck0) walks the tile's (pair, k panel) sequence; va, vb view the cursor's pair
+    int cp = tl.pair_lo, ck0 = 0;
+    GemmPair cpr = pairs[cp];
+    PanelView va, vb;
+    panel_view<T, CA, BM>(va, tid, A + cpr.a_off, tl.row0, tk.m, cpr.k, cpr.sa_m, cpr.sa_k);
+    panel_view<T, CB, BN>(vb, tid, B + cpr.b_off, tl.col0, tk.n, cpr.k, cpr.sb_n, cpr.sb_k);
+    bool cur = true;  // the cursor is on a panel of the tile (uniform)
+    T ra[MA::E], rb[MB::E];
+    int kla, klb;
+    auto load = [&]() {
+      panel_load<T, CA, BM>(ra, kla, va, tid, ck0, cpr.k, cur);
+      panel_load<T, CB, BN>(rb, klb, vb, tid, ck0, cpr.k, cur);
+      if (!cur) return;
+      ck0 += BK;
+      if (ck0 < cpr.k) return;
+      ck0 = 0;
+      if (++cp >= tl.pair_hi) {
+        cur = false;
+        return;
+      }
+      cpr = pairs[cp];
+      panel_view<T, CA, BM>(va, tid, A + cpr.a_off, tl.row0, tk.m, cpr.k, cpr.sa_m, cpr.sa_k);
+      panel_view<T, CB, BN>(vb, tid, B + cpr.b_off, tl.col0, tk.n, cpr.k, cpr.sb_n, cpr.sb_k);
+    };
+    load();
+    panel_store<T, CA, BM, LDP>(As[0], ra, kla, tid);
+    panel_store<T, CB, BN, LDP>(Bs[0], rb, klb, tid);
     __syncthreads();
     int buf = 0;
-    for (int it = 0; it < npanel; ++it) {
-      int np = p, nk0 = k0 + BK;
-      GemmPair npr = pr;
-      if (nk0 >= pr.k) {
-        np = p + 1;
-        nk0 = 0;
-        if (np < tl.pair_hi) npr = pairs[np];
-      }
-      const bool more = it + 1 < npanel;  // uniform over the work-group
-      if (more) {
-        load_a(ra, A + npr.a_off, npr, tl.row0, tk.m, nk0, tid);
-        load_b(rb, B + npr.b_off, npr, tl.col0, tk.n, nk0, tid);
-      }
+    while (true) {
+      const bool more = cur;  // a next panel (uniform)
+      load();                 // zeros past the tile's last panel
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         const T a0 = As[buf][kk + lk][wr * 32 + lr];
@@ -193,15 +245,11 @@ __global__ __launch_bounds__(256) void gemm_grouped_kernel(const BigTile* __rest
         if (live_r1 && live_c0) acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
         if (live_r1 && live_c1) acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
       }
-      if (more) {
-        store_a(As[buf ^ 1], ra, npr.sa_k == 1, tid);
-        store_b(Bs[buf ^ 1], rb, npr.sb_n == 1, tid);
-      }
+      if (!more) break;
+      panel_store<T, CA, BM, LDP>(As[buf ^ 1], ra, kla, tid);
+      panel_store<T, CB, BN, LDP>(Bs[buf ^ 1], rb, klb, tid);
       __syncthreads();
       buf ^= 1;
-      p = np;
-      k0 = nk0;
-      pr = npr;
     }
 
     if (tl.slot < 0) {
@@ -278,18 +326,34 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const RedTile* __restr
 
 static inline int groups(int items, int max_groups) { return max_groups > 0 && items > max_groups ? max_groups : items; }
 
-template <typename T, typename TC>
-void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
-                 const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups) {
+template <typename T, typename TC, bool Tab, bool CA, bool CB>
+void launch_gemm_group(const BigTile* tiles, int ntiles, const GemmTask* tasks, const GemmPair* pairs, Operands ops,
+                       double alpha, int accumulate, T* part, int max_groups, hipStream_t s) {
   if (ntiles <= 0) return;
+  hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, Tab, CA, CB>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s,
+                     tiles, ntiles, tasks, pairs, ops, alpha, accumulate, part, xcd_chunk());
+}
+
+template <typename T, typename TC, bool Tab>
+void launch_gemm_groups(const BigTile* tiles, const int (&n)[4], const GemmTask* tasks, const GemmPair* pairs,
+                        Operands ops, double alpha, int accumulate, T* part, int max_groups, hipStream_t s) {
+  launch_gemm_group<T, TC, Tab, true, true>(tiles, n[0], tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
+  tiles += n[0];
+  launch_gemm_group<T, TC, Tab, true, false>(tiles, n[1], tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
+  tiles += n[1];
+  launch_gemm_group<T, TC, Tab, false, true>(tiles, n[2], tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
+  tiles += n[2];
+  launch_gemm_group<T, TC, Tab, false, false>(tiles, n[3], tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
+}
+
+template <typename T, typename TC>
+void launch_gemm(const BigTile* tiles, const int (&ntiles)[4], const RedTile* red, int nred, void* partial,
+                 const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
+                 hipStream_t s, int max_groups) {
+  if (ntiles[0] + ntiles[1] + ntiles[2] + ntiles[3] <= 0) return;
   T* part = static_cast<T*>(partial);
-  const int xc = xcd_chunk();
-  if (ops.table)
-    hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate, part, xc);
-  else
-    hipLaunchKernelGGL((gemm_grouped_kernel<T, TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles,
-                       ntiles, tasks, pairs, ops, alpha, accumulate, part, xc);
+  if (ops.table) launch_gemm_groups<T, TC, true>(tiles, ntiles, tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
+  else launch_gemm_groups<T, TC, false>(tiles, ntiles, tasks, pairs, ops, alpha, accumulate, part, max_groups, s);
   if (nred <= 0) return;
   if (ops.table)
     hipLaunchKernelGGL((gemm_reduce_kernel<T, TC, true>), dim3(groups(nred * kRedStrips, max_groups)), dim3(256), 0,

@@ -1647,7 +1647,8 @@ void launch_gemm_split(const BigTile* tiles, int ntiles, const RedTile* red, int
 }
 
 #define MPX_INST_BIG(T, TC)                                                                                   \
-  template void launch_gemm<T, TC>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*, const GemmPair*, \
+  template void launch_gemm<T, TC>(const BigTile*, const int (&)[4], const RedTile*, int, void*, const GemmTask*,    \
+                                   const GemmPair*, \
                                    Operands, double, int, hipStream_t, int);
 MPX_INST_BIG(float, float)
 MPX_INST_BIG(float, double)

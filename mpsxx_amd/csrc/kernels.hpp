@@ -97,11 +97,13 @@ constexpr int kAxpyChunk = 1024;  // elements per work-group in the AXPY / copy 
 
 // Launchers.  T = storage/arithmetic type of A and B (double or float), TC = type of C.  max_groups > 0 caps the grid
 // (persistent form: each work-group loops over the task list with a grid stride); 0 launches one per work item.
-// 64x64 tiles, 4 waves, LDS-staged and software-pipelined (large blocks); split reductions write partial tiles
-// (type T, 64*64 per slot) that a second launch sums in slot order.
+// 64x64 tiles, 4 waves, LDS-staged and software-pipelined (large blocks), grouped by their task's layout flags 3, 1,
+// 2, 0 (ntiles[q] of group q, one launch each); split reductions write partial tiles (type T, 64*64 per slot) that a
+// second launch sums in slot order.
 template <typename T, typename TC>
-void launch_gemm(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial, const GemmTask* tasks,
-                 const GemmPair* pairs, Operands ops, double alpha, int accumulate, hipStream_t s, int max_groups = 0);
+void launch_gemm(const BigTile* tiles, const int (&ntiles)[4], const RedTile* red, int nred, void* partial,
+                 const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
+                 hipStream_t s, int max_groups = 0);
 // 128x128 tiles of the largest blocks (gemm128.hpp), grouped by their task's layout flags 3, 1, 2, 0 (ntiles[q] of
 // group q, one launch each); partial slots of 128x128
 template <typename T, typename TC>

@@ -6,7 +6,7 @@
 
 namespace mpx {
 
-template void launch_gemm<double, double>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*,
+template void launch_gemm<double, double>(const BigTile*, const int (&)[4], const RedTile*, int, void*, const GemmTask*,
                                           const GemmPair*, Operands, double, int, hipStream_t, int);
 
 }  // namespace mpx
