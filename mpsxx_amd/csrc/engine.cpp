@@ -1190,7 +1190,7 @@ template <typename T, typename TC>
 void launch_step_t(const Step& st, const Operands& ops, unsigned parts, bool x3, hipStream_t s) {
   if (st.kind == Step::GEMM) {
     if ((parts & 1u) && x3)
-      launch_gemm_x3<TC>(st.work.as<BigTile>(), st.nwork, st.work_red.as<RedTile>(), st.nwork_red, st.partial.p,
+      launch_gemm_x3<TC>(st.work.as<BigTile>(), st.nwork_layout, st.work_red.as<RedTile>(), st.nwork_red, st.partial.p,
                          st.tasks.as<GemmTask>(), st.pairs.as<GemmPair>(), ops, st.alpha, st.accumulate, s,
                          st.max_groups);
     else if (parts & 1u)

@@ -10,205 +10,6 @@
 
 namespace mpx {
 
-// ---------------------------------------------------------------------------------------------------------------------
-// Split-BF16 (x3) 64x64 tiles: FP32 operands are split on the way into LDS as x = hi + lo with hi = bf16(x) and
-// lo = bf16(x - hi), and every product is formed as hi*hi + hi*lo + lo*hi by three v_mfma_f32_16x16x32_bf16 with FP32
-// accumulation (the lo*lo term, ~2^-16 relative, is dropped).  Operand layout of the instruction: lane l supplies row
-// (column) l%16 of A (B) at 8 k values; any k assignment works as long as A and B use the same one, here
-// k = 8*(l/16) + 0..7.  Result layout as the FP32 16x16x4 instruction.  LDS holds A as [row][k] and B as [col][k] so
-// each lane reads its 8 k values with one 16-byte load.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-constexpr int XBK = 32;          // k panel (one instruction)
-constexpr int XLD = XBK + 8;     // padded LDS row (bf16), 80 bytes
-
-__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
-  hi = (__bf16)x;
-  lo = (__bf16)(x - (float)hi);
-}
-
-// Staging of one 64x32 panel of A (rows x k) or of B (columns x k), 8 elements per thread, chosen so that global loads
-// stay coalesced and LDS stores are packed:
-//   k is the operand's unit-stride direction: thread t holds k pair 2*(t & 15) + {0, 1} of rows (t >> 4) + 16 j,
-//     j = 0..3 (16 lanes read 32 consecutive k of a row), stored as one 4-byte hi pair and one lo pair per row;
-//   otherwise: thread t holds k = 8*(t >> 6) + 0..7 of row t & 63 (a wave reads 64 consecutive rows), stored as one
-//     16-byte hi vector and one lo vector.
-__device__ __forceinline__ void x3_load(float (&r)[8], const float* __restrict__ P, int64_t s_rc, int64_t s_k, int klen,
-                                        int rc0, int nrc, int k0, bool k_fast, int tid) {
-  if (k_fast) {
-    const int kp = 2 * (tid & 15);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int g = rc0 + (tid >> 4) + 16 * j;
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int gk = k0 + kp + c;
-        r[2 * j + c] = (g < nrc && gk < klen) ? P[(int64_t)g * s_rc + (int64_t)gk * s_k] : 0.f;
-      }
-    }
-  } else {
-    const int g = rc0 + (tid & 63), k8 = 8 * (tid >> 6);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int gk = k0 + k8 + c;
-      r[c] = (g < nrc && gk < klen) ? P[(int64_t)g * s_rc + (int64_t)gk * s_k] : 0.f;
-    }
-  }
-}
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void x3_store(__bf16 (*Hi)[XLD], __bf16 (*Lo)[XLD], const float (&r)[8], bool k_fast,
-                                         int tid) {
-  if (k_fast) {
-    const int kp = 2 * (tid & 15);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int rc = (tid >> 4) + 16 * j;
-      bf16x2 h, l;
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        __bf16 hc, lc;
-        split_bf16(r[2 * j + c], hc, lc);
-        h[c] = hc;
-        l[c] = lc;
-      }
-      *reinterpret_cast<bf16x2*>(&Hi[rc][kp]) = h;
-      *reinterpret_cast<bf16x2*>(&Lo[rc][kp]) = l;
-    }
-  } else {
-    const int rc = tid & 63, k8 = 8 * (tid >> 6);
-    bf16x8 h, l;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      __bf16 hc, lc;
-      split_bf16(r[c], hc, lc);
-      h[c] = hc;
-      l[c] = lc;
-    }
-    *reinterpret_cast<bf16x8*>(&Hi[rc][k8]) = h;
-    *reinterpret_cast<bf16x8*>(&Lo[rc][k8]) = l;
-  }
-}
-__device__ __forceinline__ void x3_load_a(float (&r)[8], const float* __restrict__ Ap, const GemmPair& pr, int row0,
-                                          int m, int k0, int tid) {
-  x3_load(r, Ap, pr.sa_m, pr.sa_k, pr.k, row0, m, k0, pr.sa_k == 1, tid);
-}
-__device__ __forceinline__ void x3_load_b(float (&r)[8], const float* __restrict__ Bp, const GemmPair& pr, int col0,
-                                          int n, int k0, int tid) {
-  x3_load(r, Bp, pr.sb_n, pr.sb_k, pr.k, col0, n, k0, pr.sb_n != 1, tid);
-}
-
-template <typename TC, bool Tab>
-__global__ __launch_bounds__(256) void gemm_x3_kernel(const BigTile* __restrict__ tiles, int ntiles,
-                                                      const GemmTask* __restrict__ tasks,
-                                                      const GemmPair* __restrict__ pairs, Operands ops, double alpha,
-                                                      int accumulate, float* __restrict__ partial, int xchunk) {
-  __shared__ __attribute__((aligned(16))) __bf16 Ah[2][BM][XLD];
-  __shared__ __attribute__((aligned(16))) __bf16 Al[2][BM][XLD];
-  __shared__ __attribute__((aligned(16))) __bf16 Bh[2][BN][XLD];
-  __shared__ __attribute__((aligned(16))) __bf16 Bl[2][BN][XLD];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  const int lr = lane & 15, lk = lane >> 4;
-
-  for (int w = blockIdx.x; w < ntiles; w += gridDim.x) {
-    const BigTile tl = tiles[xcd_item(w, gridDim.x, ntiles, xchunk)];
-    const GemmTask tk = tasks[tl.task];
-    const float* __restrict__ A = opnd<const float, Tab>(ops.a, tk.inst);
-    const float* __restrict__ B = opnd<const float, Tab>(ops.b, tk.inst);
-    const bool live_r[2] = {tl.row0 + wr * 32 < tk.m, tl.row0 + wr * 32 + 16 < tk.m};
-    const bool live_c[2] = {tl.col0 + wc * 32 < tk.n, tl.col0 + wc * 32 + 16 < tk.n};
-    flt4 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = flt4{0, 0, 0, 0};
-
-    int p = tl.pair_lo, k0 = 0;
-    GemmPair pr = pairs[p];
-    float ra[8], rb[8];
-    x3_load_a(ra, A + pr.a_off, pr, tl.row0, tk.m, 0, tid);
-    x3_load_b(rb, B + pr.b_off, pr, tl.col0, tk.n, 0, tid);
-    x3_store(Ah[0], Al[0], ra, pr.sa_k == 1, tid);
-    x3_store(Bh[0], Bl[0], rb, pr.sb_n != 1, tid);
-    __syncthreads();
-    int buf = 0;
-    while (true) {
-      int np = p, nk0 = k0 + XBK;
-      GemmPair npr = pr;
-      if (nk0 >= pr.k) {
-        np = p + 1;
-        nk0 = 0;
-        if (np < tl.pair_hi) npr = pairs[np];
-      }
-      const bool more = np < tl.pair_hi;  // uniform over the work-group
-      if (more) {
-        x3_load_a(ra, A + npr.a_off, npr, tl.row0, tk.m, nk0, tid);
-        x3_load_b(rb, B + npr.b_off, npr, tl.col0, tk.n, nk0, tid);
-      }
-      bf16x8 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi) {
-        ah[mi] = *reinterpret_cast<const bf16x8*>(&Ah[buf][wr * 32 + mi * 16 + lr][8 * lk]);
-        al[mi] = *reinterpret_cast<const bf16x8*>(&Al[buf][wr * 32 + mi * 16 + lr][8 * lk]);
-      }
-#pragma unroll
-      for (int ni = 0; ni < 2; ++ni) {
-        bh[ni] = *reinterpret_cast<const bf16x8*>(&Bh[buf][wc * 32 + ni * 16 + lr][8 * lk]);
-        bl[ni] = *reinterpret_cast<const bf16x8*>(&Bl[buf][wc * 32 + ni * 16 + lr][8 * lk]);
-      }
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
-          if (!(live_r[mi] && live_c[ni])) continue;  // wholly in the padding of an edge tile
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mi], bh[ni], acc[mi][ni], 0, 0, 0);
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bl[ni], acc[mi][ni], 0, 0, 0);
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mi], bh[ni], acc[mi][ni], 0, 0, 0);
-        }
-      if (!more) break;
-      x3_store(Ah[buf ^ 1], Al[buf ^ 1], ra, npr.sa_k == 1, tid);
-      x3_store(Bh[buf ^ 1], Bl[buf ^ 1], rb, npr.sb_n != 1, tid);
-      __syncthreads();
-      buf ^= 1;
-      p = np;
-      k0 = nk0;
-      pr = npr;
-    }
-
-    if (tl.slot < 0) {
-      TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = tl.row0 + wr * 32 + mi * 16 + Mfma<float>::row(lk, r);
-            const int col = tl.col0 + wc * 32 + ni * 16 + lr;
-            if (row < tk.m && col < tk.n) {
-              const int64_t idx = tk.c_off + (int64_t)row * tk.n + col;
-              const double v = alpha * (double)acc[mi][ni][r];
-              C[idx] = accumulate ? (TC)((double)C[idx] + v) : (TC)v;
-            }
-          }
-    } else {
-      float* __restrict__ P = partial + (int64_t)tl.slot * (BM * BN);
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = wr * 32 + mi * 16 + Mfma<float>::row(lk, r);
-            const int col = wc * 32 + ni * 16 + lr;
-            P[row * BN + col] = acc[mi][ni][r];
-          }
-    }
-    __syncthreads();  // every wave is done with the LDS buffers before the next tile's prologue
-  }
-}
-
 // Operand fragment of one (pair, 16-wide k panel) unit of a 32x32 tile, straight from global memory into MFMA source
 // registers: lane l takes A[row + l%16][k + l/16] and B[k + l/16][col + l%16] for four 4-wide k steps.
 template <typename T>
@@ -816,31 +617,6 @@ int xcd_chunk() {
   return c;
 }
 
-template <typename TC>
-void launch_gemm_x3(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
-                    const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
-                    hipStream_t s, int max_groups) {
-  if (ntiles <= 0) return;
-  float* part = static_cast<float*>(partial);
-  const int xc = xcd_chunk();
-  if (ops.table)
-    hipLaunchKernelGGL((gemm_x3_kernel<TC, true>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles, ntiles,
-                       tasks, pairs, ops, alpha, accumulate, part, xc);
-  else
-    hipLaunchKernelGGL((gemm_x3_kernel<TC, false>), dim3(groups(ntiles, max_groups)), dim3(256), 0, s, tiles, ntiles,
-                       tasks, pairs, ops, alpha, accumulate, part, xc);
-  if (nred <= 0) return;
-  if (ops.table)
-    hipLaunchKernelGGL((gemm_reduce_kernel<float, TC, true>), dim3(groups(nred * kRedStrips, max_groups)), dim3(256),
-                       0, s, red, nred, tasks, part, ops, alpha, accumulate);
-  else
-    hipLaunchKernelGGL((gemm_reduce_kernel<float, TC, false>), dim3(groups(nred * kRedStrips, max_groups)), dim3(256),
-                       0, s, red, nred, tasks, part, ops, alpha, accumulate);
-}
-template void launch_gemm_x3<float>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*, const GemmPair*,
-                                    Operands, double, int, hipStream_t, int);
-template void launch_gemm_x3<double>(const BigTile*, int, const RedTile*, int, void*, const GemmTask*, const GemmPair*,
-                                     Operands, double, int, hipStream_t, int);
 
 template <typename T, typename TC>
 void launch_axpy(const Chunk* chunks, int nchunks, const AxpyTask* tasks, const AxpyPair* pairs, Operands ops,

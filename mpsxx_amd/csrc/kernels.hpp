@@ -111,12 +111,13 @@ void launch_gemm128(const BigTile* tiles, const int (&ntiles)[4], const RedTile*
                     const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
                     hipStream_t s, int max_groups = 0);
 
-// Same tiles with FP32 operands multiplied as split-BF16 pairs (three bf16 MFMAs per product, FP32 accumulation).
+// Same 64x64 tiles with FP32 operands multiplied as split-BF16 pairs (three bf16 MFMAs per product, FP32
+// accumulation; gemm_x3.hpp), grouped by layout as for launch_gemm.
 template <typename TC>
-void launch_gemm_x3(const BigTile* tiles, int ntiles, const RedTile* red, int nred, void* partial,
+void launch_gemm_x3(const BigTile* tiles, const int (&ntiles)[4], const RedTile* red, int nred, void* partial,
                     const GemmTask* tasks, const GemmPair* pairs, Operands ops, double alpha, int accumulate,
                     hipStream_t s, int max_groups = 0);
-// The 128x128 tiles of launch_gemm128 as split-BF16 products (gemm_x3_128.hpp); FP32 partial slots of 128x128.  The
+// The 128x128 tiles of launch_gemm128 as split-BF16 products (gemm_x3.hpp); FP32 partial slots of 128x128.  The
 // tiles come grouped by their task's layout flags (GemmTask::layout) 3, 1, 2, 0, ntiles[q] of group q, one launch each.
 template <typename TC>
 void launch_gemm_x3_128(const BigTile* tiles, const int (&ntiles)[4], const RedTile* red, int nred, void* partial,
