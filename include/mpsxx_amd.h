@@ -32,7 +32,8 @@ enum {
   MPSXX_ERR_HIP = 3,         /* HIP runtime error                                                                   */
   MPSXX_ERR_OOM = 4,         /* device allocation failed                                                            */
   MPSXX_ERR_ARG = 5,         /* invalid argument                                                                    */
-  MPSXX_ERR_UNSUPPORTED = 6  /* contraction form outside the supported set                                          */
+  MPSXX_ERR_UNSUPPORTED = 6, /* contraction form outside the supported set                                          */
+  MPSXX_ERR_IO = 7           /* file missing, unreadable, truncated or corrupt (reference: boost archive exceptions) */
 };
 
 /* ---- arithmetic of the block contractions ------------------------------------------------------------------------- */
@@ -78,6 +79,29 @@ int mpsxx_tensor_layout(mpsxx_tensor t, int32_t* nsec, int32_t* sec_q, int32_t* 
 int mpsxx_tensor_device_ptr(mpsxx_tensor t, double** dptr);
 /* copy of x with identical structure and data (btas::Copy) */
 int mpsxx_tensor_copy(mpsxx_tensor x, mpsxx_tensor y, void* stream);
+
+/* ---- on-disk arrays (reference: load/save of src/fileio.h:9-25, boost binary archives; names of getfile,
+ * src/fileio.cpp:7-13).  Own self-describing little-endian format: magic "MPSXXQST", version 1, the structure
+ * (rank, qsym, q_total, nsec, sector quantum numbers and dimensions, blocks in ascending tag), the FP64 data in
+ * canonical order and an FNV-1a-64 checksum; readers reject a wrong magic/version, out-of-range or disallowed
+ * structure, a truncated or over-long file and a checksum mismatch with MPSXX_ERR_IO.  Files are written beside the
+ * target and renamed into place.  Rank 0 on disk is an array with no structure (size() == 0). ----------------------- */
+/* host-only (no device needed): desc's blocks in any order; data holds nelem values in canonical order (blocks by
+ * ascending tag, each row-major), as mpsxx_tensor_upload takes them */
+int mpsxx_qst_save(const char* path, const mpsxx_qst_desc* desc, const double* data, int64_t nelem);  /* rank 0: empty */
+/* sizes first (as mpsxx_tensor_info), then the arrays (as mpsxx_tensor_layout, plus data of nelem values) */
+int mpsxx_qst_file_info(const char* path, int32_t* rank, int32_t* qsym, int32_t q_total[2], int64_t* nsec_total,
+                        int64_t* nblocks, int64_t* nelem);
+int mpsxx_qst_file_read(const char* path, int32_t* nsec, int32_t* sec_q, int32_t* sec_dim, int32_t* block_idx,
+                        double* data, int64_t nelem);
+/* device tensors: save downloads and writes; load replaces t's structure and data with the file's.  Synchronous. */
+int mpsxx_tensor_save(mpsxx_tensor t, const char* path, void* stream);
+int mpsxx_tensor_load(const char* path, mpsxx_tensor t, void* stream);
+/* "[prefix]/[name].[site].[istate].[jstate].0.tmp" into buf (cap bytes including the terminating NUL); *len = the
+ * name's length (without NUL) also when cap is too small (then MPSXX_ERR_ARG).  The reference's last field is the
+ * MPI rank; one process drives one device here, so it is 0. */
+int mpsxx_getfile(const char* name, const char* prefix, int site, int istate, int jstate, char* buf, int64_t cap,
+                  int64_t* len);
 
 /* ---- BTAS machinery (include/btas/QSPARSE/QSTCONTRACT.h:16-57, QSTBLAS.h:183-226) --------------------------------- */
 /* c = alpha * contract(a[contract_a], b[contract_b]) + beta * c */
@@ -265,6 +289,17 @@ int mpsxx_dmrg_add_lower_root(mpsxx_dmrg d, mpsxx_dmrg root, double energy, void
 int mpsxx_dmrg_mpo_gradient(mpsxx_dmrg d, const mpsxx_tensor* grads, void* stream);
 /* <psi|O|psi>/<psi|psi> for psi = prod_i mps[i] by left-to-right environment chains (src/expectation.hpp:55-114) */
 int mpsxx_expectation(int nsites, const mpsxx_tensor* mpo, const mpsxx_tensor* mps, double* value, void* stream);
+/* Sweep state on disk under the reference's site-file names (src/make_random_mpss.cpp:155-217 and
+ * src/make_sweep.cpp:46-247 keep the state as "lmps" left of the centre, "wave" at it and "rmps" right of it, at
+ * getfile(name, prefix, site, iroot)): one file per site plus the record getfile("dmrg", prefix, 0, iroot, 0) holding
+ * nsites and the centre. */
+int mpsxx_dmrg_save(mpsxx_dmrg d, const char* prefix, int iroot, void* stream);
+/* Restart: reads the record and the site files and builds every right environment of the MPO on the device.  A state
+ * saved with the centre at site 0 (after mpsxx_dmrg_create or a sweep) is used as saved; any other centre is brought
+ * to site 0 as mpsxx_dmrg_create does (SVDs from the right, truncating to max_states).  Settings (form, noise, lower
+ * roots) are not part of the state and start at their defaults. */
+int mpsxx_dmrg_load(int nsites, const mpsxx_tensor* mpo, const char* prefix, int iroot, int max_states,
+                    mpsxx_dmrg* out, void* stream);
 int mpsxx_dmrg_free(mpsxx_dmrg d);
 const char* mpsxx_dmrg_last_error(void);
 

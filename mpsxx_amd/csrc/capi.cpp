@@ -7,6 +7,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include "batch.hpp"
+#include "fileio.hpp"
 #include "mpsxx_amd.h"
 #include "noalloc.hpp"
 #include "ops.hpp"
@@ -226,6 +227,90 @@ int mpsxx_tensor_device_ptr(mpsxx_tensor t, double** dptr) {
 
 int mpsxx_tensor_copy(mpsxx_tensor x, mpsxx_tensor y, void* stream) {
   return guard([&] { mpx::op_copy(T(x), T(y), S(stream)); });
+}
+
+int mpsxx_qst_save(const char* path, const mpsxx_qst_desc* desc, const double* data, int64_t nelem) {
+  return guard_host([&] {
+    if (!path) throw mpx::Error(MPSXX_ERR_ARG, "qst_save: null path");
+    if (desc && desc->rank == 0) {  // an array with no structure
+      if (nelem != 0) throw mpx::Error(MPSXX_ERR_SHAPE, "qst_save: element count mismatch");
+      mpx::write_qst(path, nullptr, nullptr);
+      return;
+    }
+    const mpx::Layout L = layout_from_desc(desc);
+    if (nelem != L.nelem) throw mpx::Error(MPSXX_ERR_SHAPE, "qst_save: element count mismatch");
+    if (nelem > 0 && !data) throw mpx::Error(MPSXX_ERR_ARG, "qst_save: null data");
+    mpx::write_qst(path, &L, data);
+  });
+}
+
+int mpsxx_qst_file_info(const char* path, int32_t* rank, int32_t* qsym, int32_t q_total[2], int64_t* nsec_total,
+                        int64_t* nblocks, int64_t* nelem) {
+  return guard_host([&] {
+    if (!path) throw mpx::Error(MPSXX_ERR_ARG, "qst_file_info: null path");
+    const mpx::QstFile f = mpx::read_qst(path, false);
+    const mpx::Layout& L = f.L;
+    int64_t ns = 0;
+    for (int i = 0; i < L.rank; ++i) ns += L.nsec(i);
+    if (rank) *rank = f.empty ? 0 : L.rank;
+    if (qsym) *qsym = f.empty ? 0 : (L.qsym ? 1 : 0);
+    if (q_total) { q_total[0] = L.q.p; q_total[1] = L.q.s; }
+    if (nsec_total) *nsec_total = ns;
+    if (nblocks) *nblocks = L.nblocks();
+    if (nelem) *nelem = L.nelem;
+  });
+}
+
+int mpsxx_qst_file_read(const char* path, int32_t* nsec, int32_t* sec_q, int32_t* sec_dim, int32_t* block_idx,
+                        double* data, int64_t nelem) {
+  return guard_host([&] {
+    if (!path) throw mpx::Error(MPSXX_ERR_ARG, "qst_file_read: null path");
+    const mpx::QstFile f = mpx::read_qst(path, true);
+    const mpx::Layout& L = f.L;
+    if (nelem != L.nelem) throw mpx::Error(MPSXX_ERR_SHAPE, "qst_file_read: element count mismatch");
+    int64_t k = 0;
+    for (int i = 0; i < L.rank; ++i) {
+      if (nsec) nsec[i] = L.nsec(i);
+      for (int j = 0; j < L.nsec(i); ++j, ++k) {
+        if (sec_q) {
+          const mpx::QN q = L.qsym ? L.qs[i][j] : mpx::QN();
+          sec_q[2 * k] = q.p;
+          sec_q[2 * k + 1] = q.s;
+        }
+        if (sec_dim) sec_dim[k] = L.ds[i][j];
+      }
+    }
+    if (block_idx && !L.bidx.empty()) std::memcpy(block_idx, L.bidx.data(), L.bidx.size() * sizeof(int32_t));
+    if (L.nelem > 0) {
+      if (!data) throw mpx::Error(MPSXX_ERR_ARG, "qst_file_read: null data");
+      std::memcpy(data, f.data.data(), f.data.size() * sizeof(double));
+    }
+  });
+}
+
+int mpsxx_tensor_save(mpsxx_tensor t, const char* path, void* stream) {
+  return guard([&] {
+    if (!path) throw mpx::Error(MPSXX_ERR_ARG, "tensor_save: null path");
+    mpx::tensor_save(T(t), path, S(stream));
+  });
+}
+
+int mpsxx_tensor_load(const char* path, mpsxx_tensor t, void* stream) {
+  return guard([&] {
+    if (!path) throw mpx::Error(MPSXX_ERR_ARG, "tensor_load: null path");
+    mpx::tensor_load(path, T(t), S(stream));
+  });
+}
+
+int mpsxx_getfile(const char* name, const char* prefix, int site, int istate, int jstate, char* buf, int64_t cap,
+                  int64_t* len) {
+  return guard_host([&] {
+    if (!name || !prefix) throw mpx::Error(MPSXX_ERR_ARG, "getfile: null name or prefix");
+    const std::string f = mpx::getfile(name, prefix, site, istate, jstate);
+    if (len) *len = (int64_t)f.size();
+    if (!buf || cap < (int64_t)f.size() + 1) throw mpx::Error(MPSXX_ERR_ARG, "getfile: buffer too small");
+    std::memcpy(buf, f.c_str(), f.size() + 1);
+  });
 }
 
 int mpsxx_contract(double alpha, mpsxx_tensor a, int conj_a, const int32_t* contract_a, mpsxx_tensor b, int conj_b,

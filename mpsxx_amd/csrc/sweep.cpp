@@ -10,8 +10,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <string>
 #include <vector>
 
+#include "fileio.hpp"
 #include "mpsxx_amd.h"
 #include "ops.hpp"
 
@@ -113,7 +115,9 @@ int bond_dim(const Layout& L, int idx) {
 
 }  // namespace
 
-Dmrg* dmrg_create(int L, const std::vector<Tensor*>& mpo, const std::vector<Tensor*>& init, int M, hipStream_t s) {
+// right_canonical: init already has the centre at site 0 (B tensors right of it), as a saved sweep state has.
+Dmrg* dmrg_create(int L, const std::vector<Tensor*>& mpo, const std::vector<Tensor*>& init, int M, hipStream_t s,
+                  bool right_canonical = false) {
   if (L < 2 || (int)mpo.size() != L || (int)init.size() != L) throw Error(MPSXX_ERR_ARG, "dmrg: need L >= 2 sites");
   auto D = std::make_unique<Dmrg>();
   D->L = L;
@@ -125,7 +129,7 @@ Dmrg* dmrg_create(int L, const std::vector<Tensor*>& mpo, const std::vector<Tens
   }
   // right-canonicalise from the right end: psi = T_{i-1} T_i, RightArrow SVD -> (U s, B_i)
   // (make_random_mpss.cpp:204-251 does the same with one-site canonicalize(0, ...))
-  for (int i = L - 1; i >= 1; --i) {
+  for (int i = right_canonical ? 0 : L - 1; i >= 1; --i) {
     Tensor psi, U, B;
     twosite(*D->mps[i - 1], *D->mps[i], psi, s);
     op_canonicalize_twosite(false, psi, U, B, M, s);
@@ -145,6 +149,37 @@ Dmrg* dmrg_create(int L, const std::vector<Tensor*>& mpo, const std::vector<Tens
   D->center = 0;
   hip_check(hipStreamSynchronize(s), "dmrg init");
   return D.release();
+}
+
+constexpr char kDmrgRecord[9] = "MPSXXDMR";
+
+// The state as site files: "lmps" left of the centre, "wave" at it, "rmps" right of it (src/make_sweep.cpp:124-247,
+// src/make_random_mpss.cpp:155-217 name them so), then the record {nsites, centre}, written last.
+void dmrg_save(const Dmrg& D, const std::string& prefix, int iroot, hipStream_t s) {
+  for (int k = 0; k < D.L; ++k) {
+    const char* name = k < D.center ? "lmps" : (k == D.center ? "wave" : "rmps");
+    tensor_save(*D.mps[k], getfile(name, prefix, k, iroot, 0), s);
+  }
+  write_record(getfile("dmrg", prefix, 0, iroot, 0), kDmrgRecord, {D.L, D.center});
+}
+
+Dmrg* dmrg_load(int L, const std::vector<Tensor*>& mpo, const std::string& prefix, int iroot, int M, hipStream_t s) {
+  const std::vector<int32_t> rec = read_record(getfile("dmrg", prefix, 0, iroot, 0), kDmrgRecord, 2);
+  if (rec[0] != L)
+    throw Error(MPSXX_ERR_ARG, "dmrg_load: the saved state has " + std::to_string(rec[0]) + " sites, not " +
+                                   std::to_string(L));
+  const int center = rec[1];
+  if (center < 0 || center >= L) throw Error(MPSXX_ERR_IO, "dmrg_load: centre out of range in the state record");
+  std::vector<Tensor> site(L);
+  std::vector<Tensor*> p(L);
+  for (int k = 0; k < L; ++k) {
+    const char* name = k < center ? "lmps" : (k == center ? "wave" : "rmps");
+    const std::string f = getfile(name, prefix, k, iroot, 0);
+    tensor_load(f, site[k], s);
+    if (site[k].empty || site[k].L.rank != 3) throw Error(MPSXX_ERR_IO, f + ": not a 3-index site tensor");
+    p[k] = &site[k];
+  }
+  return dmrg_create(L, mpo, p, M, s, center == 0);
 }
 
 // One two-site step in the merged form (src/optimize.hpp:495-630, optimize_twosite_merged): the merged block
@@ -568,6 +603,29 @@ int mpsxx_dmrg_create(int nsites, const mpsxx_tensor* mpo, const mpsxx_tensor* i
     }
     auto* h = new mpsxx_dmrg_s;
     h->d = mpx::dmrg_create(nsites, m, i, max_states, reinterpret_cast<hipStream_t>(stream));
+    *out = h;
+  });
+}
+
+int mpsxx_dmrg_save(mpsxx_dmrg d, const char* prefix, int iroot, void* stream) {
+  return dguard([&] {
+    if (!d || !d->d || !prefix) throw mpx::Error(MPSXX_ERR_ARG, "dmrg_save: null argument");
+    mpx::dmrg_save(*d->d, prefix, iroot, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int mpsxx_dmrg_load(int nsites, const mpsxx_tensor* mpo, const char* prefix, int iroot, int max_states,
+                    mpsxx_dmrg* out, void* stream) {
+  return dguard([&] {
+    if (!mpo || !prefix || !out) throw mpx::Error(MPSXX_ERR_ARG, "dmrg_load: null argument");
+    std::vector<mpx::Tensor*> m;
+    for (int k = 0; k < nsites; ++k) {
+      if (!mpo[k]) throw mpx::Error(MPSXX_ERR_ARG, "dmrg_load: null mpo tensor");
+      m.push_back(&mpo[k]->t);
+    }
+    mpx::Dmrg* D = mpx::dmrg_load(nsites, m, prefix, iroot, max_states, reinterpret_cast<hipStream_t>(stream));
+    auto* h = new mpsxx_dmrg_s;
+    h->d = D;
     *out = h;
   });
 }

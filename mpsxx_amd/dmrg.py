@@ -189,6 +189,32 @@ class DeviceDMRG:
         if rc != 0:
             raise RuntimeError("mpsxx_dmrg_add_lower_root: " + lib().mpsxx_dmrg_last_error().decode())
 
+    def save(self, prefix, iroot=0):
+        """Write the state as the reference names it (src/make_sweep.cpp:124-247): "lmps" left of the centre, "wave"
+        at it, "rmps" right of it, each at getfile(name, prefix, site, iroot), plus the record
+        getfile("dmrg", prefix, 0, iroot) of nsites and the centre (``fileio.getfile``)."""
+        rc = lib().mpsxx_dmrg_save(self.h, str(prefix).encode(), ctypes.c_int(iroot), None)
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_save: " + lib().mpsxx_dmrg_last_error().decode())
+
+    @classmethod
+    def load(cls, mpos, prefix, M, iroot=0, form=SWEEP_FOURINDEX):
+        """Restart from a state written by ``save``: the saved site tensors, every right environment rebuilt on the
+        device.  Settings (noise, lower roots) start at their defaults."""
+        self = cls.__new__(cls)
+        self.h = ctypes.c_void_p()
+        self.L = len(mpos)
+        self._mpo = [w if isinstance(w, DTensor) else DTensor(w) for w in mpos]
+        arr_m = (ctypes.c_void_p * self.L)(*[t.h.value for t in self._mpo])
+        lib().mpsxx_dmrg_last_error.restype = ctypes.c_char_p
+        rc = lib().mpsxx_dmrg_load(ctypes.c_int(self.L), arr_m, str(prefix).encode(), ctypes.c_int(iroot),
+                                   ctypes.c_int(M), ctypes.byref(self.h), None)
+        if rc != 0:
+            raise RuntimeError("mpsxx_dmrg_load: " + lib().mpsxx_dmrg_last_error().decode())
+        if form != SWEEP_FOURINDEX:
+            self.set_form(form)
+        return self
+
     def mpo_gradient(self):
         """[<psi| dH/dW_k |psi>] for every site (DTensors; W_k's block indices with quantum numbers negated)."""
         grads = [DTensor() for _ in range(self.L)]
