@@ -417,6 +417,29 @@ int mpsxx_batch_set_policy(mpsxx_batch b, int32_t nsteps, const int32_t* policy)
 int mpsxx_batch_info(mpsxx_batch b, int64_t* flops, int64_t* bytes, int32_t* nsteps, int64_t* work_items);
 int mpsxx_batch_free(mpsxx_batch b);
 
+/* Batched Davidson: Davidson::diagonalize (src/diagonalize.hpp:283-449, one operator group per instance, one root)
+ * on n independent two-site problems in lockstep.  Each step applies every instance's sigma vector in one batched
+ * launch sequence (as mpsxx_batch_sigma_twosite_run, replayed from a hipGraph when use_graph != 0) and runs the dot
+ * products, Ritz rotations, residuals, preconditioner and Gram-Schmidt of all instances as single segmented launches.
+ * Per instance the result is mpsxx_davidson_twosite's on the same inputs up to rounding: wfnc[i] (the start vector)
+ * is overwritten by the normalized eigenvector, energies[i] receives the lowest eigenvalue, n_sigma[i] and
+ * n_sigma_low[i] the sigma vectors applied to the instance while it iterated (of them in FP32 / split-BF16x3).
+ * precision: MPSXX_FP64, or MPSXX_FP32 / MPSXX_BF16X3 (low-precision sigma vectors until ||r||^2 < 1e-9 / 1e-6,
+ * then FP64 refinement, as mpsxx_davidson_twosite).  diag[i] may be empty.  Output arrays may be NULL. */
+typedef struct {
+  int32_t batch_sigmas;      /* batched sigma applications (every instance at once) */
+  int32_t batch_sigmas_low;  /* of them in FP32 / split-BF16x3 */
+  int32_t macro_iterations;  /* Davidson restarts over both phases */
+  int32_t converged;         /* instances that reached ||r||^2 < tolerance in the FP64 phase */
+  double sigma_ms;           /* device time of the batched sigma applications */
+  double seconds;            /* wall time of the call */
+} mpsxx_batch_davidson_stats;
+int mpsxx_batch_davidson_twosite(int32_t n, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
+                                 const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, const mpsxx_tensor* diag,
+                                 const mpsxx_tensor* wfnc, double tolerance, int precision, int use_graph,
+                                 double* energies, int32_t* n_sigma, int32_t* n_sigma_low,
+                                 mpsxx_batch_davidson_stats* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

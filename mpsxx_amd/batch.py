@@ -71,6 +71,31 @@ class SigmaBatch:
             pass
 
 
+class BatchDavidsonStats(ctypes.Structure):
+    _fields_ = [("batch_sigmas", ctypes.c_int32), ("batch_sigmas_low", ctypes.c_int32),
+                ("macro_iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("sigma_ms", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
+def batch_davidson_twosite(instances, tol, precision=FP64, use_graph=True):
+    """Davidson (src/diagonalize.hpp:283-449) on n independent two-site problems in lockstep, one batched sigma launch
+    sequence and one set of segmented vector operations per step for all of them (mpsxx_batch_davidson_twosite).
+    instances: sequence of (lmpo, rmpo, lopr, ropr, diag, wfnc) DTensors; every wfnc (start vector) is overwritten by
+    its normalized eigenvector.  Returns (energies, n_sigma, n_sigma_low, stats dict)."""
+    inst = [tuple(x) for x in instances]
+    n = len(inst)
+    arr = [(VP * n)(*[inst[i][k].h for i in range(n)]) for k in range(6)]
+    e = np.zeros(n)
+    ns = np.zeros(n, dtype=np.int32)
+    nl = np.zeros(n, dtype=np.int32)
+    st = BatchDavidsonStats()
+    _check(lib().mpsxx_batch_davidson_twosite(ctypes.c_int32(n), *arr, ctypes.c_double(tol), ctypes.c_int(precision),
+                                              ctypes.c_int(int(use_graph)), e.ctypes.data_as(VP),
+                                              ns.ctypes.data_as(VP), nl.ctypes.data_as(VP), ctypes.byref(st), None))
+    stats = {k: getattr(st, k) for k, _ in BatchDavidsonStats._fields_}
+    return e, ns, nl, stats
+
+
 KERNELS = ["gemm_grouped_kernel", "gemm_small_kernel", "gemm_split_kernel", "axpy_grouped_kernel", "gemm_x3_kernel",
            "gemm128_kernel", "gemm_x3_128_kernel"]
 

@@ -217,6 +217,8 @@ void BatchSigma::check_structures(hipStream_t s) {
     if (y.empty) {
       y.set_layout(out, s);
     } else if (!y.L.same_structure(out)) {
+      if (!y.ptr() && y.L.nelem > 0)  // a structure-only output (set_vectors) cannot be laid out again
+        throw Error(MPSXX_ERR_SHAPE, "batch: an output without storage must have the batch's output structure");
       // the output may only have lost blocks (e.g. a caller-side clear and refill); every block must still exist
       if (!same_shape(y.L, out)) throw Error(MPSXX_ERR_QN, "batch compute_sigma_vector: output quantum shape mismatch");
       for (int64_t b = 0; b < y.L.nblocks(); ++b)
@@ -227,6 +229,22 @@ void BatchSigma::check_structures(hipStream_t s) {
   }
 }
 
+void BatchSigma::set_vectors(const std::vector<const double*>& wfn, const std::vector<double*>& sgv) {
+  if ((!wfn.empty() && (int)wfn.size() != n_) || (!sgv.empty() && (int)sgv.size() != n_))
+    throw Error(MPSXX_ERR_ARG, "batch: one wavefunction and one sigma array per instance expected");
+  vwfn_ = wfn;
+  vsgv_ = sgv;
+}
+
+const double* BatchSigma::input_ptr(int i, int idx) const {
+  if (idx == 4 && !vwfn_.empty()) return vwfn_[i];
+  const Operand6& o = inst_[i];
+  const Tensor* in[5] = {o.lmpo, o.rmpo, o.lopr, o.ropr, o.wfn};
+  return in[idx]->ptr();
+}
+
+double* BatchSigma::output_ptr(int i) const { return vsgv_.empty() ? inst_[i].sgv->ptr() : vsgv_[i]; }
+
 void BatchSigma::upload_tables(hipStream_t s, const std::vector<void*>* out_override) {
   std::vector<char> buf(table_bytes_);
   void** T = reinterpret_cast<void**>(buf.data());
@@ -234,14 +252,12 @@ void BatchSigma::upload_tables(hipStream_t s, const std::vector<void*>* out_over
     const size_t nm = g.members.size();
     for (size_t m = 0; m < nm; ++m) {
       const int i = g.members[m];
-      const Operand6& o = inst_[i];
-      Tensor* in[5] = {o.lmpo, o.rmpo, o.lopr, o.ropr, o.wfn};
       const Plan& P = plans_[i];
       for (int j = 0; j < g.nslots; ++j) {
         void* p;
         const int si = P.slot_input[j];
-        if (si >= 0) p = P.slot_prec[j] == 1 ? P.slot_buf[j].p : (void*)in[si]->ptr();
-        else if (si == -2) p = out_override ? (*out_override)[i] : (void*)o.sgv->ptr();
+        if (si >= 0) p = P.slot_prec[j] == 1 ? P.slot_buf[j].p : (void*)input_ptr(i, si);
+        else if (si == -2) p = out_override ? (*out_override)[i] : (void*)output_ptr(i);
         else p = P.slot_buf[j].p;
         T[g.table_off + (size_t)j * nm + m] = p;
       }
@@ -250,11 +266,9 @@ void BatchSigma::upload_tables(hipStream_t s, const std::vector<void*>* out_over
   ConvSeg* segs = reinterpret_cast<ConvSeg*>(buf.data() + table_ptrs_ * sizeof(void*));
   for (size_t g = 0; g < seg_inst_.size(); ++g) {
     const int i = seg_inst_[g], j = seg_slot_[g];
-    const Operand6& o = inst_[i];
-    Tensor* in[5] = {o.lmpo, o.rmpo, o.lopr, o.ropr, o.wfn};
     const Plan& P = plans_[i];
     ConvSeg sg;
-    sg.src = in[P.slot_input[j]]->ptr();
+    sg.src = input_ptr(i, P.slot_input[j]);
     sg.dst = P.slot_buf[j].as<float>();
     sg.n = P.slot_layout[j].nelem;
     std::memcpy(&segs[g], &sg, sizeof(sg));

@@ -29,6 +29,11 @@ class BatchSigma {
   // convert_mask (bit 0 lmpo, 1 rmpo, 2 lopr, 3 ropr, 4 wfn); use_graph replays a captured graph of the launches.
   void run(unsigned convert_mask, bool use_graph, hipStream_t s);
 
+  // Redirect instance i's wfn input to wfn[i] and its sgv output to sgv[i]: device arrays laid out as that instance's
+  // wfn and sgv tensors, which then only describe the structure (a batched Davidson moves its trial and sigma vectors
+  // from call to call).  The pointer tables follow at the next run; empty lists switch the redirection off.
+  void set_vectors(const std::vector<const double*>& wfn, const std::vector<double*>& sgv);
+
   // Average device time of one run over `reps` runs (HIP events on s).
   double time(int reps, unsigned convert_mask, bool use_graph, hipStream_t s);
   // Device time and algorithmic cost of every kernel launch (outputs redirected to scratch).
@@ -94,7 +99,36 @@ class BatchSigma {
   std::map<unsigned, hipGraphExec_t> graphs_;
   std::map<unsigned, hipGraph_t> graph_src_;
   int64_t flops_ = 0, bytes_ = 0;
+  std::vector<const double*> vwfn_;  // set_vectors redirections (empty: the tensors' own buffers)
+  std::vector<double*> vsgv_;
+  const double* input_ptr(int i, int idx) const;
+  double* output_ptr(int i) const;
 };
+
+// One instance of a batched Davidson solve: the two-site problem of compute_sigma_vector (one operator group), the
+// diagonal from compute_diagonal_elements (may be empty), and wfnc, the start vector, overwritten by the normalized
+// eigenvector on the instance's Davidson layout.
+struct BatchDavInstance {
+  Tensor *lmpo, *rmpo, *lopr, *ropr, *diag, *wfnc;
+};
+struct BatchDavidsonStats {
+  std::vector<double> energy;    // lowest Ritz value of each instance at its convergence (or last step)
+  std::vector<double> rnorm;     // its last ||r||^2
+  std::vector<int> nsigma;       // sigma vectors applied to the instance while it was iterating
+  std::vector<int> nsigma_low;   // of them in FP32 / split-BF16x3
+  std::vector<int> converged;    // ||r||^2 < tol reached in the final (FP64) phase
+  int batch_sigmas = 0;          // batched applications (every instance at once)
+  int batch_sigmas_low = 0;
+  int macro_iterations = 0;
+  double sigma_ms = 0.0;         // device time of the batched applications
+  double seconds = 0.0;          // wall time of the whole solve
+};
+// Davidson (src/diagonalize.hpp:283-449) on every instance at once, in lockstep: one batched sigma application and
+// one set of segmented vector operations per step over all instances; precisions as davidson_twosite (FP32 or
+// split-BF16x3 sigma vectors until their noise floor, then FP64 refinement).  use_graph replays the sigma launches
+// from a hipGraph.
+void batch_davidson_twosite(std::vector<BatchDavInstance>& inst, double tol, int prec, bool use_graph,
+                            BatchDavidsonStats& st, hipStream_t s);
 
 }  // namespace mpx
 

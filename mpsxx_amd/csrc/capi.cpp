@@ -821,6 +821,36 @@ int mpsxx_batch_sigma_twosite_create(int32_t n, const mpsxx_tensor* lmpo, const 
   });
 }
 
+int mpsxx_batch_davidson_twosite(int32_t n, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
+                                 const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, const mpsxx_tensor* diag,
+                                 const mpsxx_tensor* wfnc, double tolerance, int precision, int use_graph,
+                                 double* energies, int32_t* n_sigma, int32_t* n_sigma_low,
+                                 mpsxx_batch_davidson_stats* stats, void* stream) {
+  return guard([&] {
+    if (n <= 0 || !lmpo || !rmpo || !lopr || !ropr || !diag || !wfnc)
+      throw mpx::Error(MPSXX_ERR_ARG, "batch Davidson: bad arguments");
+    std::vector<mpx::BatchDavInstance> inst(n);
+    for (int i = 0; i < n; ++i)
+      inst[i] = {&T(lmpo[i]), &T(rmpo[i]), &T(lopr[i]), &T(ropr[i]), &T(diag[i]), &T(wfnc[i])};
+    mpx::BatchDavidsonStats st;
+    mpx::batch_davidson_twosite(inst, tolerance, precision, use_graph != 0, st, S(stream));
+    for (int i = 0; i < n; ++i) {
+      if (energies) energies[i] = st.energy[i];
+      if (n_sigma) n_sigma[i] = st.nsigma[i];
+      if (n_sigma_low) n_sigma_low[i] = st.nsigma_low[i];
+    }
+    if (stats) {
+      stats->batch_sigmas = st.batch_sigmas;
+      stats->batch_sigmas_low = st.batch_sigmas_low;
+      stats->macro_iterations = st.macro_iterations;
+      stats->converged = 0;
+      for (int c : st.converged) stats->converged += c;
+      stats->sigma_ms = st.sigma_ms;
+      stats->seconds = st.seconds;
+    }
+  });
+}
+
 int mpsxx_batch_sigma_twosite_run(mpsxx_batch b, uint32_t convert_mask, int use_graph, void* stream) {
   return guard([&] { B(b).run(convert_mask, use_graph != 0, S(stream)); });
 }

@@ -600,6 +600,128 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restri
   if (threadIdx.x == 0) G[pair] = acc;
 }
 
+// ---- batched Davidson: segmented vector operations (one work-group per chunk of one instance's segment) ----------
+__global__ __launch_bounds__(256) void seg_gram_partial_kernel(const SegChunk* __restrict__ ch,
+                                                               const double* __restrict__ X, int64_t ldx,
+                                                               const double* __restrict__ Y, int64_t ldy, int my,
+                                                               double* partial, int nch) {
+  const SegChunk c = ch[blockIdx.x];
+  const int pair = blockIdx.y;
+  const double* x = X + (pair / my) * ldx + c.begin;
+  const double* y = Y + (pair % my) * ldy + c.begin;
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < c.len; e += 256) acc += x[e] * y[e];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[(int64_t)pair * nch + blockIdx.x] = acc;
+}
+
+// grid (ninst, npairs): instance i sums the partials of its chunks in chunk order
+__global__ __launch_bounds__(256) void seg_reduce_kernel(const double* __restrict__ partial, int nch,
+                                                         const int32_t* __restrict__ chunk0, double* G, int gstride) {
+  const int inst = blockIdx.x, pair = blockIdx.y;
+  const int c0 = chunk0[inst], c1 = chunk0[inst + 1];
+  double acc = 0.0;
+  for (int c = c0 + threadIdx.x; c < c1; c += 256) acc += partial[(int64_t)pair * nch + c];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) G[(int64_t)inst * gstride + pair] = acc;
+}
+
+__global__ __launch_bounds__(256) void seg_residual_partial_kernel(const SegChunk* __restrict__ ch,
+                                                                   const double* __restrict__ s0,
+                                                                   const double* __restrict__ t0,
+                                                                   const double* __restrict__ e,
+                                                                   double* __restrict__ r, double* partial) {
+  const SegChunk c = ch[blockIdx.x];
+  const double ev = e[c.inst];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < c.len; k += 256) {
+    const int64_t i = c.begin + k;
+    const double v = s0[i] - ev * t0[i];
+    r[i] = v;
+    acc += v * v;
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void seg_rotate2_kernel(const SegChunk* __restrict__ ch,
+                                                          const double* __restrict__ T, const double* __restrict__ S,
+                                                          int64_t ld, int m, const double* __restrict__ cf, int cld,
+                                                          int cstride, double* __restrict__ TO,
+                                                          double* __restrict__ SO) {
+  const SegChunk sc = ch[blockIdx.x];
+  const double* __restrict__ c = cf + (int64_t)sc.inst * cstride;
+  for (int k0 = threadIdx.x; k0 < sc.len; k0 += 256) {
+    const int64_t i = sc.begin + k0;
+    double t[kRotMax], sv[kRotMax];
+#pragma unroll
+    for (int k = 0; k < kRotMax; ++k) {
+      t[k] = k < m ? T[k * ld + i] : 0.0;
+      sv[k] = k < m ? S[k * ld + i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kRotMax; ++j) {
+      if (j < m) {
+        double at = 0.0, as = 0.0;
+#pragma unroll
+        for (int k = 0; k < kRotMax; ++k) {
+          const double w = k < m ? c[j * cld + k] : 0.0;
+          at += w * t[k];
+          as += w * sv[k];
+        }
+        TO[j * ld + i] = at;
+        SO[j * ld + i] = as;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void seg_combine_sub_kernel(const SegChunk* __restrict__ ch,
+                                                              const double* __restrict__ base, int64_t ld, int m,
+                                                              const double* __restrict__ cf, int cstride,
+                                                              double* __restrict__ y) {
+  const SegChunk sc = ch[blockIdx.x];
+  const double* __restrict__ c = cf + (int64_t)sc.inst * cstride;
+  for (int k0 = threadIdx.x; k0 < sc.len; k0 += 256) {
+    const int64_t i = sc.begin + k0;
+    double acc = 0.0;
+    for (int j = 0; j < m; ++j) acc += c[j] * base[j * ld + i];
+    y[i] -= acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void seg_scale_rnorm_kernel(const SegChunk* __restrict__ ch, double* x,
+                                                              const double* __restrict__ nrm2, int stride) {
+  const SegChunk sc = ch[blockIdx.x];
+  const double v = nrm2[(int64_t)sc.inst * stride], f = v > 0.0 ? 1.0 / sqrt(v) : 0.0;
+  for (int k = threadIdx.x; k < sc.len; k += 256) x[sc.begin + k] *= f;
+}
+
+__global__ __launch_bounds__(256) void seg_precondition_kernel(const SegChunk* __restrict__ ch, double* r,
+                                                               const double* __restrict__ d,
+                                                               const unsigned char* __restrict__ pres,
+                                                               const double* __restrict__ eval) {
+  const SegChunk sc = ch[blockIdx.x];
+  const double ev = eval[sc.inst];
+  for (int k = threadIdx.x; k < sc.len; k += 256) {
+    const int64_t i = sc.begin + k;
+    if (pres[i]) {
+      double denm = ev - d[i];
+      if (fabs(denm) < 1.0e-8) denm = 1.0e-8;
+      r[i] /= denm;
+    } else {
+      r[i] *= 1.0 / ev;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void seg_copy_kernel(const SegChunk* __restrict__ ch,
+                                                       const double* __restrict__ src, double* __restrict__ dst,
+                                                       const int32_t* __restrict__ mask) {
+  const SegChunk sc = ch[blockIdx.x];
+  if (mask && mask[sc.inst] == 0) return;
+  for (int k = threadIdx.x; k < sc.len; k += 256) dst[sc.begin + k] = src[sc.begin + k];
+}
 
 static inline int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
@@ -1387,6 +1509,49 @@ void launch_gram(const double* X, int64_t ldx, int mx, const double* Y, int64_t 
   hipLaunchKernelGGL(gram_partial_kernel, dim3(npart, mx * my), dim3(256), 0, s, X, ldx, Y, ldy, my, n, partial,
                      npart);
   hipLaunchKernelGGL(gram_reduce_kernel, dim3(mx * my), dim3(256), 0, s, partial, npart, G);
+}
+
+void launch_seg_gram(const SegChunk* ch, int nch, const int32_t* chunk0, int ninst, const double* X, int64_t ldx,
+                     int mx, const double* Y, int64_t ldy, int my, double* G, int gstride, double* partial,
+                     hipStream_t s) {
+  if (nch <= 0 || ninst <= 0 || mx <= 0 || my <= 0) return;
+  hipLaunchKernelGGL(seg_gram_partial_kernel, dim3(nch, mx * my), dim3(256), 0, s, ch, X, ldx, Y, ldy, my, partial,
+                     nch);
+  hipLaunchKernelGGL(seg_reduce_kernel, dim3(ninst, mx * my), dim3(256), 0, s, partial, nch, chunk0, G, gstride);
+}
+
+void launch_seg_residual(const SegChunk* ch, int nch, const int32_t* chunk0, int ninst, const double* s0,
+                         const double* t0, const double* e, double* r, double* G, int gstride, double* partial,
+                         hipStream_t s) {
+  if (nch <= 0 || ninst <= 0) return;
+  hipLaunchKernelGGL(seg_residual_partial_kernel, dim3(nch), dim3(256), 0, s, ch, s0, t0, e, r, partial);
+  hipLaunchKernelGGL(seg_reduce_kernel, dim3(ninst, 1), dim3(256), 0, s, partial, nch, chunk0, G, gstride);
+}
+
+void launch_seg_rotate2(const SegChunk* ch, int nch, const double* T, const double* S, int64_t ld, int m,
+                        const double* c, int cld, int cstride, double* TO, double* SO, hipStream_t s) {
+  if (nch > 0 && m > 0)
+    hipLaunchKernelGGL(seg_rotate2_kernel, dim3(nch), dim3(256), 0, s, ch, T, S, ld, m, c, cld, cstride, TO, SO);
+}
+
+void launch_seg_combine_sub(const SegChunk* ch, int nch, const double* base, int64_t ld, int m, const double* c,
+                            int cstride, double* y, hipStream_t s) {
+  if (nch > 0 && m > 0)
+    hipLaunchKernelGGL(seg_combine_sub_kernel, dim3(nch), dim3(256), 0, s, ch, base, ld, m, c, cstride, y);
+}
+
+void launch_seg_scale_rnorm(const SegChunk* ch, int nch, double* x, const double* nrm2, int stride, hipStream_t s) {
+  if (nch > 0) hipLaunchKernelGGL(seg_scale_rnorm_kernel, dim3(nch), dim3(256), 0, s, ch, x, nrm2, stride);
+}
+
+void launch_seg_precondition(const SegChunk* ch, int nch, double* r, const double* diag, const unsigned char* present,
+                             const double* eval, hipStream_t s) {
+  if (nch > 0) hipLaunchKernelGGL(seg_precondition_kernel, dim3(nch), dim3(256), 0, s, ch, r, diag, present, eval);
+}
+
+void launch_seg_copy(const SegChunk* ch, int nch, const double* src, double* dst, const int32_t* mask,
+                     hipStream_t s) {
+  if (nch > 0) hipLaunchKernelGGL(seg_copy_kernel, dim3(nch), dim3(256), 0, s, ch, src, dst, mask);
 }
 
 template <typename T, typename TC>

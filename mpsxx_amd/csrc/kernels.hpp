@@ -261,4 +261,38 @@ void launch_rotate2(const double* T, const double* S, int64_t ld, int m, const d
 void launch_residual(const double* s0, const double* t0, double e, double* r, int64_t n, double* G, double* partial,
                      int npart, hipStream_t s);
 
+// Davidson helpers over many independent vector spaces at once (batched Davidson): every instance owns one contiguous
+// segment of a batch vector, cut into chunks of at most kSegChunk elements (one work-group per chunk).  Per-instance
+// scalars (eigenvalues, norms, coefficient matrices) are read from device arrays indexed by the chunk's instance.
+// Reductions are deterministic: one partial sum per (chunk, pair), then each instance sums its chunks in order.
+constexpr int kSegChunk = 4096;
+struct SegChunk {
+  int64_t begin;  // first element (batch-vector offset)
+  int32_t len;    // elements, 1..kSegChunk
+  int32_t inst;
+};
+// G[i * gstride + a * my + b] = <X_a, Y_b> over instance i's segment, for a < mx, b < my; chunk0[i]..chunk0[i+1]-1 are
+// the chunks of instance i (partial: nchunks * mx * my doubles)
+void launch_seg_gram(const SegChunk* ch, int nch, const int32_t* chunk0, int ninst, const double* X, int64_t ldx,
+                     int mx, const double* Y, int64_t ldy, int my, double* G, int gstride, double* partial,
+                     hipStream_t s);
+// r = s0 - e[i] t0 on instance i's segment and G[i * gstride] = <r, r>
+void launch_seg_residual(const SegChunk* ch, int nch, const int32_t* chunk0, int ninst, const double* s0,
+                         const double* t0, const double* e, double* r, double* G, int gstride, double* partial,
+                         hipStream_t s);
+// Ritz rotation of both vector sets with instance i's coefficients c + i * cstride (TO_j = sum_k c[j*cld + k] T_k)
+void launch_seg_rotate2(const SegChunk* ch, int nch, const double* T, const double* S, int64_t ld, int m,
+                        const double* c, int cld, int cstride, double* TO, double* SO, hipStream_t s);
+// y -= sum_a c[i * cstride + a] X_a on instance i's segment
+void launch_seg_combine_sub(const SegChunk* ch, int nch, const double* base, int64_t ld, int m, const double* c,
+                            int cstride, double* y, hipStream_t s);
+// x *= 1 / sqrt(nrm2[i * stride]) (0 when it is 0) on instance i's segment
+void launch_seg_scale_rnorm(const SegChunk* ch, int nch, double* x, const double* nrm2, int stride, hipStream_t s);
+// the reference preconditioner with instance i's eigenvalue eval[i]
+void launch_seg_precondition(const SegChunk* ch, int nch, double* r, const double* diag, const unsigned char* present,
+                             const double* eval, hipStream_t s);
+// dst = src on the segments of the instances with mask[i] != 0 (mask == nullptr: every instance)
+void launch_seg_copy(const SegChunk* ch, int nch, const double* src, double* dst, const int32_t* mask,
+                     hipStream_t s);
+
 }  // namespace mpx

@@ -11,6 +11,12 @@ N(0,1)/sqrt(m) in every quantum-number-allowed block.
 two middle MPO sites, sector dimensions as above (centre N*k/norb, capacities of k sites), random entries; merged
 operators and wavefunction built on the device.
 
+``hubbard_isometric_twosite(L, M, U)``: the same middle two-site problem with the environments of a random MPS whose
+tensors left of the centre are left isometries and right of it right isometries (bond sectors as above, capped so that
+every tensor can be an isometry), renormalized on the device: H_eff = P^T H P for an isometry P, a Hermitian problem
+whose lowest eigenvalue is a variational energy of the chain, as every Davidson solve of a sweep sees it;
+``hubbard_isometric_problem`` adds the diagonal and a start vector near the lowest eigenvector.
+
 ``batch_specs(n, seed)``: the (L, U, M) sample of a batch of independent chains: L uniform over the even lengths
 16..64 (half filling with 2Sz = 0 needs N = L even), U from {2, 4, 8}, M from {128, 256, 512}.
 """
@@ -122,6 +128,145 @@ def hubbard_twosite(L=32, M=512, U=4.0, t=1.0, seed=0, width=1.5):
     renv = fill(QST((0, 0), [[(-a, -b) for a, b in rq], qwr, rq], [rd, [1] * len(qwr), rd]))
     psi = fill(mps.twosite_wavefunction(lq, ld, rq, rd, qt))
     return mpos[i], mpos[i + 1], lenv, renv, psi
+
+
+def _isometry_columns(nr, d, rng, score):
+    """An nr x d matrix with orthonormal columns: the QR factor of a Gaussian matrix (score None), or d distinct unit
+    columns at the rows of lowest score (ties in random order), i.e. a selection of basis states.  Returns (Q, the
+    scores of the d columns)."""
+    if score is None:
+        return np.linalg.qr(rng.standard_normal((nr, d)))[0], None
+    order = sorted(range(nr), key=lambda r: (score[r], rng.random()))[:d]
+    q = np.zeros((nr, d))
+    q[order, np.arange(d)] = 1.0
+    return q, [score[r] for r in order]
+
+
+def _left_isometry(lq, ld, rq, rd, rng, scores=None):
+    """Left-isometric A(l, n, r) (quantum shape (lq, site, -rq)): for every right sector the stacked (l, n) rows of
+    its blocks have orthonormal columns.  With scores (per left sector, per state: the doubly occupied sites of the
+    configuration it stands for), the columns select the configurations with the fewest doubly occupied sites; the
+    new states' scores are returned with the tensor."""
+    from .mpo import SITE_Q
+    a = mps.site_tensor(lq, ld, rq, rd)
+    out = []
+    for b, qr in enumerate(rq):
+        rows = [(x, n) for x, ql in enumerate(lq) for n, qn in enumerate(SITE_Q)
+                if (ql[0] + qn[0], ql[1] + qn[1]) == qr]
+        nr = sum(ld[x] for x, _ in rows)
+        sc = None if scores is None else [scores[x][j] + (n == 3) for x, n in rows for j in range(ld[x])]
+        q, s = _isometry_columns(nr, rd[b], rng, sc)
+        out.append(s)
+        r0 = 0
+        for x, n in rows:
+            a.blocks[(x, n, b)] = q[r0:r0 + ld[x]].reshape(ld[x], 1, rd[b])
+            r0 += ld[x]
+    return a, out
+
+
+def _right_isometry(lq, ld, rq, rd, rng, scores=None):
+    """Right-isometric B(l, n, r) (quantum shape (lq, site, -rq)): for every left sector the (n, r) columns of its
+    blocks have orthonormal rows; scores per right sector as in _left_isometry."""
+    from .mpo import SITE_Q
+    bt = mps.site_tensor(lq, ld, rq, rd)
+    out = []
+    for x, ql in enumerate(lq):
+        cols = [(n, b) for n, qn in enumerate(SITE_Q) for b, qr in enumerate(rq)
+                if (ql[0] + qn[0], ql[1] + qn[1]) == qr]
+        nc = sum(rd[b] for _, b in cols)
+        sc = None if scores is None else [scores[b][j] + (n == 3) for n, b in cols for j in range(rd[b])]
+        q, s = _isometry_columns(nc, ld[x], rng, sc)
+        out.append(s)
+        c0 = 0
+        for n, b in cols:
+            bt.blocks[(x, n, b)] = q[c0:c0 + rd[b]].T.reshape(ld[x], 1, rd[b])
+            c0 += rd[b]
+    return bt, out
+
+
+def isometric_chain(L=32, M=512, seed=0, width=1.5, basis="configurations"):
+    """Host tensors of an MPS around the middle two-site centre (i, i+1), i = L/2 - 1, at half filling and 2Sz = 0:
+    left isometries A_0..A_{i-1}, right isometries B_{i+2}..B_{L-1} (dict by site) and a random two-site psi.  Bond
+    sectors as hubbard_twosite's, each capped by the states the bond's neighbour can supply, so that every tensor can
+    be an isometry.  basis "configurations": every bond state is one occupation configuration of its side, those with
+    the fewest doubly occupied sites kept (P^T H P is then H on a selected set of configurations, whose diagonal is a
+    working Davidson preconditioner); "random": Gaussian isometries.  Returns (A, B, psi)."""
+    if basis not in ("configurations", "random"):
+        raise ValueError("basis: 'configurations' or 'random'")
+    from .mpo import SITE_Q
+    rng = np.random.default_rng(seed)
+    qt = (L, 0)
+    i = L // 2 - 1
+    lq, ld = [[(0, 0)]], [[1]]
+    for k in range(1, i + 1):
+        q = mps.left_bond_qns(L, k, qt)
+        want = gaussian_dims(q, (k * qt[0] / L, 0), M, width, caps=[sector_capacity(k, x) for x in q])
+        avail = {}
+        for ql, d in zip(lq[-1], ld[-1]):
+            for qn in SITE_Q:
+                key = (ql[0] + qn[0], ql[1] + qn[1])
+                avail[key] = avail.get(key, 0) + d
+        q2, d2 = _prune(q, [min(w, avail.get(x, 0)) for x, w in zip(q, want)])
+        lq.append(q2)
+        ld.append(d2)
+    rq, rd = {L: [(0, 0)]}, {L: [1]}
+    for k in range(L - 1, i + 1, -1):
+        q = mps.right_bond_qns(L, k, qt)
+        want = gaussian_dims(q, (-(L - k) * qt[0] / L, 0), M, width,
+                             caps=[sector_capacity(L - k, (-x[0], -x[1])) for x in q])
+        avail = {}
+        for qr, d in zip(rq[k + 1], rd[k + 1]):
+            for qn in SITE_Q:
+                key = (qr[0] - qn[0], qr[1] - qn[1])
+                avail[key] = avail.get(key, 0) + d
+        rq[k], rd[k] = _prune(q, [min(w, avail.get(x, 0)) for x, w in zip(q, want)])
+    sel = basis == "configurations"
+    A, sc = [], [[0]] if sel else None
+    for k in range(i):
+        a, sc = _left_isometry(lq[k], ld[k], lq[k + 1], ld[k + 1], rng, sc)
+        A.append(a)
+    B, sc = {}, [[0]] if sel else None
+    for k in range(L - 1, i + 1, -1):
+        B[k], sc = _right_isometry(rq[k], rd[k], rq[k + 1], rd[k + 1], rng, sc)
+    psi = mps.fill_allowed(mps.twosite_wavefunction(lq[i], ld[i], rq[i + 2], rd[i + 2], qt), rng)
+    return A, B, psi
+
+
+def hubbard_isometric_twosite(L=32, M=512, U=4.0, t=1.0, seed=0, width=1.5, basis="configurations"):
+    """Device tensors (lmpo, rmpo, lenv, renv, psi) of the middle two-site problem of the open Hubbard chain with the
+    environments of isometric_chain renormalized on the device (src/renormalize.hpp:10-32); see the module
+    docstring."""
+    from . import DTensor, renormalize
+    i = L // 2 - 1
+    mpos = hubbard_mpo(L, t, U)
+    A, B, psi = isometric_chain(L, M, seed, width, basis)
+    lenv = DTensor(mps.boundary_env())
+    for k in range(i):
+        a = DTensor(A[k])
+        lenv = renormalize(True, DTensor(mpos[k]), lenv, a, a)
+    renv = DTensor(mps.boundary_env())
+    for k in range(L - 1, i + 1, -1):
+        b = DTensor(B[k])
+        renv = renormalize(False, DTensor(mpos[k]), renv, b, b)
+    return DTensor(mpos[i]), DTensor(mpos[i + 1]), lenv, renv, DTensor(psi)
+
+
+def hubbard_isometric_problem(L=32, M=512, U=4.0, t=1.0, seed=0, width=1.5):
+    """hubbard_isometric_twosite plus what a Davidson solve takes: (lmpo, rmpo, lenv, renv, diag, psi) with the
+    diagonal of H_eff (src/compute_diagonal_elements.hpp:35-61) and a start vector in the basin of the lowest
+    eigenvector, psi = x / (diag - min(diag) + 1/2) for the random x, the role the previous step's wavefunction plays
+    in a sweep (the reference's Davidson, restarting from a single Ritz vector with a diagonal preconditioner, can
+    stall on a random start in these random subspaces)."""
+    from . import DTensor, compute_diagonal_elements_twosite
+    lm, rm, le, re_, psi = hubbard_isometric_twosite(L, M, U, t, seed, width)
+    x = psi.to_qst()
+    diag = compute_diagonal_elements_twosite(lm, rm, le, re_, DTensor(x.empty_like()))
+    d = diag.to_qst()
+    dmin = min([0.0] + [float(np.min(b)) for b in d.blocks.values() if b.size])
+    for k, b in x.blocks.items():
+        dk = d.blocks.get(k)  # a block absent from the diagonal has a zero diagonal
+        x.blocks[k] = b / ((dk if dk is not None else 0.0) - dmin + 0.5)
+    return lm, rm, le, re_, diag, DTensor(x)
 
 
 def batch_specs(n=64, seed=4):
