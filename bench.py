@@ -19,7 +19,9 @@ mixed precision (FP32 or split-BF16x3 sigma vectors, FP64 Davidson, refinement a
 dgemm in numpy's bundled OpenBLAS, on one core and on all cores given to the process, over a bounded sample of the
 headline sigma (the terms of every k-th sector of the MPO bond index; sigma is a sum over that index), compared
 with the device's FP64 result on the same sample.  Secondary legs: the two-site sigma of the Hubbard chain L = 32,
-M = 512; 64 independent Hubbard chains as one grouped launch; two-site sweeps of the Hubbard chain L = 32.
+M = 512; 64 independent Hubbard chains as one grouped launch; the ground-state problems of the same 64 chains solved
+by one batched lockstep Davidson (FP64, and FP32 / split-BF16x3 with FP64 refinement) beside the one-by-one FP64
+Davidson, with per-instance |E - E_fp64|; two-site sweeps of the Hubbard chain L = 32.
 """
 import argparse
 import ctypes
@@ -366,6 +368,92 @@ def batched_leg(n, reps):
     return res
 
 
+def batched_davidson_leg(n, tol=1e-10, max_calls=20):
+    """Config 4 as a ground-state problem: the middle two-site problem of each of the n chains of the batched leg,
+    environments from isometric configuration-selected MPS (workloads.hubbard_isometric_problem), Davidson to
+    ||r||^2 < tol in FP64 instance by instance (mpsxx_davidson_twosite, the plain FP64 GPU path) and as one batched
+    lockstep solve (mpsxx_batch_davidson_twosite) in FP64, FP32 + FP64 refinement and split-BF16x3 + FP64 refinement.
+    Each solve is repeated from its own result until converged (at most max_calls times; one call is the reference's
+    <= 4 x 10 sigma vectors), as successive sweeps restart it; later batched calls take only the instances still above
+    the tolerance.  Reports per-instance |E - E_fp64| over the instances converged in both runs and the sigma vectors
+    per second of each form (sigma vectors applied to instances still iterating / wall time)."""
+    from mpsxx_amd.batch import batch_davidson_twosite
+    specs = workloads.batch_specs(n, 4)
+    t0 = time.perf_counter()
+    probs = [workloads.hubbard_isometric_problem(L, M, U, seed=k) for k, (L, U, M) in enumerate(specs)]
+    mx.synchronize()
+    setup_s = time.perf_counter() - t0
+    flops = []
+    for p in probs:
+        fl = ctypes.c_int64()
+        mx._check(mx.lib().mpsxx_sigma_twosite_stats(*[x.h for x in p[:4]], p[5].h, ctypes.c_int(mx.FP64),
+                                                     ctypes.byref(fl), None, None))
+        flops.append(fl.value)
+    flops = np.array(flops, dtype=np.float64)
+    res = dict(instances=n, sample_seed=4, tol=tol, setup_s=setup_s,
+               environments="isometric, configuration-selected (fewest doubly occupied sites), renormalized on the "
+                            "device; start vector x / (diag - min diag + 1/2)")
+    # plain FP64 path, one instance after another
+    e_single, ns_single = np.zeros(n), np.zeros(n, dtype=np.int64)
+    mx.synchronize()
+    t0 = time.perf_counter()
+    for k, p in enumerate(probs):
+        w = mx.copy(p[5])
+        e_prev = None
+        for _ in range(max_calls):
+            e, ns = mx.davidson_twosite([p[:4]], p[4], w, tol, mx.FP64)
+            ns_single[k] += ns
+            if e_prev is not None and abs(e - e_prev) < 1e-13:
+                break
+            e_prev = e
+        e_single[k] = e
+        del w
+    mx.synchronize()
+    sec = time.perf_counter() - t0
+    res["single_fp64"] = dict(seconds=sec, sigmas=int(ns_single.sum()), sigma_per_s=float(ns_single.sum()) / sec,
+                              tflops=float((ns_single * flops).sum()) / sec / 1e12)
+    e64 = None
+    for prec in (mx.FP64, mx.FP32, mx.BF16X3):
+        print("batched davidson: %s" % PNAME[prec], file=sys.stderr, flush=True)
+        ws = [mx.copy(p[5]) for p in probs]
+        e, rn = np.zeros(n), np.full(n, np.inf)
+        ns_tot, nl_tot = np.zeros(n, dtype=np.int64), np.zeros(n, dtype=np.int64)
+        act, calls, sig_ms = list(range(n)), 0, 0.0
+        mx.synchronize()
+        t0 = time.perf_counter()
+        while act and calls < max_calls:
+            # later calls take only the instances still above the tolerance
+            ek, ns, nl, st = batch_davidson_twosite([probs[i][:5] + (ws[i],) for i in act], tol, prec, True)
+            for j, i in enumerate(act):
+                e[i], rn[i] = ek[j], st["residuals"][j]
+                ns_tot[i] += ns[j]
+                nl_tot[i] += nl[j]
+            calls += 1
+            sig_ms += st["sigma_ms"]
+            act = [i for i in act if rn[i] >= tol]
+        sec = time.perf_counter() - t0
+        conv = rn < tol
+        r = dict(seconds=sec, calls=calls, converged=int(conv.sum()), sigmas=int(ns_tot.sum()),
+                 sigmas_low=int(nl_tot.sum()), sigma_per_s=float(ns_tot.sum()) / sec,
+                 tflops=float((ns_tot * flops).sum()) / sec / 1e12, batched_sigma_device_s=sig_ms * 1e-3,
+                 speedup_vs_single_fp64=res["single_fp64"]["seconds"] / sec)
+        if prec == mx.FP64:
+            e64, conv64 = e.copy(), conv.copy()
+            both = conv
+            r["max_abs_dE_vs_single_fp64"] = float(np.max(np.abs(e - e_single)[both])) if both.any() else None
+        else:
+            d = np.abs(e - e64)
+            both = conv & conv64
+            r.update(converged_both=int(both.sum()),
+                     max_abs_dE_vs_fp64=float(d[both].max()) if both.any() else None,
+                     median_abs_dE_vs_fp64=float(np.median(d[both])) if both.any() else None,
+                     instances_within_1e8=int((d[both] <= 1e-8).sum()), abs_dE_vs_fp64=[float(x) for x in d])
+        res[{mx.FP64: "batched_fp64", mx.FP32: "batched_fp32_mixed", mx.BF16X3: "batched_bf16x3_mixed"}[prec]] = r
+        del ws
+    res["energies_fp64"] = [float(x) for x in e64]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +470,8 @@ def main():
     ap.add_argument("--hubbard", type=int, default=1, help="Hubbard L=32 M=512 two-site sigma leg (0: skip)")
     ap.add_argument("--batch", type=int, default=64, help="instances of the batched leg (0: skip)")
     ap.add_argument("--sweeps", type=int, default=1, help="Hubbard L=32 sweeps, M 128x3, 256x3, 512x2 (0: skip)")
+    ap.add_argument("--batch-davidson", type=int, default=64,
+                    help="instances of the batched Davidson (ground-state) leg (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -446,6 +536,10 @@ def main():
     if args.batch > 0:
         print("batched leg: %d instances" % args.batch, file=sys.stderr, flush=True)
         batched = batched_leg(args.batch, 20)
+    bdav = None
+    if args.batch_davidson > 0:
+        print("batched davidson leg: %d instances" % args.batch_davidson, file=sys.stderr, flush=True)
+        bdav = batched_davidson_leg(args.batch_davidson)
     hsweeps = None
     if args.sweeps > 0:
         hsweeps = hubbard_sweep_leg(32, [128] * 3 + [256] * 3 + [512] * 2, [mx.FP64, mx.FP32])
@@ -489,6 +583,7 @@ def main():
         "qc_sweeps": qc_sweeps,
         "hubbard_sigma": hub,
         "batched": batched,
+        "batched_davidson": bdav,
         "hubbard_sweeps": hsweeps,
     }
     print(json.dumps(res))

@@ -423,7 +423,8 @@ int mpsxx_batch_free(mpsxx_batch b);
  * products, Ritz rotations, residuals, preconditioner and Gram-Schmidt of all instances as single segmented launches.
  * Per instance the result is mpsxx_davidson_twosite's on the same inputs up to rounding: wfnc[i] (the start vector)
  * is overwritten by the normalized eigenvector, energies[i] receives the lowest eigenvalue, n_sigma[i] and
- * n_sigma_low[i] the sigma vectors applied to the instance while it iterated (of them in FP32 / split-BF16x3).
+ * n_sigma_low[i] the sigma vectors applied to the instance while it iterated (of them in FP32 / split-BF16x3),
+ * residuals[i] its last ||r||^2 (below tolerance iff it converged).
  * precision: MPSXX_FP64, or MPSXX_FP32 / MPSXX_BF16X3 (low-precision sigma vectors until ||r||^2 < 1e-9 / 1e-6,
  * then FP64 refinement, as mpsxx_davidson_twosite).  diag[i] may be empty.  Output arrays may be NULL. */
 typedef struct {
@@ -437,7 +438,7 @@ typedef struct {
 int mpsxx_batch_davidson_twosite(int32_t n, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
                                  const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, const mpsxx_tensor* diag,
                                  const mpsxx_tensor* wfnc, double tolerance, int precision, int use_graph,
-                                 double* energies, int32_t* n_sigma, int32_t* n_sigma_low,
+                                 double* energies, int32_t* n_sigma, int32_t* n_sigma_low, double* residuals,
                                  mpsxx_batch_davidson_stats* stats, void* stream);
 
 #ifdef __cplusplus

@@ -81,18 +81,22 @@ def batch_davidson_twosite(instances, tol, precision=FP64, use_graph=True):
     """Davidson (src/diagonalize.hpp:283-449) on n independent two-site problems in lockstep, one batched sigma launch
     sequence and one set of segmented vector operations per step for all of them (mpsxx_batch_davidson_twosite).
     instances: sequence of (lmpo, rmpo, lopr, ropr, diag, wfnc) DTensors; every wfnc (start vector) is overwritten by
-    its normalized eigenvector.  Returns (energies, n_sigma, n_sigma_low, stats dict)."""
+    its normalized eigenvector.  Returns (energies, n_sigma, n_sigma_low, stats dict); stats["residuals"] holds each
+    instance's last ||r||^2."""
     inst = [tuple(x) for x in instances]
     n = len(inst)
     arr = [(VP * n)(*[inst[i][k].h for i in range(n)]) for k in range(6)]
     e = np.zeros(n)
     ns = np.zeros(n, dtype=np.int32)
     nl = np.zeros(n, dtype=np.int32)
+    rn = np.zeros(n)
     st = BatchDavidsonStats()
     _check(lib().mpsxx_batch_davidson_twosite(ctypes.c_int32(n), *arr, ctypes.c_double(tol), ctypes.c_int(precision),
                                               ctypes.c_int(int(use_graph)), e.ctypes.data_as(VP),
-                                              ns.ctypes.data_as(VP), nl.ctypes.data_as(VP), ctypes.byref(st), None))
+                                              ns.ctypes.data_as(VP), nl.ctypes.data_as(VP), rn.ctypes.data_as(VP),
+                                              ctypes.byref(st), None))
     stats = {k: getattr(st, k) for k, _ in BatchDavidsonStats._fields_}
+    stats["residuals"] = rn
     return e, ns, nl, stats
 
 

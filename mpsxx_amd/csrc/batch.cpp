@@ -311,9 +311,9 @@ void BatchSigma::enqueue(unsigned convert_mask, hipStream_t s, const std::vector
   if (events) hip_check(hipEventRecord((*events)[e++], s), "event");
 }
 
-void BatchSigma::run(unsigned convert_mask, bool use_graph, hipStream_t s) {
+void BatchSigma::run(unsigned convert_mask, bool use_graph, hipStream_t s, bool verify) {
   if (prec_ == MPSXX_FP64) convert_mask = 0;
-  check_structures(s);
+  if (verify) check_structures(s);
   upload_tables(s, nullptr);
   if (!use_graph) {
     enqueue(convert_mask, s, nullptr);

@@ -27,7 +27,9 @@ class BatchSigma {
 
   // sgv_i += H_i * wfn_i for every instance.  FP32 batches refresh the FP32 copies of the inputs whose bit is set in
   // convert_mask (bit 0 lmpo, 1 rmpo, 2 lopr, 3 ropr, 4 wfn); use_graph replays a captured graph of the launches.
-  void run(unsigned convert_mask, bool use_graph, hipStream_t s);
+  // verify: check first that no input's block structure changed since creation (O(blocks) per tensor; a caller
+  // that holds the inputs fixed, like the batched Davidson, checks once).
+  void run(unsigned convert_mask, bool use_graph, hipStream_t s, bool verify = true);
 
   // Redirect instance i's wfn input to wfn[i] and its sgv output to sgv[i]: device arrays laid out as that instance's
   // wfn and sgv tensors, which then only describe the structure (a batched Davidson moves its trial and sigma vectors

@@ -207,28 +207,32 @@ void batch_davidson_twosite(std::vector<BatchDavInstance>& inst, double tol, int
   st.nsigma.assign(n, 0);
   st.nsigma_low.assign(n, 0);
   st.converged.assign(n, 0);
-  hipEvent_t e0, e1;
-  hip_check(hipEventCreate(&e0), "event");
-  hip_check(hipEventCreate(&e1), "event");
+  // event pairs around every batched sigma application, read once the solve has ended (no host wait per sigma)
+  std::vector<hipEvent_t> events;
+  size_t nev = 0;
+  auto event = [&]() {
+    if (nev == events.size()) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "event");
+      events.push_back(e);
+    }
+    return events[nev++];
+  };
   std::vector<const double*> win(n);
   std::vector<double*> sout(n);
   std::vector<char> active(n, 1);
   auto multH = [&](int k, bool low) {
     BatchSigma& B = low ? *blo : *b64;
-    hip_check(hipEventRecord(e0, s), "event");
+    hip_check(hipEventRecord(event(), s), "event");
     hip_check(hipMemsetAsync(Sg(k), 0, N * sizeof(double), s), "memset");
     for (int i = 0; i < n; ++i) {
       win[i] = T(k) + off[i];
       sout[i] = Sg(k) + off[i];
     }
     B.set_vectors(win, sout);
-    B.run(low ? conv_lo : 0u, use_graph, s);
+    B.run(low ? conv_lo : 0u, use_graph, s, false);  // inputs checked when the batch was built, fixed since
     if (low) conv_lo = 1u << 4;  // from now on only the trial vectors change
-    hip_check(hipEventRecord(e1, s), "event");
-    hip_check(hipEventSynchronize(e1), "event sync");
-    float ms = 0.f;
-    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-    st.sigma_ms += ms;
+    hip_check(hipEventRecord(event(), s), "event");
     ++st.batch_sigmas;
     if (low) ++st.batch_sigmas_low;
     for (int i = 0; i < n; ++i)
@@ -256,12 +260,12 @@ void batch_davidson_twosite(std::vector<BatchDavInstance>& inst, double tol, int
         launch_seg_gram(CH, nch, C0, n, T(0), N, m, Sg(0), N, m, G.as<double>(), gs, part.as<double>(), s);
         hip_check(hipMemcpyAsync(G_h.data(), G.p, (size_t)n * gs * sizeof(double), hipMemcpyDeviceToHost, s), "gram");
         hip_check(hipStreamSynchronize(s), "gram sync");
-        for (int i = 0; i < n; ++i) {
+        auto subspace = [&](int i) {
           double* c = coef_h.data() + (size_t)i * gs;
           std::fill(c, c + gs, 0.0);
           if (!active[i]) {
             for (int j = 0; j < m; ++j) c[j * max_ritz + j] = 1.0;  // frozen: identity rotation
-            continue;
+            return;
           }
           std::vector<double> heff((size_t)m * m), w, R;
           const double* g = G_h.data() + (size_t)i * gs;
@@ -273,7 +277,12 @@ void batch_davidson_twosite(std::vector<BatchDavInstance>& inst, double tol, int
           // trial_j <- sum_k rvec(k,j) trial_k (diagonalize.hpp:378-394)
           for (int j = 0; j < m; ++j)
             for (int k = 0; k < m; ++k) c[j * max_ritz + k] = R[k * m + j];
-        }
+        };
+        // the instances' subspace problems on several host threads (each writes only its own slots)
+        const int nth = std::max(1, std::min(host_threads(), n / 8));
+        parallel_for(nth, [&](int t) {
+          for (int i = t; i < n; i += nth) subspace(i);
+        });
         hip_check(hipMemcpyAsync(coef.p, coef_h.data(), (size_t)n * gs * sizeof(double), hipMemcpyHostToDevice, s),
                   "coef");
         hip_check(hipMemcpyAsync(evd.p, ev_h.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "evals");
@@ -359,8 +368,12 @@ void batch_davidson_twosite(std::vector<BatchDavInstance>& inst, double tol, int
               "result");
   }
   hip_check(hipStreamSynchronize(s), "batch davidson end");
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  for (size_t k = 0; k + 1 < nev; k += 2) {
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, events[k], events[k + 1]), "elapsed");
+    st.sigma_ms += ms;
+  }
+  for (hipEvent_t e : events) (void)hipEventDestroy(e);
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }
 

@@ -824,7 +824,7 @@ int mpsxx_batch_sigma_twosite_create(int32_t n, const mpsxx_tensor* lmpo, const 
 int mpsxx_batch_davidson_twosite(int32_t n, const mpsxx_tensor* lmpo, const mpsxx_tensor* rmpo,
                                  const mpsxx_tensor* lopr, const mpsxx_tensor* ropr, const mpsxx_tensor* diag,
                                  const mpsxx_tensor* wfnc, double tolerance, int precision, int use_graph,
-                                 double* energies, int32_t* n_sigma, int32_t* n_sigma_low,
+                                 double* energies, int32_t* n_sigma, int32_t* n_sigma_low, double* residuals,
                                  mpsxx_batch_davidson_stats* stats, void* stream) {
   return guard([&] {
     if (n <= 0 || !lmpo || !rmpo || !lopr || !ropr || !diag || !wfnc)
@@ -838,6 +838,7 @@ int mpsxx_batch_davidson_twosite(int32_t n, const mpsxx_tensor* lmpo, const mpsx
       if (energies) energies[i] = st.energy[i];
       if (n_sigma) n_sigma[i] = st.nsigma[i];
       if (n_sigma_low) n_sigma_low[i] = st.nsigma_low[i];
+      if (residuals) residuals[i] = st.rnorm[i];
     }
     if (stats) {
       stats->batch_sigmas = st.batch_sigmas;
