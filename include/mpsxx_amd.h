@@ -204,6 +204,11 @@ int mpsxx_davidson_twosite_ex(int ngroups, const mpsxx_tensor* lmpo, const mpsxx
                               int nlower, const mpsxx_tensor* lower_sigma, const mpsxx_tensor* lower_overlap,
                               const double* lower_energy, double tolerance, int precision, double* energy,
                               int32_t* n_sigma, void* stream);
+/* No reference counterpart: the Davidson solvers replay each sigma plan's launch sequence from a hipGraph captured on
+ * its first use (operand bases from a per-plan device table refreshed when they move); enable = 0 launches every
+ * kernel from the host instead (same kernels, same results).  Default: on unless MPSXX_SIGMA_GRAPH=0.  Returns the
+ * previous setting (1 on, 0 off) and applies to the whole process. */
+int mpsxx_sigma_graphs(int enable);
 
 /* ---- two-site canonicalization (src/canonicalize.hpp:27-46 -> QSTLAPACK.h:122-288) -------------------------------- */
 /* forward (LeftArrow): lmps = U (total QN 0), rmps = s*Vt; backward (RightArrow): lmps = U*s, rmps = Vt (total 0).

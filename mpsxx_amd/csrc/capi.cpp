@@ -560,6 +560,8 @@ int mpsxx_davidson_twosite_ex(int ngroups, const mpsxx_tensor* lmpo, const mpsxx
   });
 }
 
+int mpsxx_sigma_graphs(int enable) { return mpx::set_sigma_graphs(enable); }
+
 int mpsxx_sigma_twosite_stats(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
                               mpsxx_tensor wfn0, int precision, int64_t* flops, int64_t* bytes, int32_t* nsteps) {
   return guard([&] {

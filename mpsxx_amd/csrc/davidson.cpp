@@ -215,18 +215,23 @@ double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tenso
   // FP32 plans convert the environments and MPO tensors once per Davidson call (they do not change inside it) and
   // the trial vector on every application
   unsigned convert_mask = ~0u;
+  // sigma plans replayed from their hipGraphs (the launch sequence of each group captured once per plan)
+  const bool graphs = sigma_graphs_enabled();
+  auto run = [&](Plan& P, const std::vector<const double*>& in, double* v) {
+    if (graphs) run_plan_graph(P, in, v, s, convert_mask);
+    else run_plan_ex(P, in, v, s, {}, convert_mask);
+  };
   auto multH = [&](const double* c, double* v) {
     hip_check(hipEventRecord(e0, s), "event");
     hip_check(hipMemsetAsync(v, 0, n * sizeof(double), s), "memset");
     for (size_t g = 0; g < groups.size(); ++g) {
       const DavGroup& gr = groups[g];
       if (kind == 2)
-        run_plan_ex(*plans[g], {gr.lopr->ptr(), gr.ropr->ptr(), c}, v, s, {}, convert_mask);
+        run(*plans[g], {gr.lopr->ptr(), gr.ropr->ptr(), c}, v);
       else if (kind == 3)
-        run_plan_ex(*plans[g], {gr.lmpo->ptr(), gr.lopr->ptr(), gr.ropr->ptr(), c}, v, s, {}, convert_mask);
+        run(*plans[g], {gr.lmpo->ptr(), gr.lopr->ptr(), gr.ropr->ptr(), c}, v);
       else
-        run_plan_ex(*plans[g], {gr.lmpo->ptr(), gr.rmpo->ptr(), gr.lopr->ptr(), gr.ropr->ptr(), c}, v, s, {},
-                    convert_mask);
+        run(*plans[g], {gr.lmpo->ptr(), gr.rmpo->ptr(), gr.lopr->ptr(), gr.ropr->ptr(), c}, v);
     }
     convert_mask = 1u << (kind == 2 ? 2 : kind == 3 ? 3 : 4);  // from now on only the trial vector changes
     // lower roots: v -= <H_k|c> S_k + <S_k|c> H_k - E_k <S_k|c> S_k (diagonalize.hpp:72-79)

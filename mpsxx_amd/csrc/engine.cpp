@@ -1402,6 +1402,94 @@ void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out,
   hip_check(hipGetLastError(), "kernel launch");
 }
 
+PlanGraph::~PlanGraph() {
+  if (exec) {
+    (void)hipDeviceSynchronize();  // a replay of this graph may still be running (the plan was evicted meanwhile)
+    (void)hipGraphExecDestroy(exec);
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+  if (uploaded) {
+    (void)hipEventSynchronize(uploaded);
+    (void)hipEventDestroy(uploaded);
+  }
+  if (pinned) (void)hipHostFree(pinned);
+}
+
+namespace {
+// the stream plan graphs are captured on (one per host thread; captures are thread-local)
+hipStream_t capture_stream() {
+  static thread_local hipStream_t cs = [] {
+    hipStream_t x = nullptr;
+    hip_check(hipStreamCreateWithFlags(&x, hipStreamNonBlocking), "stream");
+    return x;
+  }();
+  return cs;
+}
+}  // namespace
+
+namespace {
+std::atomic<int>& sigma_graph_flag() {
+  static std::atomic<int> on([] {
+    const char* e = std::getenv("MPSXX_SIGMA_GRAPH");
+    return (e && std::string(e) == "0") ? 0 : 1;
+  }());
+  return on;
+}
+}  // namespace
+
+bool sigma_graphs_enabled() { return sigma_graph_flag().load() != 0; }
+int set_sigma_graphs(int on) { return sigma_graph_flag().exchange(on ? 1 : 0); }
+
+void run_plan_graph(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
+                    unsigned convert_mask) {
+  for (const Step& st : P.steps)
+    if (st.kind == Step::COPY) {
+      run_plan_ex(P, inputs, out, s, {}, convert_mask);
+      return;
+    }
+  std::vector<void*> ptr = slot_pointers(P, inputs, out, s, convert_mask);  // FP32 copies converted here, on s
+  const size_t ns = ptr.size();
+  if (!P.graph) {
+    auto g = std::make_shared<PlanGraph>();
+    g->table.alloc(std::max<size_t>(ns, 1) * sizeof(void*));
+    hip_check(hipHostMalloc(&g->pinned, std::max<size_t>(ns, 1) * sizeof(void*), hipHostMallocDefault),
+              "hipHostMalloc");
+    hip_check(hipEventCreateWithFlags(&g->uploaded, hipEventDisableTiming), "event");
+    P.graph = g;
+  }
+  PlanGraph& G = *P.graph;
+  if (ptr != G.shadow) {
+    hip_check(hipEventSynchronize(G.uploaded), "plan table sync");  // the previous upload may still read `pinned`
+    std::memcpy(G.pinned, ptr.data(), ns * sizeof(void*));
+    hip_check(hipMemcpyAsync(G.table.p, G.pinned, ns * sizeof(void*), hipMemcpyHostToDevice, s), "plan table");
+    hip_check(hipEventRecord(G.uploaded, s), "event");
+    G.shadow = ptr;
+  }
+  if (!G.exec) {
+    // kernels only: every operand base comes from the table (one instance, so the table is indexed by slot)
+    void** T = G.table.as<void*>();
+    const hipStream_t cap = capture_stream();
+    hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
+    try {
+      for (const Step& st : P.steps) {
+        const int b2 = st.b2 >= 0 ? st.b2 : (st.b >= 0 ? st.b : st.a);
+        const Operands ops{T + st.a, st.b >= 0 ? T + st.b : nullptr, T + b2, T + st.c, 1};
+        launch_step(st, ops, P.slot_prec[st.a] == 1, P.slot_prec[st.c] == 1, cap, kAllParts,
+                    P.precision == MPSXX_BF16X3);
+      }
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(cap, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    hip_check(hipStreamEndCapture(cap, &G.graph), "end capture");
+    hip_check(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0), "graph instantiate");
+  }
+  hip_check(hipGraphLaunch(G.exec, s), "graph launch");
+  hip_check(hipGetLastError(), "graph launch");
+}
+
 double time_launches(const std::function<void()>& launch, int reps, hipStream_t s) {
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "event");

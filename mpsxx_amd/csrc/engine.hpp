@@ -212,6 +212,22 @@ void set_gemm_policy(Step& st, const GemmPolicy& pol);
 const char* part_kernel_name(int kind);
 
 // Slot table: inputs are bound per call; intermediates are owned by the plan.
+// A plan's launch sequence captured once in a hipGraph, its kernels reading their operand bases from a device table
+// of slot pointers (the batched kernels' pointer tables with one instance) that is refreshed, outside the graph, only
+// when a slot's address changed (another trial vector, a restored buffer).
+struct PlanGraph {
+  DevBuf table;
+  void* pinned = nullptr;
+  std::vector<void*> shadow;
+  hipEvent_t uploaded = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  PlanGraph() = default;
+  PlanGraph(const PlanGraph&) = delete;
+  PlanGraph& operator=(const PlanGraph&) = delete;
+  ~PlanGraph();
+};
+
 struct Plan {
   std::vector<Step> steps;
   std::vector<Layout> slot_layout;     // per slot
@@ -227,6 +243,7 @@ struct Plan {
   // A copy is converted again when its bit is set in convert_mask, when it was never converted, or when the slot is
   // bound to a different input than the one it was converted from.
   std::vector<const double*> conv_src;
+  std::shared_ptr<PlanGraph> graph;    // run_plan_graph's capture (created on first use)
   // Release the slot buffers (intermediates, FP32 operand copies), keeping the task lists; restore() allocates them
   // again (FP32 operand copies must then be converted again).  Returns whether anything was allocated.
   void strip();
@@ -267,6 +284,13 @@ void run_plan_timed(Plan& P, const std::vector<const double*>& inputs, double* o
                     const std::vector<hipEvent_t>& events);
 // Full form: FP32 plans convert only the inputs whose bit is set in convert_mask (the others reuse the FP32 copy
 // made by an earlier call with the same input contents).
+// run_plan_ex with the plan's launches replayed from its hipGraph (FP32 operand conversions are launched before the
+// graph, the pointer table uploaded when it changed).  Plans with copy steps run as run_plan_ex does.
+void run_plan_graph(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
+                    unsigned convert_mask);
+// Whether MPSXX_SIGMA_GRAPH (default on; "0" switches it off) lets the Davidson solvers replay their sigma plans.
+bool sigma_graphs_enabled();
+int set_sigma_graphs(int on);  // returns the previous setting
 void run_plan_ex(Plan& P, const std::vector<const double*>& inputs, double* out, hipStream_t s,
                  const std::vector<hipEvent_t>& events, unsigned convert_mask);
 
