@@ -210,6 +210,15 @@ int mpsxx_davidson_twosite_ex(int ngroups, const mpsxx_tensor* lmpo, const mpsxx
  * previous setting (1 on, 0 off) and applies to the whole process. */
 int mpsxx_sigma_graphs(int enable);
 
+/* No reference counterpart (diagnostic, host only): the check made before every rocSOLVER batched SVD call of the
+ * fallback path -- rocSOLVER's documented argument bounds for an m x n column-major problem with singular vectors
+ * (lda >= m, strideA >= lda n, strideS >= min(m,n), ldu >= m, strideU >= ldu min(m,n), ldv >= min(m,n),
+ * strideV >= ldv n, strideE >= min(m,n) - 1) and that each buffer of the given byte size holds `batch` strided
+ * problems (bytesE < 0: no E array).  MPSXX_ERR_HIP with the failed rule in mpsxx_last_error when violated. */
+int mpsxx_check_gesvd_args(int m, int n, int lda, int64_t strideA, int64_t strideS, int ldu, int64_t strideU, int ldv,
+                           int64_t strideV, int64_t strideE, int batch, int64_t bytesA, int64_t bytesS,
+                           int64_t bytesU, int64_t bytesV, int64_t bytesE);
+
 /* ---- two-site canonicalization (src/canonicalize.hpp:27-46 -> QSTLAPACK.h:122-288) -------------------------------- */
 /* forward (LeftArrow): lmps = U (total QN 0), rmps = s*Vt; backward (RightArrow): lmps = U*s, rmps = Vt (total 0).
  * Keeps every singular value >= the M-th largest (M <= 0: all >= 1e-16); returns the discarded weight. */

@@ -562,6 +562,15 @@ int mpsxx_davidson_twosite_ex(int ngroups, const mpsxx_tensor* lmpo, const mpsxx
 
 int mpsxx_sigma_graphs(int enable) { return mpx::set_sigma_graphs(enable); }
 
+int mpsxx_check_gesvd_args(int m, int n, int lda, int64_t strideA, int64_t strideS, int ldu, int64_t strideU, int ldv,
+                           int64_t strideV, int64_t strideE, int batch, int64_t bytesA, int64_t bytesS,
+                           int64_t bytesU, int64_t bytesV, int64_t bytesE) {
+  return guard_host([&] {
+    mpx::gesvd_argument_check(m, n, lda, strideA, strideS, ldu, strideU, ldv, strideV, strideE, batch, bytesA, bytesS,
+                              bytesU, bytesV, bytesE);
+  });
+}
+
 int mpsxx_sigma_twosite_stats(mpsxx_tensor lmpo, mpsxx_tensor rmpo, mpsxx_tensor lopr, mpsxx_tensor ropr,
                               mpsxx_tensor wfn0, int precision, int64_t* flops, int64_t* bytes, int32_t* nsteps) {
   return guard([&] {

@@ -181,6 +181,13 @@ struct DavLowerRoot {
   Tensor *sigma, *overlap;
   double energy;
 };
+// The host-side check run before every rocSOLVER batched SVD call (rocSOLVER's documented argument bounds for an
+// m x n column-major problem with singular vectors, and the extent of each buffer from its base for `batch` strided
+// problems; bytesE < 0: no E array).  Throws Error(MPSXX_ERR_HIP) naming the failed rule.
+void gesvd_argument_check(int m, int n, int lda, int64_t sA, int64_t sS, int ldu, int64_t sU, int ldv, int64_t sV,
+                          int64_t sE, int batch, int64_t bytesA, int64_t bytesS, int64_t bytesU, int64_t bytesV,
+                          int64_t bytesE);
+
 Layout davidson_layout(const std::vector<DavGroup>& groups, const Layout& psi);
 double davidson_twosite(const std::vector<DavGroup>& groups, Tensor& diag, Tensor& wfnc, double tol, int prec,
                         DavidsonStats& st, hipStream_t s, const std::vector<DavLowerRoot>& lower = {});

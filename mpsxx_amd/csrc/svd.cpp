@@ -430,9 +430,9 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
 // (rocSOLVER's documented requirements for an m x n column-major A, k = min(m, n): lda >= m, strideA >= lda n,
 // strideS >= k, ldu >= m, strideU >= ldu k, ldv >= k, strideV >= ldv n, strideE >= k - 1), and that each device
 // buffer holds the `batch` strided problems the call reads or writes, checked on the host before every call.
-void check_gesvd_args(int m, int n, int lda, int64_t sA, int64_t sS, int ldu, int64_t sU, int ldv, int64_t sV,
-                      int64_t sE, int batch, const DevBuf& A, const DevBuf& S, const DevBuf& U, const DevBuf& V,
-                      const DevBuf* E) {
+void check_gesvd_extents(int m, int n, int lda, int64_t sA, int64_t sS, int ldu, int64_t sU, int ldv, int64_t sV,
+                         int64_t sE, int batch, int64_t bytesA, int64_t bytesS, int64_t bytesU, int64_t bytesV,
+                         int64_t bytesE) {
   const int64_t k = std::min(m, n);
   auto fail = [&](const char* what) {
     throw Error(MPSXX_ERR_HIP, std::string("svd: rocSOLVER argument check failed (") + what + ") for m " +
@@ -443,15 +443,22 @@ void check_gesvd_args(int m, int n, int lda, int64_t sA, int64_t sS, int ldu, in
   if (sS < k) fail("strideS");
   if (ldu < m || sU < (int64_t)ldu * k) fail("ldu / strideU");
   if (ldv < k || sV < (int64_t)ldv * n) fail("ldv / strideV");
-  if (E && sE < k - 1) fail("strideE");
-  auto holds = [&](const DevBuf& b, int64_t stride, int64_t extent) {
-    return (int64_t)b.bytes >= ((int64_t)(batch - 1) * stride + extent) * (int64_t)sizeof(double);
+  if (bytesE >= 0 && sE < k - 1) fail("strideE");
+  auto holds = [&](int64_t bytes, int64_t stride, int64_t extent) {
+    return bytes >= ((int64_t)(batch - 1) * stride + extent) * (int64_t)sizeof(double);
   };
-  if (!holds(A, sA, (int64_t)lda * n)) fail("A buffer");
-  if (!holds(S, sS, k)) fail("S buffer");
-  if (!holds(U, sU, (int64_t)ldu * k)) fail("U buffer");
-  if (!holds(V, sV, (int64_t)ldv * n)) fail("V buffer");
-  if (E && !holds(*E, sE, std::max<int64_t>(k - 1, 1))) fail("E buffer");
+  if (!holds(bytesA, sA, (int64_t)lda * n)) fail("A buffer");
+  if (!holds(bytesS, sS, k)) fail("S buffer");
+  if (!holds(bytesU, sU, (int64_t)ldu * k)) fail("U buffer");
+  if (!holds(bytesV, sV, (int64_t)ldv * n)) fail("V buffer");
+  if (bytesE >= 0 && !holds(bytesE, sE, std::max<int64_t>(k - 1, 1))) fail("E buffer");
+}
+
+void check_gesvd_args(int m, int n, int lda, int64_t sA, int64_t sS, int ldu, int64_t sU, int ldv, int64_t sV,
+                      int64_t sE, int batch, const DevBuf& A, const DevBuf& S, const DevBuf& U, const DevBuf& V,
+                      const DevBuf* E) {
+  check_gesvd_extents(m, n, lda, sA, sS, ldu, sU, ldv, sV, sE, batch, (int64_t)A.bytes, (int64_t)S.bytes,
+                      (int64_t)U.bytes, (int64_t)V.bytes, E ? (int64_t)E->bytes : -1);
 }
 
 // The rocSOLVER path: every block zero-padded to (mmax x nmax), one strided-batched gesvd (QR iteration) over all.
@@ -685,5 +692,11 @@ double canonicalize_core(bool forward, const Layout& X, int nr, const MergeInfo&
 }
 
 }  // namespace
+
+void gesvd_argument_check(int m, int n, int lda, int64_t sA, int64_t sS, int ldu, int64_t sU, int ldv, int64_t sV,
+                          int64_t sE, int batch, int64_t bytesA, int64_t bytesS, int64_t bytesU, int64_t bytesV,
+                          int64_t bytesE) {
+  check_gesvd_extents(m, n, lda, sA, sS, ldu, sU, ldv, sV, sE, batch, bytesA, bytesS, bytesU, bytesV, bytesE);
+}
 
 }  // namespace mpx
