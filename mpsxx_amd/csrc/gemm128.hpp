@@ -10,6 +10,8 @@
 // 16x16 sub-tiles that lie wholly in the padding of an edge tile skip their MFMAs.
 #pragma once
 
+#include <type_traits>
+
 #include "gemm_tiles.hpp"
 
 namespace mpx {
@@ -18,7 +20,7 @@ constexpr int B2M = 128, B2N = 128, LDP2 = 128 + 4;
 // FP64: one work-group per CU (each wave has the whole 512-entry register file for its 128 accumulator registers and
 // the panel prefetch; the 64-cycle FP64 MFMAs of one panel cover the prefetch latency); FP32: two
 template <typename T, typename TC, bool Tab, bool CA, bool CB>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void gemm128_kernel(const BigTile* __restrict__ tiles,
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 3) void gemm128_kernel(const BigTile* __restrict__ tiles,
                                                                              int ntiles,
                                                                              const GemmTask* __restrict__ tasks,
                                                                              const GemmPair* __restrict__ pairs,
@@ -30,7 +32,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void gemm128_kernel(co
   __shared__ T Bs[2][BK][LDP2];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  // the wave index is uniform across a wave: read it from the first lane, so that every condition derived from it is a
+  // scalar branch, not an exec mask around each MFMA
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wr = wave >> 1, wc = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
 
@@ -45,6 +49,8 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void gemm128_kernel(co
       live_r[i] = tl.row0 + wr * 64 + i * 16 < tk.m;
       live_c[i] = tl.col0 + wc * 64 + i * 16 < tk.n;
     }
+    // every 16x16 sub-tile of the wave's quarter holds real elements (all but edge tiles): MFMAs without guards
+    const bool full = live_r[3] && live_c[3];
     acc_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -79,30 +85,37 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 2) void gemm128_kernel(co
     panel_store<T, CA, B2M, LDP2>(As[0], ra, kla, tid);
     panel_store<T, CB, B2M, LDP2>(Bs[0], rb, klb, tid);
     __syncthreads();
-    int buf = 0;
-    while (true) {
-      const bool more = cur;  // a next panel (uniform)
-      load();                 // zeros past the tile's last panel
+    // the panel loop, compiled twice: without MFMA guards for full tiles, with per-sub-tile guards for edge tiles (one
+    // uniform branch per tile, so the MFMA sequence of a full tile is straight-line code)
+    auto panels = [&](auto full_tile) {
+      constexpr bool FULL = decltype(full_tile)::value;
+      int buf = 0;
+      while (true) {
+        const bool more = cur;  // a next panel (uniform)
+        load();                 // zeros past the tile's last panel
 #pragma unroll
-      for (int kk = 0; kk < BK; kk += 4) {
-        T a[4], b[4];
+        for (int kk = 0; kk < BK; kk += 4) {
+          T a[4], b[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          a[i] = As[buf][kk + lk][wr * 64 + i * 16 + lr];
-          b[i] = Bs[buf][kk + lk][wc * 64 + i * 16 + lr];
+          for (int i = 0; i < 4; ++i) {
+            a[i] = As[buf][kk + lk][wr * 64 + i * 16 + lr];
+            b[i] = Bs[buf][kk + lk][wc * 64 + i * 16 + lr];
+          }
+#pragma unroll
+          for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni)
+              if (FULL || (live_r[mi] && live_c[ni])) acc[mi][ni] = Mfma<T>::op(a[mi], b[ni], acc[mi][ni]);
         }
-#pragma unroll
-        for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-          for (int ni = 0; ni < 4; ++ni)
-            if (live_r[mi] && live_c[ni]) acc[mi][ni] = Mfma<T>::op(a[mi], b[ni], acc[mi][ni]);
+        if (!more) break;
+        panel_store<T, CA, B2M, LDP2>(As[buf ^ 1], ra, kla, tid);
+        panel_store<T, CB, B2M, LDP2>(Bs[buf ^ 1], rb, klb, tid);
+        __syncthreads();
+        buf ^= 1;
       }
-      if (!more) break;
-      panel_store<T, CA, B2M, LDP2>(As[buf ^ 1], ra, kla, tid);
-      panel_store<T, CB, B2M, LDP2>(Bs[buf ^ 1], rb, klb, tid);
-      __syncthreads();
-      buf ^= 1;
-    }
+    };
+    if (full) panels(std::true_type{});
+    else panels(std::false_type{});
 
     if (tl.slot < 0) {
       TC* __restrict__ C = opnd<TC, Tab>(ops.c, tk.inst);
