@@ -922,12 +922,12 @@ GemmPolicy policy_for(GemmPolicy p, int precision) {
 }
 
 GemmPolicy default_policy(int precision) {
-  // measured on the norb=24 M=1024 merged sigma: 128x128 tiles take FP32 from 22.0 to 19.5 ms; the FP64 variant holds
-  // one work-group per CU (its 128 accumulator registers per lane) and with the 64x64 tiles of the remaining blocks
-  // took 35.8 ms against 33.0 ms for 64x64 tiles throughout
+  // measured on the norb=24 M=1024 merged sigma: 128x128 tiles take FP32 from 22.0 to 19.5 ms; FP64 with two
+  // work-groups per CU (128 accumulator registers per lane) from 25.5 to 24.3 ms against 64x64 tiles throughout (with
+  // one work-group per CU it had been slower).  MPSXX_BIG_TILE_MIN_F64 = 0 keeps FP64 on 64x64 tiles.
   static const int f64_min = [] {
     const char* e = std::getenv("MPSXX_BIG_TILE_MIN_F64");
-    return e ? std::max(0, std::atoi(e)) : 0;
+    return e ? std::max(0, std::atoi(e)) : GemmPolicy().big_tile_min;
   }();
   GemmPolicy p = default_policy();
   if (precision == MPSXX_FP64) p.big_tile_min = f64_min;

@@ -146,8 +146,8 @@ struct GemmPolicy {
 // the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT, MPSXX_SPLIT_UNITS, MPSXX_SPLIT_BIG, MPSXX_SPLIT_SMALL,
 // MPSXX_SPLIT_SEG, MPSXX_BIG_TILE_MIN
 GemmPolicy default_policy();
-// the defaults for the block GEMMs of one precision (MPSXX_FP64 / MPSXX_FP32 / MPSXX_BF16X3): 128x128 tiles for FP32
-// and split BF16 only (MPSXX_BIG_TILE_MIN_F64 enables them for FP64)
+// the defaults for the block GEMMs of one precision (MPSXX_FP64 / MPSXX_FP32 / MPSXX_BF16X3): 128x128 tiles for
+// blocks with both sides >= big_tile_min in every precision (MPSXX_BIG_TILE_MIN_F64 sets the FP64 threshold, 0: none)
 GemmPolicy default_policy(int precision);
 // `p` with the tile settings `precision` has kernels for
 GemmPolicy policy_for(GemmPolicy p, int precision);

@@ -17,10 +17,11 @@
 namespace mpx {
 
 constexpr int B2M = 128, B2N = 128, LDP2 = 128 + 4;
-// FP64: one work-group per CU (each wave has the whole 512-entry register file for its 128 accumulator registers and
-// the panel prefetch; the 64-cycle FP64 MFMAs of one panel cover the prefetch latency); FP32: two
+// occupancy: FP64 two work-groups per CU (256 registers per lane: 128 for the accumulators, the panel prefetch, the
+// operands), FP32 three (168 registers); more waves per SIMD hide the panel loads better (FP32: 13.0 ms with two, 11.7
+// with three, 11.9 with four, which spills in the loop's set-up)
 template <typename T, typename TC, bool Tab, bool CA, bool CB>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 1 : 3) void gemm128_kernel(const BigTile* __restrict__ tiles,
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 3) void gemm128_kernel(const BigTile* __restrict__ tiles,
                                                                              int ntiles,
                                                                              const GemmTask* __restrict__ tasks,
                                                                              const GemmPair* __restrict__ pairs,
