@@ -900,6 +900,56 @@ Step make_contract_step(const ContractSym& sym, int a, int b, int c, int accumul
   return st;
 }
 
+namespace {
+DevBuf clone_buf(const DevBuf& b) {
+  DevBuf c;
+  if (!b.p) return c;
+  c.alloc(b.bytes);
+  hip_check(hipMemcpy(c.p, b.p, b.bytes, hipMemcpyDeviceToDevice), "clone step lists");
+  return c;
+}
+}  // namespace
+
+Step clone_step(const Step& st) {
+  Step t;
+  t.kind = st.kind;
+  t.a = st.a;
+  t.b = st.b;
+  t.c = st.c;
+  t.b2 = st.b2;
+  t.accumulate = st.accumulate;
+  t.alpha = st.alpha;
+  t.beta = st.beta;
+  t.nwork = st.nwork;
+  t.work = clone_buf(st.work);
+  t.tasks = clone_buf(st.tasks);
+  t.pairs = clone_buf(st.pairs);
+  t.nwork_red = st.nwork_red;
+  t.work_red = clone_buf(st.work_red);
+  t.partial = clone_buf(st.partial);  // scratch: only its size matters
+  t.nwork_small = st.nwork_small;
+  t.work_small = clone_buf(st.work_small);
+  t.nwork_split = st.nwork_split;
+  t.work_split = clone_buf(st.work_split);
+  t.nwork_split_red = st.nwork_split_red;
+  t.work_split_red = clone_buf(st.work_split_red);
+  t.split_partial = clone_buf(st.split_partial);
+  t.nwork2 = st.nwork2;
+  for (int i = 0; i < 4; ++i) {
+    t.nwork2_layout[i] = st.nwork2_layout[i];
+    t.nwork_layout[i] = st.nwork_layout[i];
+  }
+  t.work2 = clone_buf(st.work2);
+  t.nwork_red2 = st.nwork_red2;
+  t.work_red2 = clone_buf(st.work_red2);
+  t.partial2 = clone_buf(st.partial2);
+  t.max_groups = st.max_groups;
+  t.policy = st.policy;
+  t.flops = st.flops;
+  t.host = st.host;
+  return t;
+}
+
 GemmPolicy default_policy() {
   // measured on the L=32 M=512 Hubbard sigma: single-pair blocks run best one wave per tile, multi-pair blocks best
   // split from 2 units on

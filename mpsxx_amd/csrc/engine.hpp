@@ -197,6 +197,9 @@ struct Step {
   HostLists host;
 };
 
+// A copy of the step with its own device lists (copied on the device) and host lists.
+Step clone_step(const Step& st);
+
 // One kernel launch of a step, with its share of the step's algorithmic cost: flops 2*rows*cols*k summed over the
 // tiles of the launch (GEMM) or 2*len per pair (AXPY); bytes = 8 * (operand elements read once per tile + outputs).
 struct KernelPart {

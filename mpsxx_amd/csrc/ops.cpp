@@ -104,6 +104,12 @@ class PlanCache {
     ++stats_.hits;
     return touch(it);
   }
+  // the plan cached under `key` for the structures `ls`, or nullptr; no build follows a miss (counted as neither)
+  CachedPlan* lookup(const std::vector<uint64_t>& key, const std::vector<const Layout*>& ls) {
+    auto it = map_.find(key);
+    if (it == map_.end() || !it->second.first->matches(ls)) return nullptr;
+    return touch(it);
+  }
   // identity of the plan cached under `key` (0: none)
   uint64_t id_of(const std::vector<uint64_t>& key) const {
     auto it = map_.find(key);
@@ -695,6 +701,44 @@ Plan build_plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layo
   return B.finish();
 }
 
+// A sigma plan of precision `prec` made from the cached FP64 plan of the same key and structures, when there is one
+// and both precisions tile alike: the FP64 plan's steps (the costly symbolic contraction, tiling and task lists)
+// copied, its inputs and intermediates given FP32 storage (the output stays FP64), as PlanBuilder lays out a
+// low-precision sigma plan.  A mixed-precision Davidson solve builds its FP64 plans first, so its FP32 or split-BF16
+// plans come from here.  Returns false (nothing made) otherwise.
+bool plan_from_fp64(std::vector<uint64_t> key, const std::vector<const Layout*>& ls, int prec, Plan& out) {
+  static const bool off = std::getenv("MPSXX_NO_PLAN_REUSE") != nullptr;
+  if (off || prec == MPSXX_FP64) return false;
+  const GemmPolicy p64 = default_policy(MPSXX_FP64), pp = default_policy(prec);
+  if (p64.small_limit != pp.small_limit || p64.split_units != pp.split_units || p64.max_groups != pp.max_groups ||
+      p64.split_big != pp.split_big || p64.split_small != pp.split_small || p64.split_seg != pp.split_seg ||
+      p64.big_tile_min != pp.big_tile_min)
+    return false;
+  key[1] = MPSXX_FP64;
+  key.back() = 0;  // FP64 plans are shared by equal-structure groups (copy 0)
+  CachedPlan* src = cache().lookup(key, ls);
+  if (!src) return false;
+  const Plan& P = src->plan;
+  for (const Step& st : P.steps)
+    if (st.kind == Step::COPY) return false;
+  Plan Q;
+  Q.precision = prec;
+  Q.out = P.out;
+  Q.flops = P.flops;
+  Q.bytes = P.bytes;
+  Q.slot_layout = P.slot_layout;
+  Q.slot_input = P.slot_input;
+  for (size_t i = 0; i < P.slot_layout.size(); ++i) {
+    const bool low = P.slot_input[i] != -2;  // inputs and intermediates in FP32, the output in FP64
+    Q.slot_prec.push_back(low ? 1 : 0);
+    Q.slot_buf.emplace_back();
+    if (low) Q.slot_buf.back().alloc(std::max<int64_t>(P.slot_layout[i].nelem, 1) * sizeof(float));
+  }
+  for (const Step& st : P.steps) Q.steps.push_back(clone_step(st));
+  out = std::move(Q);
+  return true;
+}
+
 Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& lopr, const Layout& ropr,
                          const Layout& wfn, const Layout* out, int prec, int copy) {
   std::vector<uint64_t> key{OP_SIGMA2, (uint64_t)prec, lmpo.hash, rmpo.hash, lopr.hash, ropr.hash, wfn.hash,
@@ -704,7 +748,8 @@ Plan& plan_sigma_twosite(const Layout& lmpo, const Layout& rmpo, const Layout& l
   t_plan_key = key;
   if (cp) return cp->plan;
   auto np = std::make_unique<CachedPlan>();
-  np->plan = build_plan_sigma_twosite(lmpo, rmpo, lopr, ropr, wfn, out, prec);
+  if (!plan_from_fp64(key, ls, prec, np->plan))
+    np->plan = build_plan_sigma_twosite(lmpo, rmpo, lopr, ropr, wfn, out, prec);
   return cache().insert(key, std::move(np), ls)->plan;
 }
 
@@ -729,13 +774,15 @@ Plan& plan_sigma_onesite(const Layout& mpo, const Layout& lopr, const Layout& ro
   t_plan_key = key;
   if (cp) return cp->plan;
   auto np = std::make_unique<CachedPlan>();
-  PlanBuilder B(prec);
-  const int sm = B.input(mpo, 0), sl = B.input(lopr, 1), sr = B.input(ropr, 2), sw = B.input(wfn, 3);
-  // compute_sigma_vector.hpp:17-21: Gemm(N,N,lopr,wfn0) -> Contract(mpo) -> Contract(ropr)
-  const int s1 = B.contract(sl, false, {2}, sw, false, {0});
-  const int s2 = B.contract(s1, false, {1, 2}, sm, false, {0, 2});
-  B.contract_out(s2, false, {3, 1}, sr, false, {1, 2}, out);
-  np->plan = B.finish();
+  if (!plan_from_fp64(key, ls, prec, np->plan)) {
+    PlanBuilder B(prec);
+    const int sm = B.input(mpo, 0), sl = B.input(lopr, 1), sr = B.input(ropr, 2), sw = B.input(wfn, 3);
+    // compute_sigma_vector.hpp:17-21: Gemm(N,N,lopr,wfn0) -> Contract(mpo) -> Contract(ropr)
+    const int s1 = B.contract(sl, false, {2}, sw, false, {0});
+    const int s2 = B.contract(s1, false, {1, 2}, sm, false, {0, 2});
+    B.contract_out(s2, false, {3, 1}, sr, false, {1, 2}, out);
+    np->plan = B.finish();
+  }
   return cache().insert(key, std::move(np), ls)->plan;
 }
 
@@ -759,12 +806,14 @@ Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wf
   t_plan_key = key;
   if (cp) return cp->plan;
   auto np = std::make_unique<CachedPlan>();
-  PlanBuilder B(prec);
-  const int sl = B.input(lopr, 0), sr = B.input(ropr, 1), sw = B.input(wfn, 2);
-  // compute_sigma_vector.hpp:94-95: Gemm(N,N,lopr,wfn0) (K=1), Gemm(N,T,scr1,ropr) (K=2)
-  const int s1 = B.contract(sl, false, {2}, sw, false, {0});
-  B.contract_out(s1, false, {1, 2}, sr, false, {1, 2}, out);
-  np->plan = B.finish();
+  if (!plan_from_fp64(key, ls, prec, np->plan)) {
+    PlanBuilder B(prec);
+    const int sl = B.input(lopr, 0), sr = B.input(ropr, 1), sw = B.input(wfn, 2);
+    // compute_sigma_vector.hpp:94-95: Gemm(N,N,lopr,wfn0) (K=1), Gemm(N,T,scr1,ropr) (K=2)
+    const int s1 = B.contract(sl, false, {2}, sw, false, {0});
+    B.contract_out(s1, false, {1, 2}, sr, false, {1, 2}, out);
+    np->plan = B.finish();
+  }
   return cache().insert(key, std::move(np), ls)->plan;
 }
 
