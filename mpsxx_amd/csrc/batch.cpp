@@ -360,7 +360,7 @@ std::vector<PartTiming> BatchSigma::profile(int reps, hipStream_t s) {
   enqueue(~0u, s, nullptr);
   std::vector<PartTiming> res;
   for (int k = 0; k < (int)launches_.size(); ++k)
-    for (KernelPart kp : step_parts(launches_[k].st)) {
+    for (KernelPart kp : step_parts(launches_[k].st, launches_[k].f32 ? 4 : 8, launches_[k].cf32 ? 4 : 8)) {
       const double ms = time_launches([&] { launch_one(k, s, kp.mask); }, reps, s);
       if (prec_ == MPSXX_BF16X3 && kp.kind == KernelPart::GEMM64) kp.kind = KernelPart::GEMM_X3;
       if (prec_ == MPSXX_BF16X3 && kp.kind == KernelPart::GEMM128) kp.kind = KernelPart::GEMM_X3_128;

@@ -1275,13 +1275,13 @@ void launch_step(const Step& st, const Operands& ops, bool f32, bool cf32, hipSt
   else launch_step_t<float, double>(st, ops, parts, x3, s);
 }
 
-std::vector<KernelPart> step_parts(const Step& st) {
+std::vector<KernelPart> step_parts(const Step& st, int ea, int ec) {
   std::vector<KernelPart> out;
   if (st.kind == Step::AXPY) {
     int64_t fl = 0, by = 0;
     for (const AxpyTask& t : st.host.atasks) {
       fl += 2ll * t.len * t.pair_count;
-      by += 8ll * t.len * (t.pair_count + 1);
+      by += (int64_t)ea * t.len * t.pair_count + (int64_t)ec * t.len;
     }
     if (st.nwork > 0) out.push_back({KernelPart::AXPY, 1u, st.nwork, fl, by});
     return out;
@@ -1294,7 +1294,7 @@ std::vector<KernelPart> step_parts(const Step& st) {
     int64_t ks = 0;
     for (int p = lo; p < hi; ++p) ks += st.host.gpairs[p].k;
     fl += 2 * r * c * ks;
-    by += 8 * ((r + c) * ks);
+    by += ea * ((r + c) * ks);
   };
   GemmTiling g;
   gemm_tiles(st.host.gtasks, st.host.gpairs, st.policy, g);
@@ -1303,29 +1303,29 @@ std::vector<KernelPart> step_parts(const Step& st) {
   for (const BigTile& tl : g.big) {
     if (tl.slot >= 0) continue;
     const GemmTask& t = st.host.gtasks[tl.task];
-    by += 8 * (int64_t)std::min(kGemmTile, t.m - tl.row0) * std::min(kGemmTile, t.n - tl.col0);
+    by += ec * (int64_t)std::min(kGemmTile, t.m - tl.row0) * std::min(kGemmTile, t.n - tl.col0);
   }
   for (const RedTile& rt : g.red) {
     const GemmTask& t = st.host.gtasks[rt.task];
-    by += 8 * (int64_t)std::min(kGemmTile, t.m - rt.row0) * std::min(kGemmTile, t.n - rt.col0);
+    by += ec * (int64_t)std::min(kGemmTile, t.m - rt.row0) * std::min(kGemmTile, t.n - rt.col0);
   }
   if (!g.big.empty()) out.push_back({KernelPart::GEMM64, 1u, (int)g.big.size(), fl, by});
   fl = by = 0;
   for (const GemmTile& tl : g.small) {
     const GemmTask& t = st.host.gtasks[tl.task];
     tile_cost(tl.task, tl.row0, tl.col0, t.pair_begin, t.pair_begin + t.pair_count, kSmallTile, fl, by);
-    by += 8 * (int64_t)std::min(kSmallTile, t.m - tl.row0) * std::min(kSmallTile, t.n - tl.col0);
+    by += ec * (int64_t)std::min(kSmallTile, t.m - tl.row0) * std::min(kSmallTile, t.n - tl.col0);
   }
   if (!g.small.empty()) out.push_back({KernelPart::GEMM_SMALL, 2u, (int)g.small.size(), fl, by});
   fl = by = 0;
   for (const BigTile& tl : g.split) {
     const GemmTask& t = st.host.gtasks[tl.task];
     tile_cost(tl.task, tl.row0, tl.col0, tl.pair_lo, tl.pair_hi, kSmallTile, fl, by);
-    if (tl.slot < 0) by += 8 * (int64_t)std::min(kSmallTile, t.m - tl.row0) * std::min(kSmallTile, t.n - tl.col0);
+    if (tl.slot < 0) by += ec * (int64_t)std::min(kSmallTile, t.m - tl.row0) * std::min(kSmallTile, t.n - tl.col0);
   }
   for (const RedTile& rt : g.split_red) {
     const GemmTask& t = st.host.gtasks[rt.task];
-    by += 8 * (int64_t)std::min(kSmallTile, t.m - rt.row0) * std::min(kSmallTile, t.n - rt.col0);
+    by += ec * (int64_t)std::min(kSmallTile, t.m - rt.row0) * std::min(kSmallTile, t.n - rt.col0);
   }
   if (!g.split.empty()) out.push_back({KernelPart::GEMM_SPLIT, 4u, (int)g.split.size(), fl, by});
   fl = by = 0;
@@ -1333,12 +1333,12 @@ std::vector<KernelPart> step_parts(const Step& st) {
     tile_cost(tl.task, tl.row0, tl.col0, tl.pair_lo, tl.pair_hi, kGemmTile2, fl, by);
     if (tl.slot < 0) {
       const GemmTask& t = st.host.gtasks[tl.task];
-      by += 8 * (int64_t)std::min(kGemmTile2, t.m - tl.row0) * std::min(kGemmTile2, t.n - tl.col0);
+      by += ec * (int64_t)std::min(kGemmTile2, t.m - tl.row0) * std::min(kGemmTile2, t.n - tl.col0);
     }
   }
   for (const RedTile& rt : g.red2) {
     const GemmTask& t = st.host.gtasks[rt.task];
-    by += 8 * (int64_t)std::min(kGemmTile2, t.m - rt.row0) * std::min(kGemmTile2, t.n - rt.col0);
+    by += ec * (int64_t)std::min(kGemmTile2, t.m - rt.row0) * std::min(kGemmTile2, t.n - rt.col0);
   }
   if (!g.big2.empty()) out.push_back({KernelPart::GEMM128, 8u, (int)g.big2.size(), fl, by});
   return out;
@@ -1566,7 +1566,7 @@ std::vector<PartTiming> profile_plan_parts(Plan& P, const std::vector<const doub
     const bool f32 = P.slot_prec[st.a] == 1, cf32 = P.slot_prec[st.c] == 1;
     const Operands ops = step_operands(st, ptr);
     const bool x3 = P.precision == MPSXX_BF16X3;
-    for (KernelPart kp : step_parts(st)) {
+    for (KernelPart kp : step_parts(st, f32 ? 4 : 8, cf32 ? 4 : 8)) {
       const double ms = time_launches([&] { launch_step(st, ops, f32, cf32, s, kp.mask, x3); }, reps, s);
       if (x3 && kp.kind == KernelPart::GEMM64) kp.kind = KernelPart::GEMM_X3;
       if (x3 && kp.kind == KernelPart::GEMM128) kp.kind = KernelPart::GEMM_X3_128;

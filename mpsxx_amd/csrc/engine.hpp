@@ -201,7 +201,8 @@ struct Step {
 Step clone_step(const Step& st);
 
 // One kernel launch of a step, with its share of the step's algorithmic cost: flops 2*rows*cols*k summed over the
-// tiles of the launch (GEMM) or 2*len per pair (AXPY); bytes = 8 * (operand elements read once per tile + outputs).
+// tiles of the launch (GEMM) or 2*len per pair (AXPY); bytes = operand elements read once per tile + outputs, at their
+// storage size.
 struct KernelPart {
   enum Kind { GEMM64 = 0, GEMM_SMALL = 1, GEMM_SPLIT = 2, AXPY = 3, GEMM_X3 = 4, GEMM128 = 5, GEMM_X3_128 = 6 };
   int kind;
@@ -209,7 +210,8 @@ struct KernelPart {
   int items;
   int64_t flops, bytes;
 };
-std::vector<KernelPart> step_parts(const Step& st);
+// ea, ec: bytes per operand element and per output element of the launch (8 for FP64, 4 for FP32 storage)
+std::vector<KernelPart> step_parts(const Step& st, int ea = 8, int ec = 8);
 // (Re)build the device tile lists of a GEMM step from its host task lists.
 void set_gemm_policy(Step& st, const GemmPolicy& pol);
 const char* part_kernel_name(int kind);
