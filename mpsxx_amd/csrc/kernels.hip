@@ -1397,6 +1397,63 @@ __global__ __launch_bounds__(256) void svd_qr_apply_kernel(const SvdQrBlk* __res
   }
 }
 
+__global__ __launch_bounds__(256) void svd_sort_kernel(const SvdQrBlk* __restrict__ blks, const double* __restrict__ X,
+                                                       double* Xs, int32_t* perm) {
+  extern __shared__ double nrm[];  // kk squared norms, then kk ranks
+  const SvdQrBlk b = blks[blockIdx.x];
+  const int len = b.len, kk = b.kk;
+  int* rank = reinterpret_cast<int*>(nrm + kk);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const double* x = X + b.x_off;
+  for (int j = w; j < kk; j += 4) {
+    double ss = 0.0;
+    for (int i = lane; i < len; i += 64) {
+      const double u = x[(int64_t)j * len + i];
+      ss += u * u;
+    }
+    ss = wave_sum(ss);
+    if (lane == 0) nrm[j] = ss == ss ? ss : -1.0;  // NaN last, so the order stays total and rank a permutation
+  }
+  __syncthreads();
+  for (int j = tid; j < kk; j += 256) {
+    const double v = nrm[j];
+    int r = 0;
+    for (int i = 0; i < kk; ++i) r += (nrm[i] > v || (nrm[i] == v && i < j)) ? 1 : 0;
+    rank[j] = r;
+    perm[b.tau_off + r] = j;
+  }
+  __syncthreads();
+  double* xs = Xs + b.x_off;
+  for (int64_t e = tid; e < (int64_t)len * kk; e += 256) {
+    const int c = (int)(e / len);
+    xs[(int64_t)rank[c] * len + (e - (int64_t)c * len)] = x[e];
+  }
+}
+
+__global__ __launch_bounds__(256) void svd_unsort_rows_kernel(const SvdQrBlk* __restrict__ blks,
+                                                              const int32_t* __restrict__ perm,
+                                                              const double* __restrict__ Y, double* Yu) {
+  const SvdQrBlk b = blks[blockIdx.x];
+  const int kk = b.kk;
+  for (int64_t e = threadIdx.x; e < (int64_t)kk * kk; e += 256) {
+    const int c = (int)(e / kk), i = (int)(e - (int64_t)c * kk), p = perm[b.tau_off + i];
+    if (p >= 0 && p < kk) Yu[b.y_off + (int64_t)c * kk + p] = Y[b.y_off + e];
+  }
+}
+
+void launch_svd_sort(const SvdQrBlk* blks, int nb, const double* X, double* Xs, int32_t* perm, int kk_max,
+                     hipStream_t s) {
+  if (nb <= 0) return;
+  (void)svd_round_lds_limit();  // raises the kernels' dynamic LDS limit once
+  const size_t lds = (size_t)kk_max * (sizeof(double) + sizeof(int));
+  hipLaunchKernelGGL(svd_sort_kernel, dim3(nb), dim3(256), lds, s, blks, X, Xs, perm);
+}
+
+void launch_svd_unsort_rows(const SvdQrBlk* blks, int nb, const int32_t* perm, const double* Y, double* Yu,
+                            hipStream_t s) {
+  if (nb > 0) hipLaunchKernelGGL(svd_unsort_rows_kernel, dim3(nb), dim3(256), 0, s, blks, perm, Y, Yu);
+}
+
 size_t svd_round_lds_limit() {
   static size_t lim = 0;
   if (!lim) {
@@ -1410,7 +1467,8 @@ size_t svd_round_lds_limit() {
     const void* fns[] = {(const void*)svd_round_kernel<8>,     (const void*)svd_round_kernel<16>,
                          (const void*)svd_round_kernel<32>,    (const void*)svd_finish_kernel,
                          (const void*)svd_qr_kernel<16>,       (const void*)svd_qr_kernel<32>,
-                         (const void*)svd_qr_apply_kernel<16>, (const void*)svd_qr_apply_kernel<32>};
+                         (const void*)svd_qr_apply_kernel<16>, (const void*)svd_qr_apply_kernel<32>,
+                         (const void*)svd_sort_kernel};
     for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
   }
   return lim;

@@ -228,6 +228,15 @@ void launch_svd_qr(const SvdQrBlk* blks, int nb, double* X, double* tau, double*
 void launch_svd_rt(const SvdQrBlk* blks, int nb, const double* X, double* Y, hipStream_t s);
 void launch_svd_qr_apply(const SvdQrBlk* blks, int nb, const double* X, const double* T, const double* W, double* Z,
                          int cw, int len_max, hipStream_t s);
+// Column sorting before the QR, a cheap stand-in for column pivoting: each block's columns of X (len x kk) are copied
+// to Xs (same offsets) in order of decreasing Euclidean norm, ties by index, and perm[tau_off + r] is the column
+// placed at r.  R's rows are then graded, and the sweeps over R^T converge in fewer passes.  After the sweeps the rows
+// of Y (kk x kk, at y_off) are put back in X's column order: Yu(perm[i], c) = Y(i, c).  Needs kk_max * 12 bytes of
+// LDS.
+void launch_svd_sort(const SvdQrBlk* blks, int nb, const double* X, double* Xs, int32_t* perm, int kk_max,
+                     hipStream_t s);
+void launch_svd_unsort_rows(const SvdQrBlk* blks, int nb, const int32_t* perm, const double* Y, double* Yu,
+                            hipStream_t s);
 
 // Output of a converged block: sigma_j = |xn_j| (xn: kk columns of length m when u_from_x, else n), sorted descending
 // into S[blk * kmax + r].  U_A (m x kmax row-major, slot stride su_a) and Vt_A (kmax x nmax row-major, slot stride

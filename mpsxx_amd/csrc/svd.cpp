@@ -238,9 +238,12 @@ namespace {
 constexpr int kJacobiMaxSweeps = 60;
 
 // Batched one-sided Jacobi SVD of every block.  Each block's kk vectors X (columns or rows of A, whichever are
-// fewer) are first QR-factored, X = Q R (svd_qr_kernel), and the Jacobi sweeps (svd_round_kernel) orthogonalize the
-// columns of Y = R^T, which are graded, so the sweeps converge in a few passes: R^T W = U_R s, hence X = (Q W) s U_R^T
-// and Q W comes from applying the Householder reflectors to W (svd_qr_apply_kernel).  Blocks too long for the QR
+// fewer) are sorted by decreasing norm (X P, svd_sort_kernel; MPSXX_SVD_SORT=0 skips it) and QR-factored, X P = Q R
+// (svd_qr_kernel), and the Jacobi sweeps (svd_round_kernel) orthogonalize the columns of Y = R^T, which are graded,
+// so the sweeps converge in a few passes: R^T W = U_R s, hence X = (Q W) s (P U_R)^T; Q W comes from applying the
+// Householder reflectors to W (svd_qr_apply_kernel) and P U_R from putting Y's rows back in order.  The sort is a
+// cheap stand-in for the column pivoting of the Drmac-Veselic preconditioner: on the L = 32, M = 512 Hubbard sweeps
+// it cuts the sweeps of a centre-bond SVD from ~16 to ~9.  Blocks too long for the QR
 // kernels' LDS are swept directly (X V = U_X s).  Writes the descending singular values to Sv (kmax per block), U_A
 // row-major (mmax x kmax) to Ua and Vt_A row-major (kmax x nmax) to Vta, the slots LAPACK-style gesvd fills.  Returns
 // false, having written none of them, when a vector is longer than one work-group's LDS holds or the sweeps do not
@@ -299,7 +302,11 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
   // the swept matrices: Y = R^T (kk x kk) with QR, else X itself; V accumulates the rotations
   std::vector<SvdBlk> jb = sb;
   std::vector<SvdQrBlk> qb;
-  DevBuf Y, Z, tau, T, dqb;
+  DevBuf Y, Z, tau, T, dqb, Xs, perm;
+  const char* sort_env = std::getenv("MPSXX_SVD_SORT");
+  const bool sort_cols = qr && !(sort_env && std::strcmp(sort_env, "0") == 0);
+  double* Xq = X.as<double>();  // the matrix the QR factors (X, or X with its columns sorted)
+  int64_t y_elems = 0;
   if (qr) {
     int64_t yo = 0, zo = 0, to = 0, po = 0;
     for (int k = 0; k < nb; ++k) {
@@ -317,8 +324,15 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
     tau = DevBuf(std::max<int64_t>(to, 1) * sizeof(double));
     T = DevBuf(std::max<int64_t>(po, 1) * sizeof(double));
     dqb = upload_vec(qb);
-    launch_svd_qr(dqb.as<SvdQrBlk>(), nb, X.as<double>(), tau.as<double>(), T.as<double>(), qcw, len_max, s);
-    launch_svd_rt(dqb.as<SvdQrBlk>(), nb, X.as<double>(), Y.as<double>(), s);
+    y_elems = yo;
+    if (sort_cols) {  // the QR factors the norm-sorted columns; Y's rows are put back in order after the sweeps
+      Xs = DevBuf(std::max<int64_t>(xo, 1) * sizeof(double));
+      perm = DevBuf(std::max<int64_t>(to, 1) * sizeof(int32_t));
+      launch_svd_sort(dqb.as<SvdQrBlk>(), nb, X.as<double>(), Xs.as<double>(), perm.as<int32_t>(), kk_max, s);
+      Xq = Xs.as<double>();
+    }
+    launch_svd_qr(dqb.as<SvdQrBlk>(), nb, Xq, tau.as<double>(), T.as<double>(), qcw, len_max, s);
+    launch_svd_rt(dqb.as<SvdQrBlk>(), nb, Xq, Y.as<double>(), s);
     hip_check(hipGetLastError(), "svd qr launch");
   }
   phase();  // 1: QR
@@ -399,9 +413,14 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
   phase();  // 3: sweeps
   if (!converged) return false;
   std::vector<SvdOut> outs(nb);
+  DevBuf Yu;
   if (qr) {
-    launch_svd_qr_apply(dqb.as<SvdQrBlk>(), nb, X.as<double>(), T.as<double>(), V.as<double>(), Z.as<double>(), qcw,
-                        len_max, s);
+    launch_svd_qr_apply(dqb.as<SvdQrBlk>(), nb, Xq, T.as<double>(), V.as<double>(), Z.as<double>(), qcw, len_max, s);
+    if (sort_cols) {
+      Yu = DevBuf(std::max<int64_t>(y_elems, 1) * sizeof(double));
+      launch_svd_unsort_rows(dqb.as<SvdQrBlk>(), nb, perm.as<int32_t>(), Y.as<double>(), Yu.as<double>(), s);
+      Xj = Yu.as<double>();
+    }
     for (int k = 0; k < nb; ++k) outs[k] = {qb[k].y_off, qb[k].z_off, sb[k].m, sb[k].n, sb[k].kk, sb[k].n > sb[k].m};
   } else {
     for (int k = 0; k < nb; ++k) outs[k] = {sb[k].x_off, sb[k].v_off, sb[k].m, sb[k].n, sb[k].kk, sb[k].n <= sb[k].m};
