@@ -902,19 +902,18 @@ __device__ __forceinline__ double rcp_nr(double x) {
 
 // Rutishauser's rotation of a pair with a = |y_p|^2, b = |y_q|^2, g = y_p.y_q (identity unless real and
 // |g| > tol sqrt(a b)): t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), zeta = (b - a) / 2g, written as
-// 2g sign(b - a) / (|b - a| + sqrt((b - a)^2 + 4g^2)); c = 1 / sqrt(1 + t^2), s = c t.
+// 2g sign(b - a) / (|b - a| + sqrt((b - a)^2 + 4g^2)); c = 1 / sqrt(1 + t^2), s = c t.  Branch-free: the formulas
+// run on every pair (on a harmless argument where no rotation applies) and the identity is selected afterwards, so
+// the two rotations a thread needs per step are one interleaved dependency chain rather than two.
 __device__ __forceinline__ void svd_rotation(double a, double b, double g, bool real, double tol2, double& cs,
                                              double& sn) {
-  cs = 1.0;
-  sn = 0.0;
-  if (real && g != 0.0 && g * g > tol2 * (a * b)) {
-    const double d2 = b - a, q2 = d2 * d2 + 4.0 * g * g;
-    if (q2 > 0.0) {
-      const double tt = copysign(1.0, d2) * (2.0 * g) * rcp_nr(fabs(d2) + q2 * rsqrt_nr(q2));
-      cs = rsqrt_nr(1.0 + tt * tt);
-      sn = cs * tt;
-    }
-  }
+  const double d2 = b - a, q2 = d2 * d2 + 4.0 * g * g;
+  const bool rot = real && g != 0.0 && g * g > tol2 * (a * b) && q2 > 0.0;
+  const double q = rot ? q2 : 1.0;
+  const double tt = copysign(1.0, d2) * (2.0 * g) * rcp_nr(fabs(d2) + q * rsqrt_nr(q));
+  const double c = rsqrt_nr(1.0 + tt * tt);
+  cs = rot ? c : 1.0;
+  sn = rot ? c * tt : 0.0;
 }
 
 // Pair `t` of step `st` of a task: cross-pairs task (two groups, n0 >= n1): the n1 cross pairs (i, n0 + j) with
@@ -947,7 +946,7 @@ __global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restric
                                                         const int32_t* __restrict__ active, int32_t* rot) {
   extern __shared__ double ys[];  // COLS columns of stride ld
   __shared__ double Gs[32][33], Gs2[32][33], Qs[32][33];
-  __shared__ int scnt;
+  __shared__ int scnt, sneed;
   const SvdTask t = tasks[blockIdx.x];
   if (!active[t.blk]) return;
   const SvdBlk b = blks[t.blk];
@@ -970,7 +969,10 @@ __global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restric
   }
   svd_gather_columns(X + b.x_off, len, nc, COLS, ld, gcol, ys, tid, 256);
   for (int e = tid; e < COLS * COLS; e += 256) Qs[e / COLS][e % COLS] = (e / COLS == e % COLS) ? 1.0 : 0.0;
-  if (tid == 0) scnt = 0;
+  if (tid == 0) {
+    scnt = 0;
+    sneed = 0;
+  }
   __syncthreads();
   // G = Y^T Y (zero outside the task's nc columns)
   {
@@ -1018,14 +1020,30 @@ __global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restric
     svd_pair(t.n0, t.n1, cross, 0, bi, pi, qi, ri);
     svd_pair(t.n0, t.n1, cross, 0, bj, pj, qj, rj);
   }
+  // a task none of whose pairs would rotate on the G just formed leaves G unchanged (identity rotations), so no pair
+  // rotates later in its sweep either: it skips the sweep, exactly as if it had run it
+  int need = 0;
+  for (int e = tid; e < COLS * COLS; e += 256) {
+    const int p = e / COLS, q = e % COLS;
+    if (p < q && q < nc && (!cross || (p < t.n0 && q >= t.n0))) {
+      const double a = Gs[p][p], bq = Gs[q][q], g = Gs[p][q], d2 = bq - a;
+      if (g != 0.0 && g * g > tol2 * (a * bq) && d2 * d2 + 4.0 * g * g > 0.0) need = 1;
+    }
+  }
+  if (need) sneed = 1;
+  __syncthreads();
+  if (!sneed) return;  // uniform
   double (*Gc)[33] = Gs;
   double (*Gn)[33] = Gs2;
   int myrot = 0;
   for (int st = 0; st < nsteps; ++st) {
     if (act) {
+      // every LDS operand of the step is read up front: the G entries of both pairs and of the 2x2 block, and this
+      // thread's two rows of Q's columns p_j, q_j (no other thread writes them this step)
       const double ai = Gc[pi][pi], bi2 = Gc[qi][qi], gi = Gc[pi][qi];
       const double aj = Gc[pj][pj], bj2 = Gc[qj][qj], gj = Gc[pj][qj];
       const double b00 = Gc[pi][pj], b01 = Gc[pi][qj], b10 = Gc[qi][pj], b11 = Gc[qi][qj];
+      const double qu0 = Qs[2 * bi][pj], qv0 = Qs[2 * bi][qj], qu1 = Qs[2 * bi + 1][pj], qv1 = Qs[2 * bi + 1][qj];
       double ci, si, cj, sj;
       svd_rotation(ai, bi2, gi, ri, tol2, ci, si);
       svd_rotation(aj, bj2, gj, rj, tol2, cj, sj);
@@ -1038,12 +1056,10 @@ __global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restric
       Gn[qi][pj] = si * t00 + ci * t10;
       Gn[qi][qj] = si * t01 + ci * t11;
       if (sj != 0.0) {  // Q rows 2i, 2i+1, cols p_j, q_j <- B J_j (each Q element has one owner per step)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const double u = Qs[2 * bi + h][pj], v = Qs[2 * bi + h][qj];
-          Qs[2 * bi + h][pj] = cj * u - sj * v;
-          Qs[2 * bi + h][qj] = sj * u + cj * v;
-        }
+        Qs[2 * bi][pj] = cj * qu0 - sj * qv0;
+        Qs[2 * bi][qj] = sj * qu0 + cj * qv0;
+        Qs[2 * bi + 1][pj] = cj * qu1 - sj * qv1;
+        Qs[2 * bi + 1][qj] = sj * qu1 + cj * qv1;
       }
       if (cross) {
         svd_pair(t.n0, t.n1, cross, st + 1, bi, pi, qi, ri);
@@ -1441,6 +1457,22 @@ __global__ __launch_bounds__(256) void svd_unsort_rows_kernel(const SvdQrBlk* __
   }
 }
 
+__global__ __launch_bounds__(256) void svd_sweep_end_kernel(int32_t* rot, int32_t* active, int nb, int32_t* any) {
+  int a = 0;
+  for (int k = threadIdx.x; k < nb; k += 256) {
+    const int r = rot[k] > 0 ? 1 : 0;
+    active[k] = r;
+    rot[k] = 0;
+    a |= r;
+  }
+  a = __syncthreads_or(a);
+  if (threadIdx.x == 0) *any = a;
+}
+
+void launch_svd_sweep_end(int32_t* rot, int32_t* active, int nb, int32_t* any, hipStream_t s) {
+  hipLaunchKernelGGL(svd_sweep_end_kernel, dim3(1), dim3(256), 0, s, rot, active, nb, any);
+}
+
 void launch_svd_sort(const SvdQrBlk* blks, int nb, const double* X, double* Xs, int32_t* perm, int kk_max,
                      hipStream_t s) {
   if (nb <= 0) return;
@@ -1461,15 +1493,21 @@ size_t svd_round_lds_limit() {
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
                                                   hipSuccess || v <= 0)
       v = 65536;
-    // the round kernel's static LDS: two G buffers and Q (3 x 32 x 33 doubles) and a counter
-    const size_t stat = 3 * 32 * 33 * sizeof(double) + 64;
-    lim = (size_t)v - stat;
     const void* fns[] = {(const void*)svd_round_kernel<8>,     (const void*)svd_round_kernel<16>,
                          (const void*)svd_round_kernel<32>,    (const void*)svd_finish_kernel,
                          (const void*)svd_qr_kernel<16>,       (const void*)svd_qr_kernel<32>,
                          (const void*)svd_qr_apply_kernel<16>, (const void*)svd_qr_apply_kernel<32>,
                          (const void*)svd_sort_kernel};
+    // the dynamic limit leaves room for the largest static LDS among these kernels (the round kernel's G buffers)
+    size_t stat = 0;
+    for (const void* f : fns) {
+      hipFuncAttributes fa;
+      if (hipFuncGetAttributes(&fa, f) == hipSuccess) stat = std::max(stat, (size_t)fa.sharedSizeBytes);
+    }
+    stat = std::max(stat, (size_t)(3 * 32 * 33 * sizeof(double) + 64));
+    lim = (size_t)v - stat;
     for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+    (void)hipGetLastError();  // a refused attribute leaves the default limit; launches check their own sizes
   }
   return lim;
 }

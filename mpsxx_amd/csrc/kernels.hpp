@@ -210,6 +210,9 @@ void launch_svd_init(const SvdBlk* blks, int nb, double* V, hipStream_t s);
 // rot[blk] is increased by the number of rotations applied; blocks with active[blk] == 0 are skipped.
 void launch_svd_round(const SvdTask* tasks, int ntasks, const SvdBlk* blks, double* X, double* V,
                       const int32_t* active, int32_t* rot, int cols, int len_max, hipStream_t s);
+// End of a sweep, on the device: active[k] = (rot[k] > 0), rot[k] = 0 for the nb blocks, and *any = 1 when any block
+// rotated (0 when the sweep converged every block).
+void launch_svd_sweep_end(int32_t* rot, int32_t* active, int nb, int32_t* any, hipStream_t s);
 // QR preconditioning of the Jacobi sweeps (blocked Householder, one work-group per block, panels of kSvdQrPanel
 // columns): X (len x kk) = Q R.  The sweeps then run on Y = R^T (kk x kk), whose columns are graded, and the left
 // vectors of X are Z = Q [W; 0] with W the sweeps' accumulated rotations.  In place: R above and the Householder

@@ -236,6 +236,7 @@ double op_canonicalize_onesite(bool forward, Tensor& wfnc, Tensor& lmps, Tensor&
 namespace {
 
 constexpr int kJacobiMaxSweeps = 60;
+constexpr int kJacobiFirstCheck = 4;
 
 // Batched one-sided Jacobi SVD of every block.  Each block's kk vectors X (columns or rows of A, whichever are
 // fewer) are sorted by decreasing norm (X P, svd_sort_kernel; MPSXX_SVD_SORT=0 skips it) and QR-factored, X P = Q R
@@ -383,31 +384,34 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
   }
   DevBuf dtask = upload_vec(flat);
   phase();  // 2: schedule
+  // a block takes part in a sweep while its previous sweep rotated (the device keeps the flags between sweeps); the
+  // host reads whether any block rotated only from sweep kJacobiFirstCheck on (every sweep when tracing), since no
+  // step converges sooner, and launches the earlier sweeps back to back
   std::vector<int32_t> act(nb, 1), cnt(nb, 0);
-  DevBuf dact = upload_vec(act), drot = upload_vec(cnt);
+  DevBuf dact = upload_vec(act), drot = upload_vec(cnt), dany(sizeof(int32_t));
   bool converged = false;
   int sweep = 0;
+  const int first_check = trace ? 1 : kJacobiFirstCheck;
   std::vector<int> last_sweep(nb, 0);  // per block, the last sweep that rotated (tracing)
   while (sweep < kJacobiMaxSweeps) {
     ++sweep;
     for (size_t r = 0; r + 1 < roff.size(); ++r)
       launch_svd_round(dtask.as<SvdTask>() + roff[r], roff[r + 1] - roff[r], dblk.as<SvdBlk>(), Xj, V.as<double>(),
                        dact.as<int32_t>(), drot.as<int32_t>(), cols, jl_max, s);
+    launch_svd_sweep_end(drot.as<int32_t>(), dact.as<int32_t>(), nb, dany.as<int32_t>(), s);
     hip_check(hipGetLastError(), "svd round launch");
-    hip_check(hipMemcpyAsync(cnt.data(), drot.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s), "svd count");
+    if (sweep < first_check && sweep < kJacobiMaxSweeps) continue;
+    int32_t any = 0;
+    hip_check(hipMemcpyAsync(&any, dany.p, sizeof(int32_t), hipMemcpyDeviceToHost, s), "svd count");
+    if (trace) hip_check(hipMemcpyAsync(act.data(), dact.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s), "svd act");
     hip_check(hipStreamSynchronize(s), "svd sweep");
-    bool any = false;
-    for (int k = 0; k < nb; ++k) {
-      act[k] = cnt[k] > 0 ? 1 : 0;
-      any = any || act[k];
-      if (trace && act[k]) last_sweep[k] = sweep;
-    }
+    if (trace)
+      for (int k = 0; k < nb; ++k)
+        if (act[k]) last_sweep[k] = sweep;
     if (!any) {
       converged = true;
       break;
     }
-    hip_check(hipMemcpy(dact.p, act.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice), "svd active");
-    hip_check(hipMemsetAsync(drot.p, 0, nb * sizeof(int32_t), s), "memset");
   }
   if (sweeps_out) *sweeps_out = sweep;
   phase();  // 3: sweeps
