@@ -79,7 +79,8 @@ struct TraceScope {
 // budget (MPSXX_PLAN_CACHE_GB, default 31 % of the device's memory), the least recently used plans release their slot
 // buffers and keep their task lists (the costly part to rebuild); a released plan allocates its buffers again from
 // the device pool on its next use.  Whole plans are evicted, least recently used first, only beyond kMax plans or
-// when the task lists alone exceed the budget.  The kKeep most recently used plans are never released or evicted,
+// when the task lists of the released plans alone exceed the budget.  The kKeep most recently used plans are never
+// released or evicted,
 // so the plans an operation holds at once (a Davidson solve over its operator groups) stay valid.
 class PlanCache {
  public:
@@ -182,7 +183,14 @@ class PlanCache {
       cp->charged = cp->plan.device_bytes();
       bytes_ += cp->charged;
     }
-    while (map_.size() > kKeep && bytes_ + add > budget()) evict_back();
+    // whole plans go only while the released plans' task lists alone exceed the budget: the kKeep most recent plans
+    // (the ones in use) are not counted, since evicting other plans would not shrink them
+    auto kept = [&] {
+      size_t b = 0, n = 0;
+      for (auto it = lru_.begin(); it != lru_.end() && n < kKeep; ++it, ++n) b += map_.find(*it)->second.first->charged;
+      return b;
+    };
+    while (map_.size() > kKeep && bytes_ - kept() + add > budget()) evict_back();
   }
   using Map = std::map<std::vector<uint64_t>,
                        std::pair<std::unique_ptr<CachedPlan>, std::list<std::vector<uint64_t>>::iterator>>;
@@ -206,7 +214,7 @@ class PlanCache {
   static size_t budget() {
     static const size_t b = [] {
       if (const char* e = std::getenv("MPSXX_PLAN_CACHE_GB"))
-        return (size_t)(std::max(1.0, std::atof(e)) * 1073741824.0);
+        return (size_t)(std::max(1e-3, std::atof(e)) * 1073741824.0);  // fractions of a GB allowed (tests)
       return device_memory_fraction(0.31, 64.0);
     }();
     return b;

@@ -40,15 +40,19 @@ def test_canonicalize_matches_oracle(forward, M):
     assert rec.max_abs_diff(recr) < 1e-11 * scale
 
 
-@pytest.mark.parametrize("solver", ["jacobi", "jacobi_noqr", "rocsolver"])
+@pytest.mark.parametrize("solver", ["jacobi", "jacobi_unsorted", "jacobi_noqr", "rocsolver"])
 @pytest.mark.parametrize("forward,M", [(True, 0), (False, 0), (True, 150), (False, 150)])
 def test_canonicalize_large_blocks_matches_oracle(forward, M, solver, monkeypatch):
     """Merged blocks of up to ~150 vectors (several column groups per Jacobi task, both orientations m < n and
     m > n, a graded spectrum from a rescaled psi) against the oracle's per-block SVD, for the default batched Jacobi
-    path (QR-preconditioned), the Jacobi sweeps without the QR step (MPSXX_SVD=jacobi_noqr, the path of blocks too
-    long for the QR kernels) and the rocSOLVER path (MPSXX_SVD=rocsolver)."""
+    path (vectors sorted by norm, then QR-preconditioned), the same without the sort (MPSXX_SVD_SORT=0), the Jacobi
+    sweeps without the QR step (MPSXX_SVD=jacobi_noqr, the path of blocks too long for the QR kernels) and the
+    rocSOLVER path (MPSXX_SVD=rocsolver)."""
     print("canonicalize large", forward, M, solver, flush=True)
-    monkeypatch.setenv("MPSXX_SVD", solver)
+    if solver == "jacobi_unsorted":
+        monkeypatch.setenv("MPSXX_SVD_SORT", "0")
+    else:
+        monkeypatch.setenv("MPSXX_SVD", solver)
     rng = np.random.default_rng(11)
     ch = mps.random_chain(8, 3, (8, 0), rng, dmax=40)
     psi = ch["psi"]
