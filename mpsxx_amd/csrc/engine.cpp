@@ -59,21 +59,25 @@ void Layout::set_shape(int rank_, bool qsym_, QN q_, std::vector<std::vector<QN>
 
 void Layout::finalize() {
   const int64_t nb = (int64_t)tag.size();
-  std::vector<int64_t> order(nb);
-  std::iota(order.begin(), order.end(), 0);
-  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return tag[x] < tag[y]; });
-  std::vector<int64_t> t2;
-  std::vector<int32_t> b2;
-  t2.reserve(nb);
-  b2.reserve(nb * rank);
-  for (int64_t k = 0; k < nb; ++k) {
-    const int64_t o = order[k];
-    if (!t2.empty() && t2.back() == tag[o]) continue;
-    t2.push_back(tag[o]);
-    b2.insert(b2.end(), bidx.begin() + o * rank, bidx.begin() + (o + 1) * rank);
+  bool ordered = true;  // blocks added in strictly ascending tag order (the planner's outputs) need no sort
+  for (int64_t k = 1; k < nb && ordered; ++k) ordered = tag[k - 1] < tag[k];
+  if (!ordered) {
+    std::vector<int64_t> order(nb);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return tag[x] < tag[y]; });
+    std::vector<int64_t> t2;
+    std::vector<int32_t> b2;
+    t2.reserve(nb);
+    b2.reserve(nb * rank);
+    for (int64_t k = 0; k < nb; ++k) {
+      const int64_t o = order[k];
+      if (!t2.empty() && t2.back() == tag[o]) continue;
+      t2.push_back(tag[o]);
+      b2.insert(b2.end(), bidx.begin() + o * rank, bidx.begin() + (o + 1) * rank);
+    }
+    tag.swap(t2);
+    bidx.swap(b2);
   }
-  tag.swap(t2);
-  bidx.swap(b2);
   off.resize(tag.size());
   bsize.resize(tag.size());
   nelem = 0;
@@ -556,34 +560,79 @@ ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const 
     int32_t p, s;
     bool operator==(const KQ& o) const { return key == o.key && p == o.p && s == o.s; }
   };
-  struct KQHash {
-    size_t operator()(const KQ& x) const {
-      return std::hash<int64_t>()(x.key * 1000003 + (int64_t)x.p * 4099 + x.s);
-    }
+  // The lists are runs of one array (B blocks grouped by (key, QN), each group in ascending free_B: a stable
+  // placement in free_B order), found through a flat open-addressing table of the (key, QN) groups.
+  std::vector<int64_t> bord(nB);
+  std::vector<int64_t> gstart;  // run of group g: bord[gstart[g], gstart[g + 1])
+  size_t tcap = 16;
+  while (tcap < 2 * (size_t)nB + 2) tcap <<= 1;
+  struct Slot {
+    KQ k;
+    int32_t g;  // -1: empty
   };
-  std::unordered_map<KQ, std::vector<int64_t>, KQHash> bmap;
-  bmap.reserve(nB * 2 + 1);
+  std::vector<Slot> table(tcap, Slot{KQ{0, 0, 0}, -1});
+  auto slot_of = [&](const KQ& k) {
+    uint64_t h = (uint64_t)k.key * 0x9e3779b97f4a7c15ull ^ ((uint64_t)(uint32_t)k.p << 32 | (uint32_t)k.s);
+    h ^= h >> 31;
+    h *= 0xbf58476d1ce4e5b9ull;
+    h ^= h >> 29;
+    size_t x = (size_t)h & (tcap - 1);
+    while (table[x].g >= 0 && !(table[x].k == k)) x = (x + 1) & (tcap - 1);
+    return x;
+  };
   {
-    std::vector<int64_t> ord(nB);
-    std::iota(ord.begin(), ord.end(), 0);
-    std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return Bs[x].free < Bs[y].free; });
-    for (int64_t j : ord) {
+    std::vector<int32_t> bgrp(nB);
+    std::vector<int64_t> cnt_g;
+    for (int64_t j = 0; j < nB; ++j) {
       const QN q = qsym ? Bs[j].q : QN();
-      bmap[KQ{Bs[j].key, q.p, q.s}].push_back(j);
+      const KQ k{Bs[j].key, q.p, q.s};
+      const size_t x = slot_of(k);
+      if (table[x].g < 0) {
+        table[x] = Slot{k, (int32_t)cnt_g.size()};
+        cnt_g.push_back(0);
+      }
+      bgrp[j] = table[x].g;
+      ++cnt_g[bgrp[j]];
     }
+    gstart.assign(cnt_g.size() + 1, 0);
+    for (size_t g = 0; g < cnt_g.size(); ++g) gstart[g + 1] = gstart[g] + cnt_g[g];
+    // B blocks in ascending free_B (counting sort when the range is moderate), placed into their groups in that order
+    std::vector<int64_t> byfree(nB);
+    if (nfreeB <= (int64_t(1) << 24)) {
+      std::vector<int64_t> cf((size_t)nfreeB + 1, 0);
+      for (int64_t j = 0; j < nB; ++j) ++cf[Bs[j].free + 1];
+      for (int64_t f = 0; f < nfreeB; ++f) cf[f + 1] += cf[f];
+      for (int64_t j = 0; j < nB; ++j) byfree[cf[Bs[j].free]++] = j;
+    } else {
+      std::iota(byfree.begin(), byfree.end(), 0);
+      std::stable_sort(byfree.begin(), byfree.end(), [&](int64_t x, int64_t y) { return Bs[x].free < Bs[y].free; });
+    }
+    std::vector<int64_t> pos(gstart.begin(), gstart.end() - 1);
+    for (int64_t j : byfree) bord[pos[bgrp[j]]++] = j;
   }
+  struct Span {
+    const int64_t* p;
+    size_t n;
+    const int64_t* begin() const { return p; }
+    const int64_t* end() const { return p + n; }
+    size_t size() const { return n; }
+  };
   const QN cq = C.q;
-  auto partners = [&](int64_t i) -> const std::vector<int64_t>* {
+  auto partners = [&](int64_t i) -> Span {
     const QN need = qsym ? cq - A[i].q : QN();
-    auto it = bmap.find(KQ{A[i].key, need.p, need.s});
-    return it == bmap.end() ? nullptr : &it->second;
+    const Slot& sl = table[slot_of(KQ{A[i].key, need.p, need.s})];
+    if (sl.g < 0) return Span{nullptr, 0};
+    return Span{bord.data() + gstart[sl.g], (size_t)(gstart[sl.g + 1] - gstart[sl.g])};
   };
   // A blocks by (free_A, key), then the boundaries of the free_A groups
   std::vector<int64_t> aord(nA);
   std::iota(aord.begin(), aord.end(), 0);
-  std::sort(aord.begin(), aord.end(), [&](int64_t x, int64_t y) {
+  auto a_before = [&](int64_t x, int64_t y) {
     return A[x].free != A[y].free ? A[x].free < A[y].free : A[x].key < A[y].key;
-  });
+  };
+  bool a_sorted = true;  // stored order already (free_A, key) when the contracted indices trail the free ones
+  for (int64_t i = 1; i < nA && a_sorted; ++i) a_sorted = a_before(i - 1, i);
+  if (!a_sorted) std::sort(aord.begin(), aord.end(), a_before);
   std::vector<size_t> gbeg;
   for (size_t r = 0; r < aord.size(); ++r)
     if (r == 0 || A[aord[r]].free != A[aord[r - 1]].free) gbeg.push_back(r);
@@ -603,8 +652,7 @@ ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const 
   int64_t work = 0;
   for (size_t g = 0; g < ngroups; ++g) {
     for (size_t r = gbeg[g]; r < gbeg[g + 1]; ++r) {
-      const std::vector<int64_t>* pl = partners(aord[r]);
-      if (pl) gwork[g] += (int64_t)pl->size();
+      gwork[g] += (int64_t)partners(aord[r]).size();
     }
     work += gwork[g];
   }
@@ -639,9 +687,7 @@ ContractSym contract_symbolic(const View& Av, const std::vector<int>& ca, const 
       loc.clear();
       for (size_t r = gbeg[g]; r < gbeg[g + 1]; ++r) {
         const int64_t i = aord[r];
-        const std::vector<int64_t>* pl = partners(i);
-        if (!pl) continue;
-        for (int64_t j : *pl) {
+        for (int64_t j : partners(i)) {
           if (use_mask && !mask_allows(af * nfreeB + Bs[j].free)) continue;
           loc.push_back({Bs[j].free, A[i].key, (int32_t)i, (int32_t)j});
         }
