@@ -224,6 +224,7 @@ def qc_sweep_leg(norb, schedule, precisions, max_final=6, keep_fp64_state=False)
     for prec in precisions:
         name = {mx.FP64: "fp64", mx.FP32: "fp32_mixed", mx.BF16X3: "bf16x3_mixed"}[prec]
         print("qc sweeps: %s" % name, file=sys.stderr, flush=True)
+        mx.release_caches()  # every precision starts without the plans (and pooled buffers) of the one before
         dm = DeviceDMRG(mpos, initial_mps(norb, (norb, 0), schedule[0], 0), schedule[0], form=SWEEP_MERGED)
         secs, e, st = [], None, None
         t0 = time.perf_counter()
@@ -267,6 +268,7 @@ def hubbard_sweep_leg(L, schedule, precisions):
     for prec in precisions:
         name = {mx.FP64: "fp64", mx.FP32: "fp32_mixed", mx.BF16X3: "bf16x3_mixed"}[prec]
         print("hubbard sweeps: %s" % name, file=sys.stderr, flush=True)
+        mx.release_caches()  # every precision starts without the plans (and pooled buffers) of the one before
         dm = DeviceDMRG(mpos, init, schedule[0])
         secs, e, st = [], None, None
         t0 = time.perf_counter()

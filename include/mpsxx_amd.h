@@ -338,6 +338,12 @@ int mpsxx_set_structure_hash_bits(int32_t bits);
  * be NULL.  No reference counterpart (diagnostic). */
 int mpsxx_device_memory_stats(int64_t* mallocs, int64_t* frees, double* malloc_ms, double* free_ms, int64_t* reuses,
                               int64_t* pool_syncs, int64_t* bytes_live, int64_t* cached_bytes);
+/* Drop every cached plan (sigma, diagonal, renormalization and merged-operator plans, with their task lists,
+ * intermediates and graphs) and return the pool's unused buffers to the device, after a device-wide
+ * synchronization.  Later calls rebuild what they need, with the same results.  Call it when no other call of the
+ * library is running (from any thread).  For callers that want the device memory back, and for timings that must
+ * start without plans of an earlier run.  No reference counterpart (the reference keeps no plans). */
+int mpsxx_release_caches(void);
 /* Host threads of the contraction planner (threads <= 0: MPSXX_HOST_THREADS or the hardware concurrency, at most 16)
  * and the candidate pairs per thread below which a contraction stays on fewer threads.  Plans are identical for any
  * setting.  No reference counterpart (the reference enumerates pairs serially, STBLAS.h:316-356). */

@@ -65,6 +65,11 @@ def set_structure_hash_bits(bits):
     _check(lib().mpsxx_set_structure_hash_bits(ctypes.c_int32(bits)))
 
 
+def release_caches():
+    """Drop every cached plan and return the pool's unused device buffers (mpsxx_release_caches)."""
+    _check(lib().mpsxx_release_caches())
+
+
 def device_memory_stats():
     """Device allocation counters (mpsxx_device_memory_stats): mallocs, frees, malloc_ms, free_ms, reuses,
     pool_syncs, bytes_live, cached_bytes."""

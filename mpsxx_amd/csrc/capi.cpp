@@ -682,6 +682,14 @@ int mpsxx_plan_cache_stats(int64_t* hits, int64_t* builds, int64_t* evictions, d
   });
 }
 
+int mpsxx_release_caches(void) {
+  return guard([&] {
+    mpx::plan_cache_clear();
+    mpx::merged_plan_cache_clear();
+    mpx::device_pool_trim();
+  });
+}
+
 int mpsxx_plan_cache_collisions(int64_t* collisions) {
   return guard_host([&] {
     if (!collisions) throw mpx::Error(MPSXX_ERR_ARG, "plan cache collisions: null output");

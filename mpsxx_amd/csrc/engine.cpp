@@ -244,6 +244,14 @@ DevicePool& pool() {
 
 }  // namespace
 
+void device_pool_trim() {
+  DevicePool& P = pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  hip_check(hipDeviceSynchronize(), "device pool trim");
+  P.drain(P.safe);
+  P.drain(P.pending);
+}
+
 DeviceMemoryStats device_memory_stats() {
   DevicePool& P = pool();
   std::lock_guard<std::mutex> lk(P.mu);

@@ -106,6 +106,8 @@ ContractSym contract_symbolic(const View& A, const std::vector<int>& ca, const V
 // Planner threading: threads > 0 fixes the host thread count (0: MPSXX_HOST_THREADS or the hardware concurrency, at
 // most 16); a contraction uses one thread per min_pairs candidate pairs at most.  Plans do not depend on either.
 void set_planner_threads(int threads, int64_t min_pairs);
+// Return every pooled (released, unused) device buffer to the device, after a device-wide synchronization.
+void device_pool_trim();
 int host_threads();
 // Run f(0..n-1) on n host threads (f(0) on the caller); the first exception is rethrown after all have finished.
 void parallel_for(int n, const std::function<void(int)>& f);

@@ -418,6 +418,8 @@ bool merged_operator_scalar(bool fwd, Tensor& xmpo, Tensor& xopr, Tensor& mopr, 
 
 }  // namespace
 
+void merged_plan_cache_clear() { merged_cache().clear(); }
+
 void clear_merged_operator_cache() { merged_cache().clear(); }
 
 // Merged block operator (reference src/optimize.hpp:273-368), one operator group:

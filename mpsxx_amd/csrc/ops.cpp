@@ -235,6 +235,7 @@ PlanCache& cache() {
 }  // namespace
 
 PlanCacheStats plan_cache_stats() { return cache().stats(); }
+void plan_cache_clear() { cache().clear(); }
 
 namespace {
 thread_local std::vector<uint64_t> t_plan_key;  // key of the plan the last plan_sigma_* call returned

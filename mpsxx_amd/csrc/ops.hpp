@@ -80,6 +80,10 @@ struct PlanCacheStats {
   double build_ms = 0.0;
 };
 PlanCacheStats plan_cache_stats();
+// Drop every cached plan (their task lists, intermediates and graphs go back to the device pool).
+void plan_cache_clear();
+// Drop every cached merged-operator plan (merge.cpp).
+void merged_plan_cache_clear();
 // The most recently used cached plans are never evicted: a caller may hold references to at most this many plans
 // from the accessors below at once (the Davidson driver holds one per operator group).
 constexpr size_t kPlanCacheKeep = 16;

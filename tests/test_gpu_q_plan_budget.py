@@ -57,3 +57,24 @@ def test_small_plan_budget_gives_the_same_energies():
     st = small["stats"]
     assert st["releases"] > 0 and st["restores"] > 0 and st["evictions"] > 0
     assert ref["stats"]["evictions"] == 0
+
+
+def test_release_caches_keeps_results_and_frees_the_pool():
+    """mpsxx_release_caches between sweeps: no plan and no pooled buffer is left, and the next sweeps (which rebuild
+    their plans) give the energies of the same sweeps run without it, bit for bit."""
+    from mpsxx_amd.dmrg import DeviceDMRG, initial_mps
+    from mpsxx_amd.mpo import hubbard_mpo
+    print("release caches", flush=True)
+    out = []
+    for release in (False, True):
+        dm = DeviceDMRG(hubbard_mpo(12, 1.0, 4.0), initial_mps(12, (12, 0), 32, 0), 32)
+        es = [dm.sweep(32, 1e-10)[0]]
+        if release:
+            mx.release_caches()
+            assert mx.plan_cache_stats()["plans"] == 0
+            assert mx.device_memory_stats()["cached_bytes"] == 0
+        for M in (64, 64):
+            es.append(dm.sweep(M, 1e-10, precision=mx.FP32)[0])
+        out.append(es)
+        del dm
+    assert out[0] == out[1], out
