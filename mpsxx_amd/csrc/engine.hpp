@@ -143,7 +143,10 @@ struct GemmPolicy {
   int split_big = 16384;
   int split_small = 1024;
   int split_seg = 32;
-  int big_tile_min = 96;  // blocks with both sides >= this run as 128x128 tiles (gemm128_kernel); 0: never
+  // blocks with both sides >= this run as 128x128 tiles (gemm128_kernel); 0: never.  On the norb=24 M=1024 merged
+  // sigma 64 beats 96 by ~2 % in FP32 (11.55 against 11.77 ms) and ~1 % in FP64: the 64..95 blocks run faster as
+  // padded 128x128 tiles than as 64x64 tiles
+  int big_tile_min = 64;
 };
 // the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT, MPSXX_SPLIT_UNITS, MPSXX_SPLIT_BIG, MPSXX_SPLIT_SMALL,
 // MPSXX_SPLIT_SEG, MPSXX_BIG_TILE_MIN
