@@ -1029,12 +1029,25 @@ GemmPolicy default_policy(int precision) {
   // measured on the norb=24 M=1024 merged sigma: 128x128 tiles take FP32 from 22.0 to 19.5 ms; FP64 with two
   // work-groups per CU (128 accumulator registers per lane) from 25.5 to 24.3 ms against 64x64 tiles throughout (with
   // one work-group per CU it had been slower).  MPSXX_BIG_TILE_MIN_F64 = 0 keeps FP64 on 64x64 tiles.
+  // The threshold per precision, measured on that sigma and on the same bond of the converged M=1024 sweep state
+  // (smaller sectors; `snapshot`), FP64 / FP32 / split-BF16x3 in ms:
+  //   threshold    synthetic              converged state
+  //   FP64  96 / 192     23.98 / 24.03          9.05 / 7.26   (64x64 throughout: 25.48 / 7.12)
+  //   FP32  64 / 192     11.54 / 11.80          4.71 / 4.09
+  //   BF16x3 64 / 192     6.98 / 7.31           3.22 / 3.35
+  // so 192 for FP64 and FP32 (the QC sweeps: FP64 11.7 -> 11.0 s) and 64 for split-BF16x3.  MPSXX_BIG_TILE_MIN sets
+  // the FP32 and split-BF16x3 thresholds, MPSXX_BIG_TILE_MIN_F64 the FP64 one.
   static const int f64_min = [] {
     const char* e = std::getenv("MPSXX_BIG_TILE_MIN_F64");
     return e ? std::max(0, std::atoi(e)) : GemmPolicy().big_tile_min;
   }();
+  static const int x3_min = [] {
+    const char* e = std::getenv("MPSXX_BIG_TILE_MIN");
+    return e ? std::max(0, std::atoi(e)) : kBigTileMinX3;
+  }();
   GemmPolicy p = default_policy();
   if (precision == MPSXX_FP64) p.big_tile_min = f64_min;
+  if (precision == MPSXX_BF16X3) p.big_tile_min = x3_min;
   return policy_for(p, precision);
 }
 

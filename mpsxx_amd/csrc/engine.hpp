@@ -139,15 +139,16 @@ struct GemmPolicy {
   int max_groups = 0;
   // in 64x64-tile equivalents (a 128x128 tile counts four); measured on the norb=24 M=1024 merged sigma, whose
   // second step holds a few hundred long-K tiles: 2048 -> 16384 takes FP64 from 32.9 to 29.1 ms, FP32 from 19.7 to
-  // 17.3 ms and split-BF16x3 from 14.5 to 13.7 ms, with the L=32 M=512 Hubbard sigma unchanged
-  int split_big = 16384;
+  // 17.3 ms and split-BF16x3 from 14.5 to 13.7 ms, with the L=32 M=512 Hubbard sigma unchanged; with the later
+  // 128x128 tiles 16384 -> 32768 takes FP32 from 11.50 to 11.35 ms and FP64 from 24.16 to 24.06 ms
+  int split_big = 32768;
   int split_small = 1024;
   int split_seg = 32;
-  // blocks with both sides >= this run as 128x128 tiles (gemm128_kernel); 0: never.  On the norb=24 M=1024 merged
-  // sigma 64 beats 96 by ~2 % in FP32 (11.55 against 11.77 ms) and ~1 % in FP64: the 64..95 blocks run faster as
-  // padded 128x128 tiles than as 64x64 tiles
-  int big_tile_min = 64;
+  // blocks with both sides >= this run as 128x128 tiles (gemm128_kernel); 0: never.  The FP64 and FP32 default;
+  // split-BF16x3 uses kBigTileMinX3 (the measurements are at default_policy(int))
+  int big_tile_min = 192;
 };
+constexpr int kBigTileMinX3 = 64;  // split-BF16x3 threshold of the 128x128 tiles
 // the defaults above, overridable by MPSXX_SMALL_GEMM_LIMIT, MPSXX_SPLIT_UNITS, MPSXX_SPLIT_BIG, MPSXX_SPLIT_SMALL,
 // MPSXX_SPLIT_SEG, MPSXX_BIG_TILE_MIN
 GemmPolicy default_policy();

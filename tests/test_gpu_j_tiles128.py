@@ -45,19 +45,38 @@ def test_two_site_sigma_on_128_tiles(prec, tol):
     assert rel <= tol
 
 
+_MERGED_CHILD = r"""
+import json, sys
+sys.path[:0] = [sys.argv[1], sys.argv[1] + "/tests"]
+import mpsxx_amd as mx
+import oracle
+from mpsxx_amd import workloads
+from mpsxx_amd.batch import last_plan_tiles
+prec = int(sys.argv[2])
+Lm, Rm, psim, info = workloads.qc_merged_problem(12, 384, seed=5)
+ref = oracle.sigma_merged(Lm.to_qst(), Rm.to_qst(), psim.to_qst())
+got = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=prec).to_qst()
+t = last_plan_tiles()
+print(json.dumps([got.block_set() == ref.block_set(), got.diff_norm(ref) / ref.norm(), t["big128"], t["big128_red"]]),
+      flush=True)
+"""
+
+
 @pytest.mark.parametrize("prec,tol", [(mx.FP64, 1e-13), (mx.FP32, 2e-5), (mx.BF16X3, 1e-4)])
 def test_merged_qc_sigma_split_reductions_on_128_tiles(prec, tol):
-    """norb 12 at M = 384: merged sectors above 128 on both sides; the second product's long reductions are split
-    over work-groups (partial 128x128 tiles summed in slot order)."""
+    """norb 12 at M = 384: merged sectors of 100..190 on both sides, on 128x128 tiles from 96 rows on (the FP64 and
+    FP32 defaults start at 192, so a child process sets MPSXX_BIG_TILE_MIN(_F64) = 96; read once per process); the
+    second product's long reductions are split over work-groups (partial 128x128 tiles summed in slot order)."""
     print("128 tiles merged qc", prec, flush=True)
-    Lm, Rm, psim, info = workloads.qc_merged_problem(12, 384, seed=5)
-    ref = oracle.sigma_merged(Lm.to_qst(), Rm.to_qst(), psim.to_qst())
-    got = mx.compute_sigma_vector_merged(Lm, Rm, psim, precision=prec).to_qst()
-    tiles = last_plan_tiles()
-    rel = got.diff_norm(ref) / ref.norm()
-    print("tiles", tiles, "rel", rel, flush=True)
-    assert tiles["big128"] > 0 and tiles["big128_red"] > 0
-    assert got.block_set() == ref.block_set()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, MPSXX_BIG_TILE_MIN="96", MPSXX_BIG_TILE_MIN_F64="96")
+    r = subprocess.run([sys.executable, "-c", _MERGED_CHILD, root, str(prec)], env=env, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    same, rel, big, red = json.loads(r.stdout.strip().splitlines()[-1])
+    print("128 tiles", big, "split", red, "rel", rel, flush=True)
+    assert big > 0 and red > 0
+    assert same
     assert rel <= tol
 
 
@@ -89,7 +108,7 @@ print(json.dumps(out), flush=True)
 
 
 def test_fp64_64_tiles_when_128_tiles_are_off():
-    """FP64 runs its large blocks on gemm128_kernel<double> by default; MPSXX_BIG_TILE_MIN_F64=0 (read once per
+    """FP64 runs its large blocks (both sides >= 192) on gemm128_kernel<double> by default; MPSXX_BIG_TILE_MIN_F64=0 (read once per
     process, so in a child) keeps them on 64x64 tiles: same bar, no 128x128 tile."""
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     env = dict(os.environ, MPSXX_BIG_TILE_MIN_F64="0")
