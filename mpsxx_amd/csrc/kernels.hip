@@ -1369,23 +1369,28 @@ __global__ __launch_bounds__(256) void svd_rt_kernel(const SvdQrBlk* __restrict_
   }
 }
 
+// Z = Q [W; 0] for the CW-wide column chunk c0 of one block per work-group (the columns are independent, so the
+// chunks of one block run in parallel): the chunk starts as [W; 0], then the panels' block reflectors are applied
+// backwards, Z <- (I - V T V^T) Z, each through LDS.
 template <int CW>
-__global__ __launch_bounds__(256) void svd_qr_apply_kernel(const SvdQrBlk* __restrict__ blks,
+__global__ __launch_bounds__(256) void svd_qr_apply_kernel(const int2* __restrict__ items,
+                                                           const SvdQrBlk* __restrict__ blks,
                                                            const double* __restrict__ X, const double* __restrict__ T,
                                                            const double* __restrict__ W, double* Z) {
   extern __shared__ double sm[];
-  const SvdQrBlk b = blks[blockIdx.x];
-  const int len = b.len, kk = b.kk, ld = svd_lds_ld(len);
+  const int2 it = items[blockIdx.x];
+  const SvdQrBlk b = blks[it.x];
+  const int len = b.len, kk = b.kk, ld = svd_lds_ld(len), c0 = it.y, cw = min(CW, kk - c0);
   double* Vp = sm;
   double* Cs = Vp + ld * kSvdQrPanel;
   double* Ts = Cs + ld * CW;
   double* Ws = Ts + 256;
   double* W2 = Ws + 16 * CW;
-  double* z = Z + b.z_off;
+  double* z = Z + b.z_off + (int64_t)c0 * len;
   const int tid = threadIdx.x;
-  for (int64_t e = tid; e < (int64_t)len * kk; e += 256) {  // Z = [W; 0]
-    const int j = (int)(e / len), i = (int)(e - (int64_t)j * len);
-    z[e] = i < kk ? W[b.w_off + (int64_t)j * kk + i] : 0.0;
+  for (int e = tid; e < len * cw; e += 256) {  // the chunk of Z = [W; 0]
+    const int j = e / len, i = e - j * len;
+    z[e] = i < kk ? W[b.w_off + (int64_t)(c0 + j) * kk + i] : 0.0;
   }
   __syncthreads();
   const int np = (kk + kSvdQrPanel - 1) / kSvdQrPanel;
@@ -1398,18 +1403,14 @@ __global__ __launch_bounds__(256) void svd_qr_apply_kernel(const SvdQrBlk* __res
       Vp[e] = v;
     }
     for (int e = tid; e < 256; e += 256) Ts[e] = T[b.t_off + (int64_t)pi * 256 + e];
+    svd_qr_gather(z + p0, len, K, cw, CW, ld, Cs, tid);
     __syncthreads();
-    for (int c0 = 0; c0 < kk; c0 += CW) {
-      const int cw = min(CW, kk - c0);
-      svd_qr_gather(z + (int64_t)c0 * len + p0, len, K, cw, CW, ld, Cs, tid);
-      __syncthreads();
-      svd_qr_block_reflect<CW>(Vp, Cs, Ts, Ws, W2, K, ld, false, tid);
-      for (int e = tid; e < cw * K; e += 256) {
-        const int c = e / K, i = e - c * K;
-        z[(int64_t)(c0 + c) * len + p0 + i] = Cs[c * ld + i];
-      }
-      __syncthreads();
+    svd_qr_block_reflect<CW>(Vp, Cs, Ts, Ws, W2, K, ld, false, tid);
+    for (int e = tid; e < cw * K; e += 256) {
+      const int c = e / K, i = e - c * K;
+      z[(int64_t)c * len + p0 + i] = Cs[c * ld + i];
     }
+    __syncthreads();
   }
 }
 
@@ -1528,12 +1529,12 @@ void launch_svd_rt(const SvdQrBlk* blks, int nb, const double* X, double* Y, hip
   if (nb > 0) hipLaunchKernelGGL(svd_rt_kernel, dim3(nb), dim3(256), 0, s, blks, X, Y);
 }
 
-void launch_svd_qr_apply(const SvdQrBlk* blks, int nb, const double* X, const double* T, const double* W, double* Z,
-                         int cw, int len_max, hipStream_t s) {
-  if (nb <= 0) return;
+void launch_svd_qr_apply(const int2* items, int nitems, const SvdQrBlk* blks, const double* X, const double* T,
+                         const double* W, double* Z, int cw, int len_max, hipStream_t s) {
+  if (nitems <= 0) return;
   const size_t lds = svd_qr_lds_bytes(cw, len_max);
-  if (cw == 32) hipLaunchKernelGGL(svd_qr_apply_kernel<32>, dim3(nb), dim3(256), lds, s, blks, X, T, W, Z);
-  else hipLaunchKernelGGL(svd_qr_apply_kernel<16>, dim3(nb), dim3(256), lds, s, blks, X, T, W, Z);
+  if (cw == 32) hipLaunchKernelGGL(svd_qr_apply_kernel<32>, dim3(nitems), dim3(256), lds, s, items, blks, X, T, W, Z);
+  else hipLaunchKernelGGL(svd_qr_apply_kernel<16>, dim3(nitems), dim3(256), lds, s, items, blks, X, T, W, Z);
 }
 
 void launch_svd_init(const SvdBlk* blks, int nb, double* V, hipStream_t s) {

@@ -229,8 +229,9 @@ void launch_svd_qr(const SvdQrBlk* blks, int nb, double* X, double* tau, double*
                    hipStream_t s);
 // Y = R^T (zero above the diagonal) and Z = [W; 0] from W (kk x kk), then Z <- Q Z.
 void launch_svd_rt(const SvdQrBlk* blks, int nb, const double* X, double* Y, hipStream_t s);
-void launch_svd_qr_apply(const SvdQrBlk* blks, int nb, const double* X, const double* T, const double* W, double* Z,
-                         int cw, int len_max, hipStream_t s);
+// items: (block, first column) of every cw-wide column chunk of the blocks' Z, one work-group each.
+void launch_svd_qr_apply(const int2* items, int nitems, const SvdQrBlk* blks, const double* X, const double* T,
+                         const double* W, double* Z, int cw, int len_max, hipStream_t s);
 // Column sorting before the QR, a cheap stand-in for column pivoting: each block's columns of X (len x kk) are copied
 // to Xs (same offsets) in order of decreasing Euclidean norm, ties by index, and perm[tau_off + r] is the column
 // placed at r.  R's rows are then graded, and the sweeps over R^T converge in fewer passes.  After the sweeps the rows

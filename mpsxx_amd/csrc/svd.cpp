@@ -419,7 +419,12 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
   std::vector<SvdOut> outs(nb);
   DevBuf Yu;
   if (qr) {
-    launch_svd_qr_apply(dqb.as<SvdQrBlk>(), nb, Xq, T.as<double>(), V.as<double>(), Z.as<double>(), qcw, len_max, s);
+    std::vector<int2> items;  // the column chunks of every block's Z, one work-group each
+    for (int k = 0; k < nb; ++k)
+      for (int c0 = 0; c0 < qb[k].kk; c0 += qcw) items.push_back(make_int2(k, c0));
+    DevBuf ditems = upload_vec(items);
+    launch_svd_qr_apply(ditems.as<int2>(), (int)items.size(), dqb.as<SvdQrBlk>(), Xq, T.as<double>(), V.as<double>(),
+                        Z.as<double>(), qcw, len_max, s);
     if (sort_cols) {
       Yu = DevBuf(std::max<int64_t>(y_elems, 1) * sizeof(double));
       launch_svd_unsort_rows(dqb.as<SvdQrBlk>(), nb, perm.as<int32_t>(), Y.as<double>(), Yu.as<double>(), s);
