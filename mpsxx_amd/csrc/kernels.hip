@@ -1359,6 +1359,138 @@ __global__ __launch_bounds__(256) void svd_qr_kernel(const SvdQrBlk* __restrict_
   }
 }
 
+// The same factorization split into launches per panel, so that the trailing update, the bulk of the work, runs on
+// one work-group per 16-column chunk instead of one per block: svd_qr_panel_kernel factors panel `pi` of every block
+// (one work-group per block; blocks with fewer columns return) and writes it back with its tau and T, then
+// svd_qr_trail_kernel applies that panel's block reflector to every chunk (block, c0) right of it (other items
+// return).  Each column goes through the same operations as in svd_qr_kernel.
+__global__ __launch_bounds__(256) void svd_qr_panel_kernel(const SvdQrBlk* __restrict__ blks, double* X, double* tau,
+                                                           double* T, int pi) {
+  extern __shared__ double sm[];
+  const SvdQrBlk b = blks[blockIdx.x];
+  const int len = b.len, kk = b.kk, ld = svd_lds_ld(len), p0 = pi * kSvdQrPanel;
+  if (p0 >= kk) return;
+  double* Vp = sm;
+  double* Ts = Vp + ld * kSvdQrPanel;
+  double* Ws = Ts + 256;
+  double* red = Ws + 256;  // 4 partial sums, tau, scale
+  double* tl = red + 16;   // the panel's tau values
+  double* x = X + b.x_off;
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  const int nbp = min(kSvdQrPanel, kk - p0), K = len - p0;
+  svd_qr_gather(x + (int64_t)p0 * len + p0, len, K, nbp, kSvdQrPanel, ld, Vp, tid);
+  if (tid < 16) tl[tid] = 0.0;
+  __syncthreads();
+  for (int k = 0; k < nbp; ++k) {
+    double* col = Vp + k * ld;
+    double ss = 0.0;
+    for (int i = k + 1 + tid; i < K; i += 256) ss += col[i] * col[i];
+    ss = wave_sum(ss);
+    if (l == 0) red[w] = ss;
+    __syncthreads();
+    if (tid == 0) {
+      const double xn2 = red[0] + red[1] + red[2] + red[3], alpha = col[k];
+      double tk = 0.0, scale = 0.0, beta = alpha;
+      if (xn2 > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+        tk = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+      }
+      red[4] = tk;
+      red[5] = scale;
+      tl[k] = tk;
+      tau[b.tau_off + p0 + k] = tk;
+      col[k] = beta;
+    }
+    __syncthreads();
+    const double tk = red[4], scale = red[5];
+    for (int i = k + 1 + tid; i < K; i += 256) col[i] *= scale;
+    __syncthreads();
+    if (tk != 0.0) {  // H_k = I - tau v v^T on the panel's later columns, one wave per column
+      for (int j = k + 1 + w; j < nbp; j += 4) {
+        double* cj = Vp + j * ld;
+        double d = 0.0;
+        for (int i = k + 1 + l; i < K; i += 64) d += col[i] * cj[i];
+        d = wave_sum(d) + cj[k];
+        const double f = tk * d;
+        for (int i = k + 1 + l; i < K; i += 64) cj[i] -= f * col[i];
+        if (l == 0) cj[k] -= f;
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < nbp * K; e += 256) {
+    const int c = e / K, i = e - c * K;
+    x[(int64_t)(p0 + c) * len + p0 + i] = Vp[c * ld + i];
+  }
+  __syncthreads();
+  for (int e = tid; e < nbp * K; e += 256) {
+    const int c = e / K, i = e - c * K;
+    if (i <= c) Vp[c * ld + i] = (i == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (w == 0) {  // S = V^T V (one MFMA tile) into Ws, then T column by column
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int K4 = (K + 3) & ~3;
+    for (int k0 = 0; k0 < K4; k0 += 4) {
+      const double a = Vp[(l % 16) * ld + k0 + l / 16];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, a, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Ws[(l / 16 + 4 * j) * 16 + l % 16] = acc[j];
+  }
+  __syncthreads();
+  for (int i = 0; i < 16; ++i) {
+    if (tid < 16) {
+      const int j = tid;
+      double tv = 0.0;
+      if (j < i) {
+        double acc = 0.0;
+        for (int m = j; m < i; ++m) acc += Ts[j * 16 + m] * Ws[m * 16 + i];
+        tv = -tl[i] * acc;
+      } else if (j == i) {
+        tv = tl[i];
+      }
+      Ts[j * 16 + i] = tv;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < 256; e += 256) T[b.t_off + (int64_t)pi * 256 + e] = Ts[e];
+}
+
+__global__ __launch_bounds__(256) void svd_qr_trail_kernel(const int2* __restrict__ items,
+                                                           const SvdQrBlk* __restrict__ blks, double* X,
+                                                           const double* __restrict__ T, int pi) {
+  constexpr int CW = kSvdQrPanel;
+  extern __shared__ double sm[];
+  const int2 it = items[blockIdx.x];
+  const SvdQrBlk b = blks[it.x];
+  const int len = b.len, kk = b.kk, ld = svd_lds_ld(len), p0 = pi * kSvdQrPanel, c0 = it.y;
+  if (c0 < p0 + kSvdQrPanel || c0 >= kk) return;  // uniform: the chunk is not right of this panel
+  const int nbp = min(kSvdQrPanel, kk - p0), K = len - p0, cw = min(CW, kk - c0);
+  double* Vp = sm;
+  double* Cs = Vp + ld * kSvdQrPanel;
+  double* Ts = Cs + ld * CW;
+  double* Ws = Ts + 256;
+  double* W2 = Ws + 16 * CW;
+  double* x = X + b.x_off;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kSvdQrPanel * ld; e += 256) {
+    const int c = e / ld, i = e - c * ld;
+    double v = 0.0;
+    if (c < nbp && i < K) v = i < c ? 0.0 : (i == c ? 1.0 : x[(int64_t)(p0 + c) * len + p0 + i]);
+    Vp[e] = v;
+  }
+  for (int e = tid; e < 256; e += 256) Ts[e] = T[b.t_off + (int64_t)pi * 256 + e];
+  svd_qr_gather(x + (int64_t)c0 * len + p0, len, K, cw, CW, ld, Cs, tid);
+  __syncthreads();
+  svd_qr_block_reflect<CW>(Vp, Cs, Ts, Ws, W2, K, ld, true, tid);
+  for (int e = tid; e < cw * K; e += 256) {
+    const int c = e / K, i = e - c * K;
+    x[(int64_t)(c0 + c) * len + p0 + i] = Cs[c * ld + i];
+  }
+}
+
 __global__ __launch_bounds__(256) void svd_rt_kernel(const SvdQrBlk* __restrict__ blks, const double* __restrict__ X,
                                                      double* Y) {
   const SvdQrBlk b = blks[blockIdx.x];
@@ -1498,7 +1630,8 @@ size_t svd_round_lds_limit() {
                          (const void*)svd_round_kernel<32>,    (const void*)svd_finish_kernel,
                          (const void*)svd_qr_kernel<16>,       (const void*)svd_qr_kernel<32>,
                          (const void*)svd_qr_apply_kernel<16>, (const void*)svd_qr_apply_kernel<32>,
-                         (const void*)svd_sort_kernel};
+                         (const void*)svd_sort_kernel,         (const void*)svd_qr_panel_kernel,
+                         (const void*)svd_qr_trail_kernel};
     // the dynamic limit leaves room for the largest static LDS among these kernels (the round kernel's G buffers)
     size_t stat = 0;
     for (const void* f : fns) {
@@ -1523,6 +1656,17 @@ void launch_svd_qr(const SvdQrBlk* blks, int nb, double* X, double* tau, double*
   const size_t lds = svd_qr_lds_bytes(cw, len_max);
   if (cw == 32) hipLaunchKernelGGL(svd_qr_kernel<32>, dim3(nb), dim3(256), lds, s, blks, X, tau, T);
   else hipLaunchKernelGGL(svd_qr_kernel<16>, dim3(nb), dim3(256), lds, s, blks, X, tau, T);
+}
+
+void launch_svd_qr_split(const SvdQrBlk* blks, int nb, const int2* items, int nitems, int npanels, double* X,
+                         double* tau, double* T, int len_max, hipStream_t s) {
+  if (nb <= 0) return;
+  const size_t lds = svd_qr_lds_bytes(kSvdQrPanel, len_max);
+  for (int pi = 0; pi < npanels; ++pi) {
+    hipLaunchKernelGGL(svd_qr_panel_kernel, dim3(nb), dim3(256), lds, s, blks, X, tau, T, pi);
+    if (nitems > 0 && pi + 1 < npanels)
+      hipLaunchKernelGGL(svd_qr_trail_kernel, dim3(nitems), dim3(256), lds, s, items, blks, X, T, pi);
+  }
 }
 
 void launch_svd_rt(const SvdQrBlk* blks, int nb, const double* X, double* Y, hipStream_t s) {

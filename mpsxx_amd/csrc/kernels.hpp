@@ -227,6 +227,11 @@ struct SvdQrBlk {
 size_t svd_qr_lds_bytes(int cw, int len_max);
 void launch_svd_qr(const SvdQrBlk* blks, int nb, double* X, double* tau, double* T, int cw, int len_max,
                    hipStream_t s);
+// The same factorization as 2 x npanels launches: per panel, its factorization (one work-group per block), then the
+// update of the columns right of it in chunks of kSvdQrPanel, one work-group per item (block, first column) of
+// `items` (every block's chunks from column kSvdQrPanel on).  Needs svd_qr_lds_bytes(kSvdQrPanel, len_max) of LDS.
+void launch_svd_qr_split(const SvdQrBlk* blks, int nb, const int2* items, int nitems, int npanels, double* X,
+                         double* tau, double* T, int len_max, hipStream_t s);
 // Y = R^T (zero above the diagonal) and Z = [W; 0] from W (kk x kk), then Z <- Q Z.
 void launch_svd_rt(const SvdQrBlk* blks, int nb, const double* X, double* Y, hipStream_t s);
 // items: (block, first column) of every cw-wide column chunk of the blocks' Z, one work-group each.

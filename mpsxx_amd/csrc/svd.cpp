@@ -332,7 +332,19 @@ bool jacobi_svd(const std::vector<Blk>& blks, const std::vector<Rect>& rects, co
       launch_svd_sort(dqb.as<SvdQrBlk>(), nb, X.as<double>(), Xs.as<double>(), perm.as<int32_t>(), kk_max, s);
       Xq = Xs.as<double>();
     }
-    launch_svd_qr(dqb.as<SvdQrBlk>(), nb, Xq, tau.as<double>(), T.as<double>(), qcw, len_max, s);
+    // per panel: its factorization, then the columns right of it in 16-column chunks on work-groups of their own
+    // (MPSXX_SVD_QR_SPLIT=0: the whole factorization of a block on one work-group)
+    const char* split_env = std::getenv("MPSXX_SVD_QR_SPLIT");
+    if (split_env && std::strcmp(split_env, "0") == 0) {
+      launch_svd_qr(dqb.as<SvdQrBlk>(), nb, Xq, tau.as<double>(), T.as<double>(), qcw, len_max, s);
+    } else {
+      std::vector<int2> trail;
+      for (int k = 0; k < nb; ++k)
+        for (int c0 = kSvdQrPanel; c0 < qb[k].kk; c0 += kSvdQrPanel) trail.push_back(make_int2(k, c0));
+      DevBuf dtrail = upload_vec(trail);
+      launch_svd_qr_split(dqb.as<SvdQrBlk>(), nb, dtrail.as<int2>(), (int)trail.size(),
+                          (kk_max + kSvdQrPanel - 1) / kSvdQrPanel, Xq, tau.as<double>(), T.as<double>(), len_max, s);
+    }
     launch_svd_rt(dqb.as<SvdQrBlk>(), nb, Xq, Y.as<double>(), s);
     hip_check(hipGetLastError(), "svd qr launch");
   }
