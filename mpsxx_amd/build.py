@@ -19,7 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I" + os.path.join(ROOT, "include"), "-I" + SRC,
           "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
-# device code: no packed-FP32 instructions (build notes)
+# device code: no packed-FP32 instructions (the compiler is kept from forming v_pk_{add,mul,fma}_f32 from scalar or
+# float2 code; the kernels' FP32 arithmetic stays in unpacked instructions)
 DEVICE = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 HOST = ["-D__HIP_PLATFORM_AMD__"]

@@ -4,7 +4,7 @@
   packed index = row-major product of the merged indices, merged sectors numbered by first appearance, sub-blocks of
   one merged sector stacked in ascending packed index);
 * the merged operators L~ = merge(L W), R~ = merge(W R) against a dense einsum through the same merge map;
-* merged sigma, merged diagonal and merged renormalization against the 4-index forms after merging (SURVEY's
+* merged sigma, merged diagonal and merged renormalization against the 4-index forms after merging (the invariant
   "4-index sigma == merged sigma after merging"), which test_oracle_pins.py ties to exact diagonalization, and the
   merged renormalization chain against <Psi|H|Psi>;
 * the one-site diagonal against the diagonal of the one-site P^T H P;
