@@ -15,7 +15,8 @@ Beside it, on the same inputs: the FP64 and split-BF16x3 sigma.  The sigma on a 
 the middle bond of the state the FP64 sweeps below converge to.  The sweep half of the metric: merged two-site
 sweeps of the same chain over M 256, 512, then 1024 until the energy changes by less than 1e-10, in FP64 and in
 mixed precision (FP32 or split-BF16x3 sigma vectors, FP64 Davidson, refinement and SVD), with sweep times and
-|E - E_fp64|.  The CPU baseline: the oracle (FP64 restatement of the reference's block contraction) with its block
+|E - E_fp64|; each precision's sweeps start with the library's plan and buffer caches cleared
+(mpsxx_release_caches), so that none runs on plans an earlier leg built.  The CPU baseline: the oracle (FP64 restatement of the reference's block contraction) with its block
 dgemm in numpy's bundled OpenBLAS, on one core and on all cores given to the process, over a bounded sample of the
 headline sigma (the terms of every k-th sector of the MPO bond index; sigma is a sum over that index), compared
 with the device's FP64 result on the same sample.  Secondary legs: the two-site sigma of the Hubbard chain L = 32,
