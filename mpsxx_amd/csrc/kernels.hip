@@ -827,28 +827,20 @@ __global__ void svd_init_kernel(const SvdBlk* __restrict__ blks, double* V) {
 __host__ __device__ __forceinline__ int svd_lds_ld(int len) { return ((len + 15) & ~15) + 4; }
 
 // Columns gcol(0..nc-1) of a column-major matrix (column length len) into LDS columns of stride ld; rows len..ld-1
-// and columns nc..cols-1 are zeroed.  Up to 32 independent loads per thread are in flight before the LDS writes.
+// and columns nc..cols-1 are zeroed.  Thread tid moves rows tid, tid + nt, ... of every column (consecutive threads,
+// consecutive rows of one column), up to 32 independent loads in flight before the LDS writes.
 template <typename F>
 __device__ __forceinline__ void svd_gather_columns(const double* __restrict__ src, int len, int nc, int cols, int ld,
                                                    F gcol, double* dst, int tid, int nt) {
-  const int total = nc * len;
-  for (int base = tid; base < total; base += 32 * nt) {
-    double tmp[32];
+  for (int i = tid; i < len; i += nt) {
+    for (int c0 = 0; c0 < nc; c0 += 32) {
+      double tmp[32];
 #pragma unroll
-    for (int j = 0; j < 32; ++j) {
-      const int e = base + j * nt;
-      if (e < total) {
-        const int c = e / len;
-        tmp[j] = src[(int64_t)gcol(c) * len + (e - c * len)];
-      }
-    }
+      for (int j = 0; j < 32; ++j)
+        if (c0 + j < nc) tmp[j] = src[(int64_t)gcol(c0 + j) * len + i];
 #pragma unroll
-    for (int j = 0; j < 32; ++j) {
-      const int e = base + j * nt;
-      if (e < total) {
-        const int c = e / len;
-        dst[c * ld + (e - c * len)] = tmp[j];
-      }
+      for (int j = 0; j < 32; ++j)
+        if (c0 + j < nc) dst[(c0 + j) * ld + i] = tmp[j];
     }
   }
   const int pad = ld - len;
