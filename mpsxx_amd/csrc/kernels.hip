@@ -946,18 +946,14 @@ __global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restric
   const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
   auto gcol = [&](int c) { return c < t.n0 ? t.c0 + c : t.c1 + (c - t.n0); };
   // V's columns are only needed at the end: their loads are issued now, into registers, when they fit
+  // (thread tid holds row tid of each of the nc <= 32 columns: vectors of at most 256)
   constexpr int kVpf = 32;
-  const bool vpf = nc * kk <= 256 * kVpf;
+  const bool vpf = kk <= 256;
   double vreg[kVpf];
-  if (vpf) {
+  if (vpf && tid < kk) {
 #pragma unroll
-    for (int j = 0; j < kVpf; ++j) {
-      const int e = tid + 256 * j;
-      if (e < nc * kk) {
-        const int c = e / kk;
-        vreg[j] = V[b.v_off + (int64_t)gcol(c) * kk + (e - c * kk)];
-      }
-    }
+    for (int c = 0; c < kVpf; ++c)
+      if (c < nc) vreg[c] = V[b.v_off + (int64_t)gcol(c) * kk + tid];
   }
   svd_gather_columns(X + b.x_off, len, nc, COLS, ld, gcol, ys, tid, 256);
   for (int e = tid; e < COLS * COLS; e += 256) Qs[e / COLS][e % COLS] = (e / COLS == e % COLS) ? 1.0 : 0.0;
@@ -1077,13 +1073,10 @@ __global__ __launch_bounds__(256) void svd_round_kernel(const SvdTask* __restric
   __syncthreads();
   if (vpf) {
     const int pad = ld - kk;
+    if (tid < kk) {
 #pragma unroll
-    for (int j = 0; j < kVpf; ++j) {
-      const int e = tid + 256 * j;
-      if (e < nc * kk) {
-        const int c = e / kk;
-        ys[c * ld + (e - c * kk)] = vreg[j];
-      }
+      for (int c = 0; c < kVpf; ++c)
+        if (c < nc) ys[c * ld + tid] = vreg[c];
     }
     for (int e = tid; e < nc * pad; e += 256) ys[(e / pad) * ld + kk + e % pad] = 0.0;
     for (int e = tid; e < (COLS - nc) * ld; e += 256) ys[nc * ld + e] = 0.0;
