@@ -1161,24 +1161,16 @@ __host__ __device__ __forceinline__ size_t svd_qr_lds_doubles(int cw, int ld) {
 // ncols_alloc columns of ld rows.
 __device__ __forceinline__ void svd_qr_gather(const double* __restrict__ src, int64_t col_stride, int K, int ncols,
                                               int ncols_alloc, int ld, double* dst, int tid) {
-  const int total = ncols * K;
-  for (int base = tid; base < total; base += 8 * 256) {
-    double tmp[8];
+  // thread tid moves rows tid, tid + 256, ... of every column, up to 8 loads in flight
+  for (int i = tid; i < K; i += 256) {
+    for (int c0 = 0; c0 < ncols; c0 += 8) {
+      double tmp[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = base + j * 256;
-      if (e < total) {
-        const int c = e / K;
-        tmp[j] = src[(int64_t)c * col_stride + (e - c * K)];
-      }
-    }
+      for (int j = 0; j < 8; ++j)
+        if (c0 + j < ncols) tmp[j] = src[(int64_t)(c0 + j) * col_stride + i];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int e = base + j * 256;
-      if (e < total) {
-        const int c = e / K;
-        dst[c * ld + (e - c * K)] = tmp[j];
-      }
+      for (int j = 0; j < 8; ++j)
+        if (c0 + j < ncols) dst[(c0 + j) * ld + i] = tmp[j];
     }
   }
   const int pad = ld - K;
@@ -1470,10 +1462,8 @@ __global__ __launch_bounds__(256) void svd_qr_trail_kernel(const int2* __restric
   svd_qr_gather(x + (int64_t)c0 * len + p0, len, K, cw, CW, ld, Cs, tid);
   __syncthreads();
   svd_qr_block_reflect<CW>(Vp, Cs, Ts, Ws, W2, K, ld, true, tid);
-  for (int e = tid; e < cw * K; e += 256) {
-    const int c = e / K, i = e - c * K;
-    x[(int64_t)(c0 + c) * len + p0 + i] = Cs[c * ld + i];
-  }
+  for (int i = tid; i < K; i += 256)
+    for (int c = 0; c < cw; ++c) x[(int64_t)(c0 + c) * len + p0 + i] = Cs[c * ld + i];
 }
 
 __global__ __launch_bounds__(256) void svd_rt_kernel(const SvdQrBlk* __restrict__ blks, const double* __restrict__ X,
@@ -1523,10 +1513,8 @@ __global__ __launch_bounds__(256) void svd_qr_apply_kernel(const int2* __restric
     svd_qr_gather(z + p0, len, K, cw, CW, ld, Cs, tid);
     __syncthreads();
     svd_qr_block_reflect<CW>(Vp, Cs, Ts, Ws, W2, K, ld, false, tid);
-    for (int e = tid; e < cw * K; e += 256) {
-      const int c = e / K, i = e - c * K;
-      z[(int64_t)c * len + p0 + i] = Cs[c * ld + i];
-    }
+    for (int i = tid; i < K; i += 256)
+      for (int c = 0; c < cw; ++c) z[(int64_t)c * len + p0 + i] = Cs[c * ld + i];
     __syncthreads();
   }
 }
