@@ -142,7 +142,9 @@ struct GemmPolicy {
   // 17.3 ms and split-BF16x3 from 14.5 to 13.7 ms, with the L=32 M=512 Hubbard sigma unchanged; with the later
   // 128x128 tiles 16384 -> 32768 takes FP32 from 11.50 to 11.35 ms and FP64 from 24.16 to 24.06 ms
   int split_big = 32768;
-  int split_small = 1024;
+  // 1024 -> 4096: the merged sigma on the converged M=1024 QC state 4.14 -> 3.99 ms FP32, 3.24 -> 3.10 ms split-BF16x3,
+  // 7.33 -> 7.28 ms FP64; the L=32 M=512 Hubbard sigma (39.6 us), its 64-chain batch and sweeps unchanged
+  int split_small = 4096;
   int split_seg = 32;
   // blocks with both sides >= this run as 128x128 tiles (gemm128_kernel); 0: never.  The FP64 and FP32 default;
   // split-BF16x3 uses kBigTileMinX3 (the measurements are at default_policy(int))
