@@ -284,6 +284,9 @@ class PlanBuilder {
     if (prec != MPSXX_FP64 && prec != MPSXX_FP32 && prec != MPSXX_BF16X3) throw Error(MPSXX_ERR_ARG, "unknown precision");
     P_.precision = prec;
   }
+  // blocks with both sides <= this run as 32x32 tiles in this plan (negative: the default policy's)
+  void set_small_limit(int v) { small_limit_ = v; }
+  int small_limit_ = -1;
 
   int input(const Layout& L, int which) {
     const int id = (int)P_.slot_layout.size();
@@ -399,7 +402,12 @@ class PlanBuilder {
 
  private:
   // tile policy of the plan's GEMM steps
-  GemmPolicy policy() const { return default_policy(P_.precision); }
+  GemmPolicy policy() const {
+    GemmPolicy p = default_policy(P_.precision);
+    if (small_limit_ >= 0) p.small_limit = small_limit_;
+    return p;
+  }
+
   int add_inter(const Layout& L, int prec) {
     const int id = (int)P_.slot_layout.size();
     P_.slot_layout.push_back(L);
@@ -817,6 +825,10 @@ Plan& plan_sigma_merged(const Layout& lopr, const Layout& ropr, const Layout& wf
   auto np = std::make_unique<CachedPlan>();
   if (!plan_from_fp64(key, ls, prec, np->plan)) {
     PlanBuilder B(prec);
+    // the merged sigma's 33..64-row blocks run faster on 64x64 than on 32x32 tiles (the converged M=1024 QC state:
+    // FP32 4.0 -> 3.85 ms, split-BF16x3 3.1 -> 2.95 ms), while the two-site sigma's keep the default of 64 (its
+    // L=32 M=512 Hubbard form would take 60 instead of 40 us); MPSXX_SMALL_GEMM_LIMIT, when set, applies to both
+    if (!std::getenv("MPSXX_SMALL_GEMM_LIMIT")) B.set_small_limit(kMergedSmallLimit);
     const int sl = B.input(lopr, 0), sr = B.input(ropr, 1), sw = B.input(wfn, 2);
     // compute_sigma_vector.hpp:94-95: Gemm(N,N,lopr,wfn0) (K=1), Gemm(N,T,scr1,ropr) (K=2)
     const int s1 = B.contract(sl, false, {2}, sw, false, {0});

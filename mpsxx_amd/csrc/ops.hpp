@@ -177,6 +177,8 @@ struct DavidsonStats {
 // (products good to ~2^-16 relative).
 constexpr double kMixedTolerance = 1.0e-9;
 constexpr double kMixedToleranceX3 = 1.0e-6;
+// blocks with both sides <= this run as 32x32 tiles in the merged sigma's plans (the default policy's 64 elsewhere)
+constexpr int kMergedSmallLimit = 32;
 // A lower root k for excited states (diagonalize.hpp:36-89, Functor with N_ > 1): sigma = H|k> with the cross
 // environments of the current root and root k, overlap = S|k> with the cross overlaps, energy = E_k.  The Davidson
 // operator becomes v = Hc - sum_k [<H_k|c> S_k + <S_k|c> H_k - E_k <S_k|c> S_k] and the result is orthogonalized
